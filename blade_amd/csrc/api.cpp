@@ -1,0 +1,323 @@
+// api.cpp - the reference-compatible blade_* library API over the MI355X
+// runtime (restates the library functions of system/system.cxx:309-329,
+// system/parameters.cxx:585-786, system/structure.cxx:960-1092,
+// system/coordinates.cxx:323-368, msld/msld.cu:1570-1735 and
+// run/run.cu:633-719).  Indices arrive 1-based as from CHARMM.
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <string>
+
+#include "../../include/blade.h"
+#include "setup.h"
+
+using namespace blade;
+
+namespace {
+thread_local std::string g_apiErr;
+const double kDegrees = 0.017453292519943;  // main/defines.h:32
+}  // namespace
+
+struct System {
+  int device = 0;
+  int verbose = 0;
+  bool failed = false;
+  std::unique_ptr<Parameters> parameters;
+  std::unique_ptr<Structure> structure;
+  std::unique_ptr<Coordinates> coordinates;
+  std::unique_ptr<MsldSetup> msld;
+  std::unique_ptr<RunSetup> run;
+  bool unsupported = false;
+  std::string unsupportedWhat;
+  Topology topo;
+  bm_runparams rp{};
+  bool prepared = false;
+  bm_system *gpu = nullptr;
+  long step = 0;
+  unsigned long long seed = 0xB1ADE000ull;
+};
+
+extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(stderr, "%s", message); }
+
+static void api_fail(System *s, const std::string &m) {
+  g_apiErr = m;
+  if (s) s->failed = true;
+  blade_log(("BLaDE-MI355X error: " + m + "\n").c_str());
+}
+
+static void mark_unsupported(System *s, const char *what) {
+  s->unsupported = true;
+  s->unsupportedWhat = what;
+}
+
+extern "C" {
+
+const char *blade_last_error(void) { return g_apiErr.c_str(); }
+
+System *blade_init_system(int ngpus, int *gpus) {
+  System *s = new System();
+  if (ngpus != 1) api_fail(s, "this build runs one system on one GPU (ngpus must be 1)");
+  s->device = (gpus && ngpus > 0) ? gpus[0] : 0;
+  s->msld.reset(new MsldSetup());
+  return s;
+}
+
+void blade_dest_system(System *s) {
+  if (!s) return;
+  if (s->gpu) bm_system_destroy(s->gpu);
+  delete s;
+}
+
+void blade_set_device(System *) {}
+void blade_set_verbose(System *s, int v) { if (s) s->verbose = v; }
+
+// ---- parameters (units exactly as the reference's setters) -------------
+void blade_init_parameters(System *s) { s->parameters.reset(new Parameters()); }
+void blade_dest_parameters(System *s) { s->parameters.reset(); }
+void blade_add_parameter_atoms(System *s, const char *name, double mass) { s->parameters->atomMass[name] = mass; }
+void blade_add_parameter_bonds(System *s, const char *t1, const char *t2, double kb, double b0) {
+  s->parameters->bonds[canon({t1, t2})] = BondParam{2.0 * kb, b0};
+}
+void blade_add_parameter_angles(System *s, const char *t1, const char *t2, const char *t3, double kangle,
+                                double angle0, double kureyb, double ureyb0) {
+  s->parameters->angles[canon({t1, t2, t3})] = AngleParam{2.0 * kangle, kDegrees * angle0, 2.0 * kureyb, ureyb0};
+}
+void blade_add_parameter_dihes(System *s, const char *t1, const char *t2, const char *t3, const char *t4,
+                               double kdih, int ndih, double dih0) {
+  s->parameters->dihes[canon({t1, t2, t3, t4})].push_back(DiheParam{kdih, ndih, kDegrees * dih0});
+}
+void blade_add_parameter_imprs(System *s, const char *t1, const char *t2, const char *t3, const char *t4,
+                               double kimp, int nimp, double imp0) {
+  if (nimp < 0) { api_fail(s, "improper periodicity less than 0"); return; }
+  s->parameters->imprs[canon({t1, t2, t3, t4})] = ImprParam{nimp == 0 ? 2.0 * kimp : kimp, nimp, kDegrees * imp0};
+}
+void blade_add_parameter_cmaps(System *s, const char *, const char *, const char *, const char *, const char *,
+                               const char *, const char *, const char *, int) {
+  (void)s;  // CMAP parameters are accepted; CMAP terms are rejected at setup
+}
+void blade_add_parameter_cmaps_fill(System *, const char *, const char *, const char *, const char *,
+                                    const char *, const char *, const char *, const char *, int, int, double) {}
+void blade_add_parameter_nbonds(System *s, const char *t1, double eps, double sig, double eps14, double sig14,
+                                double e14fac, int combine) {
+  s->parameters->nbonds[t1] = NbondParam{-eps, sig, -eps14, sig14, e14fac, combine};
+}
+void blade_add_parameter_nbfixs(System *s, const char *t1, const char *t2, double eps, double sig, double eps14,
+                                double sig14) {
+  s->parameters->nbfix[canon({t1, t2})] = NbondParam{-eps, sig, -eps14, sig14, 1.0, 0};
+}
+
+// ---- structure -----------------------------------------------------------
+void blade_init_structure(System *s) { s->structure.reset(new Structure()); }
+void blade_dest_structure(System *s) { s->structure.reset(); }
+void blade_add_atom(System *s, int atomIdx, const char *segName, const char *resIdx, const char *resName,
+                    const char *atomName, const char *atomTypeName, double charge, double mass) {
+  auto &v = s->structure->atoms;
+  if ((int)v.size() < atomIdx) v.resize(atomIdx);
+  v[atomIdx - 1] = AtomRec{segName, resIdx, resName, atomName, atomTypeName, charge, mass};
+}
+void blade_add_bond(System *s, int i, int j) { s->structure->bonds.push_back({i - 1, j - 1}); }
+void blade_add_angle(System *s, int i, int j, int k) { s->structure->angles.push_back({i - 1, j - 1, k - 1}); }
+void blade_add_dihe(System *s, int i, int j, int k, int l) {
+  s->structure->dihes.push_back({i - 1, j - 1, k - 1, l - 1});
+}
+void blade_add_impr(System *s, int i, int j, int k, int l) {
+  s->structure->imprs.push_back({i - 1, j - 1, k - 1, l - 1});
+}
+void blade_add_cmap(System *s, int, int, int, int, int, int, int, int) { mark_unsupported(s, "CMAP terms"); }
+void blade_add_shake(System *s, int shakeHbond) { s->structure->shakeHbond = shakeHbond != 0; }
+
+// ---- coordinates -----------------------------------------------------------
+void blade_init_coordinates(System *s, int n) {
+  s->coordinates.reset(new Coordinates());
+  s->coordinates->n = n;
+  s->coordinates->pos.assign(3 * (size_t)n, 0.0);
+  s->coordinates->vel.assign(3 * (size_t)n, 0.0);
+}
+void blade_dest_coordinates(System *s) { s->coordinates.reset(); }
+void blade_add_coordinates_position(System *s, int i, double x, double y, double z) {
+  double *p = &s->coordinates->pos[3 * (size_t)(i - 1)];
+  p[0] = x; p[1] = y; p[2] = z;
+}
+void blade_add_coordinates_velocity(System *s, int i, double vx, double vy, double vz) {
+  double *p = &s->coordinates->vel[3 * (size_t)(i - 1)];
+  p[0] = vx; p[1] = vy; p[2] = vz;
+}
+void blade_add_coordinates_box(System *s, int name, double a, double b, double c, double alpha, double beta,
+                               double gamma) {
+  auto &C = *s->coordinates;
+  C.boxName = name;
+  C.box[0] = a; C.box[1] = b; C.box[2] = c;
+  C.angles[0] = alpha; C.angles[1] = beta; C.angles[2] = gamma;
+  if (alpha != 90 || beta != 90 || gamma != 90) mark_unsupported(s, "non-orthorhombic boxes");
+}
+
+// ---- msld ------------------------------------------------------------------
+void blade_init_msld(System *s, int nblocks) {
+  s->msld.reset(new MsldSetup());
+  int n = s->structure ? (int)s->structure->atoms.size() : 0;
+  s->msld->resize(nblocks < 1 ? 1 : nblocks, n);
+}
+void blade_dest_msld(System *s) { s->msld.reset(new MsldSetup()); }
+void blade_add_msld_atomassignment(System *s, int atomIdx, int blockIdx) {
+  auto &m = *s->msld;
+  if ((int)m.atomBlock.size() < atomIdx) m.atomBlock.resize(atomIdx, 0);
+  m.atomBlock[atomIdx - 1] = blockIdx - 1;
+}
+void blade_add_msld_initialconditions(System *s, int blockIdx, int siteIdx, double theta0, double thetaVelocity,
+                                      double thetaMass, double fixBias, double blockCharge) {
+  auto &m = *s->msld;
+  int b = blockIdx - 1;
+  m.lambdaSite[b] = siteIdx - 1;
+  m.theta[b] = theta0;
+  m.thetaVelocity[b] = thetaVelocity;
+  m.thetaMass[b] = thetaMass;
+  m.lambdaBias[b] = fixBias;
+  m.lambdaCharge[b] = blockCharge;
+}
+void blade_add_msld_termscaling(System *s, int sb, int su, int sa, int sd, int si, int sc) {
+  bool *t = s->msld->scaleTerms;
+  t[0] = sb; t[1] = su; t[2] = sa; t[3] = sd; t[4] = si; t[5] = sc;
+}
+void blade_add_msld_flags(System *s, double gamma, double fnex, int useSoftCore, int useSoftCore14,
+                          int msldEwaldType, double, double, double, double softNotBondExponent, int fix) {
+  auto &m = *s->msld;
+  m.gamma = gamma;
+  m.fnex = fnex;
+  m.useSoftCore = useSoftCore;
+  m.useSoftCore14 = useSoftCore14;
+  m.msldEwaldType = msldEwaldType;
+  m.fix = fix;
+  if (fix) mark_unsupported(s, "msld fix (ffix)");
+  (void)softNotBondExponent;
+}
+void blade_add_msld_bias(System *s, int i, int j, int type, double l0, double k, int n) {
+  if (type <= 0 || type > 12) { api_fail(s, "type of variable bias is not a recognized type"); return; }
+  s->msld->varBias.push_back(VarBias{i - 1, j - 1, type, l0, k, n});
+}
+void blade_add_msld_block_friction(System *s, int blockIdx, double friction) {
+  auto &m = *s->msld;
+  if (m.thetaFriction.empty()) m.thetaFriction.assign(m.blockCount, m.gamma);
+  m.thetaFriction[blockIdx - 1] = friction;
+}
+void blade_set_msld_block_fixed(System *s, int blockIdx, int fixed) { s->msld->blockFixed[blockIdx - 1] = fixed != 0; }
+
+// ---- run -----------------------------------------------------------------
+void blade_init_run(System *s) { s->run.reset(new RunSetup()); }
+void blade_dest_run(System *s) { s->run.reset(); }
+void blade_add_run_flags(System *s, double gamma, double betaEwald, double rCut, double rSwitch, int vdWfSwitch,
+                         int elecPME, double gridSpace, int gridx, int gridy, int gridz, int orderEwald,
+                         double shakeTolerance) {
+  auto &r = *s->run;
+  r.gamma = gamma;
+  r.betaEwald = betaEwald;
+  r.rCut = rCut;
+  r.rSwitch = rSwitch;
+  r.vdwMethod = vdWfSwitch;
+  r.elecMethod = elecPME;
+  r.gridSpace = gridSpace;
+  r.grid[0] = gridx; r.grid[1] = gridy; r.grid[2] = gridz;
+  r.orderEwald = orderEwald;
+  r.shakeTolerance = shakeTolerance;
+}
+void blade_add_run_dynopts(System *s, int step, int step0, int nsteps, double dt, double T, int freqNPT,
+                           double, double) {
+  auto &r = *s->run;
+  r.step = step;
+  r.step0 = step0;
+  r.nsteps = nsteps;
+  r.dt = dt;
+  r.T = T;
+  r.freqNPT = freqNPT;
+  if (freqNPT > 0) mark_unsupported(s, "constant pressure (freqNPT>0)");
+}
+
+void blade_set_seed(System *s, unsigned long long seed) { s->seed = seed; }
+
+int blade_prepare(System *s) {
+  if (!s || s->failed) return BM_EINVAL;
+  if (s->prepared) return BM_OK;
+  if (s->unsupported) { api_fail(s, "unsupported feature: " + s->unsupportedWhat); return BM_ENOTSUP; }
+  if (!s->parameters || !s->structure || !s->coordinates) { api_fail(s, "parameters, structure and coordinates are required"); return BM_EINVAL; }
+  if (!s->run) s->run.reset(new RunSetup());
+  auto &m = *s->msld;
+  const int n = (int)s->structure->atoms.size();
+  if ((int)m.atomBlock.size() != n) {
+    if (m.blockCount <= 1) m.resize(1, n);
+    else m.atomBlock.resize(n, 0);
+  }
+  s->run->seed = s->seed;
+  try {
+    build_topology(*s->parameters, *s->structure, m, *s->run, s->topo);
+  } catch (const SetupError &e) {
+    api_fail(s, e.msg);
+    return BM_EINVAL;
+  }
+  build_runparams(*s->run, m, *s->coordinates, n, s->rp);
+  s->topo.view();
+  s->prepared = true;
+  return BM_OK;
+}
+
+// Run::dynamics_initialize (run/run.cu:572-622): derive terms, build the
+// device system from the coordinates.
+static int ensure_gpu(System *s) {
+  if (s->gpu) return BM_OK;
+  int rc = blade_prepare(s);
+  if (rc) return rc;
+  const int n = s->topo.n;
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      x[k * (size_t)n + i] = s->coordinates->pos[3 * (size_t)i + k];
+      v[k * (size_t)n + i] = s->coordinates->vel[3 * (size_t)i + k];
+    }
+  rc = bm_system_create(s->device, &s->topo.top, &s->rp, x.data(), v.data(), s->msld->theta.data(),
+                        s->msld->thetaVelocity.data(), &s->gpu);
+  if (rc) api_fail(s, std::string("device system: ") + bm_last_error());
+  s->step = s->run->step;
+  return rc;
+}
+
+void blade_run_energy(System *s) {
+  if (!s || s->failed) return;
+  if (ensure_gpu(s)) return;
+  int rc = bm_domdec_update(s->gpu, 1);
+  if (!rc) rc = bm_calc_force(s->gpu, 0, BM_CALC_ENERGY);
+  if (!rc) rc = bm_sync(s->gpu);
+  if (rc) api_fail(s, std::string("energy: ") + bm_last_error());
+}
+
+int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
+  if (!s || s->failed) return BM_EINVAL;
+  int rc = ensure_gpu(s);
+  if (rc) return rc;
+  rc = bm_dynamics(s->gpu, s->step, nsteps, freqNRG, use_graph);
+  if (!rc) rc = bm_sync(s->gpu);
+  if (!rc) {
+    int flag = -1;
+    bm_get_nan_flag(s->gpu, &flag);
+    if (flag != -1) {
+      api_fail(s, "NaN/Inf detected at atom " + std::to_string(flag));
+      return BM_ENAN;
+    }
+  }
+  if (rc) api_fail(s, std::string("dynamics: ") + bm_last_error());
+  s->step += nsteps;
+  return rc;
+}
+
+int blade_get_energy(System *s, double *e) { return s && s->gpu ? bm_get_energy(s->gpu, e) : BM_EINVAL; }
+int blade_get_forces(System *s, double *f) { return s && s->gpu ? bm_get_forces(s->gpu, f, nullptr) : BM_EINVAL; }
+int blade_get_positions(System *s, double *p) { return s && s->gpu ? bm_get_positions(s->gpu, p, nullptr) : BM_EINVAL; }
+int blade_set_positions(System *s, const double *p) {
+  if (!s) return BM_EINVAL;
+  int rc = ensure_gpu(s);
+  return rc ? rc : bm_set_positions(s->gpu, p, nullptr);
+}
+int blade_get_lambda(System *s, double *l) { return s && s->gpu ? bm_get_lambda(s->gpu, l) : BM_EINVAL; }
+int blade_get_dudl(System *s, double *d) { return s && s->gpu ? bm_get_dudl(s->gpu, d) : BM_EINVAL; }
+const bm_topology *blade_get_topology(System *s) { return s && s->prepared ? &s->topo.top : nullptr; }
+const bm_runparams *blade_get_runparams(System *s) { return s && s->prepared ? &s->rp : nullptr; }
+bm_system *blade_get_device_system(System *s) { return s ? s->gpu : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+// bonded.hip - bonds (+Urey-Bradley), angles, dihedrals, impropers, 1-4
+// pairs, PME exclusion corrections and MSLD lambda biases in one launch
+// (restates bonded/bonded.cu:44-527, bonded/pair.cu:56-387 and
+// msld/msld.cu:1000-1130).  Each workgroup handles one term family so the
+// energy slot is uniform; geometry comes from 32-bit fractional
+// coordinates (exact minimum image) and the arithmetic is FP32.
+#include "kernels.h"
+#include "erfc_coef.h"
+
+namespace blade {
+
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 f3(float a, float b, float c) { return F3{a, b, c}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// minimum-image displacement r_i - r_j
+__device__ __forceinline__ F3 disp(const Dev &d, int i, int j) {
+  F3 r;
+  r.x = fdisp(frac_u32(d.pos[i], d.invbox[0]), frac_u32(d.pos[j], d.invbox[0]), d.fracscale[0]);
+  r.y = fdisp(frac_u32(d.pos[d.n + i], d.invbox[1]), frac_u32(d.pos[d.n + j], d.invbox[1]), d.fracscale[1]);
+  r.z = fdisp(frac_u32(d.pos[2 * d.n + i], d.invbox[2]), frac_u32(d.pos[2 * d.n + j], d.invbox[2]), d.fracscale[2]);
+  return r;
+}
+__device__ __forceinline__ void addforce(const Dev &d, int i, F3 f) {
+  atomic_fix_f(&d.force[i], f.x);
+  atomic_fix_f(&d.force[d.n + i], f.y);
+  atomic_fix_f(&d.force[2 * d.n + i], f.z);
+}
+
+// lambda scaling of a term carrying two siteBlock codes (bonded.cu:80-89)
+struct Sc { int b0, b1; float l0, l1; };
+__device__ __forceinline__ Sc scale2(const Dev &d, int2 sb) {
+  Sc s{sb.x & 0xFFFF, sb.y & 0xFFFF, 1.f, 1.f};
+  if (s.b0) {
+    s.l0 = d.lambdaf[s.b0];
+    if (s.b1) s.l1 = d.lambdaf[s.b1];
+  }
+  return s;
+}
+__device__ __forceinline__ void scale_dudl(const Dev &d, const Sc &s, float e) {
+  if (s.b0) {
+    atomic_fix_f(&d.dudl[s.b0], s.l1 * e);
+    if (s.b1) atomic_fix_f(&d.dudl[s.b1], s.l0 * e);
+  }
+}
+
+__device__ __forceinline__ float erfc_pair(float x) {
+  float ex = __expf(-x * x);
+  float t = __frcp_rn(fmaf(BM_ERFC_P, x, 1.0f));
+  float p = bm_erfc_c[BM_ERFC_DEG];
+#pragma unroll
+  for (int k = BM_ERFC_DEG - 1; k >= 0; k--) p = fmaf(p, t, bm_erfc_c[k]);
+  return ex * p;
+}
+
+// 1-4 pair energy/derivative (pair.cu:56-175); VFSWITCH energy below the
+// switch is shifted by rc^-6/rc^-12 exactly as the reference does
+__device__ void nb14_ef(const Dev &d, float4 p, float r, float &f, float &e) {
+  const float rc = d.rc, rs = d.rs;
+  f = 0.f; e = 0.f;
+  if (!(r < rc)) return;
+  const float qq = p.x, c12 = p.y, c6 = p.z, e14 = p.w;
+  float rinv = 1.f / r, rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
+  if (d.vdwmethod == BM_VFSWITCH) {
+    float rc3 = rc * rc * rc, rc6 = rc3 * rc3, rs3 = rs * rs * rs, rs6 = rs3 * rs3;
+    float k6 = rc3 / (rc3 - rs3), k12 = rc6 / (rc6 - rs6);
+    bool in = r < rs;
+    float A6 = in ? 1.f : k6, A12 = in ? 1.f : k12, B6 = in ? 0.f : 1.f / rc3, B12 = in ? 0.f : 1.f / rc6;
+    f = 6 * c6 * A6 * (rinv3 - B6) * rinv3 * rinv - 12 * c12 * A12 * (rinv6 - B12) * rinv6 * rinv;
+    float CC6 = in ? -1.f / rc6 : 0.f, CC12 = in ? -1.f / (rc6 * rc6) : 0.f;
+    e = c12 * (A12 * (rinv6 - B12) * (rinv6 - B12) + CC12) - c6 * (A6 * (rinv3 - B6) * (rinv3 - B6) + CC6);
+  } else if (d.vdwmethod == BM_VSWITCH) {
+    float c2of = rc * rc, c2on = rs * rs;
+    float rul3 = (c2of - c2on) * (c2of - c2on) * (c2of - c2on), rul12 = 12 / rul3;
+    float rijl = c2on - r * r, riju = c2of - r * r;
+    float fsw = (r < rs) ? 1.f : riju * riju * (riju - 3 * rijl) / rul3;
+    float dfsw = (r < rs) ? 0.f : rijl * riju * rul12;
+    f = fsw * (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv + dfsw * (c12 * rinv6 - c6) * rinv6;
+    e = fsw * (c12 * rinv6 - c6) * rinv6;
+  } else {
+    float r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
+    float rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
+    float rc2 = rc * rc, rc6 = rc2 * rc2 * rc2, rc12 = rc6 * rc6, rc18 = rc12 * rc6;
+    f = 6 * c6 * (rinv7 - r5 / rc12) - 12 * c12 * (rinv13 + r5 / rc18);
+    e = c12 * (rinv12 + 2 * r6 / rc18 - 3 / rc12) - c6 * (rinv6 + r6 / rc12 - 2 / rc6);
+  }
+  const float kqq = kElectricF * qq;
+  if (d.elecmethod == BM_FSWITCH) {
+    float kqqe = kqq * e14;
+    float roff2 = rc * rc, ron2 = rs * rs;
+    float ginv = 1 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
+    float A = roff2 * roff2 * (roff2 - 3 * ron2) * ginv, B = 6 * roff2 * ron2 * ginv;
+    float C = -(ron2 + roff2) * ginv, D = 2 * ginv / 5;
+    float dvc = 8 * (ron2 * roff2 * (rc - rs) - (roff2 * roff2 * rc - ron2 * ron2 * rs) / 5) * ginv;
+    float r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+    if (r <= rs) {
+      f += -kqqe * rinv * rinv;
+      e += kqqe * (rinv + dvc);
+    } else {
+      f += -kqqe * rinv * (A * rinv + B * r + 3 * C * r3 + 5 * D * r5);
+      e += kqqe * (A * (rinv - 1 / rc) + B * (rc - r) + C * (roff2 * rc - r3) + D * (roff2 * roff2 * rc - r5));
+    }
+  } else if (d.elecmethod == BM_PME) {
+    float br = d.beta * r;
+    float erfcrinv = (erfc_pair(fminf(br, BM_ERFC_XMAX)) + e14 - 1.f) * rinv;
+    f += kqq * (-erfcrinv - kTwoOverSqrtPi * d.beta * __expf(-br * br)) * rinv;
+    e += kqq * erfcrinv;
+  } else {
+    float roffinv = 1 / rc, roffinv2 = roffinv * roffinv, kqqe = kqq * e14;
+    f += -kqqe * (rinv * rinv - roffinv2);
+    e += kqqe * (rinv - 2 * roffinv + r * roffinv2);
+  }
+}
+
+struct BondedLaunch {
+  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_bias;  // first block of each family
+};
+
+__global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool energy) {
+  const int bid = blockIdx.x;
+  float e = 0.f;
+  int slot = -1;
+  if (bid < L.b_angle) {
+    const int t = (bid - L.b_bond) * blockDim.x + threadIdx.x;
+    slot = (t < d.nbond12 || bid * (int)blockDim.x < d.nbond12) ? BM_EBOND : BM_EUREY;
+    if (t < d.nbond) {
+      int2 ij = d.bond_i[t];
+      F3 dr = disp(d, ij.x, ij.y);
+      float r = sqrtf(dot3(dr, dr));
+      Sc s = scale2(d, d.bond_sb[t]);
+      float2 p = d.bond_p[t];
+      float db = r - p.y;
+      float et = 0.5f * p.x * db * db;
+      float fb = p.x * db * s.l0 * s.l1;
+      scale_dudl(d, s, et);
+      addforce(d, ij.x, (fb / r) * dr);
+      addforce(d, ij.y, (-fb / r) * dr);
+      e = et * s.l0 * s.l1;
+      // bonds and Urey-Bradley may share a workgroup: split by index
+      if (t >= d.nbond12) {
+        if (energy) atomic_fix_f(&d.energy[BM_EUREY], e);
+        e = 0.f;
+      }
+    }
+    slot = BM_EBOND;
+  } else if (bid < L.b_dihe) {
+    slot = BM_EANGLE;
+    const int t = (bid - L.b_angle) * blockDim.x + threadIdx.x;
+    if (t < d.nangle) {
+      int4 ix = d.angle_i[t];
+      F3 rij = disp(d, ix.x, ix.y), rkj = disp(d, ix.z, ix.y);
+      float dp = dot3(rij, rkj);
+      F3 cr = cross3(rij, rkj);
+      float mc = sqrtf(dot3(cr, cr));
+      float th = atan2f(mc, dp);
+      Sc s = scale2(d, d.angle_sb[t]);
+      float2 p = d.angle_p[t];
+      float dth = th - p.y;
+      float et = 0.5f * p.x * dth * dth;
+      float fa = p.x * dth * s.l0 * s.l1;
+      scale_dudl(d, s, et);
+      F3 fi = (fa / (mc * dot3(rij, rij))) * cross3(rij, cr);
+      F3 fk = (-fa / (mc * dot3(rkj, rkj))) * cross3(rkj, cr);
+      addforce(d, ix.x, fi);
+      addforce(d, ix.z, fk);
+      addforce(d, ix.y, -1.f * (fi + fk));
+      e = et * s.l0 * s.l1;
+    }
+  } else if (bid < L.b_nb14) {
+    const bool impr = bid >= L.b_impr;
+    slot = impr ? BM_EIMPR : BM_EDIHE;
+    const int t = (bid - (impr ? L.b_impr : L.b_dihe)) * blockDim.x + threadIdx.x;
+    if (t < (impr ? d.nimpr : d.ndihe)) {
+      int4 ix = impr ? d.impr_i[t] : d.dihe_i[t];
+      float4 p = impr ? d.impr_p[t] : d.dihe_p[t];
+      F3 rij = disp(d, ix.x, ix.y), rjk = disp(d, ix.y, ix.z), rkl = disp(d, ix.z, ix.w);
+      F3 m = cross3(rij, rjk), nv = cross3(rjk, rkl);
+      F3 mn = cross3(m, nv);
+      float phi = atan2f(sqrtf(dot3(mn, mn)), dot3(m, nv));
+      if (dot3(rij, nv) > 0.f) phi = -phi;  // bonded.cu:372-374
+      float fphi, et;
+      const int nper = (int)p.y;
+      if (!impr || nper > 0) {
+        float dphi = nper * phi - p.z;
+        float sn, cs;
+        sincosf(dphi, &sn, &cs);
+        fphi = -p.x * nper * sn;
+        et = p.x * (cs + 1.f);
+      } else {
+        float dphi = phi - p.z;
+        dphi -= 6.283185307179586f * floorf((dphi + 3.141592653589793f) / 6.283185307179586f);
+        fphi = p.x * dphi;
+        et = 0.5f * p.x * dphi * dphi;
+      }
+      Sc s = scale2(d, impr ? d.impr_sb[t] : d.dihe_sb[t]);
+      fphi *= s.l0 * s.l1;
+      scale_dudl(d, s, et);
+      float rjk2 = dot3(rjk, rjk), r = sqrtf(rjk2);
+      F3 fi = (-fphi * r / dot3(m, m)) * m;
+      F3 fl = (-fphi * r / dot3(nv, nv)) * nv;
+      float pp = dot3(rij, rjk) / rjk2, qq = dot3(rkl, rjk) / rjk2;
+      F3 fj = (-pp) * fi + (-qq) * fl;
+      addforce(d, ix.x, fi);
+      addforce(d, ix.w, -1.f * fl);
+      addforce(d, ix.z, fl - fj);
+      addforce(d, ix.y, fj - fi);
+      e = et * s.l0 * s.l1;
+    }
+  } else if (bid < L.b_nbex) {
+    slot = BM_ENB14;
+    const int t = (bid - L.b_nb14) * blockDim.x + threadIdx.x;
+    if (t < d.nnb14) {
+      int2 ij = d.nb14_i[t];
+      F3 dr = disp(d, ij.x, ij.y);
+      float r = sqrtf(dot3(dr, dr));
+      Sc s = scale2(d, d.nb14_sb[t]);
+      float reff = r, dredr = 1.f, dredll = 0.f;
+      if (d.softcore14 && s.b0) {
+        float rSoft = kSoftCoreRadius * (1.f - s.l0 * s.l1);
+        if (r < rSoft) {
+          float q = r / rSoft;
+          float re = 1.f - 0.5f * q;
+          re = re * q * q * q + 0.5f;
+          dredr = (3.f - 2.f * q) * q * q;
+          dredll = (re - dredr * q) * (-kSoftCoreRadius);
+          reff = re * rSoft;
+        }
+      }
+      float f, et;
+      nb14_ef(d, d.nb14_p[t], reff, f, et);
+      f *= s.l0 * s.l1;
+      scale_dudl(d, s, et + f * dredll);
+      f *= dredr;
+      addforce(d, ij.x, (f / r) * dr);
+      addforce(d, ij.y, (-f / r) * dr);
+      e = et * s.l0 * s.l1;
+    }
+  } else if (bid < L.b_bias) {
+    slot = BM_ENBRECIPEXCL;
+    const int t = (bid - L.b_nbex) * blockDim.x + threadIdx.x;
+    if (t < d.nnbex) {
+      int2 ij = d.nbex_i[t];
+      F3 dr = disp(d, ij.x, ij.y);
+      float r = sqrtf(dot3(dr, dr)), rinv = 1.f / r, br = d.beta * r;
+      float kqq = kElectricF * d.nbex_p[t];
+      // erf(br) = 1 - erfc(br); below 0.5 the series is more accurate
+      float erfv = (br < 0.5f) ? erff(br) : 1.f - erfc_pair(fminf(br, BM_ERFC_XMAX));
+      float f = kqq * (erfv * rinv - kTwoOverSqrtPi * d.beta * __expf(-br * br)) * rinv;
+      float et = -kqq * erfv * rinv;
+      Sc s = scale2(d, d.nbex_sb[t]);
+      f *= s.l0 * s.l1;
+      scale_dudl(d, s, et);
+      addforce(d, ij.x, (f / r) * dr);
+      addforce(d, ij.y, (-f / r) * dr);
+      e = et * s.l0 * s.l1;
+    }
+  } else {
+    // MSLD fixed and variable biases: one thread per block / bias
+    slot = BM_ELAMBDA;
+    const int t = threadIdx.x;
+    for (int b = t; b < d.nl; b += blockDim.x) {
+      double bias = d.fixed_bias[b];
+      if (bias != 0.0) atomic_fix(&d.dudl[b], fix_d(bias));
+      e += (float)(bias * d.lambda[b]);
+    }
+    for (int v = t; v < d.nvb; v += blockDim.x) {
+      int4 vi = d.vb_i[v];
+      double2 vp = d.vb_p[v];
+      const int i = vi.x, j = vi.y, ty = vi.z, nn = vi.w;
+      const double li = d.lambda[i], lj = d.lambda[j], k = vp.y, l0 = vp.x;
+      double ev = 0, fi = 0, fj = 0;
+      switch (ty) {
+        case 6: ev = k * li * lj; fi = k * lj; fj = k * li; break;
+        case 8: ev = k * li * lj / (li + l0); fi = k * l0 * lj / ((li + l0) * (li + l0)); fj = k * li / (li + l0); break;
+        case 10: ev = k * lj * (1 - exp(l0 * li)); fi = k * lj * (-l0 * exp(l0 * li)); fj = k * (1 - exp(l0 * li)); break;
+        case 1: if (li < l0) { ev = k * pow(li - l0, nn); fi = nn * k * pow(li - l0, nn - 1); } break;
+        case 2: if (li > l0) { ev = k * pow(li - l0, nn); fi = nn * k * pow(li - l0, nn - 1); } break;
+        case 3: ev = k * pow(li - lj, nn); fi = nn * k * pow(li - lj, nn - 1); fj = -fi; break;
+        case 4: ev = (li > l0) ? -k : -k * (1 - (li - l0) * (li - l0) / (l0 * l0)); fi = (li > l0) ? 0 : 2 * k * (li - l0) / (l0 * l0); break;
+        case 5: ev = -k * li; fi = -k; break;
+        case 7: ev = k * li * (1 - li) / (li + l0); fi = k * (l0 * (1 + l0) / ((li + l0) * (li + l0)) - 1); break;
+        case 9: ev = k * lj * (1 - pow((li + l0) / l0, nn)); fi = -k * lj * pow((li + l0) / l0, nn - 1) / l0; fj = k * (1 - pow((li + l0) / l0, nn)); break;
+        case 11: ev = k * pow(li, l0) * pow(lj, nn); fi = k * l0 * pow(li, l0 - 1) * pow(lj, nn); fj = k * nn * pow(li, l0) * pow(lj, nn - 1); break;
+        case 12: ev = k * li * pow(lj, nn) / (li + l0); fi = k * l0 * pow(lj, nn) / ((li + l0) * (li + l0)); fj = k * nn * li * pow(lj, nn - 1) / (li + l0); break;
+        default: break;
+      }
+      if (fi != 0.0) atomic_fix(&d.dudl[i], fix_d(fi));
+      if (fj != 0.0) atomic_fix(&d.dudl[j], fix_d(fj));
+      e += (float)ev;
+    }
+  }
+  if (energy) block_energy(&d.energy[slot], e);
+}
+
+void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st) {
+  auto nb = [](int n) { return (n + 255) / 256; };
+  BondedLaunch L;
+  L.b_bond = 0;
+  L.b_angle = L.b_bond + nb(d.nbond);
+  L.b_dihe = L.b_angle + nb(d.nangle);
+  L.b_impr = L.b_dihe + nb(d.ndihe);
+  L.b_nb14 = L.b_impr + nb(d.nimpr);
+  L.b_nbex = L.b_nb14 + nb(d.nnb14);
+  L.b_bias = L.b_nbex + nb(d.nnbex);
+  int total = L.b_bias + (bias ? 1 : 0);
+  if (total > 0) bonded_kernel<<<total, 256, 0, st>>>(d, L, energy);
+}
+
+}  // namespace blade

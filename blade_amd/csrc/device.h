@@ -1,0 +1,155 @@
+// device.h - shared device-side definitions for the MI355X kernels.
+//
+// Everything here is compiled for gfx950 only (wave64).  Kernels take a
+// `Dev` view by value: plain device pointers and sizes owned by the host
+// runtime (runtime.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/blade_mi355x.h"
+
+namespace blade {
+
+constexpr double kElectricD = 332.0716;            // main/defines.h:55
+constexpr double kBoltzD = 0.001987191;            // main/defines.h:53
+constexpr float kElectricF = 332.0716f;
+constexpr float kSoftCoreRadius = 4.0f;            // main/defines.h:48
+constexpr float kTwoOverSqrtPi = 1.1283791670955126f;
+constexpr float kFixScaleF = 4294967296.0f;
+constexpr double kFixScaleD = 4294967296.0;
+constexpr double kFixInvD = 1.0 / 4294967296.0;
+
+// tile list capacity helpers
+constexpr int kBlockAtoms = 32;      // atoms per block (one half-wave)
+constexpr int kTileChunk = 16;       // candidate tiles examined per wave claim
+constexpr int kMaxColumnAtoms = 2048;
+
+// Device view of one system.
+struct Dev {
+  int n, nl, nsites;             // atoms, lambda blocks, sites
+  double box[3], invbox[3];
+  float boxf[3];
+  float fracscale[3];            // box / 2^32: fractional count -> A
+  // state (SoA FP64, [x|y|z|theta])
+  double *pos, *posb, *vel, *ism;
+  long long *force;              // [3N+nL] fixed point
+  long long *dudl;               // [nL]
+  long long *energy;             // [BM_NTERMS]
+  double *lambda;                // [nL]
+  float *lambdaf;                // [nL]
+  int *nanflag;
+  // per atom
+  float *q;                      // [N]
+  int *type;                     // [N]
+  int *sb;                       // [N] siteBlock code
+  float *qlam;                   // [N] scratch: lambda*q (PME)
+  float2 *vdw;                   // [T*T] (c12, c6)
+  int ntypes;
+  // domdec
+  int ncx, ncy, ncol, maxblocks;
+  long long maxtiles;
+  unsigned *ufrac;               // [3][N] fractional coordinates at rebuild
+  int *colcount, *colstart, *colfill, *bucket;   // [ncol],[ncol+1],[ncol],[N]
+  int *blkstart;                 // [ncol+1] first block of each column
+  int *nblocks;                  // [1]
+  int *l2g;                      // [32*maxB] slot -> atom (-1 empty)
+  int *g2s;                      // [N] atom -> slot
+  unsigned *bblo, *bbhi;         // [3*maxB] rebuild boxes (exact intervals)
+  uint4 *bbc;                    // [maxB] per-step box centre (frac counts)
+  uint4 *bbh;                    // [maxB] per-step half extent (frac counts), .w = alch flag
+  uint4 *spos;                   // [32*maxB] slot (ux,uy,uz,q bits)
+  int2 *spar;                    // [32*maxB] slot (type or -1, siteBlock)
+  int *tcount, *tstart;          // [maxB],[maxB+1]
+  int *ntiles;                   // [1]
+  int *tile_i, *tile_j, *tile_ex;// [maxtiles]
+  int *tile_flag;                // [maxtiles]
+  int *nexcltiles;               // [1]
+  unsigned *emask;               // [32*n_excl]
+  int *excl_tile;                // [n_excl]
+  int nexcl;
+  int *excl;                     // [2*n_excl]
+  int *workctr;                  // [4] dynamic work counters
+  int *errflag;                  // [1] overflow / capacity errors
+  double rlist2;                 // (rcut+pad)^2, as the oracle computes it
+  double ddscale[3];             // box * 2^-32, as the oracle computes it
+  // nonbonded parameters
+  float rc, rs, beta;
+  float rc2;
+  int vdwmethod, elecmethod;
+  // PME
+  int K[3], order;
+  long long *qgrid;              // [K0*K1*K2] fixed-point charge grid
+  float2 *cgrid;                 // [K0*K1*K2] complex work grid
+  float2 *twiddle;               // [K0+K1+K2] e^{-2 pi i k/K} per axis
+  float *bmod[3];                // [K] B-spline moduli per axis
+  float pmefac;                  // 0.5 kE/(pi V)
+  float pmeexp;                  // pi^2/beta^2
+  // bonded terms (float parameters, int indices)
+  int nbond, nbond12, nangle, ndihe, nimpr, nnb14, nnbex;
+  int2 *bond_i; int2 *bond_sb; float2 *bond_p;            // (kb, b0)
+  int4 *angle_i; int2 *angle_sb; float2 *angle_p;         // (k, t0)
+  int4 *dihe_i; int2 *dihe_sb; float4 *dihe_p;            // (k, n, d0)
+  int4 *impr_i; int2 *impr_sb; float4 *impr_p;            // (k, n, d0)
+  int2 *nb14_i; int2 *nb14_sb; float4 *nb14_p;            // (qq, c12, c6, e14fac)
+  int2 *nbex_i; int2 *nbex_sb; float *nbex_p;             // qq
+  // constraints
+  int ntri, nbr1, nbr2, nbr3;
+  int4 *tri_i; double4 *tri_b;   // b0 (01,02,12), .w unused
+  int2 *br1_i; double *br1_b;
+  int4 *br2_i; double2 *br2_b;
+  int4 *br3_i; double4 *br3_b;
+  double *mass;                  // [N]
+  // msld
+  int *block_site, *site_bound, *block_fixed;
+  double *fixed_bias;
+  double *lfric, *lnoise;        // [nL] per-block theta friction/noise
+  int nvb;
+  int4 *vb_i;                    // (i, j, type, n)
+  double2 *vb_p;                 // (l0, k)
+  double fnex;
+  int softcore, softcore14;
+  // integrator
+  double halfdt, fric, noise, lfric0, lnoise0, shaketol;
+  unsigned long long seed;
+  long long *step;               // [1] step counter (RNG counter), advanced on device
+};
+
+__device__ __forceinline__ long long fix_f(float v) {
+  return (long long)__float2ll_rn(v * kFixScaleF);
+}
+__device__ __forceinline__ long long fix_d(double v) {
+  return (long long)__double2ll_rn(v * kFixScaleD);
+}
+__device__ __forceinline__ void atomic_fix(long long *p, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+}
+__device__ __forceinline__ void atomic_fix_f(long long *p, float v) { atomic_fix(p, fix_f(v)); }
+
+// 32-bit fractional coordinate of x in a periodic box of edge L:
+// floor(x*(1/L)*2^32) taken modulo 2^32.  Must round exactly like the host
+// oracle: one rounded multiply, an exact power-of-two scale, floor.
+__device__ __forceinline__ unsigned frac_u32(double x, double invL) {
+  double f = __dmul_rn(x, invL);
+  double s = f * 4294967296.0;
+  return (unsigned)(long long)floor(s);
+}
+
+// minimum-image displacement from fractional coordinates (int wrap)
+__device__ __forceinline__ float fdisp(unsigned a, unsigned b, float scale) {
+  return (float)(int)(a - b) * scale;
+}
+
+// Reduce a long long across a wave64 and add it with one atomic.
+__device__ __forceinline__ void wave_atomic_fix(long long *p, long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomic_fix(p, v);
+}
+
+// Block-wide fixed-point energy sum (any blockDim multiple of 64).
+__device__ __forceinline__ void block_energy(long long *p, float e) {
+  wave_atomic_fix(p, fix_f(e));
+}
+
+}  // namespace blade

@@ -1,0 +1,319 @@
+// domdec.hip - cell sort, blocks, tile list and exclusion masks
+// (restates domdec/assign_domain.cu:14-45, assign_blocks.cu:24-468,
+// cull.cu:44-375, assign_excl.cu:17-371, recull.cu:41-146 on 32-bit
+// fractional coordinates; orc_domdec in the oracle is the same rule set).
+//
+// Rebuild sequence (no host synchronisation, fixed grids, graph-capturable):
+//   dd_wrap_count -> dd_scan_columns -> dd_scatter -> dd_sort_columns ->
+//   dd_tile_count -> dd_scan_tiles -> dd_tile_write -> dd_excl_mark ->
+//   dd_scan_excl -> dd_excl_set
+// Every step: dd_pack (slot positions, block boxes, accumulator reset).
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace blade {
+
+// One workgroup of 1024 threads: exclusive scan of in[0..n) into out,
+// out[n] = total.  Deterministic integer arithmetic.
+__device__ void block_scan_1024(const int *in, int *out, int n, int *total) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = tid * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; i++) s += in[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - s;
+  for (int i = b; i < e; i++) {
+    int v = in[i];
+    out[i] = run;
+    run += v;
+  }
+  if (tid == 1023) {
+    out[n] = part[1023];
+    if (total) *total = part[1023];
+  }
+}
+
+__global__ void dd_zero_columns(Dev d) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < d.ncol) {
+    d.colcount[c] = 0;
+    d.colfill[c] = 0;
+  }
+}
+
+__global__ void dd_wrap_count(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  unsigned u[3];
+  for (int k = 0; k < 3; k++) {
+    double x = d.pos[k * d.n + i];
+    double w = floor(__dmul_rn(x, d.invbox[k]));
+    x = __dsub_rn(x, __dmul_rn(w, d.box[k]));
+    d.pos[k * d.n + i] = x;
+    u[k] = frac_u32(x, d.invbox[k]);
+    d.ufrac[k * d.n + i] = u[k];
+  }
+  unsigned cx = (unsigned)(((unsigned long long)u[0] * (unsigned)d.ncx) >> 32);
+  unsigned cy = (unsigned)(((unsigned long long)u[1] * (unsigned)d.ncy) >> 32);
+  atomicAdd(&d.colcount[cx * d.ncy + cy], 1);
+}
+
+// blocks per column = ceil(count/32); block starts by exclusive scan
+__global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) {
+  __shared__ int nb[4096];
+  int *tmp = d.bucket;  // not yet used this rebuild: scratch for ncol > 4096
+  const bool big = d.ncol > 4096;
+  for (int c = threadIdx.x; c < d.ncol; c += 1024) {
+    int v = (d.colcount[c] + kBlockAtoms - 1) / kBlockAtoms;
+    if (big) tmp[c] = v; else nb[c] = v;
+  }
+  __syncthreads();
+  block_scan_1024(d.colcount, d.colstart, d.ncol, nullptr);
+  __syncthreads();
+  block_scan_1024(big ? tmp : nb, d.blkstart, d.ncol, d.nblocks);
+}
+
+__global__ void dd_scatter(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  unsigned cx = (unsigned)(((unsigned long long)d.ufrac[i] * (unsigned)d.ncx) >> 32);
+  unsigned cy = (unsigned)(((unsigned long long)d.ufrac[d.n + i] * (unsigned)d.ncy) >> 32);
+  int c = cx * d.ncy + cy;
+  int s = atomicAdd(&d.colfill[c], 1);
+  d.bucket[d.colstart[c] + s] = i;
+}
+
+// One workgroup per column: rank atoms by (z fraction, atom index), place
+// them into 32-slot blocks, then the exact rebuild box of every block.
+__global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
+  __shared__ unsigned long long key[kMaxColumnAtoms];
+  const int c = blockIdx.x;
+  const int start = d.colstart[c], cnt = d.colcount[c];
+  const int b0 = d.blkstart[c], nbc = d.blkstart[c + 1] - b0;
+  if (cnt > kMaxColumnAtoms) {
+    if (threadIdx.x == 0) atomicOr(d.errflag, 1);
+    return;
+  }
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    int a = d.bucket[start + k];
+    key[k] = ((unsigned long long)d.ufrac[2 * d.n + a] << 32) | (unsigned)a;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    unsigned long long kk = key[k];
+    int rank = 0;
+    for (int m = 0; m < cnt; m++) rank += key[m] < kk;
+    int a = (int)(kk & 0xFFFFFFFFull);
+    int slot = (b0 + rank / kBlockAtoms) * kBlockAtoms + rank % kBlockAtoms;
+    d.l2g[slot] = a;
+    d.g2s[a] = slot;
+  }
+  for (int s = threadIdx.x; s < nbc * kBlockAtoms; s += blockDim.x)
+    if (s >= cnt) d.l2g[b0 * kBlockAtoms + s] = -1;
+  __threadfence_block();
+  __syncthreads();
+  for (int t = threadIdx.x; t < nbc * 3; t += blockDim.x) {
+    int b = b0 + t / 3, k = t % 3;
+    unsigned lo = 0xFFFFFFFFu, hi = 0;
+    for (int l = 0; l < kBlockAtoms; l++) {
+      int a = d.l2g[b * kBlockAtoms + l];
+      if (a < 0) continue;
+      unsigned w = d.ufrac[k * d.n + a];
+      lo = min(lo, w);
+      hi = max(hi, w);
+    }
+    d.bblo[3 * b + k] = lo;
+    d.bbhi[3 * b + k] = hi;
+  }
+}
+
+// periodic gap between the rebuild boxes of blocks I and J, squared, in A^2
+__device__ __forceinline__ bool dd_boxes_close(const Dev &d, int I, int J) {
+  const unsigned long long P = 1ull << 32;
+  double g[3];
+  for (int k = 0; k < 3; k++) {
+    unsigned long long aL = d.bblo[3 * I + k], aH = d.bbhi[3 * I + k];
+    unsigned long long bL = d.bblo[3 * J + k], bH = d.bbhi[3 * J + k];
+    unsigned long long gap = 0;
+    if (aH < bL) gap = min(bL - aH, aL + P - bH);
+    else if (bH < aL) gap = min(aL - bH, bL + P - aH);
+    g[k] = __dmul_rn((double)gap, d.ddscale[k]);
+  }
+  double s = __dadd_rn(__dmul_rn(g[0], g[0]), __dmul_rn(g[1], g[1]));
+  s = __dadd_rn(s, __dmul_rn(g[2], g[2]));
+  return s <= d.rlist2;
+}
+
+// one wave per i-block: number of J >= I within the list radius
+__global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nb = *d.nblocks;
+  if (I >= nb) return;
+  int cnt = 0;
+  for (int J0 = I; J0 < nb; J0 += 64) {
+    int J = J0 + lane;
+    bool keep = J < nb && (J == I || dd_boxes_close(d, I, J));
+    cnt += __popcll(__ballot(keep));
+  }
+  if (lane == 0) d.tcount[I] = cnt;
+}
+
+__global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
+  block_scan_1024(d.tcount, d.tstart, *d.nblocks, d.ntiles);
+}
+
+__global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nb = *d.nblocks;
+  if (I >= nb) return;
+  int base = d.tstart[I];
+  const unsigned long long below = (1ull << lane) - 1;
+  for (int J0 = I; J0 < nb; J0 += 64) {
+    int J = J0 + lane;
+    bool keep = J < nb && (J == I || dd_boxes_close(d, I, J));
+    unsigned long long m = __ballot(keep);
+    if (keep) {
+      int t = base + __popcll(m & below);
+      d.tile_i[t] = I;
+      d.tile_j[t] = J;
+      d.tile_flag[t] = 0;
+    }
+    base += __popcll(m);
+  }
+}
+
+// exclusion pair -> tile holding it (binary search in I's sorted J list)
+__global__ void dd_excl_mark(Dev d) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.nexcl) return;
+  int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
+  if (sa / kBlockAtoms > sb / kBlockAtoms) { int t = sa; sa = sb; sb = t; }
+  const int I = sa / kBlockAtoms, J = sb / kBlockAtoms;
+  int lo = d.tstart[I], hi = d.tstart[I + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (d.tile_j[mid] < J) lo = mid + 1; else hi = mid;
+  }
+  int t = (lo < d.tstart[I + 1] && d.tile_j[lo] == J) ? lo : -1;
+  d.excl_tile[e] = t;
+  if (t >= 0) d.tile_flag[t] = 1;
+}
+
+__global__ void __launch_bounds__(1024) dd_scan_excl(Dev d) {
+  const int nt = *d.ntiles;
+  // exclusive scan of flags into tile_ex, then flag-less tiles get -1
+  block_scan_1024(d.tile_flag, d.tile_ex, nt, d.nexcltiles);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nt; t += 1024)
+    if (!d.tile_flag[t]) d.tile_ex[t] = -1;
+}
+
+__global__ void dd_excl_clear(Dev d) {
+  const long long n = (long long)(*d.nexcltiles) * kBlockAtoms;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    d.emask[i] = 0u;
+}
+
+__global__ void dd_excl_set(Dev d) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.nexcl) return;
+  const int t = d.excl_tile[e];
+  if (t < 0) return;
+  int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
+  if (sa / kBlockAtoms > sb / kBlockAtoms) { int x = sa; sa = sb; sb = x; }
+  const int x = d.tile_ex[t];
+  atomicOr(&d.emask[x * kBlockAtoms + sa % kBlockAtoms], 1u << (sb % kBlockAtoms));
+  if (sa / kBlockAtoms == sb / kBlockAtoms)
+    atomicOr(&d.emask[x * kBlockAtoms + sb % kBlockAtoms], 1u << (sa % kBlockAtoms));
+}
+
+// Every step: slot coordinates, per-block box centre/half-extent for the
+// per-step cull, and reset of the force/energy accumulators.
+__global__ void __launch_bounds__(256) dd_pack(Dev d) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = *d.nblocks;
+  const int nf = 3 * d.n + d.nl;
+  if (s < nf) d.force[s] = 0;
+  if (s < d.nl) d.dudl[s] = 0;
+  if (s < BM_NTERMS) d.energy[s] = 0;
+  if (s == 0) {
+    d.workctr[0] = 0;
+    d.workctr[1] = 0;
+  }
+  if (blockIdx.x * blockDim.x >= nb * kBlockAtoms) return;  // whole wave idle
+  const bool in = s < nb * kBlockAtoms;
+  const int a = in ? d.l2g[s] : -1;
+  unsigned u[3] = {0, 0, 0};
+  if (a >= 0) {
+    for (int k = 0; k < 3; k++) u[k] = frac_u32(d.pos[k * d.n + a], d.invbox[k]);
+    d.spos[s] = make_uint4(u[0], u[1], u[2], __float_as_uint(d.q[a]));
+    d.spar[s] = make_int2(d.type[a], d.sb[a]);
+  } else if (in) {
+    d.spos[s] = make_uint4(0, 0, 0, 0);
+    d.spar[s] = make_int2(-1, 0);
+  }
+  // box of the 32-slot block (a half wave); slot 0 of a block is occupied
+  unsigned c[3], h[3];
+  for (int k = 0; k < 3; k++) {
+    unsigned ref = __shfl(u[k], 0, 32);
+    int off = (int)(u[k] - ref);
+    int mn = a >= 0 ? off : 0x7FFFFFFF, mx = a >= 0 ? off : (int)0x80000000;
+    for (int o = 16; o > 0; o >>= 1) {
+      mn = min(mn, __shfl_xor(mn, o, 32));
+      mx = max(mx, __shfl_xor(mx, o, 32));
+    }
+    c[k] = ref + (unsigned)((mn + mx) >> 1);
+    h[k] = (unsigned)(mx - mn + 1) >> 1;
+  }
+  int alch = (a >= 0 && d.sb[a] != 0) ? 1 : 0;
+  for (int o = 16; o > 0; o >>= 1) alch |= __shfl_xor(alch, o, 32);
+  if (in && (s & (kBlockAtoms - 1)) == 0) {
+    const int b = s / kBlockAtoms;
+    d.bbc[b] = make_uint4(c[0], c[1], c[2], 0);
+    d.bbh[b] = make_uint4(h[0], h[1], h[2], (unsigned)alch);
+  }
+}
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st) {
+  const int n = d.n;
+  dd_zero_columns<<<(d.ncol + 255) / 256, 256, 0, st>>>(d);
+  dd_wrap_count<<<(n + 255) / 256, 256, 0, st>>>(d);
+  dd_scan_columns<<<1, 1024, 0, st>>>(d);
+  dd_scatter<<<(n + 255) / 256, 256, 0, st>>>(d);
+  dd_sort_columns<<<d.ncol, 256, 0, st>>>(d);
+  dd_tile_count<<<(maxBlocks + 3) / 4, 256, 0, st>>>(d);
+  dd_scan_tiles<<<1, 1024, 0, st>>>(d);
+  dd_tile_write<<<(maxBlocks + 3) / 4, 256, 0, st>>>(d);
+  if (d.nexcl) dd_excl_mark<<<(d.nexcl + 255) / 256, 256, 0, st>>>(d);
+  dd_scan_excl<<<1, 1024, 0, st>>>(d);
+  if (d.nexcl) {
+    dd_excl_clear<<<256, 256, 0, st>>>(d);
+    dd_excl_set<<<(d.nexcl + 255) / 256, 256, 0, st>>>(d);
+  }
+}
+
+void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st) {
+  long long work = (long long)maxBlocks * kBlockAtoms;
+  long long nf = 3LL * d.n + d.nl;
+  if (nf > work) work = nf;
+  if (work < BM_NTERMS) work = BM_NTERMS;
+  dd_pack<<<(unsigned)((work + 255) / 256), 256, 0, st>>>(d);
+}
+
+}  // namespace blade

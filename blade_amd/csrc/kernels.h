@@ -1,0 +1,18 @@
+// kernels.h - launchers of the device kernels (one translation unit each).
+#pragma once
+
+#include "device.h"
+
+namespace blade {
+
+void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st);
+void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st);
+void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st);
+void launch_pme(const Dev &d, bool energy, hipStream_t st);
+bool pme_grid_supported(int K);
+void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st);
+void launch_update(const Dev &d, bool energy_step, hipStream_t st);
+void launch_theta_to_lambda(const Dev &d, hipStream_t st);
+void launch_set_step(const Dev &d, long long step, hipStream_t st);
+
+}  // namespace blade

@@ -1,0 +1,732 @@
+// runtime.hip - device buffers of one system and the bm_* kernel-layer ABI.
+//
+// bm_system owns every device buffer of one simulation system (the roles of
+// the reference's State, Potential, Domdec and the device half of Msld) and
+// drives the per-step launch sequence:
+//   [rebuild]  domdec kernels               (every freq_domdec steps)
+//   pack       slot coordinates, block boxes, zero accumulators
+//   nbdirect   tile kernel
+//   pme        spread, 5 FFT passes (convolution fused), gather + self
+//   bonded     bonds..nbex, lambda biases
+//   update     kick/drift, constraints, O/drift, constraints, theta->lambda
+// Each (rebuild, energy) step variant can be captured once into a hipGraph
+// and replayed; all counts the kernels need live in device memory, so the
+// captured graphs need no host input between replays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(BM_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+double bspline_host(int n, double u) {
+  if (n == 1) return (u >= 0 && u < 1) ? 1.0 : 0.0;  // M_1 on [0,1)
+  if (u <= 0 || u >= n) return 0;
+  return (u * bspline_host(n - 1, u) + (n - u) * bspline_host(n - 1, u - 1)) / (n - 1);
+}
+
+}  // namespace
+
+struct bm_system {
+  int device = 0;
+  hipStream_t st = nullptr;
+  blade::Dev d{};
+  int maxBlocks = 0;
+  long long maxTiles = 0;
+  bool alch = false, soft = false, bias = false;
+  int nwgNb = 1024;
+  int freqDomdec = 10;
+  std::vector<void *> allocs;
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipEvent_t ev[8] = {};
+  double msNb = 0, msPme = 0, msTotal = 0;
+  long nMeas = 0;
+
+  template <class T>
+  int alloc(T *&p, size_t count) {
+    if (count == 0) count = 1;
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, count * sizeof(T)));
+    HIPCHK(hipMemset(q, 0, count * sizeof(T)));
+    allocs.push_back(q);
+    p = static_cast<T *>(q);
+    return BM_OK;
+  }
+  template <class T>
+  int upload(T *&p, const std::vector<T> &v) {
+    int rc = alloc(p, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BM_OK;
+  }
+
+  void step_launches(bool rebuild, bool energy) {
+    if (rebuild) blade::launch_domdec_rebuild(d, maxBlocks, st);
+    blade::launch_domdec_pack(d, maxBlocks, st);
+    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
+    if (d.elecmethod == BM_PME) blade::launch_pme(d, energy, st);
+    blade::launch_bonded(d, energy, bias, st);
+    blade::launch_update(d, energy, st);
+  }
+
+  ~bm_system() {
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        if (graph[a][b]) (void)hipGraphExecDestroy(graph[a][b]);
+        if (graphSrc[a][b]) (void)hipGraphDestroy(graphSrc[a][b]);
+      }
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamSynchronize(st);
+    for (void *p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" {
+
+const char *bm_version(void) { return "blade-mi355x 0.1 (gfx950)"; }
+const char *bm_last_error(void) { return g_err.c_str(); }
+
+int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, const double *pos,
+                     const double *vel, const double *theta, const double *theta_vel,
+                     bm_system **out) {
+  using namespace blade;
+  if (!t || !rp || !pos || !out) return set_err(BM_EINVAL, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(BM_ENOGPU, "no HIP device");
+  if (device < 0 || device >= ndev) return set_err(BM_EINVAL, "device out of range");
+  for (int k = 0; k < 3; k++)
+    if (!(rp->box[k] > 2 * rp->rcut)) return set_err(BM_EINVAL, "box edge must exceed 2*rcut");
+  if (rp->elec_method == BM_PME) {
+    for (int k = 0; k < 3; k++)
+      if (!pme_grid_supported(rp->grid[k])) return set_err(BM_ENOTSUP, "PME grid needs factors 2,3,5");
+    if (rp->order != 4 && rp->order != 6 && rp->order != 8) return set_err(BM_ENOTSUP, "PME order 4/6/8");
+  }
+  HIPCHK(hipSetDevice(device));
+  bm_system *s = new bm_system();
+  s->device = device;
+  auto fail = [&](int rc) {
+    delete s;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(BM_EHIP, "stream"));
+  for (auto &e : s->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(set_err(BM_EHIP, "event"));
+  Dev &d = s->d;
+  const int n = t->n_atoms, nl = std::max(1, t->n_blocks);
+  d.n = n;
+  d.nl = nl;
+  d.nsites = std::max(1, t->n_sites);
+  for (int k = 0; k < 3; k++) {
+    d.box[k] = rp->box[k];
+    d.invbox[k] = 1.0 / rp->box[k];
+    d.boxf[k] = (float)rp->box[k];
+    d.fracscale[k] = (float)(rp->box[k] / 4294967296.0);
+    d.ddscale[k] = rp->box[k] * (1.0 / 4294967296.0);
+  }
+  const double rl = rp->rcut + rp->cull_pad;
+  d.rlist2 = rl * rl;
+  s->alch = t->n_blocks > 1;
+  s->soft = s->alch && rp->soft_core;
+  s->bias = s->alch;
+  s->freqDomdec = rp->freq_domdec > 0 ? rp->freq_domdec : 10;
+  // columns of about 32 atoms' volume (domdec/domdec.cu:110-118)
+  {
+    double V = rp->box[0] * rp->box[1] * rp->box[2];
+    double a = std::cbrt(32.0 * V / n);
+    d.ncx = std::max(1, (int)std::ceil(rp->box[0] / a));
+    d.ncy = std::max(1, (int)std::ceil(rp->box[1] / a));
+    d.ncol = d.ncx * d.ncy;
+  }
+  s->maxBlocks = n / kBlockAtoms + 1 + d.ncol;
+  d.maxblocks = s->maxBlocks;
+  s->maxTiles = (long long)s->maxBlocks * (s->maxBlocks + 1) / 2;
+  d.maxtiles = s->maxTiles;
+  const int MB = s->maxBlocks;
+  int rc = 0;
+#define ALLOC(p, c) \
+  if ((rc = s->alloc(p, (c)))) return fail(rc)
+#define UPLOAD(p, v) \
+  if ((rc = s->upload(p, v))) return fail(rc)
+  const size_t N = 3 * (size_t)n + nl;
+  // state
+  {
+    std::vector<double> x(N, 0.0), v(N, 0.0), ism(N, 0.0);
+    for (size_t i = 0; i < 3 * (size_t)n; i++) {
+      x[i] = pos[i];
+      v[i] = vel ? vel[i] : 0.0;
+    }
+    for (int i = 0; i < n; i++) {
+      double m = t->mass[i];
+      double w = m > 0 ? 1.0 / std::sqrt(m) : 0.0;
+      ism[i] = ism[n + i] = ism[2 * (size_t)n + i] = w;
+    }
+    for (int b = 0; b < t->n_blocks; b++) {
+      x[3 * (size_t)n + b] = theta ? theta[b] : 0.0;
+      v[3 * (size_t)n + b] = theta_vel ? theta_vel[b] : 0.0;
+      double m = t->theta_mass ? t->theta_mass[b] : 0.0;
+      ism[3 * (size_t)n + b] = m > 0 ? 1.0 / std::sqrt(m) : 0.0;
+    }
+    UPLOAD(d.pos, x);
+    UPLOAD(d.vel, v);
+    UPLOAD(d.ism, ism);
+    ALLOC(d.posb, N);
+  }
+  ALLOC(d.force, N);
+  ALLOC(d.dudl, nl);
+  ALLOC(d.energy, BM_NTERMS);
+  {
+    std::vector<double> lam(nl, 1.0);
+    UPLOAD(d.lambda, lam);
+    std::vector<float> lamf(nl, 1.0f);
+    UPLOAD(d.lambdaf, lamf);
+    std::vector<int> nf(1, -1);
+    UPLOAD(d.nanflag, nf);
+    std::vector<long long> st0(1, 0);
+    UPLOAD(d.step, st0);
+  }
+  // per-atom nonbonded data
+  {
+    std::vector<float> q(n);
+    std::vector<int> ty(n), sb(n);
+    std::vector<double> ms(n);
+    for (int i = 0; i < n; i++) {
+      q[i] = (float)t->charge[i];
+      ty[i] = t->type_idx[i];
+      sb[i] = t->site_block ? t->site_block[i] : 0;
+      ms[i] = t->mass[i];
+    }
+    UPLOAD(d.q, q);
+    UPLOAD(d.type, ty);
+    UPLOAD(d.sb, sb);
+    UPLOAD(d.mass, ms);
+    ALLOC(d.qlam, n);
+    std::vector<float2> vdw((size_t)t->n_types * t->n_types);
+    for (size_t k = 0; k < vdw.size(); k++) vdw[k] = make_float2((float)t->vdw_c12[k], (float)t->vdw_c6[k]);
+    UPLOAD(d.vdw, vdw);
+    d.ntypes = t->n_types;
+  }
+  d.rc = (float)rp->rcut;
+  d.rs = (float)rp->rswitch;
+  d.rc2 = d.rc * d.rc;
+  d.beta = (float)rp->beta_ewald;
+  d.vdwmethod = rp->vdw_method;
+  d.elecmethod = rp->elec_method;
+  // domdec buffers
+  ALLOC(d.ufrac, 3 * (size_t)n);
+  ALLOC(d.colcount, d.ncol);
+  ALLOC(d.colstart, d.ncol + 1);
+  ALLOC(d.colfill, d.ncol);
+  ALLOC(d.bucket, std::max(n, d.ncol));
+  ALLOC(d.blkstart, d.ncol + 1);
+  ALLOC(d.nblocks, 1);
+  ALLOC(d.l2g, (size_t)MB * kBlockAtoms);
+  ALLOC(d.g2s, n);
+  ALLOC(d.bblo, 3 * (size_t)MB);
+  ALLOC(d.bbhi, 3 * (size_t)MB);
+  ALLOC(d.bbc, MB);
+  ALLOC(d.bbh, MB);
+  ALLOC(d.spos, (size_t)MB * kBlockAtoms);
+  ALLOC(d.spar, (size_t)MB * kBlockAtoms);
+  ALLOC(d.tcount, MB);
+  ALLOC(d.tstart, MB + 1);
+  ALLOC(d.ntiles, 1);
+  // +1: the exclusive scans also write the total at index n
+  ALLOC(d.tile_i, s->maxTiles + 1);
+  ALLOC(d.tile_j, s->maxTiles + 1);
+  ALLOC(d.tile_ex, s->maxTiles + 1);
+  ALLOC(d.tile_flag, s->maxTiles + 1);
+  ALLOC(d.nexcltiles, 1);
+  d.nexcl = t->n_excl;
+  ALLOC(d.emask, (size_t)std::max(1, t->n_excl) * kBlockAtoms);
+  ALLOC(d.excl_tile, t->n_excl);
+  {
+    std::vector<int> ex(t->excl_idx, t->excl_idx + 2 * (size_t)t->n_excl);
+    UPLOAD(d.excl, ex);
+  }
+  ALLOC(d.workctr, 4);
+  ALLOC(d.errflag, 1);
+  // PME
+  if (rp->elec_method == BM_PME) {
+    for (int k = 0; k < 3; k++) d.K[k] = rp->grid[k];
+    d.order = rp->order;
+    const size_t G = (size_t)d.K[0] * d.K[1] * d.K[2];
+    ALLOC(d.qgrid, G);
+    ALLOC(d.cgrid, G);
+    std::vector<float2> tw;
+    for (int k = 0; k < 3; k++)
+      for (int m = 0; m < d.K[k]; m++) {
+        double a = -2.0 * M_PI * m / d.K[k];
+        tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+      }
+    UPLOAD(d.twiddle, tw);
+    for (int k = 0; k < 3; k++) {
+      // 1/|b(m)|^2 from the integer-point B-spline values (potential.cxx:1245-1287)
+      std::vector<float> bm(d.K[k]);
+      for (int i = 0; i < d.K[k]; i++) {
+        double re = 0, im = 0;
+        for (int l = 1; l < d.order; l++) {
+          double mv = bspline_host(d.order, l);
+          re += mv * std::cos(2 * M_PI * i * l / d.K[k]);
+          im += mv * std::sin(2 * M_PI * i * l / d.K[k]);
+        }
+        bm[i] = (float)(1.0 / (re * re + im * im));
+      }
+      UPLOAD(d.bmod[k], bm);
+    }
+    const double V = rp->box[0] * rp->box[1] * rp->box[2];
+    d.pmefac = (float)(0.5 * kElectricD / M_PI / V);
+    d.pmeexp = (float)(M_PI * M_PI / (rp->beta_ewald * rp->beta_ewald));
+  }
+  // bonded terms
+  {
+    d.nbond = t->n_bonds;
+    d.nbond12 = t->n_bonds12;
+    std::vector<int2> bi(d.nbond), bsb(d.nbond);
+    std::vector<float2> bp(d.nbond);
+    for (int b = 0; b < d.nbond; b++) {
+      bi[b] = make_int2(t->bond_idx[2 * b], t->bond_idx[2 * b + 1]);
+      bsb[b] = make_int2(t->bond_sb[2 * b], t->bond_sb[2 * b + 1]);
+      bp[b] = make_float2((float)t->bond_kb[b], (float)t->bond_b0[b]);
+    }
+    UPLOAD(d.bond_i, bi); UPLOAD(d.bond_sb, bsb); UPLOAD(d.bond_p, bp);
+    d.nangle = t->n_angles;
+    std::vector<int4> ai(d.nangle);
+    std::vector<int2> asb(d.nangle);
+    std::vector<float2> ap(d.nangle);
+    for (int a = 0; a < d.nangle; a++) {
+      ai[a] = make_int4(t->angle_idx[3 * a], t->angle_idx[3 * a + 1], t->angle_idx[3 * a + 2], 0);
+      asb[a] = make_int2(t->angle_sb[2 * a], t->angle_sb[2 * a + 1]);
+      ap[a] = make_float2((float)t->angle_k[a], (float)t->angle_t0[a]);
+    }
+    UPLOAD(d.angle_i, ai); UPLOAD(d.angle_sb, asb); UPLOAD(d.angle_p, ap);
+    for (int pass = 0; pass < 2; pass++) {
+      int cnt = pass ? t->n_imprs : t->n_dihes;
+      const int *idx = pass ? t->impr_idx : t->dihe_idx;
+      const int *sbv = pass ? t->impr_sb : t->dihe_sb;
+      const double *kk = pass ? t->impr_k : t->dihe_k;
+      const int *nn = pass ? t->impr_n : t->dihe_n;
+      const double *d0 = pass ? t->impr_d0 : t->dihe_d0;
+      std::vector<int4> di(cnt);
+      std::vector<int2> dsb(cnt);
+      std::vector<float4> dp(cnt);
+      for (int q = 0; q < cnt; q++) {
+        di[q] = make_int4(idx[4 * q], idx[4 * q + 1], idx[4 * q + 2], idx[4 * q + 3]);
+        dsb[q] = make_int2(sbv[2 * q], sbv[2 * q + 1]);
+        dp[q] = make_float4((float)kk[q], (float)nn[q], (float)d0[q], 0.f);
+      }
+      if (pass) { d.nimpr = cnt; UPLOAD(d.impr_i, di); UPLOAD(d.impr_sb, dsb); UPLOAD(d.impr_p, dp); }
+      else { d.ndihe = cnt; UPLOAD(d.dihe_i, di); UPLOAD(d.dihe_sb, dsb); UPLOAD(d.dihe_p, dp); }
+    }
+    d.nnb14 = t->n_nb14;
+    std::vector<int2> pi(d.nnb14), psb(d.nnb14);
+    std::vector<float4> pp(d.nnb14);
+    for (int q = 0; q < d.nnb14; q++) {
+      pi[q] = make_int2(t->nb14_idx[2 * q], t->nb14_idx[2 * q + 1]);
+      psb[q] = make_int2(t->nb14_sb[2 * q], t->nb14_sb[2 * q + 1]);
+      pp[q] = make_float4((float)t->nb14_qq[q], (float)t->nb14_c12[q], (float)t->nb14_c6[q], (float)t->nb14_e14fac[q]);
+    }
+    UPLOAD(d.nb14_i, pi); UPLOAD(d.nb14_sb, psb); UPLOAD(d.nb14_p, pp);
+    d.nnbex = rp->elec_method == BM_PME ? t->n_nbex : 0;
+    std::vector<int2> xi(d.nnbex), xsb(d.nnbex);
+    std::vector<float> xp(d.nnbex);
+    for (int q = 0; q < d.nnbex; q++) {
+      xi[q] = make_int2(t->nbex_idx[2 * q], t->nbex_idx[2 * q + 1]);
+      xsb[q] = make_int2(t->nbex_sb[2 * q], t->nbex_sb[2 * q + 1]);
+      xp[q] = (float)t->nbex_qq[q];
+    }
+    UPLOAD(d.nbex_i, xi); UPLOAD(d.nbex_sb, xsb); UPLOAD(d.nbex_p, xp);
+  }
+  // constraints
+  {
+    d.ntri = t->n_tri; d.nbr1 = t->n_br1; d.nbr2 = t->n_br2; d.nbr3 = t->n_br3;
+    std::vector<int4> ti(d.ntri), b2i(d.nbr2), b3i(d.nbr3);
+    std::vector<double4> tb(d.ntri), b3b(d.nbr3);
+    std::vector<int2> b1i(d.nbr1);
+    std::vector<double> b1b(d.nbr1);
+    std::vector<double2> b2b(d.nbr2);
+    for (int c = 0; c < d.ntri; c++) {
+      ti[c] = make_int4(t->tri_idx[3 * c], t->tri_idx[3 * c + 1], t->tri_idx[3 * c + 2], 0);
+      tb[c] = make_double4(t->tri_b0[3 * c], t->tri_b0[3 * c + 1], t->tri_b0[3 * c + 2], 0);
+    }
+    for (int c = 0; c < d.nbr1; c++) {
+      b1i[c] = make_int2(t->br1_idx[2 * c], t->br1_idx[2 * c + 1]);
+      b1b[c] = t->br1_b0[c];
+    }
+    for (int c = 0; c < d.nbr2; c++) {
+      b2i[c] = make_int4(t->br2_idx[3 * c], t->br2_idx[3 * c + 1], t->br2_idx[3 * c + 2], 0);
+      b2b[c] = make_double2(t->br2_b0[2 * c], t->br2_b0[2 * c + 1]);
+    }
+    for (int c = 0; c < d.nbr3; c++) {
+      b3i[c] = make_int4(t->br3_idx[4 * c], t->br3_idx[4 * c + 1], t->br3_idx[4 * c + 2], t->br3_idx[4 * c + 3]);
+      b3b[c] = make_double4(t->br3_b0[3 * c], t->br3_b0[3 * c + 1], t->br3_b0[3 * c + 2], 0);
+    }
+    UPLOAD(d.tri_i, ti); UPLOAD(d.tri_b, tb);
+    UPLOAD(d.br1_i, b1i); UPLOAD(d.br1_b, b1b);
+    UPLOAD(d.br2_i, b2i); UPLOAD(d.br2_b, b2b);
+    UPLOAD(d.br3_i, b3i); UPLOAD(d.br3_b, b3b);
+  }
+  // msld
+  {
+    std::vector<int> bs(nl, 0), sbd(d.nsites + 1, 0), bf(nl, 0);
+    std::vector<double> fb(nl, 0.0), lf(nl), ln(nl);
+    const double kT = kBoltzD * rp->temperature;
+    for (int b = 0; b < t->n_blocks; b++) {
+      bs[b] = t->block_site[b];
+      bf[b] = t->block_fixed ? t->block_fixed[b] : 0;
+      fb[b] = t->fixed_bias ? t->fixed_bias[b] : 0.0;
+    }
+    if (t->n_blocks >= 1 && t->site_bound)
+      for (int q = 0; q <= t->n_sites; q++) sbd[q] = t->site_bound[q];
+    else
+      sbd[1] = 1;
+    for (int b = 0; b < nl; b++) {
+      double g = (t->theta_friction && b < t->n_blocks) ? t->theta_friction[b] : rp->msld_gamma;
+      double a = std::exp(-g * rp->dt);
+      lf[b] = a;
+      ln[b] = std::sqrt((1 - a * a) * kT);
+    }
+    UPLOAD(d.block_site, bs); UPLOAD(d.site_bound, sbd); UPLOAD(d.block_fixed, bf);
+    UPLOAD(d.fixed_bias, fb); UPLOAD(d.lfric, lf); UPLOAD(d.lnoise, ln);
+    d.nvb = t->n_varbias;
+    std::vector<int4> vi(d.nvb);
+    std::vector<double2> vp(d.nvb);
+    for (int v = 0; v < d.nvb; v++) {
+      vi[v] = make_int4(t->vb_i[v], t->vb_j[v], t->vb_type[v], t->vb_n[v]);
+      vp[v] = make_double2(t->vb_l0[v], t->vb_k[v]);
+    }
+    UPLOAD(d.vb_i, vi); UPLOAD(d.vb_p, vp);
+    d.fnex = rp->fnex;
+    d.softcore = rp->soft_core;
+    d.softcore14 = rp->soft_core14;
+  }
+  // integrator parameters (system/state.cxx:282-297)
+  {
+    const double kT = kBoltzD * rp->temperature;
+    d.halfdt = 0.5 * rp->dt;
+    double a = std::exp(-rp->gamma * rp->dt);
+    d.fric = a;
+    d.noise = std::sqrt((1 - a * a) * kT);
+    d.shaketol = rp->shake_tol;
+    d.seed = rp->seed;
+  }
+  // nbdirect launch width: enough waves to fill 256 CUs several times
+  s->nwgNb = 1024;
+#undef ALLOC
+#undef UPLOAD
+  if (s->alch) launch_theta_to_lambda(d, s->st);
+  if (hipStreamSynchronize(s->st) != hipSuccess) return fail(set_err(BM_EHIP, "init sync"));
+  *out = s;
+  return BM_OK;
+}
+
+int bm_system_destroy(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  (void)hipSetDevice(s->device);
+  delete s;
+  return BM_OK;
+}
+
+int bm_sync(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  HIPCHK(hipStreamSynchronize(s->st));
+  int flag = 0;
+  HIPCHK(hipMemcpy(&flag, s->d.errflag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) return set_err(BM_EOVERFLOW, "a domdec column exceeded its capacity");
+  return BM_OK;
+}
+
+int bm_domdec_update(bm_system *s, int rebuild) {
+  if (!s) return BM_EINVAL;
+  if (rebuild) blade::launch_domdec_rebuild(s->d, s->maxBlocks, s->st);
+  blade::launch_domdec_pack(s->d, s->maxBlocks, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+
+int bm_reset_force(bm_system *s, unsigned) {
+  if (!s) return BM_EINVAL;
+  blade::launch_domdec_pack(s->d, s->maxBlocks, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+
+int bm_getforce_nbdirect(bm_system *s, unsigned flags) {
+  if (!s) return BM_EINVAL;
+  blade::launch_nbdirect(s->d, flags & BM_CALC_ENERGY, s->alch, s->soft, s->nwgNb, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+
+int bm_getforce_ewald(bm_system *s, unsigned flags) {
+  if (!s) return BM_EINVAL;
+  if (s->d.elecmethod == BM_PME) blade::launch_pme(s->d, flags & BM_CALC_ENERGY, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+
+// the self term is folded into the gather kernel of bm_getforce_ewald
+int bm_getforce_ewaldself(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
+
+int bm_getforce_bonded(bm_system *s, unsigned flags) {
+  if (!s) return BM_EINVAL;
+  blade::launch_bonded(s->d, flags & BM_CALC_ENERGY, s->bias, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+// 1-4 pairs, exclusion corrections and lambda biases share the bonded launch
+int bm_getforce_nb14(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
+int bm_getforce_nbex(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
+int bm_getforce_msldbias(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
+
+int bm_calc_force(bm_system *s, long, unsigned flags) {
+  if (!s) return BM_EINVAL;
+  const bool e = flags & BM_CALC_ENERGY;
+  blade::launch_nbdirect(s->d, e, s->alch, s->soft, s->nwgNb, s->st);
+  if (s->d.elecmethod == BM_PME) blade::launch_pme(s->d, e, s->st);
+  blade::launch_bonded(s->d, e, s->bias, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+
+int bm_update(bm_system *s, long step, int energy_step) {
+  if (!s) return BM_EINVAL;
+  blade::launch_set_step(s->d, step, s->st);
+  blade::launch_update(s->d, energy_step != 0, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+
+int bm_holonomic_position(bm_system *s) { return s ? BM_OK : BM_EINVAL; }
+int bm_holonomic_velocity(bm_system *s) { return s ? BM_OK : BM_EINVAL; }
+
+int bm_msld_theta_to_lambda(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  blade::launch_theta_to_lambda(s->d, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+int bm_msld_chain_rule(bm_system *s) { return s ? BM_OK : BM_EINVAL; }
+
+int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_graph) {
+  if (!s || nsteps < 0) return BM_EINVAL;
+  if (freq_nrg <= 0) freq_nrg = 1 << 30;
+  blade::launch_set_step(s->d, step0, s->st);
+  for (long k = 0; k < nsteps; k++) {
+    const long step = step0 + k;
+    const int rb = (step % s->freqDomdec) == 0, en = (step % freq_nrg) == 0;
+    if (!use_graph) {
+      s->step_launches(rb, en);
+      HIPCHK(hipGetLastError());
+      continue;
+    }
+    if (!s->graph[rb][en]) {
+      HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+      s->step_launches(rb, en);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamEndCapture(s->st, &g));
+      s->graphSrc[rb][en] = g;
+      HIPCHK(hipGraphInstantiate(&s->graph[rb][en], g, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(s->graph[rb][en], s->st));
+  }
+  return BM_OK;
+}
+
+// Per-phase timing with events on the system stream: nsteps eager steps
+// (rebuild every freq_domdec steps, no energies) with events around the
+// nbdirect and PME launches.  out = {ms nbdirect, ms pme, ms step} averages.
+int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
+  if (!s || nsteps <= 0 || !out) return BM_EINVAL;
+  double tn = 0, tp = 0, tt = 0;
+  blade::launch_set_step(s->d, step0, s->st);
+  for (long k = 0; k < nsteps; k++) {
+    const long step = step0 + k;
+    const bool rb = (step % s->freqDomdec) == 0;
+    HIPCHK(hipEventRecord(s->ev[0], s->st));
+    if (rb) blade::launch_domdec_rebuild(s->d, s->maxBlocks, s->st);
+    blade::launch_domdec_pack(s->d, s->maxBlocks, s->st);
+    HIPCHK(hipEventRecord(s->ev[1], s->st));
+    blade::launch_nbdirect(s->d, false, s->alch, s->soft, s->nwgNb, s->st);
+    HIPCHK(hipEventRecord(s->ev[2], s->st));
+    if (s->d.elecmethod == BM_PME) blade::launch_pme(s->d, false, s->st);
+    HIPCHK(hipEventRecord(s->ev[3], s->st));
+    blade::launch_bonded(s->d, false, s->bias, s->st);
+    blade::launch_update(s->d, false, s->st);
+    HIPCHK(hipEventRecord(s->ev[4], s->st));
+    HIPCHK(hipEventSynchronize(s->ev[4]));
+    float a = 0, b = 0, c = 0;
+    HIPCHK(hipEventElapsedTime(&a, s->ev[1], s->ev[2]));
+    HIPCHK(hipEventElapsedTime(&b, s->ev[2], s->ev[3]));
+    HIPCHK(hipEventElapsedTime(&c, s->ev[0], s->ev[4]));
+    tn += a; tp += b; tt += c;
+  }
+  out[0] = tn / nsteps;
+  out[1] = tp / nsteps;
+  out[2] = tt / nsteps;
+  s->msNb = out[0]; s->msPme = out[1]; s->msTotal = out[2]; s->nMeas = nsteps;
+  return BM_OK;
+}
+
+int bm_get_kernel_times(bm_system *s, double *ms_nb, double *ms_pme, double *ms_total, long *n) {
+  if (!s) return BM_EINVAL;
+  if (ms_nb) *ms_nb = s->msNb;
+  if (ms_pme) *ms_pme = s->msPme;
+  if (ms_total) *ms_total = s->msTotal;
+  if (n) *n = s->nMeas;
+  return BM_OK;
+}
+
+static int d2h(bm_system *s, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return BM_OK;
+}
+
+int bm_get_positions(bm_system *s, double *pos, double *theta) {
+  if (!s) return BM_EINVAL;
+  const int n = s->d.n;
+  std::vector<double> x(3 * (size_t)n + s->d.nl);
+  if (int rc = d2h(s, x.data(), s->d.pos, x.size() * sizeof(double))) return rc;
+  if (pos) std::memcpy(pos, x.data(), 3 * (size_t)n * sizeof(double));
+  if (theta) std::memcpy(theta, x.data() + 3 * (size_t)n, s->d.nl * sizeof(double));
+  return BM_OK;
+}
+
+int bm_set_positions(bm_system *s, const double *pos, const double *theta) {
+  if (!s) return BM_EINVAL;
+  const int n = s->d.n;
+  if (pos) HIPCHK(hipMemcpyAsync(s->d.pos, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s->st));
+  if (theta) {
+    HIPCHK(hipMemcpyAsync(s->d.pos + 3 * (size_t)n, theta, s->d.nl * sizeof(double), hipMemcpyHostToDevice, s->st));
+    if (s->alch) blade::launch_theta_to_lambda(s->d, s->st);
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return BM_OK;
+}
+
+int bm_get_velocities(bm_system *s, double *vel, double *theta_vel) {
+  if (!s) return BM_EINVAL;
+  const int n = s->d.n;
+  std::vector<double> v(3 * (size_t)n + s->d.nl);
+  if (int rc = d2h(s, v.data(), s->d.vel, v.size() * sizeof(double))) return rc;
+  if (vel) std::memcpy(vel, v.data(), 3 * (size_t)n * sizeof(double));
+  if (theta_vel) std::memcpy(theta_vel, v.data() + 3 * (size_t)n, s->d.nl * sizeof(double));
+  return BM_OK;
+}
+
+int bm_get_forces(bm_system *s, double *force, double *theta_force) {
+  if (!s) return BM_EINVAL;
+  const int n = s->d.n;
+  std::vector<long long> f(3 * (size_t)n + s->d.nl);
+  if (int rc = d2h(s, f.data(), s->d.force, f.size() * sizeof(long long))) return rc;
+  if (force)
+    for (size_t i = 0; i < 3 * (size_t)n; i++) force[i] = (double)f[i] * blade::kFixInvD;
+  if (theta_force)
+    for (int b = 0; b < s->d.nl; b++) theta_force[b] = (double)f[3 * (size_t)n + b] * blade::kFixInvD;
+  return BM_OK;
+}
+
+int bm_get_force_fixed(bm_system *s, long long *force) {
+  if (!s || !force) return BM_EINVAL;
+  return d2h(s, force, s->d.force, (3 * (size_t)s->d.n + s->d.nl) * sizeof(long long));
+}
+
+int bm_get_dudl(bm_system *s, double *dudl) {
+  if (!s || !dudl) return BM_EINVAL;
+  std::vector<long long> f(s->d.nl);
+  if (int rc = d2h(s, f.data(), s->d.dudl, f.size() * sizeof(long long))) return rc;
+  for (int b = 0; b < s->d.nl; b++) dudl[b] = (double)f[b] * blade::kFixInvD;
+  return BM_OK;
+}
+
+int bm_get_lambda(bm_system *s, double *lambda) {
+  if (!s || !lambda) return BM_EINVAL;
+  return d2h(s, lambda, s->d.lambda, s->d.nl * sizeof(double));
+}
+
+int bm_get_energy(bm_system *s, double *energy) {
+  if (!s || !energy) return BM_EINVAL;
+  long long e[BM_NTERMS];
+  if (int rc = d2h(s, e, s->d.energy, sizeof(e))) return rc;
+  double pot = 0;
+  for (int k = 0; k < BM_NTERMS; k++) energy[k] = (double)e[k] * blade::kFixInvD;
+  for (int k = 0; k < BM_EPOTENTIAL; k++) pot += energy[k];
+  energy[BM_EPOTENTIAL] = pot;
+  energy[BM_ETOTAL] = pot + energy[BM_EKINETIC];
+  return BM_OK;
+}
+
+int bm_get_nan_flag(bm_system *s, int *flag) {
+  if (!s || !flag) return BM_EINVAL;
+  return d2h(s, flag, s->d.nanflag, sizeof(int));
+}
+
+int bm_get_domdec_sizes(bm_system *s, int *nb, int *nt, int *ne) {
+  if (!s) return BM_EINVAL;
+  int v[3] = {0, 0, 0};
+  if (int rc = d2h(s, &v[0], s->d.nblocks, sizeof(int))) return rc;
+  if (int rc = d2h(s, &v[1], s->d.ntiles, sizeof(int))) return rc;
+  if (int rc = d2h(s, &v[2], s->d.nexcltiles, sizeof(int))) return rc;
+  if (nb) *nb = v[0];
+  if (nt) *nt = v[1];
+  if (ne) *ne = v[2];
+  return BM_OK;
+}
+
+int bm_get_domdec(bm_system *s, int *l2g, int *bcount, int *ti, int *tj, unsigned *tmask) {
+  if (!s) return BM_EINVAL;
+  int nb, nt, ne;
+  if (int rc = bm_get_domdec_sizes(s, &nb, &nt, &ne)) return rc;
+  std::vector<int> L((size_t)nb * blade::kBlockAtoms), I(nt), J(nt), X(nt);
+  std::vector<unsigned> M((size_t)std::max(ne, 1) * blade::kBlockAtoms);
+  if (int rc = d2h(s, L.data(), s->d.l2g, L.size() * sizeof(int))) return rc;
+  if (nt) {
+    if (int rc = d2h(s, I.data(), s->d.tile_i, nt * sizeof(int))) return rc;
+    if (int rc = d2h(s, J.data(), s->d.tile_j, nt * sizeof(int))) return rc;
+    if (int rc = d2h(s, X.data(), s->d.tile_ex, nt * sizeof(int))) return rc;
+  }
+  if (ne)
+    if (int rc = d2h(s, M.data(), s->d.emask, (size_t)ne * blade::kBlockAtoms * sizeof(unsigned))) return rc;
+  if (l2g) std::memcpy(l2g, L.data(), L.size() * sizeof(int));
+  if (bcount)
+    for (int b = 0; b < nb; b++) {
+      int c = 0;
+      for (int l = 0; l < blade::kBlockAtoms; l++) c += L[(size_t)b * blade::kBlockAtoms + l] >= 0;
+      bcount[b] = c;
+    }
+  for (int t = 0; t < nt; t++) {
+    if (ti) ti[t] = I[t];
+    if (tj) tj[t] = J[t];
+    if (tmask)
+      for (int l = 0; l < blade::kBlockAtoms; l++)
+        tmask[(size_t)t * blade::kBlockAtoms + l] = X[t] >= 0 ? M[(size_t)X[t] * blade::kBlockAtoms + l] : 0u;
+  }
+  return BM_OK;
+}
+
+}  // extern "C"

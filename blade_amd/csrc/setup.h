@@ -1,0 +1,137 @@
+// setup.h - host-side system description and term derivation.
+//
+// Mirrors the reference's host model: Parameters (system/parameters.h),
+// Structure (system/structure.h), Coordinates (system/coordinates.h), the
+// setup half of Msld (msld/msld.h) and Run's options (run/run.h).  The
+// derivation in Topology::build restates Potential::initialize
+// (system/potential.cxx:525-1837): bonded term lookup, Urey-Bradley,
+// exclusion sets, 1-4 pairs, PME exclusion corrections, vdW type table,
+// constraint grouping and MSLD siteBlock codes.  No HIP in this file.
+#pragma once
+
+#include <array>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/blade_mi355x.h"
+
+namespace blade {
+
+struct BondParam { double kb, b0; };
+struct AngleParam { double kangle, angle0, kureyb, ureyb0; };
+struct DiheParam { double kdih; int ndih; double dih0; };
+struct ImprParam { double kimp; int nimp; double imp0; };
+struct NbondParam { double eps, sig, eps14, sig14, e14fac; int combine; };
+
+// Type-name tuples match forwards or reversed (parameters.cxx:40-88).
+std::string canon(std::vector<std::string> t);
+
+struct Parameters {
+  std::map<std::string, double> atomMass;
+  std::map<std::string, BondParam> bonds;
+  std::map<std::string, AngleParam> angles;
+  std::map<std::string, std::vector<DiheParam>> dihes;
+  std::map<std::string, ImprParam> imprs;
+  std::map<std::string, NbondParam> nbonds;
+  std::map<std::string, NbondParam> nbfix;  // key canon({t1,t2})
+};
+
+struct AtomRec {
+  std::string segName, resIdx, resName, atomName, atomTypeName;
+  double charge, mass;
+};
+
+struct Structure {
+  std::vector<AtomRec> atoms;
+  std::vector<std::array<int, 2>> bonds;
+  std::vector<std::array<int, 3>> angles;
+  std::vector<std::array<int, 4>> dihes, imprs;
+  bool shakeHbond = false;
+};
+
+struct Coordinates {
+  int n = 0;
+  std::vector<double> pos, vel;  // AoS [n][3] as the reference stores them
+  double box[3] = {0, 0, 0};
+  double angles[3] = {90, 90, 90};
+  int boxName = 0;
+};
+
+struct VarBias { int i, j, type; double l0, k; int n; };
+
+struct MsldSetup {
+  int blockCount = 1;
+  std::vector<int> atomBlock;   // [N]
+  std::vector<int> lambdaSite;  // [nL]
+  std::vector<double> theta, thetaVelocity, thetaMass, lambdaBias, lambdaCharge;
+  std::vector<double> thetaFriction;  // empty => msld gamma
+  std::vector<int> blockFixed;
+  bool scaleTerms[6] = {true, true, true, true, true, true};
+  double gamma = 1.0 / 20.454828284385908;  // ps^-1 in AKMA (msld.cu:35)
+  double fnex = 5.5;
+  bool useSoftCore = false, useSoftCore14 = false;
+  int msldEwaldType = 2;
+  bool fix = false;
+  std::vector<VarBias> varBias;
+  int softBondCount = 0;
+  void resize(int nblocks, int natoms);
+};
+
+struct RunSetup {
+  long step = 0, step0 = 0, nsteps = 5000;
+  double dt = 0.001 * 20.454828284385908;
+  double T = 300;
+  double gamma = 1.0 / 20.454828284385908;
+  double betaEwald = 1 / 3.2, rCut = 10, rSwitch = 8.5;
+  int vdwMethod = BM_VFSWITCH, elecMethod = BM_PME;
+  double gridSpace = 1.0;
+  int grid[3] = {-1, -1, -1};
+  int orderEwald = 6;
+  double shakeTolerance = 2e-7;
+  int freqNRG = 10;
+  int freqNPT = 0;
+  unsigned long long seed = 0xB1ADE000ull;
+};
+
+// Derived term tables owning their storage; view() exposes bm_topology.
+struct Topology {
+  int n = 0;
+  std::vector<double> charge, mass;
+  std::vector<int> typeIdx, siteBlock;
+  std::vector<std::string> typeNames;
+  int nTypes = 0;
+  std::vector<double> c12, c6;
+  int nBonds12 = 0;
+  std::vector<int> bondIdx, bondSb; std::vector<double> bondKb, bondB0;
+  std::vector<int> angleIdx, angleSb; std::vector<double> angleK, angleT0;
+  std::vector<int> diheIdx, diheSb, diheN; std::vector<double> diheK, diheD0;
+  std::vector<int> imprIdx, imprSb, imprN; std::vector<double> imprK, imprD0;
+  std::vector<int> nb14Idx, nb14Sb; std::vector<double> nb14qq, nb14c12, nb14c6, nb14e14;
+  std::vector<int> nbexIdx, nbexSb; std::vector<double> nbexqq;
+  std::vector<int> exclIdx;
+  std::vector<int> triIdx, br1Idx, br2Idx, br3Idx;
+  std::vector<double> triB0, br1B0, br2B0, br3B0;
+  int nBlocks = 1, nSites = 1;
+  std::vector<int> blockSite, siteBound, blockFixed;
+  std::vector<double> thetaMass, fixedBias, thetaFriction;
+  std::vector<int> vbI, vbJ, vbType, vbN; std::vector<double> vbL0, vbK;
+  bm_topology top{};
+  const bm_topology *view();
+};
+
+// Error raised by derivation with a message (the reference calls fatal()).
+struct SetupError { std::string msg; };
+
+// Build the derived topology (Potential::initialize restatement).
+void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
+                    const RunSetup &r, Topology &t);
+
+// Run parameters for the kernels; box from coordinates.
+void build_runparams(const RunSetup &r, const MsldSetup &m, const Coordinates &c, int natoms,
+                     bm_runparams &rp);
+
+// PME grid size rule (system/potential.cxx:1205-1221).
+int pme_grid_size(double L, double gridSpace, int explicitSize);
+
+}  // namespace blade

@@ -1,0 +1,604 @@
+"""Seeded synthetic MSLD systems for the benchmark workloads.
+
+Each generator returns a `Spec`: CHARMM-style parameters, structure,
+coordinates, lambda sites and run options, which `blade_amd.system.build`
+feeds through the reference-compatible blade_* API.  Units are the
+reference's library units: kcal/mol, A, amu, e, AKMA time.
+
+Workloads (sizes as named in the benchmark descriptions):
+  water_box     N TIP3P waters on a jittered lattice, random orientations
+  config1       216 TIP3P waters, 18.856 A box, switch 8-9 A, kappa 0.34,
+                PME 20^3 order 6, 298 K, gamma 1/ps, dt 2 fs, LJ on O only
+  config2       23,558 atoms in a 62.23 A cube: 7,023 TIP3P waters plus a
+                2,489-atom synthetic C/N/O/H polymer carrying one lambda
+                site of 3 substituents x 8 atoms; switch 9-10 A, PME 64^3
+                order 6, fnex 5
+  msld_small    a few-hundred-atom water box with a 3-substituent site
+                (fast parity cases)
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PICOSECOND = 20.454828284385908  # AKMA time units per ps
+KBOLTZ = 0.001987191
+
+# ---------------------------------------------------------------------------
+# parameters (CHARMM conventions: K in kcal/mol/A^2 per the PRM files, the
+# library doubles bond/angle constants; eps negative, Rmin/2)
+# ---------------------------------------------------------------------------
+WATER_TYPES = {"OT": 15.9994, "HT": 1.008}
+WATER_BONDS = {("OT", "HT"): (450.0, 0.9572), ("HT", "HT"): (0.0, 1.5139)}
+WATER_ANGLES = {("HT", "OT", "HT"): (55.0, 104.52, 0.0, 0.0)}
+# LJ on the oxygen only (the benchmark variant of CHARMM TIP3P)
+WATER_NB = {"OT": (-0.1521, 1.7682, -0.1521, 1.7682, 1.0, 0),
+            "HT": (0.0, 0.2245, 0.0, 0.2245, 1.0, 0)}
+
+POLY_TYPES = {"CT": 12.011, "NT": 14.007, "OP": 15.9994, "HA": 1.008, "HN": 1.008, "HO": 1.008}
+POLY_BONDS = {("CT", "CT"): (222.5, 1.530), ("CT", "NT"): (320.0, 1.470), ("CT", "OP"): (360.0, 1.430),
+              ("CT", "HA"): (309.0, 1.090), ("NT", "HN"): (440.0, 1.010), ("OP", "HO"): (545.0, 0.960)}
+POLY_NB = {"CT": (-0.0780, 2.0500, -0.0100, 1.9000, 1.0, 0),
+           "NT": (-0.2000, 1.8500, -0.2000, 1.5500, 1.0, 0),
+           "OP": (-0.1521, 1.7700, -0.1521, 1.7700, 1.0, 0),
+           "HA": (-0.0220, 1.3200, -0.0220, 1.3200, 1.0, 0),
+           "HN": (-0.0460, 0.2245, -0.0460, 0.2245, 1.0, 0),
+           "HO": (-0.0460, 0.2245, -0.0460, 0.2245, 1.0, 0)}
+HEAVY_BOND = {frozenset(("CT", "CT")): 1.530, frozenset(("CT", "NT")): 1.470, frozenset(("CT", "OP")): 1.430}
+H_OF = {"CT": ("HA", 1.090), "NT": ("HN", 1.010), "OP": ("HO", 0.960)}
+
+
+@dataclass
+class Spec:
+    name: str
+    box: tuple
+    types: dict = field(default_factory=dict)        # name -> mass
+    bond_par: dict = field(default_factory=dict)     # (t1,t2) -> (kb, b0)
+    angle_par: dict = field(default_factory=dict)    # (t1,t2,t3) -> (k, t0deg, kub, s0)
+    dihe_par: dict = field(default_factory=dict)     # (t1..t4) -> [(k, n, d0deg)]
+    impr_par: dict = field(default_factory=dict)     # (t1..t4) -> (k, n, d0deg)
+    nb_par: dict = field(default_factory=dict)       # t -> (eps, rmin2, eps14, rmin2_14, e14fac, combine)
+    atoms: list = field(default_factory=list)        # (seg, resid, resname, name, type, q, mass)
+    bonds: list = field(default_factory=list)        # 0-based
+    angles: list = field(default_factory=list)
+    dihes: list = field(default_factory=list)
+    imprs: list = field(default_factory=list)
+    pos: np.ndarray = None                           # (N,3) A
+    vel: np.ndarray = None                           # (N,3) A/AKMA
+    shake: bool = True
+    # msld
+    nblocks: int = 1
+    atom_block: np.ndarray = None                    # (N,) 0-based blocks
+    block_site: list = field(default_factory=lambda: [0])
+    theta0: list = field(default_factory=lambda: [0.0])
+    theta_mass: list = field(default_factory=lambda: [0.0])
+    fixed_bias: list = field(default_factory=lambda: [0.0])
+    var_bias: list = field(default_factory=list)     # (i, j, type, l0, k, n) 0-based blocks
+    fnex: float = 5.5
+    soft_core: bool = False
+    # run options
+    dt: float = 0.002 * PICOSECOND
+    temperature: float = 298.0
+    gamma: float = 1.0 / PICOSECOND
+    msld_gamma: float = 1.0 / PICOSECOND
+    beta: float = 1.0 / 3.2
+    rcut: float = 10.0
+    rswitch: float = 9.0
+    grid: tuple = (-1, -1, -1)
+    grid_space: float = 1.0
+    order: int = 6
+    shake_tol: float = 2e-7
+    seed: int = 0xB1ADE000
+    nsteps: int = 1000
+
+    @property
+    def n(self):
+        return len(self.atoms)
+
+
+def _rot(rng):
+    """uniform random rotation matrix (unit quaternion)"""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _water_frame():
+    b, th = 0.9572, math.radians(104.52)
+    h1 = np.array([b * math.sin(th / 2), b * math.cos(th / 2), 0.0])
+    h2 = np.array([-b * math.sin(th / 2), b * math.cos(th / 2), 0.0])
+    return np.zeros(3), h1, h2
+
+
+class _Grid:
+    """periodic cell list for clash checks while building"""
+
+    def __init__(self, box, cell=3.0):
+        self.box = np.asarray(box, float)
+        self.nc = np.maximum(1, (self.box // cell).astype(int))
+        self.cells = {}
+        self.pts = []
+
+    def key(self, p):
+        f = np.floor((p % self.box) / self.box * self.nc).astype(int) % self.nc
+        return tuple(f)
+
+    def add(self, p):
+        self.pts.append(np.asarray(p, float))
+        self.cells.setdefault(self.key(p), []).append(len(self.pts) - 1)
+
+    def remove(self, i):
+        """drop point i from the cell lists (its index stays reserved)"""
+        self.cells[self.key(self.pts[i])].remove(i)
+
+    def near(self, p):
+        k = self.key(p)
+        out = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    kk = ((k[0] + dx) % self.nc[0], (k[1] + dy) % self.nc[1], (k[2] + dz) % self.nc[2])
+                    out += self.cells.get(kk, [])
+        return out
+
+    def mindist(self, p, skip=()):
+        idx = [i for i in self.near(p) if i not in skip]
+        if not idx:
+            return 1e9
+        d = np.array([self.pts[i] for i in idx]) - p
+        d -= self.box * np.round(d / self.box)
+        return float(np.sqrt((d * d).sum(1)).min())
+
+
+def _maxwell(rng, masses, T):
+    sig = np.sqrt(KBOLTZ * T / np.asarray(masses))[:, None]
+    v = rng.normal(size=(len(masses), 3)) * sig
+    m = np.asarray(masses)[:, None]
+    v -= (m * v).sum(0) / m.sum()
+    return v
+
+
+def _add_water(spec, o, rot, resid):
+    base = len(spec.atoms)
+    f0, f1, f2 = _water_frame()
+    c = o - f0
+    for nm, t, q, f in (("OH2", "OT", -0.834, f0), ("H1", "HT", 0.417, f1), ("H2", "HT", 0.417, f2)):
+        spec.atoms.append(("WAT", str(resid), "TIP3", nm, t, q, WATER_TYPES[t]))
+        spec._pos.append(o + rot @ f)
+    spec.bonds += [(base, base + 1), (base, base + 2), (base + 1, base + 2)]
+    spec.angles.append((base + 1, base, base + 2))
+
+
+def _water_params(spec):
+    spec.types.update(WATER_TYPES)
+    spec.bond_par.update(WATER_BONDS)
+    spec.angle_par.update(WATER_ANGLES)
+    spec.nb_par.update(WATER_NB)
+
+
+def water_box(nwat, box, seed, name="water"):
+    rng = np.random.default_rng(seed)
+    spec = Spec(name=name, box=tuple(float(b) for b in box))
+    spec._pos = []
+    _water_params(spec)
+    n1 = int(math.ceil(round(nwat ** (1 / 3), 9)))
+    sites = [(i, j, k) for i in range(n1) for j in range(n1) for k in range(n1)]
+    rng.shuffle(sites)
+    sp = np.array(box) / n1
+    for r, (i, j, k) in enumerate(sorted(sites[:nwat])):
+        o = (np.array([i, j, k]) + 0.5) * sp
+        _add_water(spec, o, _rot(rng), r + 1)
+    spec.pos = np.array(spec._pos)
+    spec.atom_block = np.zeros(spec.n, dtype=np.int32)
+    return spec, rng
+
+
+def config1(seed=0xB1ADE001):
+    """216 TIP3P waters, 648 atoms (the water216 benchmark case)."""
+    spec, rng = water_box(216, (18.856,) * 3, seed, name="config1-water216")
+    spec.rcut, spec.rswitch, spec.beta = 9.0, 8.0, 0.34
+    spec.grid, spec.order = (20, 20, 20), 6
+    spec.temperature = 298.0
+    spec.nsteps = 1000
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
+
+
+# ---------------------------------------------------------------------------
+# synthetic polymer
+# ---------------------------------------------------------------------------
+def _poly_params(spec):
+    spec.types.update(POLY_TYPES)
+    spec.bond_par.update(POLY_BONDS)
+    heavy = ["CT", "NT", "OP"]
+    hyd = ["HA", "HN", "HO"]
+    for a in heavy + hyd:
+        for b in heavy:
+            for c in heavy + hyd:
+                spec.angle_par[(a, b, c)] = (50.0, 111.0 if b == "CT" else 110.0, 0.0, 0.0)
+    for b in heavy:
+        for c in heavy:
+            spec.dihe_par[("X", b, c, "X")] = [(0.2, 3, 0.0), (0.1, 1, 180.0)]
+    spec.impr_par[("NT", "X", "X", "HN")] = (20.0, 0, 0.0)
+    spec.nb_par.update(POLY_NB)
+
+
+def _place(prev, prev2, prev3, b, theta, phi):
+    """position at bond length b, angle theta (prev2-prev-new), dihedral phi"""
+    bc = prev - prev2
+    bc /= np.linalg.norm(bc)
+    n = np.cross(prev2 - prev3, bc)
+    nn = np.linalg.norm(n)
+    if nn < 1e-6:
+        n = np.cross(bc, np.array([1.0, 0.3, 0.1]))
+        nn = np.linalg.norm(n)
+    n /= nn
+    m = np.cross(n, bc)
+    d2 = np.array([-b * math.cos(theta), b * math.sin(theta) * math.cos(phi), b * math.sin(theta) * math.sin(phi)])
+    return prev + d2[0] * bc + d2[1] * m + d2[2] * n
+
+
+def _polymer(spec, rng, natoms, center, radius, grid, site_at=None, nsubs=3, sub_size=8):
+    """Self-avoiding chain of C/N/O heavy atoms with hydrogens, confined to a
+    sphere, exactly `natoms` atoms including the lambda-site substituents."""
+    _poly_params(spec)
+    body = natoms - (nsubs * sub_size if site_at is not None else 0)
+    # the attachment carbon gives up one hydrogen to the substituents
+    budget = body + (1 if site_at is not None else 0)
+    nH = {"CT": 2, "NT": 1, "OP": 0}
+    pos, types, gidx = [], [], []
+
+    def ok(p, confine):
+        d = p - center
+        d -= np.array(spec.box) * np.round(d / np.array(spec.box))
+        if confine and np.linalg.norm(d) >= radius:
+            return False
+        # the 1-4 partner may sit gauche; everything farther along the chain
+        # keeps room for the hydrogens (no syn-pentane contacts)
+        if len(pos) >= 3:
+            r14 = pos[-3] - p
+            r14 -= np.array(spec.box) * np.round(r14 / np.array(spec.box))
+            if np.linalg.norm(r14) < 2.9:
+                return False
+        return grid.mindist(p, skip=set(gidx[-3:])) > 3.7
+
+    def extend(t, confine=True, tries=60):
+        if len(types) == 0:
+            p = np.array(center, float)
+        elif len(types) == 1:
+            p = pos[-1] + np.array([1.53, 0, 0])
+        elif len(types) == 2:
+            th = math.radians(111.0)
+            p = pos[-1] + 1.53 * np.array([-math.cos(th), math.sin(th), 0.0])
+        else:
+            b = HEAVY_BOND[frozenset((types[-1], t))]
+            for _ in range(tries):
+                phi = rng.choice([math.pi, math.pi / 3, -math.pi / 3]) + rng.normal(0, 0.25)
+                p = _place(pos[-1], pos[-2], pos[-3], b, math.radians(111.0), phi)
+                if ok(p, confine):
+                    break
+            else:
+                return False
+        pos.append(np.asarray(p, float))
+        types.append(t)
+        gidx.append(len(grid.pts))
+        grid.add(p)
+        return True
+
+    def retract(k):
+        """drop the last k heavy atoms (dead end)"""
+        nonlocal count
+        for _ in range(min(k, len(types) - 3)):
+            grid.remove(gidx.pop())
+            pos.pop()
+            count -= 1 + nH[types.pop()]
+
+    # methyl start (3 H), then interior units; stop with room for a
+    # terminal carbon carrying 0..3 hydrogens; back off on dead ends
+    extend("CT")
+    count = 4
+    fails = 0
+    while True:
+        t = "CT" if len(types) < 3 else rng.choice(["CT", "CT", "CT", "NT", "OP"], p=[0.25, 0.25, 0.2, 0.15, 0.15])
+        if t != "CT" and types[-1] != "CT":
+            t = "CT"
+        size = 1 + nH[t]
+        if count + size + 4 > budget:
+            break
+        if extend(t):
+            count += size
+            continue
+        fails += 1
+        if fails > 4000:
+            raise RuntimeError("polymer growth failed")
+        retract(1 + fails % 6)
+    while budget - count > 4:
+        if extend("CT", confine=False, tries=200):
+            count += 3
+        else:
+            retract(3)
+    rem = budget - count
+    while not extend("CT", confine=False, tries=200):
+        retract(2)
+        while budget - count > 4:
+            if extend("CT", confine=False, tries=200):
+                count += 3
+        rem = budget - count
+    hcount = [nH[t] for t in types]
+    hcount[0] = 3
+    hcount[-1] = rem - 1
+    assert len(types) + sum(hcount) == budget
+    q = rng.normal(0.0, 0.35, size=body)
+    q -= q.mean()
+    qi = iter(q)
+    heavy_idx = []
+    site_heavy = None
+    if site_at is not None:
+        site_heavy = max(1, int(site_at * len(types)))
+        while types[site_heavy] != "CT" or hcount[site_heavy] != 2 or site_heavy >= len(types) - 1:
+            site_heavy += 1
+        hcount[site_heavy] -= 1   # replaced by the substituent attachment
+    for k, t in enumerate(types):
+        idx = len(spec.atoms)
+        spec.atoms.append(("POLY", str(k // 4 + 1), "SYN", "%s%d" % (t[0], k), t, float(next(qi)), POLY_TYPES[t]))
+        spec._pos.append(pos[k])
+        heavy_idx.append(idx)
+    hatoms = {k: [] for k in range(len(types))}
+    def best_h(center_p, bonded, skip, blen, tries=400):
+        """hydrogen direction away from bonded partners with the largest
+        clearance from everything already placed"""
+        best, bestd = None, -1.0
+        for _ in range(tries):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            # near-tetrahedral: 100-125 degrees to every bonded partner
+            if not all(-0.57 < np.dot(d, (x - center_p) / np.linalg.norm(x - center_p)) < -0.17 for x in bonded):
+                continue
+            p = center_p + blen * d
+            m = grid.mindist(p, skip=skip)
+            if m > bestd:
+                best, bestd = p, m
+        if best is None:
+            u = sum((center_p - x) / np.linalg.norm(center_p - x) for x in bonded)
+            best = center_p + blen * u / np.linalg.norm(u)
+        return best
+
+    for k, t in enumerate(types):
+        ht, hb = H_OF[t]
+        nb = [pos[k - 1]] if k > 0 else []
+        if k + 1 < len(types):
+            nb.append(pos[k + 1])
+        skip = set(gidx[max(0, k - 1):k + 2])
+        for h in range(hcount[k]):
+            if t == "NT" and len(nb) == 2:
+                # planar amine hydrogen on the bisector
+                u = (pos[k] - nb[0]) / np.linalg.norm(pos[k] - nb[0]) + (pos[k] - nb[1]) / np.linalg.norm(pos[k] - nb[1])
+                p = pos[k] + hb * u / np.linalg.norm(u)
+            else:
+                p = best_h(pos[k], nb, skip, hb)
+            idx = len(spec.atoms)
+            spec.atoms.append(("POLY", str(k // 4 + 1), "SYN", "H%d_%d" % (k, h), ht, float(next(qi)), POLY_TYPES[ht]))
+            spec._pos.append(p)
+            skip.add(len(grid.pts))
+            grid.add(p)
+            hatoms[k].append(idx)
+            nb.append(p)
+    # bonds
+    for k in range(len(types)):
+        if k > 0:
+            spec.bonds.append((heavy_idx[k - 1], heavy_idx[k]))
+        for h in hatoms[k]:
+            spec.bonds.append((heavy_idx[k], h))
+    # substituents: CH2-CH2-OH (8 atoms) per block, all bonded to site_heavy
+    subs = []
+    if site_at is not None:
+        k = site_heavy
+        anchor = pos[k]
+        nbp = [pos[k - 1], pos[k + 1]] + [np.asarray(spec._pos[h]) for h in hatoms[k]]
+        u = sum((anchor - x) / np.linalg.norm(anchor - x) for x in nbp)
+        u /= np.linalg.norm(u)
+        near = set(gidx[max(0, k - 1):k + 2])
+        placed_all = []
+        for s in range(nsubs):
+            atoms_s = []
+            qs = rng.normal(0.0, 0.35, size=sub_size)
+            qs -= qs.mean()
+            c1 = anchor + 1.53 * u
+            # C2 and O on the largest-clearance torsions (alternative
+            # substituents of one site overlap each other by design)
+            best, bestd = None, -1
+            for _ in range(24):
+                phi = rng.uniform(-math.pi, math.pi)
+                c2 = _place(c1, anchor, pos[k - 1], 1.53, math.radians(111.0), phi)
+                phi2 = rng.uniform(-math.pi, math.pi)
+                o = _place(c2, c1, anchor, 1.43, math.radians(111.0), phi2)
+                bonded = {0: [anchor, c2], 1: [c1, o], 2: [c2]}
+                hs = []
+                for (hk, nh) in [(0, 2), (1, 2), (2, 1)]:
+                    hb = H_OF[("CT", "CT", "OP")[hk]][1]
+                    for j in range(nh):
+                        p = best_h((c1, c2, o)[hk], bonded[hk], near, hb, tries=120)
+                        bonded[hk].append(p)
+                        hs.append((hk, p))
+                m = min(grid.mindist(x, skip=near) for x in [c2, o] + [p for _, p in hs])
+                if m > bestd:
+                    best, bestd = (c2, o, hs), m
+            c2, o, hs = best
+            hv = [("C1", "CT", c1), ("C2", "CT", c2), ("O", "OP", o)]
+            hidx = []
+            for nm, t, p in hv:
+                idx = len(spec.atoms)
+                spec.atoms.append(("LIG", str(s + 1), "SUB%d" % (s + 1), nm, t, 0.0, POLY_TYPES[t]))
+                spec._pos.append(p)
+                hidx.append(idx)
+                atoms_s.append(idx)
+            for j, (hk, p) in enumerate(hs):
+                ht = H_OF[hv[hk][1]][0]
+                idx = len(spec.atoms)
+                spec.atoms.append(("LIG", str(s + 1), "SUB%d" % (s + 1), "H%d%d" % (hk, j), ht, 0.0, POLY_TYPES[ht]))
+                spec._pos.append(p)
+                atoms_s.append(idx)
+                spec.bonds.append((hidx[hk], idx))
+            spec.bonds += [(heavy_idx[k], hidx[0]), (hidx[0], hidx[1]), (hidx[1], hidx[2])]
+            for a, qv in zip(atoms_s, qs):
+                rec = list(spec.atoms[a])
+                rec[5] = float(qv)
+                spec.atoms[a] = tuple(rec)
+            subs.append(atoms_s)
+            placed_all += atoms_s
+        for a in placed_all:
+            grid.add(spec._pos[a])
+    return heavy_idx, subs
+
+
+def _angles_dihes(spec, block_of):
+    """all angles and proper dihedrals of the bond graph, skipping terms that
+    mix two blocks of one site; impropers on NT with three partners"""
+    n = spec.n
+    nbr = [[] for _ in range(n)]
+    for a, b in spec.bonds:
+        nbr[a].append(b)
+        nbr[b].append(a)
+    have = set(tuple(sorted((a[0], a[2]))) + (a[1],) for a in spec.angles)
+
+    def mixed(idx):
+        bl = {block_of[i] for i in idx if block_of[i] != 0}
+        return len(bl) > 1
+
+    def water(i):
+        return spec.atoms[i][4] in ("OT", "HT")
+
+    for j in range(n):
+        if water(j):
+            continue
+        for x in range(len(nbr[j])):
+            for y in range(x + 1, len(nbr[j])):
+                i, k = nbr[j][x], nbr[j][y]
+                key = tuple(sorted((i, k))) + (j,)
+                if key in have or mixed((i, j, k)):
+                    continue
+                ti, tk = spec.atoms[i][4], spec.atoms[k][4]
+                if ti.startswith("H") and tk.startswith("H") and spec.atoms[j][4] == "OP":
+                    continue
+                spec.angles.append((i, j, k))
+                have.add(key)
+    for a, b in spec.bonds:
+        if water(a) or spec.atoms[a][4].startswith("H") or spec.atoms[b][4].startswith("H"):
+            continue
+        for i in nbr[a]:
+            if i == b:
+                continue
+            for l in nbr[b]:
+                if l == a or l == i:
+                    continue
+                if mixed((i, a, b, l)):
+                    continue
+                spec.dihes.append((i, a, b, l))
+    for j in range(n):
+        if spec.atoms[j][4] == "NT" and len(nbr[j]) == 3:
+            hs = [x for x in nbr[j] if spec.atoms[x][4] == "HN"]
+            cs = [x for x in nbr[j] if spec.atoms[x][4] != "HN"]
+            if len(hs) == 1 and len(cs) == 2:
+                spec.imprs.append((j, cs[0], cs[1], hs[0]))
+
+
+def _fill_water(spec, rng, nwat, grid, clearance=2.5, lattice=None):
+    L = np.array(spec.box)
+    n1 = lattice or int(math.ceil((nwat * 1.32) ** (1 / 3)))
+    sp = L / n1
+    cand = []
+    for i in range(n1):
+        for j in range(n1):
+            for k in range(n1):
+                o = (np.array([i, j, k]) + 0.5) * sp + rng.uniform(-0.1, 0.1, 3)
+                if grid.mindist(o) > clearance:
+                    cand.append(o)
+    if len(cand) < nwat:
+        raise RuntimeError("only %d water sites for %d waters" % (len(cand), nwat))
+    pick = rng.choice(len(cand), size=nwat, replace=False)
+    pick.sort()
+    resid = 1
+    for pi in pick:
+        o = cand[pi]
+        best, bestd = None, -1
+        for _ in range(12):
+            R = _rot(rng)
+            f0, f1, f2 = _water_frame()
+            h1, h2 = o + R @ f1, o + R @ f2
+            dmin = min(grid.mindist(h1), grid.mindist(h2))
+            if dmin > bestd:
+                best, bestd = R, dmin
+            if dmin > 1.9:
+                break
+        _add_water(spec, o, best, resid)
+        f0, f1, f2 = _water_frame()
+        for f in (f0, f1, f2):
+            grid.add(o + best @ f)
+        resid += 1
+
+
+def _msld_site(spec, subs, fnex, theta_mass=12.0):
+    nb = 1 + len(subs)
+    spec.nblocks = nb
+    blk = np.zeros(spec.n, dtype=np.int32)
+    for s, atoms in enumerate(subs):
+        blk[atoms] = s + 1
+    spec.atom_block = blk
+    spec.block_site = [0] + [1] * len(subs)
+    spec.theta0 = [0.0] * nb
+    spec.theta_mass = [0.0] + [theta_mass] * len(subs)
+    spec.fixed_bias = [0.0] * nb
+    spec.fnex = fnex
+
+
+def config2(seed=0xB1ADE002, n_water=7023, n_poly=2489, box=62.23):
+    """23,558-atom MSLD box: TIP3P waters + synthetic polymer with one site
+    of 3 substituents x 8 atoms (the polymer count includes them)."""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name="config2-msld-23558", box=(box,) * 3)
+    spec._pos = []
+    _water_params(spec)
+    grid = _Grid(spec.box, 3.0)
+    radius = 26.0 * (n_poly / 2489.0) ** (1 / 3)
+    _, subs = _polymer(spec, rng, n_poly, np.array(spec.box) / 2, radius, grid, site_at=0.5)
+    npoly = spec.n
+    assert npoly == n_poly, (npoly, n_poly)
+    block_of = np.zeros(spec.n, dtype=np.int32)
+    for s, atoms in enumerate(subs):
+        block_of[atoms] = s + 1
+    _angles_dihes(spec, block_of)
+    _fill_water(spec, rng, n_water, grid)
+    spec.pos = np.array(spec._pos)
+    _msld_site(spec, subs, fnex=5.0)
+    spec.rcut, spec.rswitch = 10.0, 9.0
+    spec.beta = 1.0 / 3.2
+    spec.grid, spec.order = (64, 64, 64), 6
+    spec.temperature = 298.0
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
+
+
+def msld_small(seed=0xB1ADE0A0, n_water=150, n_poly=60, box=20.0, grid=20):
+    """A small MSLD system (water + short polymer + one 3-block site) whose
+    oracle evaluation takes well under a second."""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name="msld-small", box=(box,) * 3)
+    spec._pos = []
+    _water_params(spec)
+    g = _Grid(spec.box, 3.0)
+    _, subs = _polymer(spec, rng, n_poly, np.array(spec.box) / 2, 6.0, g, site_at=0.5)
+    block_of = np.zeros(spec.n, dtype=np.int32)
+    for s, atoms in enumerate(subs):
+        block_of[atoms] = s + 1
+    _angles_dihes(spec, block_of)
+    _fill_water(spec, rng, n_water, g, lattice=7)
+    spec.pos = np.array(spec._pos)
+    _msld_site(spec, subs, fnex=5.0)
+    spec.rcut, spec.rswitch = 9.0, 8.0
+    spec.beta = 0.34
+    spec.grid, spec.order = (grid, grid, grid), 6
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec

@@ -1,0 +1,247 @@
+"""Python mirror of the reference-compatible blade_* API.
+
+`BladeSystem` wraps one `System*` of libblade_mi355x.so.  Its methods carry
+the reference's names without the `blade_` prefix (init_parameters,
+add_parameter_bonds, add_atom, add_msld_flags, add_run_flags, run_energy,
+...), with the same argument meaning, 1-based indices and AKMA units, so a
+test reads like the reference's CHARMM-side calls.  `build(spec)` drives
+those calls from a synthetic `Spec`.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check, ptr
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+class BladeSystem:
+    def __init__(self, gpus=(0,)):
+        self.L = _ffi.lib()
+        arr = (C.c_int * len(gpus))(*gpus)
+        self.h = self.L.blade_init_system(len(gpus), arr)
+        self.n = 0
+        self.nblocks = 1
+
+    def close(self):
+        if self.h:
+            self.L.blade_dest_system(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference-compatible calls (include/blade.h) -----------------------
+    def init_parameters(self):
+        self.L.blade_init_parameters(self.h)
+
+    def add_parameter_atoms(self, name, mass):
+        self.L.blade_add_parameter_atoms(self.h, _b(name), mass)
+
+    def add_parameter_bonds(self, t1, t2, kb, b0):
+        self.L.blade_add_parameter_bonds(self.h, _b(t1), _b(t2), kb, b0)
+
+    def add_parameter_angles(self, t1, t2, t3, k, t0, kub, s0):
+        self.L.blade_add_parameter_angles(self.h, _b(t1), _b(t2), _b(t3), k, t0, kub, s0)
+
+    def add_parameter_dihes(self, t1, t2, t3, t4, k, n, d0):
+        self.L.blade_add_parameter_dihes(self.h, _b(t1), _b(t2), _b(t3), _b(t4), k, n, d0)
+
+    def add_parameter_imprs(self, t1, t2, t3, t4, k, n, d0):
+        self.L.blade_add_parameter_imprs(self.h, _b(t1), _b(t2), _b(t3), _b(t4), k, n, d0)
+
+    def add_parameter_nbonds(self, t, eps, sig, eps14, sig14, e14fac, combine):
+        self.L.blade_add_parameter_nbonds(self.h, _b(t), eps, sig, eps14, sig14, e14fac, combine)
+
+    def init_structure(self):
+        self.L.blade_init_structure(self.h)
+
+    def add_atom(self, idx, seg, resid, resname, name, typ, q, m):
+        self.L.blade_add_atom(self.h, idx, _b(seg), _b(resid), _b(resname), _b(name), _b(typ), q, m)
+        self.n = max(self.n, idx)
+
+    def add_bond(self, i, j):
+        self.L.blade_add_bond(self.h, i, j)
+
+    def add_angle(self, i, j, k):
+        self.L.blade_add_angle(self.h, i, j, k)
+
+    def add_dihe(self, i, j, k, l):
+        self.L.blade_add_dihe(self.h, i, j, k, l)
+
+    def add_impr(self, i, j, k, l):
+        self.L.blade_add_impr(self.h, i, j, k, l)
+
+    def add_shake(self, on):
+        self.L.blade_add_shake(self.h, int(on))
+
+    def init_coordinates(self, n):
+        self.L.blade_init_coordinates(self.h, n)
+
+    def add_coordinates_position(self, i, x, y, z):
+        self.L.blade_add_coordinates_position(self.h, i, x, y, z)
+
+    def add_coordinates_velocity(self, i, x, y, z):
+        self.L.blade_add_coordinates_velocity(self.h, i, x, y, z)
+
+    def add_coordinates_box(self, name, a, b, c, al=90.0, be=90.0, ga=90.0):
+        self.L.blade_add_coordinates_box(self.h, name, a, b, c, al, be, ga)
+
+    def init_msld(self, nblocks):
+        self.L.blade_init_msld(self.h, nblocks)
+        self.nblocks = nblocks
+
+    def add_msld_atomassignment(self, atom, block):
+        self.L.blade_add_msld_atomassignment(self.h, atom, block)
+
+    def add_msld_initialconditions(self, block, site, theta0, thetav, mass, bias, charge):
+        self.L.blade_add_msld_initialconditions(self.h, block, site, theta0, thetav, mass, bias, charge)
+
+    def add_msld_flags(self, gamma, fnex, soft, soft14, ewaldtype, krest, sbr, sbe, snbe, fix):
+        self.L.blade_add_msld_flags(self.h, gamma, fnex, soft, soft14, ewaldtype, krest, sbr, sbe, snbe, fix)
+
+    def add_msld_bias(self, i, j, typ, l0, k, n):
+        self.L.blade_add_msld_bias(self.h, i, j, typ, l0, k, n)
+
+    def init_run(self):
+        self.L.blade_init_run(self.h)
+
+    def add_run_flags(self, gamma, beta, rcut, rswitch, vdw, elec, gridspace, gx, gy, gz, order, shaketol):
+        self.L.blade_add_run_flags(self.h, gamma, beta, rcut, rswitch, vdw, elec, gridspace, gx, gy, gz,
+                                   order, shaketol)
+
+    def add_run_dynopts(self, step, step0, nsteps, dt, T, freqnpt=0, volfluct=0.0, pressure=0.0):
+        self.L.blade_add_run_dynopts(self.h, step, step0, nsteps, dt, T, freqnpt, volfluct, pressure)
+
+    def set_seed(self, seed):
+        self.L.blade_set_seed(self.h, seed)
+
+    def run_energy(self):
+        self.L.blade_run_energy(self.h)
+        err = self.L.blade_last_error().decode()
+        if err:
+            raise RuntimeError(err)
+
+    # -- extensions -----------------------------------------------------------
+    def prepare(self):
+        check(self.L.blade_prepare(self.h), "blade_prepare")
+
+    def run_dynamics(self, nsteps, freq_nrg=0, use_graph=True):
+        check(self.L.blade_run_dynamics(self.h, nsteps, freq_nrg, int(use_graph)), "blade_run_dynamics")
+
+    def topology(self):
+        p = self.L.blade_get_topology(self.h)
+        if not p:
+            self.prepare()
+            p = self.L.blade_get_topology(self.h)
+        return p
+
+    def runparams(self):
+        p = self.L.blade_get_runparams(self.h)
+        if not p:
+            self.prepare()
+            p = self.L.blade_get_runparams(self.h)
+        return p
+
+    def device(self):
+        d = self.L.blade_get_device_system(self.h)
+        if not d:
+            raise RuntimeError("device system not initialised")
+        return d
+
+    def energy(self):
+        e = np.zeros(_ffi.NTERMS)
+        check(self.L.blade_get_energy(self.h, ptr(e)), "get_energy")
+        return e
+
+    def forces(self):
+        f = np.zeros(3 * self.n)
+        check(self.L.blade_get_forces(self.h, ptr(f)), "get_forces")
+        return f
+
+    def positions(self):
+        x = np.zeros(3 * self.n)
+        check(self.L.blade_get_positions(self.h, ptr(x)), "get_positions")
+        return x
+
+    def set_positions(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(self.L.blade_set_positions(self.h, ptr(x)), "set_positions")
+
+    def lambdas(self):
+        v = np.zeros(max(1, self.nblocks))
+        check(self.L.blade_get_lambda(self.h, ptr(v)), "get_lambda")
+        return v
+
+    def dudl(self):
+        v = np.zeros(max(1, self.nblocks))
+        check(self.L.blade_get_dudl(self.h, ptr(v)), "get_dudl")
+        return v
+
+
+def soa(pos):
+    """(N,3) -> SoA flat [x..., y..., z...]"""
+    return np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T).ravel()
+
+
+def build(spec, gpus=(0,)):
+    """Create a BladeSystem from a synthetic Spec through the blade_* API."""
+    s = BladeSystem(gpus)
+    s.init_parameters()
+    for t, m in spec.types.items():
+        s.add_parameter_atoms(t, m)
+    for (a, b), (kb, b0) in spec.bond_par.items():
+        s.add_parameter_bonds(a, b, kb, b0)
+    for (a, b, c), (k, t0, kub, s0) in spec.angle_par.items():
+        s.add_parameter_angles(a, b, c, k, t0, kub, s0)
+    for key, terms in spec.dihe_par.items():
+        for (k, n, d0) in terms:
+            s.add_parameter_dihes(*key, k, n, d0)
+    for key, (k, n, d0) in spec.impr_par.items():
+        s.add_parameter_imprs(*key, k, n, d0)
+    for t, (eps, sig, eps14, sig14, e14, comb) in spec.nb_par.items():
+        s.add_parameter_nbonds(t, eps, sig, eps14, sig14, e14, comb)
+    s.init_structure()
+    for i, (seg, resid, resn, nm, t, q, m) in enumerate(spec.atoms):
+        s.add_atom(i + 1, seg, resid, resn, nm, t, q, m)
+    for a, b in spec.bonds:
+        s.add_bond(a + 1, b + 1)
+    for a, b, c in spec.angles:
+        s.add_angle(a + 1, b + 1, c + 1)
+    for a, b, c, d in spec.dihes:
+        s.add_dihe(a + 1, b + 1, c + 1, d + 1)
+    for a, b, c, d in spec.imprs:
+        s.add_impr(a + 1, b + 1, c + 1, d + 1)
+    s.add_shake(spec.shake)
+    s.init_coordinates(spec.n)
+    for i in range(spec.n):
+        s.add_coordinates_position(i + 1, *spec.pos[i])
+        if spec.vel is not None:
+            s.add_coordinates_velocity(i + 1, *spec.vel[i])
+    s.add_coordinates_box(0, *spec.box)
+    s.init_msld(spec.nblocks)
+    if spec.nblocks > 1:
+        for i in np.nonzero(spec.atom_block)[0]:
+            s.add_msld_atomassignment(int(i) + 1, int(spec.atom_block[i]) + 1)
+        for b in range(spec.nblocks):
+            s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], 0.0,
+                                         spec.theta_mass[b], spec.fixed_bias[b], 0.0)
+        s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2, 59.2, 1.0, 2.0, 1.0, 0)
+        for (i, j, t, l0, k, n) in spec.var_bias:
+            s.add_msld_bias(i + 1, j + 1, t, l0, k, n)
+    s.init_run()
+    gx, gy, gz = spec.grid
+    s.add_run_flags(spec.gamma, spec.beta, spec.rcut, spec.rswitch, 0, 1,
+                    spec.grid_space if gx <= 0 else -1.0, gx, gy, gz, spec.order, spec.shake_tol)
+    s.add_run_dynopts(0, 0, spec.nsteps, spec.dt, spec.temperature)
+    s.set_seed(spec.seed)
+    s.n = spec.n
+    s.nblocks = spec.nblocks
+    return s

@@ -1,0 +1,140 @@
+/*
+ * blade.h - the reference-compatible BLaDE library API, as CHARMM binds it.
+ *
+ * Every function keeps the reference's name, argument list, argument
+ * meaning (AKMA units, 1-based atom/block/site indices) and void return.
+ * Where the reference calls fatal() and exits, this build records the
+ * message (blade_last_error), reports it through blade_log, and makes later
+ * calls on that System no-ops.
+ *
+ * Reference declarations replaced:
+ *   system/system.h:86-91        blade_init_system .. blade_set_verbose
+ *   system/parameters.h:202-230  blade_init_parameters .. blade_add_parameter_nbfixs
+ *   system/structure.h:111-133   blade_init_structure .. blade_add_shake (+ restraint adders)
+ *   system/coordinates.h:57-63   blade_init_coordinates .. blade_add_coordinates_box
+ *   msld/msld.h:147-165          blade_init_msld .. blade_set_msld_block_fixed
+ *   run/run.h:146-158            blade_init_run .. blade_run_energy
+ *   io/io.h:69-83                blade_interpretter (subset) and blade_log
+ *
+ * Extensions (no reference counterpart) are grouped at the end: they give a
+ * C caller the dynamics loop and host readback that the reference reaches
+ * through its script interpreter and CHARMM-side copies.
+ */
+#ifndef BLADE_H
+#define BLADE_H
+
+#include "blade_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct System System;
+
+/* system/system.h:86-91 */
+System *blade_init_system(int ngpus, int *gpus);
+void blade_dest_system(System *system);
+void blade_set_device(System *system);
+void blade_set_verbose(System *system, int v);
+
+/* system/parameters.h:202-230 */
+void blade_init_parameters(System *system);
+void blade_dest_parameters(System *system);
+void blade_add_parameter_atoms(System *system, const char *name, double mass);
+void blade_add_parameter_bonds(System *system, const char *t1, const char *t2, double kb, double b0);
+void blade_add_parameter_angles(System *system, const char *t1, const char *t2, const char *t3,
+                                double kangle, double angle0, double kureyb, double ureyb0);
+void blade_add_parameter_dihes(System *system, const char *t1, const char *t2, const char *t3,
+                               const char *t4, double kdih, int ndih, double dih0);
+void blade_add_parameter_imprs(System *system, const char *t1, const char *t2, const char *t3,
+                               const char *t4, double kimp, int nimp, double imp0);
+void blade_add_parameter_cmaps(System *system, const char *t1, const char *t2, const char *t3,
+                               const char *t4, const char *t5, const char *t6, const char *t7,
+                               const char *t8, int ngrid);
+void blade_add_parameter_cmaps_fill(System *system, const char *t1, const char *t2, const char *t3,
+                                    const char *t4, const char *t5, const char *t6, const char *t7,
+                                    const char *t8, int i, int j, double kcmapij);
+void blade_add_parameter_nbonds(System *system, const char *t1, double eps, double sig,
+                                double eps14, double sig14, double e14fac, int combine);
+void blade_add_parameter_nbfixs(System *system, const char *t1, const char *t2, double eps,
+                                double sig, double eps14, double sig14);
+
+/* system/structure.h:111-133 */
+void blade_init_structure(System *system);
+void blade_dest_structure(System *system);
+void blade_add_atom(System *system, int atomIdx, const char *segName, const char *resIdx,
+                    const char *resName, const char *atomName, const char *atomTypeName,
+                    double charge, double mass);
+void blade_add_bond(System *system, int i, int j);
+void blade_add_angle(System *system, int i, int j, int k);
+void blade_add_dihe(System *system, int i, int j, int k, int l);
+void blade_add_impr(System *system, int i, int j, int k, int l);
+void blade_add_cmap(System *system, int i1, int j1, int k1, int l1, int i2, int j2, int k2, int l2);
+void blade_add_shake(System *system, int shakeHbond);
+
+/* system/coordinates.h:57-63 */
+void blade_init_coordinates(System *system, int n);
+void blade_dest_coordinates(System *system);
+void blade_add_coordinates_position(System *system, int i, double x, double y, double z);
+void blade_add_coordinates_velocity(System *system, int i, double vx, double vy, double vz);
+void blade_add_coordinates_box(System *system, int name, double a, double b, double c,
+                               double alpha, double beta, double gamma);
+
+/* msld/msld.h:147-165 */
+void blade_init_msld(System *system, int nblocks);
+void blade_dest_msld(System *system);
+void blade_add_msld_atomassignment(System *system, int atomIdx, int blockIdx);
+void blade_add_msld_initialconditions(System *system, int blockIdx, int siteIdx, double theta0,
+                                      double thetaVelocity, double thetaMass, double fixBias,
+                                      double blockCharge);
+void blade_add_msld_termscaling(System *system, int scaleBond, int scaleUrey, int scaleAngle,
+                                int scaleDihe, int scaleImpr, int scaleCmap);
+void blade_add_msld_flags(System *system, double gamma, double fnex, int useSoftCore,
+                          int useSoftCore14, int msldEwaldType, double kRestraint,
+                          double softBondRadius, double softBondExponent,
+                          double softNotBondExponent, int fix);
+void blade_add_msld_bias(System *system, int i, int j, int type, double l0, double k, int n);
+void blade_add_msld_block_friction(System *system, int blockIdx, double friction);
+void blade_set_msld_block_fixed(System *system, int blockIdx, int fixed);
+
+/* run/run.h:146-158 */
+void blade_init_run(System *system);
+void blade_dest_run(System *system);
+void blade_add_run_flags(System *system, double gamma, double betaEwald, double rCut,
+                         double rSwitch, int vdWfSwitch, int elecPME, double gridSpace,
+                         int gridx, int gridy, int gridz, int orderEwald, double shakeTolerance);
+void blade_add_run_dynopts(System *system, int step, int step0, int nsteps, double dt, double T,
+                           int freqNPT, double volumeFluctuation, double pressure);
+void blade_run_energy(System *system);
+
+/* io/io.h:82 - user-replaceable logger (weak default prints to stderr) */
+void blade_log(const char *message);
+
+/* ---- extensions ----------------------------------------------------- */
+/* last error message of this thread ("" when none) */
+const char *blade_last_error(void);
+/* Run::dynamics: nsteps steps from the current step (run/run.cu:545-570);
+ * energies on steps with step % freqNRG == 0; use_graph replays hipGraphs. */
+int blade_run_dynamics(System *system, int nsteps, int freqNRG, int use_graph);
+/* host readback after blade_run_energy / blade_run_dynamics (SoA [3N]) */
+int blade_get_energy(System *system, double *energy /*[BM_NTERMS]*/);
+int blade_get_forces(System *system, double *force);
+int blade_get_positions(System *system, double *pos);
+int blade_set_positions(System *system, const double *pos);
+int blade_get_lambda(System *system, double *lambda);
+int blade_get_dudl(System *system, double *dudl);
+/* derived term tables and run parameters (host memory owned by System) */
+const bm_topology *blade_get_topology(System *system);
+const bm_runparams *blade_get_runparams(System *system);
+/* the device system behind this System (NULL before initialisation) */
+bm_system *blade_get_device_system(System *system);
+/* derive terms without touching a GPU (returns 0 on success) */
+int blade_prepare(System *system);
+/* Philox seed of the Langevin noise */
+void blade_set_seed(System *system, unsigned long long seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,255 @@
+/*
+ * blade_mi355x.h - kernel-layer C ABI of the MI355X implementation of
+ * BLaDE's per-step force and integration path.
+ *
+ * Two layers are exported from libblade_mi355x.so:
+ *
+ *   1. The reference-compatible `blade_*` library API (include/blade.h),
+ *      the one CHARMM binds (reference: system/system.h:86-91,
+ *      run/run.h:146-158, msld/msld.h:147-165, system/structure.h:111-133,
+ *      system/coordinates.h:57-63, system/parameters.h:202-230).
+ *
+ *   2. This header: the per-term kernel layer that the reference reaches
+ *      through `getforce_X(System*,bool)` and friends.  Each entry point
+ *      takes an opaque system handle (device buffers owned by the
+ *      library) and a flag word.  No torch types, no C++ types.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative BM_E* code on failure; the library
+ *     never calls exit();
+ *   - forces are dU/dx (reference sign convention, update/update.cu:52),
+ *     accumulated as signed 64-bit fixed point with BM_FIXED_SCALE counts
+ *     per kcal/mol/A, so sums are independent of accumulation order;
+ *   - energies and dU/dlambda are 64-bit fixed point with the same scale;
+ *   - positions, velocities and theta are FP64, stored structure-of-arrays:
+ *     x[0..N) y[N..2N) z[2N..3N) theta[3N..3N+nL);
+ *   - units are AKMA (kcal/mol, A, e, amu; time in AKMA units, one
+ *     picosecond = 20.454828284385908).
+ */
+#ifndef BLADE_MI355X_H
+#define BLADE_MI355X_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM_LAYOUT_VERSION 1
+
+/* error codes */
+#define BM_OK 0
+#define BM_EINVAL (-1)     /* invalid argument or inconsistent sizes          */
+#define BM_EHIP (-2)       /* a HIP runtime call failed                        */
+#define BM_ENAN (-3)       /* NaN/Inf detected in the integrator               */
+#define BM_EOVERFLOW (-4)  /* a fixed-capacity buffer would overflow           */
+#define BM_ENOTSUP (-5)    /* feature outside the supported set                */
+#define BM_ENOGPU (-6)     /* no HIP device available                          */
+
+/* 64-bit fixed point: 2^32 counts per unit */
+#define BM_FIXED_SCALE 4294967296.0
+
+/* energy terms, same order as the reference EETerm (system/potential.h:20-44) */
+enum bm_eterm {
+  BM_EBOND = 0, BM_EANGLE, BM_EUREY, BM_EDIHE, BM_EIMPR, BM_ECMAP, BM_ENB14,
+  BM_ENBDIRECT, BM_ENBRECIP, BM_ENBRECIPSELF, BM_ENBRECIPEXCL, BM_ELAMBDA,
+  BM_ETHETA, BM_ECATS, BM_ENOE, BM_EHARMONIC, BM_EMMFP, BM_ERESD, BM_EMLP,
+  BM_EBIAS, BM_EPOTENTIAL, BM_EKINETIC, BM_ETOTAL, BM_NTERMS
+};
+
+/* nonbonded methods (run/run.h EVdw / EElec) */
+enum bm_vdw { BM_VFSWITCH = 0, BM_VSWITCH = 1, BM_VSHIFT = 2 };
+enum bm_elec { BM_FSWITCH = 0, BM_PME = 1, BM_FSHIFT = 2 };
+
+/* flags for force entry points */
+#define BM_CALC_ENERGY 1u   /* accumulate energies                          */
+#define BM_CALC_ALCH 2u     /* system carries lambda sites                  */
+#define BM_SOFTCORE 4u      /* soft-core nonbonded (msld.h useSoftCore)     */
+#define BM_SOFTCORE14 8u    /* soft-core 1-4 pairs (msld.h useSoftCore14)   */
+
+/*
+ * Derived topology: the term tables the reference's Potential::initialize
+ * builds (system/potential.cxx:525-1837), as host arrays.  siteBlock codes
+ * are (site<<16)|block exactly as merge_site_block (msld/msld.cu:470-476).
+ */
+typedef struct bm_topology {
+  int32_t n_atoms;
+  const double *charge;      /* [N]   e                                  */
+  const double *mass;        /* [N]   amu                                */
+  const int32_t *type_idx;   /* [N]   vdW type                           */
+  const int32_t *site_block; /* [N]   siteBlock code of the atom         */
+  int32_t n_types;
+  const double *vdw_c12;     /* [T*T] eps*Rmin^12                        */
+  const double *vdw_c6;      /* [T*T] 2*eps*Rmin^6                       */
+
+  int32_t n_bonds, n_bonds12;      /* 1-2 bonds first, then Urey-Bradley */
+  const int32_t *bond_idx;         /* [2n] */
+  const int32_t *bond_sb;          /* [2n] siteBlock codes */
+  const double *bond_kb, *bond_b0; /* E = kb/2 (r-b0)^2 */
+
+  int32_t n_angles;
+  const int32_t *angle_idx;        /* [3n] */
+  const int32_t *angle_sb;         /* [2n] */
+  const double *angle_k, *angle_t0;
+
+  int32_t n_dihes;
+  const int32_t *dihe_idx;         /* [4n] */
+  const int32_t *dihe_sb;          /* [2n] */
+  const double *dihe_k, *dihe_d0;  /* E = k (1+cos(n phi - d0)) */
+  const int32_t *dihe_n;
+
+  int32_t n_imprs;
+  const int32_t *impr_idx;         /* [4n] */
+  const int32_t *impr_sb;          /* [2n] */
+  const double *impr_k, *impr_d0;  /* n>0: periodic; n==0: k/2 dphi^2 */
+  const int32_t *impr_n;
+
+  int32_t n_nb14;
+  const int32_t *nb14_idx;         /* [2n] */
+  const int32_t *nb14_sb;          /* [2n] */
+  const double *nb14_qq, *nb14_c12, *nb14_c6, *nb14_e14fac;
+
+  int32_t n_nbex;                  /* PME exclusion corrections */
+  const int32_t *nbex_idx;         /* [2n] */
+  const int32_t *nbex_sb;          /* [2n] */
+  const double *nbex_qq;
+
+  int32_t n_excl;                  /* unordered excluded pairs, i<j */
+  const int32_t *excl_idx;         /* [2n] */
+
+  int32_t n_tri, n_br1, n_br2, n_br3;  /* holonomic constraint groups */
+  const int32_t *tri_idx;  const double *tri_b0;  /* [3n],[3n] b0: 01,02,12 */
+  const int32_t *br1_idx;  const double *br1_b0;  /* [2n],[n]  */
+  const int32_t *br2_idx;  const double *br2_b0;  /* [3n],[2n] */
+  const int32_t *br3_idx;  const double *br3_b0;  /* [4n],[3n] */
+
+  /* MSLD (block 0 is the environment, site 0) */
+  int32_t n_blocks, n_sites;
+  const int32_t *block_site;       /* [nL] site of each block            */
+  const int32_t *site_bound;       /* [nSites+1] first block of each site */
+  const double *theta_mass;        /* [nL] amu A^2 (0 => not integrated) */
+  const int32_t *block_fixed;      /* [nL] 1 => lambda fixed at theta    */
+  const double *fixed_bias;        /* [nL] lambdaBias                    */
+  const double *theta_friction;    /* [nL] per-block gamma (AKMA); NULL => msld gamma */
+  int32_t n_varbias;
+  const int32_t *vb_i, *vb_j, *vb_type, *vb_n;   /* [n] (msld.h:13-19) */
+  const double *vb_l0, *vb_k;
+} bm_topology;
+
+/* run parameters (run/run.cu:27-110 defaults) */
+typedef struct bm_runparams {
+  double rcut, rswitch;            /* A */
+  double beta_ewald;               /* 1/A */
+  int32_t vdw_method, elec_method; /* bm_vdw, bm_elec */
+  int32_t grid[3];                 /* PME grid */
+  int32_t order;                   /* B-spline order (4, 6, 8) */
+  double dt;                       /* AKMA time */
+  double temperature;              /* K */
+  double gamma;                    /* atom friction, 1/AKMA time */
+  double msld_gamma;               /* theta friction, 1/AKMA time */
+  double shake_tol;
+  double fnex;
+  int32_t soft_core, soft_core14;
+  double soft_bond_exp;            /* softNotBondExponent, unused: no soft terms */
+  double box[3];                   /* orthorhombic box edges, A */
+  int32_t freq_domdec;             /* list rebuild cadence (steps) */
+  double cull_pad;                 /* list padding, A */
+  uint64_t seed;                   /* Philox key */
+} bm_runparams;
+
+/* ---------------------------------------------------------------------
+ * Context: one per device and host thread (bm_ctx owns a System image).
+ * ------------------------------------------------------------------- */
+typedef struct bm_system bm_system;
+
+/* Create a system on `device` from a derived topology, run parameters and
+ * initial FP64 positions/velocities (SoA, [3N]) and theta state ([nL]).
+ * Replaces the allocation half of Potential::initialize
+ * (system/potential.cxx:525-1837), State::State (system/state.cxx:20-153)
+ * and Domdec::initialize (domdec/domdec.cu:83-199). */
+int bm_system_create(int device, const bm_topology *top, const bm_runparams *rp,
+                     const double *pos, const double *vel,
+                     const double *theta, const double *theta_vel,
+                     bm_system **out);
+int bm_system_destroy(bm_system *sys);
+
+/* Domdec::update_domdec(System*,bool) (domdec/domdec.cu:213-222):
+ * rebuild = 1 wraps positions, sorts atoms into blocks, builds the tile
+ * list and exclusion masks; rebuild = 0 only repacks block positions. */
+int bm_domdec_update(bm_system *sys, int rebuild);
+
+/* Potential::calc_force(long step,System*) (system/potential.cxx:1851-1941):
+ * all force terms at the current positions; flags = BM_CALC_ENERGY. */
+int bm_calc_force(bm_system *sys, long step, unsigned flags);
+
+/* Per-term entry points (nbdirect/nbdirect.h:7-8, nbrecip/nbrecip.h:7-8,
+ * bonded/bonded.h:7-11, bonded/pair.h:7-8): each adds its term into the
+ * fixed-point accumulators; the caller zeroes them (bm_reset_force). */
+int bm_reset_force(bm_system *sys, unsigned flags);
+int bm_getforce_nbdirect(bm_system *sys, unsigned flags);
+int bm_getforce_ewaldself(bm_system *sys, unsigned flags);
+int bm_getforce_ewald(bm_system *sys, unsigned flags);
+int bm_getforce_bonded(bm_system *sys, unsigned flags);   /* bond,urey,angle,dihe,impr */
+int bm_getforce_nb14(bm_system *sys, unsigned flags);
+int bm_getforce_nbex(bm_system *sys, unsigned flags);
+int bm_getforce_msldbias(bm_system *sys, unsigned flags); /* msld.cu:1000-1130 */
+
+/* State::update(int step,System*) (update/update.cu:233-305): one VRORV
+ * step including holonomic constraints and theta->lambda. */
+int bm_update(bm_system *sys, long step, int energy_step);
+
+/* holonomic_position / holonomic_velocity (holonomic/holonomic.h:14-15) */
+int bm_holonomic_position(bm_system *sys);
+int bm_holonomic_velocity(bm_system *sys);
+
+/* Msld::calc_lambda_from_theta / calc_thetaForce_from_lambdaForce
+ * (msld/msld.h:134-136) */
+int bm_msld_theta_to_lambda(bm_system *sys);
+int bm_msld_chain_rule(bm_system *sys);
+
+/* Run::dynamics step loop (run/run.cu:545-570) for nsteps steps starting
+ * at `step0`; energies are computed on steps with step%freq_nrg==0.
+ * use_graph = 1 replays a captured hipGraph of the non-rebuild steps. */
+int bm_dynamics(bm_system *sys, long step0, long nsteps, int freq_nrg, int use_graph);
+
+/* synchronise the system's stream */
+int bm_sync(bm_system *sys);
+
+/* ---- host readback (State::recv_* equivalents, system/state.cxx:330-365) */
+int bm_get_positions(bm_system *sys, double *pos /*[3N]*/, double *theta /*[nL]*/);
+int bm_set_positions(bm_system *sys, const double *pos, const double *theta);
+int bm_get_velocities(bm_system *sys, double *vel /*[3N]*/, double *theta_vel /*[nL]*/);
+int bm_get_forces(bm_system *sys, double *force /*[3N]*/, double *theta_force /*[nL]*/);
+int bm_get_dudl(bm_system *sys, double *dudl /*[nL]*/);
+int bm_get_lambda(bm_system *sys, double *lambda /*[nL]*/);
+int bm_get_energy(bm_system *sys, double *energy /*[BM_NTERMS]*/);
+int bm_get_nan_flag(bm_system *sys, int *flag);
+
+/* ---- domdec readback for bit-exact list checks */
+int bm_get_domdec_sizes(bm_system *sys, int *n_blocks, int *n_tiles, int *n_excl_tiles);
+int bm_get_domdec(bm_system *sys, int *local_to_global /*[32*nb], -1 = empty*/,
+                  int *block_count /*[nb]*/, int *tile_i /*[nt]*/, int *tile_j /*[nt]*/,
+                  unsigned *tile_mask /*[nt*32]*/);
+
+/* ---- timing of the hot kernels over the last bm_dynamics call */
+int bm_get_kernel_times(bm_system *sys, double *ms_nbdirect, double *ms_pme,
+                        double *ms_total, long *n_measured_steps);
+
+/* Per-phase timing: nsteps eager steps with HIP events on the system's
+ * stream around the nbdirect and PME launches; out[3] = average ms of
+ * nbdirect, PME (spread+FFT+gather) and the whole step. */
+int bm_profile_steps(bm_system *sys, long step0, long nsteps, double *out);
+
+/* raw fixed-point force accumulators [3N+nL] (bit-reproducibility checks) */
+int bm_get_force_fixed(bm_system *sys, long long *force);
+
+/* library build information and the last error message of this thread */
+const char *bm_version(void);
+const char *bm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1260 @@
+// oracle.cpp - FP64 host restatement of BLaDE's per-step path.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker for the MI355X kernels and
+// the OpenMP CPU baseline timed by bench.py.  The product library never
+// links or calls this file.  Built with -ffp-contract=off so that the
+// integer list semantics (orc_domdec) round exactly like the GPU path.
+//
+// Units AKMA (reference main/defines.h:42-61 CHARMM_UNITS branch).
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+const double kElectric = 332.0716;          // main/defines.h:55
+const double kBoltz = 0.001987191;          // main/defines.h:53
+const double softCoreRadius = 4.0;          // main/defines.h:48
+const double twoOverSqrtPi = 1.1283791670955126;
+
+inline double wrapd(double d, double L) { return d - L * std::floor(d / L + 0.5); }
+
+struct Vec3 { double x, y, z; };
+inline Vec3 v3(double a, double b, double c) { return Vec3{a, b, c}; }
+inline Vec3 operator-(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline Vec3 operator+(Vec3 a, Vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline Vec3 operator*(double s, Vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+inline double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 cross(Vec3 a, Vec3 b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline double norm(Vec3 a) { return std::sqrt(dot(a, a)); }
+
+inline Vec3 getp(const double *pos, int n, int i) { return v3(pos[i], pos[n + i], pos[2 * n + i]); }
+inline Vec3 minimg(Vec3 d, const double *L) { return v3(wrapd(d.x, L[0]), wrapd(d.y, L[1]), wrapd(d.z, L[2])); }
+inline void addf(double *f, int n, int i, Vec3 v) { f[i] += v.x; f[n + i] += v.y; f[2 * n + i] += v.z; }
+
+inline int blockOf(int sb) { return sb & 0xFFFF; }
+inline double lam(const double *lambda, int sb) { return (sb & 0xFFFF) ? lambda[sb & 0xFFFF] : 1.0; }
+
+// ---------------------------------------------------------------------
+// Pair functions
+// ---------------------------------------------------------------------
+
+struct PairOut { double f, e; };
+
+// Electrostatics + vdW of one pair at effective distance r
+// (nbdirect/nbdirect.cu:277-398).  f = dE/dr.
+PairOut nb_pair(const bm_runparams *rp, double qq, double c12, double c6, double r) {
+  const double rc = rp->rcut, rs = rp->rswitch;
+  double rinv = 1.0 / r;
+  double f = 0, e = 0;
+  if (rp->elec_method == BM_FSWITCH) {
+    double roff2 = rc * rc, ron2 = rs * rs;
+    double ginv = 1 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
+    double A = roff2 * roff2 * (roff2 - 3 * ron2) * ginv, B = 6 * roff2 * ron2 * ginv;
+    double C = -(ron2 + roff2) * ginv, D = 2 * ginv / 5;
+    double dvc = 8 * (ron2 * roff2 * (rc - rs) - (roff2 * roff2 * rc - ron2 * ron2 * rs) / 5) * ginv;
+    double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+    if (r <= rs) {
+      f = -kElectric * qq * rinv * rinv;
+      e = kElectric * qq * (rinv + dvc);
+    } else {
+      f = -kElectric * qq * rinv * (A * rinv + B * r + 3 * C * r3 + 5 * D * r5);
+      e = kElectric * qq * (A * (rinv - 1 / rc) + B * (rc - r) + C * (roff2 * rc - r3) + D * (roff2 * roff2 * rc - r5));
+    }
+  } else if (rp->elec_method == BM_PME) {
+    double br = rp->beta_ewald * r;
+    double erfcrinv = std::erfc(br) * rinv;
+    f = -kElectric * qq * (erfcrinv + twoOverSqrtPi * rp->beta_ewald * std::exp(-br * br)) * rinv;
+    e = kElectric * qq * erfcrinv;
+  } else {
+    double roffinv = 1 / rc, roffinv2 = roffinv * roffinv, kqq = kElectric * qq;
+    f = -kqq * (rinv * rinv - roffinv2);
+    e = kqq * (rinv - 2 * roffinv + r * roffinv2);
+  }
+  double rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
+  if (rp->vdw_method == BM_VFSWITCH) {
+    double rc3 = rc * rc * rc, rs3 = rs * rs * rs;
+    if (r < rs) {
+      double dv6 = 1 / (rc3 * rs3);
+      f += (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv;
+      e += c12 * (rinv6 * rinv6 - dv6 * dv6) - c6 * (rinv6 - dv6);
+    } else {
+      double k6 = rc3 / (rc3 - rs3), k12 = rc3 * rc3 / (rc3 * rc3 - rs3 * rs3), rcinv3 = 1 / rc3;
+      f += (6 * c6 * k6 * (rinv3 - rcinv3) * rinv3 - 12 * c12 * k12 * (rinv6 - rcinv3 * rcinv3) * rinv6) * rinv;
+      e += c12 * k12 * (rinv6 - rcinv3 * rcinv3) * (rinv6 - rcinv3 * rcinv3) - c6 * k6 * (rinv3 - rcinv3) * (rinv3 - rcinv3);
+    }
+  } else if (rp->vdw_method == BM_VSWITCH) {
+    if (r < rs) {
+      f += (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv;
+      e += c12 * rinv6 * rinv6 - c6 * rinv6;
+    } else {
+      double c2of = rc * rc, c2on = rs * rs;
+      double rul3 = (c2of - c2on) * (c2of - c2on) * (c2of - c2on), rul12 = 12 / rul3;
+      double rijl = c2on - r * r, riju = c2of - r * r;
+      double fsw = riju * riju * (riju - 3 * rijl) / rul3, dfsw = rijl * riju * rul12;
+      f += fsw * (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv + dfsw * (c12 * rinv6 - c6) * rinv6;
+      e += fsw * (c12 * rinv6 - c6) * rinv6;
+    }
+  } else {
+    double r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
+    double rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
+    double rc2 = rc * rc, rc6 = rc2 * rc2 * rc2, rc12 = rc6 * rc6, rc18 = rc12 * rc6;
+    f += 6 * c6 * (rinv7 - r5 / rc12) - 12 * c12 * (rinv13 + r5 / rc18);
+    e += c12 * (rinv12 + 2 * r6 / rc18 - 3 / rc12) - c6 * (rinv6 + r6 / rc12 - 2 / rc6);
+  }
+  return PairOut{f, e};
+}
+
+// 1-4 pair (bonded/pair.cu:56-175).  Note the VFSWITCH energy below the
+// switch distance is shifted by rc^-6 / rc^-12 (pair.cu:92-103), which
+// differs from the nbdirect shift; the restatement keeps it.
+PairOut nb14_pair(const bm_runparams *rp, double qq, double c12, double c6, double e14fac, double r) {
+  const double rc = rp->rcut, rs = rp->rswitch;
+  if (!(r < rc)) return PairOut{0, 0};
+  double rinv = 1 / r, rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
+  double f = 0, e = 0;
+  if (rp->vdw_method == BM_VFSWITCH) {
+    double rc3 = rc * rc * rc, rc6 = rc3 * rc3, rs3 = rs * rs * rs, rs6 = rs3 * rs3;
+    double k6 = rc3 / (rc3 - rs3), k12 = rc6 / (rc6 - rs6);
+    bool in = r < rs;
+    double A6 = in ? 1 : k6, A12 = in ? 1 : k12, B6 = in ? 0 : 1 / rc3, B12 = in ? 0 : 1 / rc6;
+    f = 6 * c6 * A6 * (rinv3 - B6) * rinv3 * rinv - 12 * c12 * A12 * (rinv6 - B12) * rinv6 * rinv;
+    double CC6 = in ? -1 / rc6 : 0, CC12 = in ? -1 / (rc6 * rc6) : 0;
+    e = c12 * (A12 * (rinv6 - B12) * (rinv6 - B12) + CC12) - c6 * (A6 * (rinv3 - B6) * (rinv3 - B6) + CC6);
+  } else if (rp->vdw_method == BM_VSWITCH) {
+    double c2of = rc * rc, c2on = rs * rs;
+    double rul3 = (c2of - c2on) * (c2of - c2on) * (c2of - c2on), rul12 = 12 / rul3;
+    double rijl = c2on - r * r, riju = c2of - r * r;
+    double fsw = (r < rs) ? 1 : riju * riju * (riju - 3 * rijl) / rul3;
+    double dfsw = (r < rs) ? 0 : rijl * riju * rul12;
+    f = fsw * (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv + dfsw * (c12 * rinv6 - c6) * rinv6;
+    e = fsw * (c12 * rinv6 - c6) * rinv6;
+  } else {
+    double r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
+    double rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
+    double rc2 = rc * rc, rc6 = rc2 * rc2 * rc2, rc12 = rc6 * rc6, rc18 = rc12 * rc6;
+    f = 6 * c6 * (rinv7 - r5 / rc12) - 12 * c12 * (rinv13 + r5 / rc18);
+    e = c12 * (rinv12 + 2 * r6 / rc18 - 3 / rc12) - c6 * (rinv6 + r6 / rc12 - 2 / rc6);
+  }
+  double kqq = kElectric * qq;
+  if (rp->elec_method == BM_FSWITCH) {
+    double kqqe14 = kqq * e14fac;
+    double roff2 = rc * rc, ron2 = rs * rs;
+    double ginv = 1 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
+    double A = roff2 * roff2 * (roff2 - 3 * ron2) * ginv, B = 6 * roff2 * ron2 * ginv;
+    double C = -(ron2 + roff2) * ginv, D = 2 * ginv / 5;
+    double dvc = 8 * (ron2 * roff2 * (rc - rs) - (roff2 * roff2 * rc - ron2 * ron2 * rs) / 5) * ginv;
+    double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+    if (r <= rs) {
+      f += -kqqe14 * rinv * rinv;
+      e += kqqe14 * (rinv + dvc);
+    } else {
+      f += -kqqe14 * rinv * (A * rinv + B * r + 3 * C * r3 + 5 * D * r5);
+      e += kqqe14 * (A * (rinv - 1 / rc) + B * (rc - r) + C * (roff2 * rc - r3) + D * (roff2 * roff2 * rc - r5));
+    }
+  } else if (rp->elec_method == BM_PME) {
+    double br = rp->beta_ewald * r;
+    double erfcrinv = (std::erfc(br) + e14fac - 1) * rinv;
+    f += kqq * (-erfcrinv - twoOverSqrtPi * rp->beta_ewald * std::exp(-br * br)) * rinv;
+    e += kqq * erfcrinv;
+  } else {
+    double roffinv = 1 / rc, roffinv2 = roffinv * roffinv, kqqe14 = kqq * e14fac;
+    f += -kqqe14 * (rinv * rinv - roffinv2);
+    e += kqqe14 * (rinv - 2 * roffinv + r * roffinv2);
+  }
+  return PairOut{f, e};
+}
+
+// Soft-core effective distance (nbdirect/nbdirect.cu:257-275).
+struct Soft { double reff, dredr, dredll; };
+inline Soft soft_core(double r, double ll) {
+  Soft s{r, 1, 0};
+  double rSoft = softCoreRadius * (1 - ll);
+  if (r < rSoft) {
+    double rdivrs = r / rSoft;
+    double re = 1 - 0.5 * rdivrs;
+    re = re * rdivrs * rdivrs * rdivrs + 0.5;
+    double dredr = (3 - 2 * rdivrs) * rdivrs * rdivrs;
+    double dredll = (re - dredr * rdivrs) * (-softCoreRadius);
+    s.reff = re * rSoft;
+    s.dredr = dredr;
+    s.dredll = dredll;
+  }
+  return s;
+}
+
+// Sorted per-atom exclusion lists (CSR).
+struct Excl {
+  std::vector<int> start, list;
+  Excl(const bm_topology *t) {
+    int n = t->n_atoms;
+    std::vector<std::vector<int>> tmp(n);
+    for (int e = 0; e < t->n_excl; e++) {
+      int a = t->excl_idx[2 * e], b = t->excl_idx[2 * e + 1];
+      tmp[a].push_back(b);
+      tmp[b].push_back(a);
+    }
+    start.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+      std::sort(tmp[i].begin(), tmp[i].end());
+      start[i + 1] = start[i] + (int)tmp[i].size();
+    }
+    list.reserve(start[n]);
+    for (int i = 0; i < n; i++) list.insert(list.end(), tmp[i].begin(), tmp[i].end());
+  }
+  bool has(int i, int j) const {
+    return std::binary_search(list.begin() + start[i], list.begin() + start[i + 1], j);
+  }
+};
+
+// ---------------------------------------------------------------------
+// nbdirect: every non-excluded pair with r<rcut, full shell per atom
+// (nbdirect/nbdirect.cu:87-490 pair semantics).
+// ---------------------------------------------------------------------
+long nbdirect(const bm_topology *t, const bm_runparams *rp, const double *pos,
+              const double *lambda, double *force, double *dudl, double *energy) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  const bool alch = t->n_blocks > 1;
+  const bool useSoft = alch && rp->soft_core;
+  Excl ex(t);
+  // cell list with cells >= rcut
+  int nc[3];
+  for (int d = 0; d < 3; d++) nc[d] = std::max(1, (int)std::floor(L[d] / rp->rcut));
+  bool useCells = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+  if (!useCells) nc[0] = nc[1] = nc[2] = 1;
+  int ncell = nc[0] * nc[1] * nc[2];
+  std::vector<int> cellOf(n), cellStart(ncell + 1, 0), cellAtoms(n);
+  for (int i = 0; i < n; i++) {
+    int c[3];
+    for (int d = 0; d < 3; d++) {
+      double x = pos[d * n + i];
+      double f = x / L[d] - std::floor(x / L[d]);
+      c[d] = std::min(nc[d] - 1, (int)(f * nc[d]));
+    }
+    cellOf[i] = (c[0] * nc[1] + c[1]) * nc[2] + c[2];
+    cellStart[cellOf[i] + 1]++;
+  }
+  for (int c = 0; c < ncell; c++) cellStart[c + 1] += cellStart[c];
+  {
+    std::vector<int> fill(cellStart.begin(), cellStart.end() - 1);
+    for (int i = 0; i < n; i++) cellAtoms[fill[cellOf[i]]++] = i;
+  }
+  int nthr = omp_get_max_threads();
+  int nl = t->n_blocks;
+  std::vector<double> dudlT((size_t)nthr * nl, 0.0);
+  double eTot = 0;
+  long pairs = 0;
+#pragma omp parallel reduction(+ : eTot, pairs)
+  {
+    double *mydudl = &dudlT[(size_t)omp_get_thread_num() * nl];
+#pragma omp for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+      Vec3 xi = getp(pos, n, i);
+      int ci = cellOf[i];
+      int cx = ci / (nc[1] * nc[2]), cy = (ci / nc[2]) % nc[1], cz = ci % nc[2];
+      Vec3 fi = v3(0, 0, 0);
+      int bi = t->site_block[i];
+      double li = lam(lambda, bi);
+      double qi = t->charge[i];
+      int ti = t->type_idx[i];
+      int span = useCells ? 1 : 0;
+      for (int ox = -span; ox <= span; ox++)
+        for (int oy = -span; oy <= span; oy++)
+          for (int oz = -span; oz <= span; oz++) {
+            int c = (((cx + ox + nc[0]) % nc[0]) * nc[1] + (cy + oy + nc[1]) % nc[1]) * nc[2] + (cz + oz + nc[2]) % nc[2];
+            for (int q = cellStart[c]; q < cellStart[c + 1]; q++) {
+              int j = cellAtoms[q];
+              if (j == i || ex.has(i, j)) continue;
+              Vec3 dr = minimg(xi - getp(pos, n, j), L);
+              double r = norm(dr);
+              if (!(r < rp->rcut)) continue;
+              if (i < j) pairs++;
+              int bj = t->site_block[j];
+              double lj = lam(lambda, bj);
+              double s = 1;
+              bool sameSite = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
+              if (alch) s = sameSite ? (bi == bj ? li : 0.0) : li * lj;
+              Soft sc{r, 1, 0};
+              if (useSoft && (bi || bj)) sc = soft_core(r, s);
+              int tj = t->type_idx[j];
+              PairOut p = nb_pair(rp, qi * t->charge[j], t->vdw_c12[ti * t->n_types + tj],
+                                  t->vdw_c6[ti * t->n_types + tj], sc.reff);
+              double fij = p.f * s;
+              if (alch && (bi || bj)) {
+                double fl = p.e + fij * sc.dredll;
+                if (sameSite) {
+                  if (bi == bj && blockOf(bi)) mydudl[blockOf(bi)] += 0.5 * fl;
+                } else if (blockOf(bi)) {
+                  mydudl[blockOf(bi)] += lj * fl;
+                }
+              }
+              fij *= sc.dredr;
+              fi = fi + (fij / r) * dr;
+              eTot += 0.5 * s * p.e;
+            }
+          }
+      force[i] += fi.x;
+      force[n + i] += fi.y;
+      force[2 * n + i] += fi.z;
+    }
+  }
+  for (int th = 0; th < nthr; th++)
+    for (int b = 0; b < nl; b++) dudl[b] += dudlT[(size_t)th * nl + b];
+  energy[BM_ENBDIRECT] += eTot;
+  return pairs;
+}
+
+// Ewald self term (nbrecip/nbrecip.cu:19-77).
+void ewaldself(const bm_topology *t, const bm_runparams *rp, const double *lambda,
+               double *dudl, double *energy) {
+  double pref = -rp->beta_ewald * kElectric / std::sqrt(M_PI);
+  double e = 0;
+  for (int i = 0; i < t->n_atoms; i++) {
+    int b = blockOf(t->site_block[i]);
+    double l = b ? lambda[b] : 1.0;
+    double q = t->charge[i];
+    double ei = pref * q * q;
+    if (b) dudl[b] += 2 * l * ei;
+    e += ei * l * l;
+  }
+  energy[BM_ENBRECIPSELF] += e;
+}
+
+// ---------------------------------------------------------------------
+// FFT (mixed radix, recursive, FP64)
+// ---------------------------------------------------------------------
+typedef std::complex<double> cplx;
+
+void fft1d(cplx *a, int n, int stride, int sign, cplx *tmp) {
+  if (n == 1) return;
+  int p = 2;
+  while (p * p <= n && n % p) p++;
+  if (n % p) p = n;
+  int m = n / p;
+  // gather p sub-sequences
+  for (int r = 0; r < p; r++)
+    for (int k = 0; k < m; k++) tmp[r * m + k] = a[(k * p + r) * stride];
+  for (int r = 0; r < p; r++) {
+    std::vector<cplx> sub(tmp + r * m, tmp + r * m + m), scratch(m);
+    fft1d(sub.data(), m, 1, sign, scratch.data());
+    std::copy(sub.begin(), sub.end(), tmp + r * m);
+  }
+  for (int k = 0; k < n; k++) {
+    cplx s = 0;
+    for (int r = 0; r < p; r++) {
+      double ang = sign * 2 * M_PI * (double)((long)r * k % n) / n;
+      s += tmp[r * m + (k % m)] * cplx(std::cos(ang), std::sin(ang));
+    }
+    a[k * stride] = s;
+  }
+}
+
+void fft3d(int K0, int K1, int K2, cplx *g, int sign) {
+  // z lines
+#pragma omp parallel
+  {
+    std::vector<cplx> line(std::max(K0, std::max(K1, K2))), tmp(line.size());
+#pragma omp for
+    for (int ij = 0; ij < K0 * K1; ij++) {
+      for (int k = 0; k < K2; k++) line[k] = g[(size_t)ij * K2 + k];
+      fft1d(line.data(), K2, 1, sign, tmp.data());
+      for (int k = 0; k < K2; k++) g[(size_t)ij * K2 + k] = line[k];
+    }
+#pragma omp for
+    for (int ik = 0; ik < K0 * K2; ik++) {
+      int i = ik / K2, k = ik % K2;
+      for (int j = 0; j < K1; j++) line[j] = g[((size_t)i * K1 + j) * K2 + k];
+      fft1d(line.data(), K1, 1, sign, tmp.data());
+      for (int j = 0; j < K1; j++) g[((size_t)i * K1 + j) * K2 + k] = line[j];
+    }
+#pragma omp for
+    for (int jk = 0; jk < K1 * K2; jk++) {
+      int j = jk / K2, k = jk % K2;
+      for (int i = 0; i < K0; i++) line[i] = g[((size_t)i * K1 + j) * K2 + k];
+      fft1d(line.data(), K0, 1, sign, tmp.data());
+      for (int i = 0; i < K0; i++) g[((size_t)i * K1 + j) * K2 + k] = line[i];
+    }
+  }
+}
+
+// Cardinal B-spline M_n(u), support [0,n] (recursion M_n(u) = (u M_{n-1}(u)
+// + (n-u) M_{n-1}(u-1))/(n-1)).
+double bspline(int n, double u) {
+  if (n == 1) return (u >= 0 && u < 1) ? 1.0 : 0.0;  // M_1 on [0,1)
+  if (u <= 0 || u >= n) return 0;
+  return (u * bspline(n - 1, u) + (n - u) * bspline(n - 1, u - 1)) / (n - 1);
+}
+
+// 1/|b(m)|^2 with b(m) built from M_n at integer points
+// (system/potential.cxx:1245-1287).
+void bsp_moduli(int n, int K, double *out) {
+  for (int i = 0; i < K; i++) {
+    double re = 0, im = 0;
+    for (int l = 1; l < n; l++) {
+      double m = bspline(n, l);
+      re += m * std::cos(2 * M_PI * i * l / K);
+      im += m * std::sin(2 * M_PI * i * l / K);
+    }
+    out[i] = 1.0 / (re * re + im * im);
+  }
+}
+
+// Weights and derivatives of the n spline points of one coordinate:
+// point (u0+k) mod K gets w[k] = M_n(frac+n-1-k) (nbrecip/nbrecip.cu:108-141).
+void spline_weights(int n, double frac, double *w, double *dw) {
+  for (int k = 0; k < n; k++) {
+    double u = frac + n - 1 - k;
+    w[k] = bspline(n, u);
+    dw[k] = bspline(n - 1, u) - bspline(n - 1, u - 1);
+  }
+}
+
+// Smooth PME (nbrecip/nbrecip.cu:83-485): lambda-scaled spread, FFT,
+// convolution with B(m) (k_e/2 pi V) exp(-pi^2 m^2/beta^2)/m^2, inverse FFT,
+// gather of force 2 l q grad(phi), dU/dl 2 q phi, energy l q phi.
+void pme(const bm_topology *t, const bm_runparams *rp, const double *pos,
+         const double *lambda, double *force, double *dudl, double *energy) {
+  const int n = t->n_atoms, ord = rp->order;
+  const int K0 = rp->grid[0], K1 = rp->grid[1], K2 = rp->grid[2];
+  const double *L = rp->box;
+  std::vector<cplx> grid((size_t)K0 * K1 * K2, 0.0);
+  std::vector<int> u0(3 * n);
+  std::vector<double> W(3 * n * ord), DW(3 * n * ord);
+  const int K[3] = {K0, K1, K2};
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      double u = pos[d * n + i] * K[d] / L[d];
+      double fl = std::floor(u);
+      int iu = (int)fl;
+      u0[3 * i + d] = ((iu % K[d]) + K[d]) % K[d];
+      spline_weights(ord, u - fl, &W[(3 * i + d) * ord], &DW[(3 * i + d) * ord]);
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    int b = blockOf(t->site_block[i]);
+    double lq = (b ? lambda[b] : 1.0) * t->charge[i];
+    for (int a = 0; a < ord; a++) {
+      int ia = (u0[3 * i] + a) % K0;
+      for (int c = 0; c < ord; c++) {
+        int ic = (u0[3 * i + 1] + c) % K1;
+        double wac = lq * W[(3 * i) * ord + a] * W[(3 * i + 1) * ord + c];
+        for (int e = 0; e < ord; e++) {
+          int ie = (u0[3 * i + 2] + e) % K2;
+          grid[((size_t)ia * K1 + ic) * K2 + ie] += wac * W[(3 * i + 2) * ord + e];
+        }
+      }
+    }
+  }
+  fft3d(K0, K1, K2, grid.data(), -1);
+  std::vector<double> bx(K0), by(K1), bz(K2);
+  bsp_moduli(ord, K0, bx.data());
+  bsp_moduli(ord, K1, by.data());
+  bsp_moduli(ord, K2, bz.data());
+  double V = L[0] * L[1] * L[2];
+  for (int i = 0; i < K0; i++) {
+    int mi = (2 * i <= K0) ? i : i - K0;
+    for (int j = 0; j < K1; j++) {
+      int mj = (2 * j <= K1) ? j : j - K1;
+      for (int k = 0; k < K2; k++) {
+        int mk = (2 * k <= K2) ? k : k - K2;
+        double m2 = (mi / L[0]) * (mi / L[0]) + (mj / L[1]) * (mj / L[1]) + (mk / L[2]) * (mk / L[2]);
+        size_t idx = ((size_t)i * K1 + j) * K2 + k;
+        if (idx == 0) { grid[0] = 0; continue; }
+        double fac = bx[i] * by[j] * bz[k] * (0.5 * kElectric / M_PI) / V *
+                     std::exp(-M_PI * M_PI * m2 / (rp->beta_ewald * rp->beta_ewald)) / m2;
+        grid[idx] *= fac;
+      }
+    }
+  }
+  fft3d(K0, K1, K2, grid.data(), +1);
+  double eTot = 0;
+  for (int i = 0; i < n; i++) {
+    int b = blockOf(t->site_block[i]);
+    double l = b ? lambda[b] : 1.0, q = t->charge[i];
+    double phi = 0, gx = 0, gy = 0, gz = 0;
+    for (int a = 0; a < ord; a++) {
+      int ia = (u0[3 * i] + a) % K0;
+      double wa = W[(3 * i) * ord + a], da = DW[(3 * i) * ord + a];
+      for (int c = 0; c < ord; c++) {
+        int ic = (u0[3 * i + 1] + c) % K1;
+        double wc = W[(3 * i + 1) * ord + c], dc = DW[(3 * i + 1) * ord + c];
+        for (int e = 0; e < ord; e++) {
+          int ie = (u0[3 * i + 2] + e) % K2;
+          double we = W[(3 * i + 2) * ord + e], de = DW[(3 * i + 2) * ord + e];
+          double P = grid[((size_t)ia * K1 + ic) * K2 + ie].real();
+          phi += P * wa * wc * we;
+          gx += P * da * wc * we;
+          gy += P * wa * dc * we;
+          gz += P * wa * wc * de;
+        }
+      }
+    }
+    force[i] += 2 * l * q * gx * K0 / L[0];
+    force[n + i] += 2 * l * q * gy * K1 / L[1];
+    force[2 * n + i] += 2 * l * q * gz * K2 / L[2];
+    if (b) dudl[b] += 2 * q * phi;
+    eTot += l * q * phi;
+  }
+  energy[BM_ENBRECIP] += eTot;
+}
+
+// ---------------------------------------------------------------------
+// Bonded terms (bonded/bonded.cu:44-527), non-soft variants.
+// ---------------------------------------------------------------------
+struct Scale2 { int b0, b1; double l0, l1; };
+inline Scale2 scale2(const int *sb, const double *lambda) {
+  Scale2 s{blockOf(sb[0]), blockOf(sb[1]), 1, 1};
+  if (s.b0) {
+    s.l0 = lambda[s.b0];
+    if (s.b1) s.l1 = lambda[s.b1];
+  }
+  return s;
+}
+inline void scale_dudl(const Scale2 &s, double e, double *dudl) {
+  if (s.b0) {
+    dudl[s.b0] += s.l1 * e;
+    if (s.b1) dudl[s.b1] += s.l0 * e;
+  }
+}
+
+void bonded(const bm_topology *t, const bm_runparams *rp, const double *pos,
+            const double *lambda, double *force, double *dudl, double *energy) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  for (int b = 0; b < t->n_bonds; b++) {
+    int i = t->bond_idx[2 * b], j = t->bond_idx[2 * b + 1];
+    Vec3 dr = minimg(getp(pos, n, i) - getp(pos, n, j), L);
+    double r = norm(dr);
+    Scale2 s = scale2(&t->bond_sb[2 * b], lambda);
+    double db = r - t->bond_b0[b];
+    double e = 0.5 * t->bond_kb[b] * db * db;
+    double f = t->bond_kb[b] * db * s.l0 * s.l1;
+    scale_dudl(s, e, dudl);
+    addf(force, n, i, (f / r) * dr);
+    addf(force, n, j, (-f / r) * dr);
+    energy[b < t->n_bonds12 ? BM_EBOND : BM_EUREY] += e * s.l0 * s.l1;
+  }
+  for (int a = 0; a < t->n_angles; a++) {
+    int i = t->angle_idx[3 * a], j = t->angle_idx[3 * a + 1], k = t->angle_idx[3 * a + 2];
+    Vec3 rij = minimg(getp(pos, n, i) - getp(pos, n, j), L);
+    Vec3 rkj = minimg(getp(pos, n, k) - getp(pos, n, j), L);
+    double dp = dot(rij, rkj);
+    Vec3 cr = cross(rij, rkj);
+    double mc = norm(cr);
+    double th = std::atan2(mc, dp);
+    Scale2 s = scale2(&t->angle_sb[2 * a], lambda);
+    double dth = th - t->angle_t0[a];
+    double e = 0.5 * t->angle_k[a] * dth * dth;
+    double f = t->angle_k[a] * dth * s.l0 * s.l1;
+    scale_dudl(s, e, dudl);
+    Vec3 fi = (f / (mc * dot(rij, rij))) * cross(rij, cr);
+    Vec3 fk = (-f / (mc * dot(rkj, rkj))) * cross(rkj, cr);
+    addf(force, n, i, fi);
+    addf(force, n, k, fk);
+    addf(force, n, j, -1.0 * (fi + fk));
+    energy[BM_EANGLE] += e * s.l0 * s.l1;
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    int cnt = pass ? t->n_imprs : t->n_dihes;
+    const int *idx = pass ? t->impr_idx : t->dihe_idx;
+    const int *sbv = pass ? t->impr_sb : t->dihe_sb;
+    for (int d = 0; d < cnt; d++) {
+      int i = idx[4 * d], j = idx[4 * d + 1], k = idx[4 * d + 2], l = idx[4 * d + 3];
+      Vec3 rij = minimg(getp(pos, n, i) - getp(pos, n, j), L);
+      Vec3 rjk = minimg(getp(pos, n, j) - getp(pos, n, k), L);
+      Vec3 rkl = minimg(getp(pos, n, k) - getp(pos, n, l), L);
+      Vec3 m = cross(rij, rjk), nv = cross(rjk, rkl);
+      double sinp = norm(cross(m, nv)), cosp = dot(m, nv);
+      double phi = std::atan2(sinp, cosp);
+      if (dot(rij, nv) > 0) phi = -phi;   // bonded.cu:372-374
+      double fphi, e;
+      if (!pass) {
+        double dphi = t->dihe_n[d] * phi - t->dihe_d0[d];
+        fphi = -t->dihe_k[d] * t->dihe_n[d] * std::sin(dphi);
+        e = t->dihe_k[d] * (std::cos(dphi) + 1);
+      } else if (t->impr_n[d] > 0) {
+        double dphi = t->impr_n[d] * phi - t->impr_d0[d];
+        fphi = -t->impr_k[d] * t->impr_n[d] * std::sin(dphi);
+        e = t->impr_k[d] * (std::cos(dphi) + 1);
+      } else {
+        double dphi = phi - t->impr_d0[d];
+        dphi -= 2 * M_PI * std::floor((dphi + M_PI) / (2 * M_PI));
+        fphi = t->impr_k[d] * dphi;
+        e = 0.5 * t->impr_k[d] * dphi * dphi;
+      }
+      Scale2 s = scale2(&sbv[2 * d], lambda);
+      fphi *= s.l0 * s.l1;
+      scale_dudl(s, e, dudl);
+      double rjk2 = dot(rjk, rjk), r = std::sqrt(rjk2);
+      Vec3 fi = (-fphi * r / dot(m, m)) * m;
+      Vec3 fl = (-fphi * r / dot(nv, nv)) * nv;   // fk before negation
+      double p = dot(rij, rjk) / rjk2, q = dot(rkl, rjk) / rjk2;
+      Vec3 fj = (-p) * fi + (-q) * fl;
+      Vec3 fkk = fl - fj;
+      Vec3 fjj = fj - fi;
+      addf(force, n, i, fi);
+      addf(force, n, l, -1.0 * fl);
+      addf(force, n, k, fkk);
+      addf(force, n, j, fjj);
+      energy[pass ? BM_EIMPR : BM_EDIHE] += e * s.l0 * s.l1;
+    }
+  }
+}
+
+// 1-4 pairs and PME exclusion corrections (bonded/pair.cu:197-387).
+void nb14(const bm_topology *t, const bm_runparams *rp, const double *pos,
+          const double *lambda, double *force, double *dudl, double *energy) {
+  const int n = t->n_atoms;
+  for (int p = 0; p < t->n_nb14; p++) {
+    int i = t->nb14_idx[2 * p], j = t->nb14_idx[2 * p + 1];
+    Vec3 dr = minimg(getp(pos, n, i) - getp(pos, n, j), rp->box);
+    double r = norm(dr);
+    Scale2 s = scale2(&t->nb14_sb[2 * p], lambda);
+    Soft sc{r, 1, 0};
+    if (rp->soft_core14 && s.b0) sc = soft_core(r, s.l0 * s.l1);
+    PairOut o = nb14_pair(rp, t->nb14_qq[p], t->nb14_c12[p], t->nb14_c6[p], t->nb14_e14fac[p], sc.reff);
+    double f = o.f * s.l0 * s.l1;
+    scale_dudl(s, o.e + f * sc.dredll, dudl);
+    f *= sc.dredr;
+    addf(force, n, i, (f / r) * dr);
+    addf(force, n, j, (-f / r) * dr);
+    energy[BM_ENB14] += o.e * s.l0 * s.l1;
+  }
+}
+
+void nbex(const bm_topology *t, const bm_runparams *rp, const double *pos,
+          const double *lambda, double *force, double *dudl, double *energy) {
+  const int n = t->n_atoms;
+  const double beta = rp->beta_ewald;
+  for (int p = 0; p < t->n_nbex; p++) {
+    int i = t->nbex_idx[2 * p], j = t->nbex_idx[2 * p + 1];
+    Vec3 dr = minimg(getp(pos, n, i) - getp(pos, n, j), rp->box);
+    double r = norm(dr), rinv = 1 / r, br = beta * r;
+    double kqq = kElectric * t->nbex_qq[p];
+    double f = kqq * (std::erf(br) * rinv - twoOverSqrtPi * beta * std::exp(-br * br)) * rinv;
+    double e = -kqq * std::erf(br) * rinv;
+    Scale2 s = scale2(&t->nbex_sb[2 * p], lambda);
+    f *= s.l0 * s.l1;
+    scale_dudl(s, e, dudl);
+    addf(force, n, i, (f / r) * dr);
+    addf(force, n, j, (-f / r) * dr);
+    energy[BM_ENBRECIPEXCL] += e * s.l0 * s.l1;
+  }
+}
+
+// Fixed and variable lambda biases (msld/msld.cu:1000-1130).
+void msldbias(const bm_topology *t, const double *lambda, double *dudl, double *energy) {
+  for (int b = 0; b < t->n_blocks; b++) {
+    double bias = t->fixed_bias ? t->fixed_bias[b] : 0.0;
+    dudl[b] += bias;
+    energy[BM_ELAMBDA] += bias * lambda[b];
+  }
+  for (int v = 0; v < t->n_varbias; v++) {
+    int i = t->vb_i[v], j = t->vb_j[v], ty = t->vb_type[v], nn = t->vb_n[v];
+    double li = lambda[i], lj = lambda[j], k = t->vb_k[v], l0 = t->vb_l0[v];
+    double e = 0, fi = 0, fj = 0;
+    switch (ty) {
+      case 6: e = k * li * lj; fi = k * lj; fj = k * li; break;
+      case 8: e = k * li * lj / (li + l0); fi = k * l0 * lj / ((li + l0) * (li + l0)); fj = k * li / (li + l0); break;
+      case 10: e = k * lj * (1 - std::exp(l0 * li)); fi = k * lj * (-l0 * std::exp(l0 * li)); fj = k * (1 - std::exp(l0 * li)); break;
+      case 1: if (li < l0) { e = k * std::pow(li - l0, nn); fi = nn * k * std::pow(li - l0, nn - 1); } break;
+      case 2: if (li > l0) { e = k * std::pow(li - l0, nn); fi = nn * k * std::pow(li - l0, nn - 1); } break;
+      case 3: e = k * std::pow(li - lj, nn); fi = nn * k * std::pow(li - lj, nn - 1); fj = -fi; break;
+      case 4: e = (li > l0) ? -k : -k * (1 - (li - l0) * (li - l0) / (l0 * l0)); fi = (li > l0) ? 0 : 2 * k * (li - l0) / (l0 * l0); break;
+      case 5: e = -k * li; fi = -k; break;
+      case 7: e = k * li * (1 - li) / (li + l0); fi = k * (l0 * (1 + l0) / ((li + l0) * (li + l0)) - 1); break;
+      case 9: e = k * lj * (1 - std::pow((li + l0) / l0, nn)); fi = -k * lj * std::pow((li + l0) / l0, nn - 1) / l0; fj = k * (1 - std::pow((li + l0) / l0, nn)); break;
+      case 11: e = k * std::pow(li, l0) * std::pow(lj, nn); fi = k * l0 * std::pow(li, l0 - 1) * std::pow(lj, nn); fj = k * nn * std::pow(li, l0) * std::pow(lj, nn - 1); break;
+      case 12: e = k * li * std::pow(lj, nn) / (li + l0); fi = k * l0 * std::pow(lj, nn) / ((li + l0) * (li + l0)); fj = k * nn * li * std::pow(lj, nn - 1) / (li + l0); break;
+      default: break;
+    }
+    dudl[i] += fi;
+    dudl[j] += fj;
+    energy[BM_ELAMBDA] += e;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Holonomic constraints in FP64.
+// ---------------------------------------------------------------------
+
+// Newton-Raphson SHAKE for a star or triangle group (holonomic.cu:430-600
+// iteration, generalised to any group of <=3 constraints).  Atoms are
+// pre-shifted to the minimum image of atom 0.  Returns iterations used.
+struct Group {
+  int na, nc;
+  int a[4];
+  int ca[3], cb[3];
+  double b0[3];
+};
+
+void solve3(int nc, double A[3][3], double *b, double *x) {
+  // Gaussian elimination with partial pivoting, nc <= 3
+  double M[3][4];
+  for (int i = 0; i < nc; i++) {
+    for (int j = 0; j < nc; j++) M[i][j] = A[i][j];
+    M[i][nc] = b[i];
+  }
+  for (int c = 0; c < nc; c++) {
+    int p = c;
+    for (int r = c + 1; r < nc; r++)
+      if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+    if (p != c)
+      for (int j = 0; j <= nc; j++) std::swap(M[c][j], M[p][j]);
+    for (int r = c + 1; r < nc; r++) {
+      double f = M[r][c] / M[c][c];
+      for (int j = c; j <= nc; j++) M[r][j] -= f * M[c][j];
+    }
+  }
+  for (int r = nc - 1; r >= 0; r--) {
+    double s = M[r][nc];
+    for (int j = r + 1; j < nc; j++) s -= M[r][j] * x[j];
+    x[r] = s / M[r][r];
+  }
+}
+
+int shake_group(const Group &g, Vec3 *x, const Vec3 *xp, const double *im, double tol, int maxit) {
+  Vec3 gp[3];
+  for (int c = 0; c < g.nc; c++) gp[c] = xp[g.ca[c]] - xp[g.cb[c]];
+  int it = 0;
+  for (;;) {
+    double res[3];
+    bool done = true;
+    for (int c = 0; c < g.nc; c++) {
+      Vec3 d = x[g.ca[c]] - x[g.cb[c]];
+      res[c] = g.b0[c] * g.b0[c] - dot(d, d);
+      if (std::fabs(res[c]) >= 2 * g.b0[c] * g.b0[c] * tol) done = false;
+    }
+    if (done || it >= maxit) break;
+    // d(res_c)/d(lambda_e): x_a += lambda_e im_a gp_e (a=ca_e), x_b -= lambda_e im_b gp_e
+    double A[3][3];
+    for (int c = 0; c < g.nc; c++) {
+      Vec3 d = x[g.ca[c]] - x[g.cb[c]];
+      for (int e = 0; e < g.nc; e++) {
+        double coef = 0;
+        if (g.ca[c] == g.ca[e]) coef += im[g.ca[c]];
+        if (g.ca[c] == g.cb[e]) coef -= im[g.ca[c]];
+        if (g.cb[c] == g.ca[e]) coef -= im[g.cb[c]];
+        if (g.cb[c] == g.cb[e]) coef += im[g.cb[c]];
+        A[c][e] = 2 * coef * dot(d, gp[e]);
+      }
+    }
+    double lamv[3];
+    solve3(g.nc, A, res, lamv);
+    for (int e = 0; e < g.nc; e++) {
+      x[g.ca[e]] = x[g.ca[e]] + (lamv[e] * im[g.ca[e]]) * gp[e];
+      x[g.cb[e]] = x[g.cb[e]] + (-lamv[e] * im[g.cb[e]]) * gp[e];
+    }
+    it++;
+  }
+  return it;
+}
+
+// Analytic SETTLE for an isosceles rigid triangle (atom 0 apex, equal
+// masses and lengths to atoms 1 and 2): rigid-body solution of the SHAKE
+// equations (Miyamoto & Kollman 1992), the role of the reference's
+// triangle kernel (holonomic.cu:253-377).
+void settle(const double *b0, Vec3 *x, const Vec3 *xp, const double *mass) {
+  double mA = mass[0], mB = mass[1], M = mA + 2 * mB;
+  double dAB = b0[0], dBC = b0[2];
+  double rc = 0.5 * dBC;
+  double h = std::sqrt(dAB * dAB - rc * rc);
+  double ra = 2 * mB * h / M, rb = h - ra;
+  Vec3 d = (1.0 / M) * (mA * x[0] + mB * x[1] + mB * x[2]);
+  Vec3 a1 = x[0] - d, b1 = x[1] - d, c1 = x[2] - d;
+  Vec3 p[3] = {v3(0, 0, 0), xp[1] - xp[0], xp[2] - xp[0]};
+  Vec3 ez = cross(p[1], p[2]);
+  ez = (1.0 / norm(ez)) * ez;
+  Vec3 ex = cross(a1, ez);
+  ex = (1.0 / norm(ex)) * ex;
+  Vec3 ey = cross(ez, ex);
+  double az = dot(a1, ez), bz = dot(b1, ez), cz = dot(c1, ez);
+  double sphi = az / ra, cphi = std::sqrt(1 - sphi * sphi);
+  double spsi = (bz - cz) / (2 * rc * cphi), cpsi = std::sqrt(1 - spsi * spsi);
+  // canonical triangle tilted by psi (about y) then phi (about x)
+  double qx[3] = {0, -rc * cpsi, rc * cpsi};
+  double qy[3] = {ra * cphi, -rb * cphi - rc * spsi * sphi, -rb * cphi + rc * spsi * sphi};
+  double qz[3] = {az, bz, cz};
+  double m[3] = {mA, mB, mB};
+  Vec3 u[3] = {a1, b1, c1};
+  double al = 0, be = 0, ga = 0;
+  for (int i = 0; i < 3; i++) {
+    double px = dot(p[i], ex), py = dot(p[i], ey);
+    double ux = dot(u[i], ex), uy = dot(u[i], ey);
+    al += m[i] * (px * qx[i] + py * qy[i]);
+    be += m[i] * (px * qy[i] - py * qx[i]);
+    ga += m[i] * (px * uy - py * ux);
+  }
+  double a2b2 = al * al + be * be;
+  double sgn = al > 0 ? 1 : -1;
+  double sth = (al * ga - sgn * be * std::sqrt(a2b2 - ga * ga)) / a2b2;
+  double cth = std::sqrt(1 - sth * sth);
+  for (int i = 0; i < 3; i++) {
+    double fx = qx[i] * cth - qy[i] * sth, fy = qx[i] * sth + qy[i] * cth;
+    x[i] = d + fx * ex + fy * ey + qz[i] * ez;
+  }
+}
+
+template <class F>
+void for_groups(const bm_topology *t, F fn) {
+  for (int c = 0; c < t->n_tri; c++) {
+    Group g{3, 3, {t->tri_idx[3 * c], t->tri_idx[3 * c + 1], t->tri_idx[3 * c + 2], -1},
+            {0, 0, 1}, {1, 2, 2}, {t->tri_b0[3 * c], t->tri_b0[3 * c + 1], t->tri_b0[3 * c + 2]}};
+    fn(g, 0);
+  }
+  for (int c = 0; c < t->n_br1; c++) {
+    Group g{2, 1, {t->br1_idx[2 * c], t->br1_idx[2 * c + 1], -1, -1}, {0}, {1}, {t->br1_b0[c]}};
+    fn(g, 1);
+  }
+  for (int c = 0; c < t->n_br2; c++) {
+    Group g{3, 2, {t->br2_idx[3 * c], t->br2_idx[3 * c + 1], t->br2_idx[3 * c + 2], -1},
+            {0, 0}, {1, 2}, {t->br2_b0[2 * c], t->br2_b0[2 * c + 1]}};
+    fn(g, 2);
+  }
+  for (int c = 0; c < t->n_br3; c++) {
+    Group g{4, 3, {t->br3_idx[4 * c], t->br3_idx[4 * c + 1], t->br3_idx[4 * c + 2], t->br3_idx[4 * c + 3]},
+            {0, 0, 0}, {1, 2, 3}, {t->br3_b0[3 * c], t->br3_b0[3 * c + 1], t->br3_b0[3 * c + 2]}};
+    fn(g, 3);
+  }
+}
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+extern "C" {
+
+int orc_threads(void) { return omp_get_max_threads(); }
+
+long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
+                const double *lambda, unsigned terms, double *force, double *dudl, double *energy) {
+  int n = t->n_atoms;
+  std::fill(force, force + 3 * n, 0.0);
+  std::fill(dudl, dudl + t->n_blocks, 0.0);
+  std::fill(energy, energy + BM_NTERMS, 0.0);
+  long pairs = 0;
+  if (terms & ORC_NBDIRECT) pairs = nbdirect(t, rp, pos, lambda, force, dudl, energy);
+  if (rp->elec_method == BM_PME) {
+    if (terms & ORC_EWALDSELF) ewaldself(t, rp, lambda, dudl, energy);
+    if (terms & ORC_PME) pme(t, rp, pos, lambda, force, dudl, energy);
+    if (terms & ORC_NBEX) nbex(t, rp, pos, lambda, force, dudl, energy);
+  }
+  if (terms & ORC_BONDED) bonded(t, rp, pos, lambda, force, dudl, energy);
+  if (terms & ORC_NB14) nb14(t, rp, pos, lambda, force, dudl, energy);
+  if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1) msldbias(t, lambda, dudl, energy);
+  double pot = 0;
+  for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
+  energy[BM_EPOTENTIAL] = pot;
+  return pairs;
+}
+
+double orc_ewald_recip_direct(const bm_topology *t, const bm_runparams *rp, const double *pos,
+                              const double *lambda, int kmax, double *force, double *dudl) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  double V = L[0] * L[1] * L[2], beta = rp->beta_ewald;
+  std::vector<double> lq(n);
+  for (int i = 0; i < n; i++) {
+    int b = blockOf(t->site_block[i]);
+    lq[i] = (b ? lambda[b] : 1.0) * t->charge[i];
+  }
+  if (force) std::fill(force, force + 3 * n, 0.0);
+  double etot = 0;
+  int nthr = omp_get_max_threads();
+  std::vector<double> fT((size_t)nthr * 3 * n, 0.0), dT((size_t)nthr * t->n_blocks, 0.0);
+  int span = 2 * kmax + 1;
+#pragma omp parallel for reduction(+ : etot) schedule(dynamic, 4)
+  for (int idx = 0; idx < span * span * span; idx++) {
+    int mx = idx / (span * span) - kmax, my = (idx / span) % span - kmax, mz = idx % span - kmax;
+    if (!mx && !my && !mz) continue;
+    double kx = mx / L[0], ky = my / L[1], kz = mz / L[2];
+    double m2 = kx * kx + ky * ky + kz * kz;
+    double g = kElectric / (2 * M_PI * V) * std::exp(-M_PI * M_PI * m2 / (beta * beta)) / m2;
+    double C = 0, S = 0;
+    for (int i = 0; i < n; i++) {
+      double th = 2 * M_PI * (kx * pos[i] + ky * pos[n + i] + kz * pos[2 * n + i]);
+      C += lq[i] * std::cos(th);
+      S += lq[i] * std::sin(th);
+    }
+    etot += g * (C * C + S * S);
+    double *f = &fT[(size_t)omp_get_thread_num() * 3 * n];
+    double *dd = &dT[(size_t)omp_get_thread_num() * t->n_blocks];
+    for (int i = 0; i < n; i++) {
+      double th = 2 * M_PI * (kx * pos[i] + ky * pos[n + i] + kz * pos[2 * n + i]);
+      double cs = std::cos(th), sn = std::sin(th);
+      double w = g * 4 * M_PI * lq[i] * (S * cs - C * sn);
+      f[i] += w * kx;
+      f[n + i] += w * ky;
+      f[2 * n + i] += w * kz;
+      int b = blockOf(t->site_block[i]);
+      if (b) dd[b] += g * 2 * t->charge[i] * (C * cs + S * sn);
+    }
+  }
+  for (int th = 0; th < nthr; th++) {
+    if (force)
+      for (int i = 0; i < 3 * n; i++) force[i] += fT[(size_t)th * 3 * n + i];
+    if (dudl)
+      for (int b = 0; b < t->n_blocks; b++) dudl[b] += dT[(size_t)th * t->n_blocks + b];
+  }
+  return etot;
+}
+
+double orc_ewald_real_images(int n, const double *q, const double *pos, const double *box,
+                             double beta, double rcut) {
+  int nim[3];
+  for (int d = 0; d < 3; d++) nim[d] = (int)std::ceil(rcut / box[d]) + 1;
+  double e = 0;
+#pragma omp parallel for reduction(+ : e)
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++)
+      for (int a = -nim[0]; a <= nim[0]; a++)
+        for (int b = -nim[1]; b <= nim[1]; b++)
+          for (int c = -nim[2]; c <= nim[2]; c++) {
+            if (i == j && !a && !b && !c) continue;
+            double dx = pos[i] - pos[j] + a * box[0], dy = pos[n + i] - pos[n + j] + b * box[1],
+                   dz = pos[2 * n + i] - pos[2 * n + j] + c * box[2];
+            double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (r < rcut) e += 0.5 * kElectric * q[i] * q[j] * std::erfc(beta * r) / r;
+          }
+  }
+  return e;
+}
+
+double orc_bspline(int order, double u) { return bspline(order, u); }
+void orc_bspline_moduli(int order, int K, double *out) { bsp_moduli(order, K, out); }
+
+void orc_fft3d(int kx, int ky, int kz, double *data, int sign) {
+  fft3d(kx, ky, kz, reinterpret_cast<cplx *>(data), sign);
+}
+
+void orc_domdec_columns(const bm_runparams *rp, int n_atoms, int *ncx, int *ncy) {
+  // approxBlockBox = (32 V / N)^(1/3); columns = ceil(L / approxBlockBox)
+  // (domdec/domdec.cu:110-118)
+  double V = rp->box[0] * rp->box[1] * rp->box[2];
+  double a = std::cbrt(32.0 * V / n_atoms);
+  *ncx = std::max(1, (int)std::ceil(rp->box[0] / a));
+  *ncy = std::max(1, (int)std::ceil(rp->box[1] / a));
+}
+
+long orc_domdec(const bm_topology *t, const bm_runparams *rp, double *pos, int ncx, int ncy,
+                int max_blocks, long max_tiles, int *l2g, int *bcount, int *n_blocks,
+                int *tile_i, int *tile_j, unsigned *tile_mask) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  std::vector<uint32_t> u(3 * (size_t)n);
+  // wrap into [0,L) and take 32-bit fractional coordinates
+  for (int d = 0; d < 3; d++) {
+    double invL = 1.0 / L[d];
+    for (int i = 0; i < n; i++) {
+      double x = pos[d * n + i];
+      double k = std::floor(x * invL);
+      double kl = k * L[d];
+      x = x - kl;
+      pos[d * n + i] = x;
+      double f = x * invL;
+      double s = f * 4294967296.0;
+      u[(size_t)d * n + i] = (uint32_t)(int64_t)std::floor(s);
+    }
+  }
+  int ncol = ncx * ncy;
+  std::vector<std::vector<int>> col(ncol);
+  for (int i = 0; i < n; i++) {
+    uint32_t cx = (uint32_t)(((uint64_t)u[i] * (uint64_t)ncx) >> 32);
+    uint32_t cy = (uint32_t)(((uint64_t)u[n + i] * (uint64_t)ncy) >> 32);
+    col[cx * ncy + cy].push_back(i);
+  }
+  int nb = 0;
+  for (int c = 0; c < ncol; c++) nb += ((int)col[c].size() + 31) / 32;
+  if (nb > max_blocks) return -1;
+  *n_blocks = nb;
+  std::vector<uint32_t> lo(3 * (size_t)nb), hi(3 * (size_t)nb);
+  std::vector<int> slotOf(n);
+  int b = 0;
+  for (int c = 0; c < ncol; c++) {
+    auto &v = col[c];
+    std::sort(v.begin(), v.end(), [&](int p, int q) {
+      uint32_t zp = u[2 * (size_t)n + p], zq = u[2 * (size_t)n + q];
+      return zp != zq ? zp < zq : p < q;
+    });
+    for (size_t s = 0; s < v.size(); s += 32, b++) {
+      int cnt = (int)std::min<size_t>(32, v.size() - s);
+      bcount[b] = cnt;
+      for (int l = 0; l < 32; l++) l2g[32 * b + l] = l < cnt ? v[s + l] : -1;
+      for (int d = 0; d < 3; d++) {
+        uint32_t mn = 0xFFFFFFFFu, mx = 0;
+        for (int l = 0; l < cnt; l++) {
+          uint32_t w = u[(size_t)d * n + v[s + l]];
+          mn = std::min(mn, w);
+          mx = std::max(mx, w);
+        }
+        lo[3 * b + d] = mn;
+        hi[3 * b + d] = mx;
+      }
+      for (int l = 0; l < cnt; l++) slotOf[v[s + l]] = 32 * b + l;
+    }
+  }
+  // tile list: J >= I, periodic interval gap of the boxes <= rcut + pad
+  const double rl = rp->rcut + rp->cull_pad;
+  const double rl2 = rl * rl;
+  double scale[3];
+  for (int d = 0; d < 3; d++) scale[d] = L[d] * (1.0 / 4294967296.0);
+  long nt = 0;
+  std::vector<long> firstTile(nb + 1);
+  for (int I = 0; I < nb; I++) {
+    firstTile[I] = nt;
+    for (int J = I; J < nb; J++) {
+      bool keep = (I == J);
+      if (!keep) {
+        double g[3];
+        for (int d = 0; d < 3; d++) {
+          uint64_t aL = lo[3 * I + d], aH = hi[3 * I + d], bL = lo[3 * J + d], bH = hi[3 * J + d];
+          uint64_t gap = 0;
+          const uint64_t P = 1ull << 32;
+          if (aH < bL) gap = std::min(bL - aH, aL + P - bH);
+          else if (bH < aL) gap = std::min(aL - bH, bL + P - aH);
+          g[d] = (double)gap * scale[d];
+        }
+        double gx = g[0] * g[0], gy = g[1] * g[1], gz = g[2] * g[2];
+        double s = gx + gy;
+        s = s + gz;
+        keep = s <= rl2;
+      }
+      if (keep) {
+        if (nt >= max_tiles) return -1;
+        tile_i[nt] = I;
+        tile_j[nt] = J;
+        for (int l = 0; l < 32; l++) tile_mask[32 * nt + l] = 0;
+        nt++;
+      }
+    }
+  }
+  firstTile[nb] = nt;
+  for (int e = 0; e < t->n_excl; e++) {
+    int sa = slotOf[t->excl_idx[2 * e]], sb = slotOf[t->excl_idx[2 * e + 1]];
+    if (sa / 32 > sb / 32) std::swap(sa, sb);
+    int I = sa / 32, J = sb / 32;
+    const int *beg = tile_j + firstTile[I], *end = tile_j + firstTile[I + 1];
+    const int *it = std::lower_bound(beg, end, J);
+    if (it == end || *it != J) continue;  // never within range of each other
+    long tt = firstTile[I] + (it - beg);
+    tile_mask[32 * tt + sa % 32] |= 1u << (sb % 32);
+    if (I == J) tile_mask[32 * tt + sb % 32] |= 1u << (sa % 32);
+  }
+  return nt;
+}
+
+void orc_theta_to_lambda(const bm_topology *t, const bm_runparams *rp, const double *theta,
+                         double *lambda) {
+  for (int s = 0; s < t->n_sites; s++) {
+    int b0 = t->site_bound[s], b1 = t->site_bound[s + 1];
+    double fix = 0, norm = 0;
+    for (int b = b0; b < b1; b++) {
+      bool fixed = t->block_fixed && t->block_fixed[b];
+      if (fixed) fix += theta[b];
+      else {
+        lambda[b] = std::exp(rp->fnex * std::sin(theta[b]));
+        norm += lambda[b];
+      }
+    }
+    double r = 1 - fix;
+    if (norm > 0) norm = r / norm;
+    for (int b = b0; b < b1; b++) {
+      bool fixed = t->block_fixed && t->block_fixed[b];
+      lambda[b] = fixed ? theta[b] : lambda[b] * norm;
+    }
+  }
+}
+
+void orc_chain_rule(const bm_topology *t, const bm_runparams *rp, const double *theta,
+                    const double *lambda, const double *dudl, double *dudtheta) {
+  for (int b = 0; b < t->n_blocks; b++) {
+    int s = t->block_site[b];
+    bool fixedb = t->block_fixed && t->block_fixed[b];
+    if (fixedb) { dudtheta[b] = 0; continue; }
+    double fix = 0, dyn = 0;
+    for (int c = t->site_bound[s]; c < t->site_bound[s + 1]; c++) {
+      bool fixedc = t->block_fixed && t->block_fixed[c];
+      if (fixedc) fix += lambda[c];
+      else dyn += lambda[c] * dudl[c];
+    }
+    double r = 1 - fix, f = dudl[b];
+    if (r > 0) f -= dyn / r;
+    dudtheta[b] = f * lambda[b] * rp->fnex * std::cos(theta[b]);
+  }
+}
+
+void orc_holonomic_position(const bm_topology *t, const bm_runparams *rp, double *x,
+                            const double *xprev, double *v, double halfdt) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  for_groups(t, [&](const Group &g, int kind) {
+    Vec3 xp[4], xn[4], shift[4];
+    double im[4], m[4];
+    Vec3 p0 = getp(xprev, n, g.a[0]);
+    for (int k = 0; k < g.na; k++) {
+      Vec3 pk = getp(xprev, n, g.a[k]);
+      shift[k] = k ? (minimg(pk - p0, L) - (pk - p0)) : v3(0, 0, 0);
+      xp[k] = pk + shift[k];
+      xn[k] = getp(x, n, g.a[k]) + shift[k];
+      m[k] = t->mass[g.a[k]];
+      im[k] = 1.0 / m[k];
+    }
+    bool iso = kind == 0 && m[1] == m[2] && g.b0[0] == g.b0[1];
+    if (iso) settle(g.b0, xn, xp, m);
+    else if (kind == 1) {
+      // single bond: closed form of the quadratic (holonomic.cu:397-403)
+      Vec3 gp = xp[0] - xp[1], s = xn[0] - xn[1];
+      double gg = dot(gp, gp), sg = dot(s, gp), ss = dot(s, s), b02 = g.b0[0] * g.b0[0];
+      double lw = (-sg + std::sqrt(sg * sg - gg * (ss - b02))) / (gg * (im[0] + im[1]));
+      xn[0] = xn[0] + (im[0] * lw) * gp;
+      xn[1] = xn[1] + (-im[1] * lw) * gp;
+    } else {
+      shake_group(g, xn, xp, im, rp->shake_tol, 10);
+    }
+    for (int k = 0; k < g.na; k++) {
+      int a = g.a[k];
+      Vec3 vv = (1.0 / halfdt) * (xn[k] - xp[k]);
+      v[a] = vv.x; v[n + a] = vv.y; v[2 * n + a] = vv.z;
+      Vec3 xo = xn[k] - shift[k];
+      x[a] = xo.x; x[n + a] = xo.y; x[2 * n + a] = xo.z;
+    }
+  });
+}
+
+void orc_holonomic_velocity(const bm_topology *t, const bm_runparams *rp, const double *x, double *v) {
+  const int n = t->n_atoms;
+  for_groups(t, [&](const Group &g, int) {
+    Vec3 xs[4], vs[4];
+    double im[4];
+    for (int k = 0; k < g.na; k++) {
+      xs[k] = getp(x, n, g.a[k]);
+      vs[k] = getp(v, n, g.a[k]);
+      im[k] = 1.0 / t->mass[g.a[k]];
+    }
+    Vec3 gc[3];
+    for (int c = 0; c < g.nc; c++) gc[c] = minimg(xs[g.ca[c]] - xs[g.cb[c]], rp->box);
+    double A[3][3], b[3], lamv[3];
+    for (int c = 0; c < g.nc; c++) {
+      b[c] = -dot(vs[g.ca[c]] - vs[g.cb[c]], gc[c]);
+      for (int e = 0; e < g.nc; e++) {
+        double coef = 0;
+        if (g.ca[c] == g.ca[e]) coef += im[g.ca[c]];
+        if (g.ca[c] == g.cb[e]) coef -= im[g.ca[c]];
+        if (g.cb[c] == g.ca[e]) coef -= im[g.cb[c]];
+        if (g.cb[c] == g.cb[e]) coef += im[g.cb[c]];
+        A[c][e] = coef * dot(gc[c], gc[e]);
+      }
+    }
+    solve3(g.nc, A, b, lamv);
+    for (int e = 0; e < g.nc; e++) {
+      vs[g.ca[e]] = vs[g.ca[e]] + (lamv[e] * im[g.ca[e]]) * gc[e];
+      vs[g.cb[e]] = vs[g.cb[e]] + (-lamv[e] * im[g.cb[e]]) * gc[e];
+    }
+    for (int k = 0; k < g.na; k++) {
+      int a = g.a[k];
+      v[a] = vs[k].x; v[n + a] = vs[k].y; v[2 * n + a] = vs[k].z;
+    }
+  });
+}
+
+void orc_philox4x32(const unsigned ctr[4], const unsigned key[2], unsigned out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0, h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+    c0 = h1 ^ c1 ^ k0;
+    c1 = l1;
+    c2 = h0 ^ c3 ^ k1;
+    c3 = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Normal deviate for (seed, step, dof): Philox block on counter
+// (dof/2, step_lo, step_hi, 0x4E524D4C), Box-Muller on two 53-bit uniforms.
+double orc_normal(uint64_t seed, uint64_t step, uint64_t dof) {
+  unsigned ctr[4] = {(unsigned)(dof >> 1), (unsigned)step, (unsigned)(step >> 32), 0x4E524D4Cu};
+  unsigned key[2] = {(unsigned)seed, (unsigned)(seed >> 32)};
+  unsigned o[4];
+  orc_philox4x32(ctr, key, o);
+  double u1 = ((double)((((uint64_t)o[0]) << 21) ^ (o[1] >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+  double u2 = ((double)((((uint64_t)o[2]) << 21) ^ (o[3] >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+  double rad = std::sqrt(-2.0 * std::log(u1));
+  double ang = 2.0 * M_PI * u2;
+  return (dof & 1) ? rad * std::sin(ang) : rad * std::cos(ang);
+}
+
+int orc_step(const bm_topology *t, const bm_runparams *rp, long step, int energy_step,
+             double *x, double *v, double *lambda, double *energy) {
+  const int n = t->n_atoms, nl = t->n_blocks, N1 = 3 * n, N = 3 * n + nl;
+  std::vector<double> f(N, 0.0), dudl(nl, 0.0), dudth(nl, 0.0), ism(N), xb(N);
+  orc_forces(t, rp, x, lambda, ORC_ALL, f.data(), dudl.data(), energy);
+  if (nl > 1) {
+    orc_chain_rule(t, rp, x + N1, lambda, dudl.data(), dudth.data());
+    for (int b = 0; b < nl; b++) f[N1 + b] = dudth[b];
+  }
+  for (int i = 0; i < n; i++) ism[i] = ism[n + i] = ism[2 * n + i] = 1 / std::sqrt(t->mass[i]);
+  for (int b = 0; b < nl; b++) ism[N1 + b] = t->theta_mass[b] > 0 ? 1 / std::sqrt(t->theta_mass[b]) : 0;
+  const double kT = kBoltz * rp->temperature, halfdt = 0.5 * rp->dt;
+  double a = std::exp(-rp->gamma * rp->dt), noise = std::sqrt((1 - a * a) * kT);
+  double al = std::exp(-rp->msld_gamma * rp->dt), noisel = std::sqrt((1 - al * al) * kT);
+  bool cons = t->n_tri + t->n_br1 + t->n_br2 + t->n_br3 > 0;
+  if (energy_step) {
+    for (int i = 0; i < N; i++)
+      if (ism[i] > 0) v[i] -= halfdt * ism[i] * ism[i] * f[i];
+    if (cons) orc_holonomic_velocity(t, rp, x, v);
+    double ke = 0;
+    for (int i = 0; i < N; i++)
+      if (ism[i] > 0) ke += 0.5 * (v[i] / ism[i]) * (v[i] / ism[i]);
+    energy[BM_EKINETIC] = ke;
+    for (int i = 0; i < N; i++)
+      if (ism[i] > 0) v[i] -= halfdt * ism[i] * ism[i] * f[i];
+  } else {
+    for (int i = 0; i < N; i++)
+      if (ism[i] > 0) v[i] -= 2 * halfdt * ism[i] * ism[i] * f[i];
+  }
+  for (int i = 0; i < N; i++) {
+    xb[i] = x[i];
+    if (ism[i] > 0) x[i] += halfdt * v[i];
+  }
+  if (cons) orc_holonomic_position(t, rp, x, xb.data(), v, halfdt);
+  for (int i = 0; i < N; i++) {
+    double fr = a, no = noise;
+    if (i >= N1) {
+      int b = i - N1;
+      if (t->theta_friction) {
+        double ab = std::exp(-t->theta_friction[b] * rp->dt);
+        fr = ab;
+        no = std::sqrt((1 - ab * ab) * kT);
+      } else {
+        fr = al;
+        no = noisel;
+      }
+    }
+    xb[i] = x[i];
+    if (ism[i] > 0) {
+      v[i] = fr * v[i] + no * ism[i] * orc_normal(rp->seed, (uint64_t)step, (uint64_t)i);
+      x[i] += halfdt * v[i];
+    }
+  }
+  if (cons) orc_holonomic_position(t, rp, x, xb.data(), v, halfdt);
+  if (nl > 1) orc_theta_to_lambda(t, rp, x + N1, lambda);
+  for (int i = 0; i < N; i++)
+    if (!std::isfinite(v[i]) || !std::isfinite(x[i])) return -1;
+  return 0;
+}
+
+}  // extern "C"
