@@ -136,6 +136,7 @@ def lib():
         "blade_get_device_system": (vp, [vp]),
         "blade_prepare": (i, [vp]),
         "blade_set_seed": (None, [vp, C.c_uint64]),
+        "blade_set_domdec_frequency": (None, [vp, i]),
         # kernel layer
         "bm_version": (s, []),
         "bm_last_error": (s, []),

@@ -35,6 +35,7 @@ struct System {
   bm_system *gpu = nullptr;
   long step = 0;
   unsigned long long seed = 0xB1ADE000ull;
+  int freqDomdec = 10;
 };
 
 extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(stderr, "%s", message); }
@@ -232,6 +233,7 @@ void blade_add_run_dynopts(System *s, int step, int step0, int nsteps, double dt
 }
 
 void blade_set_seed(System *s, unsigned long long seed) { s->seed = seed; }
+void blade_set_domdec_frequency(System *s, int steps) { s->freqDomdec = steps; }
 
 int blade_prepare(System *s) {
   if (!s || s->failed) return BM_EINVAL;
@@ -246,6 +248,7 @@ int blade_prepare(System *s) {
     else m.atomBlock.resize(n, 0);
   }
   s->run->seed = s->seed;
+  s->run->freqDomdec = s->freqDomdec;
   try {
     build_topology(*s->parameters, *s->structure, m, *s->run, s->topo);
   } catch (const SetupError &e) {

@@ -27,6 +27,24 @@ __device__ __forceinline__ F3 disp(const Dev &d, int i, int j) {
   r.z = fdisp(frac_u32(d.pos[2 * d.n + i], d.invbox[2]), frac_u32(d.pos[2 * d.n + j], d.invbox[2]), d.fracscale[2]);
   return r;
 }
+struct D3v { double x, y, z; };
+// FP64 minimum-image displacement r_i - r_j
+__device__ __forceinline__ D3v dispd(const Dev &d, int i, int j) {
+  D3v r;
+  const double *p = d.pos;
+  r.x = p[i] - p[j];
+  r.y = p[d.n + i] - p[d.n + j];
+  r.z = p[2 * d.n + i] - p[2 * d.n + j];
+  r.x -= d.box[0] * floor(r.x * d.invbox[0] + 0.5);
+  r.y -= d.box[1] * floor(r.y * d.invbox[1] + 0.5);
+  r.z -= d.box[2] * floor(r.z * d.invbox[2] + 0.5);
+  return r;
+}
+__device__ __forceinline__ void addforce_d(const Dev &d, int i, double s, D3v r) {
+  atomic_fix(&d.force[i], fix_d(s * r.x));
+  atomic_fix(&d.force[d.n + i], fix_d(s * r.y));
+  atomic_fix(&d.force[2 * d.n + i], fix_d(s * r.z));
+}
 __device__ __forceinline__ void addforce(const Dev &d, int i, F3 f) {
   atomic_fix_f(&d.force[i], f.x);
   atomic_fix_f(&d.force[d.n + i], f.y);
@@ -61,44 +79,44 @@ __device__ __forceinline__ float erfc_pair(float x) {
 
 // 1-4 pair energy/derivative (pair.cu:56-175); VFSWITCH energy below the
 // switch is shifted by rc^-6/rc^-12 exactly as the reference does
-__device__ void nb14_ef(const Dev &d, float4 p, float r, float &f, float &e) {
-  const float rc = d.rc, rs = d.rs;
-  f = 0.f; e = 0.f;
+__device__ void nb14_ef(const Dev &d, double4 p, double r, double &f, double &e) {
+  const double rc = d.rc, rs = d.rs;
+  f = 0.0; e = 0.0;
   if (!(r < rc)) return;
-  const float qq = p.x, c12 = p.y, c6 = p.z, e14 = p.w;
-  float rinv = 1.f / r, rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
+  const double qq = p.x, c12 = p.y, c6 = p.z, e14 = p.w;
+  double rinv = 1.0 / r, rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
   if (d.vdwmethod == BM_VFSWITCH) {
-    float rc3 = rc * rc * rc, rc6 = rc3 * rc3, rs3 = rs * rs * rs, rs6 = rs3 * rs3;
-    float k6 = rc3 / (rc3 - rs3), k12 = rc6 / (rc6 - rs6);
+    double rc3 = rc * rc * rc, rc6 = rc3 * rc3, rs3 = rs * rs * rs, rs6 = rs3 * rs3;
+    double k6 = rc3 / (rc3 - rs3), k12 = rc6 / (rc6 - rs6);
     bool in = r < rs;
-    float A6 = in ? 1.f : k6, A12 = in ? 1.f : k12, B6 = in ? 0.f : 1.f / rc3, B12 = in ? 0.f : 1.f / rc6;
+    double A6 = in ? 1.0 : k6, A12 = in ? 1.0 : k12, B6 = in ? 0.0 : 1.0 / rc3, B12 = in ? 0.0 : 1.0 / rc6;
     f = 6 * c6 * A6 * (rinv3 - B6) * rinv3 * rinv - 12 * c12 * A12 * (rinv6 - B12) * rinv6 * rinv;
-    float CC6 = in ? -1.f / rc6 : 0.f, CC12 = in ? -1.f / (rc6 * rc6) : 0.f;
+    double CC6 = in ? -1.0 / rc6 : 0.0, CC12 = in ? -1.0 / (rc6 * rc6) : 0.0;
     e = c12 * (A12 * (rinv6 - B12) * (rinv6 - B12) + CC12) - c6 * (A6 * (rinv3 - B6) * (rinv3 - B6) + CC6);
   } else if (d.vdwmethod == BM_VSWITCH) {
-    float c2of = rc * rc, c2on = rs * rs;
-    float rul3 = (c2of - c2on) * (c2of - c2on) * (c2of - c2on), rul12 = 12 / rul3;
-    float rijl = c2on - r * r, riju = c2of - r * r;
-    float fsw = (r < rs) ? 1.f : riju * riju * (riju - 3 * rijl) / rul3;
-    float dfsw = (r < rs) ? 0.f : rijl * riju * rul12;
+    double c2of = rc * rc, c2on = rs * rs;
+    double rul3 = (c2of - c2on) * (c2of - c2on) * (c2of - c2on), rul12 = 12 / rul3;
+    double rijl = c2on - r * r, riju = c2of - r * r;
+    double fsw = (r < rs) ? 1.0 : riju * riju * (riju - 3 * rijl) / rul3;
+    double dfsw = (r < rs) ? 0.0 : rijl * riju * rul12;
     f = fsw * (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv + dfsw * (c12 * rinv6 - c6) * rinv6;
     e = fsw * (c12 * rinv6 - c6) * rinv6;
   } else {
-    float r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
-    float rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
-    float rc2 = rc * rc, rc6 = rc2 * rc2 * rc2, rc12 = rc6 * rc6, rc18 = rc12 * rc6;
+    double r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
+    double rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
+    double rc2 = rc * rc, rc6 = rc2 * rc2 * rc2, rc12 = rc6 * rc6, rc18 = rc12 * rc6;
     f = 6 * c6 * (rinv7 - r5 / rc12) - 12 * c12 * (rinv13 + r5 / rc18);
     e = c12 * (rinv12 + 2 * r6 / rc18 - 3 / rc12) - c6 * (rinv6 + r6 / rc12 - 2 / rc6);
   }
-  const float kqq = kElectricF * qq;
+  const double kqq = kElectricD * qq;
   if (d.elecmethod == BM_FSWITCH) {
-    float kqqe = kqq * e14;
-    float roff2 = rc * rc, ron2 = rs * rs;
-    float ginv = 1 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
-    float A = roff2 * roff2 * (roff2 - 3 * ron2) * ginv, B = 6 * roff2 * ron2 * ginv;
-    float C = -(ron2 + roff2) * ginv, D = 2 * ginv / 5;
-    float dvc = 8 * (ron2 * roff2 * (rc - rs) - (roff2 * roff2 * rc - ron2 * ron2 * rs) / 5) * ginv;
-    float r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+    double kqqe = kqq * e14;
+    double roff2 = rc * rc, ron2 = rs * rs;
+    double ginv = 1 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
+    double A = roff2 * roff2 * (roff2 - 3 * ron2) * ginv, B = 6 * roff2 * ron2 * ginv;
+    double C = -(ron2 + roff2) * ginv, D = 2 * ginv / 5;
+    double dvc = 8 * (ron2 * roff2 * (rc - rs) - (roff2 * roff2 * rc - ron2 * ron2 * rs) / 5) * ginv;
+    double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
     if (r <= rs) {
       f += -kqqe * rinv * rinv;
       e += kqqe * (rinv + dvc);
@@ -107,12 +125,12 @@ __device__ void nb14_ef(const Dev &d, float4 p, float r, float &f, float &e) {
       e += kqqe * (A * (rinv - 1 / rc) + B * (rc - r) + C * (roff2 * rc - r3) + D * (roff2 * roff2 * rc - r5));
     }
   } else if (d.elecmethod == BM_PME) {
-    float br = d.beta * r;
-    float erfcrinv = (erfc_pair(fminf(br, BM_ERFC_XMAX)) + e14 - 1.f) * rinv;
-    f += kqq * (-erfcrinv - kTwoOverSqrtPi * d.beta * __expf(-br * br)) * rinv;
+    double br = d.beta * r;
+    double erfcrinv = (erfc(br) + e14 - 1.0) * rinv;
+    f += kqq * (-erfcrinv - 1.1283791670955126 * d.beta * exp(-br * br)) * rinv;
     e += kqq * erfcrinv;
   } else {
-    float roffinv = 1 / rc, roffinv2 = roffinv * roffinv, kqqe = kqq * e14;
+    double roffinv = 1 / rc, roffinv2 = roffinv * roffinv, kqqe = kqq * e14;
     f += -kqqe * (rinv * rinv - roffinv2);
     e += kqqe * (rinv - 2 * roffinv + r * roffinv2);
   }
@@ -124,11 +142,10 @@ struct BondedLaunch {
 
 __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool energy) {
   const int bid = blockIdx.x;
-  float e = 0.f;
+  double e = 0.0;
   int slot = -1;
   if (bid < L.b_angle) {
     const int t = (bid - L.b_bond) * blockDim.x + threadIdx.x;
-    slot = (t < d.nbond12 || bid * (int)blockDim.x < d.nbond12) ? BM_EBOND : BM_EUREY;
     if (t < d.nbond) {
       int2 ij = d.bond_i[t];
       F3 dr = disp(d, ij.x, ij.y);
@@ -144,8 +161,8 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
       e = et * s.l0 * s.l1;
       // bonds and Urey-Bradley may share a workgroup: split by index
       if (t >= d.nbond12) {
-        if (energy) atomic_fix_f(&d.energy[BM_EUREY], e);
-        e = 0.f;
+        if (energy) atomic_fix(&d.energy[BM_EUREY], fix_d(e));
+        e = 0.0;
       }
     }
     slot = BM_EBOND;
@@ -215,60 +232,85 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
   } else if (bid < L.b_nbex) {
     slot = BM_ENB14;
     const int t = (bid - L.b_nb14) * blockDim.x + threadIdx.x;
+    // 1-4 pairs in FP64: few terms whose energies are large and cancel
+    // against the PME terms, so FP32 rounding of charges would bias them
     if (t < d.nnb14) {
       int2 ij = d.nb14_i[t];
-      F3 dr = disp(d, ij.x, ij.y);
-      float r = sqrtf(dot3(dr, dr));
+      D3v dr = dispd(d, ij.x, ij.y);
+      double r = sqrt(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
       Sc s = scale2(d, d.nb14_sb[t]);
-      float reff = r, dredr = 1.f, dredll = 0.f;
+      const double ll = (s.b0 ? d.lambda[s.b0] : 1.0) * (s.b1 ? d.lambda[s.b1] : 1.0);
+      double reff = r, dredr = 1.0, dredll = 0.0;
       if (d.softcore14 && s.b0) {
-        float rSoft = kSoftCoreRadius * (1.f - s.l0 * s.l1);
+        double rSoft = 4.0 * (1.0 - ll);
         if (r < rSoft) {
-          float q = r / rSoft;
-          float re = 1.f - 0.5f * q;
-          re = re * q * q * q + 0.5f;
-          dredr = (3.f - 2.f * q) * q * q;
-          dredll = (re - dredr * q) * (-kSoftCoreRadius);
+          double q = r / rSoft;
+          double re = 1.0 - 0.5 * q;
+          re = re * q * q * q + 0.5;
+          dredr = (3.0 - 2.0 * q) * q * q;
+          dredll = (re - dredr * q) * (-4.0);
           reff = re * rSoft;
         }
       }
-      float f, et;
+      double f, et;
       nb14_ef(d, d.nb14_p[t], reff, f, et);
-      f *= s.l0 * s.l1;
-      scale_dudl(d, s, et + f * dredll);
+      f *= ll;
+      if (s.b0) {
+        const double g = et + f * dredll;
+        atomic_fix(&d.dudl[s.b0], fix_d((s.b1 ? d.lambda[s.b1] : 1.0) * g));
+        if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(d.lambda[s.b0] * g));
+      }
       f *= dredr;
-      addforce(d, ij.x, (f / r) * dr);
-      addforce(d, ij.y, (-f / r) * dr);
-      e = et * s.l0 * s.l1;
+      addforce_d(d, ij.x, (f / r), dr);
+      addforce_d(d, ij.y, (-f / r), dr);
+      e = et * ll;
     }
   } else if (bid < L.b_bias) {
     slot = BM_ENBRECIPEXCL;
     const int t = (bid - L.b_nbex) * blockDim.x + threadIdx.x;
+    // PME exclusion corrections in FP64 (they cancel the self term)
     if (t < d.nnbex) {
       int2 ij = d.nbex_i[t];
-      F3 dr = disp(d, ij.x, ij.y);
-      float r = sqrtf(dot3(dr, dr)), rinv = 1.f / r, br = d.beta * r;
-      float kqq = kElectricF * d.nbex_p[t];
-      // erf(br) = 1 - erfc(br); below 0.5 the series is more accurate
-      float erfv = (br < 0.5f) ? erff(br) : 1.f - erfc_pair(fminf(br, BM_ERFC_XMAX));
-      float f = kqq * (erfv * rinv - kTwoOverSqrtPi * d.beta * __expf(-br * br)) * rinv;
-      float et = -kqq * erfv * rinv;
+      D3v dr = dispd(d, ij.x, ij.y);
+      double r = sqrt(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z), rinv = 1.0 / r, br = d.beta * r;
+      double kqq = kElectricD * d.nbex_p[t];
+      double erfv = erf(br);
+      double f = kqq * (erfv * rinv - 1.1283791670955126 * d.beta * exp(-br * br)) * rinv;
+      double et = -kqq * erfv * rinv;
       Sc s = scale2(d, d.nbex_sb[t]);
-      f *= s.l0 * s.l1;
-      scale_dudl(d, s, et);
-      addforce(d, ij.x, (f / r) * dr);
-      addforce(d, ij.y, (-f / r) * dr);
-      e = et * s.l0 * s.l1;
+      const double ll = (s.b0 ? d.lambda[s.b0] : 1.0) * (s.b1 ? d.lambda[s.b1] : 1.0);
+      f *= ll;
+      if (s.b0) {
+        atomic_fix(&d.dudl[s.b0], fix_d((s.b1 ? d.lambda[s.b1] : 1.0) * et));
+        if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(d.lambda[s.b0] * et));
+      }
+      addforce_d(d, ij.x, (f / r), dr);
+      addforce_d(d, ij.y, (-f / r), dr);
+      e = et * ll;
     }
   } else {
     // MSLD fixed and variable biases: one thread per block / bias
     slot = BM_ELAMBDA;
     const int t = threadIdx.x;
-    for (int b = t; b < d.nl; b += blockDim.x) {
-      double bias = d.fixed_bias[b];
-      if (bias != 0.0) atomic_fix(&d.dudl[b], fix_d(bias));
-      e += (float)(bias * d.lambda[b]);
+    if (d.elecmethod == BM_PME && t == 0) {
+      // Ewald self term per block (nbrecip.cu:19-50): -beta kE/sqrt(pi)
+      // lambda_b^2 sum_{i in b} q_i^2, in FP64 from the block sums
+      const double pref = -(double)d.beta * kElectricD / 1.7724538509055159;
+      double es = 0.0;
+      for (int b = 0; b < d.nl; b++) {
+        const double l = b ? d.lambda[b] : 1.0;
+        const double eb = pref * d.blockq2[b];
+        if (b && eb != 0.0) atomic_fix(&d.dudl[b], fix_d(2.0 * l * eb));
+        es += eb * l * l;
+      }
+      if (energy) atomic_fix(&d.energy[BM_ENBRECIPSELF], fix_d(es));
     }
+    if (d.nl > 1)
+      for (int b = t; b < d.nl; b += blockDim.x) {
+        double bias = d.fixed_bias[b];
+        if (bias != 0.0) atomic_fix(&d.dudl[b], fix_d(bias));
+        e += bias * d.lambda[b];
+      }
     for (int v = t; v < d.nvb; v += blockDim.x) {
       int4 vi = d.vb_i[v];
       double2 vp = d.vb_p[v];
@@ -292,10 +334,10 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
       }
       if (fi != 0.0) atomic_fix(&d.dudl[i], fix_d(fi));
       if (fj != 0.0) atomic_fix(&d.dudl[j], fix_d(fj));
-      e += (float)ev;
+      e += ev;
     }
   }
-  if (energy) block_energy(&d.energy[slot], e);
+  if (energy) wave_atomic_fix(&d.energy[slot], fix_d(e));
 }
 
 void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st) {
@@ -308,8 +350,9 @@ void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st) {
   L.b_nb14 = L.b_impr + nb(d.nimpr);
   L.b_nbex = L.b_nb14 + nb(d.nnb14);
   L.b_bias = L.b_nbex + nb(d.nnbex);
-  int total = L.b_bias + (bias ? 1 : 0);
-  if (total > 0) bonded_kernel<<<total, 256, 0, st>>>(d, L, energy);
+  // the last workgroup carries the Ewald self term and the lambda biases
+  (void)bias;
+  bonded_kernel<<<L.b_bias + 1, 256, 0, st>>>(d, L, energy);
 }
 
 }  // namespace blade

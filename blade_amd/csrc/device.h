@@ -44,8 +44,12 @@ struct Dev {
   float *q;                      // [N]
   int *type;                     // [N]
   int *sb;                       // [N] siteBlock code
+  double *qd;                    // [N] charge, FP64 (energy sums, spread)
   float *qlam;                   // [N] scratch: lambda*q (PME)
   float2 *vdw;                   // [T*T] (c12, c6)
+  double2 *vdwd;                 // [T*T] (c12, c6) FP64 for energy sums
+  double *blockq2;               // [nL] sum of q^2 over each block (Ewald self term)
+  float *slo;                    // [32*maxB] slot charge low part: q - (float)q
   int ntypes;
   // domdec
   int ncx, ncy, ncol, maxblocks;
@@ -77,6 +81,7 @@ struct Dev {
   // nonbonded parameters
   float rc, rs, beta;
   float rc2;
+  double rcd, rsd, betad;        // FP64 copies for energy-step sums
   int vdwmethod, elecmethod;
   // PME
   int K[3], order;
@@ -86,14 +91,16 @@ struct Dev {
   float *bmod[3];                // [K] B-spline moduli per axis
   float pmefac;                  // 0.5 kE/(pi V)
   float pmeexp;                  // pi^2/beta^2
+  double *bmodd[3];              // FP64 moduli (energy sum)
+  double pmefacd, pmeexpd;
   // bonded terms (float parameters, int indices)
   int nbond, nbond12, nangle, ndihe, nimpr, nnb14, nnbex;
   int2 *bond_i; int2 *bond_sb; float2 *bond_p;            // (kb, b0)
   int4 *angle_i; int2 *angle_sb; float2 *angle_p;         // (k, t0)
   int4 *dihe_i; int2 *dihe_sb; float4 *dihe_p;            // (k, n, d0)
   int4 *impr_i; int2 *impr_sb; float4 *impr_p;            // (k, n, d0)
-  int2 *nb14_i; int2 *nb14_sb; float4 *nb14_p;            // (qq, c12, c6, e14fac)
-  int2 *nbex_i; int2 *nbex_sb; float *nbex_p;             // qq
+  int2 *nb14_i; int2 *nb14_sb; double4 *nb14_p;           // (qq, c12, c6, e14fac)
+  int2 *nbex_i; int2 *nbex_sb; double *nbex_p;            // qq
   // constraints
   int ntri, nbr1, nbr2, nbr3;
   int4 *tri_i; double4 *tri_b;   // b0 (01,02,12), .w unused

@@ -261,9 +261,11 @@ __global__ void __launch_bounds__(256) dd_pack(Dev d) {
     for (int k = 0; k < 3; k++) u[k] = frac_u32(d.pos[k * d.n + a], d.invbox[k]);
     d.spos[s] = make_uint4(u[0], u[1], u[2], __float_as_uint(d.q[a]));
     d.spar[s] = make_int2(d.type[a], d.sb[a]);
+    d.slo[s] = (float)(d.qd[a] - (double)d.q[a]);
   } else if (in) {
     d.spos[s] = make_uint4(0, 0, 0, 0);
     d.spar[s] = make_int2(-1, 0);
+    d.slo[s] = 0.f;
   }
   // box of the 32-slot block (a half wave); slot 0 of a block is occupied
   unsigned c[3], h[3];

@@ -65,56 +65,112 @@ __device__ __forceinline__ NbConst make_const(const Dev &d) {
   return c;
 }
 
-// Pair energy e and dE/dr f at distance r (1/r = rinv); qq already x kE.
+// One pair at distance r (1/r = rinv); qq already x kE.  f = dE/dr.  The
+// energy is returned factored, E = qq*eq + c12*eA - c6*eB, so an energy
+// sum can apply the parameters in FP64 (the factors carry only the
+// distance-dependent FP32 rounding, which does not bias the sum).
 template <int ELEC, int VDW>
 __device__ __forceinline__ void pair_ef(const NbConst &c, float qq, float c12, float c6, float r,
-                                        float rinv, float &f, float &e) {
+                                        float rinv, float &f, float &eq, float &eA, float &eB) {
   if (ELEC == BM_PME) {
     float br = c.beta * r;
     float ex = exp2f(-c.beta2log2e * r * r);
     float erfcrinv = erfc_fit(fminf(br, BM_ERFC_XMAX), ex) * rinv;
     f = -qq * (erfcrinv + kTwoOverSqrtPi * c.beta * ex) * rinv;
-    e = qq * erfcrinv;
+    eq = erfcrinv;
   } else if (ELEC == BM_FSWITCH) {
     if (r <= c.rs) {
       f = -qq * rinv * rinv;
-      e = qq * (rinv + c.dvc);
+      eq = rinv + c.dvc;
     } else {
       float r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
       f = -qq * rinv * (c.A * rinv + c.B * r + 3 * c.C * r3 + 5 * c.D * r5);
-      e = qq * (c.A * (rinv - c.rcinv) + c.B * (c.rc - r) + c.C * (c.c2of * c.rc - r3) +
-                c.D * (c.c2of * c.c2of * c.rc - r5));
+      eq = c.A * (rinv - c.rcinv) + c.B * (c.rc - r) + c.C * (c.c2of * c.rc - r3) +
+           c.D * (c.c2of * c.c2of * c.rc - r5);
     }
   } else {
     f = -qq * (rinv * rinv - c.rcinv * c.rcinv);
-    e = qq * (rinv - 2 * c.rcinv + r * c.rcinv * c.rcinv);
+    eq = rinv - 2 * c.rcinv + r * c.rcinv * c.rcinv;
   }
   float rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
   if (VDW == BM_VFSWITCH) {
     if (r < c.rs) {
       f += (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv;
-      e += c12 * (rinv6 * rinv6 - c.dv6 * c.dv6) - c6 * (rinv6 - c.dv6);
+      eA = rinv6 * rinv6 - c.dv6 * c.dv6;
+      eB = rinv6 - c.dv6;
     } else {
       float a3 = rinv3 - c.rcinv3, a6 = rinv6 - c.rcinv6;
       f += (6 * c6 * c.k6 * a3 * rinv3 - 12 * c12 * c.k12 * a6 * rinv6) * rinv;
-      e += c12 * c.k12 * a6 * a6 - c6 * c.k6 * a3 * a3;
+      eA = c.k12 * a6 * a6;
+      eB = c.k6 * a3 * a3;
     }
   } else if (VDW == BM_VSWITCH) {
     if (r < c.rs) {
       f += (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv;
-      e += c12 * rinv6 * rinv6 - c6 * rinv6;
+      eA = rinv6 * rinv6;
+      eB = rinv6;
     } else {
       float rijl = c.c2on - r * r, riju = c.c2of - r * r;
       float fsw = riju * riju * (riju - 3 * rijl) * c.rul3inv, dfsw = rijl * riju * c.rul12;
       f += fsw * (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv + dfsw * (c12 * rinv6 - c6) * rinv6;
-      e += fsw * (c12 * rinv6 - c6) * rinv6;
+      eA = fsw * rinv6 * rinv6;
+      eB = fsw * rinv6;
     }
   } else {
     float r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
     float rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
     f += 6 * c6 * (rinv7 - r5 * c.rcinv12) - 12 * c12 * (rinv13 + r5 * c.rcinv18);
-    e += c12 * (rinv12 + 2 * r6 * c.rcinv18 - 3 * c.rcinv12) - c6 * (rinv6 + r6 * c.rcinv12 - 2 * c.rcinv6);
+    eA = rinv12 + 2 * r6 * c.rcinv18 - 3 * c.rcinv12;
+    eB = rinv6 + r6 * c.rcinv12 - 2 * c.rcinv6;
   }
+}
+
+// FP64 pair energy at effective distance r (energy steps only): the total
+// potential is a small difference of large pair sums, so the sum is built
+// from FP64 pair energies (same pair set and parameters as the forces).
+template <int ELEC, int VDW>
+__device__ double pair_energy_d(const Dev &d, double qq, double c12, double c6, double r) {
+  const double rc = d.rcd, rs = d.rsd, rinv = 1.0 / r;
+  double e;
+  if (ELEC == BM_PME) {
+    e = qq * erfc(d.betad * r) * rinv;
+  } else if (ELEC == BM_FSWITCH) {
+    const double roff2 = rc * rc, ron2 = rs * rs;
+    const double ginv = 1 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
+    const double A = roff2 * roff2 * (roff2 - 3 * ron2) * ginv, B = 6 * roff2 * ron2 * ginv;
+    const double C = -(ron2 + roff2) * ginv, D = 2 * ginv / 5;
+    const double dvc = 8 * (ron2 * roff2 * (rc - rs) - (roff2 * roff2 * rc - ron2 * ron2 * rs) / 5) * ginv;
+    const double r3 = r * r * r, r5 = r3 * r * r;
+    e = (r <= rs) ? qq * (rinv + dvc)
+                  : qq * (A * (rinv - 1 / rc) + B * (rc - r) + C * (roff2 * rc - r3) + D * (roff2 * roff2 * rc - r5));
+  } else {
+    e = qq * (rinv - 2 / rc + r / (rc * rc));
+  }
+  const double rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
+  if (VDW == BM_VFSWITCH) {
+    const double rc3 = rc * rc * rc, rs3 = rs * rs * rs;
+    if (r < rs) {
+      const double dv6 = 1 / (rc3 * rs3);
+      e += c12 * (rinv6 * rinv6 - dv6 * dv6) - c6 * (rinv6 - dv6);
+    } else {
+      const double k6 = rc3 / (rc3 - rs3), k12 = rc3 * rc3 / (rc3 * rc3 - rs3 * rs3);
+      const double a3 = rinv3 - 1 / rc3, a6 = rinv6 - 1 / (rc3 * rc3);
+      e += c12 * k12 * a6 * a6 - c6 * k6 * a3 * a3;
+    }
+  } else if (VDW == BM_VSWITCH) {
+    if (r < rs) {
+      e += c12 * rinv6 * rinv6 - c6 * rinv6;
+    } else {
+      const double c2of = rc * rc, c2on = rs * rs;
+      const double rul3 = (c2of - c2on) * (c2of - c2on) * (c2of - c2on);
+      const double rijl = c2on - r * r, riju = c2of - r * r;
+      e += riju * riju * (riju - 3 * rijl) / rul3 * (c12 * rinv6 - c6) * rinv6;
+    }
+  } else {
+    const double r6 = r * r * r * r * r * r, rc6 = rc * rc * rc * rc * rc * rc;
+    e += c12 * (rinv6 * rinv6 + 2 * r6 / (rc6 * rc6 * rc6) - 3 / (rc6 * rc6)) - c6 * (rinv6 + r6 / (rc6 * rc6) - 2 / rc6);
+  }
+  return e;
 }
 
 // cull test between current block boxes (centre/half extent, fractional
@@ -145,6 +201,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   __shared__ uint4 s_pos[4][2][kBlockAtoms];
   __shared__ int2 s_par[4][2][kBlockAtoms];
   __shared__ float4 s_f[4][2][kBlockAtoms];
+  __shared__ float s_lo[4][2][kBlockAtoms];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int half = lane >> 5, li = lane & 31;
   const NbConst c = make_const(d);
@@ -155,6 +212,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   uint4 *sp = s_pos[w][half];
   int2 *spr = s_par[w][half];
   float4 *sf = s_f[w][half];
+  float *slo = s_lo[w][half];
 
   for (;;) {
     int chunk = 0;
@@ -171,6 +229,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
     uint4 pi = make_uint4(0, 0, 0, 0);
     int ti = -1, bi = 0;
     float qi = 0.f, lami = 1.f;
+    double qid = 0.0;
     long long fix_x = 0, fix_y = 0, fix_z = 0;
     float fli = 0.f;
 
@@ -201,6 +260,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
           bi = pr.y;
           ai = ti >= 0 ? d.l2g[si] : -1;
           qi = __uint_as_float(pi.w) * kElectricF;
+          if (ENERGY) qid = kElectricD * ((double)__uint_as_float(pi.w) + (double)d.slo[si]);
           lami = 1.f;
           if (ALCH && (bi & 0xFFFF)) lami = d.lambdaf[bi & 0xFFFF];
         }
@@ -214,6 +274,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
         const int sj = J * kBlockAtoms + li;
         sp[li] = d.spos[sj];
         spr[li] = d.spar[sj];
+        if (ENERGY) slo[li] = d.slo[sj];
       }
       sf[li] = make_float4(0.f, 0.f, 0.f, 0.f);
       const unsigned emask = (ex >= 0) ? d.emask[ex * kBlockAtoms + li] : 0u;
@@ -223,7 +284,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
       float fx = 0.f, fy = 0.f, fz = 0.f;
-      float etile = 0.f;
+      double etile = 0.0;
       const bool ivalid = hv && ti >= 0;
       for (int k = 0; k < kBlockAtoms; k++) {
         const int jl = (li + k) & (kBlockAtoms - 1);
@@ -260,8 +321,9 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
               }
             }
           }
-          float f, e;
-          pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, e);
+          float f, eq, eA, eB;
+          pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, eq, eA, eB);
+          const float e = qq * eq + lj.x * eA - lj.y * eB;
           if (ALCH) f *= s;
           if (ALCH && alchPair) {
             const float fl = SOFT ? e + f * dredll : e;
@@ -273,7 +335,33 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
               atomicAdd(&sf[jl].w, lami * fl);
             }
           }
-          if (ENERGY) etile += ALCH ? s * e : e;
+          if (ENERGY) {
+            // FP64 pair energy from the exact integer displacement and FP64
+            // charges / LJ parameters
+            const double2 ljd = d.vdwd[ti * d.ntypes + prj.x];
+            const double qjd = (double)__uint_as_float(pj.w) + (double)slo[jl];
+            const double ddx = (double)(int)(pi.x - pj.x) * d.ddscale[0];
+            const double ddy = (double)(int)(pi.y - pj.y) * d.ddscale[1];
+            const double ddz = (double)(int)(pi.z - pj.z) * d.ddscale[2];
+            double rd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            double sd = 1.0;
+            if (ALCH && alchPair) {
+              const int bj = prj.y;
+              const double li_d = (bi & 0xFFFF) ? d.lambda[bi & 0xFFFF] : 1.0;
+              const double lj_d = (bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0;
+              const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
+              sd = same ? (bi == bj ? li_d : 0.0) : li_d * lj_d;
+              if (SOFT) {
+                const double rSoft = 4.0 * (1.0 - sd);
+                if (rd < rSoft) {
+                  const double q = rd / rSoft;
+                  rd = ((1.0 - 0.5 * q) * q * q * q + 0.5) * rSoft;
+                }
+              }
+            }
+            (void)eq; (void)eA; (void)eB;
+            etile += sd * pair_energy_d<ELEC, VDW>(d, qid * qjd, ljd.x, ljd.y, rd);
+          }
           float fr;
           if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
           else fr = f * rinv;
@@ -284,7 +372,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
           atomicAdd(&sf[jl].z, -gz);
         }
       }
-      if (ENERGY) eacc += (double)etile;
+      if (ENERGY) eacc += etile;
       fix_x += fix_f(fx);
       fix_y += fix_f(fy);
       fix_z += fix_f(fz);

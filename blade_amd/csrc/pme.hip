@@ -16,23 +16,23 @@ namespace blade {
 
 // B-spline weights of order N: w[t] = M_N(frac+N-1-t) for grid point u0+t,
 // dw[t] its derivative (M_{N-1}(x) - M_{N-1}(x-1)).
-template <int N>
-__device__ __forceinline__ void bspline_w(float frac, float *w, float *dw) {
-  float A[N];
-  A[0] = 1.f;
+template <int N, class T>
+__device__ __forceinline__ void bspline_w(T frac, T *w, T *dw) {
+  T A[N];
+  A[0] = T(1);
 #pragma unroll
-  for (int k = 1; k < N; k++) A[k] = 0.f;
+  for (int k = 1; k < N; k++) A[k] = T(0);
 #pragma unroll
   for (int m = 2; m <= N; m++) {
     if (m == N) {
 #pragma unroll
-      for (int k = 0; k < N; k++) dw[N - 1 - k] = A[k] - (k ? A[k - 1] : 0.f);
+      for (int k = 0; k < N; k++) dw[N - 1 - k] = A[k] - (k ? A[k - 1] : T(0));
     }
-    const float inv = 1.f / (m - 1);
+    const T inv = T(1) / T(m - 1);
 #pragma unroll
     for (int k = m - 1; k >= 0; k--) {
-      float a = (frac + k) * A[k];
-      float b = k ? (m - frac - k) * A[k - 1] : 0.f;
+      T a = (frac + T(k)) * A[k];
+      T b = k ? (T(m) - frac - T(k)) * A[k - 1] : T(0);
       A[k] = (a + b) * inv;
     }
   }
@@ -41,10 +41,11 @@ __device__ __forceinline__ void bspline_w(float frac, float *w, float *dw) {
 }
 
 // grid index and fraction from the 32-bit fractional coordinate
-__device__ __forceinline__ void grid_coord(unsigned u, int K, int &u0, float &frac) {
+template <class T>
+__device__ __forceinline__ void grid_coord(unsigned u, int K, int &u0, T &frac) {
   unsigned long long p = (unsigned long long)u * (unsigned)K;
   u0 = (int)(p >> 32);
-  frac = (float)((unsigned)(p & 0xFFFFFFFFull)) * (1.0f / 4294967296.0f);
+  frac = (T)((unsigned)(p & 0xFFFFFFFFull)) * (T(1) / T(4294967296.0));
 }
 
 template <int N>
@@ -54,23 +55,24 @@ __global__ void __launch_bounds__(256) pme_spread(Dev d) {
   if (i >= d.n) return;
   const int a = (t / N) % N, c = t % N;
   const int b = d.sb[i] & 0xFFFF;
-  const float lq = d.q[i] * (b ? d.lambdaf[b] : 1.f);
-  if (lq == 0.f) return;
+  const double lq = d.qd[i] * (b ? d.lambda[b] : 1.0);
+  if (lq == 0.0) return;
   int u0[3];
-  float fr[3];
+  double fr[3];
   for (int k = 0; k < 3; k++) grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0[k], fr[k]);
-  float wx[N], wy[N], wz[N], dw[N];
+  // FP64 charge and weights: the grid is the one input of the energy sum
+  double wx[N], wy[N], wz[N], dw[N];
   bspline_w<N>(fr[0], wx, dw);
   bspline_w<N>(fr[1], wy, dw);
   bspline_w<N>(fr[2], wz, dw);
   int ix = u0[0] + a; if (ix >= d.K[0]) ix -= d.K[0];
   int iy = u0[1] + c; if (iy >= d.K[1]) iy -= d.K[1];
-  const float wac = lq * wx[a] * wy[c];
+  const double wac = lq * wx[a] * wy[c];
   long long *row = d.qgrid + ((long long)ix * d.K[1] + iy) * d.K[2];
 #pragma unroll
   for (int e = 0; e < N; e++) {
     int iz = u0[2] + e; if (iz >= d.K[2]) iz -= d.K[2];
-    atomic_fix_f(&row[iz], wac * wz[e]);
+    atomic_fix(&row[iz], fix_d(wac * wz[e]));
   }
 }
 
@@ -177,6 +179,7 @@ struct FftArgs {
   FftPlan plan;
   int axis;        // 0 = x, 1 = y, 2 = z
   int mode;        // 0 fwd from fixed-point grid, 1 fwd, 2 inverse, 3 fwd+conv+inv
+  int energy;      // mode 3: accumulate the reciprocal energy
 };
 
 // address of element m of line l along `axis`
@@ -215,6 +218,7 @@ __global__ void __launch_bounds__(256) fft_lines(Dev d, FftArgs a) {
   }
   __syncthreads();
   float2 *res;
+  double ek = 0.0;
   if (a.mode == 3) {
     res = lds_fft(buf0, buf1, a.plan, nmine, -1.f);
     // convolution: element m of line (j,k) is wavevector (m, j, k)
@@ -232,8 +236,17 @@ __global__ void __launch_bounds__(256) fft_lines(Dev d, FftArgs a) {
       if (m || j || k)
         fac = d.bmod[0][m] * d.bmod[1][j] * d.bmod[2][k] * d.pmefac * __expf(-d.pmeexp * m2) / m2;
       float2 v = res[e];
+      if (a.energy && (m || j || k)) {
+        // E = sum_k G(k) |S(k)|^2 with G in FP64 (equal to the gathered
+        // sum_i l q phi(x_i) for the same charge grid)
+        const double kxd = mi / d.box[0], kyd = mj / d.box[1], kzd = mk / d.box[2];
+        const double m2d = kxd * kxd + kyd * kyd + kzd * kzd;
+        const double g = d.bmodd[0][m] * d.bmodd[1][j] * d.bmodd[2][k] * d.pmefacd * exp(-d.pmeexpd * m2d) / m2d;
+        ek += g * ((double)v.x * v.x + (double)v.y * v.y);
+      }
       res[e] = make_float2(v.x * fac, v.y * fac);
     }
+    if (a.energy) wave_atomic_fix(&d.energy[BM_ENBRECIP], (long long)__double2ll_rn(ek * kFixScaleD));
     __syncthreads();
     float2 *other = (res == buf0) ? buf1 : buf0;
     res = lds_fft(res, other, a.plan, nmine, 1.f);
@@ -285,19 +298,13 @@ __global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
     atomic_fix_f(&d.force[i], s * gx * (d.K[0] * (float)d.invbox[0]));
     atomic_fix_f(&d.force[d.n + i], s * gy * (d.K[1] * (float)d.invbox[1]));
     atomic_fix_f(&d.force[2 * d.n + i], s * gz * (d.K[2] * (float)d.invbox[2]));
-    // Ewald self term (nbrecip.cu:19-50): -beta kE/sqrt(pi) (l q)^2
-    const double pref = -(double)d.beta * kElectricD / 1.7724538509055159;
-    const double es = pref * (double)q * (double)q;
-    if (b) {
-      atomic_fix(&d.dudl[b], fix_d(2.0 * (double)q * (double)phi + 2.0 * (double)l * es));
-    }
-    e = (double)l * (double)q * (double)phi;
-    eself = es * (double)l * (double)l;
+    // dU/dlambda_b = 2 q phi (nbrecip.cu:445-449); the energy is summed in
+    // k-space by the convolution pass
+    if (b) atomic_fix(&d.dudl[b], fix_d(2.0 * d.qd[i] * (double)phi));
   }
-  if (energy) {
-    wave_atomic_fix(&d.energy[BM_ENBRECIP], (long long)__double2ll_rn(e * kFixScaleD));
-    wave_atomic_fix(&d.energy[BM_ENBRECIPSELF], (long long)__double2ll_rn(eself * kFixScaleD));
-  }
+  (void)e;
+  (void)eself;
+  (void)energy;
 }
 
 // ---------------------------------------------------------------------
@@ -337,6 +344,7 @@ void launch_pme(const Dev &d, bool energy, hipStream_t st) {
   for (int s = 0; s < 5; s++) {
     a.axis = seq_axis[s];
     a.mode = seq_mode[s];
+    a.energy = energy && a.mode == 3;
     const int K = d.K[a.axis];
     make_plan(a.plan, K, tw[a.axis]);
     const int nl = kFftLds / K;

@@ -221,6 +221,11 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       ms[i] = t->mass[i];
     }
     UPLOAD(d.q, q);
+    std::vector<double> qd(t->charge, t->charge + n);
+    UPLOAD(d.qd, qd);
+    std::vector<double> bq2(nl, 0.0);
+    for (int i = 0; i < n; i++) bq2[(t->site_block ? t->site_block[i] : 0) & 0xFFFF] += qd[i] * qd[i];
+    UPLOAD(d.blockq2, bq2);
     UPLOAD(d.type, ty);
     UPLOAD(d.sb, sb);
     UPLOAD(d.mass, ms);
@@ -228,12 +233,18 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     std::vector<float2> vdw((size_t)t->n_types * t->n_types);
     for (size_t k = 0; k < vdw.size(); k++) vdw[k] = make_float2((float)t->vdw_c12[k], (float)t->vdw_c6[k]);
     UPLOAD(d.vdw, vdw);
+    std::vector<double2> vdwd(vdw.size());
+    for (size_t k = 0; k < vdw.size(); k++) vdwd[k] = make_double2(t->vdw_c12[k], t->vdw_c6[k]);
+    UPLOAD(d.vdwd, vdwd);
     d.ntypes = t->n_types;
   }
   d.rc = (float)rp->rcut;
   d.rs = (float)rp->rswitch;
   d.rc2 = d.rc * d.rc;
   d.beta = (float)rp->beta_ewald;
+  d.rcd = rp->rcut;
+  d.rsd = rp->rswitch;
+  d.betad = rp->beta_ewald;
   d.vdwmethod = rp->vdw_method;
   d.elecmethod = rp->elec_method;
   // domdec buffers
@@ -251,6 +262,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.bbc, MB);
   ALLOC(d.bbh, MB);
   ALLOC(d.spos, (size_t)MB * kBlockAtoms);
+  ALLOC(d.slo, (size_t)MB * kBlockAtoms);
   ALLOC(d.spar, (size_t)MB * kBlockAtoms);
   ALLOC(d.tcount, MB);
   ALLOC(d.tstart, MB + 1);
@@ -287,6 +299,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     for (int k = 0; k < 3; k++) {
       // 1/|b(m)|^2 from the integer-point B-spline values (potential.cxx:1245-1287)
       std::vector<float> bm(d.K[k]);
+      std::vector<double> bmd(d.K[k]);
       for (int i = 0; i < d.K[k]; i++) {
         double re = 0, im = 0;
         for (int l = 1; l < d.order; l++) {
@@ -294,13 +307,17 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
           re += mv * std::cos(2 * M_PI * i * l / d.K[k]);
           im += mv * std::sin(2 * M_PI * i * l / d.K[k]);
         }
-        bm[i] = (float)(1.0 / (re * re + im * im));
+        bmd[i] = 1.0 / (re * re + im * im);
+        bm[i] = (float)bmd[i];
       }
       UPLOAD(d.bmod[k], bm);
+      UPLOAD(d.bmodd[k], bmd);
     }
     const double V = rp->box[0] * rp->box[1] * rp->box[2];
-    d.pmefac = (float)(0.5 * kElectricD / M_PI / V);
-    d.pmeexp = (float)(M_PI * M_PI / (rp->beta_ewald * rp->beta_ewald));
+    d.pmefacd = 0.5 * kElectricD / M_PI / V;
+    d.pmeexpd = M_PI * M_PI / (rp->beta_ewald * rp->beta_ewald);
+    d.pmefac = (float)d.pmefacd;
+    d.pmeexp = (float)d.pmeexpd;
   }
   // bonded terms
   {
@@ -344,20 +361,20 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     }
     d.nnb14 = t->n_nb14;
     std::vector<int2> pi(d.nnb14), psb(d.nnb14);
-    std::vector<float4> pp(d.nnb14);
+    std::vector<double4> pp(d.nnb14);
     for (int q = 0; q < d.nnb14; q++) {
       pi[q] = make_int2(t->nb14_idx[2 * q], t->nb14_idx[2 * q + 1]);
       psb[q] = make_int2(t->nb14_sb[2 * q], t->nb14_sb[2 * q + 1]);
-      pp[q] = make_float4((float)t->nb14_qq[q], (float)t->nb14_c12[q], (float)t->nb14_c6[q], (float)t->nb14_e14fac[q]);
+      pp[q] = make_double4(t->nb14_qq[q], t->nb14_c12[q], t->nb14_c6[q], t->nb14_e14fac[q]);
     }
     UPLOAD(d.nb14_i, pi); UPLOAD(d.nb14_sb, psb); UPLOAD(d.nb14_p, pp);
     d.nnbex = rp->elec_method == BM_PME ? t->n_nbex : 0;
     std::vector<int2> xi(d.nnbex), xsb(d.nnbex);
-    std::vector<float> xp(d.nnbex);
+    std::vector<double> xp(d.nnbex);
     for (int q = 0; q < d.nnbex; q++) {
       xi[q] = make_int2(t->nbex_idx[2 * q], t->nbex_idx[2 * q + 1]);
       xsb[q] = make_int2(t->nbex_sb[2 * q], t->nbex_sb[2 * q + 1]);
-      xp[q] = (float)t->nbex_qq[q];
+      xp[q] = t->nbex_qq[q];
     }
     UPLOAD(d.nbex_i, xi); UPLOAD(d.nbex_sb, xsb); UPLOAD(d.nbex_p, xp);
   }

@@ -92,6 +92,7 @@ struct RunSetup {
   int freqNRG = 10;
   int freqNPT = 0;
   unsigned long long seed = 0xB1ADE000ull;
+  int freqDomdec = 10;
 };
 
 // Derived term tables owning their storage; view() exposes bm_topology.

@@ -71,6 +71,7 @@ class Spec:
     atom_block: np.ndarray = None                    # (N,) 0-based blocks
     block_site: list = field(default_factory=lambda: [0])
     theta0: list = field(default_factory=lambda: [0.0])
+    theta_v: list = None                             # theta velocities (None = 0)
     theta_mass: list = field(default_factory=lambda: [0.0])
     fixed_bias: list = field(default_factory=lambda: [0.0])
     var_bias: list = field(default_factory=list)     # (i, j, type, l0, k, n) 0-based blocks
@@ -351,20 +352,22 @@ def _polymer(spec, rng, natoms, center, radius, grid, site_at=None, nsubs=3, sub
         """hydrogen direction away from bonded partners with the largest
         clearance from everything already placed"""
         best, bestd = None, -1.0
-        for _ in range(tries):
-            d = rng.normal(size=3)
-            d /= np.linalg.norm(d)
-            # near-tetrahedral: 100-125 degrees to every bonded partner
-            if not all(-0.57 < np.dot(d, (x - center_p) / np.linalg.norm(x - center_p)) < -0.17 for x in bonded):
-                continue
-            p = center_p + blen * d
-            m = grid.mindist(p, skip=skip)
-            if m > bestd:
-                best, bestd = p, m
-        if best is None:
-            u = sum((center_p - x) / np.linalg.norm(center_p - x) for x in bonded)
-            best = center_p + blen * u / np.linalg.norm(u)
-        return best
+        units = [(x - center_p) / np.linalg.norm(x - center_p) for x in bonded]
+        # near-tetrahedral window (100-125 degrees to every bonded partner),
+        # widened until some direction fits
+        for lo, hi, n_try in ((-0.57, -0.17, tries), (-0.75, -0.05, 4 * tries), (-0.95, 0.2, 8 * tries)):
+            for _ in range(n_try):
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                if not all(lo < np.dot(d, u) < hi for u in units):
+                    continue
+                p = center_p + blen * d
+                m = grid.mindist(p, skip=skip)
+                if m > bestd:
+                    best, bestd = p, m
+            if best is not None:
+                return best
+        raise RuntimeError("no hydrogen position found")
 
     for k, t in enumerate(types):
         ht, hb = H_OF[t]
@@ -386,6 +389,26 @@ def _polymer(spec, rng, natoms, center, radius, grid, site_at=None, nsubs=3, sub
             grid.add(p)
             hatoms[k].append(idx)
             nb.append(p)
+    # charges: hydrogens positive by type with a small seeded spread, each
+    # heavy atom balances its own hydrogens plus a larger spread, the body is
+    # neutral.  Negative hydrogens would meet other hydrogens with no LJ core
+    # between them and collapse.
+    hq0 = {"HA": 0.09, "HN": 0.31, "HO": 0.42}
+    nheavy = len(types)
+    hq = iter(q[nheavy:])
+    heavy_q = []
+    for k in range(nheavy):
+        hs = 0.0
+        for h in hatoms[k]:
+            a = spec.atoms[h]
+            qh = max(0.03, hq0[a[4]] + 0.1 * float(next(hq)))
+            spec.atoms[h] = a[:5] + (qh,) + a[6:]
+            hs += qh
+        heavy_q.append(-hs + 0.5 * float(q[k]))
+    shift = (sum(heavy_q) + sum(spec.atoms[h][5] for k in range(nheavy) for h in hatoms[k])) / nheavy
+    for k in range(nheavy):
+        a = spec.atoms[heavy_idx[k]]
+        spec.atoms[heavy_idx[k]] = a[:5] + (heavy_q[k] - shift,) + a[6:]
     # bonds
     for k in range(len(types)):
         if k > 0:

@@ -123,6 +123,9 @@ class BladeSystem:
     def set_seed(self, seed):
         self.L.blade_set_seed(self.h, seed)
 
+    def set_domdec_frequency(self, steps):
+        self.L.blade_set_domdec_frequency(self.h, steps)
+
     def run_energy(self):
         self.L.blade_run_energy(self.h)
         err = self.L.blade_last_error().decode()
@@ -231,7 +234,8 @@ def build(spec, gpus=(0,)):
         for i in np.nonzero(spec.atom_block)[0]:
             s.add_msld_atomassignment(int(i) + 1, int(spec.atom_block[i]) + 1)
         for b in range(spec.nblocks):
-            s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], 0.0,
+            tv = spec.theta_v[b] if spec.theta_v is not None else 0.0
+            s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], tv,
                                          spec.theta_mass[b], spec.fixed_bias[b], 0.0)
         s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2, 59.2, 1.0, 2.0, 1.0, 0)
         for (i, j, t, l0, k, n) in spec.var_bias:
@@ -245,3 +249,52 @@ def build(spec, gpus=(0,)):
     s.n = spec.n
     s.nblocks = spec.nblocks
     return s
+
+
+# Relaxation schedule for synthetic starts: (dt fs, gamma 1/ps, steps,
+# steps between list rebuilds).  Lattice-built systems start far from
+# equilibrium (random water orientations release ~10 kcal/mol per water);
+# heavily damped short steps drain that energy before production-length
+# steps, and lists are rebuilt every step while atoms still move faster than
+# the thermal list pad assumes.
+RELAX_SCHEDULE = ((0.1, 1000.0, 400, 1), (0.5, 200.0, 400, 1), (1.0, 50.0, 400, 1),
+                  (2.0, 5.0, 300, 10))
+
+
+def state(s, n, nblocks):
+    """(pos (N,3), vel (N,3), theta, theta_v) of a running BladeSystem"""
+    dev = s.device()
+    x = np.zeros(3 * n)
+    v = np.zeros(3 * n)
+    th = np.zeros(max(1, nblocks))
+    tv = np.zeros(max(1, nblocks))
+    check(s.L.bm_get_positions(dev, ptr(x), ptr(th)), "bm_get_positions")
+    check(s.L.bm_get_velocities(dev, ptr(v), ptr(tv)), "bm_get_velocities")
+    return x.reshape(3, n).T.copy(), v.reshape(3, n).T.copy(), th, tv
+
+
+def relax(spec, schedule=RELAX_SCHEDULE, gpus=(0,), log=None):
+    """Copy of `spec` advanced through the relaxation schedule on the GPU
+    (positions, velocities and theta replaced; run options kept)."""
+    import copy
+
+    from .synthetic import PICOSECOND
+    cur = copy.copy(spec)
+    for dt_fs, gamma_ps, steps, freq_dd in schedule:
+        stage = copy.copy(cur)
+        stage.dt = dt_fs * 0.001 * PICOSECOND
+        stage.gamma = gamma_ps / PICOSECOND
+        s = build(stage, gpus)
+        s.set_domdec_frequency(freq_dd)
+        s.run_dynamics(steps - 1, freq_nrg=0)
+        s.run_dynamics(1, freq_nrg=1)
+        if log:
+            e = s.energy()
+            log("relax dt=%.1f fs gamma=%g/ps steps=%d: Epot %.1f KE %.1f" % (dt_fs, gamma_ps, steps, e[20], e[21]))
+        pos, vel, th, tv = state(s, spec.n, spec.nblocks)
+        s.close()
+        cur = copy.copy(cur)
+        cur.pos, cur.vel = pos, vel
+        if spec.nblocks > 1:
+            cur.theta0, cur.theta_v = list(th[:spec.nblocks]), list(tv[:spec.nblocks])
+    return cur

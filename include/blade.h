@@ -132,6 +132,9 @@ bm_system *blade_get_device_system(System *system);
 int blade_prepare(System *system);
 /* Philox seed of the Langevin noise */
 void blade_set_seed(System *system, unsigned long long seed);
+/* steps between neighbour-list rebuilds (default 10, domdec/domdec.cu:138);
+   1 rebuilds every step, for starts whose atoms outrun the thermal list pad */
+void blade_set_domdec_frequency(System *system, int steps);
 
 #ifdef __cplusplus
 }

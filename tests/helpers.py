@@ -25,6 +25,12 @@ def built(name):
     return s
 
 
+@functools.lru_cache(maxsize=None)
+def relaxed(name):
+    """the spec advanced through the GPU relaxation schedule (needs a GPU)"""
+    return system.relax(spec(name))
+
+
 def lambdas0(s, name):
     """lambda of the initial theta by the oracle's rule"""
     from oracle import oracle as orc

@@ -12,7 +12,7 @@ import pytest
 
 from blade_amd import system
 from oracle import oracle as orc
-from tests.helpers import built, lambdas0, rel_rms, spec
+from tests.helpers import built, lambdas0, rel_rms, relaxed, spec
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +69,14 @@ def test_config2_small_forces_energy_dudl():
     assert np.max(np.abs(dg - do)) <= DUDL_TOL
 
 
+def test_config2_forces_energy_dudl():
+    """the full 23,558-atom benchmark system"""
+    s, fo, fg, eo, eg, do, dg, frms, erel = _compare("config2")
+    assert frms <= FORCE_TOL
+    assert erel <= ENERGY_TOL
+    assert np.max(np.abs(dg - do)) <= DUDL_TOL
+
+
 def test_domdec_bit_exact():
     for name in ("config1", "msld_small", "config2_small"):
         s = built(name)
@@ -100,19 +108,23 @@ def test_domdec_bit_exact():
         assert np.array_equal(s.positions(), posw)
 
 
-def _run(name, nsteps, use_graph):
-    s = built(name)
+def _run(sp, nsteps, use_graph):
+    s = system.build(sp)
     s.run_dynamics(nsteps, freq_nrg=10, use_graph=use_graph)
-    return s.positions(), s.energy()
+    return s.positions(), s.energy(), s.lambdas()
 
 
 def test_trajectory_bit_reproducible_and_graph_equals_eager():
-    xa, ea = _run("msld_small", 25, True)
-    xb, eb = _run("msld_small", 25, True)
-    xc, ec = _run("msld_small", 25, False)
+    sp = relaxed("msld_small")
+    xa, ea, la = _run(sp, 40, True)
+    xb, eb, lb = _run(sp, 40, True)
+    xc, ec, lc = _run(sp, 40, False)
+    print("\nlambda after 40 steps", la, flush=True)
+    assert np.all(np.isfinite(xa))
     assert np.array_equal(xa, xb)
     assert np.array_equal(xa, xc)
     assert np.array_equal(ea, eb)
+    assert np.array_equal(la, lc)
 
 
 def test_dynamics_constraints_and_stability():
@@ -138,13 +150,14 @@ def test_dynamics_constraints_and_stability():
 def test_short_trajectory_tracks_oracle():
     """5 steps GPU vs FP64 oracle from the same state and Philox stream"""
     name = "msld_small"
-    s = built(name)
+    sp = relaxed(name)
+    s = system.build(sp)
+    s.prepare()
     top, rp = s.topology(), s.runparams()
-    sp = spec(name)
     n, nl = sp.n, sp.nblocks
     x = np.concatenate([system.soa(sp.pos), np.array(sp.theta0, float)])
-    v = np.concatenate([system.soa(sp.vel), np.zeros(nl)])
-    lam = lambdas0(s, name)
+    v = np.concatenate([system.soa(sp.vel), np.array(sp.theta_v, float)])
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
     # the GPU wraps positions at step 0 (rebuild); do the same for the oracle
     posw = orc.domdec(top, rp, x[:3 * n])[0]
     x[:3 * n] = posw
