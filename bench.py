@@ -30,6 +30,7 @@ FLOP_PER_PAIR = 64            # algorithmic FP32 work per in-cutoff pair
 PEAK_FP32 = 157.3e12          # MI355X vector FP32, FLOP/s
 PEAK_HBM = 8.0e12
 FREQ_NRG = 1000
+CAL_BYTES = 256 << 20
 
 
 def log(*a):
@@ -43,6 +44,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch kernels one by one instead of replaying captured graphs "
+                         "(for kernel tracers that do not follow graph launches)")
+    ap.add_argument("--state", default=None,
+                    help="npz file holding the relaxed state: loaded when present, written otherwise")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one system on one GPU; --gpus %d is not supported "
@@ -56,9 +62,18 @@ def main():
     t0 = time.time()
     spec = synthetic.config2()
     log("generated %s: %d atoms in %.1f s" % (spec.name, spec.n, time.time() - t0))
-    t0 = time.time()
-    spec = system.relax(spec, log=log)
-    log("relaxed in %.1f s" % (time.time() - t0))
+    if a.state and os.path.exists(a.state):
+        z = np.load(a.state)
+        spec.pos, spec.vel = z["pos"], z["vel"]
+        spec.theta0, spec.theta_v = list(z["theta"]), list(z["theta_v"])
+        log("loaded relaxed state from %s" % a.state)
+    else:
+        t0 = time.time()
+        spec = system.relax(spec, log=log, use_graph=not a.eager)
+        log("relaxed in %.1f s" % (time.time() - t0))
+        if a.state:
+            np.savez(a.state, pos=spec.pos, vel=spec.vel, theta=np.array(spec.theta0),
+                     theta_v=np.array(spec.theta_v))
 
     s = system.build(spec)
     s.run_energy()                       # device system + first list build
@@ -67,16 +82,16 @@ def main():
     dt_fs = spec.dt / synthetic.PICOSECOND * 1000.0
 
     # warmup (also captures the graphs)
-    check(L.bm_dynamics(dev, 1, a.warmup, FREQ_NRG, 1), "warmup")
+    graph = 0 if a.eager else 1
+    check(L.bm_dynamics(dev, 1, a.warmup, FREQ_NRG, graph), "warmup")
     check(L.bm_sync(dev), "sync")
     step0 = 1 + a.warmup
     t1 = time.perf_counter()
-    check(L.bm_dynamics(dev, step0, a.steps, FREQ_NRG, 1), "timed steps")
+    check(L.bm_dynamics(dev, step0, a.steps, FREQ_NRG, graph), "timed steps")
     check(L.bm_sync(dev), "sync")
     t2 = time.perf_counter()
     ms_step = (t2 - t1) * 1e3 / a.steps
     nsday = (1e3 / ms_step) * dt_fs * 86400e-6
-    nan = C_int = None
     import ctypes as C
     flag = C.c_int(-1)
     check(L.bm_get_nan_flag(dev, C.byref(flag)), "nan flag")
@@ -87,6 +102,11 @@ def main():
     prof = np.zeros(3)
     check(L.bm_profile_steps(dev, step0 + a.steps, 50, ptr(prof)), "profile")
     ms_nb, ms_pme, ms_eager = prof
+
+    # HBM copy calibration: a plain copy kernel over 256 MiB
+    cal = np.zeros(1)
+    check(L.bm_calibration_copy(0, CAL_BYTES, 20, ptr(cal)), "calibration copy")
+    copy_gbps = 2 * CAL_BYTES / (cal[0] * 1e-3) / 1e9
 
     # in-cutoff pair count at the current configuration (oracle count)
     from oracle import oracle as orc
@@ -136,9 +156,10 @@ def main():
         "config": {"workload": "config2-msld-23558", "atoms": n, "box_A": spec.box[0],
                    "cutoff_A": spec.rcut, "switch_A": spec.rswitch, "pme_grid": list(spec.grid),
                    "pme_order": spec.order, "dt_fs": dt_fs, "lambda_sites": 1, "substituents": 3,
-                   "list_rebuild_steps": 10, "energy_every": FREQ_NRG, "graph": True},
+                   "list_rebuild_steps": 10, "energy_every": FREQ_NRG, "graph": not a.eager},
         "pair_interactions_per_s": round(pairs / (ms_step * 1e-3), 1),
         "kernel_ms": {"nbdirect": round(ms_nb, 5), "pme": round(ms_pme, 5), "eager_step": round(ms_eager, 5)},
+        "hbm_copy_GBps": round(copy_gbps, 1),
         "roofline": {"kernel": "nbdirect", "bound": "mfma", "achieved": round(achieved / 1e12, 4),
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 5),
                      "traffic": None,

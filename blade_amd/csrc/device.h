@@ -65,13 +65,19 @@ struct Dev {
   uint4 *bbh;                    // [maxB] per-step half extent (frac counts), .w = alch flag
   uint4 *spos;                   // [32*maxB] slot (ux,uy,uz,q bits)
   int2 *spar;                    // [32*maxB] slot (type or -1, siteBlock)
-  int *tcount, *tstart;          // [maxB],[maxB+1]
+  int *tcount, *tstart;          // [maxB],[maxB+1] directed tiles per row
   int *ntiles;                   // [1]
   int *tile_i, *tile_j, *tile_ex;// [maxtiles]
   int *tile_flag;                // [maxtiles]
+  int *ccount, *cstart;          // [maxB],[maxB+1] chunks per row
+  int *nchunks;                  // [1]
+  int maxchunks;
+  int4 *chunk;                   // [maxchunks] (I, first tile, tiles, 0)
+  long long *ichunk;             // [maxchunks][3][32] i-force partials (fixed point)
+  int *ecount, *estart;          // [maxB],[maxB+1] tiles with exclusions per row
   int *nexcltiles;               // [1]
-  unsigned *emask;               // [32*n_excl]
-  int *excl_tile;                // [n_excl]
+  unsigned *emask;               // [32*2*n_excl]
+  int *excl_tile;                // [2*n_excl] tiles (I,J) and (J,I) of each pair
   int nexcl;
   int *excl;                     // [2*n_excl]
   int *workctr;                  // [4] dynamic work counters
@@ -85,7 +91,11 @@ struct Dev {
   int vdwmethod, elecmethod;
   // PME
   int K[3], order;
-  long long *qgrid;              // [K0*K1*K2] fixed-point charge grid
+  float *rgrid;                  // [K0*K1*K2] real charge grid (spread output)
+  int pnb[3];                    // spread bricks per axis (each >= order-1 points)
+  int pcap;                      // atoms per brick bin
+  int *pbcount;                  // [nbricks] atoms binned per brick, zeroed by the gather
+  int *pbatom;                   // [nbricks*pcap] atoms by home brick
   float2 *cgrid;                 // [K0*K1*K2] complex work grid
   float2 *twiddle;               // [K0+K1+K2] e^{-2 pi i k/K} per axis
   float *bmod[3];                // [K] B-spline moduli per axis
@@ -152,6 +162,27 @@ __device__ __forceinline__ float fdisp(unsigned a, unsigned b, float scale) {
 __device__ __forceinline__ void wave_atomic_fix(long long *p, long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if ((threadIdx.x & 63) == 0 && v) atomic_fix(p, v);
+}
+
+// Wave-aggregated counter claim: every lane of the wave must call it.  Lanes
+// with equal keys share one atomicAdd; each active lane gets the counter
+// value before the add plus its rank among the lanes of its key.
+__device__ __forceinline__ int wave_claim(int *counters, int key, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+  int slot = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lkey = __shfl(key, leader, 64);
+    const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+    const unsigned long long grp = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counters[lkey], __popcll(grp));
+    base = __shfl(base, leader, 64);
+    if (mine) slot = base + __popcll(grp & ((1ull << lane) - 1ull));
+    pending &= ~grp;
+  }
+  return slot;
 }
 
 // Block-wide fixed-point energy sum (any blockDim multiple of 64).

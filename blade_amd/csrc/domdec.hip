@@ -6,8 +6,20 @@
 // Rebuild sequence (no host synchronisation, fixed grids, graph-capturable):
 //   dd_wrap_count -> dd_scan_columns -> dd_scatter -> dd_sort_columns ->
 //   dd_tile_count -> dd_scan_tiles -> dd_tile_write -> dd_excl_mark ->
-//   dd_scan_excl -> dd_excl_set
+//   dd_excl_rows -> dd_scan_excl -> dd_excl_assign -> dd_excl_clear ->
+//   dd_excl_set
 // Every step: dd_pack (slot positions, block boxes, accumulator reset).
+//
+// The tile list is directed: row I holds every block J (J == I included)
+// whose rebuild box is within rcut + pad, in ascending J, so each pair of
+// blocks appears as (I,J) and (J,I).  Rows are cut into chunks of at most
+// kTileChunk tiles; the nonbonded kernel owns one chunk (one i-block) at a
+// time and never writes j-side forces.
+//
+// Column counts and bucket fills walk the slots of the previous rebuild,
+// which are ordered by column, so the lanes of a wave mostly share one
+// column and one atomic serves them all (device-scope atomics on one
+// address serialise across the chip).
 #include "kernels.h"
 
 #pragma clang fp contract(off)
@@ -51,21 +63,27 @@ __global__ void dd_zero_columns(Dev d) {
   }
 }
 
-__global__ void dd_wrap_count(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  unsigned u[3];
-  for (int k = 0; k < 3; k++) {
-    double x = d.pos[k * d.n + i];
-    double w = floor(__dmul_rn(x, d.invbox[k]));
-    x = __dsub_rn(x, __dmul_rn(w, d.box[k]));
-    d.pos[k * d.n + i] = x;
-    u[k] = frac_u32(x, d.invbox[k]);
-    d.ufrac[k * d.n + i] = u[k];
+// wrap every atom into the box, fractional coordinates, column counts
+// (one thread per slot of the previous rebuild; every atom has one slot)
+__global__ void __launch_bounds__(256) dd_wrap_count(Dev d) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = s < 32 * d.maxblocks ? d.l2g[s] : -1;
+  int col = 0;
+  if (i >= 0) {
+    unsigned u[3];
+    for (int k = 0; k < 3; k++) {
+      double x = d.pos[k * d.n + i];
+      double w = floor(__dmul_rn(x, d.invbox[k]));
+      x = __dsub_rn(x, __dmul_rn(w, d.box[k]));
+      d.pos[k * d.n + i] = x;
+      u[k] = frac_u32(x, d.invbox[k]);
+      d.ufrac[k * d.n + i] = u[k];
+    }
+    unsigned cx = (unsigned)(((unsigned long long)u[0] * (unsigned)d.ncx) >> 32);
+    unsigned cy = (unsigned)(((unsigned long long)u[1] * (unsigned)d.ncy) >> 32);
+    col = cx * d.ncy + cy;
   }
-  unsigned cx = (unsigned)(((unsigned long long)u[0] * (unsigned)d.ncx) >> 32);
-  unsigned cy = (unsigned)(((unsigned long long)u[1] * (unsigned)d.ncy) >> 32);
-  atomicAdd(&d.colcount[cx * d.ncy + cy], 1);
+  wave_claim(d.colcount, col, i >= 0);
 }
 
 // blocks per column = ceil(count/32); block starts by exclusive scan
@@ -83,14 +101,19 @@ __global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) {
   block_scan_1024(big ? tmp : nb, d.blkstart, d.ncol, d.nblocks);
 }
 
-__global__ void dd_scatter(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  unsigned cx = (unsigned)(((unsigned long long)d.ufrac[i] * (unsigned)d.ncx) >> 32);
-  unsigned cy = (unsigned)(((unsigned long long)d.ufrac[d.n + i] * (unsigned)d.ncy) >> 32);
-  int c = cx * d.ncy + cy;
-  int s = atomicAdd(&d.colfill[c], 1);
-  d.bucket[d.colstart[c] + s] = i;
+// atoms into column buckets (bucket order is arbitrary; the column sort
+// fixes the final order)
+__global__ void __launch_bounds__(256) dd_scatter(Dev d) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = s < 32 * d.maxblocks ? d.l2g[s] : -1;
+  int c = 0;
+  if (i >= 0) {
+    unsigned cx = (unsigned)(((unsigned long long)d.ufrac[i] * (unsigned)d.ncx) >> 32);
+    unsigned cy = (unsigned)(((unsigned long long)d.ufrac[d.n + i] * (unsigned)d.ncy) >> 32);
+    c = cx * d.ncy + cy;
+  }
+  const int f = wave_claim(d.colfill, c, i >= 0);
+  if (i >= 0) d.bucket[d.colstart[c] + f] = i;
 }
 
 // One workgroup per column: rank atoms by (z fraction, atom index), place
@@ -120,6 +143,11 @@ __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
   }
   for (int s = threadIdx.x; s < nbc * kBlockAtoms; s += blockDim.x)
     if (s >= cnt) d.l2g[b0 * kBlockAtoms + s] = -1;
+  // slots past the last block may hold atoms of an earlier, longer list:
+  // the next rebuild walks every slot, so clear them
+  if (c == 0)
+    for (int s = *d.nblocks * kBlockAtoms + threadIdx.x; s < d.maxblocks * kBlockAtoms; s += blockDim.x)
+      d.l2g[s] = -1;
   __threadfence_block();
   __syncthreads();
   for (int t = threadIdx.x; t < nbc * 3; t += blockDim.x) {
@@ -154,23 +182,32 @@ __device__ __forceinline__ bool dd_boxes_close(const Dev &d, int I, int J) {
   return s <= d.rlist2;
 }
 
-// one wave per i-block: number of J >= I within the list radius
+// one wave per i-block: number of blocks J within the list radius (all J)
 __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int nb = *d.nblocks;
   if (I >= nb) return;
   int cnt = 0;
-  for (int J0 = I; J0 < nb; J0 += 64) {
+  for (int J0 = 0; J0 < nb; J0 += 64) {
     int J = J0 + lane;
     bool keep = J < nb && (J == I || dd_boxes_close(d, I, J));
     cnt += __popcll(__ballot(keep));
   }
-  if (lane == 0) d.tcount[I] = cnt;
+  if (lane == 0) {
+    d.tcount[I] = cnt;
+    d.ccount[I] = (cnt + kTileChunk - 1) / kTileChunk;
+  }
 }
 
+// row starts of tiles and chunks; flags a list that outgrows its buffers
 __global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
-  block_scan_1024(d.tcount, d.tstart, *d.nblocks, d.ntiles);
+  const int nb = *d.nblocks;
+  block_scan_1024(d.tcount, d.tstart, nb, d.ntiles);
+  __syncthreads();
+  block_scan_1024(d.ccount, d.cstart, nb, d.nchunks);
+  __syncthreads();
+  if (threadIdx.x == 0 && (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks)) atomicOr(d.errflag, 4);
 }
 
 __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
@@ -178,9 +215,10 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
   const int lane = threadIdx.x & 63;
   const int nb = *d.nblocks;
   if (I >= nb) return;
+  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
   int base = d.tstart[I];
   const unsigned long long below = (1ull << lane) - 1;
-  for (int J0 = I; J0 < nb; J0 += 64) {
+  for (int J0 = 0; J0 < nb; J0 += 64) {
     int J = J0 + lane;
     bool keep = J < nb && (J == I || dd_boxes_close(d, I, J));
     unsigned long long m = __ballot(keep);
@@ -192,32 +230,68 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
     }
     base += __popcll(m);
   }
+  const int t0 = d.tstart[I], cnt = d.tcount[I], c0 = d.cstart[I];
+  for (int k = lane; k * kTileChunk < cnt; k += 64)
+    d.chunk[c0 + k] = make_int4(I, t0 + k * kTileChunk, min(kTileChunk, cnt - k * kTileChunk), 0);
 }
 
-// exclusion pair -> tile holding it (binary search in I's sorted J list)
-__global__ void dd_excl_mark(Dev d) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= d.nexcl) return;
-  int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
-  if (sa / kBlockAtoms > sb / kBlockAtoms) { int t = sa; sa = sb; sb = t; }
-  const int I = sa / kBlockAtoms, J = sb / kBlockAtoms;
+// tile (I,J) by binary search in row I's ascending J list, -1 if absent
+__device__ __forceinline__ int dd_find_tile(const Dev &d, int I, int J) {
   int lo = d.tstart[I], hi = d.tstart[I + 1];
+  const int end = hi;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
     if (d.tile_j[mid] < J) lo = mid + 1; else hi = mid;
   }
-  int t = (lo < d.tstart[I + 1] && d.tile_j[lo] == J) ? lo : -1;
-  d.excl_tile[e] = t;
-  if (t >= 0) d.tile_flag[t] = 1;
+  return (lo < end && d.tile_j[lo] == J) ? lo : -1;
+}
+
+// exclusion pair -> the tiles (I,J) and (J,I) holding it
+__global__ void dd_excl_mark(Dev d) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.nexcl) return;
+  if (*d.ntiles > d.maxtiles) return;
+  const int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
+  const int I = sa / kBlockAtoms, J = sb / kBlockAtoms;
+  const int ta = dd_find_tile(d, I, J);
+  const int tb = ta < 0 ? -1 : (I == J ? ta : dd_find_tile(d, J, I));
+  d.excl_tile[2 * e] = ta;
+  d.excl_tile[2 * e + 1] = tb;
+  if (ta >= 0) d.tile_flag[ta] = 1;
+  if (tb >= 0) d.tile_flag[tb] = 1;
+}
+
+// one wave per row: tiles of the row holding exclusions
+__global__ void __launch_bounds__(256) dd_excl_rows(Dev d) {
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (I >= *d.nblocks) return;
+  int cnt = 0;
+  if (*d.ntiles <= d.maxtiles)
+    for (int t = d.tstart[I] + lane; t - lane < d.tstart[I + 1]; t += 64)
+      cnt += __popcll(__ballot(t < d.tstart[I + 1] && d.tile_flag[t]));
+  if (lane == 0) d.ecount[I] = cnt;
 }
 
 __global__ void __launch_bounds__(1024) dd_scan_excl(Dev d) {
-  const int nt = *d.ntiles;
-  // exclusive scan of flags into tile_ex, then flag-less tiles get -1
-  block_scan_1024(d.tile_flag, d.tile_ex, nt, d.nexcltiles);
-  __syncthreads();
-  for (int t = threadIdx.x; t < nt; t += 1024)
-    if (!d.tile_flag[t]) d.tile_ex[t] = -1;
+  block_scan_1024(d.ecount, d.estart, *d.nblocks, d.nexcltiles);
+}
+
+// one wave per row: compact mask index of every flagged tile, -1 otherwise
+__global__ void __launch_bounds__(256) dd_excl_assign(Dev d) {
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (I >= *d.nblocks || *d.ntiles > d.maxtiles) return;
+  int base = d.estart[I];
+  const unsigned long long below = (1ull << lane) - 1;
+  const int t1 = d.tstart[I + 1];
+  for (int t0 = d.tstart[I]; t0 < t1; t0 += 64) {
+    const int t = t0 + lane;
+    const bool f = t < t1 && d.tile_flag[t];
+    const unsigned long long m = __ballot(f);
+    if (t < t1) d.tile_ex[t] = f ? base + __popcll(m & below) : -1;
+    base += __popcll(m);
+  }
 }
 
 __global__ void dd_excl_clear(Dev d) {
@@ -227,17 +301,17 @@ __global__ void dd_excl_clear(Dev d) {
     d.emask[i] = 0u;
 }
 
+// bit j of word i of tile (I,J): slots (I,i) and (J,j) are excluded
 __global__ void dd_excl_set(Dev d) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.nexcl) return;
-  const int t = d.excl_tile[e];
-  if (t < 0) return;
-  int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
-  if (sa / kBlockAtoms > sb / kBlockAtoms) { int x = sa; sa = sb; sb = x; }
-  const int x = d.tile_ex[t];
-  atomicOr(&d.emask[x * kBlockAtoms + sa % kBlockAtoms], 1u << (sb % kBlockAtoms));
-  if (sa / kBlockAtoms == sb / kBlockAtoms)
-    atomicOr(&d.emask[x * kBlockAtoms + sb % kBlockAtoms], 1u << (sa % kBlockAtoms));
+  if (*d.ntiles > d.maxtiles) return;
+  const int ta = d.excl_tile[2 * e], tb = d.excl_tile[2 * e + 1];
+  if (ta < 0) return;
+  const int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
+  const int la = sa % kBlockAtoms, lb = sb % kBlockAtoms;
+  atomicOr(&d.emask[d.tile_ex[ta] * kBlockAtoms + la], 1u << lb);
+  if (tb >= 0) atomicOr(&d.emask[d.tile_ex[tb] * kBlockAtoms + lb], 1u << la);
 }
 
 // Every step: slot coordinates, per-block box centre/half-extent for the
@@ -294,16 +368,21 @@ __global__ void __launch_bounds__(256) dd_pack(Dev d) {
 // ---------------------------------------------------------------------
 void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st) {
   const int n = d.n;
+  const unsigned slots = (unsigned)((32LL * maxBlocks + 255) / 256);
+  const unsigned rows = (unsigned)((maxBlocks + 3) / 4);   // a wave per block
+  (void)n;
   dd_zero_columns<<<(d.ncol + 255) / 256, 256, 0, st>>>(d);
-  dd_wrap_count<<<(n + 255) / 256, 256, 0, st>>>(d);
+  dd_wrap_count<<<slots, 256, 0, st>>>(d);
   dd_scan_columns<<<1, 1024, 0, st>>>(d);
-  dd_scatter<<<(n + 255) / 256, 256, 0, st>>>(d);
+  dd_scatter<<<slots, 256, 0, st>>>(d);
   dd_sort_columns<<<d.ncol, 256, 0, st>>>(d);
-  dd_tile_count<<<(maxBlocks + 3) / 4, 256, 0, st>>>(d);
+  dd_tile_count<<<rows, 256, 0, st>>>(d);
   dd_scan_tiles<<<1, 1024, 0, st>>>(d);
-  dd_tile_write<<<(maxBlocks + 3) / 4, 256, 0, st>>>(d);
+  dd_tile_write<<<rows, 256, 0, st>>>(d);
   if (d.nexcl) dd_excl_mark<<<(d.nexcl + 255) / 256, 256, 0, st>>>(d);
+  dd_excl_rows<<<rows, 256, 0, st>>>(d);
   dd_scan_excl<<<1, 1024, 0, st>>>(d);
+  dd_excl_assign<<<rows, 256, 0, st>>>(d);
   if (d.nexcl) {
     dd_excl_clear<<<256, 256, 0, st>>>(d);
     dd_excl_set<<<(d.nexcl + 255) / 256, 256, 0, st>>>(d);

@@ -1,18 +1,23 @@
 // nbdirect.hip - real-space nonbonded tile kernel
 // (restates nbdirect/nbdirect.cu:87-490 pair semantics).
 //
-// Work unit: a tile = (i-block I, j-block J>=I) of 32x32 slots from the
-// rebuild list.  One wave64 processes two tiles at once, one per 32-lane
-// half.  Lane li of a half holds i-atom li in registers; the 32 j-atoms of
-// its tile sit in LDS.  At step k lane li meets j-atom (li+k)&31, so the 32
-// lanes of a half touch 32 distinct j slots: j-forces accumulate in LDS
-// without atomics or cross-lane reductions.  A tile whose current block
-// boxes are farther apart than rcut is skipped (per-step cull).
+// Work unit: a chunk of at most kTileChunk directed tiles (I, J) of one row
+// I of the rebuild list.  One wave64 owns a chunk: both 32-lane halves hold
+// the 32 i-atoms of block I in registers, so i-forces accumulate without
+// any cross-lane or memory traffic, and no j-side force is ever written
+// (every pair is visited from both sides; energies take half per visit).
+//
+// Per tile and step: the tile is skipped when the current block boxes are
+// farther apart than rcut; otherwise the j-atoms within rcut of the current
+// i-box form a wave-uniform 32-bit mask, and the two halves take the set
+// bits in turns (j data broadcast from LDS).  Tiles without alchemical atoms
+// run a path with no lambda arithmetic.
 //
 // Accumulation: per-tile FP32 partial sums are converted to 64-bit fixed
-// point; i-forces stay in registers while consecutive tiles share I, and
-// are added to the global fixed-point force array with one atomic per
-// component when I changes.  Energies accumulate in FP64 per lane.
+// point and summed per lane; at the end of a chunk the i-force partials are
+// stored to the chunk's own slot.  nb_reduce then adds the chunks of every
+// block to the force array, one thread per atom: no global atomics, and the
+// result does not depend on the order in which waves claimed chunks.
 #include "kernels.h"
 #include "erfc_coef.h"
 
@@ -196,202 +201,263 @@ __device__ __forceinline__ bool boxes_within(const Dev &d, int I, int J, float r
   return g2 <= rc2 * 1.0001f;
 }
 
+// j-atom (fractional coordinates pj) within rcut of the current box of
+// block I (centre/half extent in fractional counts)
+__device__ __forceinline__ bool near_box(const Dev &d, uint4 ci, uint4 hi, uint4 pj, float rc2) {
+  float g2 = 0;
+  {
+    long long g = llabs((long long)(int)(pj.x - ci.x)) - (long long)hi.x;
+    float gf = g > 0 ? (float)g * d.fracscale[0] : 0.f;
+    g2 += gf * gf;
+  }
+  {
+    long long g = llabs((long long)(int)(pj.y - ci.y)) - (long long)hi.y;
+    float gf = g > 0 ? (float)g * d.fracscale[1] : 0.f;
+    g2 += gf * gf;
+  }
+  {
+    long long g = llabs((long long)(int)(pj.z - ci.z)) - (long long)hi.z;
+    float gf = g > 0 ? (float)g * d.fracscale[2] : 0.f;
+    g2 += gf * gf;
+  }
+  return g2 <= rc2 * 1.0001f;
+}
+
+// i-atom of a lane, held for a whole chunk
+struct IAtom {
+  uint4 p;       // fractional coordinates, charge bits
+  int t, b;      // LJ type (-1: empty slot), siteBlock
+  float q;       // kE * charge
+  float lam;     // lambda of its block (1 outside sites)
+  double qd;     // kE * charge in FP64 (energy steps)
+};
+
+constexpr int kVdwLds = 1024;   // LJ pair table entries staged in LDS
+
+// The pairs of one tile for the lane's i-atom: the set bits of jm, shared
+// out between the halves (half 0 takes the first, half 1 the second, ...).
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
+__device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const IAtom &ia, int li,
+                                           int half, const uint4 *sp, const int2 *spr, const float *slo,
+                                           const float2 *svdw, bool vdwLds, unsigned jm, unsigned em,
+                                           bool self, float &fx, float &fy, float &fz, float &fli,
+                                           double &etile) {
+  const float sx = d.fracscale[0], sy = d.fracscale[1], sz = d.fracscale[2];
+  while (jm) {
+    const int j0 = __builtin_ffs(jm) - 1;
+    jm &= jm - 1;
+    int j1 = 32;
+    if (jm) {
+      j1 = __builtin_ffs(jm) - 1;
+      jm &= jm - 1;
+    }
+    const int jl = half ? j1 : j0;
+    if (jl >= 32) continue;
+    const uint4 pj = sp[jl];
+    const int2 prj = spr[jl];
+    const float dx = fdisp(ia.p.x, pj.x, sx);
+    const float dy = fdisp(ia.p.y, pj.y, sy);
+    const float dz = fdisp(ia.p.z, pj.z, sz);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    bool ok = ia.t >= 0 && !((em >> jl) & 1u) && r2 < c.rc2;
+    if (self) ok = ok && jl != li;
+    if (!ok) continue;
+    const int vi = ia.t * d.ntypes + prj.x;
+    const float2 lj = vdwLds ? svdw[vi] : d.vdw[vi];
+    float rinv = __frsqrt_rn(r2);
+    float r = r2 * rinv;
+    const float qq = ia.q * __uint_as_float(pj.w);
+    float s = 1.f, dredr = 1.f, dredll = 0.f, lamj = 1.f;
+    const int bi = ia.b, bj = prj.y;
+    const bool alchPair = ALCH && (bi | bj);
+    if (ALCH && alchPair) {
+      if (bj & 0xFFFF) lamj = d.lambdaf[bj & 0xFFFF];
+      const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
+      s = same ? (bi == bj ? ia.lam : 0.f) : ia.lam * lamj;
+      if (SOFT) {
+        const float rSoft = kSoftCoreRadius * (1.f - s);
+        if (r < rSoft) {
+          const float q = r / rSoft;
+          float re = 1.f - 0.5f * q;
+          re = re * q * q * q + 0.5f;
+          dredr = (3.f - 2.f * q) * q * q;
+          dredll = (re - dredr * q) * (-kSoftCoreRadius);
+          r = re * rSoft;
+          rinv = 1.f / r;
+        }
+      }
+    }
+    float f, eq, eA, eB;
+    pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, eq, eA, eB);
+    if (ALCH) f *= s;
+    if (ALCH && alchPair) {
+      // dU/dlambda of the i-atom's block only: the visit from j adds j's.
+      // A same-block pair is visited twice for one lambda: half each time.
+      const float e = qq * eq + lj.x * eA - lj.y * eB;
+      const float fl = SOFT ? e + f * dredll : e;
+      const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
+      if (same) {
+        if (bi == bj) fli += 0.5f * fl;
+      } else {
+        fli += lamj * fl;
+      }
+    }
+    if (ENERGY) {
+      // FP64 pair energy from the exact integer displacement and FP64
+      // charges / LJ parameters; half per visit
+      const double2 ljd = d.vdwd[vi];
+      const double qjd = (double)__uint_as_float(pj.w) + (double)slo[jl];
+      const double ddx = (double)(int)(ia.p.x - pj.x) * d.ddscale[0];
+      const double ddy = (double)(int)(ia.p.y - pj.y) * d.ddscale[1];
+      const double ddz = (double)(int)(ia.p.z - pj.z) * d.ddscale[2];
+      double rd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+      double sd = 1.0;
+      if (ALCH && alchPair) {
+        const double li_d = (bi & 0xFFFF) ? d.lambda[bi & 0xFFFF] : 1.0;
+        const double lj_d = (bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0;
+        const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
+        sd = same ? (bi == bj ? li_d : 0.0) : li_d * lj_d;
+        if (SOFT) {
+          const double rSoft = 4.0 * (1.0 - sd);
+          if (rd < rSoft) {
+            const double q = rd / rSoft;
+            rd = ((1.0 - 0.5 * q) * q * q * q + 0.5) * rSoft;
+          }
+        }
+      }
+      (void)eq; (void)eA; (void)eB;
+      etile += 0.5 * sd * pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
+    }
+    float fr;
+    if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
+    else fr = f * rinv;
+    fx += fr * dx;
+    fy += fr * dy;
+    fz += fr * dz;
+  }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
 __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
-  __shared__ uint4 s_pos[4][2][kBlockAtoms];
-  __shared__ int2 s_par[4][2][kBlockAtoms];
-  __shared__ float4 s_f[4][2][kBlockAtoms];
-  __shared__ float s_lo[4][2][kBlockAtoms];
+  __shared__ uint4 s_pos[4][kBlockAtoms];
+  __shared__ int2 s_par[4][kBlockAtoms];
+  __shared__ float s_lo[4][kBlockAtoms];
+  __shared__ float2 s_vdw[kVdwLds];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int half = lane >> 5, li = lane & 31;
+  const int nt2 = d.ntypes * d.ntypes;
+  const bool vdwLds = nt2 <= kVdwLds;
+  if (vdwLds)
+    for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_vdw[k] = d.vdw[k];
+  __syncthreads();
   const NbConst c = make_const(d);
-  const float sx = d.fracscale[0], sy = d.fracscale[1], sz = d.fracscale[2];
-  const int ntiles = *d.ntiles;
-  const int nchunks = (ntiles + kTileChunk - 1) / kTileChunk;
+  const bool fits = *d.ntiles <= d.maxtiles && *d.nchunks <= d.maxchunks;
+  const int nchunks = fits ? *d.nchunks : 0;
   double eacc = 0.0;
-  uint4 *sp = s_pos[w][half];
-  int2 *spr = s_par[w][half];
-  float4 *sf = s_f[w][half];
-  float *slo = s_lo[w][half];
+  uint4 *sp = s_pos[w];
+  int2 *spr = s_par[w];
+  float *slo = s_lo[w];
 
-  for (;;) {
-    int chunk = 0;
-    if (lane == 0) chunk = atomicAdd(&d.workctr[0], 1);
-    chunk = __shfl(chunk, 0, 64);
-    if (chunk >= nchunks) break;
-    const int t0 = chunk * kTileChunk;
-    const int cnt = min(kTileChunk, ntiles - t0);
+  // one chunk per wave; waves past the chunk count of this step exit
+  {
+    const int ch = blockIdx.x * 4 + w;
+    if (ch >= nchunks) return;
+    const int4 cr = d.chunk[ch];
+    const int I = cr.x, t0 = cr.y, cnt = cr.z;
+    IAtom ia;
+    {
+      const int si = I * kBlockAtoms + li;
+      ia.p = d.spos[si];
+      const int2 pr = d.spar[si];
+      ia.t = pr.x;
+      ia.b = pr.y;
+      ia.q = __uint_as_float(ia.p.w) * kElectricF;
+      ia.qd = ENERGY ? kElectricD * ((double)__uint_as_float(ia.p.w) + (double)d.slo[si]) : 0.0;
+      ia.lam = (ALCH && (ia.b & 0xFFFF)) ? d.lambdaf[ia.b & 0xFFFF] : 1.f;
+    }
+    const uint4 ci = d.bbc[I], hi = d.bbh[I];
+    int tJ = 0, tE = -1;
     bool keep = false;
-    if (lane < cnt) keep = boxes_within(d, d.tile_i[t0 + lane], d.tile_j[t0 + lane], c.rc2);
+    if (lane < cnt) {
+      tJ = d.tile_j[t0 + lane];
+      tE = d.tile_ex[t0 + lane];
+      keep = boxes_within(d, I, tJ, c.rc2);
+    }
     unsigned long long act = __ballot(keep);
-
-    int curI = -1, ai = -1;
-    uint4 pi = make_uint4(0, 0, 0, 0);
-    int ti = -1, bi = 0;
-    float qi = 0.f, lami = 1.f;
-    double qid = 0.0;
-    long long fix_x = 0, fix_y = 0, fix_z = 0;
+    long long ax = 0, ay = 0, az = 0;
     float fli = 0.f;
-
-    while (true) {
-      int tA = -1, tB = -1;
-      if (act) { tA = __ffsll((long long)act) - 1; act &= act - 1; }
-      if (act) { tB = __ffsll((long long)act) - 1; act &= act - 1; }
-      const int myt = half ? tB : tA;
-      const bool hv = myt >= 0;
-      const int I = hv ? d.tile_i[t0 + myt] : -2;
-      // flush the i accumulators when this half moves to another i-block
-      if (I != curI) {
-        if (curI >= 0 && ai >= 0) {
-          atomic_fix(&d.force[ai], fix_x);
-          atomic_fix(&d.force[d.n + ai], fix_y);
-          atomic_fix(&d.force[2 * d.n + ai], fix_z);
-          if (ALCH && (bi & 0xFFFF) && fli != 0.f) atomic_fix_f(&d.dudl[bi & 0xFFFF], fli);
-        }
-        fix_x = fix_y = fix_z = 0;
-        fli = 0.f;
-        curI = I;
-        ai = -1;
-        if (hv) {
-          const int si = I * kBlockAtoms + li;
-          pi = d.spos[si];
-          int2 pr = d.spar[si];
-          ti = pr.x;
-          bi = pr.y;
-          ai = ti >= 0 ? d.l2g[si] : -1;
-          qi = __uint_as_float(pi.w) * kElectricF;
-          if (ENERGY) qid = kElectricD * ((double)__uint_as_float(pi.w) + (double)d.slo[si]);
-          lami = 1.f;
-          if (ALCH && (bi & 0xFFFF)) lami = d.lambdaf[bi & 0xFFFF];
-        }
+    // software pipeline: the j slots, exclusion word and alchemical flag of
+    // the next tile are loaded while the current tile computes
+    int k = act ? __ffsll((long long)act) - 1 : -1;
+    if (k >= 0) act &= act - 1;
+    int J = 0;
+    uint4 pj = make_uint4(0, 0, 0, 0);
+    int2 prj = make_int2(-1, 0);
+    float lo = 0.f;
+    unsigned em = 0u;
+    unsigned alchJ = 0u;
+    auto fetch = [&](int kk, int &J_, uint4 &pj_, int2 &prj_, float &lo_, unsigned &em_, unsigned &al_) {
+      J_ = __builtin_amdgcn_readlane(tJ, kk);
+      const int ex = __builtin_amdgcn_readlane(tE, kk);
+      const int sj = J_ * kBlockAtoms + li;
+      pj_ = d.spos[sj];
+      prj_ = d.spar[sj];
+      if (ENERGY) lo_ = d.slo[sj];
+      em_ = ex >= 0 ? d.emask[ex * kBlockAtoms + li] : 0u;
+      al_ = ALCH ? d.bbh[J_].w : 0u;
+    };
+    if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ);
+    while (k >= 0) {
+      const bool near = half == 0 && prj.x >= 0 && near_box(d, ci, hi, pj, c.rc2);
+      const unsigned jm = (unsigned)__ballot(near);
+      if (half == 0) {
+        sp[li] = pj;
+        spr[li] = prj;
+        if (ENERGY) slo[li] = lo;
       }
-      if (tA < 0) break;  // both halves done (tB < 0 implies tA < 0 next)
-
-      int J = -1, ex = -1;
-      if (hv) {
-        J = d.tile_j[t0 + myt];
-        ex = d.tile_ex[t0 + myt];
-        const int sj = J * kBlockAtoms + li;
-        sp[li] = d.spos[sj];
-        spr[li] = d.spar[sj];
-        if (ENERGY) slo[li] = d.slo[sj];
+      const int Jc = J;
+      const unsigned emc = em, alchJc = alchJ;
+      const int kn = act ? __ffsll((long long)act) - 1 : -1;
+      if (kn >= 0) {
+        act &= act - 1;
+        fetch(kn, J, pj, prj, lo, em, alchJ);
       }
-      sf[li] = make_float4(0.f, 0.f, 0.f, 0.f);
-      const unsigned emask = (ex >= 0) ? d.emask[ex * kBlockAtoms + li] : 0u;
-      const bool self = (I == J);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
+      wave_lds_sync();
       float fx = 0.f, fy = 0.f, fz = 0.f;
       double etile = 0.0;
-      const bool ivalid = hv && ti >= 0;
-      for (int k = 0; k < kBlockAtoms; k++) {
-        const int jl = (li + k) & (kBlockAtoms - 1);
-        const uint4 pj = sp[jl];
-        const int2 prj = spr[jl];
-        bool ok = ivalid && prj.x >= 0 && !((emask >> jl) & 1u);
-        if (self) ok = ok && k > 0 && (li + k) < kBlockAtoms;
-        const float dx = fdisp(pi.x, pj.x, sx);
-        const float dy = fdisp(pi.y, pj.y, sy);
-        const float dz = fdisp(pi.z, pj.z, sz);
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (ok && r2 < c.rc2) {
-          const float2 lj = d.vdw[ti * d.ntypes + prj.x];
-          float rinv = __frsqrt_rn(r2);
-          float r = r2 * rinv;
-          const float qq = qi * __uint_as_float(pj.w);
-          float s = 1.f, dredr = 1.f, dredll = 0.f, lamj = 1.f;
-          const int bj = prj.y;
-          const bool alchPair = ALCH && (bi | bj);
-          if (ALCH && alchPair) {
-            if (bj & 0xFFFF) lamj = d.lambdaf[bj & 0xFFFF];
-            const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-            s = same ? (bi == bj ? lami : 0.f) : lami * lamj;
-            if (SOFT) {
-              const float rSoft = kSoftCoreRadius * (1.f - s);
-              if (r < rSoft) {
-                const float q = r / rSoft;
-                float re = 1.f - 0.5f * q;
-                re = re * q * q * q + 0.5f;
-                dredr = (3.f - 2.f * q) * q * q;
-                dredll = (re - dredr * q) * (-kSoftCoreRadius);
-                r = re * rSoft;
-                rinv = 1.f / r;
-              }
-            }
-          }
-          float f, eq, eA, eB;
-          pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, eq, eA, eB);
-          const float e = qq * eq + lj.x * eA - lj.y * eB;
-          if (ALCH) f *= s;
-          if (ALCH && alchPair) {
-            const float fl = SOFT ? e + f * dredll : e;
-            const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-            if (same) {
-              if (bi == bj) fli += fl;
-            } else {
-              fli += lamj * fl;
-              atomicAdd(&sf[jl].w, lami * fl);
-            }
-          }
-          if (ENERGY) {
-            // FP64 pair energy from the exact integer displacement and FP64
-            // charges / LJ parameters
-            const double2 ljd = d.vdwd[ti * d.ntypes + prj.x];
-            const double qjd = (double)__uint_as_float(pj.w) + (double)slo[jl];
-            const double ddx = (double)(int)(pi.x - pj.x) * d.ddscale[0];
-            const double ddy = (double)(int)(pi.y - pj.y) * d.ddscale[1];
-            const double ddz = (double)(int)(pi.z - pj.z) * d.ddscale[2];
-            double rd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            double sd = 1.0;
-            if (ALCH && alchPair) {
-              const int bj = prj.y;
-              const double li_d = (bi & 0xFFFF) ? d.lambda[bi & 0xFFFF] : 1.0;
-              const double lj_d = (bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0;
-              const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-              sd = same ? (bi == bj ? li_d : 0.0) : li_d * lj_d;
-              if (SOFT) {
-                const double rSoft = 4.0 * (1.0 - sd);
-                if (rd < rSoft) {
-                  const double q = rd / rSoft;
-                  rd = ((1.0 - 0.5 * q) * q * q * q + 0.5) * rSoft;
-                }
-              }
-            }
-            (void)eq; (void)eA; (void)eB;
-            etile += sd * pair_energy_d<ELEC, VDW>(d, qid * qjd, ljd.x, ljd.y, rd);
-          }
-          float fr;
-          if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
-          else fr = f * rinv;
-          const float gx = fr * dx, gy = fr * dy, gz = fr * dz;
-          fx += gx; fy += gy; fz += gz;
-          atomicAdd(&sf[jl].x, -gx);
-          atomicAdd(&sf[jl].y, -gy);
-          atomicAdd(&sf[jl].z, -gz);
-        }
-      }
+      const bool alchTile = ALCH && (hi.w | alchJc);
+      if (alchTile)
+        tile_pairs<ELEC, VDW, ENERGY, ALCH, SOFT>(d, c, ia, li, half, sp, spr, slo, s_vdw, vdwLds, jm, emc,
+                                                  I == Jc, fx, fy, fz, fli, etile);
+      else
+        tile_pairs<ELEC, VDW, ENERGY, false, false>(d, c, ia, li, half, sp, spr, slo, s_vdw, vdwLds, jm, emc,
+                                                    I == Jc, fx, fy, fz, fli, etile);
+      ax += fix_f(fx);
+      ay += fix_f(fy);
+      az += fix_f(fz);
       if (ENERGY) eacc += etile;
-      fix_x += fix_f(fx);
-      fix_y += fix_f(fy);
-      fix_z += fix_f(fz);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (hv) {
-        const int sj = J * kBlockAtoms + li;
-        const int aj = d.l2g[sj];
-        if (aj >= 0) {
-          const float4 fj = sf[li];
-          atomic_fix_f(&d.force[aj], fj.x);
-          atomic_fix_f(&d.force[d.n + aj], fj.y);
-          atomic_fix_f(&d.force[2 * d.n + aj], fj.z);
-          const int bj = spr[li].y;
-          if (ALCH && (bj & 0xFFFF) && fj.w != 0.f) atomic_fix_f(&d.dudl[bj & 0xFFFF], fj.w);
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
+      k = kn;
+    }
+    // both halves hold partials of the same i-atoms
+    ax += __shfl_xor(ax, 32, 64);
+    ay += __shfl_xor(ay, 32, 64);
+    az += __shfl_xor(az, 32, 64);
+    if (half == 0) {
+      long long *dst = d.ichunk + (long long)ch * 3 * kBlockAtoms;
+      dst[li] = ax;
+      dst[kBlockAtoms + li] = ay;
+      dst[2 * kBlockAtoms + li] = az;
+    }
+    if (ALCH) {
+      fli += __shfl_xor(fli, 32, 64);
+      if (half == 0 && (ia.b & 0xFFFF) && fli != 0.f) atomic_fix_f(&d.dudl[ia.b & 0xFFFF], fli);
     }
   }
   if (ENERGY) {
@@ -400,9 +466,42 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   }
 }
 
+// Per block: sum the fixed-point i-force partials of the row's chunks and
+// add them to the force array (one thread per atom; kernels on the system
+// stream run in order, so a plain read-modify-write is safe).
+__global__ void __launch_bounds__(256) nb_reduce(Dev d) {
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63, half = lane >> 5, li = lane & 31;
+  if (I >= *d.nblocks) return;
+  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
+  const int c0 = d.cstart[I], c1 = d.cstart[I + 1];
+  long long sx = 0, sy = 0, sz = 0;
+  for (int c = c0 + half; c < c1; c += 2) {
+    const long long *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
+    sx += src[li];
+    sy += src[kBlockAtoms + li];
+    sz += src[2 * kBlockAtoms + li];
+  }
+  sx += __shfl_xor(sx, 32, 64);
+  sy += __shfl_xor(sy, 32, 64);
+  sz += __shfl_xor(sz, 32, 64);
+  if (half == 0) {
+    const int a = d.l2g[I * kBlockAtoms + li];
+    if (a >= 0) {
+      d.force[a] += sx;
+      d.force[d.n + a] += sy;
+      d.force[2 * d.n + a] += sz;
+    }
+  }
+}
+
+// one wave per chunk slot of the list capacity; waves past this step's
+// chunk count exit at once
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
 static void launch_t(const Dev &d, int nwg, hipStream_t st) {
-  nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT><<<nwg, 256, 0, st>>>(d);
+  (void)nwg;
+  nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT><<<(d.maxchunks + 3) / 4, 256, 0, st>>>(d);
+  nb_reduce<<<(d.maxblocks + 3) / 4, 256, 0, st>>>(d);
 }
 
 template <int ELEC, int VDW>

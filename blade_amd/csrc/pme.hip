@@ -2,14 +2,19 @@
 // (restates nbrecip/nbrecip.cu:19-485 and the grid setup of
 // system/potential.cxx:1202-1307; cuFFT is replaced by our own FFT).
 //
-//   pme_spread   lambda*q B-spline spread into a 64-bit fixed-point grid
-//                (order-independent sums: bit-reproducible)
+//   pme_bin      atoms -> bins of the grid brick holding their first
+//                stencil point
+//   pme_spread   one workgroup per brick: lambda*q B-spline weights summed
+//                as 64-bit fixed point in LDS (order-independent sums:
+//                bit-reproducible), brick stored to the real grid
 //   fft_lines    mixed-radix Stockham FFT of grid lines staged in LDS;
-//                pass z reads the fixed-point grid (and clears it for the
-//                next step), pass x fuses forward FFT, the convolution
+//                pass z reads the real grid, pass x fuses forward FFT, the
+//                convolution
 //                B(m) kE/(2 pi V) exp(-pi^2 m^2/beta^2)/m^2 and the inverse
 //   pme_gather   potential and gradient at each atom -> force, dU/dlambda,
 //                energy; the Ewald self term rides along
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace blade {
@@ -48,31 +53,154 @@ __device__ __forceinline__ void grid_coord(unsigned u, int K, int &u0, T &frac) 
   frac = (T)((unsigned)(p & 0xFFFFFFFFull)) * (T(1) / T(4294967296.0));
 }
 
+// Brick of grid point u on an axis of K points cut into nb bricks; brick b
+// covers [b*K/nb, (b+1)*K/nb).
+__device__ __forceinline__ int brick_of(int u, int K, int nb) {
+  int b = (u * nb) / K;
+  if (b + 1 < nb && ((b + 1) * K) / nb <= u) b++;
+  return b;
+}
+
+// Bin every charged atom by the brick holding its first stencil point.
+// Threads walk the list slots (spatially ordered), so a wave's atoms share
+// few bricks and one atomic serves each brick.  Bin order is arbitrary;
+// the spread sums in fixed point, so the grid does not depend on it.
+__global__ void __launch_bounds__(256) pme_bin(Dev d) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = s < *d.nblocks * kBlockAtoms ? d.l2g[s] : -1;
+  bool act = false;
+  int br = 0;
+  if (i >= 0) {
+    const int b = d.sb[i] & 0xFFFF;
+    act = d.qd[i] * (b ? d.lambda[b] : 1.0) != 0.0;
+    int bb[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const unsigned u = frac_u32(d.pos[k * d.n + i], d.invbox[k]);
+      const int u0 = (int)(((unsigned long long)u * (unsigned)d.K[k]) >> 32);
+      bb[k] = brick_of(u0, d.K[k], d.pnb[k]);
+    }
+    br = (bb[0] * d.pnb[1] + bb[1]) * d.pnb[2] + bb[2];
+  }
+  const int slot = wave_claim(d.pbcount, br, act);
+  if (act) {
+    if (slot < d.pcap) d.pbatom[(long long)br * d.pcap + slot] = i;
+    else atomicOr(d.errflag, 2);
+  }
+}
+
+// candidate atoms staged per round (keeps LDS under 64 KB for order 8)
+template <int N> constexpr int spread_stage() { return N <= 6 ? 256 : 128; }
+
+// One workgroup (256 threads) per brick.  Every brick spans >= order-1
+// points (or a whole axis), so only atoms binned in this brick or the
+// previous one along each axis reach it.  Each round stages up to 256
+// (order 8: 128) of those atoms at once (FP64 lambda*q and B-spline
+// weights), then all threads add the weights inside the brick as 64-bit
+// fixed point in LDS.  The brick is stored to the real grid with plain
+// stores: no global atomics, and no clearing pass.
 template <int N>
 __global__ void __launch_bounds__(256) pme_spread(Dev d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t / (N * N);
-  if (i >= d.n) return;
-  const int a = (t / N) % N, c = t % N;
-  const int b = d.sb[i] & 0xFFFF;
-  const double lq = d.qd[i] * (b ? d.lambda[b] : 1.0);
-  if (lq == 0.0) return;
-  int u0[3];
-  double fr[3];
-  for (int k = 0; k < 3; k++) grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0[k], fr[k]);
-  // FP64 charge and weights: the grid is the one input of the energy sum
-  double wx[N], wy[N], wz[N], dw[N];
-  bspline_w<N>(fr[0], wx, dw);
-  bspline_w<N>(fr[1], wy, dw);
-  bspline_w<N>(fr[2], wz, dw);
-  int ix = u0[0] + a; if (ix >= d.K[0]) ix -= d.K[0];
-  int iy = u0[1] + c; if (iy >= d.K[1]) iy -= d.K[1];
-  const double wac = lq * wx[a] * wy[c];
-  long long *row = d.qgrid + ((long long)ix * d.K[1] + iy) * d.K[2];
+  constexpr int kStage = spread_stage<N>();
+  extern __shared__ long long acc[];
+  __shared__ double s_w[3][N][kStage];
+  __shared__ short s_u[3][kStage];
+  __shared__ double s_lq[kStage];
+  __shared__ int s_cnt[9];
+  __shared__ int s_bin[8];
+  int bidx[3], lo[3], sz[3];
+  {
+    int r = blockIdx.x;
+    bidx[2] = r % d.pnb[2]; r /= d.pnb[2];
+    bidx[1] = r % d.pnb[1];
+    bidx[0] = r / d.pnb[1];
+  }
 #pragma unroll
-  for (int e = 0; e < N; e++) {
-    int iz = u0[2] + e; if (iz >= d.K[2]) iz -= d.K[2];
-    atomic_fix(&row[iz], fix_d(wac * wz[e]));
+  for (int k = 0; k < 3; k++) {
+    lo[k] = (bidx[k] * d.K[k]) / d.pnb[k];
+    sz[k] = ((bidx[k] + 1) * d.K[k]) / d.pnb[k] - lo[k];
+  }
+  const int rowz = sz[2] + 1;   // padded row: fewer LDS bank conflicts
+  const int npts = sz[0] * sz[1] * rowz;
+  for (int p = threadIdx.x; p < npts; p += blockDim.x) acc[p] = 0;
+  // feeding bins (this brick and the previous one along each axis, each
+  // once) and their atom counts
+  if (threadIdx.x < 8) {
+    const int f = threadIdx.x;
+    int fb[3];
+    bool dup = false;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int off = (f >> k) & 1;
+      if (off && d.pnb[k] == 1) dup = true;
+      fb[k] = bidx[k] - off < 0 ? d.pnb[k] - 1 : bidx[k] - off;
+    }
+    const int br = (fb[0] * d.pnb[1] + fb[1]) * d.pnb[2] + fb[2];
+    s_bin[f] = br;
+    s_cnt[f] = dup ? 0 : min(d.pbcount[br], d.pcap);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int f = 0; f < 8; f++) {
+      const int c = s_cnt[f];
+      s_cnt[f] = run;
+      run += c;
+    }
+    s_cnt[8] = run;
+  }
+  __syncthreads();
+  const int total = s_cnt[8];
+  for (int base = 0; base < total; base += kStage) {
+    const int na = min(kStage, total - base);
+    if ((int)threadIdx.x < na) {
+      const int a = threadIdx.x, g = base + a;
+      int f = 0;
+      while (f < 7 && s_cnt[f + 1] <= g) f++;
+      const int i = d.pbatom[(long long)s_bin[f] * d.pcap + (g - s_cnt[f])];
+      const int b = d.sb[i] & 0xFFFF;
+      s_lq[a] = d.qd[i] * (b ? d.lambda[b] : 1.0);
+      double w[N], dw[N];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        int u0;
+        double fr;
+        grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0, fr);
+        bspline_w<N>(fr, w, dw);
+#pragma unroll
+        for (int t = 0; t < N; t++) s_w[k][t][a] = w[t];
+        // stencil start relative to the brick, in [0, K)
+        int r = u0 - lo[k];
+        if (r < 0) r += d.K[k];
+        s_u[k][a] = (short)r;
+      }
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < na * N * N; it += blockDim.x) {
+      const int a = it / (N * N), xa = (it / N) % N, yc = it % N;
+      int rx = s_u[0][a] + xa;
+      if (rx >= d.K[0]) rx -= d.K[0];
+      int ry = s_u[1][a] + yc;
+      if (ry >= d.K[1]) ry -= d.K[1];
+      if (rx >= sz[0] || ry >= sz[1]) continue;
+      const double wac = s_lq[a] * s_w[0][xa][a] * s_w[1][yc][a];
+      long long *row = acc + (rx * sz[1] + ry) * rowz;
+      const int rz0 = s_u[2][a];
+#pragma unroll
+      for (int e = 0; e < N; e++) {
+        int rz = rz0 + e;
+        if (rz >= d.K[2]) rz -= d.K[2];
+        if (rz < sz[2])
+          atomicAdd(reinterpret_cast<unsigned long long *>(row + rz), (unsigned long long)fix_d(wac * s_w[2][e][a]));
+      }
+    }
+    __syncthreads();
+  }
+  for (int p = threadIdx.x; p < sz[0] * sz[1] * sz[2]; p += blockDim.x) {
+    const int pz = p % sz[2], pxy = p / sz[2];
+    const int py = pxy % sz[1], px = pxy / sz[1];
+    d.rgrid[((long long)(lo[0] + px) * d.K[1] + lo[1] + py) * d.K[2] + lo[2] + pz] =
+        (float)((double)acc[pxy * rowz + pz] * kFixInvD);
   }
 }
 
@@ -178,7 +306,7 @@ constexpr int kFftLds = 2048;  // complex elements per LDS buffer
 struct FftArgs {
   FftPlan plan;
   int axis;        // 0 = x, 1 = y, 2 = z
-  int mode;        // 0 fwd from fixed-point grid, 1 fwd, 2 inverse, 3 fwd+conv+inv
+  int mode;        // 0 fwd from the real grid, 1 fwd, 2 inverse, 3 fwd+conv+inv
   int energy;      // mode 3: accumulate the reciprocal energy
 };
 
@@ -208,9 +336,7 @@ __global__ void __launch_bounds__(256) fft_lines(Dev d, FftArgs a) {
     const long long g = line_addr(d, a.axis, l0 + line, m);
     float2 v;
     if (a.mode == 0) {
-      long long q = d.qgrid[g];
-      d.qgrid[g] = 0;
-      v = make_float2((float)((double)q * kFixInvD), 0.f);
+      v = make_float2(d.rgrid[g], 0.f);
     } else {
       v = d.cgrid[g];
     }
@@ -262,6 +388,8 @@ __global__ void __launch_bounds__(256) fft_lines(Dev d, FftArgs a) {
 template <int N>
 __global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // the spread has consumed the bins: clear them for the next step
+  if (i < d.pnb[0] * d.pnb[1] * d.pnb[2]) d.pbcount[i] = 0;
   double e = 0.0, eself = 0.0;
   if (i < d.n) {
     const int b = d.sb[i] & 0xFFFF;
@@ -295,9 +423,11 @@ __global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
       }
     }
     const float s = 2.f * l * q;
-    atomic_fix_f(&d.force[i], s * gx * (d.K[0] * (float)d.invbox[0]));
-    atomic_fix_f(&d.force[d.n + i], s * gy * (d.K[1] * (float)d.invbox[1]));
-    atomic_fix_f(&d.force[2 * d.n + i], s * gz * (d.K[2] * (float)d.invbox[2]));
+    // one thread per atom, kernels on the system stream run in order: a
+    // plain read-modify-write of the fixed-point force is safe
+    d.force[i] += fix_f(s * gx * (d.K[0] * (float)d.invbox[0]));
+    d.force[d.n + i] += fix_f(s * gy * (d.K[1] * (float)d.invbox[1]));
+    d.force[2 * d.n + i] += fix_f(s * gz * (d.K[2] * (float)d.invbox[2]));
     // dU/dlambda_b = 2 q phi (nbrecip.cu:445-449); the energy is summed in
     // k-space by the convolution pass
     if (b) atomic_fix(&d.dudl[b], fix_d(2.0 * d.qd[i] * (double)phi));
@@ -329,11 +459,14 @@ bool pme_grid_supported(int K) {
 
 void launch_pme(const Dev &d, bool energy, hipStream_t st) {
   const int N = d.order;
-  const long long nt = (long long)d.n * N * N;
-  const unsigned gs = (unsigned)((nt + 255) / 256);
-  if (N == 4) pme_spread<4><<<gs, 256, 0, st>>>(d);
-  else if (N == 6) pme_spread<6><<<gs, 256, 0, st>>>(d);
-  else pme_spread<8><<<gs, 256, 0, st>>>(d);
+  const int nbr = d.pnb[0] * d.pnb[1] * d.pnb[2];
+  int bmax = 1;
+  for (int k = 0; k < 3; k++) bmax *= (d.K[k] + d.pnb[k] - 1) / d.pnb[k] + (k == 2);
+  const size_t lds = (size_t)bmax * sizeof(long long);
+  pme_bin<<<(unsigned)((32LL * d.maxblocks + 255) / 256), 256, 0, st>>>(d);
+  if (N == 4) pme_spread<4><<<nbr, 256, lds, st>>>(d);
+  else if (N == 6) pme_spread<6><<<nbr, 256, lds, st>>>(d);
+  else pme_spread<8><<<nbr, 256, lds, st>>>(d);
   const int G = d.K[0] * d.K[1] * d.K[2];
   const float2 *tw[3] = {d.twiddle, d.twiddle + d.K[0], d.twiddle + d.K[0] + d.K[1]};
   FftArgs a;
@@ -351,7 +484,7 @@ void launch_pme(const Dev &d, bool energy, hipStream_t st) {
     const int nlines = G / K;
     fft_lines<<<(nlines + nl - 1) / nl, 256, 0, st>>>(d, a);
   }
-  const unsigned ga = (unsigned)((d.n + 255) / 256);
+  const unsigned ga = (unsigned)((std::max(d.n, nbr) + 255) / 256);
   if (N == 4) pme_gather<4><<<ga, 256, 0, st>>>(d, energy);
   else if (N == 6) pme_gather<6><<<ga, 256, 0, st>>>(d, energy);
   else pme_gather<8><<<ga, 256, 0, st>>>(d, energy);

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -54,7 +55,7 @@ struct bm_system {
   int maxBlocks = 0;
   long long maxTiles = 0;
   bool alch = false, soft = false, bias = false;
-  int nwgNb = 1024;
+  int nwgNb = 0;
   int freqDomdec = 10;
   std::vector<void *> allocs;
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -91,6 +92,7 @@ struct bm_system {
   }
 
   ~bm_system() {
+    if (st) (void)hipStreamSynchronize(st);   // no graph may still run
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++) {
         if (graph[a][b]) (void)hipGraphExecDestroy(graph[a][b]);
@@ -98,7 +100,6 @@ struct bm_system {
       }
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
-    if (st) (void)hipStreamSynchronize(st);
     for (void *p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -164,8 +165,19 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   }
   s->maxBlocks = n / kBlockAtoms + 1 + d.ncol;
   d.maxblocks = s->maxBlocks;
-  s->maxTiles = (long long)s->maxBlocks * (s->maxBlocks + 1) / 2;
+  // directed tiles: blocks within rcut+pad of a block (its box edge about
+  // the column width a) fill a sphere of radius rcut+pad+sqrt(3)a; 1.5x
+  // margin, at most every pair of blocks
+  {
+    const double V = rp->box[0] * rp->box[1] * rp->box[2];
+    const double a = std::cbrt(32.0 * V / n);
+    const double R = rl + std::sqrt(3.0) * a;
+    const double perRow = 1.5 * (4.0 / 3.0) * M_PI * R * R * R / (32.0 * V / n) + 16;
+    const long long MBl = s->maxBlocks;
+    s->maxTiles = std::min(MBl * MBl, MBl * (long long)std::ceil(perRow));
+  }
   d.maxtiles = s->maxTiles;
+  d.maxchunks = (int)(s->maxTiles / kTileChunk + s->maxBlocks + 1);
   const int MB = s->maxBlocks;
   int rc = 0;
 #define ALLOC(p, c) \
@@ -255,7 +267,13 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.bucket, std::max(n, d.ncol));
   ALLOC(d.blkstart, d.ncol + 1);
   ALLOC(d.nblocks, 1);
-  ALLOC(d.l2g, (size_t)MB * kBlockAtoms);
+  {
+    // the first rebuild walks atoms in index order; later ones walk the
+    // previous rebuild's slots
+    std::vector<int> id((size_t)MB * kBlockAtoms, -1);
+    for (int i = 0; i < n; i++) id[i] = i;
+    UPLOAD(d.l2g, id);
+  }
   ALLOC(d.g2s, n);
   ALLOC(d.bblo, 3 * (size_t)MB);
   ALLOC(d.bbhi, 3 * (size_t)MB);
@@ -272,10 +290,17 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.tile_j, s->maxTiles + 1);
   ALLOC(d.tile_ex, s->maxTiles + 1);
   ALLOC(d.tile_flag, s->maxTiles + 1);
+  ALLOC(d.ccount, MB);
+  ALLOC(d.cstart, MB + 1);
+  ALLOC(d.nchunks, 1);
+  ALLOC(d.chunk, d.maxchunks);
+  ALLOC(d.ichunk, (size_t)d.maxchunks * 3 * kBlockAtoms);
+  ALLOC(d.ecount, MB);
+  ALLOC(d.estart, MB + 1);
   ALLOC(d.nexcltiles, 1);
   d.nexcl = t->n_excl;
-  ALLOC(d.emask, (size_t)std::max(1, t->n_excl) * kBlockAtoms);
-  ALLOC(d.excl_tile, t->n_excl);
+  ALLOC(d.emask, 2 * (size_t)std::max(1, t->n_excl) * kBlockAtoms);
+  ALLOC(d.excl_tile, 2 * (size_t)t->n_excl);
   {
     std::vector<int> ex(t->excl_idx, t->excl_idx + 2 * (size_t)t->n_excl);
     UPLOAD(d.excl, ex);
@@ -287,8 +312,18 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     for (int k = 0; k < 3; k++) d.K[k] = rp->grid[k];
     d.order = rp->order;
     const size_t G = (size_t)d.K[0] * d.K[1] * d.K[2];
-    ALLOC(d.qgrid, G);
+    ALLOC(d.rgrid, G);
     ALLOC(d.cgrid, G);
+    // spread bricks: floor(K/8) per axis (at least one), so each spans >= 8
+    // points (>= order-1 for orders up to 8, or the whole axis)
+    int nbr = 1;
+    for (int k = 0; k < 3; k++) {
+      d.pnb[k] = std::max(1, d.K[k] / 8);
+      nbr *= d.pnb[k];
+    }
+    d.pcap = 64 + 4 * ((n + nbr - 1) / nbr);
+    ALLOC(d.pbcount, nbr);
+    ALLOC(d.pbatom, (size_t)nbr * d.pcap);
     std::vector<float2> tw;
     for (int k = 0; k < 3; k++)
       for (int m = 0; m < d.K[k]; m++) {
@@ -451,8 +486,6 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     d.shaketol = rp->shake_tol;
     d.seed = rp->seed;
   }
-  // nbdirect launch width: enough waves to fill 256 CUs several times
-  s->nwgNb = 1024;
 #undef ALLOC
 #undef UPLOAD
   if (s->alch) launch_theta_to_lambda(d, s->st);
@@ -473,7 +506,9 @@ int bm_sync(bm_system *s) {
   HIPCHK(hipStreamSynchronize(s->st));
   int flag = 0;
   HIPCHK(hipMemcpy(&flag, s->d.errflag, sizeof(int), hipMemcpyDeviceToHost));
-  if (flag) return set_err(BM_EOVERFLOW, "a domdec column exceeded its capacity");
+  if (flag & 1) return set_err(BM_EOVERFLOW, "a domdec column exceeded its capacity");
+  if (flag & 2) return set_err(BM_EOVERFLOW, "a PME spread brick exceeded its atom capacity");
+  if (flag & 4) return set_err(BM_EOVERFLOW, "the tile list exceeded its capacity");
   return BM_OK;
 }
 
@@ -616,6 +651,47 @@ int bm_get_kernel_times(bm_system *s, double *ms_nb, double *ms_pme, double *ms_
   if (ms_total) *ms_total = s->msTotal;
   if (n) *n = s->nMeas;
   return BM_OK;
+}
+
+}  // extern "C"
+
+__global__ void bm_calibration_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" {
+
+int bm_calibration_copy(int device, size_t bytes, int reps, double *out) {
+  if (!out || reps <= 0 || bytes < 16) return set_err(BM_EINVAL, "calibration copy arguments");
+  HIPCHK(hipSetDevice(device));
+  const size_t n = bytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = BM_OK;
+  float ms = 0;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess ||
+      hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = set_err(BM_EHIP, "calibration copy allocation");
+  } else {
+    const int grid = 256 * 32;
+    hipMemsetAsync(a, 1, n * 16, st);
+    bm_calibration_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);   // warm
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++) bm_calibration_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = set_err(BM_EHIP, "calibration copy timing");
+    out[0] = ms / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 static int d2h(bm_system *s, void *dst, const void *src, size_t bytes) {

@@ -129,104 +129,95 @@ __global__ void upd_o_drift(Dev d) {
 }
 
 // ---- constraints ------------------------------------------------------
-struct Grp {
-  int na, nc;
-  int a[4];
-  int ca[3], cb[3];
-  double b0[3];
-};
+// Group shapes are compile-time constants so every per-atom array stays in
+// registers.  Constraint c of a group joins atoms ca(c) and cb(c):
+// triangles (0,1),(0,2),(1,2); branch groups atom 0 to atom c+1.
+template <bool TRI> __device__ constexpr int ca_of(int c) { return TRI ? (c < 2 ? 0 : 1) : 0; }
+template <bool TRI> __device__ constexpr int cb_of(int c) { return TRI ? (c < 2 ? c + 1 : 2) : c + 1; }
 
-__device__ Grp load_group(const Dev &d, int g, int &kind) {
-  Grp G;
-  if (g < d.ntri) {
-    kind = 0;
-    int4 ix = d.tri_i[g];
-    double4 b = d.tri_b[g];
-    G = Grp{3, 3, {ix.x, ix.y, ix.z, -1}, {0, 0, 1}, {1, 2, 2}, {b.x, b.y, b.z}};
-    return G;
-  }
-  g -= d.ntri;
-  if (g < d.nbr1) {
-    kind = 1;
-    int2 ix = d.br1_i[g];
-    G = Grp{2, 1, {ix.x, ix.y, -1, -1}, {0, 0, 0}, {1, 0, 0}, {d.br1_b[g], 0, 0}};
-    return G;
-  }
-  g -= d.nbr1;
-  if (g < d.nbr2) {
-    kind = 2;
-    int4 ix = d.br2_i[g];
-    double2 b = d.br2_b[g];
-    G = Grp{3, 2, {ix.x, ix.y, ix.z, -1}, {0, 0, 0}, {1, 2, 0}, {b.x, b.y, 0}};
-    return G;
-  }
-  g -= d.nbr2;
-  kind = 3;
-  int4 ix = d.br3_i[g];
-  double4 b = d.br3_b[g];
-  G = Grp{4, 3, {ix.x, ix.y, ix.z, ix.w}, {0, 0, 0}, {1, 2, 3}, {b.x, b.y, b.z}};
-  return G;
+// Coefficient of lambda_e in constraint c for inverse masses im: the
+// overlap of the atoms of constraints c and e with signs.
+template <bool TRI>
+__device__ __forceinline__ double coupling(int c, int e, const double *im) {
+  const int a = ca_of<TRI>(c), b = cb_of<TRI>(c), p = ca_of<TRI>(e), q = cb_of<TRI>(e);
+  double coef = 0;
+  if (a == p) coef += im[a];
+  if (a == q) coef -= im[a];
+  if (b == p) coef -= im[b];
+  if (b == q) coef += im[b];
+  return coef;
 }
 
-__device__ void solve3d(int nc, double A[3][3], const double *b, double *x) {
-  double M[3][4];
-  for (int i = 0; i < nc; i++) {
-    for (int j = 0; j < nc; j++) M[i][j] = A[i][j];
-    M[i][nc] = b[i];
+// Gaussian elimination with partial pivoting (the pivot is the first row
+// of largest magnitude, as in the oracle), fully unrolled.
+template <int NC>
+__device__ __forceinline__ void solve_small(double (&A)[NC][NC], const double (&b)[NC], double (&x)[NC]) {
+  double M[NC][NC + 1];
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+#pragma unroll
+    for (int j = 0; j < NC; j++) M[i][j] = A[i][j];
+    M[i][NC] = b[i];
   }
-  for (int c = 0; c < nc; c++) {
-    int p = c;
-    for (int r = c + 1; r < nc; r++)
-      if (fabs(M[r][c]) > fabs(M[p][c])) p = r;
-    if (p != c)
-      for (int j = 0; j <= nc; j++) { double t = M[c][j]; M[c][j] = M[p][j]; M[p][j] = t; }
-    for (int r = c + 1; r < nc; r++) {
-      double f = M[r][c] / M[c][c];
-      for (int j = c; j <= nc; j++) M[r][j] -= f * M[c][j];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+#pragma unroll
+    for (int r = c + 1; r < NC; r++) {
+      if (fabs(M[r][c]) > fabs(M[c][c])) {
+#pragma unroll
+        for (int j = 0; j <= NC; j++) { const double t = M[c][j]; M[c][j] = M[r][j]; M[r][j] = t; }
+      }
+    }
+#pragma unroll
+    for (int r = c + 1; r < NC; r++) {
+      const double f = M[r][c] / M[c][c];
+#pragma unroll
+      for (int j = c; j <= NC; j++) M[r][j] -= f * M[c][j];
     }
   }
-  for (int r = nc - 1; r >= 0; r--) {
-    double s = M[r][nc];
-    for (int j = r + 1; j < nc; j++) s -= M[r][j] * x[j];
+#pragma unroll
+  for (int r = NC - 1; r >= 0; r--) {
+    double s = M[r][NC];
+#pragma unroll
+    for (int j = r + 1; j < NC; j++) s -= M[r][j] * x[j];
     x[r] = s / M[r][r];
   }
 }
 
-// Newton-Raphson SHAKE over <=3 constraints (holonomic.cu:430-600)
-__device__ void shake_newton(const Grp &g, D3 *x, const D3 *xp, const double *im, double tol) {
-  D3 gp[3];
-  for (int c = 0; c < g.nc; c++) gp[c] = xp[g.ca[c]] - xp[g.cb[c]];
+// Newton-Raphson SHAKE over NC <= 3 constraints (holonomic.cu:430-600)
+template <int NA, int NC, bool TRI>
+__device__ void shake_newton(const double *b0, D3 (&x)[NA], const D3 (&xp)[NA], const double (&im)[NA], double tol) {
+  D3 gp[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) gp[c] = xp[ca_of<TRI>(c)] - xp[cb_of<TRI>(c)];
   for (int it = 0;; it++) {
-    double res[3];
+    double res[NC];
     bool done = true;
-    for (int c = 0; c < g.nc; c++) {
-      D3 dd = x[g.ca[c]] - x[g.cb[c]];
-      res[c] = g.b0[c] * g.b0[c] - dotd(dd, dd);
-      if (fabs(res[c]) >= 2 * g.b0[c] * g.b0[c] * tol) done = false;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const D3 dd = x[ca_of<TRI>(c)] - x[cb_of<TRI>(c)];
+      res[c] = b0[c] * b0[c] - dotd(dd, dd);
+      if (fabs(res[c]) >= 2 * b0[c] * b0[c] * tol) done = false;
     }
     if (done || it >= 10) break;
-    double A[3][3], lam[3];
-    for (int c = 0; c < g.nc; c++) {
-      D3 dd = x[g.ca[c]] - x[g.cb[c]];
-      for (int e = 0; e < g.nc; e++) {
-        double coef = 0;
-        if (g.ca[c] == g.ca[e]) coef += im[g.ca[c]];
-        if (g.ca[c] == g.cb[e]) coef -= im[g.ca[c]];
-        if (g.cb[c] == g.ca[e]) coef -= im[g.cb[c]];
-        if (g.cb[c] == g.cb[e]) coef += im[g.cb[c]];
-        A[c][e] = 2 * coef * dotd(dd, gp[e]);
-      }
+    double A[NC][NC], lam[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const D3 dd = x[ca_of<TRI>(c)] - x[cb_of<TRI>(c)];
+#pragma unroll
+      for (int e = 0; e < NC; e++) A[c][e] = 2 * coupling<TRI>(c, e, im) * dotd(dd, gp[e]);
     }
-    solve3d(g.nc, A, res, lam);
-    for (int e = 0; e < g.nc; e++) {
-      x[g.ca[e]] = x[g.ca[e]] + (lam[e] * im[g.ca[e]]) * gp[e];
-      x[g.cb[e]] = x[g.cb[e]] + (-lam[e] * im[g.cb[e]]) * gp[e];
+    solve_small<NC>(A, res, lam);
+#pragma unroll
+    for (int e = 0; e < NC; e++) {
+      x[ca_of<TRI>(e)] = x[ca_of<TRI>(e)] + (lam[e] * im[ca_of<TRI>(e)]) * gp[e];
+      x[cb_of<TRI>(e)] = x[cb_of<TRI>(e)] + (-lam[e] * im[cb_of<TRI>(e)]) * gp[e];
     }
   }
 }
 
 // analytic SETTLE for an isosceles rigid triangle (the oracle's settle())
-__device__ void settle(const double *b0, D3 *x, const D3 *xp, const double *mass) {
+__device__ void settle(const double *b0, D3 (&x)[3], const D3 (&xp)[3], const double (&mass)[3]) {
   const double mA = mass[0], mB = mass[1], M = mA + 2 * mB;
   const double rc = 0.5 * b0[2];
   const double h = sqrt(b0[0] * b0[0] - rc * rc);
@@ -248,6 +239,7 @@ __device__ void settle(const double *b0, D3 *x, const D3 *xp, const double *mass
   const double m[3] = {mA, mB, mB};
   const D3 u[3] = {a1, b1, c1};
   double al = 0, be = 0, ga = 0;
+#pragma unroll
   for (int i = 0; i < 3; i++) {
     double px = dotd(p[i], ex), py = dotd(p[i], ey);
     double ux = dotd(u[i], ex), uy = dotd(u[i], ey);
@@ -259,42 +251,94 @@ __device__ void settle(const double *b0, D3 *x, const D3 *xp, const double *mass
   const double sgn = al > 0 ? 1 : -1;
   const double sth = (al * ga - sgn * be * sqrt(a2b2 - ga * ga)) / a2b2;
   const double cth = sqrt(1 - sth * sth);
+#pragma unroll
   for (int i = 0; i < 3; i++) {
     double fx = qx[i] * cth - qy[i] * sth, fy = qx[i] * sth + qy[i] * cth;
     x[i] = dcm + fx * ex + fy * ey + qz[i] * ez;
   }
 }
 
-__device__ void constrain_group(const Dev &d, int gi) {
-  int kind;
-  Grp g = load_group(d, gi, kind);
-  D3 xp[4], xn[4], sh[4];
-  double im[4], m[4];
-  const D3 p0 = getp(d.posb, d.n, g.a[0]);
-  for (int k = 0; k < g.na; k++) {
-    D3 pk = getp(d.posb, d.n, g.a[k]);
-    sh[k] = k ? (minimg(d, pk - p0) - (pk - p0)) : d3(0, 0, 0);
-    xp[k] = pk + sh[k];
-    xn[k] = getp(d.pos, d.n, g.a[k]) + sh[k];
-    m[k] = d.mass[g.a[k]];
-    im[k] = 1.0 / m[k];
+// Positions of one group: old (backup) positions xp and new ones xn, both
+// shifted to the periodic image nearest atom 0 (shift sh).
+template <int NA>
+struct GroupPos {
+  D3 xp[NA], xn[NA], sh[NA];
+  double m[NA], im[NA];
+};
+
+template <int NA>
+__device__ __forceinline__ void load_pos(const Dev &d, const int (&a)[NA], GroupPos<NA> &g) {
+  const D3 p0 = getp(d.posb, d.n, a[0]);
+#pragma unroll
+  for (int k = 0; k < NA; k++) {
+    const D3 pk = getp(d.posb, d.n, a[k]);
+    g.sh[k] = k ? (minimg(d, pk - p0) - (pk - p0)) : d3(0, 0, 0);
+    g.xp[k] = pk + g.sh[k];
+    g.xn[k] = getp(d.pos, d.n, a[k]) + g.sh[k];
+    g.m[k] = d.mass[a[k]];
+    g.im[k] = 1.0 / g.m[k];
   }
-  if (kind == 0 && m[1] == m[2] && g.b0[0] == g.b0[1]) {
-    settle(g.b0, xn, xp, m);
-  } else if (kind == 1) {
-    D3 gp = xp[0] - xp[1], s = xn[0] - xn[1];
-    double gg = dotd(gp, gp), sg = dotd(s, gp), ss = dotd(s, s), b02 = g.b0[0] * g.b0[0];
-    double lw = (-sg + sqrt(sg * sg - gg * (ss - b02))) / (gg * (im[0] + im[1]));
-    xn[0] = xn[0] + (im[0] * lw) * gp;
-    xn[1] = xn[1] + (-im[1] * lw) * gp;
-  } else {
-    shake_newton(g, xn, xp, im, d.shaketol);
-  }
+}
+
+template <int NA>
+__device__ __forceinline__ void store_pos(const Dev &d, const int (&a)[NA], const GroupPos<NA> &g) {
   const double ih = 1.0 / d.halfdt;
-  for (int k = 0; k < g.na; k++) {
-    setp(d.vel, d.n, g.a[k], ih * (xn[k] - xp[k]));
-    setp(d.pos, d.n, g.a[k], xn[k] - sh[k]);
+#pragma unroll
+  for (int k = 0; k < NA; k++) {
+    setp(d.vel, d.n, a[k], ih * (g.xn[k] - g.xp[k]));
+    setp(d.pos, d.n, a[k], g.xn[k] - g.sh[k]);
   }
+}
+
+__device__ void constrain_group(const Dev &d, int gi) {
+  if (gi < d.ntri) {
+    const int4 ix = d.tri_i[gi];
+    const double4 bb = d.tri_b[gi];
+    const int a[3] = {ix.x, ix.y, ix.z};
+    const double b0[3] = {bb.x, bb.y, bb.z};
+    GroupPos<3> g;
+    load_pos(d, a, g);
+    if (g.m[1] == g.m[2] && b0[0] == b0[1]) settle(b0, g.xn, g.xp, g.m);
+    else shake_newton<3, 3, true>(b0, g.xn, g.xp, g.im, d.shaketol);
+    store_pos(d, a, g);
+    return;
+  }
+  gi -= d.ntri;
+  if (gi < d.nbr1) {
+    const int2 ix = d.br1_i[gi];
+    const int a[2] = {ix.x, ix.y};
+    const double b02 = d.br1_b[gi] * d.br1_b[gi];
+    GroupPos<2> g;
+    load_pos(d, a, g);
+    const D3 gp = g.xp[0] - g.xp[1], s = g.xn[0] - g.xn[1];
+    const double gg = dotd(gp, gp), sg = dotd(s, gp), ss = dotd(s, s);
+    const double lw = (-sg + sqrt(sg * sg - gg * (ss - b02))) / (gg * (g.im[0] + g.im[1]));
+    g.xn[0] = g.xn[0] + (g.im[0] * lw) * gp;
+    g.xn[1] = g.xn[1] + (-g.im[1] * lw) * gp;
+    store_pos(d, a, g);
+    return;
+  }
+  gi -= d.nbr1;
+  if (gi < d.nbr2) {
+    const int4 ix = d.br2_i[gi];
+    const double2 bb = d.br2_b[gi];
+    const int a[3] = {ix.x, ix.y, ix.z};
+    const double b0[2] = {bb.x, bb.y};
+    GroupPos<3> g;
+    load_pos(d, a, g);
+    shake_newton<3, 2, false>(b0, g.xn, g.xp, g.im, d.shaketol);
+    store_pos(d, a, g);
+    return;
+  }
+  gi -= d.nbr2;
+  const int4 ix = d.br3_i[gi];
+  const double4 bb = d.br3_b[gi];
+  const int a[4] = {ix.x, ix.y, ix.z, ix.w};
+  const double b0[3] = {bb.x, bb.y, bb.z};
+  GroupPos<4> g;
+  load_pos(d, a, g);
+  shake_newton<4, 3, false>(b0, g.xn, g.xp, g.im, d.shaketol);
+  store_pos(d, a, g);
 }
 
 // theta -> lambda for site s (msld.cu:863-898)
@@ -319,7 +363,7 @@ __device__ void theta_to_lambda(const Dev &d, int s) {
 }
 
 // groups, then (last call of a step) sites and the step counter
-__global__ void upd_constrain(Dev d, int finish) {
+__global__ void __launch_bounds__(64) upd_constrain(Dev d, int finish) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
   if (t < ng) constrain_group(d, t);
@@ -335,39 +379,65 @@ __global__ void upd_theta_to_lambda(Dev d) {
   if (s < d.nsites) theta_to_lambda(d, s);
 }
 
-// RATTLE velocity projection (holonomic.cu:15-211) for every group
-__global__ void upd_vconstrain(Dev d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.ntri + d.nbr1 + d.nbr2 + d.nbr3) return;
-  int kind;
-  Grp g = load_group(d, t, kind);
-  D3 xs[4], vs[4];
-  double im[4];
-  for (int k = 0; k < g.na; k++) {
-    xs[k] = getp(d.pos, d.n, g.a[k]);
-    vs[k] = getp(d.vel, d.n, g.a[k]);
-    im[k] = 1.0 / d.mass[g.a[k]];
+// RATTLE velocity projection (holonomic.cu:15-211) of one group
+template <int NA, int NC, bool TRI>
+__device__ void vproject(const Dev &d, const int (&a)[NA]) {
+  D3 xs[NA], vs[NA];
+  double im[NA];
+#pragma unroll
+  for (int k = 0; k < NA; k++) {
+    xs[k] = getp(d.pos, d.n, a[k]);
+    vs[k] = getp(d.vel, d.n, a[k]);
+    im[k] = 1.0 / d.mass[a[k]];
   }
-  D3 gc[3];
-  for (int c = 0; c < g.nc; c++) gc[c] = minimg(d, xs[g.ca[c]] - xs[g.cb[c]]);
-  double A[3][3], b[3], lam[3];
-  for (int c = 0; c < g.nc; c++) {
-    b[c] = -dotd(vs[g.ca[c]] - vs[g.cb[c]], gc[c]);
-    for (int e = 0; e < g.nc; e++) {
-      double coef = 0;
-      if (g.ca[c] == g.ca[e]) coef += im[g.ca[c]];
-      if (g.ca[c] == g.cb[e]) coef -= im[g.ca[c]];
-      if (g.cb[c] == g.ca[e]) coef -= im[g.cb[c]];
-      if (g.cb[c] == g.cb[e]) coef += im[g.cb[c]];
-      A[c][e] = coef * dotd(gc[c], gc[e]);
-    }
+  D3 gc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) gc[c] = minimg(d, xs[ca_of<TRI>(c)] - xs[cb_of<TRI>(c)]);
+  double A[NC][NC], b[NC], lam[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    b[c] = -dotd(vs[ca_of<TRI>(c)] - vs[cb_of<TRI>(c)], gc[c]);
+#pragma unroll
+    for (int e = 0; e < NC; e++) A[c][e] = coupling<TRI>(c, e, im) * dotd(gc[c], gc[e]);
   }
-  solve3d(g.nc, A, b, lam);
-  for (int e = 0; e < g.nc; e++) {
-    vs[g.ca[e]] = vs[g.ca[e]] + (lam[e] * im[g.ca[e]]) * gc[e];
-    vs[g.cb[e]] = vs[g.cb[e]] + (-lam[e] * im[g.cb[e]]) * gc[e];
+  solve_small<NC>(A, b, lam);
+#pragma unroll
+  for (int e = 0; e < NC; e++) {
+    vs[ca_of<TRI>(e)] = vs[ca_of<TRI>(e)] + (lam[e] * im[ca_of<TRI>(e)]) * gc[e];
+    vs[cb_of<TRI>(e)] = vs[cb_of<TRI>(e)] + (-lam[e] * im[cb_of<TRI>(e)]) * gc[e];
   }
-  for (int k = 0; k < g.na; k++) setp(d.vel, d.n, g.a[k], vs[k]);
+#pragma unroll
+  for (int k = 0; k < NA; k++) setp(d.vel, d.n, a[k], vs[k]);
+}
+
+__global__ void __launch_bounds__(64) upd_vconstrain(Dev d) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < d.ntri) {
+    const int4 ix = d.tri_i[t];
+    const int a[3] = {ix.x, ix.y, ix.z};
+    vproject<3, 3, true>(d, a);
+    return;
+  }
+  t -= d.ntri;
+  if (t < d.nbr1) {
+    const int2 ix = d.br1_i[t];
+    const int a[2] = {ix.x, ix.y};
+    vproject<2, 1, false>(d, a);
+    return;
+  }
+  t -= d.nbr1;
+  if (t < d.nbr2) {
+    const int4 ix = d.br2_i[t];
+    const int a[3] = {ix.x, ix.y, ix.z};
+    vproject<3, 2, false>(d, a);
+    return;
+  }
+  t -= d.nbr2;
+  if (t < d.nbr3) {
+    const int4 ix = d.br3_i[t];
+    const int a[4] = {ix.x, ix.y, ix.z, ix.w};
+    vproject<4, 3, false>(d, a);
+  }
 }
 
 __global__ void upd_kinetic(Dev d) {
@@ -388,19 +458,21 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
   const int N = 3 * d.n + d.nl;
   const unsigned gN = (N + 255) / 256;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  const unsigned gG = (ng + 255) / 256 > 0 ? (ng + 255) / 256 : 1;
-  const unsigned gF = (ng + d.nsites + 1 + 255) / 256;
+  // constraint groups: 64-thread workgroups spread the few thousand groups
+  // over many CUs
+  const unsigned gG = (ng + 63) / 64 > 0 ? (ng + 63) / 64 : 1;
+  const unsigned gF = (ng + d.nsites + 1 + 63) / 64;
   if (energy_step) {
     upd_kick<<<gN, 256, 0, st>>>(d, 1);
-    if (ng) upd_vconstrain<<<gG, 256, 0, st>>>(d);
+    if (ng) upd_vconstrain<<<gG, 64, 0, st>>>(d);
     upd_kinetic<<<gN, 256, 0, st>>>(d);
     upd_kick<<<gN, 256, 0, st>>>(d, 2);
   } else {
     upd_kick<<<gN, 256, 0, st>>>(d, 0);
   }
-  if (ng) upd_constrain<<<gG, 256, 0, st>>>(d, 0);
+  if (ng) upd_constrain<<<gG, 64, 0, st>>>(d, 0);
   upd_o_drift<<<gN, 256, 0, st>>>(d);
-  upd_constrain<<<gF, 256, 0, st>>>(d, 1);
+  upd_constrain<<<gF, 64, 0, st>>>(d, 1);
 }
 
 __global__ void upd_set_step(Dev d, long long v) { *d.step = v; }

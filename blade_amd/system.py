@@ -273,7 +273,7 @@ def state(s, n, nblocks):
     return x.reshape(3, n).T.copy(), v.reshape(3, n).T.copy(), th, tv
 
 
-def relax(spec, schedule=RELAX_SCHEDULE, gpus=(0,), log=None):
+def relax(spec, schedule=RELAX_SCHEDULE, gpus=(0,), log=None, use_graph=True):
     """Copy of `spec` advanced through the relaxation schedule on the GPU
     (positions, velocities and theta replaced; run options kept)."""
     import copy
@@ -286,8 +286,8 @@ def relax(spec, schedule=RELAX_SCHEDULE, gpus=(0,), log=None):
         stage.gamma = gamma_ps / PICOSECOND
         s = build(stage, gpus)
         s.set_domdec_frequency(freq_dd)
-        s.run_dynamics(steps - 1, freq_nrg=0)
-        s.run_dynamics(1, freq_nrg=1)
+        s.run_dynamics(steps - 1, freq_nrg=0, use_graph=use_graph)
+        s.run_dynamics(1, freq_nrg=1, use_graph=use_graph)
         if log:
             e = s.energy()
             log("relax dt=%.1f fs gamma=%g/ps steps=%d: Epot %.1f KE %.1f" % (dt_fs, gamma_ps, steps, e[20], e[21]))

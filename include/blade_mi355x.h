@@ -226,7 +226,10 @@ int bm_get_lambda(bm_system *sys, double *lambda /*[nL]*/);
 int bm_get_energy(bm_system *sys, double *energy /*[BM_NTERMS]*/);
 int bm_get_nan_flag(bm_system *sys, int *flag);
 
-/* ---- domdec readback for bit-exact list checks */
+/* ---- domdec readback for bit-exact list checks.  Tiles are directed:
+   every (I,J) whose rebuild boxes lie within rcut+pad, both orders and
+   (I,I), sorted by I then J; mask word i of tile (I,J) has bit j set when
+   slot i of block I and slot j of block J are an excluded pair. */
 int bm_get_domdec_sizes(bm_system *sys, int *n_blocks, int *n_tiles, int *n_excl_tiles);
 int bm_get_domdec(bm_system *sys, int *local_to_global /*[32*nb], -1 = empty*/,
                   int *block_count /*[nb]*/, int *tile_i /*[nt]*/, int *tile_j /*[nt]*/,
@@ -240,6 +243,11 @@ int bm_get_kernel_times(bm_system *sys, double *ms_nbdirect, double *ms_pme,
  * stream around the nbdirect and PME launches; out[3] = average ms of
  * nbdirect, PME (spread+FFT+gather) and the whole step. */
 int bm_profile_steps(bm_system *sys, long step0, long nsteps, double *out);
+
+/* HBM calibration: `reps` launches of a plain 16-byte-vector copy kernel
+   (bm_calibration_copy_kernel) over `bytes` bytes on `device`; out[0] = mean
+   ms per launch.  Measurement support, no counterpart in the reference. */
+int bm_calibration_copy(int device, size_t bytes, int reps, double *out);
 
 /* raw fixed-point force accumulators [3N+nL] (bit-reproducibility checks) */
 int bm_get_force_fixed(bm_system *sys, long long *force);

@@ -1004,7 +1004,9 @@ long orc_domdec(const bm_topology *t, const bm_runparams *rp, double *pos, int n
       for (int l = 0; l < cnt; l++) slotOf[v[s + l]] = 32 * b + l;
     }
   }
-  // tile list: J >= I, periodic interval gap of the boxes <= rcut + pad
+  // directed tile list: for every block I, every block J (J == I included)
+  // whose rebuild box lies within rcut + pad (periodic interval gap), in
+  // ascending J; the box test is symmetric, so (I,J) and (J,I) both appear
   const double rl = rp->rcut + rp->cull_pad;
   const double rl2 = rl * rl;
   double scale[3];
@@ -1013,7 +1015,7 @@ long orc_domdec(const bm_topology *t, const bm_runparams *rp, double *pos, int n
   std::vector<long> firstTile(nb + 1);
   for (int I = 0; I < nb; I++) {
     firstTile[I] = nt;
-    for (int J = I; J < nb; J++) {
+    for (int J = 0; J < nb; J++) {
       bool keep = (I == J);
       if (!keep) {
         double g[3];
@@ -1040,16 +1042,20 @@ long orc_domdec(const bm_topology *t, const bm_runparams *rp, double *pos, int n
     }
   }
   firstTile[nb] = nt;
-  for (int e = 0; e < t->n_excl; e++) {
-    int sa = slotOf[t->excl_idx[2 * e]], sb = slotOf[t->excl_idx[2 * e + 1]];
-    if (sa / 32 > sb / 32) std::swap(sa, sb);
-    int I = sa / 32, J = sb / 32;
+  // exclusion masks: bit j of word i of tile (I,J) marks slots (I,i), (J,j)
+  auto tileOf = [&](int I, int J) -> long {
     const int *beg = tile_j + firstTile[I], *end = tile_j + firstTile[I + 1];
     const int *it = std::lower_bound(beg, end, J);
-    if (it == end || *it != J) continue;  // never within range of each other
-    long tt = firstTile[I] + (it - beg);
-    tile_mask[32 * tt + sa % 32] |= 1u << (sb % 32);
-    if (I == J) tile_mask[32 * tt + sb % 32] |= 1u << (sa % 32);
+    return (it == end || *it != J) ? -1 : firstTile[I] + (it - beg);
+  };
+  for (int e = 0; e < t->n_excl; e++) {
+    const int sa = slotOf[t->excl_idx[2 * e]], sb = slotOf[t->excl_idx[2 * e + 1]];
+    const int I = sa / 32, J = sb / 32;
+    const long ta = tileOf(I, J);
+    if (ta < 0) continue;  // never within range of each other
+    tile_mask[32 * ta + sa % 32] |= 1u << (sb % 32);
+    const long tb = tileOf(J, I);
+    tile_mask[32 * tb + sb % 32] |= 1u << (sa % 32);
   }
   return nt;
 }

@@ -64,8 +64,10 @@ void orc_fft3d(int kx, int ky, int kz, double *data, int sign);
  * domdec/assign_excl.cu:17-371) restated on integer fractional
  * coordinates.  Wraps pos in place.  Outputs (caller-sized):
  *   local_to_global[32*max_blocks] (-1 = empty slot), block_count[max_blocks],
- *   tile_i/tile_j[max_tiles], tile_mask[32*max_tiles] (bit j of word i set
- *   when slot i of block I and slot j of block J are an excluded pair).
+ *   tile_i/tile_j[max_tiles]: directed tiles, every (I,J) whose rebuild
+ *   boxes lie within rcut+pad (both orders, and (I,I)), by I then J;
+ *   tile_mask[32*max_tiles]: bit j of word i set when slot i of block I and
+ *   slot j of block J are an excluded pair.
  * Returns the number of tiles, and *n_blocks; -1 when a buffer is too small. */
 long orc_domdec(const bm_topology *top, const bm_runparams *rp, double *pos,
                 int ncol_x, int ncol_y, int max_blocks, long max_tiles,

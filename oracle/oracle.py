@@ -125,12 +125,13 @@ def domdec_columns(rp, n):
 
 
 def domdec(top, rp, pos):
-    """list semantics: (wrapped pos, l2g[32*nb], block_count[nb], tiles (nt,2), masks (nt,32))"""
+    """list semantics: (wrapped pos, l2g[32*nb], block_count[nb], directed tiles (nt,2),
+    masks (nt,32))"""
     t = top.contents
     n = t.n_atoms
     ncx, ncy = domdec_columns(rp, n)
     maxb = n // 32 + 1 + ncx * ncy
-    maxt = maxb * (maxb + 1) // 2
+    maxt = maxb * maxb
     pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
     l2g = np.zeros(32 * maxb, dtype=np.int32)
     bc = np.zeros(maxb, dtype=np.int32)
