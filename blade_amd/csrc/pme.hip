@@ -306,7 +306,8 @@ constexpr int kFftLds = 2048;  // complex elements per LDS buffer
 struct FftArgs {
   FftPlan plan;
   int axis;        // 0 = x, 1 = y, 2 = z
-  int mode;        // 0 fwd from the real grid, 1 fwd, 2 inverse, 3 fwd+conv+inv
+  int mode;        // 0 fwd from the real grid, 1 fwd, 2 inverse, 3 fwd+conv+inv,
+                   // 4 inverse with the real part stored to the real grid
   int energy;      // mode 3: accumulate the reciprocal energy
 };
 
@@ -377,13 +378,225 @@ __global__ void __launch_bounds__(256) fft_lines(Dev d, FftArgs a) {
     float2 *other = (res == buf0) ? buf1 : buf0;
     res = lds_fft(res, other, a.plan, nmine, 1.f);
   } else {
-    res = lds_fft(buf0, buf1, a.plan, nmine, a.mode == 2 ? 1.f : -1.f);
+    res = lds_fft(buf0, buf1, a.plan, nmine, a.mode >= 2 ? 1.f : -1.f);
   }
   for (int e = threadIdx.x; e < nmine * K; e += blockDim.x) {
     const int line = e % nmine, m = e / nmine;
-    d.cgrid[line_addr(d, a.axis, l0 + line, m)] = res[line * K + m];
+    const long long g = line_addr(d, a.axis, l0 + line, m);
+    if (a.mode == 4) d.rgrid[g] = res[line * K + m].x;
+    else d.cgrid[g] = res[line * K + m];
   }
 }
+
+// ---------------------------------------------------------------------
+// Compile-time FFT: transform size, radix sequence, line count and LDS
+// strides are template constants, so every index is shift/multiply
+// arithmetic.  Stockham stages ping-pong between two LDS buffers.
+// ---------------------------------------------------------------------
+constexpr float kC3 = -0.5f, kS3 = 0.86602540378443865f;                // 2 pi/3
+constexpr float kC51 = 0.30901699437494742f, kS51 = 0.95105651629515357f;  // 2 pi/5
+constexpr float kC52 = -0.80901699437494742f, kS52 = 0.58778525229247313f; // 4 pi/5
+constexpr float kR2 = 0.70710678118654752f;
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (s i) for s = +-1
+__device__ __forceinline__ float2 cmuli(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// length-R DFT in registers, X_k = sum_r v_r exp(sign 2 pi i r k / R)
+template <int R>
+__device__ __forceinline__ void ct_dft(float2 (&v)[R], float sign) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const float2 s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+    const float2 s1 = cadd(v[1], v[3]), d1 = cmuli(csub(v[1], v[3]), sign);
+    v[0] = cadd(s0, s1);
+    v[2] = csub(s0, s1);
+    v[1] = cadd(d0, d1);
+    v[3] = csub(d0, d1);
+  } else if constexpr (R == 8) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    ct_dft<4>(e, sign);
+    ct_dft<4>(o, sign);
+    // o_k * exp(sign 2 pi i k/8)
+    const float2 w1 = make_float2(kR2 * (o[1].x - sign * o[1].y), kR2 * (o[1].y + sign * o[1].x));
+    const float2 w2 = cmuli(o[2], sign);
+    const float2 w3 = make_float2(-kR2 * (o[3].x + sign * o[3].y), kR2 * (sign * o[3].x - o[3].y));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
+  } else if constexpr (R == 3) {
+    const float2 s = cadd(v[1], v[2]), dd = csub(v[1], v[2]);
+    const float2 m = make_float2(v[0].x + kC3 * s.x, v[0].y + kC3 * s.y);
+    const float2 r = cmuli(dd, sign * kS3);
+    v[0] = cadd(v[0], s);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  } else {
+    static_assert(R == 5, "radix 2, 3, 4, 5 or 8");
+    const float2 s1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+    const float2 s2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    const float2 m1 = make_float2(v[0].x + kC51 * s1.x + kC52 * s2.x, v[0].y + kC51 * s1.y + kC52 * s2.y);
+    const float2 m2 = make_float2(v[0].x + kC52 * s1.x + kC51 * s2.x, v[0].y + kC52 * s1.y + kC51 * s2.y);
+    const float2 r1 = make_float2(kS51 * d1.x + kS52 * d2.x, kS51 * d1.y + kS52 * d2.y);
+    const float2 r2 = make_float2(kS52 * d1.x - kS51 * d2.x, kS52 * d1.y - kS51 * d2.y);
+    const float2 q1 = cmuli(r1, sign), q2 = cmuli(r2, sign);
+    v[0] = make_float2(v[0].x + s1.x + s2.x, v[0].y + s1.y + s2.y);
+    v[1] = cadd(m1, q1);
+    v[4] = csub(m1, q1);
+    v[2] = cadd(m2, q2);
+    v[3] = csub(m2, q2);
+  }
+}
+
+constexpr int ct_radix(int n) {
+  return n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : 5;
+}
+constexpr bool ct_ok(int n) {
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+// One Stockham stage over NL lines of K points; element e of line l at
+// buf[l*LS + e*ES].  LINEFAST: consecutive threads take consecutive lines
+// (for ES > 1), otherwise consecutive butterflies of one line.
+template <int K, int R, int Ns, int NL, int ES, int LS, bool LINEFAST>
+__device__ __forceinline__ void ct_stage(const float2 *src, float2 *dst, const float2 *tw, float sign) {
+  constexpr int NB = K / R, TS = K / (Ns * R);
+  for (int b = threadIdx.x; b < NL * NB; b += blockDim.x) {
+    const int line = LINEFAST ? b % NL : b / NB;
+    const int j = LINEFAST ? b / NL : b % NB;
+    const float2 *in = src + line * LS;
+    float2 *out = dst + line * LS;
+    const int jm = j % Ns;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      v[r] = in[(j + r * NB) * ES];
+      if (Ns > 1 && r) {
+        // table: exp(-2 pi i m/K); the inverse transform conjugates it
+        float2 w = tw[jm * r * TS];
+        w.y *= -sign;
+        v[r] = make_float2(v[r].x * w.x - v[r].y * w.y, v[r].x * w.y + v[r].y * w.x);
+      }
+    }
+    ct_dft<R>(v, sign);
+    const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; r++) out[(idxD + r * Ns) * ES] = v[r];
+  }
+}
+
+// all stages; returns the buffer holding the result
+template <int K, int Ns, int Rem, int NL, int ES, int LS, bool LINEFAST>
+__device__ __forceinline__ float2 *ct_fft(float2 *src, float2 *dst, const float2 *tw, float sign) {
+  if constexpr (Rem == 1) {
+    return src;
+  } else {
+    constexpr int R = ct_radix(Rem);
+    ct_stage<K, R, Ns, NL, ES, LS, LINEFAST>(src, dst, tw, sign);
+    __syncthreads();
+    return ct_fft<K, Ns * R, Rem / R, NL, ES, LS, LINEFAST>(dst, src, tw, sign);
+  }
+}
+
+// Forward transform along z then y of one x-plane (K1 x K2, in LDS): reads
+// the real charge grid, writes the complex grid.
+template <int K1, int K2>
+__global__ void __launch_bounds__(256) fft_yz_fwd(Dev d) {
+  __shared__ float2 b0[K1 * K2], b1[K1 * K2];
+  __shared__ float2 t1[K1], t2[K2];
+  const long long base = (long long)blockIdx.x * K1 * K2;
+  for (int m = threadIdx.x; m < K1; m += blockDim.x) t1[m] = d.twiddle[d.K[0] + m];
+  for (int m = threadIdx.x; m < K2; m += blockDim.x) t2[m] = d.twiddle[d.K[0] + K1 + m];
+  for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) b0[e] = make_float2(d.rgrid[base + e], 0.f);
+  __syncthreads();
+  float2 *r = ct_fft<K2, 1, K2, K1, 1, K2, false>(b0, b1, t2, -1.f);
+  float2 *o = r == b0 ? b1 : b0;
+  r = ct_fft<K1, 1, K1, K2, K2, 1, true>(r, o, t1, -1.f);
+  for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) d.cgrid[base + e] = r[e];
+}
+
+// Inverse transform along y then z of one x-plane; the real part (the
+// potential on the grid) goes to the real grid.
+template <int K1, int K2>
+__global__ void __launch_bounds__(256) fft_yz_inv(Dev d) {
+  __shared__ float2 b0[K1 * K2], b1[K1 * K2];
+  __shared__ float2 t1[K1], t2[K2];
+  const long long base = (long long)blockIdx.x * K1 * K2;
+  for (int m = threadIdx.x; m < K1; m += blockDim.x) t1[m] = d.twiddle[d.K[0] + m];
+  for (int m = threadIdx.x; m < K2; m += blockDim.x) t2[m] = d.twiddle[d.K[0] + K1 + m];
+  for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) b0[e] = d.cgrid[base + e];
+  __syncthreads();
+  float2 *r = ct_fft<K1, 1, K1, K2, K2, 1, true>(b0, b1, t1, 1.f);
+  float2 *o = r == b0 ? b1 : b0;
+  r = ct_fft<K2, 1, K2, K1, 1, K2, false>(r, o, t2, 1.f);
+  for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) d.rgrid[base + e] = r[e].x;
+}
+
+// Lines along x (NL per workgroup): forward transform, the convolution
+// B(m) kE/(2 pi V) exp(-pi^2 m^2/beta^2)/m^2 (with the reciprocal energy
+// sum on energy steps), inverse transform.
+template <int K0, int NL>
+__global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
+  __shared__ float2 b0[K0 * NL], b1[K0 * NL];
+  __shared__ float2 t0[K0];
+  const int nlines = d.K[1] * d.K[2];
+  const int l0 = blockIdx.x * NL;
+  const long long plane = (long long)d.K[1] * d.K[2];
+  for (int m = threadIdx.x; m < K0; m += blockDim.x) t0[m] = d.twiddle[m];
+  // element m of line l at b[m*NL + l]: consecutive threads, consecutive lines
+  for (int e = threadIdx.x; e < K0 * NL; e += blockDim.x) {
+    const int l = e % NL, m = e / NL;
+    b0[e] = l0 + l < nlines ? d.cgrid[m * plane + l0 + l] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 *r = ct_fft<K0, 1, K0, NL, NL, 1, true>(b0, b1, t0, -1.f);
+  const float L0 = (float)d.box[0], L1 = (float)d.box[1], L2 = (float)d.box[2];
+  double ek = 0.0;
+  for (int e = threadIdx.x; e < K0 * NL; e += blockDim.x) {
+    const int l = e % NL, m = e / NL;
+    const int line = l0 + l;
+    if (line >= nlines) continue;
+    const int j = line / d.K[2], k = line - j * d.K[2];
+    const int mi = (2 * m <= K0) ? m : m - K0;
+    const int mj = (2 * j <= d.K[1]) ? j : j - d.K[1];
+    const int mk = (2 * k <= d.K[2]) ? k : k - d.K[2];
+    const float kx = mi / L0, ky = mj / L1, kz = mk / L2;
+    const float m2 = kx * kx + ky * ky + kz * kz;
+    float fac = 0.f;
+    if (m || j || k) fac = d.bmod[0][m] * d.bmod[1][j] * d.bmod[2][k] * d.pmefac * __expf(-d.pmeexp * m2) / m2;
+    const float2 v = r[e];
+    if (energy && (m || j || k)) {
+      const double kxd = mi / d.box[0], kyd = mj / d.box[1], kzd = mk / d.box[2];
+      const double m2d = kxd * kxd + kyd * kyd + kzd * kzd;
+      const double g = d.bmodd[0][m] * d.bmodd[1][j] * d.bmodd[2][k] * d.pmefacd * exp(-d.pmeexpd * m2d) / m2d;
+      ek += g * ((double)v.x * v.x + (double)v.y * v.y);
+    }
+    r[e] = make_float2(v.x * fac, v.y * fac);
+  }
+  if (energy) wave_atomic_fix(&d.energy[BM_ENBRECIP], (long long)__double2ll_rn(ek * kFixScaleD));
+  __syncthreads();
+  float2 *o = r == b0 ? b1 : b0;
+  r = ct_fft<K0, 1, K0, NL, NL, 1, true>(r, o, t0, 1.f);
+  for (int e = threadIdx.x; e < K0 * NL; e += blockDim.x) {
+    const int l = e % NL, m = e / NL;
+    if (l0 + l < nlines) d.cgrid[m * plane + l0 + l] = r[e];
+  }
+}
+
+// sizes with compiled plane / line kernels
+#define BM_FFT_SIZES(X) X(16) X(18) X(20) X(24) X(27) X(32) X(36) X(40) X(48) X(54) X(64)
 
 template <int N>
 __global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
@@ -407,12 +620,12 @@ __global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
       int ix = u0[0] + a; if (ix >= d.K[0]) ix -= d.K[0];
       for (int c = 0; c < N; c++) {
         int iy = u0[1] + c; if (iy >= d.K[1]) iy -= d.K[1];
-        const float2 *row = d.cgrid + ((long long)ix * d.K[1] + iy) * d.K[2];
+        const float *row = d.rgrid + ((long long)ix * d.K[1] + iy) * d.K[2];
         float sp = 0.f, sd = 0.f;
 #pragma unroll
         for (int t = 0; t < N; t++) {
           int iz = u0[2] + t; if (iz >= d.K[2]) iz -= d.K[2];
-          const float P = row[iz].x;
+          const float P = row[iz];
           sp += P * wz[t];
           sd += P * dz[t];
         }
@@ -467,22 +680,46 @@ void launch_pme(const Dev &d, bool energy, hipStream_t st) {
   if (N == 4) pme_spread<4><<<nbr, 256, lds, st>>>(d);
   else if (N == 6) pme_spread<6><<<nbr, 256, lds, st>>>(d);
   else pme_spread<8><<<nbr, 256, lds, st>>>(d);
-  const int G = d.K[0] * d.K[1] * d.K[2];
-  const float2 *tw[3] = {d.twiddle, d.twiddle + d.K[0], d.twiddle + d.K[0] + d.K[1]};
-  FftArgs a;
-  // forward z (from the fixed-point grid), forward y, x with convolution,
-  // inverse y, inverse z
-  const int seq_axis[5] = {2, 1, 0, 1, 2};
-  const int seq_mode[5] = {0, 1, 3, 2, 2};
-  for (int s = 0; s < 5; s++) {
-    a.axis = seq_axis[s];
-    a.mode = seq_mode[s];
-    a.energy = energy && a.mode == 3;
-    const int K = d.K[a.axis];
-    make_plan(a.plan, K, tw[a.axis]);
-    const int nl = kFftLds / K;
-    const int nlines = G / K;
-    fft_lines<<<(nlines + nl - 1) / nl, 256, 0, st>>>(d, a);
+  bool ct0 = false, ct1 = false;
+#define BM_IS(SZ) ct0 = ct0 || d.K[0] == SZ; ct1 = ct1 || d.K[1] == SZ;
+  BM_FFT_SIZES(BM_IS)
+#undef BM_IS
+  if (ct0 && ct1 && d.K[1] == d.K[2] && d.K[1] * d.K[2] <= 4096) {
+    // plane path: forward z+y per x-plane, x with convolution, inverse y+z
+    const unsigned gx = (unsigned)((d.K[1] * d.K[2] + 31) / 32);
+    switch (d.K[1]) {
+#define BM_YZF(SZ) case SZ: fft_yz_fwd<SZ, SZ><<<d.K[0], 256, 0, st>>>(d); break;
+      BM_FFT_SIZES(BM_YZF)
+#undef BM_YZF
+    }
+    switch (d.K[0]) {
+#define BM_XC(SZ) case SZ: fft_x_conv<SZ, 32><<<gx, 256, 0, st>>>(d, energy ? 1 : 0); break;
+      BM_FFT_SIZES(BM_XC)
+#undef BM_XC
+    }
+    switch (d.K[1]) {
+#define BM_YZI(SZ) case SZ: fft_yz_inv<SZ, SZ><<<d.K[0], 256, 0, st>>>(d); break;
+      BM_FFT_SIZES(BM_YZI)
+#undef BM_YZI
+    }
+  } else {
+    const int G = d.K[0] * d.K[1] * d.K[2];
+    const float2 *tw[3] = {d.twiddle, d.twiddle + d.K[0], d.twiddle + d.K[0] + d.K[1]};
+    FftArgs a;
+    // forward z (from the real grid), forward y, x with convolution,
+    // inverse y, inverse z (real part to the real grid)
+    const int seq_axis[5] = {2, 1, 0, 1, 2};
+    const int seq_mode[5] = {0, 1, 3, 2, 4};
+    for (int s = 0; s < 5; s++) {
+      a.axis = seq_axis[s];
+      a.mode = seq_mode[s];
+      a.energy = energy && a.mode == 3;
+      const int K = d.K[a.axis];
+      make_plan(a.plan, K, tw[a.axis]);
+      const int nl = kFftLds / K;
+      const int nlines = G / K;
+      fft_lines<<<(nlines + nl - 1) / nl, 256, 0, st>>>(d, a);
+    }
   }
   const unsigned ga = (unsigned)((std::max(d.n, nbr) + 255) / 256);
   if (N == 4) pme_gather<4><<<ga, 256, 0, st>>>(d, energy);

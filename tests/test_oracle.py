@@ -1,0 +1,425 @@
+"""Pins the FP64 oracle (CPU only).
+
+The reference ships no golden vectors for the force/integration path, so
+each oracle function is checked against something the mathematics fixes
+independently of its own formulas:
+
+  philox           Random123 known-answer vectors for Philox4x32-10
+  normal           moments of the Box-Muller stream
+  bspline          partition of unity, symmetry, closed-form values
+  fft3d            numpy.fft
+  PME              Madelung constant of rock salt; direct Ewald k-sum;
+                   direct real-space image sum
+  every force term central finite differences of the energy
+  dU/dlambda       finite differences in lambda
+  chain rule       finite differences through theta -> lambda
+  switch functions continuity of energy and force at rswitch, zero at rcut
+  SETTLE/SHAKE     constraint lengths after the position step; zero bond
+                   rate after the velocity projection
+  domdec           every pair within rcut lies in a listed tile; masks mark
+                   exactly the excluded pairs
+"""
+import math
+
+import numpy as np
+import pytest
+
+from blade_amd import synthetic, system
+from oracle import oracle as orc
+from tests.helpers import built, spec
+
+KE = 332.0716
+E_BOND, E_ANGLE, E_UREY, E_DIHE, E_IMPR, E_CMAP, E_NB14, E_NBDIRECT, E_NBRECIP, E_SELF, E_EXCL, E_LAMBDA = range(12)
+E_POT = 20
+
+
+# ---------------------------------------------------------------------------
+# random numbers
+# ---------------------------------------------------------------------------
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in kat:
+        assert orc.philox(ctr, key) == want
+
+
+def test_normal_moments():
+    x = np.array([orc.normal(0x1234, step, dof) for step in range(40) for dof in range(500)])
+    assert abs(x.mean()) < 0.02
+    assert abs(x.var() - 1) < 0.03
+    assert abs(np.mean(x ** 4) - 3) < 0.15
+    # the two normals of one Box-Muller pair are independent
+    a = np.array([orc.normal(7, s, 2 * k) for s in range(20) for k in range(200)])
+    b = np.array([orc.normal(7, s, 2 * k + 1) for s in range(20) for k in range(200)])
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.05
+
+
+# ---------------------------------------------------------------------------
+# B-splines and FFT
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [4, 6, 8])
+def test_bspline_identities(n):
+    for u in np.linspace(0.0, 1.0, 7, endpoint=False):
+        s = sum(orc.bspline(n, u + t) for t in range(n))
+        assert abs(s - 1) < 1e-14
+    for u in np.linspace(0.1, n - 0.1, 13):
+        assert abs(orc.bspline(n, u) - orc.bspline(n, n - u)) < 1e-14
+    assert orc.bspline(n, 0.0) == 0.0 and orc.bspline(n, float(n)) == 0.0
+
+
+def test_bspline_closed_form_cubic():
+    # M_4 at the integers: 1/6, 2/3, 1/6
+    assert abs(orc.bspline(4, 1.0) - 1 / 6) < 1e-15
+    assert abs(orc.bspline(4, 2.0) - 2 / 3) < 1e-15
+    assert abs(orc.bspline(4, 3.0) - 1 / 6) < 1e-15
+    # M_2 is the hat function
+    for u in (0.25, 0.5, 1.5):
+        assert abs(orc.bspline(2, u) - (1 - abs(u - 1))) < 1e-15
+
+
+def test_bspline_moduli_match_definition():
+    # 1/|sum_{k=0}^{n-2} M_n(k+1) exp(2 pi i m k / K)|^2
+    for n, K in ((4, 20), (6, 64), (8, 27)):
+        got = orc.bspline_moduli(n, K)
+        m = np.arange(K)
+        s = sum(orc.bspline(n, k + 1.0) * np.exp(2j * np.pi * m * k / K) for k in range(n - 1))
+        assert np.allclose(got, 1 / np.abs(s) ** 2, rtol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 10), (20, 20, 20), (16, 27, 18)])
+def test_fft3d_matches_numpy(shape):
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    assert np.allclose(orc.fft3d(g, -1), np.fft.fftn(g), atol=1e-9)
+    assert np.allclose(orc.fft3d(g, +1), np.fft.ifftn(g) * g.size, atol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Ewald electrostatics
+# ---------------------------------------------------------------------------
+def _ions(positions, charges, box, beta=0.34, rcut=10.0, rswitch=9.0, grid=32, order=6):
+    s = synthetic.Spec(name="ions", box=tuple(box))
+    s.types.update({"NA": 22.99, "CL": 35.45})
+    s.nb_par.update({"NA": (0.0, 1.5, 0.0, 1.5, 1.0, 0), "CL": (0.0, 2.0, 0.0, 2.0, 1.0, 0)})
+    for i, q in enumerate(charges):
+        t = "NA" if q > 0 else "CL"
+        s.atoms.append(("ION", str(i + 1), "ION", t, t, float(q), s.types[t]))
+    s.pos = np.asarray(positions, float)
+    s.atom_block = np.zeros(len(charges), dtype=np.int32)
+    s.shake = False
+    s.beta, s.rcut, s.rswitch, s.grid, s.order = beta, rcut, rswitch, (grid, grid, grid), order
+    return s
+
+
+def _rock_salt(ncell, a):
+    base_na = [(0, 0, 0), (0, .5, .5), (.5, 0, .5), (.5, .5, 0)]
+    pos, q = [], []
+    for i in range(ncell):
+        for j in range(ncell):
+            for k in range(ncell):
+                for b in base_na:
+                    p = (np.array(b) + (i, j, k)) * a
+                    pos.append(p)
+                    q.append(1.0)
+                    pos.append(p + (a / 2, 0, 0))
+                    q.append(-1.0)
+    return np.array(pos), np.array(q)
+
+
+def test_pme_madelung_rock_salt():
+    a, ncell = 5.64, 4
+    pos, q = _rock_salt(ncell, a)
+    box = (ncell * a,) * 3
+    s = system.build(_ions(pos, q, box))
+    s.prepare()
+    f, d, e, _ = orc.forces(s.topology(), s.runparams(), system.soa(pos), np.ones(1))
+    madelung = 1.7475645946331822
+    want = -(len(q) // 2) * madelung * KE / (a / 2)
+    got = e[E_NBDIRECT] + e[E_NBRECIP] + e[E_SELF]
+    assert abs(got - want) / abs(want) < 2e-6
+    # every ion of a perfect lattice is at a force minimum
+    assert np.abs(f).max() < 1e-4
+
+
+def _random_ions(n, box, seed):
+    rng = np.random.default_rng(seed)
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)   # neutral for even n
+    rng.shuffle(q)
+    # keep ions at least 2 A apart
+    pos = []
+    while len(pos) < n:
+        p = rng.uniform(0, box, size=3)
+        if all(np.linalg.norm((p - o) - box * np.round((p - o) / box)) > 2.0 for o in pos):
+            pos.append(p)
+    return np.array(pos), q
+
+
+def test_pme_reciprocal_matches_direct_kspace_sum():
+    box = 20.0
+    pos, q = _random_ions(40, box, 11)
+    s = system.build(_ions(pos, q, (box,) * 3, beta=0.34, rcut=9.0, rswitch=8.0, grid=40, order=8))
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    x = system.soa(pos)
+    fp, dp, ep, _ = orc.forces(top, rp, x, np.ones(1), orc.PME)
+    e, fd, dd = orc.ewald_recip_direct(top, rp, x, np.ones(1), 14)
+    assert abs(ep[E_NBRECIP] - e) / abs(e) < 1e-6
+    assert np.sqrt(np.mean((fp - fd) ** 2)) / np.sqrt(np.mean(fd ** 2)) < 1e-5
+
+
+def test_real_space_matches_image_sum():
+    box = 20.0
+    pos, q = _random_ions(30, box, 5)
+    s = system.build(_ions(pos, q, (box,) * 3, beta=0.34, rcut=9.0, rswitch=8.0, grid=20))
+    s.prepare()
+    f, d, e, _ = orc.forces(s.topology(), s.runparams(), system.soa(pos), np.ones(1), orc.NBDIRECT)
+    want = orc.ewald_real_images(q, system.soa(pos), np.array([box] * 3), 0.34, 9.0)
+    assert abs(e[E_NBDIRECT] - want) < 1e-9 * abs(want) + 1e-9
+
+
+# ---------------------------------------------------------------------------
+# finite differences of every term
+# ---------------------------------------------------------------------------
+TERM_GROUPS = [
+    ("bonded", orc.BONDED, [E_BOND, E_ANGLE, E_UREY, E_DIHE, E_IMPR]),
+    ("nb14", orc.NB14, [E_NB14]),
+    ("nbex", orc.NBEX, [E_EXCL]),
+    ("nbdirect", orc.NBDIRECT, [E_NBDIRECT]),
+    ("pme", orc.PME, [E_NBRECIP]),
+]
+
+
+def _fd_check(sp_name, terms, eidx, atoms, lam=None, h=1e-5, tol=2e-6):
+    s = built(sp_name)
+    top, rp = s.topology(), s.runparams()
+    n = spec(sp_name).n
+    x0 = system.soa(spec(sp_name).pos)
+    if lam is None:
+        lam = lambdas_of(sp_name, s)
+    f, _, _, _ = orc.forces(top, rp, x0, lam, terms)
+    scale = max(1.0, float(np.abs(f).max()))
+    worst = 0.0
+    for a in atoms:
+        for k in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[k * n + a] += h
+            xm[k * n + a] -= h
+            ep = sum(orc.forces(top, rp, xp, lam, terms)[2][i] for i in eidx)
+            em = sum(orc.forces(top, rp, xm, lam, terms)[2][i] for i in eidx)
+            fd = (ep - em) / (2 * h)
+            worst = max(worst, abs(fd - f[k * n + a]) / scale)
+    assert worst < tol, worst
+
+
+def lambdas_of(name, s):
+    t = s.topology().contents
+    if t.n_blocks <= 1:
+        return np.ones(1)
+    return orc.theta_to_lambda(s.topology(), s.runparams(), np.array(spec(name).theta0, float))
+
+
+def _polymer_atoms(name, k=6):
+    sp = spec(name)
+    idx = [i for i, a in enumerate(sp.atoms) if a[0] in ("POLY", "LIG")]
+    rng = np.random.default_rng(0)
+    return list(rng.choice(idx, size=min(k, len(idx)), replace=False)) + [idx[0]]
+
+
+@pytest.mark.parametrize("group", TERM_GROUPS, ids=[g[0] for g in TERM_GROUPS])
+def test_forces_are_energy_gradients(group):
+    name, terms, eidx = group
+    atoms = _polymer_atoms("msld_small")
+    if name in ("nbdirect", "pme"):
+        atoms = atoms[:3] + [0, 1, 2]
+    _fd_check("msld_small", terms, eidx, atoms)
+
+
+def test_dudl_matches_lambda_differences():
+    name = "msld_small"
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    x0 = system.soa(spec(name).pos)
+    lam = lambdas_of(name, s)
+    _, dudl, _, _ = orc.forces(top, rp, x0, lam)
+    h = 1e-6
+    for b in range(1, len(lam)):
+        lp, lm = lam.copy(), lam.copy()
+        lp[b] += h
+        lm[b] -= h
+        ep = orc.forces(top, rp, x0, lp)[2][E_POT]
+        em = orc.forces(top, rp, x0, lm)[2][E_POT]
+        assert abs((ep - em) / (2 * h) - dudl[b]) < 1e-4 * max(1.0, abs(dudl[b]))
+
+
+def test_chain_rule_matches_theta_differences():
+    name = "msld_small"
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    th = np.array(spec(name).theta0, float) + np.array([0.0, 0.3, -0.7, 1.1])[: len(spec(name).theta0)]
+    lam = orc.theta_to_lambda(top, rp, th)
+    dudl = np.array([0.0, 3.0, -2.0, 5.0])[: len(lam)]
+    got = orc.chain_rule(top, rp, th, lam, dudl)
+    h = 1e-6
+    for i in range(1, len(th)):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        dl = (orc.theta_to_lambda(top, rp, tp) - orc.theta_to_lambda(top, rp, tm)) / (2 * h)
+        assert abs(got[i] - np.dot(dudl, dl)) < 1e-7
+
+
+def test_theta_to_lambda_simplex():
+    name = "msld_small"
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    rng = np.random.default_rng(2)
+    for _ in range(10):
+        th = rng.uniform(-3, 3, size=len(spec(name).theta0))
+        lam = orc.theta_to_lambda(top, rp, th)
+        assert abs(lam[1:].sum() - 1) < 1e-14
+        assert np.all(lam[1:] > 0)
+
+
+# ---------------------------------------------------------------------------
+# switching functions
+# ---------------------------------------------------------------------------
+def _lj_pair(r, rcut=10.0, rswitch=8.5):
+    s = synthetic.Spec(name="lj", box=(30.0, 30.0, 30.0))
+    s.types.update({"AR": 39.9})
+    s.nb_par.update({"AR": (-0.238, 1.88, -0.238, 1.88, 1.0, 0)})
+    s.atoms = [("A", "1", "AR", "AR", "AR", 0.0, 39.9), ("A", "2", "AR", "AR", "AR", 0.0, 39.9)]
+    s.pos = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]])
+    s.atom_block = np.zeros(2, dtype=np.int32)
+    s.shake = False
+    s.rcut, s.rswitch, s.grid = rcut, rswitch, (30, 30, 30)
+    b = system.build(s)
+    b.prepare()
+    f, _, e, _ = orc.forces(b.topology(), b.runparams(), system.soa(s.pos), np.ones(1), orc.NBDIRECT)
+    return e[E_NBDIRECT], f[1]   # energy, dE/dx of atom 1
+
+
+def test_vdw_force_switch_continuity():
+    rc, rs = 10.0, 8.5
+    eps = 1e-7
+    for r0 in (rs, rc):
+        e1, f1 = _lj_pair(r0 - eps, rc, rs)
+        e2, f2 = _lj_pair(r0 + eps, rc, rs)
+        assert abs(e1 - e2) < 1e-8
+        assert abs(f1 - f2) < 1e-7
+    e, f = _lj_pair(rc - 1e-9, rc, rs)
+    assert abs(e) < 1e-9 and abs(f) < 1e-8
+    # inside the switch the pair force is the plain Lennard-Jones force
+    r = 4.0
+    e, f = _lj_pair(r, rc, rs)
+    eps_lj, rmin = 0.238, 2 * 1.88
+    flj = 12 * eps_lj * (rmin ** 6 / r ** 7 - rmin ** 12 / r ** 13)
+    assert abs(f - flj) < 1e-10 * abs(flj)
+
+
+def test_erfc_fit_error_bound():
+    import re
+    src = open(__file__.replace("tests/test_oracle.py", "blade_amd/csrc/erfc_coef.h")).read()
+    c = [float(v) for v in re.findall(r"([-+]?\d\.\d+(?:[eE][-+]?\d+)?)f", src.split("bm_erfc_c")[1])]
+    p = float(re.search(r"BM_ERFC_P\s+([\d.eE+-]+)f", src).group(1))
+    xmax = float(re.search(r"BM_ERFC_XMAX\s+([\d.eE+-]+)f", src).group(1))
+    worst = 0.0
+    for x in np.linspace(0, xmax, 2001):
+        t = 1 / (1 + p * x)
+        poly = sum(ck * t ** k for k, ck in enumerate(c))
+        worst = max(worst, abs(math.exp(-x * x) * poly - math.erfc(x)) / math.erfc(x))
+    assert worst < 2e-7
+
+
+# ---------------------------------------------------------------------------
+# constraints
+# ---------------------------------------------------------------------------
+def _constraint_pairs(top, triangles=True, branches=True):
+    t = top.contents
+    out = []
+    if triangles:
+        for (a, b, c), (d01, d02, d12) in zip(t.array("tri_idx", t.n_tri, 3), t.array("tri_b0", t.n_tri, 3)):
+            out += [(a, b, d01), (a, c, d02), (b, c, d12)]
+    if not branches:
+        return out
+    for (a, b), d in zip(t.array("br1_idx", t.n_br1, 2), t.array("br1_b0", t.n_br1)):
+        out.append((a, b, d))
+    for (a, b, c), (d1, d2) in zip(t.array("br2_idx", t.n_br2, 3), t.array("br2_b0", t.n_br2, 2)):
+        out += [(a, b, d1), (a, c, d2)]
+    for (a, b, c, e), (d1, d2, d3) in zip(t.array("br3_idx", t.n_br3, 4), t.array("br3_b0", t.n_br3, 3)):
+        out += [(a, b, d1), (a, c, d2), (a, e, d3)]
+    return out
+
+
+def test_holonomic_position_and_velocity():
+    name = "msld_small"
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    sp = spec(name)
+    n = sp.n
+    box = np.array(sp.box)
+    xprev = system.soa(sp.pos)
+    rng = np.random.default_rng(4)
+    x = xprev + rng.normal(scale=0.02, size=xprev.shape)
+    v = np.zeros_like(x)
+    halfdt = 0.5 * sp.dt
+    xn, vn = orc.holonomic_position(top, rp, x, xprev, v, halfdt)
+    P = xn.reshape(3, n).T
+
+    def worst_of(pairs):
+        w = 0.0
+        for a, b, d in pairs:
+            r = P[a] - P[b]
+            r -= box * np.round(r / box)
+            w = max(w, abs(np.linalg.norm(r) - d) / d)
+        return w
+    # water triangles: analytic SETTLE, exact to rounding
+    assert worst_of(_constraint_pairs(top, branches=False)) < 1e-10
+    # SHAKE groups stop at |b0^2 - r^2| < 2 b0^2 tol
+    tol = rp.contents.shake_tol
+    assert worst_of(_constraint_pairs(top, triangles=False)) < 1.01 * tol
+    pairs = _constraint_pairs(top)
+    assert len(pairs) > 0
+    # the velocity projection removes every bond-length rate
+    vr = rng.normal(size=xn.shape)
+    vp = orc.holonomic_velocity(top, rp, xn, vr).reshape(3, n).T
+    worst = 0.0
+    for a, b, d in pairs:
+        r = P[a] - P[b]
+        r -= box * np.round(r / box)
+        worst = max(worst, abs(np.dot(r, vp[a] - vp[b])) / d)
+    assert worst < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# neighbour list
+# ---------------------------------------------------------------------------
+def test_domdec_covers_every_pair_and_exclusion():
+    name = "msld_small"
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    sp = spec(name)
+    n = sp.n
+    posw, l2g, bc, tiles, masks = orc.domdec(top, rp, system.soa(sp.pos))
+    slot = np.empty(n, dtype=int)
+    slot[l2g[l2g >= 0]] = np.nonzero(l2g >= 0)[0]
+    tile_of = {(int(i), int(j)): k for k, (i, j) in enumerate(tiles)}
+    P = posw.reshape(3, n).T
+    box = np.array(sp.box)
+    rc = rp.contents.rcut
+    d = P[:, None, :] - P[None, :, :]
+    d -= box * np.round(d / box)
+    r = np.sqrt((d * d).sum(-1))
+    t = top.contents
+    excl = {tuple(sorted(p)) for p in t.array("excl_idx", t.n_excl, 2).tolist()}
+    for i, j in zip(*np.nonzero(r < rc)):
+        if i == j:
+            continue
+        si, sj = slot[i], slot[j]
+        k = tile_of[(si // 32, sj // 32)]
+        bit = (masks[k][si % 32] >> (sj % 32)) & 1
+        assert bit == ((min(i, j), max(i, j)) in excl)
