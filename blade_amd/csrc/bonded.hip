@@ -3,7 +3,9 @@
 // (restates bonded/bonded.cu:44-527, bonded/pair.cu:56-387 and
 // msld/msld.cu:1000-1130).  Each workgroup handles one term family so the
 // energy slot is uniform; geometry comes from 32-bit fractional
-// coordinates (exact minimum image) and the arithmetic is FP32.
+// coordinates (exact minimum image) and the arithmetic is FP32 (1-4 pairs
+// and exclusion corrections FP64).  Forces go to per-term slots, not to the
+// force array: no global atomics (the force assembly sums them per atom).
 #include "kernels.h"
 #include "erfc_coef.h"
 
@@ -40,15 +42,17 @@ __device__ __forceinline__ D3v dispd(const Dev &d, int i, int j) {
   r.z -= d.box[2] * floor(r.z * d.invbox[2] + 0.5);
   return r;
 }
-__device__ __forceinline__ void addforce_d(const Dev &d, int i, double s, D3v r) {
-  atomic_fix(&d.force[i], fix_d(s * r.x));
-  atomic_fix(&d.force[d.n + i], fix_d(s * r.y));
-  atomic_fix(&d.force[2 * d.n + i], fix_d(s * r.z));
+// per-atom force of a term into its slot (fixed point); the force
+// assembly adds every atom's slots
+__device__ __forceinline__ void put_d(const Dev &d, int slot, double s, D3v r) {
+  d.bslot[slot] = fix_d(s * r.x);
+  d.bslot[d.nslot + slot] = fix_d(s * r.y);
+  d.bslot[2 * d.nslot + slot] = fix_d(s * r.z);
 }
-__device__ __forceinline__ void addforce(const Dev &d, int i, F3 f) {
-  atomic_fix_f(&d.force[i], f.x);
-  atomic_fix_f(&d.force[d.n + i], f.y);
-  atomic_fix_f(&d.force[2 * d.n + i], f.z);
+__device__ __forceinline__ void put_f(const Dev &d, int slot, F3 f) {
+  d.bslot[slot] = fix_f(f.x);
+  d.bslot[d.nslot + slot] = fix_f(f.y);
+  d.bslot[2 * d.nslot + slot] = fix_f(f.z);
 }
 
 // lambda scaling of a term carrying two siteBlock codes (bonded.cu:80-89)
@@ -156,8 +160,8 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
       float et = 0.5f * p.x * db * db;
       float fb = p.x * db * s.l0 * s.l1;
       scale_dudl(d, s, et);
-      addforce(d, ij.x, (fb / r) * dr);
-      addforce(d, ij.y, (-fb / r) * dr);
+      put_f(d, d.sl_bond + 2 * t, (fb / r) * dr);
+      put_f(d, d.sl_bond + 2 * t + 1, (-fb / r) * dr);
       e = et * s.l0 * s.l1;
       // bonds and Urey-Bradley may share a workgroup: split by index
       if (t >= d.nbond12) {
@@ -184,9 +188,9 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
       scale_dudl(d, s, et);
       F3 fi = (fa / (mc * dot3(rij, rij))) * cross3(rij, cr);
       F3 fk = (-fa / (mc * dot3(rkj, rkj))) * cross3(rkj, cr);
-      addforce(d, ix.x, fi);
-      addforce(d, ix.z, fk);
-      addforce(d, ix.y, -1.f * (fi + fk));
+      put_f(d, d.sl_angle + 3 * t, fi);
+      put_f(d, d.sl_angle + 3 * t + 2, fk);
+      put_f(d, d.sl_angle + 3 * t + 1, -1.f * (fi + fk));
       e = et * s.l0 * s.l1;
     }
   } else if (bid < L.b_nb14) {
@@ -223,10 +227,11 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
       F3 fl = (-fphi * r / dot3(nv, nv)) * nv;
       float pp = dot3(rij, rjk) / rjk2, qq = dot3(rkl, rjk) / rjk2;
       F3 fj = (-pp) * fi + (-qq) * fl;
-      addforce(d, ix.x, fi);
-      addforce(d, ix.w, -1.f * fl);
-      addforce(d, ix.z, fl - fj);
-      addforce(d, ix.y, fj - fi);
+      const int s0 = (impr ? d.sl_impr : d.sl_dihe) + 4 * t;
+      put_f(d, s0, fi);
+      put_f(d, s0 + 3, -1.f * fl);
+      put_f(d, s0 + 2, fl - fj);
+      put_f(d, s0 + 1, fj - fi);
       e = et * s.l0 * s.l1;
     }
   } else if (bid < L.b_nbex) {
@@ -261,8 +266,8 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
         if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(d.lambda[s.b0] * g));
       }
       f *= dredr;
-      addforce_d(d, ij.x, (f / r), dr);
-      addforce_d(d, ij.y, (-f / r), dr);
+      put_d(d, d.sl_nb14 + 2 * t, (f / r), dr);
+      put_d(d, d.sl_nb14 + 2 * t + 1, (-f / r), dr);
       e = et * ll;
     }
   } else if (bid < L.b_bias) {
@@ -284,8 +289,8 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
         atomic_fix(&d.dudl[s.b0], fix_d((s.b1 ? d.lambda[s.b1] : 1.0) * et));
         if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(d.lambda[s.b0] * et));
       }
-      addforce_d(d, ij.x, (f / r), dr);
-      addforce_d(d, ij.y, (-f / r), dr);
+      put_d(d, d.sl_nbex + 2 * t, (f / r), dr);
+      put_d(d, d.sl_nbex + 2 * t + 1, (-f / r), dr);
       e = et * ll;
     }
   } else {

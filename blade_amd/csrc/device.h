@@ -21,6 +21,9 @@ constexpr float kFixScaleF = 4294967296.0f;
 constexpr double kFixScaleD = 4294967296.0;
 constexpr double kFixInvD = 1.0 / 4294967296.0;
 
+// force assembly components (force_assemble mask)
+constexpr unsigned kAsmNb = 1u, kAsmBonded = 2u, kAsmPme = 4u;
+
 // tile list capacity helpers
 constexpr int kBlockAtoms = 32;      // atoms per block (one half-wave)
 constexpr int kTileChunk = 16;       // candidate tiles examined per wave claim
@@ -111,6 +114,12 @@ struct Dev {
   int4 *impr_i; int2 *impr_sb; float4 *impr_p;            // (k, n, d0)
   int2 *nb14_i; int2 *nb14_sb; double4 *nb14_p;           // (qq, c12, c6, e14fac)
   int2 *nbex_i; int2 *nbex_sb; double *nbex_p;            // qq
+  // bonded force slots: every term stores its per-atom forces (fixed
+  // point) in slots of its own; a static CSR lists each atom's slots
+  int sl_bond, sl_angle, sl_dihe, sl_impr, sl_nb14, sl_nbex, nslot;
+  long long *bslot;              // [3][nslot]
+  int *bcsr_start;               // [N+1]
+  int *bcsr;                     // [nslot]
   // constraints
   int ntri, nbr1, nbr2, nbr3;
   int4 *tri_i; double4 *tri_b;   // b0 (01,02,12), .w unused

@@ -8,7 +8,12 @@ namespace blade {
 void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st);
 void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st);
 void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st);
-void launch_pme(const Dev &d, bool energy, hipStream_t st);
+// PME charge grid: bins, brick spread, FFT with the convolution (the
+// potential grid is left in rgrid for the force assembly)
+void launch_pme_grid(const Dev &d, bool energy, hipStream_t st);
+// per-atom force assembly: adds the components in mask (kAsmNb chunk
+// partials, kAsmBonded slots, kAsmPme gather) to the force array
+void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st);
 bool pme_grid_supported(int K);
 void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st);
 void launch_update(const Dev &d, bool energy_step, hipStream_t st);

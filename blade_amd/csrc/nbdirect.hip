@@ -15,17 +15,19 @@
 //
 // Accumulation: per-tile FP32 partial sums are converted to 64-bit fixed
 // point and summed per lane; at the end of a chunk the i-force partials are
-// stored to the chunk's own slot.  nb_reduce then adds the chunks of every
-// block to the force array, one thread per atom: no global atomics, and the
-// result does not depend on the order in which waves claimed chunks.
+// stored to the chunk's own slot.  The force assembly (pme.hip) adds the
+// chunks of every atom's block to the force array, one thread per atom: no
+// global atomics, and the result does not depend on the order in which the
+// chunks ran.
 #include "kernels.h"
 #include "erfc_coef.h"
 
 namespace blade {
 
-// erfc(x) = exp(-x^2) * poly(1/(1+P x)); returns erfc and exp(-x^2)
+// erfc(x) = exp(-x^2) * poly(1/(1+P x)) given ex = exp(-x^2).  The
+// hardware reciprocal (1 ulp) replaces the IEEE division sequence.
 __device__ __forceinline__ float erfc_fit(float x, float ex) {
-  float t = __frcp_rn(fmaf(BM_ERFC_P, x, 1.0f));
+  float t = __builtin_amdgcn_rcpf(fmaf(BM_ERFC_P, x, 1.0f));
   float p = bm_erfc_c[BM_ERFC_DEG];
 #pragma unroll
   for (int k = BM_ERFC_DEG - 1; k >= 0; k--) p = fmaf(p, t, bm_erfc_c[k]);
@@ -79,7 +81,8 @@ __device__ __forceinline__ void pair_ef(const NbConst &c, float qq, float c12, f
                                         float rinv, float &f, float &eq, float &eA, float &eB) {
   if (ELEC == BM_PME) {
     float br = c.beta * r;
-    float ex = exp2f(-c.beta2log2e * r * r);
+    // argument >= -(beta rcut)^2 log2(e), far above the denormal range
+    float ex = __builtin_amdgcn_exp2f(-c.beta2log2e * r * r);
     float erfcrinv = erfc_fit(fminf(br, BM_ERFC_XMAX), ex) * rinv;
     f = -qq * (erfcrinv + kTwoOverSqrtPi * c.beta * ex) * rinv;
     eq = erfcrinv;
@@ -99,16 +102,14 @@ __device__ __forceinline__ void pair_ef(const NbConst &c, float qq, float c12, f
   }
   float rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
   if (VDW == BM_VFSWITCH) {
-    if (r < c.rs) {
-      f += (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv;
-      eA = rinv6 * rinv6 - c.dv6 * c.dv6;
-      eB = rinv6 - c.dv6;
-    } else {
-      float a3 = rinv3 - c.rcinv3, a6 = rinv6 - c.rcinv6;
-      f += (6 * c6 * c.k6 * a3 * rinv3 - 12 * c12 * c.k12 * a6 * rinv6) * rinv;
-      eA = c.k12 * a6 * a6;
-      eB = c.k6 * a3 * a3;
-    }
+    // branch-free: the lanes of a wave straddle rswitch
+    const bool in = r < c.rs;
+    const float a3 = rinv3 - c.rcinv3, a6 = rinv6 - c.rcinv6;
+    const float A = in ? rinv6 : c.k6 * a3 * rinv3;
+    const float B = in ? rinv6 * rinv6 : c.k12 * a6 * rinv6;
+    f += (6 * c6 * A - 12 * c12 * B) * rinv;
+    eA = in ? rinv6 * rinv6 - c.dv6 * c.dv6 : c.k12 * a6 * a6;
+    eB = in ? rinv6 - c.dv6 : c.k6 * a3 * a3;
   } else if (VDW == BM_VSWITCH) {
     if (r < c.rs) {
       f += (6 * c6 - 12 * c12 * rinv6) * rinv6 * rinv;
@@ -232,14 +233,16 @@ struct IAtom {
   double qd;     // kE * charge in FP64 (energy steps)
 };
 
-constexpr int kVdwLds = 1024;   // LJ pair table entries staged in LDS
+// LJ pair table staged in LDS (dynamic size ntypes^2); tables beyond
+// kVdwLdsMax entries are read from global memory by a separate variant
+constexpr int kVdwLdsMax = 6144;
 
 // The pairs of one tile for the lane's i-atom: the set bits of jm, shared
 // out between the halves (half 0 takes the first, half 1 the second, ...).
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
 __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const IAtom &ia, int li,
                                            int half, const uint4 *sp, const int2 *spr, const float *slo,
-                                           const float2 *svdw, bool vdwLds, unsigned jm, unsigned em,
+                                           const float2 *svdw, unsigned jm, unsigned em,
                                            bool self, float &fx, float &fy, float &fz, float &fli,
                                            double &etile) {
   const float sx = d.fracscale[0], sy = d.fracscale[1], sz = d.fracscale[2];
@@ -263,7 +266,7 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     if (self) ok = ok && jl != li;
     if (!ok) continue;
     const int vi = ia.t * d.ntypes + prj.x;
-    const float2 lj = vdwLds ? svdw[vi] : d.vdw[vi];
+    const float2 lj = svdw[vi];
     float rinv = __frsqrt_rn(r2);
     float r = r2 * rinv;
     const float qq = ia.q * __uint_as_float(pj.w);
@@ -283,7 +286,7 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
           dredr = (3.f - 2.f * q) * q * q;
           dredll = (re - dredr * q) * (-kSoftCoreRadius);
           r = re * rSoft;
-          rinv = 1.f / r;
+          rinv = __builtin_amdgcn_rcpf(r);
         }
       }
     }
@@ -343,19 +346,20 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool GTABLE>
 __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   __shared__ uint4 s_pos[4][kBlockAtoms];
   __shared__ int2 s_par[4][kBlockAtoms];
   __shared__ float s_lo[4][kBlockAtoms];
-  __shared__ float2 s_vdw[kVdwLds];
+  extern __shared__ float2 s_vdw_dyn[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int half = lane >> 5, li = lane & 31;
-  const int nt2 = d.ntypes * d.ntypes;
-  const bool vdwLds = nt2 <= kVdwLds;
-  if (vdwLds)
-    for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_vdw[k] = d.vdw[k];
-  __syncthreads();
+  if (!GTABLE) {
+    const int nt2 = d.ntypes * d.ntypes;
+    for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_vdw_dyn[k] = d.vdw[k];
+    __syncthreads();
+  }
+  const float2 *s_vdw = GTABLE ? d.vdw : s_vdw_dyn;
   const NbConst c = make_const(d);
   const bool fits = *d.ntiles <= d.maxtiles && *d.nchunks <= d.maxchunks;
   const int nchunks = fits ? *d.nchunks : 0;
@@ -433,10 +437,10 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
       double etile = 0.0;
       const bool alchTile = ALCH && (hi.w | alchJc);
       if (alchTile)
-        tile_pairs<ELEC, VDW, ENERGY, ALCH, SOFT>(d, c, ia, li, half, sp, spr, slo, s_vdw, vdwLds, jm, emc,
+        tile_pairs<ELEC, VDW, ENERGY, ALCH, SOFT>(d, c, ia, li, half, sp, spr, slo, s_vdw, jm, emc,
                                                   I == Jc, fx, fy, fz, fli, etile);
       else
-        tile_pairs<ELEC, VDW, ENERGY, false, false>(d, c, ia, li, half, sp, spr, slo, s_vdw, vdwLds, jm, emc,
+        tile_pairs<ELEC, VDW, ENERGY, false, false>(d, c, ia, li, half, sp, spr, slo, s_vdw, jm, emc,
                                                     I == Jc, fx, fy, fz, fli, etile);
       ax += fix_f(fx);
       ay += fix_f(fy);
@@ -466,42 +470,17 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   }
 }
 
-// Per block: sum the fixed-point i-force partials of the row's chunks and
-// add them to the force array (one thread per atom; kernels on the system
-// stream run in order, so a plain read-modify-write is safe).
-__global__ void __launch_bounds__(256) nb_reduce(Dev d) {
-  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63, half = lane >> 5, li = lane & 31;
-  if (I >= *d.nblocks) return;
-  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
-  const int c0 = d.cstart[I], c1 = d.cstart[I + 1];
-  long long sx = 0, sy = 0, sz = 0;
-  for (int c = c0 + half; c < c1; c += 2) {
-    const long long *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
-    sx += src[li];
-    sy += src[kBlockAtoms + li];
-    sz += src[2 * kBlockAtoms + li];
-  }
-  sx += __shfl_xor(sx, 32, 64);
-  sy += __shfl_xor(sy, 32, 64);
-  sz += __shfl_xor(sz, 32, 64);
-  if (half == 0) {
-    const int a = d.l2g[I * kBlockAtoms + li];
-    if (a >= 0) {
-      d.force[a] += sx;
-      d.force[d.n + a] += sy;
-      d.force[2 * d.n + a] += sz;
-    }
-  }
-}
-
 // one wave per chunk slot of the list capacity; waves past this step's
 // chunk count exit at once
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
 static void launch_t(const Dev &d, int nwg, hipStream_t st) {
   (void)nwg;
-  nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT><<<(d.maxchunks + 3) / 4, 256, 0, st>>>(d);
-  nb_reduce<<<(d.maxblocks + 3) / 4, 256, 0, st>>>(d);
+  const unsigned grid = (d.maxchunks + 3) / 4;
+  const int nt2 = d.ntypes * d.ntypes;
+  if (nt2 <= kVdwLdsMax)
+    nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, false><<<grid, 256, nt2 * sizeof(float2), st>>>(d);
+  else
+    nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, true><<<grid, 256, 0, st>>>(d);
 }
 
 template <int ELEC, int VDW>

@@ -11,8 +11,8 @@
 //                pass z reads the real grid, pass x fuses forward FFT, the
 //                convolution
 //                B(m) kE/(2 pi V) exp(-pi^2 m^2/beta^2)/m^2 and the inverse
-//   pme_gather   potential and gradient at each atom -> force, dU/dlambda,
-//                energy; the Ewald self term rides along
+//   force_assemble  potential gradient at each atom -> force, dU/dlambda;
+//                together with the nonbonded and bonded contributions
 #include <algorithm>
 
 #include "kernels.h"
@@ -598,18 +598,30 @@ __global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
 // sizes with compiled plane / line kernels
 #define BM_FFT_SIZES(X) X(16) X(18) X(20) X(24) X(27) X(32) X(36) X(40) X(48) X(54) X(64)
 
+// Per-atom force assembly, one thread per list slot (spatial order, so the
+// PME stencils of a wave overlap in cache).  Adds, in fixed point, the
+// components selected by mask: the PME force 2 lambda q grad(phi) and
+// dU/dlambda = 2 q phi from the potential grid (N = spline order; N = 0
+// compiles no PME part), the nonbonded chunk partials of the atom's block,
+// and the atom's bonded slots.  One thread per atom and stream order make
+// the read-modify-write of the force array safe without atomics.
 template <int N>
-__global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  // the spread has consumed the bins: clear them for the next step
-  if (i < d.pnb[0] * d.pnb[1] * d.pnb[2]) d.pbcount[i] = 0;
-  double e = 0.0, eself = 0.0;
-  if (i < d.n) {
+__global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  // the spread has consumed the PME bins: clear them for the next step
+  if (N && (mask & kAsmPme) && s < d.pnb[0] * d.pnb[1] * d.pnb[2]) d.pbcount[s] = 0;
+  if (s >= *d.nblocks * kBlockAtoms) return;
+  const int i = d.l2g[s];
+  if (i < 0) return;
+  long long fx = 0, fy = 0, fz = 0;
+  if constexpr (N > 0) {
+   if (mask & kAsmPme) {
     const int b = d.sb[i] & 0xFFFF;
     const float q = d.q[i];
     const float l = b ? d.lambdaf[b] : 1.f;
     int u0[3];
     float fr[3];
+#pragma unroll
     for (int k = 0; k < 3; k++) grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0[k], fr[k]);
     float wx[N], wy[N], wz[N], dx[N], dy[N], dz[N];
     bspline_w<N>(fr[0], wx, dx);
@@ -635,19 +647,37 @@ __global__ void __launch_bounds__(256) pme_gather(Dev d, bool energy) {
         gz += wx[a] * wy[c] * sd;
       }
     }
-    const float s = 2.f * l * q;
-    // one thread per atom, kernels on the system stream run in order: a
-    // plain read-modify-write of the fixed-point force is safe
-    d.force[i] += fix_f(s * gx * (d.K[0] * (float)d.invbox[0]));
-    d.force[d.n + i] += fix_f(s * gy * (d.K[1] * (float)d.invbox[1]));
-    d.force[2 * d.n + i] += fix_f(s * gz * (d.K[2] * (float)d.invbox[2]));
+    const float sc = 2.f * l * q;
+    fx += fix_f(sc * gx * (d.K[0] * (float)d.invbox[0]));
+    fy += fix_f(sc * gy * (d.K[1] * (float)d.invbox[1]));
+    fz += fix_f(sc * gz * (d.K[2] * (float)d.invbox[2]));
     // dU/dlambda_b = 2 q phi (nbrecip.cu:445-449); the energy is summed in
     // k-space by the convolution pass
     if (b) atomic_fix(&d.dudl[b], fix_d(2.0 * d.qd[i] * (double)phi));
+   }
   }
-  (void)e;
-  (void)eself;
-  (void)energy;
+  if ((mask & kAsmNb) && *d.ntiles <= d.maxtiles && *d.nchunks <= d.maxchunks) {
+    const int I = s / kBlockAtoms, li = s % kBlockAtoms;
+    const int c1 = d.cstart[I + 1];
+    for (int c = d.cstart[I]; c < c1; c++) {
+      const long long *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
+      fx += src[li];
+      fy += src[kBlockAtoms + li];
+      fz += src[2 * kBlockAtoms + li];
+    }
+  }
+  if (mask & kAsmBonded) {
+    const int k1 = d.bcsr_start[i + 1];
+    for (int k = d.bcsr_start[i]; k < k1; k++) {
+      const int q = d.bcsr[k];
+      fx += d.bslot[q];
+      fy += d.bslot[d.nslot + q];
+      fz += d.bslot[2 * d.nslot + q];
+    }
+  }
+  d.force[i] += fx;
+  d.force[d.n + i] += fy;
+  d.force[2 * d.n + i] += fz;
 }
 
 // ---------------------------------------------------------------------
@@ -670,7 +700,7 @@ bool pme_grid_supported(int K) {
   return n == 1;
 }
 
-void launch_pme(const Dev &d, bool energy, hipStream_t st) {
+void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
   const int N = d.order;
   const int nbr = d.pnb[0] * d.pnb[1] * d.pnb[2];
   int bmax = 1;
@@ -721,10 +751,16 @@ void launch_pme(const Dev &d, bool energy, hipStream_t st) {
       fft_lines<<<(nlines + nl - 1) / nl, 256, 0, st>>>(d, a);
     }
   }
-  const unsigned ga = (unsigned)((std::max(d.n, nbr) + 255) / 256);
-  if (N == 4) pme_gather<4><<<ga, 256, 0, st>>>(d, energy);
-  else if (N == 6) pme_gather<6><<<ga, 256, 0, st>>>(d, energy);
-  else pme_gather<8><<<ga, 256, 0, st>>>(d, energy);
+}
+
+void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st) {
+  const int nbr = d.elecmethod == BM_PME ? d.pnb[0] * d.pnb[1] * d.pnb[2] : 0;
+  const unsigned g = (unsigned)((std::max(32LL * d.maxblocks, (long long)nbr) + 255) / 256);
+  if (d.elecmethod != BM_PME) mask &= ~kAsmPme;
+  if (!(mask & kAsmPme)) force_assemble<0><<<g, 256, 0, st>>>(d, mask);
+  else if (d.order == 4) force_assemble<4><<<g, 256, 0, st>>>(d, mask);
+  else if (d.order == 6) force_assemble<6><<<g, 256, 0, st>>>(d, mask);
+  else force_assemble<8><<<g, 256, 0, st>>>(d, mask);
 }
 
 }  // namespace blade

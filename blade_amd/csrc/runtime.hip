@@ -82,12 +82,19 @@ struct bm_system {
     return BM_OK;
   }
 
+  // all force terms: nbdirect chunk partials, bonded slots and the PME
+  // grid, then one per-atom assembly into the force array
+  void force_launches(bool energy) {
+    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
+    blade::launch_bonded(d, energy, bias, st);
+    if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st);
+    blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st);
+  }
+
   void step_launches(bool rebuild, bool energy) {
     if (rebuild) blade::launch_domdec_rebuild(d, maxBlocks, st);
     blade::launch_domdec_pack(d, maxBlocks, st);
-    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
-    if (d.elecmethod == BM_PME) blade::launch_pme(d, energy, st);
-    blade::launch_bonded(d, energy, bias, st);
+    force_launches(energy);
     blade::launch_update(d, energy, st);
   }
 
@@ -412,6 +419,37 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       xp[q] = t->nbex_qq[q];
     }
     UPLOAD(d.nbex_i, xi); UPLOAD(d.nbex_sb, xsb); UPLOAD(d.nbex_p, xp);
+    // bonded force slots and the atom -> slot CSR (the topology is static)
+    d.sl_bond = 0;
+    d.sl_angle = d.sl_bond + 2 * d.nbond;
+    d.sl_dihe = d.sl_angle + 3 * d.nangle;
+    d.sl_impr = d.sl_dihe + 4 * d.ndihe;
+    d.sl_nb14 = d.sl_impr + 4 * d.nimpr;
+    d.sl_nbex = d.sl_nb14 + 2 * d.nnb14;
+    d.nslot = d.sl_nbex + 2 * d.nnbex;
+    std::vector<int> owner(d.nslot);
+    for (int q = 0; q < d.nbond; q++)
+      for (int k = 0; k < 2; k++) owner[d.sl_bond + 2 * q + k] = t->bond_idx[2 * q + k];
+    for (int q = 0; q < d.nangle; q++)
+      for (int k = 0; k < 3; k++) owner[d.sl_angle + 3 * q + k] = t->angle_idx[3 * q + k];
+    for (int q = 0; q < d.ndihe; q++)
+      for (int k = 0; k < 4; k++) owner[d.sl_dihe + 4 * q + k] = t->dihe_idx[4 * q + k];
+    for (int q = 0; q < d.nimpr; q++)
+      for (int k = 0; k < 4; k++) owner[d.sl_impr + 4 * q + k] = t->impr_idx[4 * q + k];
+    for (int q = 0; q < d.nnb14; q++)
+      for (int k = 0; k < 2; k++) owner[d.sl_nb14 + 2 * q + k] = t->nb14_idx[2 * q + k];
+    for (int q = 0; q < d.nnbex; q++)
+      for (int k = 0; k < 2; k++) owner[d.sl_nbex + 2 * q + k] = t->nbex_idx[2 * q + k];
+    std::vector<int> cstartv(n + 1, 0), cs(d.nslot);
+    for (int q = 0; q < d.nslot; q++) cstartv[owner[q] + 1]++;
+    for (int a = 0; a < n; a++) cstartv[a + 1] += cstartv[a];
+    {
+      std::vector<int> fill(cstartv.begin(), cstartv.end() - 1);
+      for (int q = 0; q < d.nslot; q++) cs[fill[owner[q]]++] = q;
+    }
+    ALLOC(d.bslot, 3 * (size_t)d.nslot);
+    UPLOAD(d.bcsr_start, cstartv);
+    UPLOAD(d.bcsr, cs);
   }
   // constraints
   {
@@ -527,26 +565,33 @@ int bm_reset_force(bm_system *s, unsigned) {
   return BM_OK;
 }
 
+// Each term launch is followed by an assembly of that term's forces into
+// the force array; bm_calc_force assembles all terms in one pass.
 int bm_getforce_nbdirect(bm_system *s, unsigned flags) {
   if (!s) return BM_EINVAL;
   blade::launch_nbdirect(s->d, flags & BM_CALC_ENERGY, s->alch, s->soft, s->nwgNb, s->st);
+  blade::launch_force_assemble(s->d, blade::kAsmNb, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
 
 int bm_getforce_ewald(bm_system *s, unsigned flags) {
   if (!s) return BM_EINVAL;
-  if (s->d.elecmethod == BM_PME) blade::launch_pme(s->d, flags & BM_CALC_ENERGY, s->st);
+  if (s->d.elecmethod == BM_PME) {
+    blade::launch_pme_grid(s->d, flags & BM_CALC_ENERGY, s->st);
+    blade::launch_force_assemble(s->d, blade::kAsmPme, s->st);
+  }
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
 
-// the self term is folded into the gather kernel of bm_getforce_ewald
+// the self term rides on the bonded launch (its last workgroup)
 int bm_getforce_ewaldself(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
 
 int bm_getforce_bonded(bm_system *s, unsigned flags) {
   if (!s) return BM_EINVAL;
   blade::launch_bonded(s->d, flags & BM_CALC_ENERGY, s->bias, s->st);
+  blade::launch_force_assemble(s->d, blade::kAsmBonded, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
@@ -557,10 +602,7 @@ int bm_getforce_msldbias(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL;
 
 int bm_calc_force(bm_system *s, long, unsigned flags) {
   if (!s) return BM_EINVAL;
-  const bool e = flags & BM_CALC_ENERGY;
-  blade::launch_nbdirect(s->d, e, s->alch, s->soft, s->nwgNb, s->st);
-  if (s->d.elecmethod == BM_PME) blade::launch_pme(s->d, e, s->st);
-  blade::launch_bonded(s->d, e, s->bias, s->st);
+  s->force_launches(flags & BM_CALC_ENERGY);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
@@ -611,7 +653,8 @@ int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_gra
 
 // Per-phase timing with events on the system stream: nsteps eager steps
 // (rebuild every freq_domdec steps, no energies) with events around the
-// nbdirect and PME launches.  out = {ms nbdirect, ms pme, ms step} averages.
+// nbdirect launch and around the PME grid (bin, spread, FFT) plus the force
+// assembly with the PME gather.  out = {ms nbdirect, ms pme, ms step}.
 int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
   if (!s || nsteps <= 0 || !out) return BM_EINVAL;
   double tn = 0, tp = 0, tt = 0;
@@ -625,15 +668,17 @@ int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
     HIPCHK(hipEventRecord(s->ev[1], s->st));
     blade::launch_nbdirect(s->d, false, s->alch, s->soft, s->nwgNb, s->st);
     HIPCHK(hipEventRecord(s->ev[2], s->st));
-    if (s->d.elecmethod == BM_PME) blade::launch_pme(s->d, false, s->st);
-    HIPCHK(hipEventRecord(s->ev[3], s->st));
     blade::launch_bonded(s->d, false, s->bias, s->st);
+    HIPCHK(hipEventRecord(s->ev[3], s->st));
+    if (s->d.elecmethod == BM_PME) blade::launch_pme_grid(s->d, false, s->st);
+    blade::launch_force_assemble(s->d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, s->st);
+    HIPCHK(hipEventRecord(s->ev[5], s->st));
     blade::launch_update(s->d, false, s->st);
     HIPCHK(hipEventRecord(s->ev[4], s->st));
     HIPCHK(hipEventSynchronize(s->ev[4]));
     float a = 0, b = 0, c = 0;
     HIPCHK(hipEventElapsedTime(&a, s->ev[1], s->ev[2]));
-    HIPCHK(hipEventElapsedTime(&b, s->ev[2], s->ev[3]));
+    HIPCHK(hipEventElapsedTime(&b, s->ev[3], s->ev[5]));
     HIPCHK(hipEventElapsedTime(&c, s->ev[0], s->ev[4]));
     tn += a; tp += b; tt += c;
   }
