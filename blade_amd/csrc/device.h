@@ -162,6 +162,11 @@ __device__ __forceinline__ unsigned frac_u32(double x, double invL) {
   return (unsigned)(long long)floor(s);
 }
 
+// first non-finite value wins the NaN flag (atom or DOF index)
+__device__ __forceinline__ void nan_check(const Dev &d, double v, int i) {
+  if (!isfinite(v)) atomicCAS(d.nanflag, -1, i);
+}
+
 // minimum-image displacement from fractional coordinates (int wrap)
 __device__ __forceinline__ float fdisp(unsigned a, unsigned b, float scale) {
   return (float)(int)(a - b) * scale;

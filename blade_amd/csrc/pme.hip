@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
 // compiles no PME part), the nonbonded chunk partials of the atom's block,
 // and the atom's bonded slots.  One thread per atom and stream order make
 // the read-modify-write of the force array safe without atomics.
-template <int N>
+template <int N, bool KICK>
 __global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   // the spread has consumed the PME bins: clear them for the next step
@@ -675,9 +675,26 @@ __global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
       fz += d.bslot[2 * d.nslot + q];
     }
   }
-  d.force[i] += fx;
-  d.force[d.n + i] += fy;
-  d.force[2 * d.n + i] += fz;
+  const long long F[3] = {d.force[i] + fx, d.force[d.n + i] + fy, d.force[2 * d.n + i] + fz};
+#pragma unroll
+  for (int k = 0; k < 3; k++) d.force[k * d.n + i] = F[k];
+  if (KICK) {
+    // V (full kick), backup, half drift R: as upd_kick mode 0
+    const double ism = d.ism[i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int j = k * d.n + i;
+      const double f = (double)F[k] * kFixInvD;
+      const double x = d.pos[j];
+      d.posb[j] = x;
+      if (ism > 0) {
+        const double v = d.vel[j] - 2.0 * d.halfdt * ism * ism * f;
+        d.vel[j] = v;
+        d.pos[j] = x + d.halfdt * v;
+        nan_check(d, v, i);
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -753,14 +770,20 @@ void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
   }
 }
 
-void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st) {
+template <bool KICK>
+static void assemble_t(const Dev &d, unsigned mask, unsigned g, hipStream_t st) {
+  if (!(mask & kAsmPme)) force_assemble<0, KICK><<<g, 256, 0, st>>>(d, mask);
+  else if (d.order == 4) force_assemble<4, KICK><<<g, 256, 0, st>>>(d, mask);
+  else if (d.order == 6) force_assemble<6, KICK><<<g, 256, 0, st>>>(d, mask);
+  else force_assemble<8, KICK><<<g, 256, 0, st>>>(d, mask);
+}
+
+void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st, bool kick) {
   const int nbr = d.elecmethod == BM_PME ? d.pnb[0] * d.pnb[1] * d.pnb[2] : 0;
   const unsigned g = (unsigned)((std::max(32LL * d.maxblocks, (long long)nbr) + 255) / 256);
   if (d.elecmethod != BM_PME) mask &= ~kAsmPme;
-  if (!(mask & kAsmPme)) force_assemble<0><<<g, 256, 0, st>>>(d, mask);
-  else if (d.order == 4) force_assemble<4><<<g, 256, 0, st>>>(d, mask);
-  else if (d.order == 6) force_assemble<6><<<g, 256, 0, st>>>(d, mask);
-  else force_assemble<8><<<g, 256, 0, st>>>(d, mask);
+  if (kick) assemble_t<true>(d, mask, g, st);
+  else assemble_t<false>(d, mask, g, st);
 }
 
 }  // namespace blade

@@ -84,18 +84,20 @@ struct bm_system {
 
   // all force terms: nbdirect chunk partials, bonded slots and the PME
   // grid, then one per-atom assembly into the force array
-  void force_launches(bool energy) {
+  // (kick: the assembly also kicks the atoms of a non-energy step)
+  void force_launches(bool energy, bool kick = false) {
     blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
     blade::launch_bonded(d, energy, bias, st);
     if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st);
-    blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st);
+    blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st, kick);
   }
 
   void step_launches(bool rebuild, bool energy) {
     if (rebuild) blade::launch_domdec_rebuild(d, maxBlocks, st);
     blade::launch_domdec_pack(d, maxBlocks, st);
-    force_launches(energy);
-    blade::launch_update(d, energy, st);
+    force_launches(energy, !energy);
+    if (energy) blade::launch_update(d, true, st);
+    else blade::launch_update_after_kick(d, st);
   }
 
   ~bm_system() {

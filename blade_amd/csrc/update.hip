@@ -77,9 +77,6 @@ __device__ double theta_force(const Dev &d, int b) {
   return f * d.lambda[b] * d.fnex * cos(d.pos[3 * d.n + b]);
 }
 
-__device__ __forceinline__ void nan_check(const Dev &d, double v, int i) {
-  if (!isfinite(v)) atomicCAS(d.nanflag, -1, i);
-}
 
 // mode 0: v -= dt f/m, backup, x += dt/2 v; mode 1: v -= dt/2 f/m;
 // mode 2: v -= dt/2 f/m, backup, x += dt/2 v
@@ -362,11 +359,34 @@ __device__ void theta_to_lambda(const Dev &d, int s) {
   }
 }
 
-// groups, then (last call of a step) sites and the step counter
+// full kick of theta DOF b (the atoms' kick ran in the force assembly)
+__device__ void theta_kick(const Dev &d, int b) {
+  const int i = 3 * d.n + b;
+  const double f = theta_force(d, b);
+  d.force[i] = fix_d(f);
+  const double ism = d.ism[i];
+  const double x = d.pos[i];
+  d.posb[i] = x;
+  if (ism > 0) {
+    const double v = d.vel[i] - 2.0 * d.halfdt * ism * ism * f;
+    d.vel[i] = v;
+    d.pos[i] = x + d.halfdt * v;
+    nan_check(d, v, i);
+  }
+}
+
+// groups, then (last call of a step) sites and the step counter;
+// finish == 2: first call of a step whose atoms were kicked by the force
+// assembly: the theta DOFs are kicked here
 __global__ void __launch_bounds__(64) upd_constrain(Dev d, int finish) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
   if (t < ng) constrain_group(d, t);
+  if (finish == 2) {
+    const int b = t - ng;
+    if (b >= 0 && b < d.nl && d.nl > 1) theta_kick(d, b);
+    return;
+  }
   if (finish) {
     const int s = t - ng;
     if (s >= 0 && s < d.nsites && d.nl > 1) theta_to_lambda(d, s);
@@ -473,6 +493,16 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
   if (ng) upd_constrain<<<gG, 64, 0, st>>>(d, 0);
   upd_o_drift<<<gN, 256, 0, st>>>(d);
   upd_constrain<<<gF, 64, 0, st>>>(d, 1);
+}
+
+// the rest of a non-energy step after the force assembly kicked the atoms
+void launch_update_after_kick(const Dev &d, hipStream_t st) {
+  const int N = 3 * d.n + d.nl;
+  const unsigned gN = (N + 255) / 256;
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  upd_constrain<<<(ng + d.nl + 63) / 64, 64, 0, st>>>(d, 2);
+  upd_o_drift<<<gN, 256, 0, st>>>(d);
+  upd_constrain<<<(ng + d.nsites + 1 + 63) / 64, 64, 0, st>>>(d, 1);
 }
 
 __global__ void upd_set_step(Dev d, long long v) { *d.step = v; }

@@ -1,0 +1,104 @@
+"""Topology derivation (CPU): the exclusion, 1-4 and PME exclusion-correction
+lists and the 1-4 parameters derived by the library (blade_prepare) against
+an independent restatement of the reference's rules:
+
+  src/system/potential.cxx:970-1072  exclusions grown along bonds: bonded
+      pairs, then partners of bonded partners, then one more bond; a pair is
+      added only if its atoms interact (src/msld/msld.cu:675-678: same block,
+      or blocks of different sites) and it is not already excluded; every
+      pair of atoms in different blocks of one site is excluded as well
+  src/system/potential.cxx:1093-1142 1-4 pairs = the third-bond exclusions,
+      eps14 = sqrt(eps14_i eps14_j), Rmin14 = Rmin14/2_i + Rmin14/2_j,
+      c12 = eps Rmin^12, c6 = 2 eps Rmin^6, e14fac = sqrt(e_i) sqrt(e_j)
+  src/system/potential.cxx:1144-1163 + src/msld/msld.cu:623-668 (mode 2)
+      PME corrections: every exclusion that is not a 1-4 pair, except pairs
+      of different blocks of one site
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import built, spec
+
+
+def _derive(sp):
+    n = sp.n
+    block = np.asarray(sp.atom_block)
+    site = np.asarray(sp.block_site)
+
+    def interacting(i, j):
+        return block[i] == block[j] or site[block[i]] != site[block[j]]
+
+    excluded = set()
+    levels = []
+    # level 1: bonded pairs
+    lvl = set()
+    for a, b in sp.bonds:
+        p = (min(a, b), max(a, b))
+        if p not in excluded and interacting(a, b):
+            lvl.add(p)
+            excluded.add(p)
+    levels.append(lvl)
+    # levels 2 and 3: extend a pair of the previous level by one bond
+    for _ in range(2):
+        prev = levels[-1]
+        part = [set() for _ in range(n)]
+        for i, j in prev:
+            part[i].add(j)
+            part[j].add(i)
+        lvl = set()
+        for a, b in sp.bonds:
+            for x, y in ((a, b), (b, a)):
+                for k in part[y]:
+                    if k == x:
+                        continue
+                    p = (min(x, k), max(x, k))
+                    if p not in excluded and interacting(x, k):
+                        lvl.add(p)
+                        excluded.add(p)
+        levels.append(lvl)
+    msld = set()
+    for i in range(n):
+        for j in range(i + 1, n):
+            if not interacting(i, j):
+                msld.add((i, j))
+    allx = excluded | msld
+    nb14 = levels[2]
+    nbex = {p for p in allx - nb14
+            if not (block[p[0]] != block[p[1]] and site[block[p[0]]] == site[block[p[1]]])}
+    return allx, nb14, nbex
+
+
+@pytest.mark.parametrize("name", ["config1", "msld_small", "config2_small"])
+def test_exclusion_lists_follow_the_reference_rules(name):
+    sp = spec(name)
+    s = built(name)                     # owns the topology tables
+    t = s.topology().contents
+    allx, nb14, nbex = _derive(sp)
+    got_x = {tuple(sorted(p)) for p in t.array("excl_idx", t.n_excl, 2).tolist()}
+    got_14 = {tuple(sorted(p)) for p in t.array("nb14_idx", t.n_nb14, 2).tolist()}
+    got_ex = {tuple(sorted(p)) for p in t.array("nbex_idx", t.n_nbex, 2).tolist()}
+    assert got_x == allx
+    assert got_14 == nb14
+    assert got_ex == nbex
+    assert len(got_14) == t.n_nb14 and len(got_ex) == t.n_nbex
+
+
+def test_nb14_parameters():
+    name = "config2_small"
+    sp = spec(name)
+    s = built(name)
+    t = s.topology().contents
+    idx = t.array("nb14_idx", t.n_nb14, 2)
+    qq, c12, c6, e14 = (t.array(k, t.n_nb14) for k in ("nb14_qq", "nb14_c12", "nb14_c6", "nb14_e14fac"))
+    q = t.array("charge", t.n_atoms)
+    for k, (i, j) in enumerate(idx):
+        ti, tj = sp.atoms[i][4], sp.atoms[j][4]
+        eps_i, _, e14_i, r14_i, f_i, _ = sp.nb_par[ti]
+        eps_j, _, e14_j, r14_j, f_j, _ = sp.nb_par[tj]
+        eps = np.sqrt(e14_i * e14_j)     # the API negates the input eps (both negative)
+        rmin = r14_i + r14_j
+        s6 = rmin ** 6
+        assert abs(c12[k] - eps * s6 * s6) <= 1e-12 * abs(eps * s6 * s6)
+        assert abs(c6[k] - 2 * eps * s6) <= 1e-12 * abs(2 * eps * s6)
+        assert abs(e14[k] - np.sqrt(f_i) * np.sqrt(f_j)) < 1e-15
+        assert abs(qq[k] - q[i] * q[j]) < 1e-15
