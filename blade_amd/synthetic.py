@@ -88,6 +88,8 @@ class Spec:
     grid: tuple = (-1, -1, -1)
     grid_space: float = 1.0
     order: int = 6
+    vdw_method: int = 0      # 0 force switch, 1 potential switch, 2 shift
+    elec_method: int = 1     # 0 force switch, 1 PME, 2 force shift
     shake_tol: float = 2e-7
     seed: int = 0xB1ADE000
     nsteps: int = 1000

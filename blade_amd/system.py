@@ -242,7 +242,7 @@ def build(spec, gpus=(0,)):
             s.add_msld_bias(i + 1, j + 1, t, l0, k, n)
     s.init_run()
     gx, gy, gz = spec.grid
-    s.add_run_flags(spec.gamma, spec.beta, spec.rcut, spec.rswitch, 0, 1,
+    s.add_run_flags(spec.gamma, spec.beta, spec.rcut, spec.rswitch, spec.vdw_method, spec.elec_method,
                     spec.grid_space if gx <= 0 else -1.0, gx, gy, gz, spec.order, spec.shake_tol)
     s.add_run_dynopts(0, 0, spec.nsteps, spec.dt, spec.temperature)
     s.set_seed(spec.seed)
