@@ -117,9 +117,12 @@ __global__ void __launch_bounds__(256) dd_scatter(Dev d) {
 }
 
 // One workgroup per column: rank atoms by (z fraction, atom index), place
-// them into 32-slot blocks, then the exact rebuild box of every block.
+// them into 32-slot blocks, then the exact rebuild box of every block from
+// the fractions held in LDS (one half-wave per block).
 __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
   __shared__ unsigned long long key[kMaxColumnAtoms];
+  __shared__ unsigned sx[kMaxColumnAtoms], sy[kMaxColumnAtoms];
+  __shared__ short sorted[kMaxColumnAtoms];
   const int c = blockIdx.x;
   const int start = d.colstart[c], cnt = d.colcount[c];
   const int b0 = d.blkstart[c], nbc = d.blkstart[c + 1] - b0;
@@ -130,6 +133,8 @@ __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
     int a = d.bucket[start + k];
     key[k] = ((unsigned long long)d.ufrac[2 * d.n + a] << 32) | (unsigned)a;
+    sx[k] = d.ufrac[a];
+    sy[k] = d.ufrac[d.n + a];
   }
   __syncthreads();
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
@@ -140,6 +145,7 @@ __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
     int slot = (b0 + rank / kBlockAtoms) * kBlockAtoms + rank % kBlockAtoms;
     d.l2g[slot] = a;
     d.g2s[a] = slot;
+    sorted[rank] = (short)k;
   }
   for (int s = threadIdx.x; s < nbc * kBlockAtoms; s += blockDim.x)
     if (s >= cnt) d.l2g[b0 * kBlockAtoms + s] = -1;
@@ -148,30 +154,39 @@ __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
   if (c == 0)
     for (int s = *d.nblocks * kBlockAtoms + threadIdx.x; s < d.maxblocks * kBlockAtoms; s += blockDim.x)
       d.l2g[s] = -1;
-  __threadfence_block();
   __syncthreads();
-  for (int t = threadIdx.x; t < nbc * 3; t += blockDim.x) {
-    int b = b0 + t / 3, k = t % 3;
-    unsigned lo = 0xFFFFFFFFu, hi = 0;
-    for (int l = 0; l < kBlockAtoms; l++) {
-      int a = d.l2g[b * kBlockAtoms + l];
-      if (a < 0) continue;
-      unsigned w = d.ufrac[k * d.n + a];
-      lo = min(lo, w);
-      hi = max(hi, w);
+  // block boxes: half-wave h of the workgroup takes blocks h, h+8, ...
+  const int h = threadIdx.x >> 5, l = threadIdx.x & 31;
+  for (int bb = h; bb < ((nbc + 7) / 8) * 8; bb += 8) {
+    const int r = bb * kBlockAtoms + l;
+    const bool in = bb < nbc && r < cnt;
+    unsigned lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    if (in) {
+      const int k = sorted[r];
+      const unsigned u[3] = {sx[k], sy[k], (unsigned)(key[k] >> 32)};
+      for (int q = 0; q < 3; q++) lo[q] = hi[q] = u[q];
     }
-    d.bblo[3 * b + k] = lo;
-    d.bbhi[3 * b + k] = hi;
+    for (int q = 0; q < 3; q++)
+      for (int o = 16; o > 0; o >>= 1) {
+        lo[q] = min(lo[q], (unsigned)__shfl_xor((int)lo[q], o, 32));
+        hi[q] = max(hi[q], (unsigned)__shfl_xor((int)hi[q], o, 32));
+      }
+    if (bb < nbc && l < 3) {
+      d.bblo[3 * (b0 + bb) + l] = l == 0 ? lo[0] : l == 1 ? lo[1] : lo[2];
+      d.bbhi[3 * (b0 + bb) + l] = l == 0 ? hi[0] : l == 1 ? hi[1] : hi[2];
+    }
   }
 }
 
-// periodic gap between the rebuild boxes of blocks I and J, squared, in A^2
-__device__ __forceinline__ bool dd_boxes_close(const Dev &d, int I, int J) {
+// Periodic gap between two rebuild boxes (fractional counts), squared, in
+// A^2, against (rcut+pad)^2; rounding as in the oracle (no contraction).
+__device__ __forceinline__ bool dd_close(const Dev &d, const unsigned (&aLo)[3], const unsigned (&aHi)[3],
+                                         const unsigned (&bLo)[3], const unsigned (&bHi)[3]) {
   const unsigned long long P = 1ull << 32;
   double g[3];
+#pragma unroll
   for (int k = 0; k < 3; k++) {
-    unsigned long long aL = d.bblo[3 * I + k], aH = d.bbhi[3 * I + k];
-    unsigned long long bL = d.bblo[3 * J + k], bH = d.bbhi[3 * J + k];
+    unsigned long long aL = aLo[k], aH = aHi[k], bL = bLo[k], bH = bHi[k];
     unsigned long long gap = 0;
     if (aH < bL) gap = min(bL - aH, aL + P - bH);
     else if (bH < aL) gap = min(aL - bH, bL + P - aH);
@@ -182,16 +197,49 @@ __device__ __forceinline__ bool dd_boxes_close(const Dev &d, int I, int J) {
   return s <= d.rlist2;
 }
 
+constexpr int kBoxLds = 2048;   // block boxes staged in LDS by the tile kernels
+
+// stage the first kBoxLds block boxes in LDS (every thread of the block
+// must call it)
+__device__ __forceinline__ void dd_stage_boxes(const Dev &d, int nb, unsigned *lo, unsigned *hi) {
+  const int m = 3 * min(nb, kBoxLds);
+  for (int k = threadIdx.x; k < m; k += blockDim.x) {
+    lo[k] = d.bblo[k];
+    hi[k] = d.bbhi[k];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool dd_tile_keep(const Dev &d, const unsigned *lo, const unsigned *hi, int I,
+                                             const unsigned (&aLo)[3], const unsigned (&aHi)[3], int J) {
+  if (J == I) return true;
+  unsigned bLo[3], bHi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    bLo[k] = J < kBoxLds ? lo[3 * J + k] : d.bblo[3 * J + k];
+    bHi[k] = J < kBoxLds ? hi[3 * J + k] : d.bbhi[3 * J + k];
+  }
+  return dd_close(d, aLo, aHi, bLo, bHi);
+}
+
 // one wave per i-block: number of blocks J within the list radius (all J)
 __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
+  __shared__ unsigned s_lo[3 * kBoxLds], s_hi[3 * kBoxLds];
+  const int nb = *d.nblocks;
+  dd_stage_boxes(d, nb, s_lo, s_hi);
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int nb = *d.nblocks;
   if (I >= nb) return;
+  unsigned aLo[3], aHi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    aLo[k] = d.bblo[3 * I + k];
+    aHi[k] = d.bbhi[3 * I + k];
+  }
   int cnt = 0;
   for (int J0 = 0; J0 < nb; J0 += 64) {
-    int J = J0 + lane;
-    bool keep = J < nb && (J == I || dd_boxes_close(d, I, J));
+    const int J = J0 + lane;
+    const bool keep = J < nb && dd_tile_keep(d, s_lo, s_hi, I, aLo, aHi, J);
     cnt += __popcll(__ballot(keep));
   }
   if (lane == 0) {
@@ -211,16 +259,24 @@ __global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
 }
 
 __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
+  __shared__ unsigned s_lo[3 * kBoxLds], s_hi[3 * kBoxLds];
+  const int nb = *d.nblocks;
+  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
+  dd_stage_boxes(d, nb, s_lo, s_hi);
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int nb = *d.nblocks;
   if (I >= nb) return;
-  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
+  unsigned aLo[3], aHi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    aLo[k] = d.bblo[3 * I + k];
+    aHi[k] = d.bbhi[3 * I + k];
+  }
   int base = d.tstart[I];
   const unsigned long long below = (1ull << lane) - 1;
   for (int J0 = 0; J0 < nb; J0 += 64) {
-    int J = J0 + lane;
-    bool keep = J < nb && (J == I || dd_boxes_close(d, I, J));
+    const int J = J0 + lane;
+    const bool keep = J < nb && dd_tile_keep(d, s_lo, s_hi, I, aLo, aHi, J);
     unsigned long long m = __ballot(keep);
     if (keep) {
       int t = base + __popcll(m & below);

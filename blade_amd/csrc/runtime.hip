@@ -51,6 +51,8 @@ double bspline_host(int n, double u) {
 struct bm_system {
   int device = 0;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;         // side branch of the force DAG (default priority)
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   blade::Dev d{};
   int maxBlocks = 0;
   long long maxTiles = 0;
@@ -82,13 +84,19 @@ struct bm_system {
     return BM_OK;
   }
 
-  // all force terms: nbdirect chunk partials, bonded slots and the PME
-  // grid, then one per-atom assembly into the force array
-  // (kick: the assembly also kicks the atoms of a non-energy step)
+  // all force terms: nbdirect chunk partials on the main stream while the
+  // bonded slots and the PME grid run on the side stream (they only read
+  // positions), then one per-atom assembly into the force array after both
+  // (kick: the assembly also kicks the atoms of a non-energy step).  Under
+  // stream capture the fork/join becomes two parallel graph branches.
   void force_launches(bool energy, bool kick = false) {
+    (void)hipEventRecord(evFork, st);
+    (void)hipStreamWaitEvent(st2, evFork, 0);
+    blade::launch_bonded(d, energy, bias, st2);
+    if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st2);
+    (void)hipEventRecord(evJoin, st2);
     blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
-    blade::launch_bonded(d, energy, bias, st);
-    if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st);
+    (void)hipStreamWaitEvent(st, evJoin, 0);
     blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st, kick);
   }
 
@@ -101,6 +109,7 @@ struct bm_system {
   }
 
   ~bm_system() {
+    if (st2) (void)hipStreamSynchronize(st2);
     if (st) (void)hipStreamSynchronize(st);   // no graph may still run
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++) {
@@ -109,7 +118,10 @@ struct bm_system {
       }
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (evFork) (void)hipEventDestroy(evFork);
+    if (evJoin) (void)hipEventDestroy(evJoin);
     for (void *p : allocs) (void)hipFree(p);
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -142,7 +154,10 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     delete s;
     return rc;
   };
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->evJoin, hipEventDisableTiming) != hipSuccess)
     return fail(set_err(BM_EHIP, "stream"));
   for (auto &e : s->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(set_err(BM_EHIP, "event"));
