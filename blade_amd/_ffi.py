@@ -137,6 +137,12 @@ def lib():
         "blade_prepare": (i, [vp]),
         "blade_set_seed": (None, [vp, C.c_uint64]),
         "blade_set_domdec_frequency": (None, [vp, i]),
+        "blade_read_parameter_file": (i, [vp, C.c_char_p]),
+        "blade_read_psf": (i, [vp, C.c_char_p]),
+        "blade_read_crd": (i, [vp, C.c_char_p]),
+        "blade_read_pdb": (i, [vp, C.c_char_p]),
+        "blade_structure_atom_count": (i, [vp]),
+        "blade_structure_atom_names": (i, [vp, i, C.c_char_p, C.c_char_p, C.c_char_p, i]),
         # kernel layer
         "bm_version": (s, []),
         "bm_last_error": (s, []),

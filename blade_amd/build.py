@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libblade_mi355x.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 HIP_SOURCES = ["domdec.hip", "nbdirect.hip", "pme.hip", "bonded.hip", "update.hip", "runtime.hip"]
-CXX_SOURCES = ["setup.cpp", "api.cpp"]
+CXX_SOURCES = ["setup.cpp", "api.cpp", "files.cpp"]
 
 # gfx950 only; packed FP32 instructions disabled
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

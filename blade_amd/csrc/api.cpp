@@ -46,6 +46,12 @@ static void api_fail(System *s, const std::string &m) {
   blade_log(("BLaDE-MI355X error: " + m + "\n").c_str());
 }
 
+namespace blade {
+// error path shared with the file readers (files.cpp)
+void api_report_error(System *s, const std::string &m) { api_fail(s, m); }
+bool api_has_structure(System *s) { return s && s->structure; }
+}  // namespace blade
+
 static void mark_unsupported(System *s, const char *what) {
   s->unsupported = true;
   s->unsupportedWhat = what;
@@ -126,6 +132,15 @@ void blade_add_impr(System *s, int i, int j, int k, int l) {
 }
 void blade_add_cmap(System *s, int, int, int, int, int, int, int, int) { mark_unsupported(s, "CMAP terms"); }
 void blade_add_shake(System *s, int shakeHbond) { s->structure->shakeHbond = shakeHbond != 0; }
+int blade_structure_atom_count(System *s) { return (s && s->structure) ? (int)s->structure->atoms.size() : 0; }
+int blade_structure_atom_names(System *s, int atomIdx, char *seg, char *res, char *name, int len) {
+  if (!s || !s->structure || atomIdx < 1 || atomIdx > (int)s->structure->atoms.size() || len <= 0) return -1;
+  const AtomRec &a = s->structure->atoms[atomIdx - 1];
+  std::snprintf(seg, len, "%s", a.segName.c_str());
+  std::snprintf(res, len, "%s", a.resIdx.c_str());
+  std::snprintf(name, len, "%s", a.atomName.c_str());
+  return 0;
+}
 
 // ---- coordinates -----------------------------------------------------------
 void blade_init_coordinates(System *s, int n) {
@@ -153,10 +168,12 @@ void blade_add_coordinates_box(System *s, int name, double a, double b, double c
 }
 
 // ---- msld ------------------------------------------------------------------
+// nblocks substituent blocks plus the environment block 0
+// (msld/msld.cu:1555-1563: blockCount = nblocks + 1)
 void blade_init_msld(System *s, int nblocks) {
   s->msld.reset(new MsldSetup());
   int n = s->structure ? (int)s->structure->atoms.size() : 0;
-  s->msld->resize(nblocks < 1 ? 1 : nblocks, n);
+  s->msld->resize(nblocks < 0 ? 1 : nblocks + 1, n);
 }
 void blade_dest_msld(System *s) { s->msld.reset(new MsldSetup()); }
 void blade_add_msld_atomassignment(System *s, int atomIdx, int blockIdx) {
@@ -311,7 +328,16 @@ int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
 
 int blade_get_energy(System *s, double *e) { return s && s->gpu ? bm_get_energy(s->gpu, e) : BM_EINVAL; }
 int blade_get_forces(System *s, double *f) { return s && s->gpu ? bm_get_forces(s->gpu, f, nullptr) : BM_EINVAL; }
-int blade_get_positions(System *s, double *p) { return s && s->gpu ? bm_get_positions(s->gpu, p, nullptr) : BM_EINVAL; }
+// before the device system exists: the coordinates as supplied (SoA)
+int blade_get_positions(System *s, double *p) {
+  if (!s || !p) return BM_EINVAL;
+  if (s->gpu) return bm_get_positions(s->gpu, p, nullptr);
+  if (!s->coordinates) return BM_EINVAL;
+  const int n = s->coordinates->n;
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) p[(size_t)k * n + i] = s->coordinates->pos[3 * (size_t)i + k];
+  return BM_OK;
+}
 int blade_set_positions(System *s, const double *p) {
   if (!s) return BM_EINVAL;
   int rc = ensure_gpu(s);

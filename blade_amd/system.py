@@ -126,6 +126,22 @@ class BladeSystem:
     def set_domdec_frequency(self, steps):
         self.L.blade_set_domdec_frequency(self.h, steps)
 
+    def _read(self, fn, path):
+        if fn(self.h, str(path).encode()) != 0:
+            raise RuntimeError(self.L.blade_last_error().decode())
+
+    def read_parameter_file(self, path):
+        self._read(self.L.blade_read_parameter_file, path)
+
+    def read_psf(self, path):
+        self._read(self.L.blade_read_psf, path)
+
+    def read_crd(self, path):
+        self._read(self.L.blade_read_crd, path)
+
+    def read_pdb(self, path):
+        self._read(self.L.blade_read_pdb, path)
+
     def run_energy(self):
         self.L.blade_run_energy(self.h)
         err = self.L.blade_last_error().decode()
@@ -229,7 +245,7 @@ def build(spec, gpus=(0,)):
         if spec.vel is not None:
             s.add_coordinates_velocity(i + 1, *spec.vel[i])
     s.add_coordinates_box(0, *spec.box)
-    s.init_msld(spec.nblocks)
+    s.init_msld(spec.nblocks - 1)          # substituent blocks; block 0 is the environment
     if spec.nblocks > 1:
         for i in np.nonzero(spec.atom_block)[0]:
             s.add_msld_atomassignment(int(i) + 1, int(spec.atom_block[i]) + 1)
@@ -248,6 +264,34 @@ def build(spec, gpus=(0,)):
     s.set_seed(spec.seed)
     s.n = spec.n
     s.nblocks = spec.nblocks
+    return s
+
+
+def load_charmm(prm, psf, crd, box, shake=True, gpus=(0,), dt_ps=0.002, temperature=300.0, gamma_ps=1.0,
+                beta=1.0 / 3.2, rcut=10.0, rswitch=8.5, grid_space=1.0, order=6, seed=0xB1ADE000):
+    """BladeSystem from CHARMM files, as the script commands `parameters
+    file`, `structure file psf`, `structure shake hbond`, `coordinates file
+    crd`, `coordinates box`, `msld nblocks 0` and `run setvariable` would
+    set it up (no lambda sites)."""
+    from .synthetic import PICOSECOND
+    s = BladeSystem(gpus)
+    s.init_parameters()
+    for p in ([prm] if isinstance(prm, (str, bytes)) or not hasattr(prm, "__iter__") else prm):
+        s.read_parameter_file(p)
+    s.read_psf(psf)
+    s.add_shake(1 if shake else 0)
+    if crd.lower().endswith(".pdb"):
+        s.read_pdb(crd)
+    else:
+        s.read_crd(crd)
+    s.add_coordinates_box(0, *box)
+    s.init_msld(0)
+    s.init_run()
+    s.add_run_flags(gamma_ps / PICOSECOND, beta, rcut, rswitch, 0, 1, grid_space, -1, -1, -1, order, 2e-7)
+    s.add_run_dynopts(0, 0, 0, dt_ps * PICOSECOND, temperature)
+    s.set_seed(seed)
+    s.n = s.L.blade_structure_atom_count(s.h)
+    s.nblocks = 1
     return s
 
 

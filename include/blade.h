@@ -136,6 +136,19 @@ void blade_set_seed(System *system, unsigned long long seed);
    1 rebuilds every step, for starts whose atoms outrun the thermal list pad */
 void blade_set_domdec_frequency(System *system, int steps);
 
+/* CHARMM input files, read through the setters above (the script commands
+   "parameters file", "structure file psf", "coordinates file crd|pdb":
+   system/parameters.cxx:129-484, system/structure.cxx:566-826,
+   system/coordinates.cxx:152-274).  0 on success. */
+int blade_read_parameter_file(System *system, const char *path);
+int blade_read_psf(System *system, const char *path);
+int blade_read_crd(System *system, const char *path);
+int blade_read_pdb(System *system, const char *path);
+/* structure queries: atom count, and (segment, residue id, atom name) of
+   a 1-based atom into buffers of len bytes */
+int blade_structure_atom_count(System *system);
+int blade_structure_atom_names(System *system, int atomIdx, char *seg, char *res, char *name, int len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,3 +82,19 @@ def test_per_term_calls_sum_to_full_force():
     assert L.bm_get_force_fixed(dev, parts.ctypes.data_as(C.POINTER(C.c_longlong))) == 0
     n3 = 3 * sp.n
     assert np.array_equal(parts[:n3], full[:n3])
+
+
+def test_reference_water216_deck():
+    """the reference's test/water216 PSF/CRD/PRM (fixtures), read from files"""
+    from tests.test_files import BOX, CRD, PRM, PSF
+    s = system.load_charmm(PRM, PSF, CRD, BOX, beta=1 / 3.125, rcut=9.0, rswitch=8.0)
+    s.run_energy()
+    top, rp = s.topology(), s.runparams()
+    fo, do, eo, pairs = orc.forces(top, rp, s.positions(), np.ones(1))
+    fg, eg = s.forces(), s.energy()
+    frms = rel_rms(fg, fo)
+    erel = abs(eg[20] - eo[20]) / abs(eo[20])
+    print("\n[water216 deck] pairs=%d force relRMS %.3e energy %.6f (oracle %.6f) rel %.3e"
+          % (pairs, frms, eg[20], eo[20], erel), flush=True)
+    assert frms <= FORCE_TOL
+    assert erel <= ENERGY_TOL
