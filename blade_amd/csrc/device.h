@@ -127,6 +127,8 @@ struct Dev {
   int4 *br2_i; double2 *br2_b;
   int4 *br3_i; double4 *br3_b;
   double *mass;                  // [N]
+  int nfree;                     // atoms in no constraint group
+  int *freeatom;                 // [nfree]
   // msld
   int *block_site, *site_bound, *block_fixed;
   double *fixed_bias;

@@ -496,6 +496,25 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     UPLOAD(d.br1_i, b1i); UPLOAD(d.br1_b, b1b);
     UPLOAD(d.br2_i, b2i); UPLOAD(d.br2_b, b2b);
     UPLOAD(d.br3_i, b3i); UPLOAD(d.br3_b, b3b);
+    // atoms in no group; the step kernels rely on every atom being in at
+    // most one group (they move a group's atoms in one thread)
+    std::vector<int> owner(n, 0);
+    auto own = [&](int a) { return ++owner[a] == 1; };
+    bool disjoint = true;
+    for (int c = 0; c < d.ntri; c++)
+      for (int k = 0; k < 3; k++) disjoint &= own(t->tri_idx[3 * c + k]);
+    for (int c = 0; c < d.nbr1; c++)
+      for (int k = 0; k < 2; k++) disjoint &= own(t->br1_idx[2 * c + k]);
+    for (int c = 0; c < d.nbr2; c++)
+      for (int k = 0; k < 3; k++) disjoint &= own(t->br2_idx[3 * c + k]);
+    for (int c = 0; c < d.nbr3; c++)
+      for (int k = 0; k < 4; k++) disjoint &= own(t->br3_idx[4 * c + k]);
+    if (!disjoint) return fail(set_err(BM_EINVAL, "an atom belongs to two constraint groups"));
+    std::vector<int> fr;
+    for (int a = 0; a < n; a++)
+      if (!owner[a]) fr.push_back(a);
+    d.nfree = (int)fr.size();
+    UPLOAD(d.freeatom, fr);
   }
   // msld
   {

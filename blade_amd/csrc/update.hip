@@ -104,10 +104,9 @@ __global__ void upd_kick(Dev d, int mode) {
   }
 }
 
-__global__ void upd_o_drift(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N1 = 3 * d.n, N = N1 + d.nl;
-  if (i >= N) return;
+// O (friction + Philox noise), backup, half drift R of DOF i
+__device__ __forceinline__ void o_drift_dof(const Dev &d, int i) {
+  const int N1 = 3 * d.n;
   const double ism = d.ism[i];
   const double x = d.pos[i];
   d.posb[i] = x;
@@ -123,6 +122,11 @@ __global__ void upd_o_drift(Dev d) {
     d.pos[i] = x + d.halfdt * v;
     nan_check(d, v, i < N1 ? i % d.n : i);
   }
+}
+
+__global__ void upd_o_drift(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * d.n + d.nl) o_drift_dof(d, i);
 }
 
 // ---- constraints ------------------------------------------------------
@@ -287,6 +291,17 @@ __device__ __forceinline__ void store_pos(const Dev &d, const int (&a)[NA], cons
   }
 }
 
+// O and half drift of the three DOFs of each atom of a group, right after
+// the group's constraint step (same thread, so its writes are visible)
+template <int NA>
+__device__ __forceinline__ void o_drift_atoms(const Dev &d, const int (&a)[NA]) {
+#pragma unroll
+  for (int k = 0; k < NA; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o_drift_dof(d, c * d.n + a[k]);
+}
+
+template <bool ODRIFT>
 __device__ void constrain_group(const Dev &d, int gi) {
   if (gi < d.ntri) {
     const int4 ix = d.tri_i[gi];
@@ -298,6 +313,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
     if (g.m[1] == g.m[2] && b0[0] == b0[1]) settle(b0, g.xn, g.xp, g.m);
     else shake_newton<3, 3, true>(b0, g.xn, g.xp, g.im, d.shaketol);
     store_pos(d, a, g);
+    if (ODRIFT) o_drift_atoms(d, a);
     return;
   }
   gi -= d.ntri;
@@ -313,6 +329,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
     g.xn[0] = g.xn[0] + (g.im[0] * lw) * gp;
     g.xn[1] = g.xn[1] + (-g.im[1] * lw) * gp;
     store_pos(d, a, g);
+    if (ODRIFT) o_drift_atoms(d, a);
     return;
   }
   gi -= d.nbr1;
@@ -325,6 +342,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
     load_pos(d, a, g);
     shake_newton<3, 2, false>(b0, g.xn, g.xp, g.im, d.shaketol);
     store_pos(d, a, g);
+    if (ODRIFT) o_drift_atoms(d, a);
     return;
   }
   gi -= d.nbr2;
@@ -336,6 +354,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
   load_pos(d, a, g);
   shake_newton<4, 3, false>(b0, g.xn, g.xp, g.im, d.shaketol);
   store_pos(d, a, g);
+  if (ODRIFT) o_drift_atoms(d, a);
 }
 
 // theta -> lambda for site s (msld.cu:863-898)
@@ -375,18 +394,11 @@ __device__ void theta_kick(const Dev &d, int b) {
   }
 }
 
-// groups, then (last call of a step) sites and the step counter;
-// finish == 2: first call of a step whose atoms were kicked by the force
-// assembly: the theta DOFs are kicked here
+// groups, then (last call of a step) sites and the step counter
 __global__ void __launch_bounds__(64) upd_constrain(Dev d, int finish) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  if (t < ng) constrain_group(d, t);
-  if (finish == 2) {
-    const int b = t - ng;
-    if (b >= 0 && b < d.nl && d.nl > 1) theta_kick(d, b);
-    return;
-  }
+  if (t < ng) constrain_group<false>(d, t);
   if (finish) {
     const int s = t - ng;
     if (s >= 0 && s < d.nsites && d.nl > 1) theta_to_lambda(d, s);
@@ -495,13 +507,35 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
   upd_constrain<<<gF, 64, 0, st>>>(d, 1);
 }
 
+// First half of a non-energy step after the force assembly kicked the
+// atoms: each constraint group, then O and half drift of its atoms in the
+// same thread; the unconstrained atoms and the theta DOFs (kick, O, drift)
+// take the threads after the groups.
+__global__ void __launch_bounds__(64) upd_constrain_o(Dev d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  if (t < ng) {
+    constrain_group<true>(d, t);
+    return;
+  }
+  const int f = t - ng;
+  if (f < d.nfree) {
+    const int a = d.freeatom[f];
+#pragma unroll
+    for (int c = 0; c < 3; c++) o_drift_dof(d, c * d.n + a);
+    return;
+  }
+  const int b = f - d.nfree;
+  if (b < d.nl && d.nl > 1) {
+    theta_kick(d, b);
+    o_drift_dof(d, 3 * d.n + b);
+  }
+}
+
 // the rest of a non-energy step after the force assembly kicked the atoms
 void launch_update_after_kick(const Dev &d, hipStream_t st) {
-  const int N = 3 * d.n + d.nl;
-  const unsigned gN = (N + 255) / 256;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  upd_constrain<<<(ng + d.nl + 63) / 64, 64, 0, st>>>(d, 2);
-  upd_o_drift<<<gN, 256, 0, st>>>(d);
+  upd_constrain_o<<<(ng + d.nfree + d.nl + 63) / 64, 64, 0, st>>>(d);
   upd_constrain<<<(ng + d.nsites + 1 + 63) / 64, 64, 0, st>>>(d, 1);
 }
 
