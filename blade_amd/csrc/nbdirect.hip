@@ -91,14 +91,24 @@ __device__ __forceinline__ void pair_ef(const NbConst &c, float qq, float c12, f
       f = -qq * rinv * rinv;
       eq = rinv + c.dvc;
     } else {
-      float r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
-      f = -qq * rinv * (c.A * rinv + c.B * r + 3 * c.C * r3 + 5 * c.D * r5);
-      eq = c.A * (rinv - c.rcinv) + c.B * (c.rc - r) + c.C * (c.c2of * c.rc - r3) +
-           c.D * (c.c2of * c.c2of * c.rc - r5);
+      // A/r^2 + B + 3C r^2 + 5D r^4 = g (roff^2-r^2)^2 (roff^2+2r^2-3ron^2)/r^2
+      // with g = 1/(roff^2-ron^2)^3: the product form has no cancellation
+      const float r2 = r * r, du = c.c2of - r2;
+      f = -qq * rinv * rinv * c.rul3inv * du * du * (c.c2of + 2 * r2 - 3 * c.c2on);
+      // the energy polynomial cancels to a few digits: FP64 (it is only
+      // formed for the lambda derivative of alchemical pairs)
+      const double rcd = c.rc, rsd = c.rs, roff2 = rcd * rcd, ron2 = rsd * rsd;
+      const double g = 1.0 / ((roff2 - ron2) * (roff2 - ron2) * (roff2 - ron2));
+      const double A = roff2 * roff2 * (roff2 - 3 * ron2) * g, B = 6 * roff2 * ron2 * g;
+      const double C = -(ron2 + roff2) * g, D = 0.4 * g;
+      const double rd = r, r3 = rd * rd * rd, r5 = r3 * rd * rd;
+      eq = (float)(A * (1.0 / rd - 1.0 / rcd) + B * (rcd - rd) + C * (roff2 * rcd - r3) +
+                   D * (roff2 * roff2 * rcd - r5));
     }
   } else {
     f = -qq * (rinv * rinv - c.rcinv * c.rcinv);
-    eq = rinv - 2 * c.rcinv + r * c.rcinv * c.rcinv;
+    const double rd = r, rci = 1.0 / (double)c.rc;
+    eq = (float)(1.0 / rd - 2 * rci + rd * rci * rci);
   }
   float rinv3 = rinv * rinv * rinv, rinv6 = rinv3 * rinv3;
   if (VDW == BM_VFSWITCH) {
@@ -123,11 +133,14 @@ __device__ __forceinline__ void pair_ef(const NbConst &c, float qq, float c12, f
       eB = fsw * rinv6;
     }
   } else {
-    float r2 = r * r, r5 = r2 * r2 * r, r6 = r2 * r2 * r2;
+    float r2 = r * r, r5 = r2 * r2 * r;
     float rinv7 = rinv6 * rinv, rinv12 = rinv6 * rinv6, rinv13 = rinv12 * rinv;
     f += 6 * c6 * (rinv7 - r5 * c.rcinv12) - 12 * c12 * (rinv13 + r5 * c.rcinv18);
-    eA = rinv12 + 2 * r6 * c.rcinv18 - 3 * c.rcinv12;
-    eB = rinv6 + r6 * c.rcinv12 - 2 * c.rcinv6;
+    // shifted energies vanish as (rc-r)^2: FP64 (alchemical pairs only)
+    const double ri = 1.0 / (double)r, ri6 = ri * ri * ri * ri * ri * ri, rd = r;
+    const double r6d = rd * rd * rd * rd * rd * rd;
+    eA = (float)(ri6 * ri6 + 2 * r6d * (double)c.rcinv18 - 3 * (double)c.rcinv12);
+    eB = (float)(ri6 + r6d * (double)c.rcinv12 - 2 * (double)c.rcinv6);
   }
 }
 
