@@ -37,6 +37,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic():
+    """HBM bytes per nbdirect launch from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a calibrated copy kernel), or
+    None when there is no summary"""
+    import re
+    path = os.path.join(ROOT, "profiles", "traffic_msld23558.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^nbdirect_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +62,10 @@ def main():
                          "(for kernel tracers that do not follow graph launches)")
     ap.add_argument("--state", default=None,
                     help="npz file holding the relaxed state: loaded when present, written otherwise")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead: N independent MSLD systems (3,000-25,000 atoms, 2-6 substituents) "
+                         "advanced by one graph launch per step, against the same systems run one by one")
+    ap.add_argument("--state-dir", default=None, help="relaxed batch member states (npz per member)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one system on one GPU; --gpus %d is not supported "
@@ -59,6 +75,8 @@ def main():
 
     from blade_amd import synthetic, system, _ffi
     from blade_amd._ffi import check, ptr
+    if a.batch:
+        return batch_main(a)
 
     t0 = time.time()
     spec = synthetic.config2()
@@ -163,7 +181,7 @@ def main():
         "hbm_copy_GBps": round(copy_gbps, 1),
         "roofline": {"kernel": "nbdirect", "bound": "mfma", "achieved": round(achieved / 1e12, 4),
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 5),
-                     "traffic": None,
+                     "traffic": profiled_traffic(),
                      "note": "compute roof; nbdirect runs on the FP32 vector ALUs, whose peak equals "
                              "the listed FP32 figure",
                      "work": "%d in-cutoff pairs x %d FLOP per launch" % (pairs, FLOP_PER_PAIR)},
@@ -171,6 +189,87 @@ def main():
     }
     print(json.dumps(line), flush=True)
     s.close()
+    return 0
+
+
+def _member(args):
+    from blade_amd import synthetic
+    natoms, nsubs, seed = args
+    return synthetic.batch_member(seed, natoms, nsubs)
+
+
+def _relaxed(spec, path, log_fn):
+    from blade_amd import system
+    if path and os.path.exists(path):
+        z = np.load(path)
+        spec.pos, spec.vel = z["pos"], z["vel"]
+        spec.theta0, spec.theta_v = list(z["theta"]), list(z["theta_v"])
+        return spec
+    spec = system.relax(spec)
+    if path:
+        np.savez(path, pos=spec.pos, vel=spec.vel, theta=np.array(spec.theta0), theta_v=np.array(spec.theta_v))
+    return spec
+
+
+def batch_main(a):
+    """aggregate ns/day of N systems: one graph launch per step for all of
+    them, against the same systems advanced one after another"""
+    import concurrent.futures as cf
+    from blade_amd import synthetic, system
+    from blade_amd._ffi import check
+    members = synthetic.batch_set(a.batch)
+    t0 = time.time()
+    with cf.ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        specs = list(ex.map(_member, members))
+    log("generated %d systems (%d atoms) in %.1f s" % (len(specs), sum(s.n for s in specs), time.time() - t0))
+    if a.state_dir:
+        os.makedirs(a.state_dir, exist_ok=True)
+    t0 = time.time()
+    specs = [_relaxed(sp, os.path.join(a.state_dir, "m%02d.npz" % k) if a.state_dir else None, log)
+             for k, sp in enumerate(specs)]
+    log("relaxed in %.1f s" % (time.time() - t0))
+    systems = [system.build(sp) for sp in specs]
+    for s in systems:
+        s.run_energy()
+    dt_fs = specs[0].dt / synthetic.PICOSECOND * 1000.0
+    # one by one: every member's own graphs, members in turn
+    for s in systems:
+        check(s.L.bm_dynamics(s.device(), 1, a.warmup, FREQ_NRG, 1), "warmup")
+        check(s.L.bm_sync(s.device()), "sync")
+    t1 = time.perf_counter()
+    for s in systems:
+        check(s.L.bm_dynamics(s.device(), 1 + a.warmup, a.steps, FREQ_NRG, 1), "plain steps")
+    for s in systems:
+        check(s.L.bm_sync(s.device()), "sync")
+    t_plain = time.perf_counter() - t1
+    # batched: one graph launch per step advances every member
+    b = system.Batch(systems)
+    step0 = 1 + a.warmup + a.steps
+    b.dynamics(step0, a.warmup, FREQ_NRG)
+    b.sync()
+    t1 = time.perf_counter()
+    b.dynamics(step0 + a.warmup, a.steps, FREQ_NRG)
+    b.sync()
+    t_batch = time.perf_counter() - t1
+    n = len(systems)
+    agg = lambda t: n * a.steps / t * dt_fs * 86400e-6
+    line = {
+        "metric": "aggregate ns/day at 2 fs, %d batched MSLD systems" % n,
+        "value": round(agg(t_batch), 2), "unit": "ns/day", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(t_batch * 1e3 / a.steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 pair/spline math, fp64 integration, 64-bit fixed-point accumulation",
+        "data": "synthetic (seeded generator, GPU-relaxed)",
+        "config": {"workload": "batch-%d-msld" % n, "atoms_total": sum(s.n for s in specs),
+                   "atoms_range": [min(s.n for s in specs), max(s.n for s in specs)],
+                   "substituents_range": [min(m[1] for m in members), max(m[1] for m in members)],
+                   "cutoff_A": 10.0, "grid_spacing_A": 1.0, "graph": True},
+        "one_by_one": {"value": round(agg(t_plain), 2), "ms_per_step_all": round(t_plain * 1e3 / a.steps, 5)},
+        "batched_over_one_by_one": round(t_plain / t_batch, 3),
+    }
+    print(json.dumps(line), flush=True)
+    b.close()
+    for s in systems:
+        s.close()
     return 0
 
 

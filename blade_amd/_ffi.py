@@ -159,6 +159,10 @@ def lib():
         "bm_dynamics": (i, [vp, C.c_long, C.c_long, i, i]),
         "bm_profile_steps": (i, [vp, C.c_long, C.c_long, _f64p]),
         "bm_calibration_copy": (i, [i, C.c_size_t, i, _f64p]),
+        "bm_batch_create": (i, [i, C.POINTER(vp), C.POINTER(vp)]),
+        "bm_batch_dynamics": (i, [vp, C.c_long, C.c_long, i]),
+        "bm_batch_sync": (i, [vp]),
+        "bm_batch_destroy": (i, [vp]),
         "bm_sync": (i, [vp]),
         "bm_get_positions": (i, [vp, _f64p, _f64p]),
         "bm_set_positions": (i, [vp, _f64p, _f64p]),

@@ -687,6 +687,117 @@ int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_gra
   return BM_OK;
 }
 
+// ---------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------
+}  // extern "C"
+
+struct bm_batch {
+  std::vector<bm_system *> sys;
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  ~bm_batch() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        if (graph[a][b]) (void)hipGraphExecDestroy(graph[a][b]);
+        if (graphSrc[a][b]) (void)hipGraphDestroy(graphSrc[a][b]);
+      }
+    for (auto e : join)
+      if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" {
+
+int bm_batch_create(int n, bm_system *const *systems, bm_batch **out) {
+  if (n <= 0 || !systems || !out) return set_err(BM_EINVAL, "batch arguments");
+  *out = nullptr;
+  for (int k = 0; k < n; k++) {
+    if (!systems[k]) return set_err(BM_EINVAL, "null batch member");
+    if (systems[k]->device != systems[0]->device) return set_err(BM_EINVAL, "batch members on different devices");
+    if (systems[k]->freqDomdec != systems[0]->freqDomdec)
+      return set_err(BM_EINVAL, "batch members with different list rebuild cadence");
+  }
+  HIPCHK(hipSetDevice(systems[0]->device));
+  bm_batch *b = new bm_batch();
+  b->device = systems[0]->device;
+  b->sys.assign(systems, systems + n);
+  b->join.assign(n, nullptr);
+  bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; ok && k < n; k++) ok = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    delete b;
+    return set_err(BM_EHIP, "batch stream/events");
+  }
+  *out = b;
+  return BM_OK;
+}
+
+int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
+  if (!b || nsteps < 0) return BM_EINVAL;
+  if (freq_nrg <= 0) freq_nrg = 1 << 30;
+  HIPCHK(hipSetDevice(b->device));
+  // every member's step counter, ordered before the first graph launch;
+  // each member's own work is joined into the batch stream by the graph
+  for (bm_system *s : b->sys) {
+    HIPCHK(hipEventRecord(b->fork, s->st));
+    HIPCHK(hipStreamWaitEvent(b->st, b->fork, 0));
+  }
+  for (bm_system *s : b->sys) blade::launch_set_step(s->d, step0, b->st);
+  const int freqDomdec = b->sys[0]->freqDomdec;
+  for (long k = 0; k < nsteps; k++) {
+    const long step = step0 + k;
+    const int rb = (step % freqDomdec) == 0, en = (step % freq_nrg) == 0;
+    if (!b->graph[rb][en]) {
+      HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      HIPCHK(hipEventRecord(b->fork, b->st));
+      for (size_t m = 0; m < b->sys.size(); m++) {
+        bm_system *s = b->sys[m];
+        HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
+        s->step_launches(rb, en);
+        HIPCHK(hipEventRecord(b->join[m], s->st));
+        HIPCHK(hipStreamWaitEvent(b->st, b->join[m], 0));
+      }
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamEndCapture(b->st, &g));
+      b->graphSrc[rb][en] = g;
+      HIPCHK(hipGraphInstantiate(&b->graph[rb][en], g, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->graph[rb][en], b->st));
+  }
+  // members' later work (getters, eager steps) follows the batch
+  HIPCHK(hipEventRecord(b->fork, b->st));
+  for (bm_system *s : b->sys) HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
+  return BM_OK;
+}
+
+int bm_batch_sync(bm_batch *b) {
+  if (!b) return BM_EINVAL;
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (bm_system *s : b->sys) {
+    if (int rc = bm_sync(s)) return rc;
+    int nan = -1;
+    if (int rc = bm_get_nan_flag(s, &nan)) return rc;
+    if (nan != -1) return set_err(BM_ENAN, "NaN/Inf in a batch member");
+  }
+  return BM_OK;
+}
+
+int bm_batch_destroy(bm_batch *b) {
+  if (!b) return BM_EINVAL;
+  (void)hipSetDevice(b->device);
+  delete b;
+  return BM_OK;
+}
+
 // Per-phase timing with events on the system stream: nsteps eager steps
 // (rebuild every freq_domdec steps, no energies) with events around the
 // nbdirect launch and around the PME grid (bin, spread, FFT) plus the force

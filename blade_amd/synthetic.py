@@ -606,6 +606,48 @@ def config2(seed=0xB1ADE002, n_water=7023, n_poly=2489, box=62.23):
     return spec
 
 
+WATER_DENSITY = 0.0978   # atoms per A^3 in a TIP3P box
+
+
+def batch_member(seed, natoms, nsubs, core=None):
+    """One member of the batched workload: TIP3P water at box density plus
+    a ligand (a 30-60-atom core, seeded) carrying one lambda site of
+    `nsubs` substituents x 8 atoms; cutoff 10 A, switch 9-10 A, PME grid
+    spacing <= 1 A (the size rule picks the grid), order 6, fnex 5."""
+    rng = np.random.default_rng(seed)
+    if core is None:
+        core = int(rng.integers(30, 61))
+    n_lig = core + 8 * nsubs
+    n_water = (natoms - n_lig) // 3
+    box = ((3 * n_water + n_lig) / WATER_DENSITY) ** (1 / 3)
+    spec = Spec(name="batch-%d-%d" % (natoms, nsubs), box=(box,) * 3)
+    spec._pos = []
+    _water_params(spec)
+    g = _Grid(spec.box, 3.0)
+    radius = 6.0 * (n_lig / 60.0) ** (1 / 3) + 2.0
+    _, subs = _polymer(spec, rng, n_lig, np.array(spec.box) / 2, radius, g, site_at=0.5, nsubs=nsubs)
+    block_of = np.zeros(spec.n, dtype=np.int32)
+    for s, atoms in enumerate(subs):
+        block_of[atoms] = s + 1
+    _angles_dihes(spec, block_of)
+    _fill_water(spec, rng, n_water, g)
+    spec.pos = np.array(spec._pos)
+    _msld_site(spec, subs, fnex=5.0)
+    spec.rcut, spec.rswitch = 10.0, 9.0
+    spec.beta = 1.0 / 3.2
+    spec.grid, spec.grid_space, spec.order = (-1, -1, -1), 1.0, 6
+    spec.temperature = 298.0
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
+
+
+def batch_set(n=32, seed=0xB1ADE004, lo=3000, hi=25000):
+    """(natoms, nsubs, seed) of the n members: atom counts uniform in
+    [lo, hi], 2-6 substituents, seeded"""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(lo, hi + 1)), int(rng.integers(2, 7)), seed + 1 + k) for k in range(n)]
+
+
 def msld_small(seed=0xB1ADE0A0, n_water=150, n_poly=60, box=20.0, grid=20):
     """A small MSLD system (water + short polymer + one 3-block site) whose
     oracle evaluation takes well under a second."""

@@ -305,6 +305,30 @@ RELAX_SCHEDULE = ((0.1, 1000.0, 400, 1), (0.5, 200.0, 400, 1), (1.0, 50.0, 400, 
                   (2.0, 5.0, 300, 10))
 
 
+class Batch:
+    """Independent BladeSystems on one GPU advanced together: one graph
+    launch per step runs every member's step as a parallel branch."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        self.L = self.systems[0].L
+        devs = (C.c_void_p * len(self.systems))(*[s.device() for s in self.systems])
+        h = C.c_void_p()
+        check(self.L.bm_batch_create(len(self.systems), devs, C.byref(h)), "bm_batch_create")
+        self.h = h
+
+    def dynamics(self, step0, nsteps, freq_nrg=0):
+        check(self.L.bm_batch_dynamics(self.h, step0, nsteps, freq_nrg), "bm_batch_dynamics")
+
+    def sync(self):
+        check(self.L.bm_batch_sync(self.h), "bm_batch_sync")
+
+    def close(self):
+        if self.h:
+            self.L.bm_batch_destroy(self.h)
+            self.h = None
+
+
 def state(s, n, nblocks):
     """(pos (N,3), vel (N,3), theta, theta_v) of a running BladeSystem"""
     dev = s.device()

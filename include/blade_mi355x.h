@@ -210,8 +210,21 @@ int bm_msld_chain_rule(bm_system *sys);
 
 /* Run::dynamics step loop (run/run.cu:545-570) for nsteps steps starting
  * at `step0`; energies are computed on steps with step%freq_nrg==0.
- * use_graph = 1 replays a captured hipGraph of the non-rebuild steps. */
+ * use_graph = 1 replays captured hipGraphs, one per (rebuild, energy)
+ * step variant. */
 int bm_dynamics(bm_system *sys, long step0, long nsteps, int freq_nrg, int use_graph);
+
+/* ---- batches: independent systems advanced together.  One captured
+ * graph per (rebuild, energy) step variant holds every member's whole step
+ * as a parallel branch, so one graph launch advances all members.  Members
+ * must live on one device and share the list rebuild cadence.  No
+ * counterpart in the reference (it runs one system per GPU and process). */
+typedef struct bm_batch bm_batch;
+int bm_batch_create(int n, bm_system *const *systems, bm_batch **out);
+int bm_batch_dynamics(bm_batch *batch, long step0, long nsteps, int freq_nrg);
+/* synchronise every member; BM_ENAN / BM_EOVERFLOW as bm_sync reports them */
+int bm_batch_sync(bm_batch *batch);
+int bm_batch_destroy(bm_batch *batch);
 
 /* synchronise the system's stream */
 int bm_sync(bm_system *sys);
