@@ -237,9 +237,8 @@ def batch_main(a):
         check(s.L.bm_dynamics(s.device(), 1, a.warmup, FREQ_NRG, 1), "warmup")
         check(s.L.bm_sync(s.device()), "sync")
     t1 = time.perf_counter()
-    for s in systems:
+    for s in systems:   # each member alone: finish it before the next starts
         check(s.L.bm_dynamics(s.device(), 1 + a.warmup, a.steps, FREQ_NRG, 1), "plain steps")
-    for s in systems:
         check(s.L.bm_sync(s.device()), "sync")
     t_plain = time.perf_counter() - t1
     # batched: one graph launch per step advances every member
