@@ -407,7 +407,9 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
       keep = boxes_within(d, I, tJ, c.rc2);
     }
     unsigned long long act = __ballot(keep);
-    long long ax = 0, ay = 0, az = 0;
+    // i-partials of the whole chunk in FP32 (at most 16 tiles, a fixed
+    // order), converted to fixed point once per chunk
+    float fx = 0.f, fy = 0.f, fz = 0.f;
     float fli = 0.f;
     // software pipeline: the j slots, exclusion word and alchemical flag of
     // the next tile are loaded while the current tile computes
@@ -446,7 +448,6 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
         fetch(kn, J, pj, prj, lo, em, alchJ);
       }
       wave_lds_sync();
-      float fx = 0.f, fy = 0.f, fz = 0.f;
       double etile = 0.0;
       const bool alchTile = ALCH && (hi.w | alchJc);
       if (alchTile)
@@ -455,14 +456,12 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
       else
         tile_pairs<ELEC, VDW, ENERGY, false, false>(d, c, ia, li, half, sp, spr, slo, s_vdw, jm, emc,
                                                     I == Jc, fx, fy, fz, fli, etile);
-      ax += fix_f(fx);
-      ay += fix_f(fy);
-      az += fix_f(fz);
       if (ENERGY) eacc += etile;
       wave_lds_sync();
       k = kn;
     }
     // both halves hold partials of the same i-atoms
+    long long ax = fix_f(fx), ay = fix_f(fy), az = fix_f(fz);
     ax += __shfl_xor(ax, 32, 64);
     ay += __shfl_xor(ay, 32, 64);
     az += __shfl_xor(az, 32, 64);
