@@ -37,6 +37,7 @@ struct Dev {
   float fracscale[3];            // box / 2^32: fractional count -> A
   // state (SoA FP64, [x|y|z|theta])
   double *pos, *posb, *vel, *ism;
+  double *xi;                    // [3N+nL] standard normals of the current step
   long long *force;              // [3N+nL] fixed point
   long long *dudl;               // [nL]
   long long *energy;             // [BM_NTERMS]
@@ -98,7 +99,8 @@ struct Dev {
   int pnb[3];                    // spread bricks per axis (each >= order-1 points)
   int pcap;                      // atoms per brick bin
   int *pbcount;                  // [nbricks] atoms binned per brick, zeroed by the gather
-  int *pbatom;                   // [nbricks*pcap] atoms by home brick
+  double *pbrec;                 // [nbricks*pcap] spline records by home brick
+                                 // (pme_spline_rec_words(order) doubles each)
   float2 *cgrid;                 // [K0*K1*K2] complex work grid
   float2 *twiddle;               // [K0+K1+K2] e^{-2 pi i k/K} per axis
   float *bmod[3];                // [K] B-spline moduli per axis
@@ -144,6 +146,10 @@ struct Dev {
   long long *step;               // [1] step counter (RNG counter), advanced on device
 };
 
+// One atom's PME spread record: lambda*q, the B-spline weights along each
+// axis and the first stencil point per axis (as doubles: exact small ints).
+__host__ __device__ constexpr int pme_spline_rec_words(int order) { return 1 + 3 * order + 3; }
+
 __device__ __forceinline__ long long fix_f(float v) {
   return (long long)__float2ll_rn(v * kFixScaleF);
 }
@@ -182,23 +188,29 @@ __device__ __forceinline__ void wave_atomic_fix(long long *p, long long v) {
 
 // Wave-aggregated counter claim: every lane of the wave must call it.  Lanes
 // with equal keys share one atomicAdd; each active lane gets the counter
-// value before the add plus its rank among the lanes of its key.
+// value before the add plus its rank among the lanes of its key.  The key
+// groups are found first (ballots only), then every group's leader issues
+// its atomic at once: one round trip per wave, whatever the number of keys.
 __device__ __forceinline__ int wave_claim(int *counters, int key, bool active) {
   const int lane = threadIdx.x & 63;
   unsigned long long pending = __ballot(active);
-  int slot = 0;
+  unsigned long long grp = 0;
+  int leader = lane;
   while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const int lkey = __shfl(key, leader, 64);
+    const int l = __ffsll((long long)pending) - 1;
+    const int lkey = __shfl(key, l, 64);
     const bool mine = ((pending >> lane) & 1ull) && key == lkey;
-    const unsigned long long grp = __ballot(mine);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&counters[lkey], __popcll(grp));
-    base = __shfl(base, leader, 64);
-    if (mine) slot = base + __popcll(grp & ((1ull << lane) - 1ull));
-    pending &= ~grp;
+    const unsigned long long g = __ballot(mine);
+    if (mine) {
+      grp = g;
+      leader = l;
+    }
+    pending &= ~g;
   }
-  return slot;
+  int base = 0;
+  if (active && lane == leader) base = atomicAdd(&counters[key], __popcll(grp));
+  base = __shfl(base, leader, 64);
+  return active ? base + __popcll(grp & ((1ull << lane) - 1ull)) : 0;
 }
 
 // Block-wide fixed-point energy sum (any blockDim multiple of 64).

@@ -250,34 +250,31 @@ struct IAtom {
 // kVdwLdsMax entries are read from global memory by a separate variant
 constexpr int kVdwLdsMax = 6144;
 
-// The pairs of one tile for the lane's i-atom: the set bits of jm, shared
-// out between the halves (half 0 takes the first, half 1 the second, ...).
-template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
+// The pairs of one tile for the lane's i-atom: the nj j-atoms near the
+// i-box, compacted into sp/spr/slo (sjx: their slot index in block J),
+// shared out between the halves (half 0 takes entries 0, 2, ..., half 1
+// entries 1, 3, ...).  CHK: the tile has exclusions or is the diagonal
+// tile (I == J), so the exclusion bit and j == i are tested.
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool CHK>
 __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const IAtom &ia, int li,
                                            int half, const uint4 *sp, const int2 *spr, const float *slo,
-                                           const float2 *svdw, unsigned jm, unsigned em,
-                                           bool self, float &fx, float &fy, float &fz, float &fli,
-                                           double &etile) {
+                                           const unsigned char *sjx, const float2 *svdw, int nj,
+                                           unsigned em, bool self, float &fx, float &fy, float &fz,
+                                           float &fli, double &etile) {
   const float sx = d.fracscale[0], sy = d.fracscale[1], sz = d.fracscale[2];
-  while (jm) {
-    const int j0 = __builtin_ffs(jm) - 1;
-    jm &= jm - 1;
-    int j1 = 32;
-    if (jm) {
-      j1 = __builtin_ffs(jm) - 1;
-      jm &= jm - 1;
-    }
-    const int jl = half ? j1 : j0;
-    if (jl >= 32) continue;
+  for (int jl = half; jl < nj; jl += 2) {
     const uint4 pj = sp[jl];
-    const int2 prj = spr[jl];
     const float dx = fdisp(ia.p.x, pj.x, sx);
     const float dy = fdisp(ia.p.y, pj.y, sy);
     const float dz = fdisp(ia.p.z, pj.z, sz);
     const float r2 = dx * dx + dy * dy + dz * dz;
-    bool ok = ia.t >= 0 && !((em >> jl) & 1u) && r2 < c.rc2;
-    if (self) ok = ok && jl != li;
+    bool ok = ia.t >= 0 && r2 < c.rc2;
+    if (CHK) {
+      const int jo = sjx[jl];
+      ok = ok && !((em >> jo) & 1u) && !(self && jo == li);
+    }
     if (!ok) continue;
+    const int2 prj = spr[jl];
     const int vi = ia.t * d.ntypes + prj.x;
     const float2 lj = svdw[vi];
     float rinv = __frsqrt_rn(r2);
@@ -364,6 +361,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   __shared__ uint4 s_pos[4][kBlockAtoms];
   __shared__ int2 s_par[4][kBlockAtoms];
   __shared__ float s_lo[4][kBlockAtoms];
+  __shared__ unsigned char s_jx[4][kBlockAtoms];
   extern __shared__ float2 s_vdw_dyn[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int half = lane >> 5, li = lane & 31;
@@ -380,6 +378,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   uint4 *sp = s_pos[w];
   int2 *spr = s_par[w];
   float *slo = s_lo[w];
+  unsigned char *sjx = s_jx[w];
 
   // one chunk per wave; waves past the chunk count of this step exit
   {
@@ -421,9 +420,12 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
     float lo = 0.f;
     unsigned em = 0u;
     unsigned alchJ = 0u;
-    auto fetch = [&](int kk, int &J_, uint4 &pj_, int2 &prj_, float &lo_, unsigned &em_, unsigned &al_) {
+    int tEc = -1;
+    auto fetch = [&](int kk, int &J_, uint4 &pj_, int2 &prj_, float &lo_, unsigned &em_, unsigned &al_,
+                     int &ex_) {
       J_ = __builtin_amdgcn_readlane(tJ, kk);
       const int ex = __builtin_amdgcn_readlane(tE, kk);
+      ex_ = ex;
       const int sj = J_ * kBlockAtoms + li;
       pj_ = d.spos[sj];
       prj_ = d.spar[sj];
@@ -431,31 +433,41 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
       em_ = ex >= 0 ? d.emask[ex * kBlockAtoms + li] : 0u;
       al_ = ALCH ? d.bbh[J_].w : 0u;
     };
-    if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ);
+    if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ, tEc);
     while (k >= 0) {
+      // the j-atoms near the i-box, compacted to the front of the staging
       const bool near = half == 0 && prj.x >= 0 && near_box(d, ci, hi, pj, c.rc2);
-      const unsigned jm = (unsigned)__ballot(near);
-      if (half == 0) {
-        sp[li] = pj;
-        spr[li] = prj;
-        if (ENERGY) slo[li] = lo;
+      const unsigned long long nm = __ballot(near);
+      const int nj = __popcll(nm);
+      if (near) {
+        const int e = __popcll(nm & ((1ull << li) - 1ull));
+        sp[e] = pj;
+        spr[e] = prj;
+        sjx[e] = (unsigned char)li;
+        if (ENERGY) slo[e] = lo;
       }
-      const int Jc = J;
+      const int Jc = J, ex = tEc;
       const unsigned emc = em, alchJc = alchJ;
       const int kn = act ? __ffsll((long long)act) - 1 : -1;
       if (kn >= 0) {
         act &= act - 1;
-        fetch(kn, J, pj, prj, lo, em, alchJ);
+        fetch(kn, J, pj, prj, lo, em, alchJ, tEc);
       }
       wave_lds_sync();
       double etile = 0.0;
       const bool alchTile = ALCH && (hi.w | alchJc);
-      if (alchTile)
-        tile_pairs<ELEC, VDW, ENERGY, ALCH, SOFT>(d, c, ia, li, half, sp, spr, slo, s_vdw, jm, emc,
-                                                  I == Jc, fx, fy, fz, fli, etile);
-      else
-        tile_pairs<ELEC, VDW, ENERGY, false, false>(d, c, ia, li, half, sp, spr, slo, s_vdw, jm, emc,
-                                                    I == Jc, fx, fy, fz, fli, etile);
+      const bool self = I == Jc, chk = self || ex >= 0;
+#define BM_TILE(AL, SO, CK)                                                                        \
+  tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK>(d, c, ia, li, half, sp, spr, slo, sjx, s_vdw, nj, emc, \
+                                            self, fx, fy, fz, fli, etile)
+      if (alchTile) {
+        if (chk) BM_TILE(ALCH, SOFT, true);
+        else BM_TILE(ALCH, SOFT, false);
+      } else {
+        if (chk) BM_TILE(false, false, true);
+        else BM_TILE(false, false, false);
+      }
+#undef BM_TILE
       if (ENERGY) eacc += etile;
       wave_lds_sync();
       k = kn;

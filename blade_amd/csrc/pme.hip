@@ -2,8 +2,9 @@
 // (restates nbrecip/nbrecip.cu:19-485 and the grid setup of
 // system/potential.cxx:1202-1307; cuFFT is replaced by our own FFT).
 //
-//   pme_bin      atoms -> bins of the grid brick holding their first
-//                stencil point
+//   pme_bin      atoms -> spline records (lambda*q, FP64 B-spline weights,
+//                first stencil point) in bins of the grid brick holding
+//                their first stencil point
 //   pme_spread   one workgroup per brick: lambda*q B-spline weights summed
 //                as 64-bit fixed point in LDS (order-independent sums:
 //                bit-reproducible), brick stored to the real grid
@@ -61,53 +62,129 @@ __device__ __forceinline__ int brick_of(int u, int K, int nb) {
   return b;
 }
 
-// Bin every charged atom by the brick holding its first stencil point.
-// Threads walk the list slots (spatially ordered), so a wave's atoms share
-// few bricks and one atomic serves each brick.  Bin order is arbitrary;
-// the spread sums in fixed point, so the grid does not depend on it.
+// Bin every charged atom by the brick holding its first stencil point, as
+// a spread record: lambda*q, the FP64 B-spline weights of each axis and the
+// first stencil point (computed once per atom; every brick the stencil
+// reaches reads them).  Threads walk the list slots (spatially ordered), so
+// a wave's atoms share few bricks and one atomic serves each brick.  Bin
+// order is arbitrary; the spread sums in fixed point, so the grid does not
+// depend on it.
+template <int N>
 __global__ void __launch_bounds__(256) pme_bin(Dev d) {
+  constexpr int W = pme_spline_rec_words(N);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = s < *d.nblocks * kBlockAtoms ? d.l2g[s] : -1;
   bool act = false;
   int br = 0;
+  double lq = 0.0, w[3][N];
+  int u0[3] = {0, 0, 0};
   if (i >= 0) {
     const int b = d.sb[i] & 0xFFFF;
-    act = d.qd[i] * (b ? d.lambda[b] : 1.0) != 0.0;
+    lq = d.qd[i] * (b ? d.lambda[b] : 1.0);
+    act = lq != 0.0;
     int bb[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const unsigned u = frac_u32(d.pos[k * d.n + i], d.invbox[k]);
-      const int u0 = (int)(((unsigned long long)u * (unsigned)d.K[k]) >> 32);
-      bb[k] = brick_of(u0, d.K[k], d.pnb[k]);
+      double fr, dw[N];
+      grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0[k], fr);
+      bspline_w<N>(fr, w[k], dw);
+      bb[k] = brick_of(u0[k], d.K[k], d.pnb[k]);
     }
     br = (bb[0] * d.pnb[1] + bb[1]) * d.pnb[2] + bb[2];
   }
   const int slot = wave_claim(d.pbcount, br, act);
   if (act) {
-    if (slot < d.pcap) d.pbatom[(long long)br * d.pcap + slot] = i;
-    else atomicOr(d.errflag, 2);
+    if (slot < d.pcap) {
+      double *r = d.pbrec + ((long long)br * d.pcap + slot) * W;
+      r[0] = lq;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int t = 0; t < N; t++) r[1 + k * N + t] = w[k][t];
+#pragma unroll
+      for (int k = 0; k < 3; k++) r[1 + 3 * N + k] = (double)u0[k];
+    } else {
+      atomicOr(d.errflag, 2);
+    }
   }
 }
 
-// candidate atoms staged per round (keeps LDS under 64 KB for order 8)
-template <int N> constexpr int spread_stage() { return N <= 6 ? 256 : 128; }
+// candidate atoms staged per round (about 45 KB of LDS per workgroup, so
+// three bricks share a CU)
+template <int N> constexpr int spread_stage() { return N <= 4 ? 256 : N <= 6 ? 192 : 128; }
+
+// v * 2^32 rounded to the nearest integer (ties to even), as
+// __double2ll_rn(v * 2^32) gives, for |v * 2^32| < 2^51: the sum with
+// 1.5 * 2^52 is rounded to an integer by the addition itself, and its low
+// mantissa bits are that integer plus a constant.
+__device__ __forceinline__ long long fix_d_small(double v) {
+  const double magic = 6755399441055744.0;   // 1.5 * 2^52
+  return __double_as_longlong(fma(v, 4294967296.0, magic)) - __double_as_longlong(magic);
+}
+
+// Exclusive prefix sum of v over a 256-thread workgroup; the sum of all
+// values goes to total.  Every thread must call it.
+__device__ __forceinline__ int block_scan256(int v, int *s_wsum, int &total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int t = s_wsum[k];
+    if (k < w) pre += t;
+    tot += t;
+  }
+  total = tot;
+  return pre + x - v;
+}
+
+// The part of a stencil of N points starting at r (relative to the brick,
+// in [0, K)) that falls inside a brick of sz points: stencil points
+// t0 .. t0+n-1 at brick positions p0 .. p0+n-1 (mod K when the brick is the
+// whole axis), packed as t0 | n << 4 | p0 << 8.
+template <int N>
+__device__ __forceinline__ int stencil_clip(int r, int sz, int K) {
+  int t0 = 0, n = N, p0 = r;
+  if (sz < K) {
+    if (r < sz) {
+      n = min(N, sz - r);
+    } else {          // starts in the previous brick and wraps into this one
+      t0 = K - r;
+      if (t0 >= N) return 0;
+      n = N - t0;
+      p0 = 0;
+    }
+  }
+  return t0 | n << 4 | p0 << 8;
+}
 
 // One workgroup (256 threads) per brick.  Every brick spans >= order-1
 // points (or a whole axis), so only atoms binned in this brick or the
-// previous one along each axis reach it.  Each round stages up to 256
-// (order 8: 128) of those atoms at once (FP64 lambda*q and B-spline
-// weights), then all threads add the weights inside the brick as 64-bit
-// fixed point in LDS.  The brick is stored to the real grid with plain
-// stores: no global atomics, and no clearing pass.
+// previous one along each axis reach it.  Each round stages up to
+// spread_stage<N>() of their spline records at once (FP64 lambda*q, B-spline
+// weights) with the part of each stencil inside the brick, lists the (atom, x, y)
+// stencil rows that cross the brick, and gives each thread one row at a
+// time; its z points are added as 64-bit fixed point in LDS.  The brick is
+// stored to the real grid with plain stores: no global atomics, and no
+// clearing pass.
 template <int N>
 __global__ void __launch_bounds__(256) pme_spread(Dev d) {
   constexpr int kStage = spread_stage<N>();
   extern __shared__ long long acc[];
   __shared__ double s_w[3][N][kStage];
-  __shared__ short s_u[3][kStage];
+  __shared__ int s_clip[3][kStage];
   __shared__ double s_lq[kStage];
+  __shared__ unsigned short s_row[kStage * N * N];   // atom | x << 8 | y << 11
   __shared__ int s_cnt[9];
   __shared__ int s_bin[8];
+  __shared__ int s_wsum[4];
   int bidx[3], lo[3], sz[3];
   {
     int r = blockIdx.x;
@@ -153,45 +230,55 @@ __global__ void __launch_bounds__(256) pme_spread(Dev d) {
   const int total = s_cnt[8];
   for (int base = 0; base < total; base += kStage) {
     const int na = min(kStage, total - base);
+    int nrow = 0;
+    int clip[3] = {0, 0, 0};
     if ((int)threadIdx.x < na) {
       const int a = threadIdx.x, g = base + a;
       int f = 0;
       while (f < 7 && s_cnt[f + 1] <= g) f++;
-      const int i = d.pbatom[(long long)s_bin[f] * d.pcap + (g - s_cnt[f])];
-      const int b = d.sb[i] & 0xFFFF;
-      s_lq[a] = d.qd[i] * (b ? d.lambda[b] : 1.0);
-      double w[N], dw[N];
+      const double *rec = d.pbrec + ((long long)s_bin[f] * d.pcap + (g - s_cnt[f])) * pme_spline_rec_words(N);
+      s_lq[a] = rec[0];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        int u0;
-        double fr;
-        grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0, fr);
-        bspline_w<N>(fr, w, dw);
 #pragma unroll
-        for (int t = 0; t < N; t++) s_w[k][t][a] = w[t];
+        for (int t = 0; t < N; t++) s_w[k][t][a] = rec[1 + k * N + t];
         // stencil start relative to the brick, in [0, K)
-        int r = u0 - lo[k];
+        int r = (int)rec[1 + 3 * N + k] - lo[k];
         if (r < 0) r += d.K[k];
-        s_u[k][a] = (short)r;
+        clip[k] = stencil_clip<N>(r, sz[k], d.K[k]);
+        s_clip[k][a] = clip[k];
       }
+      nrow = ((clip[0] >> 4) & 15) * ((clip[1] >> 4) & 15);
+    }
+    int nrows;
+    const int off = block_scan256(nrow, s_wsum, nrows);
+    if (nrow) {
+      const int nx = (clip[0] >> 4) & 15, ny = (clip[1] >> 4) & 15;
+      int q = off;
+      for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++) s_row[q++] = (unsigned short)(threadIdx.x | x << 8 | y << 11);
     }
     __syncthreads();
-    for (int it = threadIdx.x; it < na * N * N; it += blockDim.x) {
-      const int a = it / (N * N), xa = (it / N) % N, yc = it % N;
-      int rx = s_u[0][a] + xa;
+    for (int it = threadIdx.x; it < nrows; it += blockDim.x) {
+      const unsigned e = s_row[it];
+      const int a = e & 255, x = (e >> 8) & 7, y = (e >> 11) & 7;
+      const int cx = s_clip[0][a], cy = s_clip[1][a], cz = s_clip[2][a];
+      int rx = (cx >> 8) + x;
       if (rx >= d.K[0]) rx -= d.K[0];
-      int ry = s_u[1][a] + yc;
+      int ry = (cy >> 8) + y;
       if (ry >= d.K[1]) ry -= d.K[1];
-      if (rx >= sz[0] || ry >= sz[1]) continue;
-      const double wac = s_lq[a] * s_w[0][xa][a] * s_w[1][yc][a];
+      const double wac = s_lq[a] * s_w[0][(cx & 15) + x][a] * s_w[1][(cy & 15) + y][a];
       long long *row = acc + (rx * sz[1] + ry) * rowz;
-      const int rz0 = s_u[2][a];
+      const int tz = cz & 15, nz = (cz >> 4) & 15, pz0 = cz >> 8;
 #pragma unroll
-      for (int e = 0; e < N; e++) {
-        int rz = rz0 + e;
-        if (rz >= d.K[2]) rz -= d.K[2];
-        if (rz < sz[2])
-          atomicAdd(reinterpret_cast<unsigned long long *>(row + rz), (unsigned long long)fix_d(wac * s_w[2][e][a]));
+      for (int z = 0; z < N; z++) {
+        if (z < nz) {
+          int rz = pz0 + z;
+          if (rz >= d.K[2]) rz -= d.K[2];
+          // |lambda q w w w| <= |q| < 2^19: within fix_d_small's range
+          atomicAdd(reinterpret_cast<unsigned long long *>(row + rz),
+                    (unsigned long long)fix_d_small(wac * s_w[2][tz + z][a]));
+        }
       }
     }
     __syncthreads();
@@ -511,9 +598,10 @@ __device__ __forceinline__ float2 *ct_fft(float2 *src, float2 *dst, const float2
 }
 
 // Forward transform along z then y of one x-plane (K1 x K2, in LDS): reads
-// the real charge grid, writes the complex grid.
+// the real charge grid, writes the complex grid.  1024 threads: there are
+// only K0 planes, so each plane's workgroup carries 16 waves.
 template <int K1, int K2>
-__global__ void __launch_bounds__(256) fft_yz_fwd(Dev d) {
+__global__ void __launch_bounds__(1024) fft_yz_fwd(Dev d) {
   __shared__ float2 b0[K1 * K2], b1[K1 * K2];
   __shared__ float2 t1[K1], t2[K2];
   const long long base = (long long)blockIdx.x * K1 * K2;
@@ -530,7 +618,7 @@ __global__ void __launch_bounds__(256) fft_yz_fwd(Dev d) {
 // Inverse transform along y then z of one x-plane; the real part (the
 // potential on the grid) goes to the real grid.
 template <int K1, int K2>
-__global__ void __launch_bounds__(256) fft_yz_inv(Dev d) {
+__global__ void __launch_bounds__(1024) fft_yz_inv(Dev d) {
   __shared__ float2 b0[K1 * K2], b1[K1 * K2];
   __shared__ float2 t1[K1], t2[K2];
   const long long base = (long long)blockIdx.x * K1 * K2;
@@ -598,101 +686,153 @@ __global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
 // sizes with compiled plane / line kernels
 #define BM_FFT_SIZES(X) X(16) X(18) X(20) X(24) X(27) X(32) X(36) X(40) X(48) X(54) X(64)
 
-// Per-atom force assembly, one thread per list slot (spatial order, so the
-// PME stencils of a wave overlap in cache).  Adds, in fixed point, the
-// components selected by mask: the PME force 2 lambda q grad(phi) and
-// dU/dlambda = 2 q phi from the potential grid (N = spline order; N = 0
-// compiles no PME part), the nonbonded chunk partials of the atom's block,
-// and the atom's bonded slots.  One thread per atom and stream order make
-// the read-modify-write of the force array safe without atomics.
+// Per-atom force assembly, eight lanes per list slot (slots in spatial
+// order, so the PME stencils of a wave overlap in cache).  Adds, in fixed
+// point, the components selected by mask: the PME force 2 lambda q
+// grad(phi) and dU/dlambda = 2 q phi from the potential grid (N = spline
+// order; N = 0 compiles no PME part), the nonbonded chunk partials of the
+// atom's block, and the atom's bonded slots.  The lanes split the stencil,
+// the chunks and the slots; integer sums across them are exact.  One lane
+// per force component and stream order make the read-modify-write of the
+// force array safe without atomics.
 template <int N, bool KICK>
 __global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  // the spread has consumed the PME bins: clear them for the next step
-  if (N && (mask & kAsmPme) && s < d.pnb[0] * d.pnb[1] * d.pnb[2]) d.pbcount[s] = 0;
-  if (s >= *d.nblocks * kBlockAtoms) return;
-  const int i = d.l2g[s];
-  if (i < 0) return;
+  // eight lanes per list slot; every lane of a group takes the same branches
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = g >> 3, sub = g & 7;
+  const bool live = s < *d.nblocks * kBlockAtoms;
+  const int i = live ? d.l2g[s] : -1;
+  // loads that depend only on the slot and the atom come first; every
+  // store and atomic is left to the end, so none of them orders the loads
+  const int j = sub * d.n + i;
+  long long F0 = 0;
+  double x0 = 0.0, v0 = 0.0, ism = 0.0;
+  if (i >= 0 && sub < 3) {
+    F0 = d.force[j];
+    if (KICK) {
+      ism = d.ism[i];
+      x0 = d.pos[j];
+      v0 = d.vel[j];
+    }
+  }
   long long fx = 0, fy = 0, fz = 0;
+  long long dl = 0;
+  int bl = 0;
   if constexpr (N > 0) {
-   if (mask & kAsmPme) {
-    const int b = d.sb[i] & 0xFFFF;
-    const float q = d.q[i];
-    const float l = b ? d.lambdaf[b] : 1.f;
+   if ((mask & kAsmPme) && i >= 0) {
+    // grid coordinates from the slot's packed fractional position; lane sub
+    // takes stencil offset z = u0 + sub of every (x, y) row, so the eight
+    // lanes of an atom read one short stretch of a grid row together
+    const uint4 ps = d.spos[s];
+    const int b = d.spar[s].y & 0xFFFF;
     int u0[3];
     float fr[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) grid_coord(frac_u32(d.pos[k * d.n + i], d.invbox[k]), d.K[k], u0[k], fr[k]);
+    grid_coord(ps.x, d.K[0], u0[0], fr[0]);
+    grid_coord(ps.y, d.K[1], u0[1], fr[1]);
+    grid_coord(ps.z, d.K[2], u0[2], fr[2]);
     float wx[N], wy[N], wz[N], dx[N], dy[N], dz[N];
     bspline_w<N>(fr[0], wx, dx);
     bspline_w<N>(fr[1], wy, dy);
     bspline_w<N>(fr[2], wz, dz);
-    float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    for (int a = 0; a < N; a++) {
-      int ix = u0[0] + a; if (ix >= d.K[0]) ix -= d.K[0];
-      for (int c = 0; c < N; c++) {
-        int iy = u0[1] + c; if (iy >= d.K[1]) iy -= d.K[1];
-        const float *row = d.rgrid + ((long long)ix * d.K[1] + iy) * d.K[2];
-        float sp = 0.f, sd = 0.f;
+    float wt = 0.f, dt = 0.f;
 #pragma unroll
-        for (int t = 0; t < N; t++) {
-          int iz = u0[2] + t; if (iz >= d.K[2]) iz -= d.K[2];
-          const float P = row[iz];
-          sp += P * wz[t];
-          sd += P * dz[t];
+    for (int t = 0; t < N; t++)
+      if (t == sub) {
+        wt = wz[t];
+        dt = dz[t];
+      }
+    float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    if (sub < N) {
+      int iz = u0[2] + sub;
+      if (iz >= d.K[2]) iz -= d.K[2];
+      // Q = sum_xy wx wy P, Gx = sum_xy dx wy P, Gy = sum_xy wx dy P
+      float Q = 0.f, Gx = 0.f, Gy = 0.f;
+#pragma unroll
+      for (int a = 0; a < N; a++) {
+        int ix = u0[0] + a;
+        if (ix >= d.K[0]) ix -= d.K[0];
+        const float *plane = d.rgrid + (long long)ix * d.K[1] * d.K[2] + iz;
+        float sy = 0.f, syd = 0.f;
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          int iy = u0[1] + c;
+          if (iy >= d.K[1]) iy -= d.K[1];
+          const float P = plane[(long long)iy * d.K[2]];
+          sy += wy[c] * P;
+          syd += dy[c] * P;
         }
-        phi += wx[a] * wy[c] * sp;
-        gx += dx[a] * wy[c] * sp;
-        gy += wx[a] * dy[c] * sp;
-        gz += wx[a] * wy[c] * sd;
+        Q += wx[a] * sy;
+        Gx += dx[a] * sy;
+        Gy += wx[a] * syd;
+      }
+      phi = wt * Q;
+      gx = wt * Gx;
+      gy = wt * Gy;
+      gz = dt * Q;
+    }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+      phi += __shfl_xor(phi, o, 64);
+      gx += __shfl_xor(gx, o, 64);
+      gy += __shfl_xor(gy, o, 64);
+      gz += __shfl_xor(gz, o, 64);
+    }
+    if (sub == 0) {
+      const float sc = 2.f * (b ? d.lambdaf[b] : 1.f) * __uint_as_float(ps.w);
+      fx = fix_f(sc * gx * (d.K[0] * (float)d.invbox[0]));
+      fy = fix_f(sc * gy * (d.K[1] * (float)d.invbox[1]));
+      fz = fix_f(sc * gz * (d.K[2] * (float)d.invbox[2]));
+      // dU/dlambda_b = 2 q phi (nbrecip.cu:445-449); the energy is summed
+      // in k-space by the convolution pass
+      if (b) {
+        bl = b;
+        dl = fix_d(2.0 * d.qd[i] * (double)phi);
       }
     }
-    const float sc = 2.f * l * q;
-    fx += fix_f(sc * gx * (d.K[0] * (float)d.invbox[0]));
-    fy += fix_f(sc * gy * (d.K[1] * (float)d.invbox[1]));
-    fz += fix_f(sc * gz * (d.K[2] * (float)d.invbox[2]));
-    // dU/dlambda_b = 2 q phi (nbrecip.cu:445-449); the energy is summed in
-    // k-space by the convolution pass
-    if (b) atomic_fix(&d.dudl[b], fix_d(2.0 * d.qd[i] * (double)phi));
    }
   }
-  if ((mask & kAsmNb) && *d.ntiles <= d.maxtiles && *d.nchunks <= d.maxchunks) {
+  if ((mask & kAsmNb) && i >= 0 && *d.ntiles <= d.maxtiles && *d.nchunks <= d.maxchunks) {
     const int I = s / kBlockAtoms, li = s % kBlockAtoms;
     const int c1 = d.cstart[I + 1];
-    for (int c = d.cstart[I]; c < c1; c++) {
+    for (int c = d.cstart[I] + sub; c < c1; c += 8) {
       const long long *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
       fx += src[li];
       fy += src[kBlockAtoms + li];
       fz += src[2 * kBlockAtoms + li];
     }
   }
-  if (mask & kAsmBonded) {
+  if ((mask & kAsmBonded) && i >= 0) {
     const int k1 = d.bcsr_start[i + 1];
-    for (int k = d.bcsr_start[i]; k < k1; k++) {
+    for (int k = d.bcsr_start[i] + sub; k < k1; k += 8) {
       const int q = d.bcsr[k];
       fx += d.bslot[q];
       fy += d.bslot[d.nslot + q];
       fz += d.bslot[2 * d.nslot + q];
     }
   }
-  const long long F[3] = {d.force[i] + fx, d.force[d.n + i] + fy, d.force[2 * d.n + i] + fz};
+  // integer sums: the same total in any order
 #pragma unroll
-  for (int k = 0; k < 3; k++) d.force[k * d.n + i] = F[k];
+  for (int o = 1; o < 8; o <<= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+    fz += __shfl_xor(fz, o, 64);
+  }
+  // the spread has consumed the PME bins: clear them for the next step
+  if (N && (mask & kAsmPme) && g < d.pnb[0] * d.pnb[1] * d.pnb[2]) d.pbcount[g] = 0;
+  if (bl) atomic_fix(&d.dudl[bl], dl);
+  if (i < 0 || sub >= 3) return;
+  // lanes 0, 1, 2 finish the x, y, z components
+  const long long F = F0 + (sub == 0 ? fx : sub == 1 ? fy : fz);
+  d.force[j] = F;
   if (KICK) {
     // V (full kick), backup, half drift R: as upd_kick mode 0
-    const double ism = d.ism[i];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const int j = k * d.n + i;
-      const double f = (double)F[k] * kFixInvD;
-      const double x = d.pos[j];
-      d.posb[j] = x;
-      if (ism > 0) {
-        const double v = d.vel[j] - 2.0 * d.halfdt * ism * ism * f;
-        d.vel[j] = v;
-        d.pos[j] = x + d.halfdt * v;
-        nan_check(d, v, i);
-      }
+    const double f = (double)F * kFixInvD;
+    d.posb[j] = x0;
+    if (ism > 0) {
+      const double v = v0 - 2.0 * d.halfdt * ism * ism * f;
+      d.vel[j] = v;
+      d.pos[j] = x0 + d.halfdt * v;
+      nan_check(d, v, i);
     }
   }
 }
@@ -723,29 +863,36 @@ void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
   int bmax = 1;
   for (int k = 0; k < 3; k++) bmax *= (d.K[k] + d.pnb[k] - 1) / d.pnb[k] + (k == 2);
   const size_t lds = (size_t)bmax * sizeof(long long);
-  pme_bin<<<(unsigned)((32LL * d.maxblocks + 255) / 256), 256, 0, st>>>(d);
-  if (N == 4) pme_spread<4><<<nbr, 256, lds, st>>>(d);
-  else if (N == 6) pme_spread<6><<<nbr, 256, lds, st>>>(d);
-  else pme_spread<8><<<nbr, 256, lds, st>>>(d);
+  const unsigned gb = (unsigned)((32LL * d.maxblocks + 255) / 256);
+  if (N == 4) {
+    pme_bin<4><<<gb, 256, 0, st>>>(d);
+    pme_spread<4><<<nbr, 256, lds, st>>>(d);
+  } else if (N == 6) {
+    pme_bin<6><<<gb, 256, 0, st>>>(d);
+    pme_spread<6><<<nbr, 256, lds, st>>>(d);
+  } else {
+    pme_bin<8><<<gb, 256, 0, st>>>(d);
+    pme_spread<8><<<nbr, 256, lds, st>>>(d);
+  }
   bool ct0 = false, ct1 = false;
 #define BM_IS(SZ) ct0 = ct0 || d.K[0] == SZ; ct1 = ct1 || d.K[1] == SZ;
   BM_FFT_SIZES(BM_IS)
 #undef BM_IS
   if (ct0 && ct1 && d.K[1] == d.K[2] && d.K[1] * d.K[2] <= 4096) {
     // plane path: forward z+y per x-plane, x with convolution, inverse y+z
-    const unsigned gx = (unsigned)((d.K[1] * d.K[2] + 31) / 32);
+    const unsigned gx = (unsigned)((d.K[1] * d.K[2] + 15) / 16);
     switch (d.K[1]) {
-#define BM_YZF(SZ) case SZ: fft_yz_fwd<SZ, SZ><<<d.K[0], 256, 0, st>>>(d); break;
+#define BM_YZF(SZ) case SZ: fft_yz_fwd<SZ, SZ><<<d.K[0], 1024, 0, st>>>(d); break;
       BM_FFT_SIZES(BM_YZF)
 #undef BM_YZF
     }
     switch (d.K[0]) {
-#define BM_XC(SZ) case SZ: fft_x_conv<SZ, 32><<<gx, 256, 0, st>>>(d, energy ? 1 : 0); break;
+#define BM_XC(SZ) case SZ: fft_x_conv<SZ, 16><<<gx, 256, 0, st>>>(d, energy ? 1 : 0); break;
       BM_FFT_SIZES(BM_XC)
 #undef BM_XC
     }
     switch (d.K[1]) {
-#define BM_YZI(SZ) case SZ: fft_yz_inv<SZ, SZ><<<d.K[0], 256, 0, st>>>(d); break;
+#define BM_YZI(SZ) case SZ: fft_yz_inv<SZ, SZ><<<d.K[0], 1024, 0, st>>>(d); break;
       BM_FFT_SIZES(BM_YZI)
 #undef BM_YZI
     }
@@ -780,7 +927,7 @@ static void assemble_t(const Dev &d, unsigned mask, unsigned g, hipStream_t st) 
 
 void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st, bool kick) {
   const int nbr = d.elecmethod == BM_PME ? d.pnb[0] * d.pnb[1] * d.pnb[2] : 0;
-  const unsigned g = (unsigned)((std::max(32LL * d.maxblocks, (long long)nbr) + 255) / 256);
+  const unsigned g = (unsigned)((std::max(8LL * 32LL * d.maxblocks, (long long)nbr) + 255) / 256);
   if (d.elecmethod != BM_PME) mask &= ~kAsmPme;
   if (kick) assemble_t<true>(d, mask, g, st);
   else assemble_t<false>(d, mask, g, st);

@@ -92,6 +92,8 @@ struct bm_system {
   void force_launches(bool energy, bool kick = false) {
     (void)hipEventRecord(evFork, st);
     (void)hipStreamWaitEvent(st2, evFork, 0);
+    // a non-energy step's O step runs after the assembly: its normals here
+    if (kick) blade::launch_normals(d, st2);
     blade::launch_bonded(d, energy, bias, st2);
     if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st2);
     (void)hipEventRecord(evJoin, st2);
@@ -231,6 +233,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     UPLOAD(d.vel, v);
     UPLOAD(d.ism, ism);
     ALLOC(d.posb, N);
+    ALLOC(d.xi, N);
   }
   ALLOC(d.force, N);
   ALLOC(d.dudl, nl);
@@ -338,16 +341,16 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     const size_t G = (size_t)d.K[0] * d.K[1] * d.K[2];
     ALLOC(d.rgrid, G);
     ALLOC(d.cgrid, G);
-    // spread bricks: floor(K/8) per axis (at least one), so each spans >= 8
-    // points (>= order-1 for orders up to 8, or the whole axis)
+    // spread bricks: floor(K/max(order-1, 6)) per axis (at least one), so
+    // each spans >= order-1 points (or the whole axis)
     int nbr = 1;
     for (int k = 0; k < 3; k++) {
-      d.pnb[k] = std::max(1, d.K[k] / 8);
+      d.pnb[k] = std::max(1, d.K[k] / std::max(d.order - 1, 6));
       nbr *= d.pnb[k];
     }
     d.pcap = 64 + 4 * ((n + nbr - 1) / nbr);
     ALLOC(d.pbcount, nbr);
-    ALLOC(d.pbatom, (size_t)nbr * d.pcap);
+    ALLOC(d.pbrec, (size_t)nbr * d.pcap * pme_spline_rec_words(d.order));
     std::vector<float2> tw;
     for (int k = 0; k < 3; k++)
       for (int m = 0; m < d.K[k]; m++) {

@@ -51,16 +51,27 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
   return c;
 }
 
-// standard normal for (seed, step, dof): the same rule as the oracle
-__device__ double normal_dof(unsigned long long seed, unsigned long long step, unsigned long long dof) {
-  uint4 c = make_uint4((unsigned)(dof >> 1), (unsigned)step, (unsigned)(step >> 32), 0x4E524D4Cu);
+// standard normals of DOFs 2p and 2p+1 for (seed, step): the same rule as
+// the oracle (one Philox block, Box-Muller cosine for the even DOF, sine for
+// the odd one)
+__device__ double2 normal_pair(unsigned long long seed, unsigned long long step, unsigned long long p) {
+  uint4 c = make_uint4((unsigned)p, (unsigned)step, (unsigned)(step >> 32), 0x4E524D4Cu);
   uint4 o = philox(c, make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
   double u1 = ((double)((((unsigned long long)o.x) << 21) ^ (o.y >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
   double u2 = ((double)((((unsigned long long)o.z) << 21) ^ (o.w >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
   double rad = sqrt(-2.0 * log(u1));
   double sn, cs;
   sincospi(2.0 * u2, &sn, &cs);
-  return (dof & 1) ? rad * sn : rad * cs;
+  return make_double2(rad * cs, rad * sn);
+}
+
+__global__ void upd_normals(Dev d) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = 3 * d.n + d.nl;
+  if (2 * p >= N) return;
+  const double2 z = normal_pair(d.seed, (unsigned long long)(*d.step), (unsigned long long)p);
+  d.xi[2 * p] = z.x;
+  if (2 * p + 1 < N) d.xi[2 * p + 1] = z.y;
 }
 
 // dU/dtheta of block b from the fixed-point dU/dlambda (msld.cu:937-975)
@@ -116,8 +127,7 @@ __device__ __forceinline__ void o_drift_dof(const Dev &d, int i) {
       fr = d.lfric[i - N1];
       no = d.lnoise[i - N1];
     }
-    const double xi = normal_dof(d.seed, (unsigned long long)(*d.step), (unsigned long long)i);
-    double v = fr * d.vel[i] + no * ism * xi;
+    double v = fr * d.vel[i] + no * ism * d.xi[i];
     d.vel[i] = v;
     d.pos[i] = x + d.halfdt * v;
     nan_check(d, v, i < N1 ? i % d.n : i);
@@ -281,28 +291,31 @@ __device__ __forceinline__ void load_pos(const Dev &d, const int (&a)[NA], Group
   }
 }
 
-template <int NA>
-__device__ __forceinline__ void store_pos(const Dev &d, const int (&a)[NA], const GroupPos<NA> &g) {
-  const double ih = 1.0 / d.halfdt;
+// Every lane of a group's four solves the whole group (the same inputs give
+// the same result in each); lane k stores atom k and, with ODRIFT, runs the
+// O step and half drift of that atom's three DOFs.
+template <int NA, bool ODRIFT>
+__device__ __forceinline__ void finish_atom(const Dev &d, const int (&a)[NA], const GroupPos<NA> &g, int k) {
+  int am = -1;
+  D3 xn = d3(0, 0, 0), xp = d3(0, 0, 0), sh = d3(0, 0, 0);
 #pragma unroll
-  for (int k = 0; k < NA; k++) {
-    setp(d.vel, d.n, a[k], ih * (g.xn[k] - g.xp[k]));
-    setp(d.pos, d.n, a[k], g.xn[k] - g.sh[k]);
-  }
-}
-
-// O and half drift of the three DOFs of each atom of a group, right after
-// the group's constraint step (same thread, so its writes are visible)
-template <int NA>
-__device__ __forceinline__ void o_drift_atoms(const Dev &d, const int (&a)[NA]) {
+  for (int m = 0; m < NA; m++)
+    if (m == k) {
+      am = a[m];
+      xn = g.xn[m];
+      xp = g.xp[m];
+      sh = g.sh[m];
+    }
+  if (am < 0) return;
+  setp(d.vel, d.n, am, (1.0 / d.halfdt) * (xn - xp));
+  setp(d.pos, d.n, am, xn - sh);
+  if (ODRIFT)
 #pragma unroll
-  for (int k = 0; k < NA; k++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) o_drift_dof(d, c * d.n + a[k]);
+    for (int c = 0; c < 3; c++) o_drift_dof(d, c * d.n + am);
 }
 
 template <bool ODRIFT>
-__device__ void constrain_group(const Dev &d, int gi) {
+__device__ void constrain_group(const Dev &d, int gi, int k) {
   if (gi < d.ntri) {
     const int4 ix = d.tri_i[gi];
     const double4 bb = d.tri_b[gi];
@@ -312,8 +325,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
     load_pos(d, a, g);
     if (g.m[1] == g.m[2] && b0[0] == b0[1]) settle(b0, g.xn, g.xp, g.m);
     else shake_newton<3, 3, true>(b0, g.xn, g.xp, g.im, d.shaketol);
-    store_pos(d, a, g);
-    if (ODRIFT) o_drift_atoms(d, a);
+    finish_atom<3, ODRIFT>(d, a, g, k);
     return;
   }
   gi -= d.ntri;
@@ -328,8 +340,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
     const double lw = (-sg + sqrt(sg * sg - gg * (ss - b02))) / (gg * (g.im[0] + g.im[1]));
     g.xn[0] = g.xn[0] + (g.im[0] * lw) * gp;
     g.xn[1] = g.xn[1] + (-g.im[1] * lw) * gp;
-    store_pos(d, a, g);
-    if (ODRIFT) o_drift_atoms(d, a);
+    finish_atom<2, ODRIFT>(d, a, g, k);
     return;
   }
   gi -= d.nbr1;
@@ -341,8 +352,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
     GroupPos<3> g;
     load_pos(d, a, g);
     shake_newton<3, 2, false>(b0, g.xn, g.xp, g.im, d.shaketol);
-    store_pos(d, a, g);
-    if (ODRIFT) o_drift_atoms(d, a);
+    finish_atom<3, ODRIFT>(d, a, g, k);
     return;
   }
   gi -= d.nbr2;
@@ -353,8 +363,7 @@ __device__ void constrain_group(const Dev &d, int gi) {
   GroupPos<4> g;
   load_pos(d, a, g);
   shake_newton<4, 3, false>(b0, g.xn, g.xp, g.im, d.shaketol);
-  store_pos(d, a, g);
-  if (ODRIFT) o_drift_atoms(d, a);
+  finish_atom<4, ODRIFT>(d, a, g, k);
 }
 
 // theta -> lambda for site s (msld.cu:863-898)
@@ -394,15 +403,16 @@ __device__ void theta_kick(const Dev &d, int b) {
   }
 }
 
-// groups, then (last call of a step) sites and the step counter
+// groups (four lanes each), then (last call of a step) sites and the step
+// counter
 __global__ void __launch_bounds__(64) upd_constrain(Dev d, int finish) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  if (t < ng) constrain_group<false>(d, t);
+  if (t < 4 * ng) constrain_group<false>(d, t >> 2, t & 3);
   if (finish) {
-    const int s = t - ng;
+    const int s = t - 4 * ng;
     if (s >= 0 && s < d.nsites && d.nl > 1) theta_to_lambda(d, s);
-    if (t == ng + d.nsites) *d.step += 1;
+    if (s == d.nsites) *d.step += 1;
   }
 }
 
@@ -492,11 +502,13 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
   // constraint groups: 64-thread workgroups spread the few thousand groups
   // over many CUs
-  const unsigned gG = (ng + 63) / 64 > 0 ? (ng + 63) / 64 : 1;
-  const unsigned gF = (ng + d.nsites + 1 + 63) / 64;
+  const unsigned gV = (ng + 63) / 64 > 0 ? (ng + 63) / 64 : 1;
+  const unsigned gG = (4 * ng + 63) / 64 > 0 ? (4 * ng + 63) / 64 : 1;
+  const unsigned gF = (4 * ng + d.nsites + 1 + 63) / 64;
+  launch_normals(d, st);
   if (energy_step) {
     upd_kick<<<gN, 256, 0, st>>>(d, 1);
-    if (ng) upd_vconstrain<<<gG, 64, 0, st>>>(d);
+    if (ng) upd_vconstrain<<<gV, 64, 0, st>>>(d);
     upd_kinetic<<<gN, 256, 0, st>>>(d);
     upd_kick<<<gN, 256, 0, st>>>(d, 2);
   } else {
@@ -508,17 +520,17 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
 }
 
 // First half of a non-energy step after the force assembly kicked the
-// atoms: each constraint group, then O and half drift of its atoms in the
-// same thread; the unconstrained atoms and the theta DOFs (kick, O, drift)
-// take the threads after the groups.
+// atoms: each constraint group (four lanes), then O and half drift of each
+// of its atoms in that atom's lane; the unconstrained atoms and the theta
+// DOFs (kick, O, drift) take the threads after the groups.
 __global__ void __launch_bounds__(64) upd_constrain_o(Dev d) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  if (t < ng) {
-    constrain_group<true>(d, t);
+  if (t < 4 * ng) {
+    constrain_group<true>(d, t >> 2, t & 3);
     return;
   }
-  const int f = t - ng;
+  const int f = t - 4 * ng;
   if (f < d.nfree) {
     const int a = d.freeatom[f];
 #pragma unroll
@@ -535,8 +547,13 @@ __global__ void __launch_bounds__(64) upd_constrain_o(Dev d) {
 // the rest of a non-energy step after the force assembly kicked the atoms
 void launch_update_after_kick(const Dev &d, hipStream_t st) {
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  upd_constrain_o<<<(ng + d.nfree + d.nl + 63) / 64, 64, 0, st>>>(d);
-  upd_constrain<<<(ng + d.nsites + 1 + 63) / 64, 64, 0, st>>>(d, 1);
+  upd_constrain_o<<<(4 * ng + d.nfree + d.nl + 63) / 64, 64, 0, st>>>(d);
+  upd_constrain<<<(4 * ng + d.nsites + 1 + 63) / 64, 64, 0, st>>>(d, 1);
+}
+
+void launch_normals(const Dev &d, hipStream_t st) {
+  const int pairs = (3 * d.n + d.nl + 1) / 2;
+  upd_normals<<<(pairs + 255) / 256, 256, 0, st>>>(d);
 }
 
 __global__ void upd_set_step(Dev d, long long v) { *d.step = v; }
