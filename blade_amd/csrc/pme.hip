@@ -109,9 +109,9 @@ __global__ void __launch_bounds__(256) pme_bin(Dev d) {
   }
 }
 
-// candidate atoms staged per round (about 45 KB of LDS per workgroup, so
-// three bricks share a CU)
-template <int N> constexpr int spread_stage() { return N <= 4 ? 256 : N <= 6 ? 192 : 128; }
+// candidate atoms staged per round (LDS per workgroup stays near 60 KB, so
+// two bricks share a CU)
+template <int N> constexpr int spread_stage() { return N <= 6 ? 256 : 128; }
 
 // v * 2^32 rounded to the nearest integer (ties to even), as
 // __double2ll_rn(v * 2^32) gives, for |v * 2^32| < 2^51: the sum with
@@ -197,7 +197,9 @@ __global__ void __launch_bounds__(256) pme_spread(Dev d) {
     lo[k] = (bidx[k] * d.K[k]) / d.pnb[k];
     sz[k] = ((bidx[k] + 1) * d.K[k]) / d.pnb[k] - lo[k];
   }
-  const int rowz = sz[2] + 1;   // padded row: fewer LDS bank conflicts
+  // padded row of an odd number of 64-bit words: lanes on consecutive rows
+  // fall in different LDS banks
+  const int rowz = sz[2] + 1 + (sz[2] & 1);
   const int npts = sz[0] * sz[1] * rowz;
   for (int p = threadIdx.x; p < npts; p += blockDim.x) acc[p] = 0;
   // feeding bins (this brick and the previous one along each axis, each
@@ -743,10 +745,10 @@ __global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
       }
     float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     if (sub < N) {
-      int iz = u0[2] + sub;
-      if (iz >= d.K[2]) iz -= d.K[2];
       // Q = sum_xy wx wy P, Gx = sum_xy dx wy P, Gy = sum_xy wx dy P
       float Q = 0.f, Gx = 0.f, Gy = 0.f;
+      int iz = u0[2] + sub;
+      if (iz >= d.K[2]) iz -= d.K[2];
 #pragma unroll
       for (int a = 0; a < N; a++) {
         int ix = u0[0] + a;
@@ -861,7 +863,7 @@ void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
   const int N = d.order;
   const int nbr = d.pnb[0] * d.pnb[1] * d.pnb[2];
   int bmax = 1;
-  for (int k = 0; k < 3; k++) bmax *= (d.K[k] + d.pnb[k] - 1) / d.pnb[k] + (k == 2);
+  for (int k = 0; k < 3; k++) bmax *= (d.K[k] + d.pnb[k] - 1) / d.pnb[k] + (k == 2 ? 2 : 0);
   const size_t lds = (size_t)bmax * sizeof(long long);
   const unsigned gb = (unsigned)((32LL * d.maxblocks + 255) / 256);
   if (N == 4) {

@@ -341,11 +341,12 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     const size_t G = (size_t)d.K[0] * d.K[1] * d.K[2];
     ALLOC(d.rgrid, G);
     ALLOC(d.cgrid, G);
-    // spread bricks: floor(K/max(order-1, 6)) per axis (at least one), so
-    // each spans >= order-1 points (or the whole axis)
+    // spread bricks: floor(K/8) per axis (at least one), so each spans >= 8
+    // points (>= order-1 for orders up to 8, or the whole axis); a 64^3 grid
+    // gives 512 bricks, one workgroup each, all resident at once
     int nbr = 1;
     for (int k = 0; k < 3; k++) {
-      d.pnb[k] = std::max(1, d.K[k] / std::max(d.order - 1, 6));
+      d.pnb[k] = std::max(1, d.K[k] / 8);
       nbr *= d.pnb[k];
     }
     d.pcap = 64 + 4 * ((n + nbr - 1) / nbr);

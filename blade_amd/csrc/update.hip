@@ -291,11 +291,46 @@ __device__ __forceinline__ void load_pos(const Dev &d, const int (&a)[NA], Group
   }
 }
 
-// Every lane of a group's four solves the whole group (the same inputs give
-// the same result in each); lane k stores atom k and, with ODRIFT, runs the
-// O step and half drift of that atom's three DOFs.
-template <int NA, bool ODRIFT>
-__device__ __forceinline__ void finish_atom(const Dev &d, const int (&a)[NA], const GroupPos<NA> &g, int k) {
+// The constraint step of one group kind on positions g.  Kinds: 0
+// triangle (SETTLE when isosceles, else SHAKE), 1 one bond (analytic),
+// 2 and 3 two and three bonds to a centre atom (Newton SHAKE).
+template <int KIND, int NA>
+__device__ __forceinline__ void solve_kind(const Dev &d, const double *b0, GroupPos<NA> &g) {
+  if constexpr (KIND == 0) {
+    if (g.m[1] == g.m[2] && b0[0] == b0[1]) settle(b0, g.xn, g.xp, g.m);
+    else shake_newton<3, 3, true>(b0, g.xn, g.xp, g.im, d.shaketol);
+  } else if constexpr (KIND == 1) {
+    const double b02 = b0[0] * b0[0];
+    const D3 gp = g.xp[0] - g.xp[1], s = g.xn[0] - g.xn[1];
+    const double gg = dotd(gp, gp), sg = dotd(s, gp), ss = dotd(s, s);
+    const double lw = (-sg + sqrt(sg * sg - gg * (ss - b02))) / (gg * (g.im[0] + g.im[1]));
+    g.xn[0] = g.xn[0] + (g.im[0] * lw) * gp;
+    g.xn[1] = g.xn[1] + (-g.im[1] * lw) * gp;
+  } else if constexpr (KIND == 2) {
+    shake_newton<3, 2, false>(b0, g.xn, g.xp, g.im, d.shaketol);
+  } else {
+    shake_newton<4, 3, false>(b0, g.xn, g.xp, g.im, d.shaketol);
+  }
+}
+
+__device__ __forceinline__ D3 shfl3(D3 v, int src) {
+  return d3(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64));
+}
+
+// One lane (k) of a group's four.  Every lane solves the whole group (the
+// same inputs give the same result in each) and lane k finishes atom k:
+//   TAIL = false: stores the constrained position and v = dx/(dt/2);
+//   TAIL = true:  then runs the O step and half drift of the atom's three
+//     DOFs in registers, the four lanes exchange the new positions by
+//     shuffles, and every lane solves the group's second constraint step
+//     with the first step's result as reference positions.
+// The arithmetic is that of the separate kernels (constraint, O + drift,
+// constraint), operation for operation.
+template <int KIND, int NA, bool TAIL>
+__device__ void group_lane(const Dev &d, const int (&a)[NA], const double *b0, int k) {
+  GroupPos<NA> g;
+  load_pos(d, a, g);
+  solve_kind<KIND, NA>(d, b0, g);
   int am = -1;
   D3 xn = d3(0, 0, 0), xp = d3(0, 0, 0), sh = d3(0, 0, 0);
 #pragma unroll
@@ -306,41 +341,77 @@ __device__ __forceinline__ void finish_atom(const Dev &d, const int (&a)[NA], co
       xp = g.xp[m];
       sh = g.sh[m];
     }
-  if (am < 0) return;
-  setp(d.vel, d.n, am, (1.0 / d.halfdt) * (xn - xp));
-  setp(d.pos, d.n, am, xn - sh);
-  if (ODRIFT)
+  const double ih = 1.0 / d.halfdt;
+  const D3 v1 = ih * (xn - xp), x1 = xn - sh;
+  if (!TAIL) {
+    if (am < 0) return;
+    setp(d.vel, d.n, am, v1);
+    setp(d.pos, d.n, am, x1);
+    return;
+  }
+  // O and half drift (o_drift_dof) of the lane's atom
+  double x2[3] = {x1.x, x1.y, x1.z}, vv[3] = {v1.x, v1.y, v1.z};
+  if (am >= 0) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) o_drift_dof(d, c * d.n + am);
+    for (int c = 0; c < 3; c++) {
+      const int i = c * d.n + am;
+      const double ism = d.ism[i];
+      if (ism > 0) {
+        const double v = d.fric * vv[c] + d.noise * ism * d.xi[i];
+        vv[c] = v;
+        x2[c] = x2[c] + d.halfdt * v;
+        nan_check(d, v, am);
+      }
+    }
+  }
+  const D3 x2v = d3(x2[0], x2[1], x2[2]);
+  const int base = (threadIdx.x & 63) & ~3;
+  GroupPos<NA> h;
+  D3 X1[NA];
+#pragma unroll
+  for (int m = 0; m < NA; m++) {
+    X1[m] = shfl3(x1, base + m);
+    h.xn[m] = shfl3(x2v, base + m);
+    h.m[m] = g.m[m];
+    h.im[m] = g.im[m];
+  }
+#pragma unroll
+  for (int m = 0; m < NA; m++) {
+    h.sh[m] = m ? (minimg(d, X1[m] - X1[0]) - (X1[m] - X1[0])) : d3(0, 0, 0);
+    h.xp[m] = X1[m] + h.sh[m];
+    h.xn[m] = h.xn[m] + h.sh[m];
+  }
+  solve_kind<KIND, NA>(d, b0, h);
+  if (am < 0) return;
+  D3 yn = d3(0, 0, 0), yp = d3(0, 0, 0), ys = d3(0, 0, 0);
+#pragma unroll
+  for (int m = 0; m < NA; m++)
+    if (m == k) {
+      yn = h.xn[m];
+      yp = h.xp[m];
+      ys = h.sh[m];
+    }
+  setp(d.posb, d.n, am, x1);
+  setp(d.vel, d.n, am, ih * (yn - yp));
+  setp(d.pos, d.n, am, yn - ys);
 }
 
-template <bool ODRIFT>
+template <bool TAIL>
 __device__ void constrain_group(const Dev &d, int gi, int k) {
   if (gi < d.ntri) {
     const int4 ix = d.tri_i[gi];
     const double4 bb = d.tri_b[gi];
     const int a[3] = {ix.x, ix.y, ix.z};
     const double b0[3] = {bb.x, bb.y, bb.z};
-    GroupPos<3> g;
-    load_pos(d, a, g);
-    if (g.m[1] == g.m[2] && b0[0] == b0[1]) settle(b0, g.xn, g.xp, g.m);
-    else shake_newton<3, 3, true>(b0, g.xn, g.xp, g.im, d.shaketol);
-    finish_atom<3, ODRIFT>(d, a, g, k);
+    group_lane<0, 3, TAIL>(d, a, b0, k);
     return;
   }
   gi -= d.ntri;
   if (gi < d.nbr1) {
     const int2 ix = d.br1_i[gi];
     const int a[2] = {ix.x, ix.y};
-    const double b02 = d.br1_b[gi] * d.br1_b[gi];
-    GroupPos<2> g;
-    load_pos(d, a, g);
-    const D3 gp = g.xp[0] - g.xp[1], s = g.xn[0] - g.xn[1];
-    const double gg = dotd(gp, gp), sg = dotd(s, gp), ss = dotd(s, s);
-    const double lw = (-sg + sqrt(sg * sg - gg * (ss - b02))) / (gg * (g.im[0] + g.im[1]));
-    g.xn[0] = g.xn[0] + (g.im[0] * lw) * gp;
-    g.xn[1] = g.xn[1] + (-g.im[1] * lw) * gp;
-    finish_atom<2, ODRIFT>(d, a, g, k);
+    const double b0[1] = {d.br1_b[gi]};
+    group_lane<1, 2, TAIL>(d, a, b0, k);
     return;
   }
   gi -= d.nbr1;
@@ -349,10 +420,7 @@ __device__ void constrain_group(const Dev &d, int gi, int k) {
     const double2 bb = d.br2_b[gi];
     const int a[3] = {ix.x, ix.y, ix.z};
     const double b0[2] = {bb.x, bb.y};
-    GroupPos<3> g;
-    load_pos(d, a, g);
-    shake_newton<3, 2, false>(b0, g.xn, g.xp, g.im, d.shaketol);
-    finish_atom<3, ODRIFT>(d, a, g, k);
+    group_lane<2, 3, TAIL>(d, a, b0, k);
     return;
   }
   gi -= d.nbr2;
@@ -360,10 +428,7 @@ __device__ void constrain_group(const Dev &d, int gi, int k) {
   const double4 bb = d.br3_b[gi];
   const int a[4] = {ix.x, ix.y, ix.z, ix.w};
   const double b0[3] = {bb.x, bb.y, bb.z};
-  GroupPos<4> g;
-  load_pos(d, a, g);
-  shake_newton<4, 3, false>(b0, g.xn, g.xp, g.im, d.shaketol);
-  finish_atom<4, ODRIFT>(d, a, g, k);
+  group_lane<3, 4, TAIL>(d, a, b0, k);
 }
 
 // theta -> lambda for site s (msld.cu:863-898)
@@ -519,11 +584,25 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
   upd_constrain<<<gF, 64, 0, st>>>(d, 1);
 }
 
-// First half of a non-energy step after the force assembly kicked the
-// atoms: each constraint group (four lanes), then O and half drift of each
-// of its atoms in that atom's lane; the unconstrained atoms and the theta
-// DOFs (kick, O, drift) take the threads after the groups.
-__global__ void __launch_bounds__(64) upd_constrain_o(Dev d) {
+// The rest of a non-energy step after the force assembly kicked the atoms
+// (V and half drift): each constraint group on four lanes (constraint, O
+// and half drift, constraint); the unconstrained atoms (O and half drift);
+// and, in the last workgroup, the theta DOFs (kick, O, half drift), then
+// theta -> lambda and the step counter.
+__global__ void __launch_bounds__(64) upd_tail(Dev d) {
+  if (blockIdx.x == gridDim.x - 1) {
+    if (d.nl > 1) {
+      for (int b = threadIdx.x; b < d.nl; b += blockDim.x) {
+        theta_kick(d, b);
+        o_drift_dof(d, 3 * d.n + b);
+      }
+      __syncthreads();
+      for (int s = threadIdx.x; s < d.nsites; s += blockDim.x) theta_to_lambda(d, s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *d.step += 1;
+    return;
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
   if (t < 4 * ng) {
@@ -535,20 +614,13 @@ __global__ void __launch_bounds__(64) upd_constrain_o(Dev d) {
     const int a = d.freeatom[f];
 #pragma unroll
     for (int c = 0; c < 3; c++) o_drift_dof(d, c * d.n + a);
-    return;
-  }
-  const int b = f - d.nfree;
-  if (b < d.nl && d.nl > 1) {
-    theta_kick(d, b);
-    o_drift_dof(d, 3 * d.n + b);
   }
 }
 
-// the rest of a non-energy step after the force assembly kicked the atoms
 void launch_update_after_kick(const Dev &d, hipStream_t st) {
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  upd_constrain_o<<<(4 * ng + d.nfree + d.nl + 63) / 64, 64, 0, st>>>(d);
-  upd_constrain<<<(4 * ng + d.nsites + 1 + 63) / 64, 64, 0, st>>>(d, 1);
+  // the groups and free atoms, then one workgroup for the theta DOFs
+  upd_tail<<<(4 * ng + d.nfree + 63) / 64 + 1, 64, 0, st>>>(d);
 }
 
 void launch_normals(const Dev &d, hipStream_t st) {
