@@ -117,10 +117,15 @@ def main():
     if flag.value != -1:
         raise RuntimeError("NaN during the timed steps (atom %d)" % flag.value)
 
-    # per-kernel times with HIP events on the system stream (eager steps)
+    # the nonbonded kernel's average launch: back-to-back launches between
+    # HIP events on the stream it runs on (the roofline's time)
+    tnb = np.zeros(1)
+    check(L.bm_time_nbdirect(dev, 20, ptr(tnb)), "nbdirect timing")
+    ms_nb = float(tnb[0])
+    # per-phase times with HIP events on the system stream (eager steps)
     prof = np.zeros(3)
     check(L.bm_profile_steps(dev, step0 + a.steps, 50, ptr(prof)), "profile")
-    ms_nb, ms_pme, ms_eager = prof
+    _, ms_pme, ms_eager = prof
 
     # HBM copy calibration: a plain copy kernel over 256 MiB
     cal = np.zeros(1)

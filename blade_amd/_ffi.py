@@ -158,6 +158,7 @@ def lib():
         "bm_update": (i, [vp, C.c_long, i]),
         "bm_dynamics": (i, [vp, C.c_long, C.c_long, i, i]),
         "bm_profile_steps": (i, [vp, C.c_long, C.c_long, _f64p]),
+        "bm_time_nbdirect": (i, [vp, C.c_int, _f64p]),
         "bm_calibration_copy": (i, [i, C.c_size_t, i, _f64p]),
         "bm_batch_create": (i, [i, C.POINTER(vp), C.POINTER(vp)]),
         "bm_batch_dynamics": (i, [vp, C.c_long, C.c_long, i]),

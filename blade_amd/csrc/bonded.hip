@@ -45,14 +45,16 @@ __device__ __forceinline__ D3v dispd(const Dev &d, int i, int j) {
 // per-atom force of a term into its slot (fixed point); the force
 // assembly adds every atom's slots
 __device__ __forceinline__ void put_d(const Dev &d, int slot, double s, D3v r) {
-  d.bslot[slot] = fix_d(s * r.x);
-  d.bslot[d.nslot + slot] = fix_d(s * r.y);
-  d.bslot[2 * d.nslot + slot] = fix_d(s * r.z);
+  const int p = d.bspos[slot];
+  d.bslot[p] = fix_d(s * r.x);
+  d.bslot[d.nslot + p] = fix_d(s * r.y);
+  d.bslot[2 * d.nslot + p] = fix_d(s * r.z);
 }
 __device__ __forceinline__ void put_f(const Dev &d, int slot, F3 f) {
-  d.bslot[slot] = fix_f(f.x);
-  d.bslot[d.nslot + slot] = fix_f(f.y);
-  d.bslot[2 * d.nslot + slot] = fix_f(f.z);
+  const int p = d.bspos[slot];
+  d.bslot[p] = fix_f(f.x);
+  d.bslot[d.nslot + p] = fix_f(f.y);
+  d.bslot[2 * d.nslot + p] = fix_f(f.z);
 }
 
 // lambda scaling of a term carrying two siteBlock codes (bonded.cu:80-89)

@@ -77,7 +77,8 @@ struct Dev {
   int *nchunks;                  // [1]
   int maxchunks;
   int4 *chunk;                   // [maxchunks] (I, first tile, tiles, 0)
-  long long *ichunk;             // [maxchunks][3][32] i-force partials (fixed point)
+  float *ichunk;                 // [maxchunks][3][32] i-force partials of a chunk (FP32, fixed
+                                 // order); the force assembly converts them to fixed point
   int *ecount, *estart;          // [maxB],[maxB+1] tiles with exclusions per row
   int *nexcltiles;               // [1]
   unsigned *emask;               // [32*2*n_excl]
@@ -117,11 +118,12 @@ struct Dev {
   int2 *nb14_i; int2 *nb14_sb; double4 *nb14_p;           // (qq, c12, c6, e14fac)
   int2 *nbex_i; int2 *nbex_sb; double *nbex_p;            // qq
   // bonded force slots: every term stores its per-atom forces (fixed
-  // point) in slots of its own; a static CSR lists each atom's slots
+  // point) in slots of its own, laid out atom by atom (a static CSR), so
+  // the force assembly reads each atom's slots as one contiguous run
   int sl_bond, sl_angle, sl_dihe, sl_impr, sl_nb14, sl_nbex, nslot;
-  long long *bslot;              // [3][nslot]
-  int *bcsr_start;               // [N+1]
-  int *bcsr;                     // [nslot]
+  long long *bslot;              // [3][nslot] in atom order
+  int *bcsr_start;               // [N+1] first slot position of each atom
+  int *bspos;                    // [nslot] term slot -> position in atom order
   // constraints
   int ntri, nbr1, nbr2, nbr3;
   int4 *tri_i; double4 *tri_b;   // b0 (01,02,12), .w unused

@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
     }
     unsigned long long act = __ballot(keep);
     // i-partials of the whole chunk in FP32 (at most 16 tiles, a fixed
-    // order), converted to fixed point once per chunk
+    // order), stored once per chunk
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float fli = 0.f;
     // software pipeline: the j slots, exclusion word and alchemical flag of
@@ -472,16 +472,16 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
       wave_lds_sync();
       k = kn;
     }
-    // both halves hold partials of the same i-atoms
-    long long ax = fix_f(fx), ay = fix_f(fy), az = fix_f(fz);
-    ax += __shfl_xor(ax, 32, 64);
-    ay += __shfl_xor(ay, 32, 64);
-    az += __shfl_xor(az, 32, 64);
+    // both halves hold partials of the same i-atoms: (own + other) is the
+    // same sum in either half
+    fx += __shfl_xor(fx, 32, 64);
+    fy += __shfl_xor(fy, 32, 64);
+    fz += __shfl_xor(fz, 32, 64);
     if (half == 0) {
-      long long *dst = d.ichunk + (long long)ch * 3 * kBlockAtoms;
-      dst[li] = ax;
-      dst[kBlockAtoms + li] = ay;
-      dst[2 * kBlockAtoms + li] = az;
+      float *dst = d.ichunk + (long long)ch * 3 * kBlockAtoms;
+      dst[li] = fx;
+      dst[kBlockAtoms + li] = fy;
+      dst[2 * kBlockAtoms + li] = fz;
     }
     if (ALCH) {
       fli += __shfl_xor(fli, 32, 64);

@@ -797,19 +797,18 @@ __global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
     const int I = s / kBlockAtoms, li = s % kBlockAtoms;
     const int c1 = d.cstart[I + 1];
     for (int c = d.cstart[I] + sub; c < c1; c += 8) {
-      const long long *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
-      fx += src[li];
-      fy += src[kBlockAtoms + li];
-      fz += src[2 * kBlockAtoms + li];
+      const float *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
+      fx += fix_f(src[li]);
+      fy += fix_f(src[kBlockAtoms + li]);
+      fz += fix_f(src[2 * kBlockAtoms + li]);
     }
   }
   if ((mask & kAsmBonded) && i >= 0) {
     const int k1 = d.bcsr_start[i + 1];
     for (int k = d.bcsr_start[i] + sub; k < k1; k += 8) {
-      const int q = d.bcsr[k];
-      fx += d.bslot[q];
-      fy += d.bslot[d.nslot + q];
-      fz += d.bslot[2 * d.nslot + q];
+      fx += d.bslot[k];
+      fy += d.bslot[d.nslot + k];
+      fz += d.bslot[2 * d.nslot + k];
     }
   }
   // integer sums: the same total in any order

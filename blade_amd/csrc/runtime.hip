@@ -461,16 +461,16 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       for (int k = 0; k < 2; k++) owner[d.sl_nb14 + 2 * q + k] = t->nb14_idx[2 * q + k];
     for (int q = 0; q < d.nnbex; q++)
       for (int k = 0; k < 2; k++) owner[d.sl_nbex + 2 * q + k] = t->nbex_idx[2 * q + k];
-    std::vector<int> cstartv(n + 1, 0), cs(d.nslot);
+    std::vector<int> cstartv(n + 1, 0), pos(d.nslot);
     for (int q = 0; q < d.nslot; q++) cstartv[owner[q] + 1]++;
     for (int a = 0; a < n; a++) cstartv[a + 1] += cstartv[a];
     {
       std::vector<int> fill(cstartv.begin(), cstartv.end() - 1);
-      for (int q = 0; q < d.nslot; q++) cs[fill[owner[q]]++] = q;
+      for (int q = 0; q < d.nslot; q++) pos[q] = fill[owner[q]]++;
     }
     ALLOC(d.bslot, 3 * (size_t)d.nslot);
     UPLOAD(d.bcsr_start, cstartv);
-    UPLOAD(d.bcsr, cs);
+    UPLOAD(d.bspos, pos);
   }
   // constraints
   {
@@ -837,6 +837,18 @@ int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
   out[1] = tp / nsteps;
   out[2] = tt / nsteps;
   s->msNb = out[0]; s->msPme = out[1]; s->msTotal = out[2]; s->nMeas = nsteps;
+  return BM_OK;
+}
+
+int bm_time_nbdirect(bm_system *s, int reps, double *out) {
+  if (!s || reps <= 0 || !out) return BM_EINVAL;
+  HIPCHK(hipEventRecord(s->ev[0], s->st));
+  for (int r = 0; r < reps; r++) blade::launch_nbdirect(s->d, false, s->alch, s->soft, s->nwgNb, s->st);
+  HIPCHK(hipEventRecord(s->ev[1], s->st));
+  HIPCHK(hipEventSynchronize(s->ev[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+  out[0] = ms / reps;
   return BM_OK;
 }
 

@@ -257,6 +257,13 @@ int bm_get_kernel_times(bm_system *sys, double *ms_nbdirect, double *ms_pme,
  * nbdirect, PME (spread+FFT+gather) and the whole step. */
 int bm_profile_steps(bm_system *sys, long step0, long nsteps, double *out);
 
+/* Average duration of the nonbonded kernel: `reps` back-to-back launches
+   (non-energy variant, current list and positions) between two HIP events on
+   the system's stream; out[0] = ms per launch.  The launches rewrite the
+   chunk partials and add dU/dlambda contributions that the next step's pack
+   clears.  Measurement support, no counterpart in the reference. */
+int bm_time_nbdirect(bm_system *sys, int reps, double *out);
+
 /* HBM calibration: `reps` launches of a plain 16-byte-vector copy kernel
    (bm_calibration_copy_kernel) over `bytes` bytes on `device`; out[0] = mean
    ms per launch.  Measurement support, no counterpart in the reference. */
