@@ -582,12 +582,19 @@ int bm_system_destroy(bm_system *s) {
 int bm_sync(bm_system *s) {
   if (!s) return BM_EINVAL;
   HIPCHK(hipStreamSynchronize(s->st));
-  int flag = 0;
+  int flag = 0, nan = -1;
   HIPCHK(hipMemcpy(&flag, s->d.errflag, sizeof(int), hipMemcpyDeviceToHost));
-  if (flag & 1) return set_err(BM_EOVERFLOW, "a domdec column exceeded its capacity");
-  if (flag & 2) return set_err(BM_EOVERFLOW, "a PME spread brick exceeded its atom capacity");
-  if (flag & 4) return set_err(BM_EOVERFLOW, "the tile list exceeded its capacity");
-  return BM_OK;
+  HIPCHK(hipMemcpy(&nan, s->d.nanflag, sizeof(int), hipMemcpyDeviceToHost));
+  if (!flag && nan < 0) return BM_OK;
+  // every condition that was raised, the first non-finite value first (a
+  // blow-up also scatters atoms into one column)
+  std::string msg;
+  if (nan >= 0) msg += "non-finite velocity (first at atom or DOF " + std::to_string(nan) + "); ";
+  if (flag & 4) msg += "the tile list exceeded its capacity; ";
+  if (flag & 2) msg += "a PME spread brick exceeded its atom capacity; ";
+  if (flag & 1) msg += "a domdec column exceeded its capacity; ";
+  msg.resize(msg.size() - 2);
+  return set_err(nan >= 0 ? BM_ENAN : BM_EOVERFLOW, msg.c_str());
 }
 
 int bm_domdec_update(bm_system *s, int rebuild) {

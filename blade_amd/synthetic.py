@@ -244,9 +244,12 @@ def _place(prev, prev2, prev3, b, theta, phi):
     return prev + d2[0] * bc + d2[1] * m + d2[2] * n
 
 
-def _polymer(spec, rng, natoms, center, radius, grid, site_at=None, nsubs=3, sub_size=8):
+def _polymer(spec, rng, natoms, center, radius, grid, site_at=None, nsubs=3, sub_size=8, typed_subs=False):
     """Self-avoiding chain of C/N/O heavy atoms with hydrogens, confined to a
-    sphere, exactly `natoms` atoms including the lambda-site substituents."""
+    sphere, exactly `natoms` atoms including the lambda-site substituents.
+    typed_subs: substituent charges by the body's rule (hydrogens positive by
+    type, heavy atoms balancing their hydrogens, each substituent neutral)
+    instead of plain N(0, 0.35) draws."""
     _poly_params(spec)
     body = natoms - (nsubs * sub_size if site_at is not None else 0)
     # the attachment carbon gives up one hydrogen to the substituents
@@ -468,6 +471,13 @@ def _polymer(spec, rng, natoms, center, radius, grid, site_at=None, nsubs=3, sub
                 atoms_s.append(idx)
                 spec.bonds.append((hidx[hk], idx))
             spec.bonds += [(heavy_idx[k], hidx[0]), (hidx[0], hidx[1]), (hidx[1], hidx[2])]
+            if typed_subs:
+                # hydrogens positive by type, heavy atoms balance their own
+                # hydrogens plus a spread, the substituent is neutral
+                qh = [max(0.03, hq0[spec.atoms[a][4]] + 0.1 * float(qv)) for a, qv in zip(atoms_s[3:], qs[3:])]
+                qheavy = [-sum(q for (hk, _), q in zip(hs, qh) if hk == j) + 0.5 * float(qs[j]) for j in range(3)]
+                shift = (sum(qheavy) + sum(qh)) / 3.0
+                qs = [q - shift for q in qheavy] + qh
             for a, qv in zip(atoms_s, qs):
                 rec = list(spec.atoms[a])
                 rec[5] = float(qv)
@@ -609,11 +619,50 @@ def config2(seed=0xB1ADE002, n_water=7023, n_poly=2489, box=62.23):
 WATER_DENSITY = 0.0978   # atoms per A^3 in a TIP3P box
 
 
+def _ligand_clash(spec, block_of, dmin=1.7):
+    """True if two atoms of the ligand built so far (every atom of `spec`)
+    sit closer than dmin although they interact: not within two bonds of
+    each other, and not in different blocks of the one lambda site
+    (alternative substituents overlap by design)."""
+    n = len(spec._pos)
+    nbr = [set() for _ in range(n)]
+    for a, b in spec.bonds:
+        nbr[a].add(b)
+        nbr[b].add(a)
+    P = np.asarray(spec._pos, float)
+    L = np.array(spec.box)
+    for a in range(n):
+        near = set(nbr[a])
+        for b in nbr[a]:
+            near |= nbr[b]
+        d = P[a + 1:] - P[a]
+        d -= L * np.round(d / L)
+        r = np.linalg.norm(d, axis=1)
+        for b in np.nonzero(r < dmin)[0] + a + 1:
+            b = int(b)
+            if b in near:
+                continue
+            if block_of[a] and block_of[b] and block_of[a] != block_of[b]:
+                continue
+            return True
+    return False
+
+
 def batch_member(seed, natoms, nsubs, core=None):
     """One member of the batched workload: TIP3P water at box density plus
     a ligand (a 30-60-atom core, seeded) carrying one lambda site of
     `nsubs` substituents x 8 atoms; cutoff 10 A, switch 9-10 A, PME grid
-    spacing <= 1 A (the size rule picks the grid), order 6, fnex 5."""
+    spacing <= 1 A (the size rule picks the grid), order 6, fnex 5.  A
+    ligand whose substituents land on core atoms is rebuilt from the next
+    derived seed (seed + attempt * 2^32)."""
+    for attempt in range(64):
+        spec = _batch_member_once(seed + (attempt << 32), natoms, nsubs, core)
+        if spec is not None:
+            return spec
+    raise RuntimeError("batch member %d: no clash-free ligand in 64 seeds" % seed)
+
+
+def _batch_member_once(seed, natoms, nsubs, core):
     rng = np.random.default_rng(seed)
     if core is None:
         core = int(rng.integers(30, 61))
@@ -625,10 +674,13 @@ def batch_member(seed, natoms, nsubs, core=None):
     _water_params(spec)
     g = _Grid(spec.box, 3.0)
     radius = 6.0 * (n_lig / 60.0) ** (1 / 3) + 2.0
-    _, subs = _polymer(spec, rng, n_lig, np.array(spec.box) / 2, radius, g, site_at=0.5, nsubs=nsubs)
+    _, subs = _polymer(spec, rng, n_lig, np.array(spec.box) / 2, radius, g, site_at=0.5, nsubs=nsubs,
+                       typed_subs=True)
     block_of = np.zeros(spec.n, dtype=np.int32)
     for s, atoms in enumerate(subs):
         block_of[atoms] = s + 1
+    if _ligand_clash(spec, block_of):
+        return None
     _angles_dihes(spec, block_of)
     _fill_water(spec, rng, n_water, g)
     spec.pos = np.array(spec._pos)
