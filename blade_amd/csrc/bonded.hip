@@ -146,8 +146,22 @@ struct BondedLaunch {
   int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_bias;  // first block of each family
 };
 
-__global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool energy) {
-  const int bid = blockIdx.x;
+// workgroups of 256 threads per term family, in family order; the last
+// workgroup carries the Ewald self term and the lambda biases
+__host__ __device__ inline BondedLaunch bonded_layout(const Dev &d) {
+  auto nb = [](int n) { return (n + 255) / 256; };
+  BondedLaunch L;
+  L.b_bond = 0;
+  L.b_angle = L.b_bond + nb(d.nbond);
+  L.b_dihe = L.b_angle + nb(d.nangle);
+  L.b_impr = L.b_dihe + nb(d.ndihe);
+  L.b_nb14 = L.b_impr + nb(d.nimpr);
+  L.b_nbex = L.b_nb14 + nb(d.nnb14);
+  L.b_bias = L.b_nbex + nb(d.nnbex);
+  return L;
+}
+
+__device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L, int bid, bool energy) {
   double e = 0.0;
   int slot = -1;
   if (bid < L.b_angle) {
@@ -347,19 +361,23 @@ __global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool
   if (energy) wave_atomic_fix(&d.energy[slot], fix_d(e));
 }
 
-void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st) {
-  auto nb = [](int n) { return (n + 255) / 256; };
-  BondedLaunch L;
-  L.b_bond = 0;
-  L.b_angle = L.b_bond + nb(d.nbond);
-  L.b_dihe = L.b_angle + nb(d.nangle);
-  L.b_impr = L.b_dihe + nb(d.ndihe);
-  L.b_nb14 = L.b_impr + nb(d.nimpr);
-  L.b_nbex = L.b_nb14 + nb(d.nnb14);
-  L.b_bias = L.b_nbex + nb(d.nnbex);
-  // the last workgroup carries the Ewald self term and the lambda biases
-  (void)bias;
-  bonded_kernel<<<L.b_bias + 1, 256, 0, st>>>(d, L, energy);
+__global__ void __launch_bounds__(256) bonded_kernel(Dev d, BondedLaunch L, bool energy) {
+  bonded_body(d, L, blockIdx.x, energy);
 }
+
+__global__ void __launch_bounds__(256) bonded_multi(Multi m, bool energy) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  bonded_body(d, bonded_layout(d), b.bid, energy);
+}
+
+int grid_bonded(const Dev &d) { return bonded_layout(d).b_bias + 1; }
+
+void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st) {
+  (void)bias;
+  bonded_kernel<<<grid_bonded(d), 256, 0, st>>>(d, bonded_layout(d), energy);
+}
+
+void launch_bonded_multi(const Multi &m, bool energy, hipStream_t st) { bonded_multi<<<m.nwg, 256, 0, st>>>(m, energy); }
 
 }  // namespace blade

@@ -215,6 +215,31 @@ __device__ __forceinline__ int wave_claim(int *counters, int key, bool active) {
   return active ? base + __popcll(grp & ((1ull << lane) - 1ull)) : 0;
 }
 
+// Several systems in one launch (batches): the workgroups of system k are
+// [wg0[k], wg0[k+1]) of the grid; devs holds device copies of the systems'
+// Dev.  A kernel body runs with its system's Dev and its workgroup index
+// and count within that system, exactly as a launch of that system alone.
+struct Multi {
+  const Dev *devs;   // [nsys]
+  const int *wg0;    // [nsys+1]
+  int nsys, nwg;     // systems, workgroups in all
+};
+
+struct MultiBlk {
+  int sys, bid, nblk;
+};
+
+__device__ __forceinline__ MultiBlk multi_block(const Multi &m) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = m.nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.wg0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return MultiBlk{lo, b - m.wg0[lo], m.wg0[lo + 1] - m.wg0[lo]};
+}
+
 // Block-wide fixed-point energy sum (any blockDim multiple of 64).
 __device__ __forceinline__ void block_energy(long long *p, float e) {
   wave_atomic_fix(p, fix_f(e));

@@ -372,8 +372,8 @@ __global__ void dd_excl_set(Dev d) {
 
 // Every step: slot coordinates, per-block box centre/half-extent for the
 // per-step cull, and reset of the force/energy accumulators.
-__global__ void __launch_bounds__(256) dd_pack(Dev d) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void dd_pack_body(const Dev &d, int bid) {
+  const int s = bid * blockDim.x + threadIdx.x;
   const int nb = *d.nblocks;
   const int nf = 3 * d.n + d.nl;
   if (s < nf) d.force[s] = 0;
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) dd_pack(Dev d) {
     d.workctr[0] = 0;
     d.workctr[1] = 0;
   }
-  if (blockIdx.x * blockDim.x >= nb * kBlockAtoms) return;  // whole wave idle
+  if (bid * (int)blockDim.x >= nb * kBlockAtoms) return;  // whole wave idle
   const bool in = s < nb * kBlockAtoms;
   const int a = in ? d.l2g[s] : -1;
   unsigned u[3] = {0, 0, 0};
@@ -419,6 +419,14 @@ __global__ void __launch_bounds__(256) dd_pack(Dev d) {
   }
 }
 
+__global__ void __launch_bounds__(256) dd_pack(Dev d) { dd_pack_body(d, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) dd_pack_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_pack_body(d, b.bid);
+}
+
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
@@ -445,12 +453,19 @@ void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st) {
   }
 }
 
-void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st) {
-  long long work = (long long)maxBlocks * kBlockAtoms;
+int grid_pack(const Dev &d) {
+  long long work = (long long)d.maxblocks * kBlockAtoms;
   long long nf = 3LL * d.n + d.nl;
   if (nf > work) work = nf;
   if (work < BM_NTERMS) work = BM_NTERMS;
-  dd_pack<<<(unsigned)((work + 255) / 256), 256, 0, st>>>(d);
+  return (int)((work + 255) / 256);
 }
+
+void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st) {
+  (void)maxBlocks;
+  dd_pack<<<grid_pack(d), 256, 0, st>>>(d);
+}
+
+void launch_pack_multi(const Multi &m, hipStream_t st) { dd_pack_multi<<<m.nwg, 256, 0, st>>>(m); }
 
 }  // namespace blade

@@ -26,5 +26,37 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st);
 void launch_update_after_kick(const Dev &d, hipStream_t st);
 void launch_theta_to_lambda(const Dev &d, hipStream_t st);
 void launch_set_step(const Dev &d, long long step, hipStream_t st);
+// the FFT part of launch_pme_grid alone
+void launch_pme_fft(const Dev &d, bool energy, hipStream_t st);
+
+// ---- several systems per launch (Multi, device.h) ----------------------
+// Workgroups one system needs per kernel type; the single-system launches
+// above use the same counts, so a system's work is the same either way.
+int grid_pack(const Dev &d);
+int grid_normals(const Dev &d);
+int grid_bonded(const Dev &d);
+int grid_pme_bin(const Dev &d);
+int grid_pme_spread(const Dev &d);
+size_t lds_pme_spread(const Dev &d);
+bool fft_plane_path(const Dev &d);
+int grid_fft_planes(const Dev &d);
+int grid_fft_x(const Dev &d);
+int grid_nbdirect(const Dev &d);
+size_t lds_nbdirect(const Dev &d);
+int grid_assemble(const Dev &d);
+int grid_tail(const Dev &d);
+// Every system of a Multi must share the template keys of its kernel: the
+// spline order (bin, spread, assembly), the grid sizes (FFT), the methods
+// and flags (nbdirect).
+void launch_pack_multi(const Multi &m, hipStream_t st);
+void launch_normals_multi(const Multi &m, hipStream_t st);
+void launch_bonded_multi(const Multi &m, bool energy, hipStream_t st);
+void launch_pme_bin_multi(const Multi &m, int order, hipStream_t st);
+void launch_pme_spread_multi(const Multi &m, int order, size_t lds, hipStream_t st);
+void launch_pme_fft_multi(const Multi &mp, const Multi &mx, int K0, int K1, bool energy, hipStream_t st);
+void launch_nbdirect_multi(const Multi &m, int elec, int vdw, bool energy, bool alch, bool soft, size_t lds,
+                           hipStream_t st);
+void launch_assemble_multi(const Multi &m, int order, unsigned mask, bool kick, hipStream_t st);
+void launch_tail_multi(const Multi &m, hipStream_t st);
 
 }  // namespace blade

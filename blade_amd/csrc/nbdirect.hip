@@ -357,7 +357,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool GTABLE>
-__global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
+__device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int nblk) {
   __shared__ uint4 s_pos[4][kBlockAtoms];
   __shared__ int2 s_par[4][kBlockAtoms];
   __shared__ float s_lo[4][kBlockAtoms];
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
 
   // one chunk per wave; waves past the chunk count of this step exit
   {
-    const int ch = blockIdx.x * 4 + w;
+    const int ch = bid * 4 + w;
     if (ch >= nchunks) return;
     const int4 cr = d.chunk[ch];
     const int I = cr.x, t0 = cr.y, cnt = cr.z;
@@ -494,17 +494,43 @@ __global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) {
   }
 }
 
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool GTABLE>
+__global__ void __launch_bounds__(256) nbdirect_kernel(Dev d) { nbdirect_kernel_body<ELEC, VDW, ENERGY, ALCH, SOFT, GTABLE>(d, blockIdx.x, gridDim.x); }
+
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool GTABLE>
+__global__ void __launch_bounds__(256) nbdirect_kernel_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  nbdirect_kernel_body<ELEC, VDW, ENERGY, ALCH, SOFT, GTABLE>(d, b.bid, b.nblk);
+}
+
 // one wave per chunk slot of the list capacity; waves past this step's
 // chunk count exit at once
+int grid_nbdirect(const Dev &d) { return (d.maxchunks + 3) / 4; }
+
+// the LJ table in LDS (0: the table is read from global memory)
+size_t lds_nbdirect(const Dev &d) {
+  const int nt2 = d.ntypes * d.ntypes;
+  return nt2 <= kVdwLdsMax ? (size_t)nt2 * sizeof(float2) : 0;
+}
+
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
 static void launch_t(const Dev &d, int nwg, hipStream_t st) {
   (void)nwg;
-  const unsigned grid = (d.maxchunks + 3) / 4;
-  const int nt2 = d.ntypes * d.ntypes;
-  if (nt2 <= kVdwLdsMax)
-    nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, false><<<grid, 256, nt2 * sizeof(float2), st>>>(d);
+  const unsigned grid = (unsigned)grid_nbdirect(d);
+  const size_t lds = lds_nbdirect(d);
+  if (lds)
+    nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, false><<<grid, 256, lds, st>>>(d);
   else
     nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, true><<<grid, 256, 0, st>>>(d);
+}
+
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
+static void launch_mt(const Multi &m, size_t lds, hipStream_t st) {
+  if (lds)
+    nbdirect_kernel_multi<ELEC, VDW, ENERGY, ALCH, SOFT, false><<<m.nwg, 256, lds, st>>>(m);
+  else
+    nbdirect_kernel_multi<ELEC, VDW, ENERGY, ALCH, SOFT, true><<<m.nwg, 256, 0, st>>>(m);
 }
 
 template <int ELEC, int VDW>
@@ -518,6 +544,39 @@ static void launch_ev(const Dev &d, bool energy, bool alch, bool soft, int nwg, 
   } else {
     if (energy) launch_t<ELEC, VDW, true, true, true>(d, nwg, st);
     else launch_t<ELEC, VDW, false, true, true>(d, nwg, st);
+  }
+}
+
+template <int ELEC, int VDW>
+static void launch_mev(const Multi &m, bool energy, bool alch, bool soft, size_t lds, hipStream_t st) {
+  if (!alch) {
+    if (energy) launch_mt<ELEC, VDW, true, false, false>(m, lds, st);
+    else launch_mt<ELEC, VDW, false, false, false>(m, lds, st);
+  } else if (!soft) {
+    if (energy) launch_mt<ELEC, VDW, true, true, false>(m, lds, st);
+    else launch_mt<ELEC, VDW, false, true, false>(m, lds, st);
+  } else {
+    if (energy) launch_mt<ELEC, VDW, true, true, true>(m, lds, st);
+    else launch_mt<ELEC, VDW, false, true, true>(m, lds, st);
+  }
+}
+
+// systems sharing the methods and flags; lds: the largest LJ table of the
+// group (0 when any of them is read from global memory)
+void launch_nbdirect_multi(const Multi &m, int elec, int vdw, bool energy, bool alch, bool soft, size_t lds,
+                           hipStream_t st) {
+  if (elec == BM_PME) {
+    if (vdw == BM_VFSWITCH) launch_mev<BM_PME, BM_VFSWITCH>(m, energy, alch, soft, lds, st);
+    else if (vdw == BM_VSWITCH) launch_mev<BM_PME, BM_VSWITCH>(m, energy, alch, soft, lds, st);
+    else launch_mev<BM_PME, BM_VSHIFT>(m, energy, alch, soft, lds, st);
+  } else if (elec == BM_FSWITCH) {
+    if (vdw == BM_VFSWITCH) launch_mev<BM_FSWITCH, BM_VFSWITCH>(m, energy, alch, soft, lds, st);
+    else if (vdw == BM_VSWITCH) launch_mev<BM_FSWITCH, BM_VSWITCH>(m, energy, alch, soft, lds, st);
+    else launch_mev<BM_FSWITCH, BM_VSHIFT>(m, energy, alch, soft, lds, st);
+  } else {
+    if (vdw == BM_VFSWITCH) launch_mev<BM_FSHIFT, BM_VFSWITCH>(m, energy, alch, soft, lds, st);
+    else if (vdw == BM_VSWITCH) launch_mev<BM_FSHIFT, BM_VSWITCH>(m, energy, alch, soft, lds, st);
+    else launch_mev<BM_FSHIFT, BM_VSHIFT>(m, energy, alch, soft, lds, st);
   }
 }
 

@@ -70,9 +70,9 @@ __device__ __forceinline__ int brick_of(int u, int K, int nb) {
 // order is arbitrary; the spread sums in fixed point, so the grid does not
 // depend on it.
 template <int N>
-__global__ void __launch_bounds__(256) pme_bin(Dev d) {
+__device__ __forceinline__ void pme_bin_body(const Dev &d, int bid, int nblk) {
   constexpr int W = pme_spline_rec_words(N);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = bid * blockDim.x + threadIdx.x;
   const int i = s < *d.nblocks * kBlockAtoms ? d.l2g[s] : -1;
   bool act = false;
   int br = 0;
@@ -107,6 +107,16 @@ __global__ void __launch_bounds__(256) pme_bin(Dev d) {
       atomicOr(d.errflag, 2);
     }
   }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) pme_bin(Dev d) { pme_bin_body<N>(d, blockIdx.x, gridDim.x); }
+
+template <int N>
+__global__ void __launch_bounds__(256) pme_bin_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  pme_bin_body<N>(d, b.bid, b.nblk);
 }
 
 // candidate atoms staged per round (LDS per workgroup stays near 60 KB, so
@@ -175,7 +185,7 @@ __device__ __forceinline__ int stencil_clip(int r, int sz, int K) {
 // stored to the real grid with plain stores: no global atomics, and no
 // clearing pass.
 template <int N>
-__global__ void __launch_bounds__(256) pme_spread(Dev d) {
+__device__ __forceinline__ void pme_spread_body(const Dev &d, int bid, int nblk) {
   constexpr int kStage = spread_stage<N>();
   extern __shared__ long long acc[];
   __shared__ double s_w[3][N][kStage];
@@ -187,7 +197,7 @@ __global__ void __launch_bounds__(256) pme_spread(Dev d) {
   __shared__ int s_wsum[4];
   int bidx[3], lo[3], sz[3];
   {
-    int r = blockIdx.x;
+    int r = bid;
     bidx[2] = r % d.pnb[2]; r /= d.pnb[2];
     bidx[1] = r % d.pnb[1];
     bidx[0] = r / d.pnb[1];
@@ -291,6 +301,16 @@ __global__ void __launch_bounds__(256) pme_spread(Dev d) {
     d.rgrid[((long long)(lo[0] + px) * d.K[1] + lo[1] + py) * d.K[2] + lo[2] + pz] =
         (float)((double)acc[pxy * rowz + pz] * kFixInvD);
   }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) pme_spread(Dev d) { pme_spread_body<N>(d, blockIdx.x, gridDim.x); }
+
+template <int N>
+__global__ void __launch_bounds__(256) pme_spread_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  pme_spread_body<N>(d, b.bid, b.nblk);
 }
 
 // ---------------------------------------------------------------------
@@ -603,10 +623,10 @@ __device__ __forceinline__ float2 *ct_fft(float2 *src, float2 *dst, const float2
 // the real charge grid, writes the complex grid.  1024 threads: there are
 // only K0 planes, so each plane's workgroup carries 16 waves.
 template <int K1, int K2>
-__global__ void __launch_bounds__(1024) fft_yz_fwd(Dev d) {
+__device__ __forceinline__ void fft_yz_fwd_body(const Dev &d, int bid, int nblk) {
   __shared__ float2 b0[K1 * K2], b1[K1 * K2];
   __shared__ float2 t1[K1], t2[K2];
-  const long long base = (long long)blockIdx.x * K1 * K2;
+  const long long base = (long long)bid * K1 * K2;
   for (int m = threadIdx.x; m < K1; m += blockDim.x) t1[m] = d.twiddle[d.K[0] + m];
   for (int m = threadIdx.x; m < K2; m += blockDim.x) t2[m] = d.twiddle[d.K[0] + K1 + m];
   for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) b0[e] = make_float2(d.rgrid[base + e], 0.f);
@@ -617,13 +637,23 @@ __global__ void __launch_bounds__(1024) fft_yz_fwd(Dev d) {
   for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) d.cgrid[base + e] = r[e];
 }
 
+template <int K1, int K2>
+__global__ void __launch_bounds__(1024) fft_yz_fwd(Dev d) { fft_yz_fwd_body<K1, K2>(d, blockIdx.x, gridDim.x); }
+
+template <int K1, int K2>
+__global__ void __launch_bounds__(1024) fft_yz_fwd_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  fft_yz_fwd_body<K1, K2>(d, b.bid, b.nblk);
+}
+
 // Inverse transform along y then z of one x-plane; the real part (the
 // potential on the grid) goes to the real grid.
 template <int K1, int K2>
-__global__ void __launch_bounds__(1024) fft_yz_inv(Dev d) {
+__device__ __forceinline__ void fft_yz_inv_body(const Dev &d, int bid, int nblk) {
   __shared__ float2 b0[K1 * K2], b1[K1 * K2];
   __shared__ float2 t1[K1], t2[K2];
-  const long long base = (long long)blockIdx.x * K1 * K2;
+  const long long base = (long long)bid * K1 * K2;
   for (int m = threadIdx.x; m < K1; m += blockDim.x) t1[m] = d.twiddle[d.K[0] + m];
   for (int m = threadIdx.x; m < K2; m += blockDim.x) t2[m] = d.twiddle[d.K[0] + K1 + m];
   for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) b0[e] = d.cgrid[base + e];
@@ -634,15 +664,25 @@ __global__ void __launch_bounds__(1024) fft_yz_inv(Dev d) {
   for (int e = threadIdx.x; e < K1 * K2; e += blockDim.x) d.rgrid[base + e] = r[e].x;
 }
 
+template <int K1, int K2>
+__global__ void __launch_bounds__(1024) fft_yz_inv(Dev d) { fft_yz_inv_body<K1, K2>(d, blockIdx.x, gridDim.x); }
+
+template <int K1, int K2>
+__global__ void __launch_bounds__(1024) fft_yz_inv_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  fft_yz_inv_body<K1, K2>(d, b.bid, b.nblk);
+}
+
 // Lines along x (NL per workgroup): forward transform, the convolution
 // B(m) kE/(2 pi V) exp(-pi^2 m^2/beta^2)/m^2 (with the reciprocal energy
 // sum on energy steps), inverse transform.
 template <int K0, int NL>
-__global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
+__device__ __forceinline__ void fft_x_conv_body(const Dev &d, int bid, int nblk, int energy) {
   __shared__ float2 b0[K0 * NL], b1[K0 * NL];
   __shared__ float2 t0[K0];
   const int nlines = d.K[1] * d.K[2];
-  const int l0 = blockIdx.x * NL;
+  const int l0 = bid * NL;
   const long long plane = (long long)d.K[1] * d.K[2];
   for (int m = threadIdx.x; m < K0; m += blockDim.x) t0[m] = d.twiddle[m];
   // element m of line l at b[m*NL + l]: consecutive threads, consecutive lines
@@ -685,6 +725,16 @@ __global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
   }
 }
 
+template <int K0, int NL>
+__global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) { fft_x_conv_body<K0, NL>(d, blockIdx.x, gridDim.x, energy); }
+
+template <int K0, int NL>
+__global__ void __launch_bounds__(256) fft_x_conv_multi(Multi m, int energy) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  fft_x_conv_body<K0, NL>(d, b.bid, b.nblk, energy);
+}
+
 // sizes with compiled plane / line kernels
 #define BM_FFT_SIZES(X) X(16) X(18) X(20) X(24) X(27) X(32) X(36) X(40) X(48) X(54) X(64)
 
@@ -698,9 +748,9 @@ __global__ void __launch_bounds__(256) fft_x_conv(Dev d, int energy) {
 // per force component and stream order make the read-modify-write of the
 // force array safe without atomics.
 template <int N, bool KICK>
-__global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
+__device__ __forceinline__ void force_assemble_body(const Dev &d, int bid, int nblk, unsigned mask) {
   // eight lanes per list slot; every lane of a group takes the same branches
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = bid * blockDim.x + threadIdx.x;
   const int s = g >> 3, sub = g & 7;
   const bool live = s < *d.nblocks * kBlockAtoms;
   const int i = live ? d.l2g[s] : -1;
@@ -838,6 +888,16 @@ __global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) {
   }
 }
 
+template <int N, bool KICK>
+__global__ void __launch_bounds__(256) force_assemble(Dev d, unsigned mask) { force_assemble_body<N, KICK>(d, blockIdx.x, gridDim.x, mask); }
+
+template <int N, bool KICK>
+__global__ void __launch_bounds__(256) force_assemble_multi(Multi m, unsigned mask) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  force_assemble_body<N, KICK>(d, b.bid, b.nblk, mask);
+}
+
 // ---------------------------------------------------------------------
 static void make_plan(FftPlan &p, int K, const float2 *tw) {
   p.K = K;
@@ -858,30 +918,33 @@ bool pme_grid_supported(int K) {
   return n == 1;
 }
 
-void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
-  const int N = d.order;
-  const int nbr = d.pnb[0] * d.pnb[1] * d.pnb[2];
+int grid_pme_bin(const Dev &d) { return (int)((32LL * d.maxblocks + 255) / 256); }
+
+int grid_pme_spread(const Dev &d) { return d.pnb[0] * d.pnb[1] * d.pnb[2]; }
+
+size_t lds_pme_spread(const Dev &d) {
   int bmax = 1;
   for (int k = 0; k < 3; k++) bmax *= (d.K[k] + d.pnb[k] - 1) / d.pnb[k] + (k == 2 ? 2 : 0);
-  const size_t lds = (size_t)bmax * sizeof(long long);
-  const unsigned gb = (unsigned)((32LL * d.maxblocks + 255) / 256);
-  if (N == 4) {
-    pme_bin<4><<<gb, 256, 0, st>>>(d);
-    pme_spread<4><<<nbr, 256, lds, st>>>(d);
-  } else if (N == 6) {
-    pme_bin<6><<<gb, 256, 0, st>>>(d);
-    pme_spread<6><<<nbr, 256, lds, st>>>(d);
-  } else {
-    pme_bin<8><<<gb, 256, 0, st>>>(d);
-    pme_spread<8><<<nbr, 256, lds, st>>>(d);
-  }
+  return (size_t)bmax * sizeof(long long);
+}
+
+// the three-kernel plane FFT applies (compiled size, square yz-plane in LDS)
+bool fft_plane_path(const Dev &d) {
   bool ct0 = false, ct1 = false;
 #define BM_IS(SZ) ct0 = ct0 || d.K[0] == SZ; ct1 = ct1 || d.K[1] == SZ;
   BM_FFT_SIZES(BM_IS)
 #undef BM_IS
-  if (ct0 && ct1 && d.K[1] == d.K[2] && d.K[1] * d.K[2] <= 4096) {
+  return ct0 && ct1 && d.K[1] == d.K[2] && d.K[1] * d.K[2] <= 4096;
+}
+
+int grid_fft_planes(const Dev &d) { return d.K[0]; }
+
+int grid_fft_x(const Dev &d) { return (d.K[1] * d.K[2] + 15) / 16; }
+
+void launch_pme_fft(const Dev &d, bool energy, hipStream_t st) {
+  if (fft_plane_path(d)) {
     // plane path: forward z+y per x-plane, x with convolution, inverse y+z
-    const unsigned gx = (unsigned)((d.K[1] * d.K[2] + 15) / 16);
+    const unsigned gx = (unsigned)grid_fft_x(d);
     switch (d.K[1]) {
 #define BM_YZF(SZ) case SZ: fft_yz_fwd<SZ, SZ><<<d.K[0], 1024, 0, st>>>(d); break;
       BM_FFT_SIZES(BM_YZF)
@@ -918,6 +981,56 @@ void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
   }
 }
 
+void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
+  const int N = d.order;
+  const unsigned nbr = (unsigned)grid_pme_spread(d);
+  const size_t lds = lds_pme_spread(d);
+  const unsigned gb = (unsigned)grid_pme_bin(d);
+  if (N == 4) {
+    pme_bin<4><<<gb, 256, 0, st>>>(d);
+    pme_spread<4><<<nbr, 256, lds, st>>>(d);
+  } else if (N == 6) {
+    pme_bin<6><<<gb, 256, 0, st>>>(d);
+    pme_spread<6><<<nbr, 256, lds, st>>>(d);
+  } else {
+    pme_bin<8><<<gb, 256, 0, st>>>(d);
+    pme_spread<8><<<nbr, 256, lds, st>>>(d);
+  }
+  launch_pme_fft(d, energy, st);
+}
+
+void launch_pme_bin_multi(const Multi &m, int order, hipStream_t st) {
+  if (order == 4) pme_bin_multi<4><<<m.nwg, 256, 0, st>>>(m);
+  else if (order == 6) pme_bin_multi<6><<<m.nwg, 256, 0, st>>>(m);
+  else pme_bin_multi<8><<<m.nwg, 256, 0, st>>>(m);
+}
+
+void launch_pme_spread_multi(const Multi &m, int order, size_t lds, hipStream_t st) {
+  if (order == 4) pme_spread_multi<4><<<m.nwg, 256, lds, st>>>(m);
+  else if (order == 6) pme_spread_multi<6><<<m.nwg, 256, lds, st>>>(m);
+  else pme_spread_multi<8><<<m.nwg, 256, lds, st>>>(m);
+}
+
+// systems of one grid size on the plane path: mp launches K0 workgroups per
+// system, mx grid_fft_x() per system
+void launch_pme_fft_multi(const Multi &mp, const Multi &mx, int K0, int K1, bool energy, hipStream_t st) {
+  switch (K1) {
+#define BM_YZF(SZ) case SZ: fft_yz_fwd_multi<SZ, SZ><<<mp.nwg, 1024, 0, st>>>(mp); break;
+    BM_FFT_SIZES(BM_YZF)
+#undef BM_YZF
+  }
+  switch (K0) {
+#define BM_XC(SZ) case SZ: fft_x_conv_multi<SZ, 16><<<mx.nwg, 256, 0, st>>>(mx, energy ? 1 : 0); break;
+    BM_FFT_SIZES(BM_XC)
+#undef BM_XC
+  }
+  switch (K1) {
+#define BM_YZI(SZ) case SZ: fft_yz_inv_multi<SZ, SZ><<<mp.nwg, 1024, 0, st>>>(mp); break;
+    BM_FFT_SIZES(BM_YZI)
+#undef BM_YZI
+  }
+}
+
 template <bool KICK>
 static void assemble_t(const Dev &d, unsigned mask, unsigned g, hipStream_t st) {
   if (!(mask & kAsmPme)) force_assemble<0, KICK><<<g, 256, 0, st>>>(d, mask);
@@ -926,12 +1039,34 @@ static void assemble_t(const Dev &d, unsigned mask, unsigned g, hipStream_t st) 
   else force_assemble<8, KICK><<<g, 256, 0, st>>>(d, mask);
 }
 
-void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st, bool kick) {
+int grid_assemble(const Dev &d) {
   const int nbr = d.elecmethod == BM_PME ? d.pnb[0] * d.pnb[1] * d.pnb[2] : 0;
-  const unsigned g = (unsigned)((std::max(8LL * 32LL * d.maxblocks, (long long)nbr) + 255) / 256);
+  return (int)((std::max(8LL * 32LL * d.maxblocks, (long long)nbr) + 255) / 256);
+}
+
+void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st, bool kick) {
+  const unsigned g = (unsigned)grid_assemble(d);
   if (d.elecmethod != BM_PME) mask &= ~kAsmPme;
   if (kick) assemble_t<true>(d, mask, g, st);
   else assemble_t<false>(d, mask, g, st);
+}
+
+// order: the spline order of PME systems, 0 for the others (no gather)
+void launch_assemble_multi(const Multi &m, int order, unsigned mask, bool kick, hipStream_t st) {
+  if (order == 0) mask &= ~kAsmPme;
+#define BM_ASM(NN)                                                       \
+  if (kick) force_assemble_multi<NN, true><<<m.nwg, 256, 0, st>>>(m, mask); \
+  else force_assemble_multi<NN, false><<<m.nwg, 256, 0, st>>>(m, mask)
+  if (order == 0) {
+    BM_ASM(0);
+  } else if (order == 4) {
+    BM_ASM(4);
+  } else if (order == 6) {
+    BM_ASM(6);
+  } else {
+    BM_ASM(8);
+  }
+#undef BM_ASM
 }
 
 }  // namespace blade

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <tuple>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -703,16 +705,35 @@ int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_gra
 // ---------------------------------------------------------------------
 }  // extern "C"
 
+// One launch of a kernel type over several members: the members, the
+// workgroup prefix table and device copies of the members' Dev.
+struct MultiPlan {
+  std::vector<int> members;
+  std::vector<int> wg0;            // host copy, [n+1]
+  blade::Multi m{};
+  blade::Dev *devs = nullptr;      // device
+  int *wg0_d = nullptr;            // device
+  // the group's template keys
+  int order = 0, K0 = 0, K1 = 0, elec = 0, vdw = 0;
+  bool alch = false, soft = false;
+  size_t lds = 0;
+};
+
 struct bm_batch {
   std::vector<bm_system *> sys;
   int device = 0;
-  hipStream_t st = nullptr;
-  hipEvent_t fork = nullptr;
+  hipStream_t st = nullptr, st2 = nullptr;
+  hipEvent_t fork = nullptr, fork2 = nullptr, join2 = nullptr;
   std::vector<hipEvent_t> join;
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  // fused launches of a non-energy step
+  MultiPlan pack, normals, bonded, tail;
+  std::vector<MultiPlan> bin, spread, fftp, fftx, nb, asmb;
+  std::vector<int> fftSingle;      // PME members off the plane FFT path
   ~bm_batch() {
     if (st) (void)hipStreamSynchronize(st);
+    if (st2) (void)hipStreamSynchronize(st2);
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++) {
         if (graph[a][b]) (void)hipGraphExecDestroy(graph[a][b]);
@@ -720,10 +741,149 @@ struct bm_batch {
       }
     for (auto e : join)
       if (e) (void)hipEventDestroy(e);
-    if (fork) (void)hipEventDestroy(fork);
+    for (hipEvent_t e : {fork, fork2, join2})
+      if (e) (void)hipEventDestroy(e);
+    for (MultiPlan *p : plans()) {
+      if (p->devs) (void)hipFree(p->devs);
+      if (p->wg0_d) (void)hipFree(p->wg0_d);
+    }
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
+  std::vector<MultiPlan *> plans() {
+    std::vector<MultiPlan *> v = {&pack, &normals, &bonded, &tail};
+    for (auto *g : {&bin, &spread, &fftp, &fftx, &nb, &asmb})
+      for (MultiPlan &p : *g) v.push_back(&p);
+    return v;
+  }
+  // the members' current Dev into every plan's device copies
+  int upload_devs() {
+    for (MultiPlan *p : plans()) {
+      std::vector<blade::Dev> h;
+      for (int k : p->members) h.push_back(sys[k]->d);
+      if (hipMemcpy(p->devs, h.data(), h.size() * sizeof(blade::Dev), hipMemcpyHostToDevice) != hipSuccess)
+        return set_err(BM_EHIP, "batch: Dev upload");
+    }
+    return BM_OK;
+  }
+  // one non-energy step with one launch per kernel type and group
+  void fused_step(bool rebuild) {
+    if (rebuild) {
+      // list rebuilds stay per member, as parallel branches
+      (void)hipEventRecord(fork, st);
+      for (size_t k = 0; k < sys.size(); k++) {
+        bm_system *s = sys[k];
+        (void)hipStreamWaitEvent(s->st, fork, 0);
+        blade::launch_domdec_rebuild(s->d, s->maxBlocks, s->st);
+        (void)hipEventRecord(join[k], s->st);
+        (void)hipStreamWaitEvent(st, join[k], 0);
+      }
+    }
+    blade::launch_pack_multi(pack.m, st);
+    (void)hipEventRecord(fork2, st);
+    (void)hipStreamWaitEvent(st2, fork2, 0);
+    blade::launch_normals_multi(normals.m, st2);
+    blade::launch_bonded_multi(bonded.m, false, st2);
+    for (size_t g = 0; g < bin.size(); g++) {
+      blade::launch_pme_bin_multi(bin[g].m, bin[g].order, st2);
+      blade::launch_pme_spread_multi(spread[g].m, spread[g].order, spread[g].lds, st2);
+    }
+    for (size_t g = 0; g < fftp.size(); g++)
+      blade::launch_pme_fft_multi(fftp[g].m, fftx[g].m, fftp[g].K0, fftp[g].K1, false, st2);
+    for (int k : fftSingle) blade::launch_pme_fft(sys[k]->d, false, st2);
+    (void)hipEventRecord(join2, st2);
+    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, p.alch, p.soft, p.lds, st);
+    (void)hipStreamWaitEvent(st, join2, 0);
+    for (MultiPlan &p : asmb)
+      blade::launch_assemble_multi(p.m, p.order, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, true, st);
+    blade::launch_tail_multi(tail.m, st);
+  }
 };
+
+// members -> plan with per-member workgroup counts from grid()
+template <class F>
+static int make_plan(bm_batch *b, MultiPlan &p, const std::vector<int> &members, F grid) {
+  p.members = members;
+  p.wg0.assign(1, 0);
+  for (int k : members) p.wg0.push_back(p.wg0.back() + grid(b->sys[k]->d));
+  if (hipMalloc(&p.devs, std::max<size_t>(1, members.size()) * sizeof(blade::Dev)) != hipSuccess ||
+      hipMalloc(&p.wg0_d, p.wg0.size() * sizeof(int)) != hipSuccess ||
+      hipMemcpy(p.wg0_d, p.wg0.data(), p.wg0.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(BM_EHIP, "batch: plan tables");
+  p.m.devs = p.devs;
+  p.m.wg0 = p.wg0_d;
+  p.m.nsys = (int)members.size();
+  p.m.nwg = p.wg0.back();
+  return BM_OK;
+}
+
+// members grouped by key(), one plan per key
+template <class K, class F>
+static int make_groups(bm_batch *b, std::vector<MultiPlan> &out, const std::vector<int> &members, K key, F grid) {
+  std::vector<std::pair<decltype(key(0)), std::vector<int>>> groups;
+  for (int k : members) {
+    auto kk = key(k);
+    auto it = std::find_if(groups.begin(), groups.end(), [&](const auto &g) { return g.first == kk; });
+    if (it == groups.end()) groups.push_back({kk, {k}});
+    else it->second.push_back(k);
+  }
+  out.resize(groups.size());
+  for (size_t g = 0; g < groups.size(); g++)
+    if (int rc = make_plan(b, out[g], groups[g].second, grid)) return rc;
+  return BM_OK;
+}
+
+static int build_plans(bm_batch *b) {
+  using blade::Dev;
+  const int n = (int)b->sys.size();
+  std::vector<int> all(n), pme, plane;
+  for (int k = 0; k < n; k++) {
+    all[k] = k;
+    if (b->sys[k]->d.elecmethod == BM_PME) {
+      pme.push_back(k);
+      if (blade::fft_plane_path(b->sys[k]->d)) plane.push_back(k);
+      else b->fftSingle.push_back(k);
+    }
+  }
+  auto D = [&](int k) -> const Dev & { return b->sys[k]->d; };
+  int rc;
+  if ((rc = make_plan(b, b->pack, all, blade::grid_pack))) return rc;
+  if ((rc = make_plan(b, b->normals, all, blade::grid_normals))) return rc;
+  if ((rc = make_plan(b, b->bonded, all, blade::grid_bonded))) return rc;
+  if ((rc = make_plan(b, b->tail, all, blade::grid_tail))) return rc;
+  if ((rc = make_groups(b, b->bin, pme, [&](int k) { return D(k).order; }, blade::grid_pme_bin))) return rc;
+  if ((rc = make_groups(b, b->spread, pme, [&](int k) { return D(k).order; }, blade::grid_pme_spread))) return rc;
+  for (size_t g = 0; g < b->bin.size(); g++) {
+    b->bin[g].order = b->spread[g].order = D(b->bin[g].members[0]).order;
+    for (int k : b->spread[g].members) b->spread[g].lds = std::max(b->spread[g].lds, blade::lds_pme_spread(D(k)));
+  }
+  auto fkey = [&](int k) { return std::make_pair(D(k).K[0], D(k).K[1]); };
+  if ((rc = make_groups(b, b->fftp, plane, fkey, blade::grid_fft_planes))) return rc;
+  if ((rc = make_groups(b, b->fftx, plane, fkey, blade::grid_fft_x))) return rc;
+  for (size_t g = 0; g < b->fftp.size(); g++) {
+    b->fftp[g].K0 = D(b->fftp[g].members[0]).K[0];
+    b->fftp[g].K1 = D(b->fftp[g].members[0]).K[1];
+  }
+  auto nkey = [&](int k) {
+    const bm_system *s = b->sys[k];
+    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, s->alch, s->soft, blade::lds_nbdirect(s->d) == 0);
+  };
+  if ((rc = make_groups(b, b->nb, all, nkey, blade::grid_nbdirect))) return rc;
+  for (MultiPlan &p : b->nb) {
+    const bm_system *s = b->sys[p.members[0]];
+    p.elec = s->d.elecmethod;
+    p.vdw = s->d.vdwmethod;
+    p.alch = s->alch;
+    p.soft = s->soft;
+    p.lds = 0;
+    if (blade::lds_nbdirect(s->d))
+      for (int k : p.members) p.lds = std::max(p.lds, blade::lds_nbdirect(D(k)));
+  }
+  auto akey = [&](int k) { return D(k).elecmethod == BM_PME ? D(k).order : 0; };
+  if ((rc = make_groups(b, b->asmb, all, akey, blade::grid_assemble))) return rc;
+  for (MultiPlan &p : b->asmb) p.order = akey(p.members[0]);
+  return b->upload_devs();
+}
 
 extern "C" {
 
@@ -742,11 +902,18 @@ int bm_batch_create(int n, bm_system *const *systems, bm_batch **out) {
   b->sys.assign(systems, systems + n);
   b->join.assign(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+            hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->join2, hipEventDisableTiming) == hipSuccess;
   for (int k = 0; ok && k < n; k++) ok = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     delete b;
     return set_err(BM_EHIP, "batch stream/events");
+  }
+  if (int rc = build_plans(b)) {
+    delete b;
+    return rc;
   }
   *out = b;
   return BM_OK;
@@ -756,6 +923,7 @@ int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
   if (!b || nsteps < 0) return BM_EINVAL;
   if (freq_nrg <= 0) freq_nrg = 1 << 30;
   HIPCHK(hipSetDevice(b->device));
+  if (int rc = b->upload_devs()) return rc;
   // every member's step counter, ordered before the first graph launch;
   // each member's own work is joined into the batch stream by the graph
   for (bm_system *s : b->sys) {
@@ -769,13 +937,19 @@ int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
     const int rb = (step % freqDomdec) == 0, en = (step % freq_nrg) == 0;
     if (!b->graph[rb][en]) {
       HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-      HIPCHK(hipEventRecord(b->fork, b->st));
-      for (size_t m = 0; m < b->sys.size(); m++) {
-        bm_system *s = b->sys[m];
-        HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
-        s->step_launches(rb, en);
-        HIPCHK(hipEventRecord(b->join[m], s->st));
-        HIPCHK(hipStreamWaitEvent(b->st, b->join[m], 0));
+      if (!en) {
+        // non-energy steps: one launch per kernel type over all members
+        b->fused_step(rb);
+      } else {
+        // energy steps: every member's own step as a parallel branch
+        HIPCHK(hipEventRecord(b->fork, b->st));
+        for (size_t m = 0; m < b->sys.size(); m++) {
+          bm_system *s = b->sys[m];
+          HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
+          s->step_launches(rb, en);
+          HIPCHK(hipEventRecord(b->join[m], s->st));
+          HIPCHK(hipStreamWaitEvent(b->st, b->join[m], 0));
+        }
       }
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamEndCapture(b->st, &g));

@@ -65,13 +65,21 @@ __device__ double2 normal_pair(unsigned long long seed, unsigned long long step,
   return make_double2(rad * cs, rad * sn);
 }
 
-__global__ void upd_normals(Dev d) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void upd_normals_body(const Dev &d, int bid) {
+  const int p = bid * blockDim.x + threadIdx.x;
   const int N = 3 * d.n + d.nl;
   if (2 * p >= N) return;
   const double2 z = normal_pair(d.seed, (unsigned long long)(*d.step), (unsigned long long)p);
   d.xi[2 * p] = z.x;
   if (2 * p + 1 < N) d.xi[2 * p + 1] = z.y;
+}
+
+__global__ void __launch_bounds__(256) upd_normals(Dev d) { upd_normals_body(d, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) upd_normals_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  upd_normals_body(d, b.bid);
 }
 
 // dU/dtheta of block b from the fixed-point dU/dlambda (msld.cu:937-975)
@@ -589,8 +597,8 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
 // and half drift, constraint); the unconstrained atoms (O and half drift);
 // and, in the last workgroup, the theta DOFs (kick, O, half drift), then
 // theta -> lambda and the step counter.
-__global__ void __launch_bounds__(64) upd_tail(Dev d) {
-  if (blockIdx.x == gridDim.x - 1) {
+__device__ __forceinline__ void upd_tail_body(const Dev &d, int bid, int nblk) {
+  if (bid == nblk - 1) {
     if (d.nl > 1) {
       for (int b = threadIdx.x; b < d.nl; b += blockDim.x) {
         theta_kick(d, b);
@@ -603,7 +611,7 @@ __global__ void __launch_bounds__(64) upd_tail(Dev d) {
     if (threadIdx.x == 0) *d.step += 1;
     return;
   }
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bid * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
   if (t < 4 * ng) {
     constrain_group<true>(d, t >> 2, t & 3);
@@ -617,16 +625,29 @@ __global__ void __launch_bounds__(64) upd_tail(Dev d) {
   }
 }
 
-void launch_update_after_kick(const Dev &d, hipStream_t st) {
-  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
-  // the groups and free atoms, then one workgroup for the theta DOFs
-  upd_tail<<<(4 * ng + d.nfree + 63) / 64 + 1, 64, 0, st>>>(d);
+__global__ void __launch_bounds__(64) upd_tail(Dev d) { upd_tail_body(d, blockIdx.x, gridDim.x); }
+
+__global__ void __launch_bounds__(64) upd_tail_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  upd_tail_body(d, b.bid, b.nblk);
 }
 
-void launch_normals(const Dev &d, hipStream_t st) {
-  const int pairs = (3 * d.n + d.nl + 1) / 2;
-  upd_normals<<<(pairs + 255) / 256, 256, 0, st>>>(d);
+// the groups and free atoms, then one workgroup for the theta DOFs
+int grid_tail(const Dev &d) {
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  return (4 * ng + d.nfree + 63) / 64 + 1;
 }
+
+int grid_normals(const Dev &d) { return ((3 * d.n + d.nl + 1) / 2 + 255) / 256; }
+
+void launch_update_after_kick(const Dev &d, hipStream_t st) { upd_tail<<<grid_tail(d), 64, 0, st>>>(d); }
+
+void launch_normals(const Dev &d, hipStream_t st) { upd_normals<<<grid_normals(d), 256, 0, st>>>(d); }
+
+void launch_tail_multi(const Multi &m, hipStream_t st) { upd_tail_multi<<<m.nwg, 64, 0, st>>>(m); }
+
+void launch_normals_multi(const Multi &m, hipStream_t st) { upd_normals_multi<<<m.nwg, 256, 0, st>>>(m); }
 
 __global__ void upd_set_step(Dev d, long long v) { *d.step = v; }
 
