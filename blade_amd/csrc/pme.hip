@@ -119,9 +119,12 @@ __global__ void __launch_bounds__(256) pme_bin_multi(Multi m) {
   pme_bin_body<N>(d, b.bid, b.nblk);
 }
 
-// candidate atoms staged per round (LDS per workgroup stays near 60 KB, so
-// two bricks share a CU)
+// candidate atoms staged per round: LDS per workgroup stays near 60 KB, so
+// two bricks share a CU (a single system's 64^3 grid has two per CU); the
+// batched launch, with many more bricks, stages half as many per round so
+// four bricks share a CU
 template <int N> constexpr int spread_stage() { return N <= 6 ? 256 : 128; }
+template <int N> constexpr int spread_stage_multi() { return N <= 6 ? 128 : 64; }
 
 // v * 2^32 rounded to the nearest integer (ties to even), as
 // __double2ll_rn(v * 2^32) gives, for |v * 2^32| < 2^51: the sum with
@@ -184,9 +187,8 @@ __device__ __forceinline__ int stencil_clip(int r, int sz, int K) {
 // time; its z points are added as 64-bit fixed point in LDS.  The brick is
 // stored to the real grid with plain stores: no global atomics, and no
 // clearing pass.
-template <int N>
+template <int N, int kStage>
 __device__ __forceinline__ void pme_spread_body(const Dev &d, int bid, int nblk) {
-  constexpr int kStage = spread_stage<N>();
   extern __shared__ long long acc[];
   __shared__ double s_w[3][N][kStage];
   __shared__ int s_clip[3][kStage];
@@ -304,13 +306,15 @@ __device__ __forceinline__ void pme_spread_body(const Dev &d, int bid, int nblk)
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) pme_spread(Dev d) { pme_spread_body<N>(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) pme_spread(Dev d) {
+  pme_spread_body<N, spread_stage<N>()>(d, blockIdx.x, gridDim.x);
+}
 
 template <int N>
 __global__ void __launch_bounds__(256) pme_spread_multi(Multi m) {
   const MultiBlk b = multi_block(m);
   const Dev d = m.devs[b.sys];
-  pme_spread_body<N>(d, b.bid, b.nblk);
+  pme_spread_body<N, spread_stage_multi<N>()>(d, b.bid, b.nblk);
 }
 
 // ---------------------------------------------------------------------

@@ -788,6 +788,8 @@ struct bm_batch {
       blade::launch_pme_bin_multi(bin[g].m, bin[g].order, st2);
       blade::launch_pme_spread_multi(spread[g].m, spread[g].order, spread[g].lds, st2);
     }
+    // FFT groups in turn on st2: they finish well inside nbdirect's window,
+    // and more streams would share the four hardware queues with it
     for (size_t g = 0; g < fftp.size(); g++)
       blade::launch_pme_fft_multi(fftp[g].m, fftx[g].m, fftp[g].K0, fftp[g].K1, false, st2);
     for (int k : fftSingle) blade::launch_pme_fft(sys[k]->d, false, st2);
