@@ -1016,10 +1016,12 @@ void launch_pme_spread_multi(const Multi &m, int order, size_t lds, hipStream_t 
 }
 
 // systems of one grid size on the plane path: mp launches K0 workgroups per
-// system, mx grid_fft_x() per system
+// system, mx grid_fft_x() per system.  The plane kernels run 256 threads
+// here (the bodies loop over blockDim), so they fit on CUs beside the
+// nonbonded kernel of the same step.
 void launch_pme_fft_multi(const Multi &mp, const Multi &mx, int K0, int K1, bool energy, hipStream_t st) {
   switch (K1) {
-#define BM_YZF(SZ) case SZ: fft_yz_fwd_multi<SZ, SZ><<<mp.nwg, 1024, 0, st>>>(mp); break;
+#define BM_YZF(SZ) case SZ: fft_yz_fwd_multi<SZ, SZ><<<mp.nwg, 256, 0, st>>>(mp); break;
     BM_FFT_SIZES(BM_YZF)
 #undef BM_YZF
   }
@@ -1029,7 +1031,7 @@ void launch_pme_fft_multi(const Multi &mp, const Multi &mx, int K0, int K1, bool
 #undef BM_XC
   }
   switch (K1) {
-#define BM_YZI(SZ) case SZ: fft_yz_inv_multi<SZ, SZ><<<mp.nwg, 1024, 0, st>>>(mp); break;
+#define BM_YZI(SZ) case SZ: fft_yz_inv_multi<SZ, SZ><<<mp.nwg, 256, 0, st>>>(mp); break;
     BM_FFT_SIZES(BM_YZI)
 #undef BM_YZI
   }

@@ -781,6 +781,9 @@ struct bm_batch {
     }
     blade::launch_pack_multi(pack.m, st);
     (void)hipEventRecord(fork2, st);
+    // nbdirect first: the largest kernel takes the CUs before the second
+    // stream's smaller ones fill them
+    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, p.alch, p.soft, p.lds, st);
     (void)hipStreamWaitEvent(st2, fork2, 0);
     blade::launch_normals_multi(normals.m, st2);
     blade::launch_bonded_multi(bonded.m, false, st2);
@@ -794,7 +797,6 @@ struct bm_batch {
       blade::launch_pme_fft_multi(fftp[g].m, fftx[g].m, fftp[g].K0, fftp[g].K1, false, st2);
     for (int k : fftSingle) blade::launch_pme_fft(sys[k]->d, false, st2);
     (void)hipEventRecord(join2, st2);
-    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, p.alch, p.soft, p.lds, st);
     (void)hipStreamWaitEvent(st, join2, 0);
     for (MultiPlan &p : asmb)
       blade::launch_assemble_multi(p.m, p.order, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, true, st);
