@@ -215,10 +215,15 @@ int bm_msld_chain_rule(bm_system *sys);
 int bm_dynamics(bm_system *sys, long step0, long nsteps, int freq_nrg, int use_graph);
 
 /* ---- batches: independent systems advanced together.  One captured
- * graph per (rebuild, energy) step variant holds every member's whole step
- * as a parallel branch, so one graph launch advances all members.  Members
- * must live on one device and share the list rebuild cadence.  No
- * counterpart in the reference (it runs one system per GPU and process). */
+ * graph per (rebuild, energy) step variant advances every member: on
+ * non-energy steps one launch per kernel type covers all members (members
+ * are grouped only by spline order, PME grid size and nonbonded methods);
+ * list rebuilds and energy steps run each member's kernels as parallel
+ * branches.  Members stay bit-identical to the same systems run alone.
+ * Members must live on one device and share the list rebuild cadence; the
+ * device copies of their state descriptors are refreshed at each
+ * bm_batch_dynamics call.  No counterpart in the reference (it runs one
+ * system per GPU and process). */
 typedef struct bm_batch bm_batch;
 int bm_batch_create(int n, bm_system *const *systems, bm_batch **out);
 int bm_batch_dynamics(bm_batch *batch, long step0, long nsteps, int freq_nrg);
