@@ -67,6 +67,8 @@ def main():
                     help="instead: N independent MSLD systems (3,000-25,000 atoms, 2-6 substituents) "
                          "advanced by one graph launch per step, against the same systems run one by one")
     ap.add_argument("--state-dir", default=None, help="relaxed batch member states (npz per member)")
+    ap.add_argument("--fft", choices=("fp32", "mfma"), default="fp32",
+                    help="PME FFT form: FP32 Stockham, or DFT contractions on the matrix cores in split-BF16x3")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave out the 32-system batched figure that the default line carries")
     a = ap.parse_args()
@@ -83,6 +85,7 @@ def main():
 
     t0 = time.time()
     spec = synthetic.config2()
+    spec.fft_mode = 1 if a.fft == "mfma" else 0
     log("generated %s: %d atoms in %.1f s" % (spec.name, spec.n, time.time() - t0))
     if a.state and os.path.exists(a.state):
         z = np.load(a.state)
@@ -183,7 +186,8 @@ def main():
         "config": {"workload": "config2-msld-23558", "atoms": n, "box_A": spec.box[0],
                    "cutoff_A": spec.rcut, "switch_A": spec.rswitch, "pme_grid": list(spec.grid),
                    "pme_order": spec.order, "dt_fs": dt_fs, "lambda_sites": 1, "substituents": 3,
-                   "list_rebuild_steps": 10, "energy_every": FREQ_NRG, "graph": not a.eager},
+                   "list_rebuild_steps": 10, "energy_every": FREQ_NRG, "graph": not a.eager,
+                   "pme_fft": "fp32 Stockham" if a.fft == "fp32" else "MFMA split-BF16x3 DFT"},
         "pair_interactions_per_s": round(pairs / (ms_step * 1e-3), 1),
         "kernel_ms": {"nbdirect": round(ms_nb, 5), "pme": round(ms_pme, 5), "eager_step": round(ms_eager, 5)},
         "hbm_copy_GBps": round(copy_gbps, 1),

@@ -69,7 +69,7 @@ class RunParams(C.Structure):
         ("gamma", C.c_double), ("msld_gamma", C.c_double), ("shake_tol", C.c_double),
         ("fnex", C.c_double), ("soft_core", C.c_int32), ("soft_core14", C.c_int32),
         ("soft_bond_exp", C.c_double), ("box", C.c_double * 3), ("freq_domdec", C.c_int32),
-        ("cull_pad", C.c_double), ("seed", C.c_uint64),
+        ("cull_pad", C.c_double), ("seed", C.c_uint64), ("fft_mode", C.c_int32),
     ]
 
 
@@ -137,6 +137,7 @@ def lib():
         "blade_prepare": (i, [vp]),
         "blade_set_seed": (None, [vp, C.c_uint64]),
         "blade_set_domdec_frequency": (None, [vp, i]),
+        "blade_set_fft_mode": (None, [vp, i]),
         "blade_read_parameter_file": (i, [vp, C.c_char_p]),
         "blade_read_psf": (i, [vp, C.c_char_p]),
         "blade_read_crd": (i, [vp, C.c_char_p]),

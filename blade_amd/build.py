@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libblade_mi355x.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-HIP_SOURCES = ["domdec.hip", "nbdirect.hip", "pme.hip", "bonded.hip", "update.hip", "runtime.hip"]
+HIP_SOURCES = ["domdec.hip", "nbdirect.hip", "pme.hip", "fft_mfma.hip", "bonded.hip", "update.hip", "runtime.hip"]
 CXX_SOURCES = ["setup.cpp", "api.cpp", "files.cpp"]
 
 # gfx950 only; the packed FP32 instructions (v_pk_*_f32) are disabled: some

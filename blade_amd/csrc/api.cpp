@@ -36,6 +36,7 @@ struct System {
   long step = 0;
   unsigned long long seed = 0xB1ADE000ull;
   int freqDomdec = 10;
+  int fftMode = 0;
 };
 
 extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(stderr, "%s", message); }
@@ -251,6 +252,7 @@ void blade_add_run_dynopts(System *s, int step, int step0, int nsteps, double dt
 
 void blade_set_seed(System *s, unsigned long long seed) { s->seed = seed; }
 void blade_set_domdec_frequency(System *s, int steps) { s->freqDomdec = steps; }
+void blade_set_fft_mode(System *s, int mode) { s->fftMode = mode; }
 
 int blade_prepare(System *s) {
   if (!s || s->failed) return BM_EINVAL;
@@ -266,6 +268,7 @@ int blade_prepare(System *s) {
   }
   s->run->seed = s->seed;
   s->run->freqDomdec = s->freqDomdec;
+  s->run->fftMode = s->fftMode;
   try {
     build_topology(*s->parameters, *s->structure, m, *s->run, s->topo);
   } catch (const SetupError &e) {

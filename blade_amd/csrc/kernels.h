@@ -28,6 +28,9 @@ void launch_theta_to_lambda(const Dev &d, hipStream_t st);
 void launch_set_step(const Dev &d, long long step, hipStream_t st);
 // the FFT part of launch_pme_grid alone
 void launch_pme_fft(const Dev &d, bool energy, hipStream_t st);
+// the MFMA split-BF16x3 form of it (every grid edge fft_mfma_supported)
+bool fft_mfma_supported(int K);
+void launch_pme_fft_mfma(const Dev &d, bool energy, hipStream_t st);
 
 // ---- several systems per launch (Multi, device.h) ----------------------
 // Workgroups one system needs per kernel type; the single-system launches

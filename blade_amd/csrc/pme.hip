@@ -946,6 +946,10 @@ int grid_fft_planes(const Dev &d) { return d.K[0]; }
 int grid_fft_x(const Dev &d) { return (d.K[1] * d.K[2] + 15) / 16; }
 
 void launch_pme_fft(const Dev &d, bool energy, hipStream_t st) {
+  if (d.fft_mode == 1) {
+    launch_pme_fft_mfma(d, energy, st);
+    return;
+  }
   if (fft_plane_path(d)) {
     // plane path: forward z+y per x-plane, x with convolution, inverse y+z
     const unsigned gx = (unsigned)grid_fft_x(d);

@@ -26,6 +26,20 @@
 
 #include "kernels.h"
 
+// bf16 of a float, rounded to nearest even (finite values), and back
+static uint16_t bf16_rne(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static double bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -150,6 +164,11 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     for (int k = 0; k < 3; k++)
       if (!pme_grid_supported(rp->grid[k])) return set_err(BM_ENOTSUP, "PME grid needs factors 2,3,5");
     if (rp->order != 4 && rp->order != 6 && rp->order != 8) return set_err(BM_ENOTSUP, "PME order 4/6/8");
+    if (rp->fft_mode != 0 && rp->fft_mode != 1) return set_err(BM_EINVAL, "fft_mode 0 or 1");
+    if (rp->fft_mode == 1)
+      for (int k = 0; k < 3; k++)
+        if (!fft_mfma_supported(rp->grid[k]))
+          return set_err(BM_ENOTSUP, "the MFMA FFT needs grid edges that are multiples of 16 from 16 to 128");
   }
   HIPCHK(hipSetDevice(device));
   bm_system *s = new bm_system();
@@ -361,6 +380,40 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
         tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
       }
     UPLOAD(d.twiddle, tw);
+    d.fft_mode = rp->fft_mode;
+    if (d.fft_mode == 1) {
+      // real-form DFT matrices [[C, -sS], [sS, C]] per axis and direction,
+      // split into bf16 hi + lo from FP64, in the order the MFMA A operand
+      // reads them: fragment (row tile rt, k-step ks), lane l, element j
+      // holds M[32 rt + l%32][16 ks + 8(l/32) + j]
+      for (int ax = 0; ax < 3; ax++) {
+        const int K = d.K[ax], KS = 2 * K / 16, RT = 2 * K / 32;
+        for (int dir = 0; dir < 2; dir++) {
+          const double sgn = dir ? 1.0 : -1.0;
+          std::vector<uint16_t> hi((size_t)4 * K * K), lo((size_t)4 * K * K);
+          for (int rt = 0; rt < RT; rt++)
+            for (int ks = 0; ks < KS; ks++)
+              for (int l = 0; l < 64; l++)
+                for (int j = 0; j < 8; j++) {
+                  const int m = 32 * rt + (l & 31), kk = 16 * ks + 8 * (l >> 5) + j;
+                  const double th = 2.0 * M_PI * (double)((long long)(m % K) * (kk % K) % K) / K;
+                  const double c = std::cos(th), sn = std::sin(th);
+                  double v;
+                  if (m < K) v = kk < K ? c : -sgn * sn;
+                  else v = kk < K ? sgn * sn : c;
+                  const uint16_t h = bf16_rne((float)v);
+                  const size_t at = (((size_t)rt * KS + ks) * 64 + l) * 8 + j;
+                  hi[at] = h;
+                  lo[at] = bf16_rne((float)(v - bf16_value(h)));
+                }
+          uint16_t *ph = nullptr, *pl = nullptr;
+          UPLOAD(ph, hi);
+          UPLOAD(pl, lo);
+          d.mdft[ax][dir][0] = ph;
+          d.mdft[ax][dir][1] = pl;
+        }
+      }
+    }
     for (int k = 0; k < 3; k++) {
       // 1/|b(m)|^2 from the integer-point B-spline values (potential.cxx:1245-1287)
       std::vector<float> bm(d.K[k]);

@@ -93,6 +93,7 @@ struct RunSetup {
   int freqNPT = 0;
   unsigned long long seed = 0xB1ADE000ull;
   int freqDomdec = 10;
+  int fftMode = 0;       // PME FFT form (bm_runparams.fft_mode)
 };
 
 // Derived term tables owning their storage; view() exposes bm_topology.

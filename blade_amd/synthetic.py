@@ -90,6 +90,7 @@ class Spec:
     order: int = 6
     vdw_method: int = 0      # 0 force switch, 1 potential switch, 2 shift
     elec_method: int = 1     # 0 force switch, 1 PME, 2 force shift
+    fft_mode: int = 0        # PME FFT: 0 FP32 Stockham, 1 MFMA split-BF16x3
     shake_tol: float = 2e-7
     seed: int = 0xB1ADE000
     nsteps: int = 1000

@@ -126,6 +126,9 @@ class BladeSystem:
     def set_domdec_frequency(self, steps):
         self.L.blade_set_domdec_frequency(self.h, steps)
 
+    def set_fft_mode(self, mode):
+        self.L.blade_set_fft_mode(self.h, mode)
+
     def _read(self, fn, path):
         if fn(self.h, str(path).encode()) != 0:
             raise RuntimeError(self.L.blade_last_error().decode())
@@ -262,6 +265,7 @@ def build(spec, gpus=(0,)):
                     spec.grid_space if gx <= 0 else -1.0, gx, gy, gz, spec.order, spec.shake_tol)
     s.add_run_dynopts(0, 0, spec.nsteps, spec.dt, spec.temperature)
     s.set_seed(spec.seed)
+    s.set_fft_mode(getattr(spec, "fft_mode", 0))
     s.n = spec.n
     s.nblocks = spec.nblocks
     return s

@@ -135,6 +135,10 @@ void blade_set_seed(System *system, unsigned long long seed);
 /* steps between neighbour-list rebuilds (default 10, domdec/domdec.cu:138);
    1 rebuilds every step, for starts whose atoms outrun the thermal list pad */
 void blade_set_domdec_frequency(System *system, int steps);
+/* PME FFT form: 0 FP32 Stockham (default); 1 DFT contractions on the matrix
+   cores in split-BF16x3 with FP32 accumulation (grid edges multiples of 16
+   from 16 to 128).  No counterpart in the reference (cuFFT). */
+void blade_set_fft_mode(System *system, int mode);
 
 /* CHARMM input files, read through the setters above (the script commands
    "parameters file", "structure file psf", "coordinates file crd|pdb":

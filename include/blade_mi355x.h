@@ -156,6 +156,9 @@ typedef struct bm_runparams {
   int32_t freq_domdec;             /* list rebuild cadence (steps) */
   double cull_pad;                 /* list padding, A */
   uint64_t seed;                   /* Philox key */
+  int32_t fft_mode;                /* PME FFT: 0 FP32 Stockham; 1 DFT contractions on
+                                      MFMA in split-BF16x3 (every grid edge a multiple
+                                      of 16 from 16 to 128, else BM_ENOTSUP) */
 } bm_runparams;
 
 /* ---------------------------------------------------------------------

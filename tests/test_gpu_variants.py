@@ -67,6 +67,36 @@ def test_pme_orders_and_fft_paths(order, grid):
     _check(_variant("config1", order=order, grid=grid), "order %d grid %s" % (order, grid))
 
 
+@pytest.mark.parametrize("base,grid", [("config1", (32, 32, 32)), ("msld_small", (32, 32, 32)),
+                                       ("config2_small", (48, 48, 48))])
+def test_mfma_split_bf16_fft(base, grid):
+    """The PME FFT as DFT contractions on the matrix cores in split-BF16x3
+    (FP32 accumulation).  Forces keep the path's own bar (1e-5 relative RMS,
+    dU/dlambda 1e-3 kcal/mol).  The split products carry about 2^-17
+    relative error per term through five passes, so the reciprocal-space
+    energy is held to 1e-4 relative (the FP32 Stockham path is timed beside
+    it on the same system and its error printed for comparison)."""
+    errs = {}
+    for mode in (1, 0):
+        sp = _variant(base, grid=grid, fft_mode=mode)
+        s = system.build(sp)
+        s.run_energy()
+        top, rp = s.topology(), s.runparams()
+        t = top.contents
+        lam = (orc.theta_to_lambda(top, rp, np.array(sp.theta0, float)) if t.n_blocks > 1 else np.ones(1))
+        fo, do, eo, pairs = orc.forces(top, rp, s.positions(), lam)
+        fg, eg, dg = s.forces(), s.energy(), s.dudl()
+        errs[mode] = (rel_rms(fg, fo), abs(eg[8] - eo[8]) / abs(eo[8]), abs(eg[8] - eo[8]),
+                      abs(eg[20] - eo[20]) / abs(eo[20]),
+                      float(np.max(np.abs(dg - do))) if len(do) > 1 else 0.0)
+        print("\n[%s fft %s %s] force relRMS %.3e recip rel %.3e (%.3e kcal/mol) potential rel %.3e dudl %.2e"
+              % (("mfma" if mode else "fp32", base, grid) + errs[mode]), flush=True)
+    frms, rrel, _, _, ddl = errs[1]
+    assert frms <= FORCE_TOL
+    assert rrel <= 1e-4
+    assert ddl <= DUDL_TOL
+
+
 def test_per_term_calls_sum_to_full_force():
     sp = spec("msld_small")
     s = system.build(sp)
