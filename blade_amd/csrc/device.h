@@ -110,7 +110,7 @@ struct Dev {
   double *bmodd[3];              // FP64 moduli (energy sum)
   double pmefacd, pmeexpd;
   int fft_mode;                  // 0 FP32 Stockham, 1 MFMA split-BF16x3 DFT
-  const void *mdft[3][2][2];     // per axis, forward/inverse, hi/lo: bf16 A fragments (fft_mfma)
+  const void *mdft[3][2];        // per axis, hi/lo: bf16 A fragments of the forward DFT matrix (fft_mfma)
   // bonded terms (float parameters, int indices)
   int nbond, nbond12, nangle, ndihe, nimpr, nnb14, nnbex;
   int2 *bond_i; int2 *bond_sb; float2 *bond_p;            // (kb, b0)

@@ -382,36 +382,32 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     UPLOAD(d.twiddle, tw);
     d.fft_mode = rp->fft_mode;
     if (d.fft_mode == 1) {
-      // real-form DFT matrices [[C, -sS], [sS, C]] per axis and direction,
-      // split into bf16 hi + lo from FP64, in the order the MFMA A operand
-      // reads them: fragment (row tile rt, k-step ks), lane l, element j
-      // holds M[32 rt + l%32][16 ks + 8(l/32) + j]
+      // the real-form forward DFT matrix [[C, S], [-S, C]] per axis, split
+      // into bf16 hi + lo from FP64, in the order the MFMA A operand reads
+      // it: fragment (row tile rt, k-step ks), lane l, element j holds
+      // M[32 rt + l%32][16 ks + 8(l/32) + j]; the kernels negate the
+      // off-diagonal blocks for the inverse
       for (int ax = 0; ax < 3; ax++) {
         const int K = d.K[ax], KS = 2 * K / 16, RT = 2 * K / 32;
-        for (int dir = 0; dir < 2; dir++) {
-          const double sgn = dir ? 1.0 : -1.0;
-          std::vector<uint16_t> hi((size_t)4 * K * K), lo((size_t)4 * K * K);
-          for (int rt = 0; rt < RT; rt++)
-            for (int ks = 0; ks < KS; ks++)
-              for (int l = 0; l < 64; l++)
-                for (int j = 0; j < 8; j++) {
-                  const int m = 32 * rt + (l & 31), kk = 16 * ks + 8 * (l >> 5) + j;
-                  const double th = 2.0 * M_PI * (double)((long long)(m % K) * (kk % K) % K) / K;
-                  const double c = std::cos(th), sn = std::sin(th);
-                  double v;
-                  if (m < K) v = kk < K ? c : -sgn * sn;
-                  else v = kk < K ? sgn * sn : c;
-                  const uint16_t h = bf16_rne((float)v);
-                  const size_t at = (((size_t)rt * KS + ks) * 64 + l) * 8 + j;
-                  hi[at] = h;
-                  lo[at] = bf16_rne((float)(v - bf16_value(h)));
-                }
-          uint16_t *ph = nullptr, *pl = nullptr;
-          UPLOAD(ph, hi);
-          UPLOAD(pl, lo);
-          d.mdft[ax][dir][0] = ph;
-          d.mdft[ax][dir][1] = pl;
-        }
+        std::vector<uint16_t> hi((size_t)4 * K * K), lo((size_t)4 * K * K);
+        for (int rt = 0; rt < RT; rt++)
+          for (int ks = 0; ks < KS; ks++)
+            for (int l = 0; l < 64; l++)
+              for (int j = 0; j < 8; j++) {
+                const int m = 32 * rt + (l & 31), kk = 16 * ks + 8 * (l >> 5) + j;
+                const double th = 2.0 * M_PI * (double)((long long)(m % K) * (kk % K) % K) / K;
+                const double c = std::cos(th), sn = std::sin(th);
+                const double v = (m < K) == (kk < K) ? c : (m < K ? sn : -sn);
+                const uint16_t h = bf16_rne((float)v);
+                const size_t at = (((size_t)rt * KS + ks) * 64 + l) * 8 + j;
+                hi[at] = h;
+                lo[at] = bf16_rne((float)(v - bf16_value(h)));
+              }
+        uint16_t *ph = nullptr, *pl = nullptr;
+        UPLOAD(ph, hi);
+        UPLOAD(pl, lo);
+        d.mdft[ax][0] = ph;
+        d.mdft[ax][1] = pl;
       }
     }
     for (int k = 0; k < 3; k++) {
@@ -783,7 +779,7 @@ struct bm_batch {
   // fused launches of a non-energy step
   MultiPlan pack, normals, bonded, tail;
   std::vector<MultiPlan> bin, spread, fftp, fftx, nb, asmb;
-  std::vector<int> fftSingle;      // PME members off the plane FFT path
+  std::vector<int> fftSingle;      // PME members off the FP32 plane FFT path (other sizes, MFMA FFT)
   ~bm_batch() {
     if (st) (void)hipStreamSynchronize(st);
     if (st2) (void)hipStreamSynchronize(st2);
@@ -898,7 +894,7 @@ static int build_plans(bm_batch *b) {
     all[k] = k;
     if (b->sys[k]->d.elecmethod == BM_PME) {
       pme.push_back(k);
-      if (blade::fft_plane_path(b->sys[k]->d)) plane.push_back(k);
+      if (b->sys[k]->d.fft_mode == 0 && blade::fft_plane_path(b->sys[k]->d)) plane.push_back(k);
       else b->fftSingle.push_back(k);
     }
   }
