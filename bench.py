@@ -31,6 +31,7 @@ PEAK_FP32 = 157.3e12          # MI355X vector FP32, FLOP/s
 PEAK_HBM = 8.0e12
 FREQ_NRG = 1000
 BATCH_STEPS, BATCH_WARMUP = 300, 30   # the default line's batch figure
+PREC_STEPS, PREC_WARMUP = 1000, 100   # the default line's precision-form figure
 CAL_BYTES = 256 << 20
 
 
@@ -69,6 +70,10 @@ def main():
     ap.add_argument("--state-dir", default=None, help="relaxed batch member states (npz per member)")
     ap.add_argument("--fft", choices=("fp32", "mfma"), default="fp32",
                     help="PME FFT form: FP32 Stockham, or DFT contractions on the matrix cores in split-BF16x3")
+    ap.add_argument("--precision", action="store_true",
+                    help="print only the precision-form line (92,224 atoms, PME 96^3 on the MFMA FFT)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="leave the precision-form figure out of the default line")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave out the 32-system batched figure that the default line carries")
     a = ap.parse_args()
@@ -82,6 +87,9 @@ def main():
     from blade_amd._ffi import check, ptr
     if a.batch:
         return batch_main(a)
+    if a.precision:
+        print(json.dumps(precision_run(a.steps, a.warmup)), flush=True)
+        return 0
 
     t0 = time.time()
     spec = synthetic.config2()
@@ -213,8 +221,72 @@ def main():
                                 "config": bl["config"]}
         except Exception as e:   # the main figure stands; say why the batch one is missing
             line["batch_32"] = {"error": str(e)[:300]}
+    if not a.no_precision:
+        try:
+            line["precision_92224"] = precision_run(PREC_STEPS, PREC_WARMUP)
+        except Exception as e:
+            line["precision_92224"] = {"error": str(e)[:300]}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _timed_ms(s, steps, warmup):
+    """ms per step of `steps` graph-replayed steps after `warmup` more"""
+    from blade_amd._ffi import check
+    L, dev = s.L, s.device()
+    check(L.bm_dynamics(dev, 1, warmup, FREQ_NRG, 1), "warmup")
+    check(L.bm_sync(dev), "sync")
+    t1 = time.perf_counter()
+    check(L.bm_dynamics(dev, 1 + warmup, steps, FREQ_NRG, 1), "timed steps")
+    check(L.bm_sync(dev), "sync")
+    return (time.perf_counter() - t1) * 1e3 / steps
+
+
+def precision_run(steps, warmup):
+    """The precision form: the 92,224-atom box (two lambda sites x 4
+    substituents, PME 96^3 order 6) with the PME FFT as split-BF16x3 DFT
+    contractions on the matrix cores, GPU-relaxed, timed like the main
+    figure; the same state with the FP32 FFT beside it; and the errors of
+    forces, reciprocal and total energy and dU/dlambda against the FP64
+    oracle at the relaxed state."""
+    import copy
+    from blade_amd import synthetic, system
+    from oracle import oracle as orc
+    t0 = time.time()
+    spec = synthetic.config3()
+    log("generated %s: %d atoms in %.1f s" % (spec.name, spec.n, time.time() - t0))
+    t0 = time.time()
+    spec = system.relax(spec)
+    log("relaxed in %.1f s" % (time.time() - t0))
+    dt_fs = spec.dt / synthetic.PICOSECOND * 1000.0
+    out = {}
+    for mode in (1, 0):
+        sp = copy.copy(spec)
+        sp.fft_mode = mode
+        s = system.build(sp)
+        s.run_energy()
+        if mode == 1:
+            top, rp = s.topology(), s.runparams()
+            lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+            fo, do, eo, pairs = orc.forces(top, rp, s.positions(), lam)
+            fg, eg, dg = s.forces(), s.energy(), s.dudl()
+            err = {"force_rel_rms": float(np.sqrt(np.mean((fg - fo) ** 2)) / np.sqrt(np.mean(fo ** 2))),
+                   "recip_energy_rel": abs(eg[8] - eo[8]) / abs(eo[8]),
+                   "potential_rel": abs(eg[20] - eo[20]) / abs(eo[20]),
+                   "dudl_max_abs_kcal": float(np.max(np.abs(dg - do)))}
+        ms = _timed_ms(s, steps, warmup)
+        s.close()
+        out[mode] = ms
+    ms = out[1]
+    return {"metric": "ns/day at 2 fs (92,224-atom precision-form box, MFMA FFT)",
+            "value": round((1e3 / ms) * dt_fs * 86400e-6, 2), "unit": "ns/day", "ms_per_step": round(ms, 5),
+            "steps": steps, "warmup": warmup,
+            "fp32_fft_value": round((1e3 / out[0]) * dt_fs * 86400e-6, 2),
+            "errors_vs_fp64_oracle": {k: float("%.3e" % v) for k, v in err.items()},
+            "config": {"workload": spec.name, "atoms": spec.n, "box_A": list(spec.box), "lambda_sites": 2,
+                       "substituents": [4, 4], "pme_grid": list(spec.grid), "pme_order": spec.order,
+                       "pme_fft": "MFMA split-BF16x3 DFT", "cutoff_A": spec.rcut, "switch_A": spec.rswitch,
+                       "dt_fs": dt_fs}}
 
 
 def _member(args):

@@ -13,8 +13,16 @@ Workloads (sizes as named in the benchmark descriptions):
                 2,489-atom synthetic C/N/O/H polymer carrying one lambda
                 site of 3 substituents x 8 atoms; switch 9-10 A, PME 64^3
                 order 6, fnex 5
+  config3       92,224 atoms in a 96 x 96 x 100 A box: about 29,000 TIP3P
+                waters, 78 Na+/Cl- pairs (150 mM), 16 seeded synthetic chains
+                of 1.5-3 kDa, two of them carrying a lambda site of 4
+                substituents x 8 atoms; switch 9-10 A, PME 96^3 order 6 with
+                the split-BF16x3 MFMA FFT
   msld_small    a few-hundred-atom water box with a 3-substituent site
                 (fast parity cases)
+  two_site_small  a thousand-atom box with two chains, each carrying a
+                lambda site (3 and 2 substituents), and two ion pairs; only
+                dihedrals scale with lambda
 """
 import math
 from dataclasses import dataclass, field
@@ -44,6 +52,10 @@ POLY_NB = {"CT": (-0.0780, 2.0500, -0.0100, 1.9000, 1.0, 0),
            "HA": (-0.0220, 1.3200, -0.0220, 1.3200, 1.0, 0),
            "HN": (-0.0460, 0.2245, -0.0460, 0.2245, 1.0, 0),
            "HO": (-0.0460, 0.2245, -0.0460, 0.2245, 1.0, 0)}
+# CHARMM sodium and chloride
+ION_TYPES = {"SOD": 22.98977, "CLA": 35.45}
+ION_NB = {"SOD": (-0.0469, 1.41075, -0.0469, 1.41075, 1.0, 0),
+          "CLA": (-0.1500, 2.27000, -0.1500, 2.27000, 1.0, 0)}
 HEAVY_BOND = {frozenset(("CT", "CT")): 1.530, frozenset(("CT", "NT")): 1.470, frozenset(("CT", "OP")): 1.430}
 H_OF = {"CT": ("HA", 1.090), "NT": ("HN", 1.010), "OP": ("HO", 0.960)}
 
@@ -77,6 +89,7 @@ class Spec:
     var_bias: list = field(default_factory=list)     # (i, j, type, l0, k, n) 0-based blocks
     fnex: float = 5.5
     soft_core: bool = False
+    scale_terms: tuple = (1, 1, 1, 1, 1, 1)           # lambda-scaled bond, urey, angle, dihe, impr, cmap
     # run options
     dt: float = 0.002 * PICOSECOND
     temperature: float = 298.0
@@ -577,17 +590,144 @@ def _fill_water(spec, rng, nwat, grid, clearance=2.5, lattice=None):
 
 
 def _msld_site(spec, subs, fnex, theta_mass=12.0):
-    nb = 1 + len(subs)
-    spec.nblocks = nb
+    _msld_sites(spec, [subs], fnex, theta_mass)
+
+
+def _msld_sites(spec, sites, fnex, theta_mass=12.0):
+    """lambda sites, each a list of substituent atom lists: block 0 is the
+    environment, then the blocks of site 1, of site 2, ..."""
     blk = np.zeros(spec.n, dtype=np.int32)
-    for s, atoms in enumerate(subs):
-        blk[atoms] = s + 1
+    block_site = [0]
+    for si, subs in enumerate(sites):
+        for atoms in subs:
+            block_site.append(si + 1)
+            blk[atoms] = len(block_site) - 1
+    nb = len(block_site)
+    spec.nblocks = nb
     spec.atom_block = blk
-    spec.block_site = [0] + [1] * len(subs)
+    spec.block_site = block_site
     spec.theta0 = [0.0] * nb
-    spec.theta_mass = [0.0] + [theta_mass] * len(subs)
+    spec.theta_mass = [0.0] + [theta_mass] * (nb - 1)
     spec.fixed_bias = [0.0] * nb
     spec.fnex = fnex
+
+
+def _add_ions(spec, rng, npairs, grid, clearance=3.0):
+    """npairs Na+/Cl- at uniform random sites at least `clearance` from
+    everything placed so far (the water fill then keeps its own clearance
+    from them)"""
+    spec.types.update(ION_TYPES)
+    spec.nb_par.update(ION_NB)
+    L = np.array(spec.box)
+    resid = 1
+    for kind, q in [("SOD", 1.0)] * npairs + [("CLA", -1.0)] * npairs:
+        for _ in range(1000):
+            p = rng.uniform(0.0, 1.0, 3) * L
+            if grid.mindist(p) > clearance:
+                break
+        else:
+            raise RuntimeError("no room for an ion")
+        spec.atoms.append(("ION", str(resid), kind, kind, kind, q, ION_TYPES[kind]))
+        spec._pos.append(p)
+        grid.add(p)
+        resid += 1
+
+
+def _chains(spec, rng, sizes, grid, site_nsubs):
+    """seeded chains of the given atom counts, each confined to a sphere
+    (the polymer density of the 23,558-atom box) at a random centre whose
+    sphere overlaps no earlier one; chain c carries a lambda site of
+    site_nsubs[c] substituents when that entry is nonzero.  Returns the
+    sites (lists of substituent atom lists)."""
+    L = np.array(spec.box)
+    sites, placed = [], []
+    for c, n in enumerate(sizes):
+        radius = max(7.0, 28.0 * (n / 2489.0) ** (1 / 3))
+        for _ in range(10000):
+            center = rng.uniform(0.0, 1.0, 3) * L
+            ok = True
+            for c0, r0 in placed:
+                d = center - c0
+                d -= L * np.round(d / L)
+                ok = ok and float(np.linalg.norm(d)) > radius + r0
+            if ok:
+                break
+        else:
+            raise RuntimeError("no room for chain %d" % c)
+        placed.append((center, radius))
+        ns = site_nsubs[c] if c < len(site_nsubs) else 0
+        _, subs = _polymer(spec, rng, n, center, radius, grid, site_at=0.5 if ns else None,
+                           nsubs=max(ns, 1), typed_subs=True)
+        if ns:
+            sites.append(subs)
+    return sites
+
+
+def _sites_block_of(spec, sites):
+    block_of = np.zeros(spec.n, dtype=np.int32)
+    b = 0
+    for subs in sites:
+        for atoms in subs:
+            b += 1
+            block_of[atoms] = b
+    return block_of
+
+
+def config3(seed=0xB1ADE003, box=(96.0, 96.0, 100.0), n_chains=16, chain_atoms=5302,
+            n_ion_pairs=78, n_water=28922):
+    """92,224-atom precision-form box: chains of 240-480 atoms (1.5-3 kDa)
+    summing to chain_atoms, the first two carrying a lambda site of 4
+    substituents x 8 atoms, Na+/Cl- pairs at 150 mM, TIP3P water filling the
+    rest; PME 96^3 order 6 on the MFMA FFT.  The sites use the options of
+    the reference's MSLD test decks: soft-core nonbonded terms and
+    `removescaling bond urey angle impr` (only dihedrals scale with lambda),
+    so a substituent at small lambda keeps its shape."""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name="config3-precision-92224", box=tuple(float(b) for b in box))
+    spec._pos = []
+    _water_params(spec)
+    grid = _Grid(spec.box, 3.0)
+    sizes = rng.integers(240, 481, n_chains).astype(float)
+    sizes = np.floor(sizes * chain_atoms / sizes.sum()).astype(int)
+    sizes[-1] += chain_atoms - int(sizes.sum())
+    sites = _chains(spec, rng, [int(x) for x in sizes], grid, [4, 4])
+    assert spec.n == chain_atoms, (spec.n, chain_atoms)
+    _angles_dihes(spec, _sites_block_of(spec, sites))
+    _add_ions(spec, rng, n_ion_pairs, grid)
+    _fill_water(spec, rng, n_water, grid, lattice=35)
+    spec.pos = np.array(spec._pos) % np.array(spec.box)   # chain spheres may cross the faces
+    _msld_sites(spec, sites, fnex=5.0)
+    spec.soft_core = True
+    spec.scale_terms = (0, 0, 0, 1, 0, 1)
+    spec.rcut, spec.rswitch = 10.0, 9.0
+    spec.beta = 1.0 / 3.2
+    spec.grid, spec.order = (96, 96, 96), 6
+    spec.fft_mode = 1
+    spec.temperature = 298.0
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
+
+
+def two_site_small(seed=0xB1ADE0C0, box=24.0, grid=24):
+    """two 50-atom chains carrying lambda sites of 3 and 2 substituents,
+    two Na+/Cl- pairs and water: the multi-site parity case"""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name="two-site-small", box=(box,) * 3)
+    spec._pos = []
+    _water_params(spec)
+    g = _Grid(spec.box, 3.0)
+    sites = _chains(spec, rng, [50, 50], g, [3, 2])
+    _angles_dihes(spec, _sites_block_of(spec, sites))
+    _add_ions(spec, rng, 2, g)
+    _fill_water(spec, rng, 330, g, lattice=9)
+    spec.pos = np.array(spec._pos)
+    _msld_sites(spec, sites, fnex=5.0)
+    spec.scale_terms = (0, 0, 0, 1, 0, 1)   # `removescaling bond urey angle impr`
+    spec.rcut, spec.rswitch = 9.0, 8.0
+    spec.beta = 0.34
+    spec.grid, spec.order = (grid, grid, grid), 6
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
 
 
 def config2(seed=0xB1ADE002, n_water=7023, n_poly=2489, box=62.23):

@@ -107,6 +107,9 @@ class BladeSystem:
     def add_msld_flags(self, gamma, fnex, soft, soft14, ewaldtype, krest, sbr, sbe, snbe, fix):
         self.L.blade_add_msld_flags(self.h, gamma, fnex, soft, soft14, ewaldtype, krest, sbr, sbe, snbe, fix)
 
+    def add_msld_termscaling(self, bond, urey, angle, dihe, impr, cmap):
+        self.L.blade_add_msld_termscaling(self.h, bond, urey, angle, dihe, impr, cmap)
+
     def add_msld_bias(self, i, j, typ, l0, k, n):
         self.L.blade_add_msld_bias(self.h, i, j, typ, l0, k, n)
 
@@ -257,6 +260,7 @@ def build(spec, gpus=(0,)):
             s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], tv,
                                          spec.theta_mass[b], spec.fixed_bias[b], 0.0)
         s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2, 59.2, 1.0, 2.0, 1.0, 0)
+        s.add_msld_termscaling(*[int(x) for x in spec.scale_terms])
         for (i, j, t, l0, k, n) in spec.var_bias:
             s.add_msld_bias(i + 1, j + 1, t, l0, k, n)
     s.init_run()

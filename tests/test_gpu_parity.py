@@ -77,8 +77,41 @@ def test_config2_forces_energy_dudl():
     assert np.max(np.abs(dg - do)) <= DUDL_TOL
 
 
+def test_two_site_small_forces_energy_dudl():
+    """two lambda sites: blocks of different sites scale by the product of
+    their lambdas, blocks of one site do not interact"""
+    s, fo, fg, eo, eg, do, dg, frms, erel = _compare("two_site_small")
+    assert frms <= FORCE_TOL
+    assert erel <= ENERGY_TOL
+    assert np.max(np.abs(dg - do)) <= DUDL_TOL
+
+
+def test_config3_precision_form():
+    """92,224 atoms, two lambda sites x 4 substituents, PME 96^3 on the
+    split-BF16x3 MFMA FFT, from the GPU-relaxed state.  Forces and
+    dU/dlambda keep the path's bars; the reciprocal-space energy is held to
+    1e-4 relative and the total-potential error is printed."""
+    sp = relaxed("config3")
+    assert sp.fft_mode == 1
+    s = system.build(sp)
+    s.run_energy()
+    top, rp = s.topology(), s.runparams()
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    fo, do, eo, pairs = orc.forces(top, rp, s.positions(), lam)
+    fg, eg, dg = s.forces(), s.energy(), s.dudl()
+    frms = rel_rms(fg, fo)
+    rrel = abs(eg[8] - eo[8]) / abs(eo[8])
+    erel = abs(eg[20] - eo[20]) / abs(eo[20])
+    ddl = float(np.max(np.abs(dg - do)))
+    print("\n[config3] N=%d pairs=%d force relRMS %.3e recip %.6f vs %.6f rel %.3e potential %.4f vs %.4f rel %.3e "
+          "max |dU/dl| err %.2e" % (sp.n, pairs, frms, eg[8], eo[8], rrel, eg[20], eo[20], erel, ddl), flush=True)
+    assert frms <= FORCE_TOL
+    assert rrel <= 1e-4
+    assert ddl <= DUDL_TOL
+
+
 def test_domdec_bit_exact():
-    for name in ("config1", "msld_small", "config2_small"):
+    for name in ("config1", "msld_small", "config2_small", "two_site_small"):
         s = built(name)
         top, rp = s.topology(), s.runparams()
         x0 = system.soa(spec(name).pos)
@@ -147,9 +180,9 @@ def test_dynamics_constraints_and_stability():
     assert np.all(np.isfinite(x))
 
 
-def test_short_trajectory_tracks_oracle():
+@pytest.mark.parametrize("name", ["msld_small", "two_site_small"])
+def test_short_trajectory_tracks_oracle(name):
     """5 steps GPU vs FP64 oracle from the same state and Philox stream"""
-    name = "msld_small"
     sp = relaxed(name)
     s = system.build(sp)
     s.prepare()

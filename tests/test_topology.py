@@ -68,7 +68,7 @@ def _derive(sp):
     return allx, nb14, nbex
 
 
-@pytest.mark.parametrize("name", ["config1", "msld_small", "config2_small"])
+@pytest.mark.parametrize("name", ["config1", "msld_small", "config2_small", "two_site_small"])
 def test_exclusion_lists_follow_the_reference_rules(name):
     sp = spec(name)
     s = built(name)                     # owns the topology tables
@@ -81,6 +81,37 @@ def test_exclusion_lists_follow_the_reference_rules(name):
     assert got_14 == nb14
     assert got_ex == nbex
     assert len(got_14) == t.n_nb14 and len(got_ex) == t.n_nbex
+
+
+@pytest.mark.parametrize("name", ["msld_small", "two_site_small"])
+def test_bonded_scaling_follows_the_reference_rule(name):
+    """src/msld/msld.cu:507-553 with no restrained atoms or soft bonds: a
+    term of a scaled type carries the distinct blocks of its atoms in
+    descending order as (site << 16) | block codes, padded with 0; a term of
+    a type named by `removescaling` carries none.  msld_small scales every
+    type, two_site_small only dihedrals (and CMAP)."""
+    sp = spec(name)
+    s = built(name)
+    t = s.topology().contents
+    block = np.asarray(sp.atom_block)
+    site = np.asarray(sp.block_site)
+    seen_scaled = seen_unscaled = 0
+    for key, count, width, typ in (("bond", t.n_bonds12, 2, 0), ("angle", t.n_angles, 3, 2),
+                                   ("dihe", t.n_dihes, 4, 3), ("impr", t.n_imprs, 4, 4)):
+        idx = t.array(key + "_idx", count, width)
+        sb = t.array(key + "_sb", count, 2)
+        for atoms, got in zip(idx, sb):
+            bl = sorted({int(block[a]) for a in atoms if block[a]}, reverse=True)
+            if sp.scale_terms[typ] and bl:
+                want = [(int(site[b]) << 16) | b for b in bl] + [0] * (2 - len(bl))
+                seen_scaled += 1
+            else:
+                want = [0, 0]
+                seen_unscaled += bool(bl)
+            assert list(got) == want, (key, atoms, got, want)
+    assert seen_scaled > 0
+    if name == "two_site_small":
+        assert seen_unscaled > 0
 
 
 def test_nb14_parameters():
