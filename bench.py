@@ -74,6 +74,8 @@ def main():
                     help="print only the precision-form line (92,224 atoms, PME 96^3 on the MFMA FFT)")
     ap.add_argument("--no-precision", action="store_true",
                     help="leave the precision-form figure out of the default line")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default FFT form, list cadence (10) and tile chunk (16) instead of autotuning")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave out the 32-system batched figure that the default line carries")
     a = ap.parse_args()
@@ -109,6 +111,14 @@ def main():
                      theta_v=np.array(spec.theta_v))
 
     s = system.build(spec)
+    tune = None
+    if not a.no_tune:
+        # FFT form, list cadence and tile chunk timed from this state; the
+        # chosen ones build the device system below
+        t0 = time.time()
+        tune = s.autotune(300)
+        log("autotune in %.1f s: fft_mode %d, rebuild every %d steps, %d tiles per work unit (%.4f ms/step)"
+            % (time.time() - t0, tune["fft_mode"], tune["freq_domdec"], tune["tile_chunk"], tune["ms_per_step"]))
     s.run_energy()                       # device system + first list build
     dev = s.device()
     L = s.L
@@ -194,8 +204,12 @@ def main():
         "config": {"workload": "config2-msld-23558", "atoms": n, "box_A": spec.box[0],
                    "cutoff_A": spec.rcut, "switch_A": spec.rswitch, "pme_grid": list(spec.grid),
                    "pme_order": spec.order, "dt_fs": dt_fs, "lambda_sites": 1, "substituents": 3,
-                   "list_rebuild_steps": 10, "energy_every": FREQ_NRG, "graph": not a.eager,
-                   "pme_fft": "fp32 Stockham" if a.fft == "fp32" else "MFMA split-BF16x3 DFT"},
+                   "list_rebuild_steps": tune["freq_domdec"] if tune else 10, "energy_every": FREQ_NRG,
+                   "graph": not a.eager,
+                   "pme_fft": "MFMA split-BF16x3 DFT" if (tune["fft_mode"] if tune else a.fft == "mfma")
+                   else "fp32 Stockham",
+                   "tiles_per_work_unit": tune["tile_chunk"] if tune else 16},
+        "autotune": tune,
         "pair_interactions_per_s": round(pairs / (ms_step * 1e-3), 1),
         "kernel_ms": {"nbdirect": round(ms_nb, 5), "pme": round(ms_pme, 5), "eager_step": round(ms_eager, 5)},
         "hbm_copy_GBps": round(copy_gbps, 1),
@@ -213,7 +227,7 @@ def main():
         # (one launch per kernel type and step) against the same systems one
         # by one, both timed over the same steps
         try:
-            bl = batch_run(32, BATCH_STEPS, BATCH_WARMUP, a.state_dir)
+            bl = batch_run(32, BATCH_STEPS, BATCH_WARMUP, a.state_dir, autotune=not a.no_tune)
             line["batch_32"] = {"metric": bl["metric"], "value": bl["value"], "unit": "ns/day",
                                 "ms_per_step": bl["ms_per_step"], "steps": bl["steps"],
                                 "one_by_one": bl["one_by_one"]["value"],
@@ -310,11 +324,11 @@ def _relaxed(spec, path, log_fn):
 
 def batch_main(a):
     """print the batched line (--batch N)"""
-    print(json.dumps(batch_run(a.batch, a.steps, a.warmup, a.state_dir)), flush=True)
+    print(json.dumps(batch_run(a.batch, a.steps, a.warmup, a.state_dir, autotune=not a.no_tune)), flush=True)
     return 0
 
 
-def batch_run(nsys, steps, warmup, state_dir=None):
+def batch_run(nsys, steps, warmup, state_dir=None, autotune=True):
     """aggregate ns/day of nsys systems: one graph launch per step for all
     of them, against the same systems advanced one after another"""
     import concurrent.futures as cf
@@ -331,11 +345,12 @@ def batch_run(nsys, steps, warmup, state_dir=None):
     specs = [_relaxed(sp, os.path.join(state_dir, "m%02d.npz" % k) if state_dir else None, log)
              for k, sp in enumerate(specs)]
     log("relaxed in %.1f s" % (time.time() - t0))
+    dt_fs = specs[0].dt / synthetic.PICOSECOND * 1000.0
+    # one by one: the plain per-system form (default cadence 10, 16 tiles
+    # per work unit), every member's own graphs, members in turn
     systems = [system.build(sp) for sp in specs]
     for s in systems:
         s.run_energy()
-    dt_fs = specs[0].dt / synthetic.PICOSECOND * 1000.0
-    # one by one: every member's own graphs, members in turn
     for s in systems:
         check(s.L.bm_dynamics(s.device(), 1, warmup, FREQ_NRG, 1), "warmup")
         check(s.L.bm_sync(s.device()), "sync")
@@ -344,13 +359,24 @@ def batch_run(nsys, steps, warmup, state_dir=None):
         check(s.L.bm_dynamics(s.device(), 1 + warmup, steps, FREQ_NRG, 1), "plain steps")
         check(s.L.bm_sync(s.device()), "sync")
     t_plain = time.perf_counter() - t1
-    # batched: one launch per kernel type and step for every member
+    for s in systems:
+        s.close()
+    # batched: the same starting states, one launch per kernel type and step
+    # for every member, with the cadence and tile chunk timed on the batch
+    systems = [system.build(sp) for sp in specs]
+    tune = None
+    if autotune:
+        t0 = time.time()
+        tune = system.autotune_batch(systems, 150)
+        log("batch autotune in %.1f s: rebuild every %d steps, %d tiles per work unit"
+            % (time.time() - t0, tune["freq_domdec"], tune["tile_chunk"]))
+    for s in systems:
+        s.run_energy()
     b = system.Batch(systems)
-    step0 = 1 + warmup + steps
-    b.dynamics(step0, warmup, FREQ_NRG)
+    b.dynamics(1, warmup, FREQ_NRG)
     b.sync()
     t1 = time.perf_counter()
-    b.dynamics(step0 + warmup, steps, FREQ_NRG)
+    b.dynamics(1 + warmup, steps, FREQ_NRG)
     b.sync()
     t_batch = time.perf_counter() - t1
     n = len(systems)
@@ -364,7 +390,10 @@ def batch_run(nsys, steps, warmup, state_dir=None):
         "config": {"workload": "batch-%d-msld" % n, "atoms_total": sum(s.n for s in specs),
                    "atoms_range": [min(s.n for s in specs), max(s.n for s in specs)],
                    "substituents_range": [min(m[1] for m in members), max(m[1] for m in members)],
-                   "cutoff_A": 10.0, "grid_spacing_A": 1.0, "graph": True},
+                   "cutoff_A": 10.0, "grid_spacing_A": 1.0, "graph": True,
+                   "list_rebuild_steps": tune["freq_domdec"] if tune else 10,
+                   "tiles_per_work_unit": tune["tile_chunk"] if tune else 16},
+        "autotune": tune,
         "one_by_one": {"value": round(agg(t_plain), 2), "ms_per_step_all": round(t_plain * 1e3 / steps, 5)},
         "batched_over_one_by_one": round(t_plain / t_batch, 3),
     }

@@ -70,7 +70,27 @@ class RunParams(C.Structure):
         ("fnex", C.c_double), ("soft_core", C.c_int32), ("soft_core14", C.c_int32),
         ("soft_bond_exp", C.c_double), ("box", C.c_double * 3), ("freq_domdec", C.c_int32),
         ("cull_pad", C.c_double), ("seed", C.c_uint64), ("fft_mode", C.c_int32),
+        ("tile_chunk", C.c_int32),
     ]
+
+
+class TuneTrial(C.Structure):
+    """Mirror of bm_tune_trial."""
+    _fields_ = [("fft_mode", C.c_int32), ("freq_domdec", C.c_int32), ("tile_chunk", C.c_int32),
+                ("status", C.c_int32), ("ms_per_step", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    """Mirror of bm_tune_result."""
+    _fields_ = [("fft_mode", C.c_int32), ("freq_domdec", C.c_int32), ("tile_chunk", C.c_int32),
+                ("ms_per_step", C.c_double), ("n_trials", C.c_int32), ("trial", TuneTrial * 32)]
+
+    def as_dict(self):
+        return {"fft_mode": self.fft_mode, "freq_domdec": self.freq_domdec, "tile_chunk": self.tile_chunk,
+                "ms_per_step": round(self.ms_per_step, 5),
+                "trials": [{"fft_mode": t.fft_mode, "freq_domdec": t.freq_domdec, "tile_chunk": t.tile_chunk,
+                            "status": t.status, "ms_per_step": round(t.ms_per_step, 5)}
+                           for t in self.trial[:self.n_trials]]}
 
 
 _lib = None
@@ -138,6 +158,9 @@ def lib():
         "blade_set_seed": (None, [vp, C.c_uint64]),
         "blade_set_domdec_frequency": (None, [vp, i]),
         "blade_set_fft_mode": (None, [vp, i]),
+        "blade_set_tile_chunk": (None, [vp, i]),
+        "blade_autotune": (i, [vp, i, C.POINTER(TuneResult)]),
+        "blade_batch_autotune": (i, [i, C.POINTER(vp), i, C.POINTER(TuneResult)]),
         "blade_read_parameter_file": (i, [vp, C.c_char_p]),
         "blade_read_psf": (i, [vp, C.c_char_p]),
         "blade_read_crd": (i, [vp, C.c_char_p]),
@@ -160,6 +183,11 @@ def lib():
         "bm_dynamics": (i, [vp, C.c_long, C.c_long, i, i]),
         "bm_profile_steps": (i, [vp, C.c_long, C.c_long, _f64p]),
         "bm_time_nbdirect": (i, [vp, C.c_int, _f64p]),
+        "bm_autotune": (i, [i, C.POINTER(Topology), C.POINTER(RunParams), _f64p, _f64p, _f64p, _f64p, i,
+                            C.POINTER(TuneResult)]),
+        "bm_batch_autotune": (i, [i, i, C.POINTER(C.POINTER(Topology)), C.POINTER(C.POINTER(RunParams)),
+                                  C.POINTER(_f64p), C.POINTER(_f64p), C.POINTER(_f64p), C.POINTER(_f64p), i,
+                                  C.POINTER(TuneResult)]),
         "bm_calibration_copy": (i, [i, C.c_size_t, i, _f64p]),
         "bm_batch_create": (i, [i, C.POINTER(vp), C.POINTER(vp)]),
         "bm_batch_dynamics": (i, [vp, C.c_long, C.c_long, i]),

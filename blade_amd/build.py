@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libblade_mi355x.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 HIP_SOURCES = ["domdec.hip", "nbdirect.hip", "pme.hip", "fft_mfma.hip", "bonded.hip", "update.hip", "runtime.hip"]
-CXX_SOURCES = ["setup.cpp", "api.cpp", "files.cpp"]
+CXX_SOURCES = ["setup.cpp", "api.cpp", "files.cpp", "autotune.cpp"]
 
 # gfx950 only; the packed FP32 instructions (v_pk_*_f32) are disabled: some
 # of their forms have given wrong results on this hardware without an error

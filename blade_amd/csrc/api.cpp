@@ -37,6 +37,7 @@ struct System {
   unsigned long long seed = 0xB1ADE000ull;
   int freqDomdec = 10;
   int fftMode = 0;
+  int tileChunk = 0;
 };
 
 extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(stderr, "%s", message); }
@@ -253,6 +254,7 @@ void blade_add_run_dynopts(System *s, int step, int step0, int nsteps, double dt
 void blade_set_seed(System *s, unsigned long long seed) { s->seed = seed; }
 void blade_set_domdec_frequency(System *s, int steps) { s->freqDomdec = steps; }
 void blade_set_fft_mode(System *s, int mode) { s->fftMode = mode; }
+void blade_set_tile_chunk(System *s, int tiles) { s->tileChunk = tiles; }
 
 int blade_prepare(System *s) {
   if (!s || s->failed) return BM_EINVAL;
@@ -269,6 +271,7 @@ int blade_prepare(System *s) {
   s->run->seed = s->seed;
   s->run->freqDomdec = s->freqDomdec;
   s->run->fftMode = s->fftMode;
+  s->run->tileChunk = s->tileChunk;
   try {
     build_topology(*s->parameters, *s->structure, m, *s->run, s->topo);
   } catch (const SetupError &e) {
@@ -281,24 +284,94 @@ int blade_prepare(System *s) {
   return BM_OK;
 }
 
+// the coordinates and velocities as supplied, SoA
+static void soa_state(System *s, std::vector<double> &x, std::vector<double> &v) {
+  const int n = s->topo.n;
+  x.assign(3 * (size_t)n, 0.0);
+  v.assign(3 * (size_t)n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      x[k * (size_t)n + i] = s->coordinates->pos[3 * (size_t)i + k];
+      v[k * (size_t)n + i] = s->coordinates->vel[3 * (size_t)i + k];
+    }
+}
+
 // Run::dynamics_initialize (run/run.cu:572-622): derive terms, build the
 // device system from the coordinates.
 static int ensure_gpu(System *s) {
   if (s->gpu) return BM_OK;
   int rc = blade_prepare(s);
   if (rc) return rc;
-  const int n = s->topo.n;
-  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n);
-  for (int i = 0; i < n; i++)
-    for (int k = 0; k < 3; k++) {
-      x[k * (size_t)n + i] = s->coordinates->pos[3 * (size_t)i + k];
-      v[k * (size_t)n + i] = s->coordinates->vel[3 * (size_t)i + k];
-    }
+  std::vector<double> x, v;
+  soa_state(s, x, v);
   rc = bm_system_create(s->device, &s->topo.top, &s->rp, x.data(), v.data(), s->msld->theta.data(),
                         s->msld->thetaVelocity.data(), &s->gpu);
   if (rc) api_fail(s, std::string("device system: ") + bm_last_error());
   s->step = s->run->step;
   return rc;
+}
+
+int blade_autotune(System *s, int steps, bm_tune_result *result) {
+  if (!s || s->failed) return BM_EINVAL;
+  if (s->gpu) {
+    api_fail(s, "blade_autotune comes before the first energy or dynamics call");
+    return BM_EINVAL;
+  }
+  int rc = blade_prepare(s);
+  if (rc) return rc;
+  std::vector<double> x, v;
+  soa_state(s, x, v);
+  bm_tune_result r{};
+  rc = bm_autotune(s->device, &s->topo.top, &s->rp, x.data(), v.data(), s->msld->theta.data(),
+                   s->msld->thetaVelocity.data(), steps, &r);
+  if (rc) {
+    api_fail(s, std::string("autotune: ") + bm_last_error());
+    return rc;
+  }
+  s->fftMode = r.fft_mode;
+  s->freqDomdec = r.freq_domdec;
+  s->tileChunk = r.tile_chunk;
+  s->prepared = false;   // run parameters are derived again with the chosen values
+  if (result) *result = r;
+  return BM_OK;
+}
+
+int blade_batch_autotune(int n, System *const *sys, int steps, bm_tune_result *result) {
+  if (n < 1 || !sys) return BM_EINVAL;
+  std::vector<std::vector<double>> x(n), v(n);
+  std::vector<const bm_topology *> tops(n);
+  std::vector<const bm_runparams *> rps(n);
+  std::vector<const double *> px(n), pv(n), pt(n), ptv(n);
+  for (int k = 0; k < n; k++) {
+    System *s = sys[k];
+    if (!s || s->failed) return BM_EINVAL;
+    if (s->gpu) {
+      api_fail(s, "blade_batch_autotune comes before the first energy or dynamics call");
+      return BM_EINVAL;
+    }
+    if (int rc = blade_prepare(s)) return rc;
+    soa_state(s, x[k], v[k]);
+    tops[k] = &s->topo.top;
+    rps[k] = &s->rp;
+    px[k] = x[k].data();
+    pv[k] = v[k].data();
+    pt[k] = s->msld->theta.data();
+    ptv[k] = s->msld->thetaVelocity.data();
+  }
+  bm_tune_result r{};
+  int rc = bm_batch_autotune(sys[0]->device, n, tops.data(), rps.data(), px.data(), pv.data(), pt.data(),
+                             ptv.data(), steps, &r);
+  if (rc) {
+    api_fail(sys[0], std::string("batch autotune: ") + bm_last_error());
+    return rc;
+  }
+  for (int k = 0; k < n; k++) {
+    sys[k]->freqDomdec = r.freq_domdec;
+    sys[k]->tileChunk = r.tile_chunk;
+    sys[k]->prepared = false;
+  }
+  if (result) *result = r;
+  return BM_OK;
 }
 
 void blade_run_energy(System *s) {

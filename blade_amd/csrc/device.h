@@ -26,7 +26,7 @@ constexpr unsigned kAsmNb = 1u, kAsmBonded = 2u, kAsmPme = 4u;
 
 // tile list capacity helpers
 constexpr int kBlockAtoms = 32;      // atoms per block (one half-wave)
-constexpr int kTileChunk = 16;       // candidate tiles examined per wave claim
+constexpr int kTileChunk = 16;       // most directed tiles in one nonbonded work unit
 constexpr int kMaxColumnAtoms = 2048;
 
 // Device view of one system.
@@ -110,6 +110,7 @@ struct Dev {
   double *bmodd[3];              // FP64 moduli (energy sum)
   double pmefacd, pmeexpd;
   int fft_mode;                  // 0 FP32 Stockham, 1 MFMA split-BF16x3 DFT
+  int tchunk;                    // directed tiles per nonbonded work unit (<= kTileChunk)
   const void *mdft[3][2];        // per axis, hi/lo: bf16 A fragments of the forward DFT matrix (fft_mfma)
   // bonded terms (float parameters, int indices)
   int nbond, nbond12, nangle, ndihe, nimpr, nnb14, nnbex;

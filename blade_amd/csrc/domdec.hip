@@ -13,7 +13,7 @@
 // The tile list is directed: row I holds every block J (J == I included)
 // whose rebuild box is within rcut + pad, in ascending J, so each pair of
 // blocks appears as (I,J) and (J,I).  Rows are cut into chunks of at most
-// kTileChunk tiles; the nonbonded kernel owns one chunk (one i-block) at a
+// d.tchunk tiles; the nonbonded kernel owns one chunk (one i-block) at a
 // time and never writes j-side forces.
 //
 // Column counts and bucket fills walk the slots of the previous rebuild,
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
   }
   if (lane == 0) {
     d.tcount[I] = cnt;
-    d.ccount[I] = (cnt + kTileChunk - 1) / kTileChunk;
+    d.ccount[I] = (cnt + d.tchunk - 1) / d.tchunk;
   }
 }
 
@@ -287,8 +287,8 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
     base += __popcll(m);
   }
   const int t0 = d.tstart[I], cnt = d.tcount[I], c0 = d.cstart[I];
-  for (int k = lane; k * kTileChunk < cnt; k += 64)
-    d.chunk[c0 + k] = make_int4(I, t0 + k * kTileChunk, min(kTileChunk, cnt - k * kTileChunk), 0);
+  const int tc = d.tchunk;
+  for (int k = lane; k * tc < cnt; k += 64) d.chunk[c0 + k] = make_int4(I, t0 + k * tc, min(tc, cnt - k * tc), 0);
 }
 
 // tile (I,J) by binary search in row I's ascending J list, -1 if absent

@@ -1,7 +1,7 @@
 // nbdirect.hip - real-space nonbonded tile kernel
 // (restates nbdirect/nbdirect.cu:87-490 pair semantics).
 //
-// Work unit: a chunk of at most kTileChunk directed tiles (I, J) of one row
+// Work unit: a chunk of at most d.tchunk directed tiles (I, J) of one row
 // I of the rebuild list.  One wave64 owns a chunk: both 32-lane halves hold
 // the 32 i-atoms of block I in registers, so i-forces accumulate without
 // any cross-lane or memory traffic, and no j-side force is ever written

@@ -157,6 +157,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(BM_ENOGPU, "no HIP device");
+  if (rp->tile_chunk < 0 || rp->tile_chunk > kTileChunk) return set_err(BM_EINVAL, "tile_chunk 0..16");
   if (device < 0 || device >= ndev) return set_err(BM_EINVAL, "device out of range");
   for (int k = 0; k < 3; k++)
     if (!(rp->box[k] > 2 * rp->rcut)) return set_err(BM_EINVAL, "box edge must exceed 2*rcut");
@@ -224,7 +225,8 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     s->maxTiles = std::min(MBl * MBl, MBl * (long long)std::ceil(perRow));
   }
   d.maxtiles = s->maxTiles;
-  d.maxchunks = (int)(s->maxTiles / kTileChunk + s->maxBlocks + 1);
+  d.tchunk = rp->tile_chunk > 0 ? rp->tile_chunk : kTileChunk;
+  d.maxchunks = (int)(s->maxTiles / d.tchunk + s->maxBlocks + 1);
   const int MB = s->maxBlocks;
   int rc = 0;
 #define ALLOC(p, c) \

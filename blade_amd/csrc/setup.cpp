@@ -462,6 +462,7 @@ void build_runparams(const RunSetup &r, const MsldSetup &m, const Coordinates &c
   rp.soft_bond_exp = 1.0;
   rp.freq_domdec = r.freqDomdec > 0 ? r.freqDomdec : 10;
   rp.fft_mode = r.fftMode;
+  rp.tile_chunk = r.tileChunk;
   // cullPad = 2 sqrt(30 kB T) freqDomdec dt (domdec/domdec.cu:138)
   rp.cull_pad = 2 * std::sqrt(30 * kBoltz * r.T) * rp.freq_domdec * r.dt;
   rp.seed = r.seed;

@@ -94,6 +94,7 @@ struct RunSetup {
   unsigned long long seed = 0xB1ADE000ull;
   int freqDomdec = 10;
   int fftMode = 0;       // PME FFT form (bm_runparams.fft_mode)
+  int tileChunk = 0;     // directed tiles per nonbonded work unit (bm_runparams.tile_chunk)
 };
 
 // Derived term tables owning their storage; view() exposes bm_topology.

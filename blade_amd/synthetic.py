@@ -104,6 +104,8 @@ class Spec:
     vdw_method: int = 0      # 0 force switch, 1 potential switch, 2 shift
     elec_method: int = 1     # 0 force switch, 1 PME, 2 force shift
     fft_mode: int = 0        # PME FFT: 0 FP32 Stockham, 1 MFMA split-BF16x3
+    freq_domdec: int = 10    # neighbour-list rebuild cadence (steps)
+    tile_chunk: int = 0      # directed tiles per nonbonded work unit (0: 16)
     shake_tol: float = 2e-7
     seed: int = 0xB1ADE000
     nsteps: int = 1000

@@ -132,6 +132,18 @@ class BladeSystem:
     def set_fft_mode(self, mode):
         self.L.blade_set_fft_mode(self.h, mode)
 
+    def set_tile_chunk(self, tiles):
+        self.L.blade_set_tile_chunk(self.h, tiles)
+
+    def autotune(self, steps=300):
+        """time candidate FFT forms, list cadences and tile chunks from the
+        current coordinates and keep the fastest (before the first energy
+        or dynamics call); returns the chosen values and every trial"""
+        from ._ffi import TuneResult
+        r = TuneResult()
+        check(self.L.blade_autotune(self.h, int(steps), C.byref(r)), "blade_autotune")
+        return r.as_dict()
+
     def _read(self, fn, path):
         if fn(self.h, str(path).encode()) != 0:
             raise RuntimeError(self.L.blade_last_error().decode())
@@ -270,6 +282,8 @@ def build(spec, gpus=(0,)):
     s.add_run_dynopts(0, 0, spec.nsteps, spec.dt, spec.temperature)
     s.set_seed(spec.seed)
     s.set_fft_mode(getattr(spec, "fft_mode", 0))
+    s.set_domdec_frequency(getattr(spec, "freq_domdec", 10))
+    s.set_tile_chunk(getattr(spec, "tile_chunk", 0))
     s.n = spec.n
     s.nblocks = spec.nblocks
     return s
@@ -335,6 +349,18 @@ class Batch:
         if self.h:
             self.L.bm_batch_destroy(self.h)
             self.h = None
+
+
+def autotune_batch(systems, steps=150):
+    """Time shared list cadences and tile chunks for BladeSystems that will
+    run together as a Batch (before their first energy or dynamics call)
+    and set the fastest on all of them; returns the choice and every trial."""
+    from ._ffi import TuneResult
+    L = systems[0].L
+    hs = (C.c_void_p * len(systems))(*[s.h for s in systems])
+    r = TuneResult()
+    check(L.blade_batch_autotune(len(systems), hs, int(steps), C.byref(r)), "blade_batch_autotune")
+    return r.as_dict()
 
 
 def state(s, n, nblocks):

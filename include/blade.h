@@ -135,6 +135,17 @@ void blade_set_seed(System *system, unsigned long long seed);
 /* steps between neighbour-list rebuilds (default 10, domdec/domdec.cu:138);
    1 rebuilds every step, for starts whose atoms outrun the thermal list pad */
 void blade_set_domdec_frequency(System *system, int steps);
+/* directed tiles per nonbonded work unit, 1..16 (default 16); no
+   counterpart in the reference */
+void blade_set_tile_chunk(System *system, int tiles);
+/* Time candidate FFT forms, list rebuild cadences and tile chunks from the
+   current coordinates (bm_autotune) and keep the fastest for the runs that
+   follow.  Call after setup and before the first energy or dynamics call;
+   `result` may be NULL.  No counterpart in the reference. */
+int blade_autotune(System *system, int steps, struct bm_tune_result *result);
+/* The same for n systems to be advanced together (bm_batch_autotune): the
+   cadence and tile chunk chosen for the batch are set on every system. */
+int blade_batch_autotune(int n, System *const *systems, int steps, struct bm_tune_result *result);
 /* PME FFT form: 0 FP32 Stockham (default); 1 DFT contractions on the matrix
    cores in split-BF16x3 with FP32 accumulation (grid edges multiples of 16
    from 16 to 128).  No counterpart in the reference (cuFFT). */

@@ -159,6 +159,8 @@ typedef struct bm_runparams {
   int32_t fft_mode;                /* PME FFT: 0 FP32 Stockham; 1 DFT contractions on
                                       MFMA in split-BF16x3 (every grid edge a multiple
                                       of 16 from 16 to 128, else BM_ENOTSUP) */
+  int32_t tile_chunk;              /* directed tiles per nonbonded work unit (one
+                                      wave), 1..16; 0 => 16 */
 } bm_runparams;
 
 /* ---------------------------------------------------------------------
@@ -255,6 +257,43 @@ int bm_get_domdec_sizes(bm_system *sys, int *n_blocks, int *n_tiles, int *n_excl
 int bm_get_domdec(bm_system *sys, int *local_to_global /*[32*nb], -1 = empty*/,
                   int *block_count /*[nb]*/, int *tile_i /*[nt]*/, int *tile_j /*[nt]*/,
                   unsigned *tile_mask /*[nt*32]*/);
+
+/* ---- autotuner (no counterpart in the reference, which fixes these) --
+ * Times candidate step configurations from one starting state and reports
+ * the fastest: the PME FFT form (fft_mode 0, and 1 where every grid edge
+ * allows it), the list rebuild cadence (freq_domdec 5, 10, 15, 20; the
+ * list pad grows with it as 2 sqrt(30 kB T) freq dt, domdec/domdec.cu:138)
+ * and the tiles per nonbonded work unit (tile_chunk 4, 8, 16), by
+ * coordinate descent in that order from the values in `rp`.  Each
+ * candidate is a fresh device system advanced `steps` graph-replayed
+ * steps (rounded up to whole rebuild periods) after a warmup that captures
+ * its graphs, timed on the host around synchronised launches; no energy
+ * steps.  The caller's state is only read.  A candidate that fails (no
+ * room, a NaN) is reported with its error code and never chosen. */
+#define BM_TUNE_MAX_TRIALS 32
+typedef struct bm_tune_trial {
+  int32_t fft_mode, freq_domdec, tile_chunk, status;
+  double ms_per_step;
+} bm_tune_trial;
+typedef struct bm_tune_result {
+  int32_t fft_mode, freq_domdec, tile_chunk;    /* the fastest candidate */
+  double ms_per_step;
+  int32_t n_trials;
+  bm_tune_trial trial[BM_TUNE_MAX_TRIALS];
+} bm_tune_result;
+int bm_autotune(int device, const bm_topology *top, const bm_runparams *rp,
+                const double *pos, const double *vel,
+                const double *theta, const double *theta_vel,
+                int steps, bm_tune_result *out);
+/* The same for n systems advanced together (bm_batch_dynamics): the list
+ * cadence (10, 15, 20) and tile chunk (16, 8, 4) shared by every member,
+ * tile chunk first; each member keeps its FFT form (fft_mode is reported
+ * as -1).  Each candidate is a fresh batch of fresh device systems. */
+int bm_batch_autotune(int device, int n, const bm_topology *const *tops,
+                      const bm_runparams *const *rps,
+                      const double *const *pos, const double *const *vel,
+                      const double *const *theta, const double *const *theta_vel,
+                      int steps, bm_tune_result *out);
 
 /* ---- timing of the hot kernels over the last bm_dynamics call */
 int bm_get_kernel_times(bm_system *sys, double *ms_nbdirect, double *ms_pme,

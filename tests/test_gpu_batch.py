@@ -5,6 +5,8 @@ kernel type over all members (non-energy steps, with and without a list
 rebuild) or every member's own step as a parallel graph branch (energy
 steps); either way the schedule changes, never the arithmetic.  Energies
 every 7 steps and rebuilds every 10 cover all four step variants."""
+import copy
+
 import numpy as np
 import pytest
 
@@ -22,7 +24,10 @@ def _members():
 
 
 def test_batch_members_equal_solo_runs():
-    specs = _members()
+    """members with different tile chunks (4, default 16, 8): each keeps its
+    own in the batch"""
+    specs = [copy.copy(sp) for sp in _members()]
+    specs[0].tile_chunk, specs[2].tile_chunk = 4, 8
     solo = []
     for sp in specs:
         s = system.build(sp)
@@ -46,6 +51,31 @@ def test_batch_members_equal_solo_runs():
         assert np.all(np.isfinite(x))
         assert np.array_equal(x, xs) and np.array_equal(v, vs)
         assert np.array_equal(th, ths)
+    b.close()
+    for s in systems:
+        s.close()
+
+
+def test_batch_autotune_sets_a_measured_choice():
+    """the batch tuner's cadence and tile chunk are the fastest of its
+    trials, every trial ran, and every member's run parameters carry them"""
+    specs = _members()
+    systems = [system.build(sp) for sp in specs]
+    r = system.autotune_batch(systems, 30)
+    print("\n[batch autotune] %s" % r, flush=True)
+    trials = r["trials"]
+    assert len(trials) >= 4
+    assert all(t["status"] == 0 and t["ms_per_step"] > 0 for t in trials)
+    assert r["ms_per_step"] == min(t["ms_per_step"] for t in trials)
+    for s in systems:
+        s.run_energy()
+        rp = s.runparams().contents
+        assert (rp.freq_domdec, rp.tile_chunk) == (r["freq_domdec"], r["tile_chunk"])
+    b = system.Batch(systems)
+    b.dynamics(1, 2 * r["freq_domdec"], FREQ_NRG)
+    b.sync()
+    for sp, s in zip(specs, systems):
+        assert np.all(np.isfinite(system.state(s, sp.n, sp.nblocks)[0]))
     b.close()
     for s in systems:
         s.close()

@@ -97,6 +97,36 @@ def test_mfma_split_bf16_fft(base, grid):
     assert ddl <= DUDL_TOL
 
 
+@pytest.mark.parametrize("chunk", [1, 4, 8])
+def test_tile_chunk_shapes(chunk):
+    """directed tiles per nonbonded work unit below the default 16: the FP32
+    partial sums group differently, the bars are the same"""
+    _check(_variant("msld_small", tile_chunk=chunk), "tile chunk %d" % chunk)
+
+
+def test_autotune_picks_the_fastest_measured_candidate():
+    """the tuner's choice is the fastest of its trials, every trial ran, the
+    tuned system keeps the parity bars and its run parameters carry the
+    choice"""
+    sp = _variant("msld_small", grid=(32, 32, 32))
+    s = system.build(sp)
+    r = s.autotune(40)
+    print("\n[autotune] %s" % r, flush=True)
+    trials = r["trials"]
+    assert len(trials) >= 7                      # 2 FFT forms, 4 cadences, 3 chunks, shared base
+    assert all(t["status"] == 0 and t["ms_per_step"] > 0 for t in trials)
+    assert {t["fft_mode"] for t in trials} == {0, 1}
+    assert r["ms_per_step"] == min(t["ms_per_step"] for t in trials)
+    s.run_energy()
+    rp = s.runparams().contents
+    assert (rp.fft_mode, rp.freq_domdec, rp.tile_chunk) == (r["fft_mode"], r["freq_domdec"], r["tile_chunk"])
+    top = s.topology()
+    lam = orc.theta_to_lambda(top, s.runparams(), np.array(sp.theta0, float))
+    fo, do, eo, pairs = orc.forces(top, s.runparams(), s.positions(), lam)
+    assert rel_rms(s.forces(), fo) <= FORCE_TOL
+    assert float(np.max(np.abs(s.dudl() - do))) <= DUDL_TOL
+
+
 def test_per_term_calls_sum_to_full_force():
     sp = spec("msld_small")
     s = system.build(sp)
