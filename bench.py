@@ -72,6 +72,10 @@ def main():
                     help="PME FFT form: FP32 Stockham, or DFT contractions on the matrix cores in split-BF16x3")
     ap.add_argument("--precision", action="store_true",
                     help="print only the precision-form line (92,224 atoms, PME 96^3 on the MFMA FFT)")
+    ap.add_argument("--prec-state", default=None,
+                    help="relaxed precision-form state (npz): loaded if present, else written after relaxing")
+    ap.add_argument("--prec-fft", choices=("both", "mfma", "fp32"), default="both",
+                    help="FFT forms timed in the precision-form run (errors are measured on the MFMA one)")
     ap.add_argument("--no-precision", action="store_true",
                     help="leave the precision-form figure out of the default line")
     ap.add_argument("--no-tune", action="store_true",
@@ -90,7 +94,9 @@ def main():
     if a.batch:
         return batch_main(a)
     if a.precision:
-        print(json.dumps(precision_run(a.steps, a.warmup)), flush=True)
+        print(json.dumps(precision_run(a.steps, a.warmup, graph=not a.eager, state=a.prec_state,
+                                       modes={"both": (1, 0), "mfma": (1,), "fp32": (0,)}[a.prec_fft])),
+              flush=True)
         return 0
 
     t0 = time.time()
@@ -244,19 +250,20 @@ def main():
     return 0
 
 
-def _timed_ms(s, steps, warmup):
-    """ms per step of `steps` graph-replayed steps after `warmup` more"""
+def _timed_ms(s, steps, warmup, graph=True):
+    """ms per step of `steps` steps (graph-replayed, or eager launches)
+    after `warmup` more"""
     from blade_amd._ffi import check
     L, dev = s.L, s.device()
-    check(L.bm_dynamics(dev, 1, warmup, FREQ_NRG, 1), "warmup")
+    check(L.bm_dynamics(dev, 1, warmup, FREQ_NRG, int(graph)), "warmup")
     check(L.bm_sync(dev), "sync")
     t1 = time.perf_counter()
-    check(L.bm_dynamics(dev, 1 + warmup, steps, FREQ_NRG, 1), "timed steps")
+    check(L.bm_dynamics(dev, 1 + warmup, steps, FREQ_NRG, int(graph)), "timed steps")
     check(L.bm_sync(dev), "sync")
     return (time.perf_counter() - t1) * 1e3 / steps
 
 
-def precision_run(steps, warmup):
+def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):
     """The precision form: the 92,224-atom box (two lambda sites x 4
     substituents, PME 96^3 order 6) with the PME FFT as split-BF16x3 DFT
     contractions on the matrix cores, GPU-relaxed, timed like the main
@@ -269,12 +276,21 @@ def precision_run(steps, warmup):
     t0 = time.time()
     spec = synthetic.config3()
     log("generated %s: %d atoms in %.1f s" % (spec.name, spec.n, time.time() - t0))
-    t0 = time.time()
-    spec = system.relax(spec)
-    log("relaxed in %.1f s" % (time.time() - t0))
+    if state and os.path.exists(state):
+        z = np.load(state)
+        spec.pos, spec.vel = z["pos"], z["vel"]
+        spec.theta0, spec.theta_v = list(z["theta"]), list(z["theta_v"])
+        log("loaded relaxed state from %s" % state)
+    else:
+        t0 = time.time()
+        spec = system.relax(spec, use_graph=graph)
+        log("relaxed in %.1f s" % (time.time() - t0))
+        if state:
+            np.savez(state, pos=spec.pos, vel=spec.vel, theta=np.array(spec.theta0),
+                     theta_v=np.array(spec.theta_v))
     dt_fs = spec.dt / synthetic.PICOSECOND * 1000.0
-    out = {}
-    for mode in (1, 0):
+    out, err = {}, None
+    for mode in modes:
         sp = copy.copy(spec)
         sp.fft_mode = mode
         s = system.build(sp)
@@ -288,18 +304,19 @@ def precision_run(steps, warmup):
                    "recip_energy_rel": abs(eg[8] - eo[8]) / abs(eo[8]),
                    "potential_rel": abs(eg[20] - eo[20]) / abs(eo[20]),
                    "dudl_max_abs_kcal": float(np.max(np.abs(dg - do)))}
-        ms = _timed_ms(s, steps, warmup)
+        ms = _timed_ms(s, steps, warmup, graph)
         s.close()
         out[mode] = ms
-    ms = out[1]
-    return {"metric": "ns/day at 2 fs (92,224-atom precision-form box, MFMA FFT)",
-            "value": round((1e3 / ms) * dt_fs * 86400e-6, 2), "unit": "ns/day", "ms_per_step": round(ms, 5),
-            "steps": steps, "warmup": warmup,
-            "fp32_fft_value": round((1e3 / out[0]) * dt_fs * 86400e-6, 2),
-            "errors_vs_fp64_oracle": {k: float("%.3e" % v) for k, v in err.items()},
+    nsday = lambda m: round((1e3 / out[m]) * dt_fs * 86400e-6, 2) if m in out else None
+    ms = out[modes[0]]
+    return {"metric": "ns/day at 2 fs (92,224-atom precision-form box, %s FFT)" % ("MFMA" if modes[0] else "FP32"),
+            "value": nsday(modes[0]), "unit": "ns/day", "ms_per_step": round(ms, 5),
+            "steps": steps, "warmup": warmup, "fp32_fft_value": nsday(0),
+            "errors_vs_fp64_oracle": {k: float("%.3e" % v) for k, v in err.items()} if err else None,
             "config": {"workload": spec.name, "atoms": spec.n, "box_A": list(spec.box), "lambda_sites": 2,
                        "substituents": [4, 4], "pme_grid": list(spec.grid), "pme_order": spec.order,
-                       "pme_fft": "MFMA split-BF16x3 DFT", "cutoff_A": spec.rcut, "switch_A": spec.rswitch,
+                       "pme_fft": "MFMA split-BF16x3 DFT" if modes[0] else "fp32 Stockham",
+                       "cutoff_A": spec.rcut, "switch_A": spec.rswitch,
                        "dt_fs": dt_fs}}
 
 

@@ -159,30 +159,52 @@ __global__ void __launch_bounds__((64 * DftShape<K, K>::W)) fft_mfma_yz(Dev d) {
   A.load(d.mdft[2][0], d.mdft[2][1], rt);   // the y and z matrices are equal when K1 == K2
   if (INV) A.flip(rt);
   const long long base = (long long)blockIdx.x * K * K;
-  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-    const int y = e / K, z = e - y * K;
-    if (INV) {
-      const float2 v = d.cgrid[base + e];
-      re[y * LD + z] = v.x;
-      im[y * LD + z] = v.y;
-    } else {
-      re[y * LD + z] = d.rgrid[base + e];
+  // the plane's loads are all issued before the first LDS store
+  constexpr int NTH = 64 * DftShape<K, K>::W, NIT = (K * K + NTH - 1) / NTH;
+  if (INV) {
+    float2 v[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH;
+      v[it] = e < K * K ? d.cgrid[base + e] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH, y = e / K, z = e - y * K;
+      if (e < K * K) {
+        re[y * LD + z] = v[it].x;
+        im[y * LD + z] = v[it].y;
+      }
+    }
+  } else {
+    float v[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH;
+      v[it] = e < K * K ? d.rgrid[base + e] : 0.f;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH, y = e / K, z = e - y * K;
+      if (e < K * K) re[y * LD + z] = v[it];
     }
   }
   __syncthreads();
   if (!INV) {
     dft_lines<K, K, true, false>(re, im, LD, 1, A);    // z, real input
     dft_lines<K, K, false, false>(re, im, 1, LD, A);   // y
-    for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-      const int y = e / K, z = e - y * K;
-      d.cgrid[base + e] = make_float2(re[y * LD + z], im[y * LD + z]);
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH, y = e / K, z = e - y * K;
+      if (e < K * K) d.cgrid[base + e] = make_float2(re[y * LD + z], im[y * LD + z]);
     }
   } else {
     dft_lines<K, K, false, false>(re, im, 1, LD, A);   // y
     dft_lines<K, K, false, true>(re, im, LD, 1, A);    // z, real output
-    for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-      const int y = e / K, z = e - y * K;
-      d.rgrid[base + e] = re[y * LD + z];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH, y = e / K, z = e - y * K;
+      if (e < K * K) d.rgrid[base + e] = re[y * LD + z];
     }
   }
 }
@@ -212,25 +234,30 @@ __global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_pass(Dev d
   A.load(d.mdft[AXIS][0], d.mdft[AXIS][1], rt);
   if (MODE >= 3) A.flip(rt);
   const int L0 = blockIdx.x * 32;
-  for (int e = threadIdx.x; e < 32 * K; e += blockDim.x) {
-    int n, k;
-    if (AXIS == 2) { n = e / K; k = e - n * K; }
-    else { k = e / 32; n = e - k * 32; }
+  // 32K elements over 64 W = 4K threads: 8 per thread, loads first
+  constexpr int NTH = 64 * DftShape<K, 32>::W, NIT = 32 * K / NTH;
+  static_assert(NIT * NTH == 32 * K, "whole tiles per thread");
+  float2 v[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int e = threadIdx.x + it * NTH;
+    const int n = AXIS == 2 ? e / K : e % 32, k = AXIS == 2 ? e % K : e / 32;
     const long long a = grid_at<AXIS>(d, L0 + n, k);
-    if (MODE == 0) {
-      re[k * LD + n] = d.rgrid[a];
-    } else {
-      const float2 v = d.cgrid[a];
-      re[k * LD + n] = v.x;
-      im[k * LD + n] = v.y;
-    }
+    v[it] = MODE == 0 ? make_float2(d.rgrid[a], 0.f) : d.cgrid[a];
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int e = threadIdx.x + it * NTH;
+    const int n = AXIS == 2 ? e / K : e % 32, k = AXIS == 2 ? e % K : e / 32;
+    re[k * LD + n] = v[it].x;
+    if (MODE != 0) im[k * LD + n] = v[it].y;
   }
   __syncthreads();
   dft_lines<K, 32, MODE == 0, MODE == 4>(re, im, 1, LD, A);
-  for (int e = threadIdx.x; e < 32 * K; e += blockDim.x) {
-    int n, k;
-    if (AXIS == 2) { n = e / K; k = e - n * K; }
-    else { k = e / 32; n = e - k * 32; }
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int e = threadIdx.x + it * NTH;
+    const int n = AXIS == 2 ? e / K : e % 32, k = AXIS == 2 ? e % K : e / 32;
     const long long a = grid_at<AXIS>(d, L0 + n, k);
     if (MODE == 4) d.rgrid[a] = re[k * LD + n];
     else d.cgrid[a] = make_float2(re[k * LD + n], im[k * LD + n]);
@@ -248,18 +275,30 @@ __global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_x_conv(Dev
   DftRows<K> A;
   A.load(d.mdft[0][0], d.mdft[0][1], rt);
   const int L0 = blockIdx.x * 32;
-  for (int e = threadIdx.x; e < 32 * K; e += blockDim.x) {
-    const int k = e / 32, n = e - k * 32;
-    const float2 v = d.cgrid[grid_at<0>(d, L0 + n, k)];
-    re[k * LD + n] = v.x;
-    im[k * LD + n] = v.y;
+  constexpr int NTH = 64 * DftShape<K, 32>::W, NIT = 32 * K / NTH;
+  static_assert(NIT * NTH == 32 * K, "whole tiles per thread");
+  {
+    float2 v[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH;
+      v[it] = d.cgrid[grid_at<0>(d, L0 + e % 32, e / 32)];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + it * NTH, k = e / 32, n = e % 32;
+      re[k * LD + n] = v[it].x;
+      im[k * LD + n] = v[it].y;
+    }
   }
   __syncthreads();
   dft_lines<K, 32, false, false>(re, im, 1, LD, A);
   A.flip(rt);
   const float L0f = (float)d.box[0], L1f = (float)d.box[1], L2f = (float)d.box[2];
   double ek = 0.0;
-  for (int e = threadIdx.x; e < 32 * K; e += blockDim.x) {
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int e = threadIdx.x + it * NTH;
     const int m = e / 32, n = e - m * 32;
     const int line = L0 + n;
     const int j = line / d.K[2], k = line - j * d.K[2];
@@ -283,8 +322,9 @@ __global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_x_conv(Dev
   if (energy) wave_atomic_fix(&d.energy[BM_ENBRECIP], (long long)__double2ll_rn(ek * kFixScaleD));
   __syncthreads();
   dft_lines<K, 32, false, false>(re, im, 1, LD, A);
-  for (int e = threadIdx.x; e < 32 * K; e += blockDim.x) {
-    const int k = e / 32, n = e - k * 32;
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int e = threadIdx.x + it * NTH, k = e / 32, n = e % 32;
     d.cgrid[grid_at<0>(d, L0 + n, k)] = make_float2(re[k * LD + n], im[k * LD + n]);
   }
 }
