@@ -79,7 +79,9 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="leave the precision-form figure out of the default line")
     ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default FFT form, list cadence (10) and tile chunk (16) instead of autotuning")
+                    help="no autotuning: the FFT form of --fft, the cadence of --freq-domdec, the chunk of --tile-chunk")
+    ap.add_argument("--freq-domdec", type=int, default=10, help="list rebuild cadence without autotuning")
+    ap.add_argument("--tile-chunk", type=int, default=16, help="tiles per nonbonded work unit without autotuning")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave out the 32-system batched figure that the default line carries")
     a = ap.parse_args()
@@ -102,6 +104,7 @@ def main():
     t0 = time.time()
     spec = synthetic.config2()
     spec.fft_mode = 1 if a.fft == "mfma" else 0
+    spec.freq_domdec, spec.tile_chunk = a.freq_domdec, a.tile_chunk
     log("generated %s: %d atoms in %.1f s" % (spec.name, spec.n, time.time() - t0))
     if a.state and os.path.exists(a.state):
         z = np.load(a.state)
@@ -210,11 +213,11 @@ def main():
         "config": {"workload": "config2-msld-23558", "atoms": n, "box_A": spec.box[0],
                    "cutoff_A": spec.rcut, "switch_A": spec.rswitch, "pme_grid": list(spec.grid),
                    "pme_order": spec.order, "dt_fs": dt_fs, "lambda_sites": 1, "substituents": 3,
-                   "list_rebuild_steps": tune["freq_domdec"] if tune else 10, "energy_every": FREQ_NRG,
+                   "list_rebuild_steps": tune["freq_domdec"] if tune else a.freq_domdec, "energy_every": FREQ_NRG,
                    "graph": not a.eager,
                    "pme_fft": "MFMA split-BF16x3 DFT" if (tune["fft_mode"] if tune else a.fft == "mfma")
                    else "fp32 Stockham",
-                   "tiles_per_work_unit": tune["tile_chunk"] if tune else 16},
+                   "tiles_per_work_unit": tune["tile_chunk"] if tune else a.tile_chunk},
         "autotune": tune,
         "pair_interactions_per_s": round(pairs / (ms_step * 1e-3), 1),
         "kernel_ms": {"nbdirect": round(ms_nb, 5), "pme": round(ms_pme, 5), "eager_step": round(ms_eager, 5)},
