@@ -47,6 +47,7 @@ class Topology(C.Structure):
         ("fixed_bias", _f64p), ("theta_friction", _f64p),
         ("n_varbias", C.c_int32), ("vb_i", _i32p), ("vb_j", _i32p), ("vb_type", _i32p),
         ("vb_n", _i32p), ("vb_l0", _f64p), ("vb_k", _f64p),
+        ("n_cats", C.c_int32), ("cats_bound", _i32p), ("cats_idx", _i32p), ("cats_k", C.c_double),
     ]
 
     def array(self, name, count, width=1):
@@ -138,6 +139,8 @@ def lib():
         "blade_add_msld_bias": (None, [vp, i, i, i, d, d, i]),
         "blade_add_msld_block_friction": (None, [vp, i, d]),
         "blade_set_msld_block_fixed": (None, [vp, i, i]),
+        "blade_add_msld_atomrestraint": (None, [vp]),
+        "blade_add_msld_atomrestraint_element": (None, [vp, i]),
         "blade_init_run": (None, [vp]),
         "blade_add_run_flags": (None, [vp, d, d, d, d, i, i, d, i, i, i, i, d]),
         "blade_add_run_dynopts": (None, [vp, i, i, i, d, d, i, d, d]),

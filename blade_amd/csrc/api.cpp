@@ -199,9 +199,11 @@ void blade_add_msld_termscaling(System *s, int sb, int su, int sa, int sd, int s
   t[0] = sb; t[1] = su; t[2] = sa; t[3] = sd; t[4] = si; t[5] = sc;
 }
 void blade_add_msld_flags(System *s, double gamma, double fnex, int useSoftCore, int useSoftCore14,
-                          int msldEwaldType, double, double, double, double softNotBondExponent, int fix) {
+                          int msldEwaldType, double kRestraint, double, double, double softNotBondExponent,
+                          int fix) {
   auto &m = *s->msld;
   m.gamma = gamma;
+  m.kRestraint = kRestraint;
   m.fnex = fnex;
   m.useSoftCore = useSoftCore;
   m.useSoftCore14 = useSoftCore14;
@@ -220,6 +222,11 @@ void blade_add_msld_block_friction(System *s, int blockIdx, double friction) {
   m.thetaFriction[blockIdx - 1] = friction;
 }
 void blade_set_msld_block_fixed(System *s, int blockIdx, int fixed) { s->msld->blockFixed[blockIdx - 1] = fixed != 0; }
+void blade_add_msld_atomrestraint(System *s) { s->msld->atomRestraints.emplace_back(); }
+void blade_add_msld_atomrestraint_element(System *s, int i) {
+  if (s->msld->atomRestraints.empty()) s->msld->atomRestraints.emplace_back();
+  s->msld->atomRestraints.back().push_back(i - 1);
+}
 
 // ---- run -----------------------------------------------------------------
 void blade_init_run(System *s) { s->run.reset(new RunSetup()); }

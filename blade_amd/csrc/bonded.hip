@@ -143,7 +143,7 @@ __device__ void nb14_ef(const Dev &d, double4 p, double r, double &f, double &e)
 }
 
 struct BondedLaunch {
-  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_bias;  // first block of each family
+  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_cats, b_bias;  // first block of each family
 };
 
 // workgroups of 256 threads per term family, in family order; the last
@@ -157,7 +157,8 @@ __host__ __device__ inline BondedLaunch bonded_layout(const Dev &d) {
   L.b_impr = L.b_dihe + nb(d.ndihe);
   L.b_nb14 = L.b_impr + nb(d.nimpr);
   L.b_nbex = L.b_nb14 + nb(d.nnb14);
-  L.b_bias = L.b_nbex + nb(d.nnbex);
+  L.b_cats = L.b_nbex + nb(d.nnbex);
+  L.b_bias = L.b_cats + nb(d.ncats);
   return L;
 }
 
@@ -286,7 +287,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       put_d(d, d.sl_nb14 + 2 * t + 1, (-f / r), dr);
       e = et * ll;
     }
-  } else if (bid < L.b_bias) {
+  } else if (bid < L.b_cats) {
     slot = BM_ENBRECIPEXCL;
     const int t = (bid - L.b_nbex) * blockDim.x + threadIdx.x;
     // PME exclusion corrections in FP64 (they cancel the self term)
@@ -308,6 +309,38 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       put_d(d, d.sl_nbex + 2 * t, (f / r), dr);
       put_d(d, d.sl_nbex + 2 * t + 1, (-f / r), dr);
       e = et * ll;
+    }
+  } else if (bid < L.b_bias) {
+    // MSLD atom restraints (msld.cu:1328-1364), one thread per group in
+    // FP64: the centroid from minimum-image offsets to the first atom,
+    // dU/dx_i = k (x_i - centroid), not lambda-scaled
+    slot = BM_ECATS;
+    const int t = (bid - L.b_cats) * blockDim.x + threadIdx.x;
+    if (t < d.ncats) {
+      const int m0 = d.cats_bound[t], m1 = d.cats_bound[t + 1];
+      const int a0 = d.cats_idx[m0];
+      D3v c{0.0, 0.0, 0.0};
+      for (int m = m0 + 1; m < m1; m++) {
+        const D3v r = dispd(d, d.cats_idx[m], a0);
+        c.x += r.x;
+        c.y += r.y;
+        c.z += r.z;
+      }
+      const double inv = 1.0 / (m1 - m0);
+      c.x *= inv;
+      c.y *= inv;
+      c.z *= inv;
+      for (int m = m0; m < m1; m++) {
+        D3v r = m == m0 ? D3v{0.0, 0.0, 0.0} : dispd(d, d.cats_idx[m], a0);
+        r.x -= c.x;
+        r.y -= c.y;
+        r.z -= c.z;
+        r.x -= d.box[0] * floor(r.x * d.invbox[0] + 0.5);
+        r.y -= d.box[1] * floor(r.y * d.invbox[1] + 0.5);
+        r.z -= d.box[2] * floor(r.z * d.invbox[2] + 0.5);
+        put_d(d, d.sl_cats + m, d.cats_k, r);
+        e += 0.5 * d.cats_k * (r.x * r.x + r.y * r.y + r.z * r.z);
+      }
     }
   } else {
     // MSLD fixed and variable biases: one thread per block / bias

@@ -123,7 +123,7 @@ struct Dev {
   // bonded force slots: every term stores its per-atom forces (fixed
   // point) in slots of its own, laid out atom by atom (a static CSR), so
   // the force assembly reads each atom's slots as one contiguous run
-  int sl_bond, sl_angle, sl_dihe, sl_impr, sl_nb14, sl_nbex, nslot;
+  int sl_bond, sl_angle, sl_dihe, sl_impr, sl_nb14, sl_nbex, sl_cats, nslot;
   long long *bslot;              // [3][nslot] in atom order
   int *bcsr_start;               // [N+1] first slot position of each atom
   int *bspos;                    // [nslot] term slot -> position in atom order
@@ -140,6 +140,9 @@ struct Dev {
   int *block_site, *site_bound, *block_fixed;
   double *fixed_bias;
   double *lfric, *lnoise;        // [nL] per-block theta friction/noise
+  int ncats;                     // MSLD atom restraint groups
+  int *cats_bound, *cats_idx;    // [ncats+1] first member, [members] atoms
+  double cats_k;                 // kRestraint
   int nvb;
   int4 *vb_i;                    // (i, j, type, n)
   double2 *vb_p;                 // (l0, k)

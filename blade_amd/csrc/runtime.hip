@@ -493,6 +493,15 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       xp[q] = t->nbex_qq[q];
     }
     UPLOAD(d.nbex_i, xi); UPLOAD(d.nbex_sb, xsb); UPLOAD(d.nbex_p, xp);
+    // atom restraint groups
+    d.ncats = t->n_cats;
+    const int ncm = d.ncats ? t->cats_bound[d.ncats] : 0;
+    {
+      std::vector<int> cb(t->cats_bound, t->cats_bound + (d.ncats ? d.ncats + 1 : 0));
+      std::vector<int> ci(t->cats_idx, t->cats_idx + ncm);
+      if (d.ncats) { UPLOAD(d.cats_bound, cb); UPLOAD(d.cats_idx, ci); }
+      d.cats_k = t->cats_k;
+    }
     // bonded force slots and the atom -> slot CSR (the topology is static)
     d.sl_bond = 0;
     d.sl_angle = d.sl_bond + 2 * d.nbond;
@@ -500,7 +509,8 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     d.sl_impr = d.sl_dihe + 4 * d.ndihe;
     d.sl_nb14 = d.sl_impr + 4 * d.nimpr;
     d.sl_nbex = d.sl_nb14 + 2 * d.nnb14;
-    d.nslot = d.sl_nbex + 2 * d.nnbex;
+    d.sl_cats = d.sl_nbex + 2 * d.nnbex;
+    d.nslot = d.sl_cats + ncm;
     std::vector<int> owner(d.nslot);
     for (int q = 0; q < d.nbond; q++)
       for (int k = 0; k < 2; k++) owner[d.sl_bond + 2 * q + k] = t->bond_idx[2 * q + k];
@@ -514,6 +524,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       for (int k = 0; k < 2; k++) owner[d.sl_nb14 + 2 * q + k] = t->nb14_idx[2 * q + k];
     for (int q = 0; q < d.nnbex; q++)
       for (int k = 0; k < 2; k++) owner[d.sl_nbex + 2 * q + k] = t->nbex_idx[2 * q + k];
+    for (int q = 0; q < ncm; q++) owner[d.sl_cats + q] = t->cats_idx[q];
     std::vector<int> cstartv(n + 1, 0), pos(d.nslot);
     for (int q = 0; q < d.nslot; q++) cstartv[owner[q] + 1]++;
     for (int a = 0; a < n; a++) cstartv[a + 1] += cstartv[a];

@@ -54,15 +54,25 @@ int merge_site_block(int site, int block) {
 
 struct Scaler {
   const MsldSetup &m;
+  std::vector<char> restrained;   // [N] atom is in an atom restraint group
+  explicit Scaler(const MsldSetup &ms) : m(ms), restrained(ms.atomBlock.size(), 0) {
+    for (const auto &g : m.atomRestraints)
+      for (int a : g)
+        if (a >= 0 && a < (int)restrained.size()) restrained[a] = 1;
+  }
   // Msld::bonded_scaling (msld/msld.cu:509-553): distinct blocks of the
-  // atoms in descending order, at most nsc of them.
+  // atoms in descending order, at most nsc of them.  A type named by
+  // removescaling keeps an atom's block only when every atom of the term in
+  // that block is restrained (no soft bonds in this build).
   void bonded(const int *idx, int nat, int type, int nsc, int *sb) const {
     int block[5] = {m.blockCount, 0, 0, 0, 0};
     bool scale = m.scaleTerms[type];
     for (int i = 1; i < nsc + 2; i++)
       for (int j = 0; j < nat; j++) {
         int ab = m.atomBlock[idx[j]];
-        if (!scale) ab = 0;  // no restrained atoms in this build
+        if (!scale)
+          for (int k = 0; k < nat; k++)
+            if (m.atomBlock[idx[k]] == m.atomBlock[idx[j]] && !restrained[idx[k]]) ab = 0;
         if (ab > block[i] && ab < block[i - 1]) block[i] = ab;
       }
     for (int i = 1; i < nsc + 1; i++)
@@ -397,6 +407,18 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     t.vbI.push_back(v.i); t.vbJ.push_back(v.j); t.vbType.push_back(v.type);
     t.vbN.push_back(v.n); t.vbL0.push_back(v.l0); t.vbK.push_back(v.k);
   }
+  // atom restraint groups (msld.cu:755-776), empty groups dropped
+  t.catsBound.assign(1, 0);
+  t.catsIdx.clear();
+  for (const auto &g : m.atomRestraints) {
+    if (g.empty()) continue;
+    for (int a : g) {
+      if (a < 0 || a >= n) fail("atom restraint index out of range");
+      t.catsIdx.push_back(a);
+    }
+    t.catsBound.push_back((int)t.catsIdx.size());
+  }
+  t.catsK = m.kRestraint;
   (void)kBoltz;
 }
 
@@ -435,6 +457,10 @@ const bm_topology *Topology::view() {
   v.n_varbias = (int)vbK.size();
   v.vb_i = vbI.data(); v.vb_j = vbJ.data(); v.vb_type = vbType.data(); v.vb_n = vbN.data();
   v.vb_l0 = vbL0.data(); v.vb_k = vbK.data();
+  v.n_cats = catsBound.empty() ? 0 : (int)catsBound.size() - 1;
+  v.cats_bound = catsBound.empty() ? nullptr : catsBound.data();
+  v.cats_idx = catsIdx.data();
+  v.cats_k = catsK;
   return &top;
 }
 

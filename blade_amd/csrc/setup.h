@@ -74,6 +74,8 @@ struct MsldSetup {
   int msldEwaldType = 2;
   bool fix = false;
   std::vector<VarBias> varBias;
+  std::vector<std::vector<int>> atomRestraints;   // 0-based atoms of each group
+  double kRestraint = 59.2;                       // kcal/mol/A^2 (msld.cu:83)
   int softBondCount = 0;
   void resize(int nblocks, int natoms);
 };
@@ -119,6 +121,7 @@ struct Topology {
   std::vector<int> blockSite, siteBound, blockFixed;
   std::vector<double> thetaMass, fixedBias, thetaFriction;
   std::vector<int> vbI, vbJ, vbType, vbN; std::vector<double> vbL0, vbK;
+  std::vector<int> catsBound, catsIdx; double catsK = 0;
   bm_topology top{};
   const bm_topology *view();
 };

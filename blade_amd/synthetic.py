@@ -22,7 +22,8 @@ Workloads (sizes as named in the benchmark descriptions):
                 (fast parity cases)
   two_site_small  a thousand-atom box with two chains, each carrying a
                 lambda site (3 and 2 substituents), and two ion pairs; only
-                dihedrals scale with lambda
+                dihedrals scale with lambda, and the substituents' first
+                atoms are held together by atom restraints
 """
 import math
 from dataclasses import dataclass, field
@@ -90,6 +91,8 @@ class Spec:
     fnex: float = 5.5
     soft_core: bool = False
     scale_terms: tuple = (1, 1, 1, 1, 1, 1)           # lambda-scaled bond, urey, angle, dihe, impr, cmap
+    atom_restraints: list = field(default_factory=list)  # groups of 0-based atoms held to their centroid
+    k_restraint: float = 59.2                         # kcal/mol/A^2
     # run options
     dt: float = 0.002 * PICOSECOND
     temperature: float = 298.0
@@ -725,6 +728,8 @@ def two_site_small(seed=0xB1ADE0C0, box=24.0, grid=24):
     spec.pos = np.array(spec._pos)
     _msld_sites(spec, sites, fnex=5.0)
     spec.scale_terms = (0, 0, 0, 1, 0, 1)   # `removescaling bond urey angle impr`
+    # the first heavy atom of each substituent of a site held together
+    spec.atom_restraints = [[subs[0] for subs in site] for site in sites]
     spec.rcut, spec.rswitch = 9.0, 8.0
     spec.beta = 0.34
     spec.grid, spec.order = (grid, grid, grid), 6

@@ -107,6 +107,12 @@ class BladeSystem:
     def add_msld_flags(self, gamma, fnex, soft, soft14, ewaldtype, krest, sbr, sbe, snbe, fix):
         self.L.blade_add_msld_flags(self.h, gamma, fnex, soft, soft14, ewaldtype, krest, sbr, sbe, snbe, fix)
 
+    def add_msld_atomrestraint(self, atoms):
+        """one restraint group of 1-based atom indices"""
+        self.L.blade_add_msld_atomrestraint(self.h)
+        for a in atoms:
+            self.L.blade_add_msld_atomrestraint_element(self.h, int(a))
+
     def add_msld_termscaling(self, bond, urey, angle, dihe, impr, cmap):
         self.L.blade_add_msld_termscaling(self.h, bond, urey, angle, dihe, impr, cmap)
 
@@ -271,8 +277,11 @@ def build(spec, gpus=(0,)):
             tv = spec.theta_v[b] if spec.theta_v is not None else 0.0
             s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], tv,
                                          spec.theta_mass[b], spec.fixed_bias[b], 0.0)
-        s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2, 59.2, 1.0, 2.0, 1.0, 0)
+        s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2,
+                         getattr(spec, "k_restraint", 59.2), 1.0, 2.0, 1.0, 0)
         s.add_msld_termscaling(*[int(x) for x in spec.scale_terms])
+        for g in getattr(spec, "atom_restraints", []):
+            s.add_msld_atomrestraint([a + 1 for a in g])
         for (i, j, t, l0, k, n) in spec.var_bias:
             s.add_msld_bias(i + 1, j + 1, t, l0, k, n)
     s.init_run()

@@ -96,6 +96,10 @@ void blade_add_msld_flags(System *system, double gamma, double fnex, int useSoft
 void blade_add_msld_bias(System *system, int i, int j, int type, double l0, double k, int n);
 void blade_add_msld_block_friction(System *system, int blockIdx, double friction);
 void blade_set_msld_block_fixed(System *system, int blockIdx, int fixed);
+/* MSLD atom restraints (msld/msld.h:160-161): a new group, then its atoms
+   (1-based); the force constant is blade_add_msld_flags' kRestraint */
+void blade_add_msld_atomrestraint(System *system);
+void blade_add_msld_atomrestraint_element(System *system, int atomIdx);
 
 /* run/run.h:146-158 */
 void blade_init_run(System *system);

@@ -135,6 +135,14 @@ typedef struct bm_topology {
   int32_t n_varbias;
   const int32_t *vb_i, *vb_j, *vb_type, *vb_n;   /* [n] (msld.h:13-19) */
   const double *vb_l0, *vb_k;
+
+  /* MSLD atom restraints (msld.h:73-79, msld.cu:1328-1364): each group's
+     atoms are pulled to their centroid, E = k/2 sum |x_i - centroid|^2,
+     not lambda-scaled (energy term BM_ECATS) */
+  int32_t n_cats;                  /* groups                              */
+  const int32_t *cats_bound;       /* [n+1] first member of each group    */
+  const int32_t *cats_idx;         /* [cats_bound[n]] member atoms        */
+  double cats_k;                   /* kRestraint, kcal/mol/A^2            */
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */

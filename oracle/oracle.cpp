@@ -684,6 +684,26 @@ void msldbias(const bm_topology *t, const double *lambda, double *dudl, double *
   }
 }
 
+// MSLD atom restraints (msld/msld.cu:1328-1364): each group's atoms are
+// pulled to the group centroid (minimum-image offsets from its first
+// atom), E = k/2 sum |x_i - c|^2 and dU/dx_i = k (x_i - c); not scaled.
+void cats(const bm_topology *t, const bm_runparams *rp, const double *pos, double *force, double *energy) {
+  const int n = t->n_atoms;
+  for (int g = 0; g < t->n_cats; g++) {
+    const int m0 = t->cats_bound[g], m1 = t->cats_bound[g + 1];
+    const Vec3 x0 = getp(pos, n, t->cats_idx[m0]);
+    Vec3 c = v3(0, 0, 0);
+    for (int m = m0 + 1; m < m1; m++) c = c + minimg(getp(pos, n, t->cats_idx[m]) - x0, rp->box);
+    c = (1.0 / (m1 - m0)) * c;
+    for (int m = m0; m < m1; m++) {
+      const int a = t->cats_idx[m];
+      const Vec3 dx = minimg(getp(pos, n, a) - x0 - c, rp->box);
+      addf(force, n, a, t->cats_k * dx);
+      energy[BM_ECATS] += 0.5 * t->cats_k * dot(dx, dx);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------
 // Holonomic constraints in FP64.
 // ---------------------------------------------------------------------
@@ -853,6 +873,7 @@ long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
   if (terms & ORC_BONDED) bonded(t, rp, pos, lambda, force, dudl, energy);
   if (terms & ORC_NB14) nb14(t, rp, pos, lambda, force, dudl, energy);
   if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1) msldbias(t, lambda, dudl, energy);
+  if ((terms & ORC_MSLDBIAS) && t->n_cats > 0) cats(t, rp, pos, force, energy);
   double pot = 0;
   for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
   energy[BM_EPOTENTIAL] = pot;

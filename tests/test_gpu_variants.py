@@ -58,6 +58,19 @@ def test_switching_methods(vdw, elec):
     _check(_variant("msld_small", vdw_method=vdw, elec_method=elec), "vdw%d elec%d" % (vdw, elec))
 
 
+def test_atom_restraints_displaced():
+    """MSLD atom restraints with the restrained atoms moved apart by up to
+    0.5 A per axis, so every group pulls (energy term BM_ECATS)"""
+    sp = _variant("two_site_small")
+    pos = np.array(sp.pos, float)
+    rng = np.random.default_rng(9)
+    for g in sp.atom_restraints:
+        for a in g:
+            pos[a] += rng.uniform(-0.5, 0.5, 3)
+    sp.pos = pos
+    _check(sp, "atom restraints")
+
+
 def test_soft_core_site():
     _check(_variant("msld_small", soft_core=True), "soft core")
 

@@ -239,6 +239,53 @@ def test_forces_are_energy_gradients(group):
     _fd_check("msld_small", terms, eidx, atoms)
 
 
+def test_atom_restraints_pairwise_identity_and_gradient():
+    """MSLD atom restraints: with the restrained atoms moved apart (one
+    group across a periodic face), the energy equals the pairwise form
+    k/(2n) sum_{i<j} |x_i - x_j|^2 over minimum-image differences, and
+    the forces are its gradient."""
+    name = "two_site_small"
+    sp = spec(name)
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    t = top.contents
+    assert t.n_cats == len(sp.atom_restraints) >= 2
+    n = sp.n
+    L = np.array(sp.box)
+    x = system.soa(sp.pos).copy()
+    rng = np.random.default_rng(5)
+    groups = [list(g) for g in sp.atom_restraints]
+    for g in groups:
+        for a in g:
+            for k in range(3):
+                x[k * n + a] += rng.uniform(-0.6, 0.6)
+    a0 = groups[0][0]
+    x[a0] = 0.05                      # first group: one atom just inside x = 0 ...
+    x[groups[0][1]] = L[0] - 0.2      # ... and one just below x = L
+    lam = lambdas_of(name, s)
+    f, _, e, _ = orc.forces(top, rp, x, lam, orc.MSLDBIAS)
+    want = 0.0
+    for g in groups:
+        P = np.array([[x[k * n + a] for k in range(3)] for a in g])
+        for i in range(len(g)):
+            for j in range(i + 1, len(g)):
+                d = P[i] - P[j]
+                d -= L * np.round(d / L)
+                want += sp.k_restraint / (2 * len(g)) * float(d @ d)
+    assert abs(e[13] - want) <= 1e-9 * max(1.0, want), (e[13], want)
+    h, worst = 1e-5, 0.0
+    for g in groups:
+        for a in g:
+            for k in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[k * n + a] += h
+                xm[k * n + a] -= h
+                fd = (orc.forces(top, rp, xp, lam, orc.MSLDBIAS)[2][13]
+                      - orc.forces(top, rp, xm, lam, orc.MSLDBIAS)[2][13]) / (2 * h)
+                worst = max(worst, abs(fd - f[k * n + a]))
+    assert worst < 1e-5, worst
+
+
 def test_dudl_matches_lambda_differences():
     name = "msld_small"
     s = built(name)

@@ -85,33 +85,38 @@ def test_exclusion_lists_follow_the_reference_rules(name):
 
 @pytest.mark.parametrize("name", ["msld_small", "two_site_small"])
 def test_bonded_scaling_follows_the_reference_rule(name):
-    """src/msld/msld.cu:507-553 with no restrained atoms or soft bonds: a
-    term of a scaled type carries the distinct blocks of its atoms in
-    descending order as (site << 16) | block codes, padded with 0; a term of
-    a type named by `removescaling` carries none.  msld_small scales every
-    type, two_site_small only dihedrals (and CMAP)."""
+    """src/msld/msld.cu:507-553 without soft bonds: a term carries the
+    distinct blocks of its atoms in descending order as (site << 16) | block
+    codes, padded with 0.  For a type named by `removescaling` an atom's
+    block counts only when every atom of the term in that block is
+    restrained.  msld_small scales every type; two_site_small only
+    dihedrals (and CMAP), with each substituent's first atom restrained."""
     sp = spec(name)
     s = built(name)
     t = s.topology().contents
     block = np.asarray(sp.atom_block)
     site = np.asarray(sp.block_site)
-    seen_scaled = seen_unscaled = 0
+    restrained = {a for g in sp.atom_restraints for a in g}
+    seen_scaled = seen_unscaled = seen_kept = 0
     for key, count, width, typ in (("bond", t.n_bonds12, 2, 0), ("angle", t.n_angles, 3, 2),
                                    ("dihe", t.n_dihes, 4, 3), ("impr", t.n_imprs, 4, 4)):
         idx = t.array(key + "_idx", count, width)
         sb = t.array(key + "_sb", count, 2)
         for atoms, got in zip(idx, sb):
-            bl = sorted({int(block[a]) for a in atoms if block[a]}, reverse=True)
-            if sp.scale_terms[typ] and bl:
-                want = [(int(site[b]) << 16) | b for b in bl] + [0] * (2 - len(bl))
-                seen_scaled += 1
+            if sp.scale_terms[typ]:
+                kept = {int(block[a]) for a in atoms if block[a]}
             else:
-                want = [0, 0]
-                seen_unscaled += bool(bl)
+                kept = {int(block[a]) for a in atoms if block[a] and
+                        all(b in restrained for b in atoms if block[b] == block[a])}
+                seen_kept += bool(kept)
+            bl = sorted(kept, reverse=True)
+            want = [(int(site[b]) << 16) | b for b in bl] + [0] * (2 - len(bl))
+            seen_scaled += bool(bl)
+            seen_unscaled += bool(not bl and any(block[a] for a in atoms))
             assert list(got) == want, (key, atoms, got, want)
     assert seen_scaled > 0
     if name == "two_site_small":
-        assert seen_unscaled > 0
+        assert seen_unscaled > 0 and seen_kept > 0
 
 
 def test_nb14_parameters():
