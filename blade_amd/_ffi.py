@@ -162,6 +162,10 @@ def lib():
         "blade_set_domdec_frequency": (None, [vp, i]),
         "blade_set_fft_mode": (None, [vp, i]),
         "blade_set_tile_chunk": (None, [vp, i]),
+        "blade_set_output_nrg": (i, [vp, C.c_char_p, i]),
+        "blade_set_output_lmd": (i, [vp, C.c_char_p, i]),
+        "blade_write_checkpoint": (i, [vp, C.c_char_p]),
+        "blade_read_checkpoint": (i, [vp, C.c_char_p]),
         "blade_autotune": (i, [vp, i, C.POINTER(TuneResult)]),
         "blade_batch_autotune": (i, [i, C.POINTER(vp), i, C.POINTER(TuneResult)]),
         "blade_read_parameter_file": (i, [vp, C.c_char_p]),

@@ -3,8 +3,11 @@
 // system/parameters.cxx:585-786, system/structure.cxx:960-1092,
 // system/coordinates.cxx:323-368, msld/msld.cu:1570-1735 and
 // run/run.cu:633-719).  Indices arrive 1-based as from CHARMM.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <string>
 
@@ -38,6 +41,13 @@ struct System {
   int freqDomdec = 10;
   int fftMode = 0;
   int tileChunk = 0;
+  // output files written by blade_run_dynamics (run/run.cu:572-590)
+  FILE *fpNRG = nullptr, *fpLMD = nullptr;
+  int freqNRG = 0, freqLMD = 0;
+  ~System() {
+    if (fpNRG) fclose(fpNRG);
+    if (fpLMD) fclose(fpLMD);
+  }
 };
 
 extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(stderr, "%s", message); }
@@ -390,12 +400,55 @@ void blade_run_energy(System *s) {
   if (rc) api_fail(s, std::string("energy: ") + bm_last_error());
 }
 
+// one .nrg line (io/io.cxx:376-387): the step, then every energy term
+static void write_nrg(System *s, long step) {
+  double e[BM_NTERMS];
+  if (bm_get_energy(s->gpu, e)) return;
+  fprintf(s->fpNRG, "%10ld", step);
+  for (int i = 0; i < BM_NTERMS; i++) fprintf(s->fpNRG, " %12.4f", e[i]);
+  fprintf(s->fpNRG, "\n");
+}
+
+// one text .lmd line (io/io.cxx:350-358): the step, then lambda of every
+// block after the environment
+static void write_lmd(System *s, long step) {
+  const int nl = s->topo.top.n_blocks;
+  std::vector<double> l(nl > 0 ? nl : 1);
+  if (bm_get_lambda(s->gpu, l.data())) return;
+  fprintf(s->fpLMD, "%10ld", step);
+  for (int i = 1; i < nl; i++) fprintf(s->fpLMD, " %8.6f", l[i]);
+  fprintf(s->fpLMD, "\n");
+}
+
+// the first step >= from that is a multiple of f (f > 0)
+static long next_multiple(long from, int f) { return ((from + f - 1) / f) * f; }
+
 int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
   if (!s || s->failed) return BM_EINVAL;
   int rc = ensure_gpu(s);
   if (rc) return rc;
-  rc = bm_dynamics(s->gpu, s->step, nsteps, freqNRG, use_graph);
-  if (!rc) rc = bm_sync(s->gpu);
+  if (s->fpNRG || s->fpLMD) {
+    // segments that end on output steps; energies on the .nrg steps
+    // (run/run.cu:545-570 prints after each step's update)
+    const int fe = s->fpNRG ? s->freqNRG : freqNRG;
+    const long end = s->step + nsteps;
+    long t = s->step;
+    while (!rc && t < end) {
+      long stop = end - 1;
+      if (s->fpNRG) stop = std::min(stop, next_multiple(t, s->freqNRG));
+      if (s->fpLMD) stop = std::min(stop, next_multiple(t, s->freqLMD));
+      rc = bm_dynamics(s->gpu, t, stop - t + 1, fe, use_graph);
+      if (!rc) rc = bm_sync(s->gpu);
+      if (!rc && s->fpNRG && stop % s->freqNRG == 0) write_nrg(s, stop);
+      if (!rc && s->fpLMD && stop % s->freqLMD == 0) write_lmd(s, stop);
+      t = stop + 1;
+    }
+    if (s->fpNRG) fflush(s->fpNRG);
+    if (s->fpLMD) fflush(s->fpLMD);
+  } else {
+    rc = bm_dynamics(s->gpu, s->step, nsteps, freqNRG, use_graph);
+    if (!rc) rc = bm_sync(s->gpu);
+  }
   if (!rc) {
     int flag = -1;
     bm_get_nan_flag(s->gpu, &flag);
@@ -407,6 +460,115 @@ int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
   if (rc) api_fail(s, std::string("dynamics: ") + bm_last_error());
   s->step += nsteps;
   return rc;
+}
+
+int blade_set_output_nrg(System *s, const char *path, int freq) {
+  if (!s || freq < 0) return BM_EINVAL;
+  if (s->fpNRG) fclose(s->fpNRG);
+  s->fpNRG = nullptr;
+  s->freqNRG = 0;
+  if (!path || !freq) return BM_OK;
+  s->fpNRG = fopen(path, "w");
+  if (!s->fpNRG) {
+    api_fail(s, std::string("cannot open ") + path);
+    return BM_EINVAL;
+  }
+  s->freqNRG = freq;
+  return BM_OK;
+}
+
+int blade_set_output_lmd(System *s, const char *path, int freq) {
+  if (!s || freq < 0) return BM_EINVAL;
+  if (s->fpLMD) fclose(s->fpLMD);
+  s->fpLMD = nullptr;
+  s->freqLMD = 0;
+  if (!path || !freq) return BM_OK;
+  s->fpLMD = fopen(path, "w");
+  if (!s->fpLMD) {
+    api_fail(s, std::string("cannot open ") + path);
+    return BM_EINVAL;
+  }
+  s->freqLMD = freq;
+  return BM_OK;
+}
+
+// Binary checkpoint: the exact FP64 state, so a run continued from it is
+// bit-identical to the uninterrupted one when it was written on a list
+// rebuild step (step a multiple of the domdec cadence).  Layout: 8-byte
+// magic "BLADEMCK", int64 step, int32 atoms, int32 blocks, f64 box[3],
+// then f64 positions [3N] and velocities [3N] (SoA, x then y then z),
+// theta [nL] and theta velocities [nL].
+static const char kCkMagic[8] = {'B', 'L', 'A', 'D', 'E', 'M', 'C', 'K'};
+
+int blade_write_checkpoint(System *s, const char *path) {
+  if (!s || !path || !s->gpu) return BM_EINVAL;
+  const int n = s->topo.n, nl = std::max(1, s->topo.top.n_blocks);
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), th(nl), tv(nl);
+  int rc = bm_get_positions(s->gpu, x.data(), th.data());
+  if (!rc) rc = bm_get_velocities(s->gpu, v.data(), tv.data());
+  if (rc) return rc;
+  FILE *fp = fopen(path, "wb");
+  if (!fp) {
+    api_fail(s, std::string("cannot open ") + path);
+    return BM_EINVAL;
+  }
+  const int64_t step = s->step;
+  const int32_t hn[2] = {n, nl};
+  bool ok = fwrite(kCkMagic, 1, 8, fp) == 8 && fwrite(&step, 8, 1, fp) == 1 && fwrite(hn, 4, 2, fp) == 2 &&
+            fwrite(s->rp.box, 8, 3, fp) == 3 && fwrite(x.data(), 8, x.size(), fp) == x.size() &&
+            fwrite(v.data(), 8, v.size(), fp) == v.size() && fwrite(th.data(), 8, nl, fp) == (size_t)nl &&
+            fwrite(tv.data(), 8, nl, fp) == (size_t)nl;
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) api_fail(s, std::string("checkpoint write failed: ") + path);
+  return ok ? BM_OK : BM_EINVAL;
+}
+
+int blade_read_checkpoint(System *s, const char *path) {
+  if (!s || !path || s->failed) return BM_EINVAL;
+  if (s->gpu) {
+    api_fail(s, "blade_read_checkpoint comes before the first energy or dynamics call");
+    return BM_EINVAL;
+  }
+  int rc = blade_prepare(s);
+  if (rc) return rc;
+  FILE *fp = fopen(path, "rb");
+  if (!fp) {
+    api_fail(s, std::string("cannot open ") + path);
+    return BM_EINVAL;
+  }
+  char magic[8];
+  int64_t step = 0;
+  int32_t hn[2] = {0, 0};
+  double box[3];
+  const int n = s->topo.n, nl = std::max(1, s->topo.top.n_blocks);
+  bool ok = fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkMagic, 8) == 0 && fread(&step, 8, 1, fp) == 1 &&
+            fread(hn, 4, 2, fp) == 2 && fread(box, 8, 3, fp) == 3;
+  if (ok && (hn[0] != n || hn[1] != nl)) {
+    fclose(fp);
+    api_fail(s, "checkpoint does not match the system (atoms or blocks)");
+    return BM_EINVAL;
+  }
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), th(nl), tv(nl);
+  ok = ok && fread(x.data(), 8, x.size(), fp) == x.size() && fread(v.data(), 8, v.size(), fp) == v.size() &&
+       fread(th.data(), 8, nl, fp) == (size_t)nl && fread(tv.data(), 8, nl, fp) == (size_t)nl;
+  fclose(fp);
+  if (!ok) {
+    api_fail(s, std::string("not a checkpoint file: ") + path);
+    return BM_EINVAL;
+  }
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      s->coordinates->pos[3 * (size_t)i + k] = x[(size_t)k * n + i];
+      s->coordinates->vel[3 * (size_t)i + k] = v[(size_t)k * n + i];
+    }
+  for (int k = 0; k < 3; k++) s->coordinates->box[k] = box[k];
+  if (s->topo.top.n_blocks > 1) {
+    s->msld->theta.assign(th.begin(), th.end());
+    s->msld->thetaVelocity.assign(tv.begin(), tv.end());
+  }
+  s->run->step = step;
+  s->prepared = false;   // the box may differ from the setup's
+  return BM_OK;
 }
 
 int blade_get_energy(System *s, double *e) { return s && s->gpu ? bm_get_energy(s->gpu, e) : BM_EINVAL; }

@@ -138,6 +138,19 @@ class BladeSystem:
     def set_fft_mode(self, mode):
         self.L.blade_set_fft_mode(self.h, mode)
 
+    def set_output(self, nrg=None, lmd=None, freq_nrg=10, freq_lmd=10):
+        """.nrg and .lmd files written by run_dynamics every freq steps"""
+        check(self.L.blade_set_output_nrg(self.h, str(nrg).encode() if nrg else None, int(freq_nrg) if nrg else 0),
+              "blade_set_output_nrg")
+        check(self.L.blade_set_output_lmd(self.h, str(lmd).encode() if lmd else None, int(freq_lmd) if lmd else 0),
+              "blade_set_output_lmd")
+
+    def write_checkpoint(self, path):
+        check(self.L.blade_write_checkpoint(self.h, str(path).encode()), "blade_write_checkpoint")
+
+    def read_checkpoint(self, path):
+        check(self.L.blade_read_checkpoint(self.h, str(path).encode()), "blade_read_checkpoint")
+
     def set_tile_chunk(self, tiles):
         self.L.blade_set_tile_chunk(self.h, tiles)
 

@@ -139,6 +139,21 @@ void blade_set_seed(System *system, unsigned long long seed);
 /* steps between neighbour-list rebuilds (default 10, domdec/domdec.cu:138);
    1 rebuilds every step, for starts whose atoms outrun the thermal list pad */
 void blade_set_domdec_frequency(System *system, int steps);
+/* Output files of blade_run_dynamics, as the reference's run module writes
+   them (run/run.cu:572-590, io/io.cxx:350-387): every `freq` steps one
+   .nrg line (step, then every energy term, " %12.4f") or one text .lmd
+   line (step, then lambda of every block after the environment, " %8.6f").
+   Energies are computed on the .nrg steps.  path NULL or freq 0 closes. */
+int blade_set_output_nrg(System *system, const char *path, int freq);
+int blade_set_output_lmd(System *system, const char *path, int freq);
+/* Binary checkpoint of the exact FP64 state (step, box, positions,
+   velocities, theta, theta velocities), replacing the reference's "%f"
+   text checkpoint (io/io.cxx:416-536).  A run continued from a checkpoint
+   written on a list rebuild step (a multiple of the domdec cadence) is
+   bit-identical to the uninterrupted run.  Read it after setup and before
+   the first energy or dynamics call. */
+int blade_write_checkpoint(System *system, const char *path);
+int blade_read_checkpoint(System *system, const char *path);
 /* directed tiles per nonbonded work unit, 1..16 (default 16); no
    counterpart in the reference */
 void blade_set_tile_chunk(System *system, int tiles);

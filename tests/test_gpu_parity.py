@@ -201,3 +201,40 @@ def test_short_trajectory_tracks_oracle(name):
     d = np.abs(xg - x[:3 * n])
     print("\nmax |x_gpu - x_oracle| after 5 steps: %.3e A" % d.max(), flush=True)
     assert d.max() < 1e-4
+
+
+def test_checkpoint_restart_is_bit_identical_and_outputs(tmp_path):
+    """40 steps in one run against 20 steps, a binary checkpoint (written on
+    a list rebuild step), a fresh system reading it and 20 more steps: the
+    states are bit-identical.  The run's .nrg and .lmd files hold the
+    reference's line layout at their steps: the potential column is the sum
+    of the term columns and the total is potential + kinetic."""
+    sp = relaxed("msld_small")
+    a = system.build(sp)
+    a.set_output(nrg=tmp_path / "a.nrg", lmd=tmp_path / "a.lmd", freq_nrg=10, freq_lmd=5)
+    a.run_dynamics(40, freq_nrg=10)
+    b = system.build(sp)
+    b.run_dynamics(20, freq_nrg=10)
+    b.write_checkpoint(tmp_path / "ck.bin")
+    b.close()
+    c = system.build(sp)
+    c.read_checkpoint(tmp_path / "ck.bin")
+    c.run_dynamics(20, freq_nrg=10)
+    xa, va, _, _ = system.state(a, sp.n, sp.nblocks)
+    xc, vc, _, _ = system.state(c, sp.n, sp.nblocks)
+    assert np.array_equal(xa, xc) and np.array_equal(va, vc)
+    assert np.array_equal(a.lambdas(), c.lambdas())
+    a.set_output()                      # closes (and flushes) the files
+    nrg = [l.split() for l in open(tmp_path / "a.nrg")]
+    lmd = [l.split() for l in open(tmp_path / "a.lmd")]
+    assert [int(l[0]) for l in nrg] == [0, 10, 20, 30]
+    assert all(len(l) == 1 + 23 for l in nrg)
+    for l in nrg:
+        e = np.array([float(v) for v in l[1:]])
+        assert np.all(np.isfinite(e)) and e[20] != 0.0
+        assert abs(e[:20].sum() - e[20]) < 2e-3
+        assert abs(e[20] + e[21] - e[22]) < 2e-4
+    assert [int(l[0]) for l in lmd] == list(range(0, 40, 5))
+    assert all(len(l) == sp.nblocks for l in lmd)
+    lam = [float(v) for v in lmd[-1][1:]]
+    assert abs(sum(lam) - 1.0) < 1e-5
