@@ -51,6 +51,20 @@ def profiled_traffic():
     return int(m.group(1)) if m else None
 
 
+def profiled_valu_insts():
+    """VALU wave-instructions per nbdirect launch from the committed
+    rocprofv3 SQ counter summary, or None"""
+    import re
+    path = os.path.join(ROOT, "profiles", "valu_nbdirect_msld23558.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^nbdirect_valu_insts_per_launch:\s*(\d+)", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
+PEAK_VALU_ISSUE = 256 * 2.4e9   # wave64 VALU instructions per second: one per CU per cycle
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +191,13 @@ def main():
     pairs = orc.forces(top, rp, x, lam, orc.NBDIRECT)[3]
     flops = FLOP_PER_PAIR * pairs
     achieved = flops / (ms_nb * 1e-3)
+    # the issue-rate view of the same launch: VALU wave-instructions per
+    # launch (SQ_INSTS_VALU from the committed counter summary) over this
+    # run's launch time, against one instruction per CU per cycle
+    vi = profiled_valu_insts()
+    valu_issue = None if vi is None else {
+        "insts_per_launch": vi, "achieved_G_per_s": round(vi / (ms_nb * 1e-3) / 1e9, 1),
+        "peak_G_per_s": round(PEAK_VALU_ISSUE / 1e9, 1), "frac": round(vi / (ms_nb * 1e-3) / PEAK_VALU_ISSUE, 4)}
 
     cpu = None
     if not a.no_cpu and a.cpu_steps > 0:
@@ -227,7 +248,8 @@ def main():
                      "traffic": profiled_traffic(),
                      "note": "compute roof; nbdirect runs on the FP32 vector ALUs, whose peak equals "
                              "the listed FP32 figure",
-                     "work": "%d in-cutoff pairs x %d FLOP per launch" % (pairs, FLOP_PER_PAIR)},
+                     "work": "%d in-cutoff pairs x %d FLOP per launch" % (pairs, FLOP_PER_PAIR),
+                     "valu_issue": valu_issue},
         "cpu_baseline": cpu,
     }
     s.close()
