@@ -32,6 +32,7 @@ PEAK_HBM = 8.0e12
 FREQ_NRG = 1000
 BATCH_STEPS, BATCH_WARMUP = 300, 30   # the default line's batch figure
 PREC_STEPS, PREC_WARMUP = 1000, 100   # the default line's precision-form figure
+COMB_STEPS, COMB_WARMUP = 400, 40     # the default line's combined-form figure
 CAL_BYTES = 256 << 20
 
 
@@ -90,6 +91,10 @@ def main():
                     help="relaxed precision-form state (npz): loaded if present, else written after relaxing")
     ap.add_argument("--prec-fft", choices=("both", "mfma", "fp32"), default="both",
                     help="FFT forms timed in the precision-form run (errors are measured on the MFMA one)")
+    ap.add_argument("--combined", action="store_true",
+                    help="print only the combined-form line (8 lambda-replicas of a 60,000-atom four-site box)")
+    ap.add_argument("--no-combined", action="store_true",
+                    help="leave the combined-form figure out of the default line")
     ap.add_argument("--no-precision", action="store_true",
                     help="leave the precision-form figure out of the default line")
     ap.add_argument("--no-tune", action="store_true",
@@ -109,6 +114,9 @@ def main():
     from blade_amd._ffi import check, ptr
     if a.batch:
         return batch_main(a)
+    if a.combined:
+        print(json.dumps(combined_run(a.steps, a.warmup)), flush=True)
+        return 0
     if a.precision:
         print(json.dumps(precision_run(a.steps, a.warmup, graph=not a.eager, state=a.prec_state,
                                        modes={"both": (1, 0), "mfma": (1,), "fp32": (0,)}[a.prec_fft])),
@@ -271,6 +279,11 @@ def main():
             line["precision_92224"] = precision_run(PREC_STEPS, PREC_WARMUP)
         except Exception as e:
             line["precision_92224"] = {"error": str(e)[:300]}
+    if not a.no_combined:
+        try:
+            line["combined_8x60000"] = combined_run(COMB_STEPS, COMB_WARMUP)
+        except Exception as e:
+            line["combined_8x60000"] = {"error": str(e)[:300]}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -444,6 +457,94 @@ def batch_run(nsys, steps, warmup, state_dir=None, autotune=True):
         s.close()
     return line
 
+
+
+def combined_run(steps, warmup, nrep=8, frames=20, frame_every=10):
+    """The combined form: nrep lambda-replicas of the 60,000-atom box with
+    four lambda sites of 4-8 substituents (27 blocks; soft core, atom
+    restraints, per-block fixed biases, PME 80^3 on the MFMA FFT), the
+    structure relaxed once on the GPU and each replica started from its own
+    theta, biases and velocities.  Timed one by one (each replica's own
+    graphs, in turn) and batched (one graph launch per step for all), the
+    same steps; then frames every frame_every steps give per-block lambda
+    means over all replicas, and replica 0's last frame is checked against
+    the FP64 oracle (forces and dU/dlambda)."""
+    from blade_amd import synthetic, system
+    from oracle import oracle as orc
+    t0 = time.time()
+    base = synthetic.config5(0)
+    log("generated %s: %d atoms, %d blocks in %.1f s" % (base.name, base.n, base.nblocks, time.time() - t0))
+    t0 = time.time()
+    base = system.relax(base)
+    log("relaxed in %.1f s" % (time.time() - t0))
+    specs = [synthetic.replica_of(base, r) for r in range(nrep)]
+    dt_fs = base.dt / synthetic.PICOSECOND * 1000.0
+
+    def timed(run_one_by_one):
+        systems = [system.build(sp) for sp in specs]
+        for s in systems:
+            s.run_energy()
+        if run_one_by_one:
+            for s in systems:
+                check(s.L.bm_dynamics(s.device(), 1, warmup, FREQ_NRG, 1), "warmup")
+                check(s.L.bm_sync(s.device()), "sync")
+            t1 = time.perf_counter()
+            for s in systems:
+                check(s.L.bm_dynamics(s.device(), 1 + warmup, steps, FREQ_NRG, 1), "plain steps")
+                check(s.L.bm_sync(s.device()), "sync")
+            dt = time.perf_counter() - t1
+            for s in systems:
+                s.close()
+            return dt, None, None
+        b = system.Batch(systems)
+        b.dynamics(1, warmup, FREQ_NRG)
+        b.sync()
+        t1 = time.perf_counter()
+        b.dynamics(1 + warmup, steps, FREQ_NRG)
+        b.sync()
+        return time.perf_counter() - t1, b, systems
+
+    from blade_amd._ffi import check
+    t_plain, _, _ = timed(True)
+    t_batch, b, systems = timed(False)
+    # lambda frames: every frame_every steps, all replicas
+    lam = []
+    step = 1 + warmup + steps
+    for f in range(frames):
+        b.dynamics(step, frame_every, 0)
+        b.sync()
+        step += frame_every
+        lam += [s.lambdas() for s in systems]
+    lam = np.array(lam)
+    # replica 0's last frame against the oracle (a fresh system at that state)
+    x, v, th, tv = system.state(systems[0], base.n, base.nblocks)
+    b.close()
+    for s in systems:
+        s.close()
+    import copy
+    sp0 = copy.copy(specs[0])
+    sp0.pos, sp0.vel, sp0.theta0, sp0.theta_v = x, v, list(th[:base.nblocks]), list(tv[:base.nblocks])
+    s = system.build(sp0)
+    s.run_energy()
+    top, rp = s.topology(), s.runparams()
+    lo = orc.theta_to_lambda(top, rp, np.array(sp0.theta0, float))
+    fo, do, eo, _ = orc.forces(top, rp, s.positions(), lo)
+    fg, dg = s.forces(), s.dudl()
+    err = {"force_rel_rms": float(np.sqrt(np.mean((fg - fo) ** 2)) / np.sqrt(np.mean(fo ** 2))),
+           "dudl_max_abs_kcal": float(np.max(np.abs(dg - do)))}
+    s.close()
+    agg = lambda t: nrep * steps / t * dt_fs * 86400e-6
+    return {"metric": "aggregate ns/day at 2 fs, %d lambda-replicas of the 60,000-atom four-site box" % nrep,
+            "value": round(agg(t_batch), 2), "unit": "ns/day", "ms_per_step": round(t_batch * 1e3 / steps, 5),
+            "steps": steps, "warmup": warmup,
+            "one_by_one": round(agg(t_plain), 2), "batched_over_one_by_one": round(t_plain / t_batch, 3),
+            "lambda_mean_per_block": [round(float(m), 3) for m in lam.mean(axis=0)],
+            "lambda_frames": int(lam.shape[0]),
+            "errors_vs_fp64_oracle": {k: float("%.3e" % v) for k, v in err.items()},
+            "config": {"workload": "config5-combined-8x60000", "atoms_per_replica": base.n, "replicas": nrep,
+                       "lambda_sites": 4, "blocks": base.nblocks, "box_A": list(base.box),
+                       "pme_grid": list(base.grid), "pme_fft": "MFMA split-BF16x3 DFT",
+                       "cutoff_A": base.rcut, "switch_A": base.rswitch, "dt_fs": dt_fs}}
 
 if __name__ == "__main__":
     sys.exit(main())

@@ -18,6 +18,12 @@ Workloads (sizes as named in the benchmark descriptions):
                 of 1.5-3 kDa, two of them carrying a lambda site of 4
                 substituents x 8 atoms; switch 9-10 A, PME 96^3 order 6 with
                 the split-BF16x3 MFMA FFT
+  config5       one of 8 lambda-replicas of a 60,000-atom box: TIP3P water
+                plus a 6,000-atom set of seeded chains, four of them
+                carrying a lambda site of 4-8 substituents (up to 29
+                blocks), per-block fixed biases, switch 9-10 A, PME 80^3
+                order 6 on the MFMA FFT; replicas of one relaxed box differ
+                in biases and velocities
   msld_small    a few-hundred-atom water box with a 3-substituent site
                 (fast parity cases)
   two_site_small  a thousand-atom box with two chains, each carrying a
@@ -638,23 +644,24 @@ def _add_ions(spec, rng, npairs, grid, clearance=3.0):
         resid += 1
 
 
-def _chains(spec, rng, sizes, grid, site_nsubs):
+def _chains(spec, rng, sizes, grid, site_nsubs, rscale=28.0, sep=1.0):
     """seeded chains of the given atom counts, each confined to a sphere
     (the polymer density of the 23,558-atom box) at a random centre whose
-    sphere overlaps no earlier one; chain c carries a lambda site of
+    sphere overlaps no earlier one (centres at least sep times the sum of
+    the radii apart); chain c carries a lambda site of
     site_nsubs[c] substituents when that entry is nonzero.  Returns the
     sites (lists of substituent atom lists)."""
     L = np.array(spec.box)
     sites, placed = [], []
     for c, n in enumerate(sizes):
-        radius = max(7.0, 28.0 * (n / 2489.0) ** (1 / 3))
+        radius = max(7.0, rscale * (n / 2489.0) ** (1 / 3))
         for _ in range(10000):
             center = rng.uniform(0.0, 1.0, 3) * L
             ok = True
             for c0, r0 in placed:
                 d = center - c0
                 d -= L * np.round(d / L)
-                ok = ok and float(np.linalg.norm(d)) > radius + r0
+                ok = ok and float(np.linalg.norm(d)) > sep * (radius + r0)
             if ok:
                 break
         else:
@@ -710,6 +717,57 @@ def config3(seed=0xB1ADE003, box=(96.0, 96.0, 100.0), n_chains=16, chain_atoms=5
     spec.fft_mode = 1
     spec.temperature = 298.0
     spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
+
+
+def config5(replica=0, seed=0xB1ADE005, natoms=60000, chain_atoms=6000, n_chains=10, box=84.8):
+    """One lambda-replica of the combined workload.  The structure (chains,
+    sites, water) comes from `seed` alone, so every replica is the same box;
+    the replica index seeds the initial theta, the per-block fixed biases
+    (N(0, 1) kcal/mol) and the velocities.  Sites use the reference decks'
+    options: soft core, only dihedrals lambda-scaled, each site's
+    substituents' first atoms restrained together."""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name="config5-combined-60000-r%d" % replica, box=(box,) * 3)
+    spec._pos = []
+    _water_params(spec)
+    grid = _Grid(spec.box, 3.0)
+    sizes = rng.integers(500, 701, n_chains).astype(float)
+    sizes = np.floor(sizes * chain_atoms / sizes.sum()).astype(int)
+    sizes[-1] += chain_atoms - int(sizes.sum())
+    nsubs = [int(x) for x in rng.integers(4, 9, 4)]
+    sites = _chains(spec, rng, [int(x) for x in sizes], grid, nsubs, rscale=26.0)
+    assert spec.n == chain_atoms, (spec.n, chain_atoms)
+    _angles_dihes(spec, _sites_block_of(spec, sites))
+    _fill_water(spec, rng, (natoms - chain_atoms) // 3, grid)
+    spec.pos = np.array(spec._pos) % np.array(spec.box)
+    _msld_sites(spec, sites, fnex=5.0)
+    spec.soft_core = True
+    spec.scale_terms = (0, 0, 0, 1, 0, 1)
+    spec.atom_restraints = [[subs[0] for subs in site] for site in sites]
+    spec.rcut, spec.rswitch = 10.0, 9.0
+    spec.beta = 1.0 / 3.2
+    spec.grid, spec.order = (80, 80, 80), 6
+    spec.fft_mode = 1
+    spec.temperature = 298.0
+    return replica_of(spec, replica, seed, new_theta=True)
+
+
+def replica_of(base, replica, seed=0xB1ADE005, new_theta=False):
+    """a lambda-replica of `base` (positions kept): fixed biases N(0, 1)
+    kcal/mol and Maxwell velocities from seed + 1000 + replica, and with
+    new_theta an initial theta uniform in [-1, 1] (otherwise the base's
+    theta, so a relaxed base stays consistent with its lambdas)"""
+    import copy
+    spec = copy.copy(base)
+    spec.name = base.name.rsplit("-r", 1)[0] + "-r%d" % replica
+    rr = np.random.default_rng(seed + 1000 + replica)
+    nb = spec.nblocks
+    theta = [0.0] + [float(x) for x in rr.uniform(-1.0, 1.0, nb - 1)]
+    if new_theta:
+        spec.theta0, spec.theta_v = theta, None
+    spec.fixed_bias = [0.0] + [float(x) for x in rr.normal(0.0, 1.0, nb - 1)]
+    spec.vel = _maxwell(rr, [a[6] for a in spec.atoms], spec.temperature)
     return spec
 
 

@@ -18,6 +18,8 @@ def spec(name):
         return synthetic.config2(seed=0xB1ADE0B0, n_water=1200, n_poly=400, box=36.0)
     if name == "config3":
         return synthetic.config3()
+    if name == "config5":
+        return synthetic.config5(0)
     if name == "two_site_small":
         return synthetic.two_site_small()
     raise KeyError(name)

@@ -24,10 +24,12 @@ def _members():
 
 
 def test_batch_members_equal_solo_runs():
-    """members with different tile chunks (4, default 16, 8): each keeps its
-    own in the batch"""
+    """members with different tile chunks (4, default 16, 8) each keep their
+    own in the batch; the second member runs its PME FFT on the matrix cores
+    (32^3), outside the fused FP32 FFT launches"""
     specs = [copy.copy(sp) for sp in _members()]
     specs[0].tile_chunk, specs[2].tile_chunk = 4, 8
+    specs[1].fft_mode, specs[1].grid = 1, (32, 32, 32)
     solo = []
     for sp in specs:
         s = system.build(sp)

@@ -110,6 +110,27 @@ def test_config3_precision_form():
     assert ddl <= DUDL_TOL
 
 
+def test_config5_four_site_replica():
+    """one replica of the combined workload (60,000 atoms, four lambda sites
+    of 6, 6, 6 and 8 substituents: 27 blocks, soft core, atom restraints,
+    per-block fixed biases, PME 80^3 on the MFMA FFT) from the GPU-relaxed
+    state: forces and dU/dlambda keep the path's bars"""
+    sp = relaxed("config5")
+    assert sp.nblocks == 27 and sp.fft_mode == 1
+    s = system.build(sp)
+    s.run_energy()
+    top, rp = s.topology(), s.runparams()
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    fo, do, eo, pairs = orc.forces(top, rp, s.positions(), lam)
+    fg, eg, dg = s.forces(), s.energy(), s.dudl()
+    frms = rel_rms(fg, fo)
+    ddl = float(np.max(np.abs(dg - do)))
+    print("\n[config5] N=%d blocks=%d force relRMS %.3e potential rel %.3e max |dU/dl| err %.2e"
+          % (sp.n, sp.nblocks, frms, abs(eg[20] - eo[20]) / abs(eo[20]), ddl), flush=True)
+    assert frms <= FORCE_TOL
+    assert ddl <= DUDL_TOL
+
+
 def test_domdec_bit_exact():
     for name in ("config1", "msld_small", "config2_small", "two_site_small"):
         s = built(name)
