@@ -199,6 +199,7 @@ def lib():
         "bm_batch_create": (i, [i, C.POINTER(vp), C.POINTER(vp)]),
         "bm_batch_dynamics": (i, [vp, C.c_long, C.c_long, i]),
         "bm_batch_sync": (i, [vp]),
+        "bm_batch_exchange": (i, [vp, C.c_long, i, C.c_uint64, _i32p, _i32p, _f64p]),
         "bm_batch_destroy": (i, [vp]),
         "bm_sync": (i, [vp]),
         "bm_get_positions": (i, [vp, _f64p, _f64p]),

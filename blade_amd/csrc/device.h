@@ -183,6 +183,24 @@ __device__ __forceinline__ void nan_check(const Dev &d, double v, int i) {
   if (!isfinite(v)) atomicCAS(d.nanflag, -1, i);
 }
 
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -----------------
+__host__ __device__ inline uint4 philox(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+    unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+    uint4 o;
+    o.x = (unsigned)(p1 >> 32) ^ c.y ^ k.x;
+    o.y = (unsigned)p1;
+    o.z = (unsigned)(p0 >> 32) ^ c.w ^ k.y;
+    o.w = (unsigned)p0;
+    c = o;
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
 // minimum-image displacement from fractional coordinates (int wrap)
 __device__ __forceinline__ float fdisp(unsigned a, unsigned b, float scale) {
   return (float)(int)(a - b) * scale;

@@ -75,6 +75,7 @@ struct bm_system {
   bool alch = false, soft = false, bias = false;
   int nwgNb = 0;
   int freqDomdec = 10;
+  double kT = 0.0;                   // kB T of the thermostat (replica exchange)
   std::vector<void *> allocs;
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -203,6 +204,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   s->soft = s->alch && rp->soft_core;
   s->bias = s->alch;
   s->freqDomdec = rp->freq_domdec > 0 ? rp->freq_domdec : 10;
+  s->kT = kBoltzD * rp->temperature;
   // columns of about 32 atoms' volume (domdec/domdec.cu:110-118)
   {
     double V = rp->box[0] * rp->box[1] * rp->box[2];
@@ -1028,6 +1030,81 @@ int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
   HIPCHK(hipEventRecord(b->fork, b->st));
   for (bm_system *s : b->sys) HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
   return BM_OK;
+}
+
+// potential energy of a member at its current positions (a list rebuild,
+// then one energy evaluation)
+static int member_energy(bm_system *s, double &e) {
+  double E[BM_NTERMS];
+  int rc = bm_domdec_update(s, 1);
+  if (!rc) rc = bm_calc_force(s, 0, BM_CALC_ENERGY);
+  if (!rc) rc = bm_sync(s);
+  if (!rc) rc = bm_get_energy(s, E);
+  e = E[BM_EPOTENTIAL];
+  return rc;
+}
+
+// swap a device array of two members through a scratch buffer
+static int swap_dev(bm_system *a, bm_system *b, double *pa, double *pb, double *tmp, size_t n, hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(tmp, pa, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(pa, pb, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(pb, tmp, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  (void)a;
+  (void)b;
+  return BM_OK;
+}
+
+int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t *replica, int32_t *accepted,
+                      double *dW) {
+  if (!b || (parity != 0 && parity != 1)) return set_err(BM_EINVAL, "exchange arguments");
+  const int n = (int)b->sys.size();
+  for (int k = 1; k < n; k++)
+    if (b->sys[k]->d.n != b->sys[0]->d.n || b->sys[k]->d.nl != b->sys[0]->d.nl)
+      return set_err(BM_EINVAL, "replica exchange needs members of equal atom and block counts");
+  if (int rc = bm_batch_sync(b)) return rc;
+  HIPCHK(hipSetDevice(b->device));
+  const size_t N = 3 * (size_t)b->sys[0]->d.n + b->sys[0]->d.nl;
+  double *tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, N * sizeof(double)));
+  std::vector<double> e0(n, 0.0), e1(n, 0.0);
+  int rc = BM_OK;
+  for (int k = 0; k < n && !rc; k++) rc = member_energy(b->sys[k], e0[k]);
+  auto swap_pos = [&](bm_system *x, bm_system *y) {
+    int r = swap_dev(x, y, x->d.pos, y->d.pos, tmp, N, b->st);
+    if (!r && x->alch) blade::launch_theta_to_lambda(x->d, b->st);
+    if (!r && y->alch) blade::launch_theta_to_lambda(y->d, b->st);
+    if (!r && hipStreamSynchronize(b->st) != hipSuccess) r = BM_EHIP;
+    return r;
+  };
+  for (int k = parity; k + 1 < n && !rc; k += 2) rc = swap_pos(b->sys[k], b->sys[k + 1]);
+  for (int k = 0; k < n && !rc; k++)
+    if (k >= parity && k - parity < 2 * ((n - parity) / 2)) rc = member_energy(b->sys[k], e1[k]);
+  for (int k = parity; k + 1 < n && !rc; k += 2) {
+    bm_system *x = b->sys[k], *y = b->sys[k + 1];
+    const double w = (e1[k] - e0[k]) / x->kT + (e1[k + 1] - e0[k + 1]) / y->kT;
+    // uniform in (0,1) from (seed, step, pair)
+    const uint4 o = blade::philox(make_uint4((unsigned)k, (unsigned)step, (unsigned)((unsigned long long)step >> 32),
+                                             0x52455821u),
+                                  make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
+    const double u = ((double)((((unsigned long long)o.x) << 21) ^ (o.y >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+    const bool acc = u < std::exp(-w);
+    if (dW) dW[k] = w;
+    if (accepted) accepted[k] = acc ? 1 : 0;
+    if (acc) {
+      rc = swap_dev(x, y, x->d.vel, y->d.vel, tmp, N, b->st);
+      if (replica) std::swap(replica[k], replica[k + 1]);
+    } else {
+      rc = swap_pos(x, y);
+    }
+  }
+  // every member's list from its final positions, for the steps that follow
+  for (int k = 0; k < n && !rc; k++) {
+    rc = bm_domdec_update(b->sys[k], 1);
+    if (!rc) rc = bm_sync(b->sys[k]);
+  }
+  (void)hipFree(tmp);
+  return rc;
 }
 
 int bm_batch_sync(bm_batch *b) {

@@ -33,24 +33,6 @@ __device__ __forceinline__ D3 minimg(const Dev &d, D3 v) {
   return d3(wrapd(v.x, d.box[0], d.invbox[0]), wrapd(v.y, d.box[1], d.invbox[1]), wrapd(v.z, d.box[2], d.invbox[2]));
 }
 
-// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -----------------
-__device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
-#pragma unroll
-  for (int r = 0; r < 10; r++) {
-    unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
-    unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
-    uint4 o;
-    o.x = (unsigned)(p1 >> 32) ^ c.y ^ k.x;
-    o.y = (unsigned)p1;
-    o.z = (unsigned)(p0 >> 32) ^ c.w ^ k.y;
-    o.w = (unsigned)p0;
-    c = o;
-    k.x += 0x9E3779B9u;
-    k.y += 0xBB67AE85u;
-  }
-  return c;
-}
-
 // standard normals of DOFs 2p and 2p+1 for (seed, step): the same rule as
 // the oracle (one Philox block, Box-Muller cosine for the even DOF, sine for
 // the odd one)

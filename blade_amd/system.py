@@ -367,6 +367,19 @@ class Batch:
     def sync(self):
         check(self.L.bm_batch_sync(self.h), "bm_batch_sync")
 
+    def exchange(self, step, parity, seed, replica):
+        """one round of neighbour replica exchange (bm_batch_exchange):
+        returns (replica labels, accepted per lower member, dW per lower
+        member)"""
+        n = len(self.systems)
+        rep = np.ascontiguousarray(replica, dtype=np.int32).copy()
+        acc = np.zeros(n, dtype=np.int32)
+        dw = np.zeros(n)
+        i32 = C.POINTER(C.c_int32)
+        check(self.L.bm_batch_exchange(self.h, int(step), int(parity), int(seed), rep.ctypes.data_as(i32),
+                                       acc.ctypes.data_as(i32), ptr(dw)), "bm_batch_exchange")
+        return rep, acc, dw
+
     def close(self):
         if self.h:
             self.L.bm_batch_destroy(self.h)

@@ -242,6 +242,21 @@ int bm_batch_create(int n, bm_system *const *systems, bm_batch **out);
 int bm_batch_dynamics(bm_batch *batch, long step0, long nsteps, int freq_nrg);
 /* synchronise every member; BM_ENAN / BM_EOVERFLOW as bm_sync reports them */
 int bm_batch_sync(bm_batch *batch);
+/* Replica exchange between neighbouring members (update/rex.cxx:15-125,
+ * with the MPI ranks replaced by the members of one batch): members k and
+ * k+1 pair up for k even (parity 0) or odd (parity 1); each pair's
+ * positions and theta are swapped on the device and
+ *   dW = (E_k(x_k+1) - E_k(x_k)) / kT_k + (E_k+1(x_k) - E_k+1(x_k+1)) / kT_k+1
+ * from potential energies evaluated before and after the swap; the swap is
+ * kept with probability min(1, exp(-dW)) (uniform from Philox keyed by
+ * (seed, step, k)), with the velocities swapped too and replica[k],
+ * replica[k+1] exchanged; otherwise the positions are swapped back.  Each
+ * member's list is rebuilt afterwards.  accepted[k] and dW[k] are written
+ * for the lower member of each pair (either may be NULL).  Members need
+ * equal atom and block counts.  No counterpart in the reference's
+ * single-system library (its exchange runs over MPI, one rank per system). */
+int bm_batch_exchange(bm_batch *batch, long step, int parity, uint64_t seed,
+                      int32_t *replica, int32_t *accepted, double *dW);
 int bm_batch_destroy(bm_batch *batch);
 
 /* synchronise the system's stream */

@@ -81,3 +81,78 @@ def test_batch_autotune_sets_a_measured_choice():
     b.close()
     for s in systems:
         s.close()
+
+
+def test_replica_exchange_swaps_and_metropolis_energy():
+    """Neighbour replica exchange inside a batch: three replicas of one box
+    that differ only in per-block fixed biases, so the exchange energy has a
+    closed form, dW = (b_k - b_k+1) . (lambda(x_k+1) - lambda(x_k)) / kT.
+    Replicas 0 and 1 share their biases (dW = 0: always accepted, so their
+    positions, theta and velocities trade places exactly); replica 2 sits
+    out the parity-0 round unchanged; the parity-1 round pairs 1 and 2 and
+    leaves both in a state that is either swapped or restored, bit for bit,
+    as its acceptance says."""
+    base = relaxed("msld_small")
+    specs = []
+    for k, bias in enumerate(([0.0, 1.0, -0.5, 0.3], [0.0, 1.0, -0.5, 0.3], [0.0, -2.0, 3.0, 1.0])):
+        sp = copy.copy(base)
+        sp.fixed_bias = list(bias)
+        sp.vel = synthetic._maxwell(np.random.default_rng(70 + k), [a[6] for a in sp.atoms], sp.temperature)
+        if k == 2:                  # its own lambdas, so the parity-1 dW is not small
+            sp.theta0 = [t + 0.8 * (j % 2 * 2 - 1) for j, t in enumerate(sp.theta0)]
+            sp.theta_v = None
+        specs.append(sp)
+    systems = [system.build(sp) for sp in specs]
+    for s in systems:
+        s.run_energy()
+    b = system.Batch(systems)
+    b.dynamics(1, 10, 0)
+    b.sync()
+    n, nb = specs[0].n, specs[0].nblocks
+    kT = synthetic.KBOLTZ * specs[0].temperature
+
+    L = np.array(specs[0].box, float)        # positions come back as (N, 3)
+
+    def wrap(x):
+        # the list rebuild's wrap into [0, L) (x - floor(x / L) L), which the
+        # exchange's energy evaluations apply
+        return x - np.floor(x * (1.0 / L)) * L
+
+    def states():
+        st = []
+        for s in systems:
+            x, v, th, tv = system.state(s, n, nb)
+            st.append((wrap(x), v, th, tv))
+        return st, [s.lambdas() for s in systems]
+
+    before, lam = states()
+    rep, acc, dw = b.exchange(10, 0, 1234, [0, 1, 2])
+    after, _ = states()
+    print("\n[rex parity 0] accepted %s dW %s" % (acc[:2], dw[:2]), flush=True)
+    assert acc[0] == 1 and abs(dw[0]) < 1e-9 and list(rep) == [1, 0, 2]
+    for q in range(4):
+        assert np.array_equal(after[0][q], before[1][q]) and np.array_equal(after[1][q], before[0][q])
+        assert np.array_equal(after[2][q], before[2][q])
+    before, lam = after, [s.lambdas() for s in systems]
+    rep2, acc2, dw2 = b.exchange(10, 1, 1234, rep)
+    after, _ = states()
+    bias1, bias2 = np.array(specs[1].fixed_bias), np.array(specs[2].fixed_bias)
+    want = float((bias1 - bias2) @ (lam[2] - lam[1])) / kT
+    print("[rex parity 1] accepted %d dW %.6f closed form %.6f" % (acc2[1], dw2[1], want), flush=True)
+    assert abs(dw2[1] - want) <= 1e-6 * max(1.0, abs(want))
+    src = {1: 2, 2: 1} if acc2[1] else {1: 1, 2: 2}
+    for k in (1, 2):
+        for q in range(3):          # positions, velocities (swapped only if accepted), theta
+            if q == 1:
+                want_v = before[src[k]][1]
+                assert np.array_equal(after[k][1], want_v)
+            else:
+                assert np.array_equal(after[k][q], before[src[k]][q])
+    assert list(rep2) == ([1, 2, 0] if acc2[1] else [1, 0, 2])
+    b.dynamics(11, 10, 0)           # the exchanged batch keeps stepping
+    b.sync()
+    for s in systems:
+        assert np.all(np.isfinite(system.state(s, n, nb)[0]))
+    b.close()
+    for s in systems:
+        s.close()
