@@ -99,7 +99,7 @@ def test_replica_exchange_swaps_and_metropolis_energy():
         sp.fixed_bias = list(bias)
         sp.vel = synthetic._maxwell(np.random.default_rng(70 + k), [a[6] for a in sp.atoms], sp.temperature)
         if k == 2:                  # its own lambdas, so the parity-1 dW is not small
-            sp.theta0 = [t + 0.8 * (j % 2 * 2 - 1) for j, t in enumerate(sp.theta0)]
+            sp.theta0 = [t if j == 0 else t + 0.8 * (j % 2 * 2 - 1) for j, t in enumerate(sp.theta0)]
             sp.theta_v = None
         specs.append(sp)
     systems = [system.build(sp) for sp in specs]
