@@ -165,6 +165,7 @@ def lib():
         "blade_set_output_nrg": (i, [vp, C.c_char_p, i]),
         "blade_set_output_lmd": (i, [vp, C.c_char_p, i]),
         "blade_write_checkpoint": (i, [vp, C.c_char_p]),
+        "blade_get_npt_stats": (i, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), _f64p]),
         "blade_read_checkpoint": (i, [vp, C.c_char_p]),
         "blade_autotune": (i, [vp, i, C.POINTER(TuneResult)]),
         "blade_batch_autotune": (i, [i, C.POINTER(vp), i, C.POINTER(TuneResult)]),
@@ -199,6 +200,10 @@ def lib():
         "bm_batch_create": (i, [i, C.POINTER(vp), C.POINTER(vp)]),
         "bm_batch_dynamics": (i, [vp, C.c_long, C.c_long, i]),
         "bm_batch_sync": (i, [vp]),
+        "bm_volume_move": (i, [vp, _f64p]),
+        "bm_volume_restore": (i, [vp, _f64p]),
+        "bm_get_box": (i, [vp, _f64p]),
+        "bm_uniform": (C.c_double, [C.c_uint64, C.c_long, C.c_uint32]),
         "bm_batch_exchange": (i, [vp, C.c_long, i, C.c_uint64, _i32p, _i32p, _f64p]),
         "bm_batch_destroy": (i, [vp]),
         "bm_sync": (i, [vp]),

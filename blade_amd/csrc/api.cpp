@@ -44,6 +44,7 @@ struct System {
   // output files written by blade_run_dynamics (run/run.cu:572-590)
   FILE *fpNRG = nullptr, *fpLMD = nullptr;
   int freqNRG = 0, freqLMD = 0;
+  long nptMoves = 0, nptAccepted = 0;   // barostat counters
   ~System() {
     if (fpNRG) fclose(fpNRG);
     if (fpLMD) fclose(fpLMD);
@@ -257,7 +258,7 @@ void blade_add_run_flags(System *s, double gamma, double betaEwald, double rCut,
   r.shakeTolerance = shakeTolerance;
 }
 void blade_add_run_dynopts(System *s, int step, int step0, int nsteps, double dt, double T, int freqNPT,
-                           double, double) {
+                           double volumeFluctuation, double pressure) {
   auto &r = *s->run;
   r.step = step;
   r.step0 = step0;
@@ -265,7 +266,8 @@ void blade_add_run_dynopts(System *s, int step, int step0, int nsteps, double dt
   r.dt = dt;
   r.T = T;
   r.freqNPT = freqNPT;
-  if (freqNPT > 0) mark_unsupported(s, "constant pressure (freqNPT>0)");
+  r.volumeFluctuation = volumeFluctuation;
+  r.pressure = pressure;
 }
 
 void blade_set_seed(System *s, unsigned long long seed) { s->seed = seed; }
@@ -423,20 +425,96 @@ static void write_lmd(System *s, long step) {
 // the first step >= from that is a multiple of f (f > 0)
 static long next_multiple(long from, int f) { return ((from + f - 1) / f) * f; }
 
+// potential energy at the current positions (a list rebuild and one
+// energy evaluation)
+static int potential_now(bm_system *g, double &e) {
+  double E[BM_NTERMS];
+  int rc = bm_domdec_update(g, 1);
+  if (!rc) rc = bm_calc_force(g, 0, BM_CALC_ENERGY);
+  if (!rc) rc = bm_sync(g);
+  if (!rc) rc = bm_get_energy(g, E);
+  e = E[BM_EPOTENTIAL];
+  return rc;
+}
+
+// uniforms of the barostat, keyed by (seed, step, draw); stream tag "NPT"
+static double npt_uniform(unsigned long long seed, long step, unsigned k) {
+  return bm_uniform(seed, step, 0x4E505400u | k);
+}
+
+// Monte Carlo volume move (update/pressure.cu:15-185): Gaussian volume
+// draws (one for a cubic box, x=y and z for a tetragonal one, one per axis
+// otherwise), every atom scaled with constraint groups kept rigid, and the
+// move kept with probability min(1, exp(-dW/kT)),
+//   dW = E_new - E_old + P (V_new - V_old) - N kT log(V_new / V_old),
+// N = atoms - constraints; a rejected move restores positions and box.
+static int pressure_coupling(System *s, long step) {
+  bm_system *g = s->gpu;
+  const RunSetup &r = *s->run;
+  double box0[3];
+  int rc = bm_get_box(g, box0);
+  double e0 = 0.0, e1 = 0.0;
+  if (!rc) rc = potential_now(g, e0);
+  if (rc) return rc;
+  const double V0 = box0[0] * box0[1] * box0[2];
+  auto gauss = [&](unsigned k) {
+    const double u1 = npt_uniform(s->seed, step, 2 * k + 1), u2 = npt_uniform(s->seed, step, 2 * k + 2);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  };
+  double scale[3], vn[3];
+  const int name = s->coordinates->boxName;
+  for (int k = 0; k < 3; k++) {
+    const int draw = name == 0 ? 0 : (name == 1 ? (k < 2 ? 0 : 1) : k);
+    vn[k] = V0 + r.volumeFluctuation * gauss((unsigned)draw);
+  }
+  for (int k = 0; k < 3; k++) {
+    if (!(vn[k] > 0)) return BM_OK;   // no move
+    scale[k] = std::exp(std::log(vn[k] / V0) / 3);
+  }
+  const double V1 = V0 * scale[0] * scale[1] * scale[2];
+  rc = bm_volume_move(g, scale);
+  if (rc == BM_EINVAL) return BM_OK;   // the box would pass 2*rcut: no move
+  if (!rc) rc = potential_now(g, e1);
+  if (rc) return rc;
+  const bm_topology &t = s->topo.top;
+  const double N = t.n_atoms - (2.0 * t.n_tri + t.n_br1 + 2.0 * t.n_br2 + 3.0 * t.n_br3);
+  const double kT = 0.001987191 * r.T;
+  const double dW = e1 - e0 + r.pressure * (V1 - V0) - N * kT * std::log(V1 / V0);
+  s->nptMoves++;
+  if (npt_uniform(s->seed, step, 0) < std::exp(-dW / kT)) {
+    s->nptAccepted++;
+    double b[3];
+    bm_get_box(g, b);
+    for (int k = 0; k < 3; k++) s->rp.box[k] = s->coordinates->box[k] = b[k];
+  } else {
+    rc = bm_volume_restore(g, box0);
+  }
+  if (!rc) rc = bm_domdec_update(g, 1);
+  if (!rc) rc = bm_sync(g);
+  return rc;
+}
+
 int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
   if (!s || s->failed) return BM_EINVAL;
   int rc = ensure_gpu(s);
   if (rc) return rc;
-  if (s->fpNRG || s->fpLMD) {
+  const int fnpt = s->run->freqNPT;
+  if (s->fpNRG || s->fpLMD || fnpt > 0) {
     // segments that end on output steps; energies on the .nrg steps
-    // (run/run.cu:545-570 prints after each step's update)
+    // (run/run.cu:545-570 prints after each step's update); a barostat
+    // move before every step that is a multiple of freqNPT
     const int fe = s->fpNRG ? s->freqNRG : freqNRG;
     const long end = s->step + nsteps;
     long t = s->step;
     while (!rc && t < end) {
+      if (fnpt > 0 && t % fnpt == 0) {
+        rc = pressure_coupling(s, t);
+        if (rc) break;
+      }
       long stop = end - 1;
       if (s->fpNRG) stop = std::min(stop, next_multiple(t, s->freqNRG));
       if (s->fpLMD) stop = std::min(stop, next_multiple(t, s->freqLMD));
+      if (fnpt > 0) stop = std::min(stop, next_multiple(t + 1, fnpt) - 1);
       rc = bm_dynamics(s->gpu, t, stop - t + 1, fe, use_graph);
       if (!rc) rc = bm_sync(s->gpu);
       if (!rc && s->fpNRG && stop % s->freqNRG == 0) write_nrg(s, stop);
@@ -568,6 +646,17 @@ int blade_read_checkpoint(System *s, const char *path) {
   }
   s->run->step = step;
   s->prepared = false;   // the box may differ from the setup's
+  return BM_OK;
+}
+
+int blade_get_npt_stats(System *s, long *moves, long *accepted, double *box) {
+  if (!s) return BM_EINVAL;
+  if (moves) *moves = s->nptMoves;
+  if (accepted) *accepted = s->nptAccepted;
+  if (box) {
+    if (s->gpu) return bm_get_box(s->gpu, box);
+    for (int k = 0; k < 3; k++) box[k] = s->coordinates ? s->coordinates->box[k] : 0.0;
+  }
   return BM_OK;
 }
 

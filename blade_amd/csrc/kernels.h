@@ -25,6 +25,10 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st);
 // needs launch_normals of the same step first
 void launch_update_after_kick(const Dev &d, hipStream_t st);
 void launch_theta_to_lambda(const Dev &d, hipStream_t st);
+// MC barostat: back up and scale every atom, rectify constraint groups
+// (oldbox: the box before the move); restore the backup
+void launch_volume_move(const Dev &d, const double *scale, const double *oldbox, hipStream_t st);
+void launch_volume_restore(const Dev &d, hipStream_t st);
 void launch_set_step(const Dev &d, long long step, hipStream_t st);
 // the FFT part of launch_pme_grid alone
 void launch_pme_fft(const Dev &d, bool energy, hipStream_t st);

@@ -1032,6 +1032,12 @@ int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
   return BM_OK;
 }
 
+double bm_uniform(uint64_t seed, long step, uint32_t k) {
+  const uint4 o = blade::philox(make_uint4(k, (unsigned)step, (unsigned)((unsigned long long)step >> 32), 0x554E4946u),
+                                make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
+  return ((double)((((unsigned long long)o.x) << 21) ^ (o.y >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
 // potential energy of a member at its current positions (a list rebuild,
 // then one energy evaluation)
 static int member_energy(bm_system *s, double &e) {
@@ -1083,11 +1089,8 @@ int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t
   for (int k = parity; k + 1 < n && !rc; k += 2) {
     bm_system *x = b->sys[k], *y = b->sys[k + 1];
     const double w = (e1[k] - e0[k]) / x->kT + (e1[k + 1] - e0[k + 1]) / y->kT;
-    // uniform in (0,1) from (seed, step, pair)
-    const uint4 o = blade::philox(make_uint4((unsigned)k, (unsigned)step, (unsigned)((unsigned long long)step >> 32),
-                                             0x52455821u),
-                                  make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
-    const double u = ((double)((((unsigned long long)o.x) << 21) ^ (o.y >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+    // uniform in (0,1) from (seed, step, pair); stream tag "REX"
+    const double u = bm_uniform(seed, step, 0x52455800u | (unsigned)k);
     const bool acc = u < std::exp(-w);
     if (dW) dW[k] = w;
     if (accepted) accepted[k] = acc ? 1 : 0;
@@ -1239,6 +1242,62 @@ int bm_get_positions(bm_system *s, double *pos, double *theta) {
   if (int rc = d2h(s, x.data(), s->d.pos, x.size() * sizeof(double))) return rc;
   if (pos) std::memcpy(pos, x.data(), 3 * (size_t)n * sizeof(double));
   if (theta) std::memcpy(theta, x.data() + 3 * (size_t)n, s->d.nl * sizeof(double));
+  return BM_OK;
+}
+
+// the box-dependent members of the device system, as bm_system_create sets
+// them; captured step graphs hold the old values, so they are dropped and
+// captured again on next use
+static void set_box(bm_system *s, const double *box) {
+  blade::Dev &d = s->d;
+  for (int k = 0; k < 3; k++) {
+    d.box[k] = box[k];
+    d.invbox[k] = 1.0 / box[k];
+    d.boxf[k] = (float)box[k];
+    d.fracscale[k] = (float)(box[k] / 4294967296.0);
+    d.ddscale[k] = box[k] * (1.0 / 4294967296.0);
+  }
+  const double V = box[0] * box[1] * box[2];
+  d.pmefacd = 0.5 * blade::kElectricD / M_PI / V;
+  d.pmefac = (float)d.pmefacd;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      if (s->graph[a][b]) (void)hipGraphExecDestroy(s->graph[a][b]);
+      if (s->graphSrc[a][b]) (void)hipGraphDestroy(s->graphSrc[a][b]);
+      s->graph[a][b] = nullptr;
+      s->graphSrc[a][b] = nullptr;
+    }
+}
+
+int bm_volume_move(bm_system *s, const double *scale) {
+  if (!s || !scale) return BM_EINVAL;
+  HIPCHK(hipSetDevice(s->device));
+  double oldbox[3], box[3];
+  for (int k = 0; k < 3; k++) {
+    oldbox[k] = s->d.box[k];
+    box[k] = s->d.box[k] * scale[k];
+    if (!(scale[k] > 0) || !(box[k] > 2 * s->d.rc)) return set_err(BM_EINVAL, "volume move: box edge must exceed 2*rcut");
+  }
+  blade::launch_volume_move(s->d, scale, oldbox, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->st));
+  set_box(s, box);
+  return BM_OK;
+}
+
+int bm_volume_restore(bm_system *s, const double *box) {
+  if (!s || !box) return BM_EINVAL;
+  HIPCHK(hipSetDevice(s->device));
+  blade::launch_volume_restore(s->d, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->st));
+  set_box(s, box);
+  return BM_OK;
+}
+
+int bm_get_box(bm_system *s, double *box) {
+  if (!s || !box) return BM_EINVAL;
+  for (int k = 0; k < 3; k++) box[k] = s->d.box[k];
   return BM_OK;
 }
 

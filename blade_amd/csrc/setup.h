@@ -93,6 +93,8 @@ struct RunSetup {
   double shakeTolerance = 2e-7;
   int freqNRG = 10;
   int freqNPT = 0;
+  double volumeFluctuation = 100.0;   // A^3 (run.cu:67)
+  double pressure = 0.000014584319218586;   // 1 atm in kcal/mol/A^3 (run.cu:68)
   unsigned long long seed = 0xB1ADE000ull;
   int freqDomdec = 10;
   int fftMode = 0;       // PME FFT form (bm_runparams.fft_mode)

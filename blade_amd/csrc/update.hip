@@ -637,6 +637,81 @@ void launch_set_step(const Dev &d, long long step, hipStream_t st) {
   upd_set_step<<<1, 1, 0, st>>>(d, step);
 }
 
+// ---- MC barostat volume move (update/pressure.cu:63-118) ----------------
+// backup, then every atom scaled per axis
+__global__ void __launch_bounds__(256) upd_scale_positions(Dev d, double sx, double sy, double sz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const double s[3] = {sx, sy, sz};
+  for (int k = 0; k < 3; k++) {
+    const double x = d.pos[k * d.n + i];
+    d.posb[k * d.n + i] = x;
+    d.pos[k * d.n + i] = x * s[k];
+  }
+}
+
+// each constraint group's other atoms back at the scaled first atom plus
+// their old minimum-image offsets (holonomic/rectify.cu:200-320), so the
+// constrained lengths do not change
+__device__ __forceinline__ void rectify_group(const Dev &d, const int *idx, int na, D3 oldbox) {
+  const D3 b0 = getp(d.posb, d.n, idx[0]), x0 = getp(d.pos, d.n, idx[0]);
+  for (int j = 1; j < na; j++) {
+    D3 r = d3(d.posb[idx[j]] - b0.x, d.posb[d.n + idx[j]] - b0.y, d.posb[2 * d.n + idx[j]] - b0.z);
+    r.x -= oldbox.x * floor(r.x / oldbox.x + 0.5);
+    r.y -= oldbox.y * floor(r.y / oldbox.y + 0.5);
+    r.z -= oldbox.z * floor(r.z / oldbox.z + 0.5);
+    setp(d.pos, d.n, idx[j], d3(x0.x + r.x, x0.y + r.y, x0.z + r.z));
+  }
+}
+
+__global__ void __launch_bounds__(256) upd_rectify_groups(Dev d, double bx, double by, double bz) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const D3 ob = d3(bx, by, bz);
+  int idx[4];
+  if (g < d.ntri) {
+    const int4 t = d.tri_i[g];
+    idx[0] = t.x; idx[1] = t.y; idx[2] = t.z;
+    rectify_group(d, idx, 3, ob);
+    return;
+  }
+  g -= d.ntri;
+  if (g < d.nbr1) {
+    const int2 t = d.br1_i[g];
+    idx[0] = t.x; idx[1] = t.y;
+    rectify_group(d, idx, 2, ob);
+    return;
+  }
+  g -= d.nbr1;
+  if (g < d.nbr2) {
+    const int4 t = d.br2_i[g];
+    idx[0] = t.x; idx[1] = t.y; idx[2] = t.z;
+    rectify_group(d, idx, 3, ob);
+    return;
+  }
+  g -= d.nbr2;
+  if (g < d.nbr3) {
+    const int4 t = d.br3_i[g];
+    idx[0] = t.x; idx[1] = t.y; idx[2] = t.z; idx[3] = t.w;
+    rectify_group(d, idx, 4, ob);
+  }
+}
+
+// restore the backup (a rejected move)
+__global__ void __launch_bounds__(256) upd_restore_positions(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * d.n) d.pos[i] = d.posb[i];
+}
+
+void launch_volume_move(const Dev &d, const double *scale, const double *oldbox, hipStream_t st) {
+  upd_scale_positions<<<(d.n + 255) / 256, 256, 0, st>>>(d, scale[0], scale[1], scale[2]);
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  if (ng) upd_rectify_groups<<<(ng + 255) / 256, 256, 0, st>>>(d, oldbox[0], oldbox[1], oldbox[2]);
+}
+
+void launch_volume_restore(const Dev &d, hipStream_t st) {
+  upd_restore_positions<<<(3 * d.n + 255) / 256, 256, 0, st>>>(d);
+}
+
 void launch_theta_to_lambda(const Dev &d, hipStream_t st) {
   upd_theta_to_lambda<<<(d.nsites + 63) / 64, 64, 0, st>>>(d);
 }

@@ -114,6 +114,9 @@ class Spec:
     elec_method: int = 1     # 0 force switch, 1 PME, 2 force shift
     fft_mode: int = 0        # PME FFT: 0 FP32 Stockham, 1 MFMA split-BF16x3
     freq_domdec: int = 10    # neighbour-list rebuild cadence (steps)
+    freq_npt: int = 0        # Monte Carlo barostat move every freq_npt steps (0: constant volume)
+    volume_fluct: float = 100.0   # rms volume move, A^3
+    pressure_atm: float = 1.0
     tile_chunk: int = 0      # directed tiles per nonbonded work unit (0: 16)
     shake_tol: float = 2e-7
     seed: int = 0xB1ADE000

@@ -14,6 +14,8 @@ import numpy as np
 from . import _ffi
 from ._ffi import check, ptr
 
+ATMOSPHERE = 0.000014584319218586   # kcal/mol/A^3 (main/defines.h:58)
+
 
 def _b(s):
     return s.encode() if isinstance(s, str) else s
@@ -128,6 +130,13 @@ class BladeSystem:
 
     def add_run_dynopts(self, step, step0, nsteps, dt, T, freqnpt=0, volfluct=0.0, pressure=0.0):
         self.L.blade_add_run_dynopts(self.h, step, step0, nsteps, dt, T, freqnpt, volfluct, pressure)
+
+    def npt_stats(self):
+        """(volume moves tried, kept, current box) of the barostat"""
+        m, a = C.c_long(), C.c_long()
+        box = np.zeros(3)
+        check(self.L.blade_get_npt_stats(self.h, C.byref(m), C.byref(a), ptr(box)), "blade_get_npt_stats")
+        return m.value, a.value, box
 
     def set_seed(self, seed):
         self.L.blade_set_seed(self.h, seed)
@@ -301,7 +310,8 @@ def build(spec, gpus=(0,)):
     gx, gy, gz = spec.grid
     s.add_run_flags(spec.gamma, spec.beta, spec.rcut, spec.rswitch, spec.vdw_method, spec.elec_method,
                     spec.grid_space if gx <= 0 else -1.0, gx, gy, gz, spec.order, spec.shake_tol)
-    s.add_run_dynopts(0, 0, spec.nsteps, spec.dt, spec.temperature)
+    s.add_run_dynopts(0, 0, spec.nsteps, spec.dt, spec.temperature, getattr(spec, "freq_npt", 0),
+                      getattr(spec, "volume_fluct", 100.0), getattr(spec, "pressure_atm", 1.0) * ATMOSPHERE)
     s.set_seed(spec.seed)
     s.set_fft_mode(getattr(spec, "fft_mode", 0))
     s.set_domdec_frequency(getattr(spec, "freq_domdec", 10))

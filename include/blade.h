@@ -153,6 +153,10 @@ int blade_set_output_lmd(System *system, const char *path, int freq);
    bit-identical to the uninterrupted run.  Read it after setup and before
    the first energy or dynamics call. */
 int blade_write_checkpoint(System *system, const char *path);
+/* barostat counters of blade_run_dynamics with freqNPT > 0 (volume moves
+   tried and kept) and the current box; no counterpart in the reference,
+   which prints them to its log */
+int blade_get_npt_stats(System *system, long *moves, long *accepted, double *box);
 int blade_read_checkpoint(System *system, const char *path);
 /* directed tiles per nonbonded work unit, 1..16 (default 16); no
    counterpart in the reference */

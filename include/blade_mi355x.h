@@ -265,6 +265,21 @@ int bm_sync(bm_system *sys);
 /* ---- host readback (State::recv_* equivalents, system/state.cxx:330-365) */
 int bm_get_positions(bm_system *sys, double *pos /*[3N]*/, double *theta /*[nL]*/);
 int bm_set_positions(bm_system *sys, const double *pos, const double *theta);
+
+/* ---- MC barostat volume moves (update/pressure.cu:63-118) ------------
+ * bm_volume_move backs up the positions, scales every atom by scale[3]
+ * (per axis), puts each constraint group's other atoms back at its scaled
+ * first atom plus their old minimum-image offsets
+ * (holonomic/rectify.cu:200-320) and scales the box; bm_volume_restore
+ * puts the backed-up positions and the given box back (a rejected move).
+ * Either invalidates the list: call bm_domdec_update(sys, 1) before the
+ * next force evaluation. */
+int bm_volume_move(bm_system *sys, const double *scale);
+/* a uniform in (0, 1) from Philox4x32-10 keyed by (seed, step, k): the
+   draws of the runtime's Monte Carlo decisions (barostat, replica exchange) */
+double bm_uniform(uint64_t seed, long step, uint32_t k);
+int bm_volume_restore(bm_system *sys, const double *box);
+int bm_get_box(bm_system *sys, double *box);
 int bm_get_velocities(bm_system *sys, double *vel /*[3N]*/, double *theta_vel /*[nL]*/);
 int bm_get_forces(bm_system *sys, double *force /*[3N]*/, double *theta_force /*[nL]*/);
 int bm_get_dudl(bm_system *sys, double *dudl /*[nL]*/);

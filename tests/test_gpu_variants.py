@@ -171,3 +171,82 @@ def test_reference_water216_deck():
           % (pairs, frms, eg[20], eo[20], erel), flush=True)
     assert frms <= FORCE_TOL
     assert erel <= ENERGY_TOL
+
+
+def test_barostat_volume_move_restore_and_npt_run():
+    """Monte Carlo barostat: a volume move scales free atoms exactly, keeps
+    every constraint group's internal offsets, and the forces and energy at
+    the new box match the oracle's at that box; restoring gives back the
+    positions bit for bit; an NPT run (a move every 10 steps) is
+    bit-reproducible and keeps its counters."""
+    from tests.helpers import relaxed
+    sp = relaxed("msld_small")
+    s = system.build(sp)
+    s.run_energy()
+    L, dev = s.L, s.device()
+    n = sp.n
+    box0 = np.zeros(3)
+    assert L.bm_get_box(dev, box0.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    x0 = s.positions().copy()
+    scale = np.array([1.01, 0.99, 1.005])
+    assert L.bm_volume_move(dev, scale.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    x1 = s.positions().copy()
+    box1 = np.zeros(3)
+    L.bm_get_box(dev, box1.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(box1, box0 * scale)
+    top = s.topology()
+    t = top.contents
+    grouped = set()
+    for key, cnt, w in (("tri_idx", t.n_tri, 3), ("br1_idx", t.n_br1, 2), ("br2_idx", t.n_br2, 3),
+                        ("br3_idx", t.n_br3, 4)):
+        for g in t.array(key, cnt, w):
+            grouped |= set(int(a) for a in g)
+            for j in g[1:]:
+                for k in range(3):
+                    d0 = x0[k * n + j] - x0[k * n + g[0]]
+                    d0 -= box0[k] * np.floor(d0 / box0[k] + 0.5)
+                    d1 = x1[k * n + j] - x1[k * n + g[0]]
+                    assert abs(d1 - d0) < 1e-12
+    free = np.array([a for a in range(n) if a not in grouped], dtype=np.int64)
+    # every first atom of a group and every free atom is scaled exactly
+    heads = np.array(sorted({int(g[0]) for key, cnt, w in (("tri_idx", t.n_tri, 3), ("br1_idx", t.n_br1, 2),
+                                                          ("br2_idx", t.n_br2, 3), ("br3_idx", t.n_br3, 4))
+                             for g in t.array(key, cnt, w)}), dtype=np.int64)
+    for idx in (free, heads):
+        for k in range(3):
+            assert np.array_equal(x1[k * n + idx], x0[k * n + idx] * scale[k])
+    # forces and energy at the new box against the oracle
+    assert L.bm_domdec_update(dev, 1) == 0
+    assert L.bm_calc_force(dev, 0, 1) == 0
+    assert L.bm_sync(dev) == 0
+    rp = copy.copy(s.runparams().contents)
+    for k in range(3):
+        rp.box[k] = box1[k]
+    lam = orc.theta_to_lambda(top, C.pointer(rp), np.array(sp.theta0, float))
+    fo, do, eo, _ = orc.forces(top, C.pointer(rp), s.positions(), lam)
+    fg, eg = s.forces(), s.energy()
+    frms = rel_rms(fg, fo)
+    erel = abs(eg[20] - eo[20]) / abs(eo[20])
+    print("\n[volume move] force relRMS %.3e energy rel %.3e" % (frms, erel), flush=True)
+    assert frms <= FORCE_TOL and erel <= ENERGY_TOL
+    assert L.bm_volume_restore(dev, box0.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    x2 = s.positions()
+    # restored positions equal the backup taken before the move (the list
+    # rebuild above does not touch the backup)
+    assert np.array_equal(x2, x0)
+    s.close()
+    # NPT runs: a move every 10 steps, twice from the same state
+    runs = []
+    for _ in range(2):
+        sp2 = copy.copy(sp)
+        sp2.freq_npt, sp2.volume_fluct = 10, 30.0
+        s2 = system.build(sp2)
+        s2.run_dynamics(100, freq_nrg=0)
+        moves, acc, box = s2.npt_stats()
+        runs.append((s2.positions().copy(), box.copy(), moves, acc))
+        s2.close()
+    print("[npt] moves %d accepted %d box %s" % (runs[0][2], runs[0][3], runs[0][1]), flush=True)
+    assert runs[0][2] == 10 and 0 <= runs[0][3] <= 10
+    assert np.all(np.isfinite(runs[0][0]))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][3] == runs[1][3]
