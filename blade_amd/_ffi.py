@@ -48,6 +48,8 @@ class Topology(C.Structure):
         ("n_varbias", C.c_int32), ("vb_i", _i32p), ("vb_j", _i32p), ("vb_type", _i32p),
         ("vb_n", _i32p), ("vb_l0", _f64p), ("vb_k", _f64p),
         ("n_cats", C.c_int32), ("cats_bound", _i32p), ("cats_idx", _i32p), ("cats_k", C.c_double),
+        ("block_charge", _f64p), ("cr_k1", C.c_double), ("cr_k2", C.c_double), ("cr_k3", C.c_double),
+        ("cr_q0", C.c_double), ("cr_w", C.c_double),
     ]
 
     def array(self, name, count, width=1):
@@ -140,6 +142,7 @@ def lib():
         "blade_add_msld_block_friction": (None, [vp, i, d]),
         "blade_set_msld_block_fixed": (None, [vp, i, i]),
         "blade_add_msld_atomrestraint": (None, [vp]),
+        "blade_add_msld_charges": (None, [vp, d, d, d, d, d]),
         "blade_add_msld_atomrestraint_element": (None, [vp, i]),
         "blade_init_run": (None, [vp]),
         "blade_add_run_flags": (None, [vp, d, d, d, d, i, i, d, i, i, i, i, d]),

@@ -234,6 +234,10 @@ void blade_add_msld_block_friction(System *s, int blockIdx, double friction) {
 }
 void blade_set_msld_block_fixed(System *s, int blockIdx, int fixed) { s->msld->blockFixed[blockIdx - 1] = fixed != 0; }
 void blade_add_msld_atomrestraint(System *s) { s->msld->atomRestraints.emplace_back(); }
+void blade_add_msld_charges(System *s, double k1, double k2, double k3, double q0, double w) {
+  auto &m = *s->msld;
+  m.kCR1 = k1; m.kCR2 = k2; m.kCR3 = k3; m.q0CR3 = q0; m.wCR3 = w;
+}
 void blade_add_msld_atomrestraint_element(System *s, int i) {
   if (s->msld->atomRestraints.empty()) s->msld->atomRestraints.emplace_back();
   s->msld->atomRestraints.back().push_back(i - 1);

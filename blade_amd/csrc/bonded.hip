@@ -359,6 +359,40 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       }
       if (energy) atomic_fix(&d.energy[BM_ENBRECIPSELF], fix_d(es));
     }
+    if (d.ncr && t == 0) {
+      // charge restraints on the lambda-weighted net charge Q
+      // (msld/msld.cu:1400-1530); dE/dQ times q_b goes to every block but
+      // the environment, the energy to the bias term
+      double q = 0.0;
+      for (int b = 0; b < d.nl; b++) q += (b ? d.lambda[b] : 1.0) * d.blockcharge[b];
+      const double vol = d.box[0] * d.box[1] * d.box[2];
+      double g = 0.0, ec = 0.0;
+      if (d.elecmethod == BM_PME) {
+        if (d.crk1 != 0.0) {
+          // -(2 pi/3) kE k1 / V: the uniform-background shift of a net charge
+          const double k = -2.0943951023931955 * kElectricD * d.crk1 / vol;
+          g += k;
+          ec += k * q;
+        }
+        if (d.crk2 != 0.0) {
+          // -pi kE k2 / (V beta^2), with pi^2/beta^2 held in FP64
+          const double k = -kElectricD * d.crk2 * d.pmeexpd / (3.141592653589793 * vol);
+          g += k * q;
+          ec += 0.5 * k * q * q;
+        }
+      }
+      if (d.crk3 != 0.0) {
+        double dq = q - d.crq0;
+        if (d.crw > 0.0) dq = fabs(dq) <= d.crw ? 0.0 : dq - copysign(d.crw, dq);
+        g += d.crk3 * dq;
+        ec += 0.5 * d.crk3 * dq * dq;
+      }
+      for (int b = 1; b < d.nl; b++) {
+        const double f = g * d.blockcharge[b];
+        if (f != 0.0) atomic_fix(&d.dudl[b], fix_d(f));
+      }
+      if (energy) atomic_fix(&d.energy[BM_EBIAS], fix_d(ec));
+    }
     if (d.nl > 1)
       for (int b = t; b < d.nl; b += blockDim.x) {
         double bias = d.fixed_bias[b];

@@ -143,6 +143,9 @@ struct Dev {
   int ncats;                     // MSLD atom restraint groups
   int *cats_bound, *cats_idx;    // [ncats+1] first member, [members] atoms
   double cats_k;                 // kRestraint
+  int ncr;                       // charge restraints on Q = sum_b lambda_b q_b
+  double *blockcharge;           // [nL] q_b
+  double crk1, crk2, crk3, crq0, crw;
   int nvb;
   int4 *vb_i;                    // (i, j, type, n)
   double2 *vb_p;                 // (l0, k)

@@ -608,6 +608,13 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     }
     UPLOAD(d.block_site, bs); UPLOAD(d.site_bound, sbd); UPLOAD(d.block_fixed, bf);
     UPLOAD(d.fixed_bias, fb); UPLOAD(d.lfric, lf); UPLOAD(d.lnoise, ln);
+    d.crk1 = t->cr_k1; d.crk2 = t->cr_k2; d.crk3 = t->cr_k3; d.crq0 = t->cr_q0; d.crw = t->cr_w;
+    d.ncr = nl > 1 && (d.crk1 != 0 || d.crk2 != 0 || d.crk3 != 0);
+    if (d.ncr) {
+      std::vector<double> bq(nl, 0.0);
+      for (int b = 0; b < t->n_blocks && t->block_charge; b++) bq[b] = t->block_charge[b];
+      UPLOAD(d.blockcharge, bq);
+    }
     d.nvb = t->n_varbias;
     std::vector<int4> vi(d.nvb);
     std::vector<double2> vp(d.nvb);

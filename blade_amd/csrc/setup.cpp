@@ -419,6 +419,9 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     t.catsBound.push_back((int)t.catsIdx.size());
   }
   t.catsK = m.kRestraint;
+  t.blockCharge = m.lambdaCharge;
+  if (m.kCR2 != 0 && m.kCR2 != 1) fail("only kChargeRestraint2 values of 0 or 1 are allowed");
+  t.crK1 = m.kCR1; t.crK2 = m.kCR2; t.crK3 = m.kCR3; t.crQ0 = m.q0CR3; t.crW = m.wCR3;
   (void)kBoltz;
 }
 
@@ -461,6 +464,8 @@ const bm_topology *Topology::view() {
   v.cats_bound = catsBound.empty() ? nullptr : catsBound.data();
   v.cats_idx = catsIdx.data();
   v.cats_k = catsK;
+  v.block_charge = blockCharge.data();
+  v.cr_k1 = crK1; v.cr_k2 = crK2; v.cr_k3 = crK3; v.cr_q0 = crQ0; v.cr_w = crW;
   return &top;
 }
 

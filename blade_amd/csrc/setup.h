@@ -76,6 +76,7 @@ struct MsldSetup {
   std::vector<VarBias> varBias;
   std::vector<std::vector<int>> atomRestraints;   // 0-based atoms of each group
   double kRestraint = 59.2;                       // kcal/mol/A^2 (msld.cu:83)
+  double kCR1 = 0, kCR2 = 0, kCR3 = 0, q0CR3 = 0, wCR3 = 0;   // charge restraints
   int softBondCount = 0;
   void resize(int nblocks, int natoms);
 };
@@ -124,6 +125,7 @@ struct Topology {
   std::vector<double> thetaMass, fixedBias, thetaFriction;
   std::vector<int> vbI, vbJ, vbType, vbN; std::vector<double> vbL0, vbK;
   std::vector<int> catsBound, catsIdx; double catsK = 0;
+  std::vector<double> blockCharge; double crK1 = 0, crK2 = 0, crK3 = 0, crQ0 = 0, crW = 0;
   bm_topology top{};
   const bm_topology *view();
 };

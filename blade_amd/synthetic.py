@@ -99,6 +99,8 @@ class Spec:
     scale_terms: tuple = (1, 1, 1, 1, 1, 1)           # lambda-scaled bond, urey, angle, dihe, impr, cmap
     atom_restraints: list = field(default_factory=list)  # groups of 0-based atoms held to their centroid
     k_restraint: float = 59.2                         # kcal/mol/A^2
+    block_charge: list = None                         # per-block net charge q_b (None = 0)
+    charge_restraints: tuple = None                   # (k1, k2, k3, q0, w) on Q = sum lambda_b q_b
     # run options
     dt: float = 0.002 * PICOSECOND
     temperature: float = 298.0

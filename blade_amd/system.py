@@ -115,6 +115,11 @@ class BladeSystem:
         for a in atoms:
             self.L.blade_add_msld_atomrestraint_element(self.h, int(a))
 
+    def add_msld_charges(self, k1=0.0, k2=0.0, k3=0.0, q0=0.0, w=0.0):
+        """charge restraints on the lambda-weighted net charge of the blocks'
+        initial-condition charges (msld/msld.h:154)"""
+        self.L.blade_add_msld_charges(self.h, float(k1), float(k2), float(k3), float(q0), float(w))
+
     def add_msld_termscaling(self, bond, urey, angle, dihe, impr, cmap):
         self.L.blade_add_msld_termscaling(self.h, bond, urey, angle, dihe, impr, cmap)
 
@@ -298,10 +303,13 @@ def build(spec, gpus=(0,)):
         for b in range(spec.nblocks):
             tv = spec.theta_v[b] if spec.theta_v is not None else 0.0
             s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], tv,
-                                         spec.theta_mass[b], spec.fixed_bias[b], 0.0)
+                                         spec.theta_mass[b], spec.fixed_bias[b],
+                                         spec.block_charge[b] if getattr(spec, "block_charge", None) else 0.0)
         s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2,
                          getattr(spec, "k_restraint", 59.2), 1.0, 2.0, 1.0, 0)
         s.add_msld_termscaling(*[int(x) for x in spec.scale_terms])
+        if getattr(spec, "charge_restraints", None):
+            s.add_msld_charges(*spec.charge_restraints)
         for g in getattr(spec, "atom_restraints", []):
             s.add_msld_atomrestraint([a + 1 for a in g])
         for (i, j, t, l0, k, n) in spec.var_bias:

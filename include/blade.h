@@ -100,6 +100,10 @@ void blade_set_msld_block_fixed(System *system, int blockIdx, int fixed);
    (1-based); the force constant is blade_add_msld_flags' kRestraint */
 void blade_add_msld_atomrestraint(System *system);
 void blade_add_msld_atomrestraint_element(System *system, int atomIdx);
+/* MSLD charge restraints (msld/msld.h:154) on the lambda-weighted net
+   charge of the blocks' blockCharge values */
+void blade_add_msld_charges(System *system, double kChargeRestraint1, double kChargeRestraint2,
+                            double kChargeRestraint3, double q0ChargeRestraint3, double wChargeRestraint3);
 
 /* run/run.h:146-158 */
 void blade_init_run(System *system);

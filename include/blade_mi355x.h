@@ -143,6 +143,13 @@ typedef struct bm_topology {
   const int32_t *cats_bound;       /* [n+1] first member of each group    */
   const int32_t *cats_idx;         /* [cats_bound[n]] member atoms        */
   double cats_k;                   /* kRestraint, kcal/mol/A^2            */
+
+  /* MSLD charge restraints on Q = sum_b lambda_b q_b (msld.cu:1400-1551);
+     energy term BM_EBIAS.  cr_k1 (PME only): E = -(2 pi/3) kE k1 Q / V;
+     cr_k2 (PME only, 0 or 1): E = -(pi/2) kE k2 Q^2 / (V beta^2);
+     cr_k3: E = k3/2 dQ^2, dQ = distance of Q - q0 from the band [-w, w] */
+  const double *block_charge;      /* [nL] q_b (lambdaCharge)            */
+  double cr_k1, cr_k2, cr_k3, cr_q0, cr_w;
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */

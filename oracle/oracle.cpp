@@ -684,6 +684,43 @@ void msldbias(const bm_topology *t, const double *lambda, double *dudl, double *
   }
 }
 
+// MSLD charge restraints (msld/msld.cu:1400-1530) on Q = sum_b lambda_b q_b,
+// lambda_0 = 1: the first two are PME-only, E1 = k Q with
+// k = -(2 pi/3) kE k1 / V and E2 = k Q^2 / 2 with k = -pi kE k2 / (V beta^2);
+// the third is harmonic, k3/2 dQ^2, in the distance dQ of Q - q0 from the
+// flat band [-w, w].  dU/dlambda_b = dE/dQ q_b for b > 0; energy BM_EBIAS.
+void chargerestraints(const bm_topology *t, const bm_runparams *rp, const double *lambda, double *dudl,
+                      double *energy) {
+  if (!t->block_charge) return;
+  double q = 0;
+  for (int b = 0; b < t->n_blocks; b++) q += (b ? lambda[b] : 1.0) * t->block_charge[b];
+  const double vol = rp->box[0] * rp->box[1] * rp->box[2];
+  double dedq[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+  if (rp->elec_method == BM_PME && t->cr_k1 != 0) {
+    const double k = -(2 * M_PI / 3) * kElectric * t->cr_k1 / vol;
+    e[0] = k * q;
+    dedq[0] = k;
+  }
+  if (rp->elec_method == BM_PME && t->cr_k2 != 0) {
+    const double k = -M_PI * kElectric * t->cr_k2 / (vol * rp->beta_ewald * rp->beta_ewald);
+    e[1] = 0.5 * k * q * q;
+    dedq[1] = k * q;
+  }
+  if (t->cr_k3 != 0) {
+    double dq = q - t->cr_q0;
+    if (t->cr_w > 0) {
+      if (std::fabs(dq) <= t->cr_w) dq = 0;
+      else dq -= std::copysign(t->cr_w, dq);
+    }
+    e[2] = 0.5 * t->cr_k3 * dq * dq;
+    dedq[2] = t->cr_k3 * dq;
+  }
+  for (int v = 0; v < 3; v++) {
+    for (int b = 1; b < t->n_blocks; b++) dudl[b] += dedq[v] * t->block_charge[b];
+    energy[BM_EBIAS] += e[v];
+  }
+}
+
 // MSLD atom restraints (msld/msld.cu:1328-1364): each group's atoms are
 // pulled to the group centroid (minimum-image offsets from its first
 // atom), E = k/2 sum |x_i - c|^2 and dU/dx_i = k (x_i - c); not scaled.
@@ -874,6 +911,8 @@ long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
   if (terms & ORC_NB14) nb14(t, rp, pos, lambda, force, dudl, energy);
   if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1) msldbias(t, lambda, dudl, energy);
   if ((terms & ORC_MSLDBIAS) && t->n_cats > 0) cats(t, rp, pos, force, energy);
+  if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1 && (t->cr_k1 != 0 || t->cr_k2 != 0 || t->cr_k3 != 0))
+    chargerestraints(t, rp, lambda, dudl, energy);
   double pot = 0;
   for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
   energy[BM_EPOTENTIAL] = pot;

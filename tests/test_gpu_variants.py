@@ -71,6 +71,23 @@ def test_atom_restraints_displaced():
     _check(sp, "atom restraints")
 
 
+@pytest.mark.parametrize("k,q0,w", [((1.0, 1.0, 3.0), 0.1, 0.05), ((0.0, 0.0, 5.0), -0.3, 0.0)])
+def test_charge_restraints(k, q0, w):
+    """MSLD charge restraints on the lambda-weighted net charge of the
+    blocks (energy term BM_EBIAS, FP64 on the device): the term itself to
+    one quantum of the 2^-32 kcal/mol fixed-point energy sum plus 1e-9
+    relative, and dU/dlambda to the path's bar"""
+    sp = _variant("two_site_small", block_charge=[0.4, 1.0, -1.0, 0.5, -0.5, 0.25],
+                  charge_restraints=(k[0], k[1], k[2], q0, w))
+    s = _check(sp, "charge restraints %s q0 %g w %g" % (k, q0, w))
+    top, rp = s.topology(), s.runparams()
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    _, do, eo, _ = orc.forces(top, rp, s.positions(), lam, orc.MSLDBIAS)
+    eg = s.energy()
+    assert eo[19] != 0.0
+    assert abs(eg[19] - eo[19]) <= 2.0 ** -32 + 1e-9 * abs(eo[19]), (eg[19], eo[19])
+
+
 def test_soft_core_site():
     _check(_variant("msld_small", soft_core=True), "soft core")
 

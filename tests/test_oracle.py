@@ -286,6 +286,78 @@ def test_atom_restraints_pairwise_identity_and_gradient():
     assert worst < 1e-5, worst
 
 
+E_BIAS = 19
+WIGNER_SC = 2.837297479480620   # Wigner lattice-sum constant of a simple cubic box
+
+
+def _charge_restrained(k, q_blocks, beta=None):
+    sp = synthetic.two_site_small()
+    sp.block_charge = list(q_blocks)
+    sp.charge_restraints = tuple(k)
+    if beta is not None:
+        sp.beta = beta
+    s = system.build(sp)
+    s.prepare()
+    return sp, s
+
+
+def test_charge_restraint_background_term_keeps_ewald_energy_beta_independent():
+    """The second charge restraint is the energy of the neutralising
+    background of a net charge: added to the reciprocal and self energy of a
+    lone ion it makes the Ewald energy independent of beta, equal to the
+    Wigner self-energy -kE q^2 xi / (2 L) of a charge in a cubic box."""
+    q = 1.0
+    vals = []
+    for beta in (0.30, 0.40):
+        sp, s = _charge_restrained((0.0, 1.0, 0.0, 0.0, 0.0), [q] + [0.0] * 5, beta)
+        L = sp.box[0]
+        lam = orc.theta_to_lambda(s.topology(), s.runparams(), np.array(sp.theta0, float))
+        e_bg = orc.forces(s.topology(), s.runparams(), system.soa(sp.pos), lam, orc.MSLDBIAS)[2][E_BIAS]
+        ion = system.build(_ions(np.array([[3.0, 4.0, 5.0]]), [q], (L,) * 3, beta=beta, rcut=10.0,
+                                 rswitch=9.0, grid=48, order=8))
+        ion.prepare()
+        e = orc.forces(ion.topology(), ion.runparams(), np.array([3.0, 4.0, 5.0]), np.ones(1))[2]
+        vals.append(e[E_NBRECIP] + e[E_SELF] + e_bg)
+    want = -KE * q * q * WIGNER_SC / (2 * L)
+    for v in vals:
+        assert abs(v - want) / abs(want) < 1e-5, (vals, want)
+
+
+def test_charge_restraints_dudl_and_flat_band():
+    """dU/dlambda of all three charge restraints matches differences of the
+    energy in lambda; the harmonic one is zero inside |Q - q0| <= w and
+    grows from the band's edge."""
+    qb = [0.4, 1.0, -1.0, 0.5, -0.5, 0.25]
+    sp, s = _charge_restrained((1.0, 1.0, 3.0, 0.1, 0.05), qb)
+    top, rp = s.topology(), s.runparams()
+    nb = top.contents.n_blocks
+    assert nb == len(qb)
+    x = system.soa(sp.pos)
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    _, d, e, _ = orc.forces(top, rp, x, lam, orc.MSLDBIAS)
+    h = 1e-6
+    for b in range(1, nb):
+        lp, lm = lam.copy(), lam.copy()
+        lp[b] += h
+        lm[b] -= h
+        fd = (orc.forces(top, rp, x, lp, orc.MSLDBIAS)[2][E_POT]
+              - orc.forces(top, rp, x, lm, orc.MSLDBIAS)[2][E_POT]) / (2 * h)
+        assert abs(fd - d[b]) < 1e-6 * max(1.0, abs(d[b])), (b, fd, d[b])
+    # harmonic restraint alone, q0 placed relative to Q
+    Q = float(np.dot(lam, qb))
+    for q0, want in ((Q, 0.0), (Q - 0.04, 0.0), (Q + 0.05, 0.0), (Q - 0.35, 0.5 * 3.0 * 0.3 ** 2),
+                     (Q + 0.45, 0.5 * 3.0 * 0.4 ** 2)):
+        _, s3 = _charge_restrained((0.0, 0.0, 3.0, q0, 0.05), qb)
+        got = orc.forces(s3.topology(), s3.runparams(), x, lam, orc.MSLDBIAS)[2][E_BIAS]
+        assert abs(got - want) < 1e-12, (q0, got, want)
+    # the first restraint is linear in Q: 2Q gives twice the energy
+    _, s1 = _charge_restrained((1.0, 0.0, 0.0, 0.0, 0.0), qb)
+    _, s2 = _charge_restrained((1.0, 0.0, 0.0, 0.0, 0.0), [2 * v for v in qb])
+    e1 = orc.forces(s1.topology(), s1.runparams(), x, lam, orc.MSLDBIAS)[2][E_BIAS]
+    e2 = orc.forces(s2.topology(), s2.runparams(), x, lam, orc.MSLDBIAS)[2][E_BIAS]
+    assert e1 != 0.0 and abs(e2 - 2 * e1) < 1e-12 * abs(e1)
+
+
 def test_dudl_matches_lambda_differences():
     name = "msld_small"
     s = built(name)
