@@ -50,6 +50,8 @@ class Topology(C.Structure):
         ("n_cats", C.c_int32), ("cats_bound", _i32p), ("cats_idx", _i32p), ("cats_k", C.c_double),
         ("block_charge", _f64p), ("cr_k1", C.c_double), ("cr_k2", C.c_double), ("cr_k3", C.c_double),
         ("cr_q0", C.c_double), ("cr_w", C.c_double),
+        ("n_cmaps", C.c_int32), ("cmap_idx", _i32p), ("cmap_sb", _i32p), ("cmap_type", _i32p),
+        ("n_cmap_types", C.c_int32), ("cmap_ngrid", _i32p), ("cmap_off", _i32p), ("cmap_grid", _f64p),
     ]
 
     def array(self, name, count, width=1):
@@ -143,6 +145,9 @@ def lib():
         "blade_set_msld_block_fixed": (None, [vp, i, i]),
         "blade_add_msld_atomrestraint": (None, [vp]),
         "blade_add_msld_charges": (None, [vp, d, d, d, d, d]),
+        "blade_add_parameter_cmaps": (None, [vp] + [s] * 8 + [i]),
+        "blade_add_parameter_cmaps_fill": (None, [vp] + [s] * 8 + [i, i, d]),
+        "blade_add_cmap": (None, [vp] + [i] * 8),
         "blade_add_msld_atomrestraint_element": (None, [vp, i]),
         "blade_init_run": (None, [vp]),
         "blade_add_run_flags": (None, [vp, d, d, d, d, i, i, d, i, i, i, i, d]),
@@ -167,6 +172,7 @@ def lib():
         "blade_set_tile_chunk": (None, [vp, i]),
         "blade_set_output_nrg": (i, [vp, C.c_char_p, i]),
         "blade_set_output_lmd": (i, [vp, C.c_char_p, i]),
+        "blade_set_lmd_format": (None, [vp, i]),
         "blade_write_checkpoint": (i, [vp, C.c_char_p]),
         "blade_get_npt_stats": (i, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), _f64p]),
         "blade_read_checkpoint": (i, [vp, C.c_char_p]),

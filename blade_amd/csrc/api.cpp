@@ -44,6 +44,7 @@ struct System {
   // output files written by blade_run_dynamics (run/run.cu:572-590)
   FILE *fpNRG = nullptr, *fpLMD = nullptr;
   int freqNRG = 0, freqLMD = 0;
+  bool hrLMD = true;                    // text .lmd, else XDR frames
   long nptMoves = 0, nptAccepted = 0;   // barostat counters
   ~System() {
     if (fpNRG) fclose(fpNRG);
@@ -111,12 +112,20 @@ void blade_add_parameter_imprs(System *s, const char *t1, const char *t2, const 
   if (nimp < 0) { api_fail(s, "improper periodicity less than 0"); return; }
   s->parameters->imprs[canon({t1, t2, t3, t4})] = ImprParam{nimp == 0 ? 2.0 * kimp : kimp, nimp, kDegrees * imp0};
 }
-void blade_add_parameter_cmaps(System *s, const char *, const char *, const char *, const char *, const char *,
-                               const char *, const char *, const char *, int) {
-  (void)s;  // CMAP parameters are accepted; CMAP terms are rejected at setup
+void blade_add_parameter_cmaps(System *s, const char *t1, const char *t2, const char *t3, const char *t4,
+                               const char *t5, const char *t6, const char *t7, const char *t8, int ngrid) {
+  if (ngrid < 4 || ngrid > 60) { api_fail(s, "CMAP grid must have 4 to 60 points per 360 degrees"); return; }
+  s->parameters->cmaps[{t1, t2, t3, t4, t5, t6, t7, t8}] = {ngrid, std::vector<double>((size_t)ngrid * ngrid, 0.0)};
 }
-void blade_add_parameter_cmaps_fill(System *, const char *, const char *, const char *, const char *,
-                                    const char *, const char *, const char *, const char *, int, int, double) {}
+// grid value (i, j), 1-based: i steps phi and j psi from -180 degrees
+void blade_add_parameter_cmaps_fill(System *s, const char *t1, const char *t2, const char *t3, const char *t4,
+                                    const char *t5, const char *t6, const char *t7, const char *t8, int i, int j,
+                                    double kcmapij) {
+  auto it = s->parameters->cmaps.find({t1, t2, t3, t4, t5, t6, t7, t8});
+  if (it == s->parameters->cmaps.end()) { api_fail(s, "CMAP fill before its grid was declared"); return; }
+  const int n = it->second.first;
+  if (i >= 1 && i <= n && j >= 1 && j <= n) it->second.second[(size_t)n * (i - 1) + (j - 1)] = kcmapij;
+}
 void blade_add_parameter_nbonds(System *s, const char *t1, double eps, double sig, double eps14, double sig14,
                                 double e14fac, int combine) {
   s->parameters->nbonds[t1] = NbondParam{-eps, sig, -eps14, sig14, e14fac, combine};
@@ -143,7 +152,9 @@ void blade_add_dihe(System *s, int i, int j, int k, int l) {
 void blade_add_impr(System *s, int i, int j, int k, int l) {
   s->structure->imprs.push_back({i - 1, j - 1, k - 1, l - 1});
 }
-void blade_add_cmap(System *s, int, int, int, int, int, int, int, int) { mark_unsupported(s, "CMAP terms"); }
+void blade_add_cmap(System *s, int i1, int j1, int k1, int l1, int i2, int j2, int k2, int l2) {
+  s->structure->cmaps.push_back({i1 - 1, j1 - 1, k1 - 1, l1 - 1, i2 - 1, j2 - 1, k2 - 1, l2 - 1});
+}
 void blade_add_shake(System *s, int shakeHbond) { s->structure->shakeHbond = shakeHbond != 0; }
 int blade_structure_atom_count(System *s) { return (s && s->structure) ? (int)s->structure->atoms.size() : 0; }
 int blade_structure_atom_names(System *s, int atomIdx, char *seg, char *res, char *name, int len) {
@@ -417,10 +428,30 @@ static void write_nrg(System *s, long step) {
 
 // one text .lmd line (io/io.cxx:350-358): the step, then lambda of every
 // block after the environment
+// one XDR word: 4 bytes, most significant first
+static void put_xdr(FILE *fp, uint32_t w) {
+  const unsigned char b[4] = {(unsigned char)(w >> 24), (unsigned char)(w >> 16), (unsigned char)(w >> 8),
+                              (unsigned char)w};
+  fwrite(b, 1, 4, fp);
+}
+
+// lambda frame (io/io.cxx:347-370): text "%10ld" step and " %8.6f" lambdas
+// of the substituent blocks, or in XDR the block count then each lambda as
+// an IEEE single
 static void write_lmd(System *s, long step) {
   const int nl = s->topo.top.n_blocks;
   std::vector<double> l(nl > 0 ? nl : 1);
   if (bm_get_lambda(s->gpu, l.data())) return;
+  if (!s->hrLMD) {
+    put_xdr(s->fpLMD, (uint32_t)(nl - 1));
+    for (int i = 1; i < nl; i++) {
+      const float f = (float)l[i];
+      uint32_t w;
+      memcpy(&w, &f, 4);
+      put_xdr(s->fpLMD, w);
+    }
+    return;
+  }
   fprintf(s->fpLMD, "%10ld", step);
   for (int i = 1; i < nl; i++) fprintf(s->fpLMD, " %8.6f", l[i]);
   fprintf(s->fpLMD, "\n");
@@ -573,6 +604,8 @@ int blade_set_output_lmd(System *s, const char *path, int freq) {
   s->freqLMD = freq;
   return BM_OK;
 }
+
+void blade_set_lmd_format(System *s, int humanReadable) { s->hrLMD = humanReadable != 0; }
 
 // Binary checkpoint: the exact FP64 state, so a run continued from it is
 // bit-identical to the uninterrupted one when it was written on a list

@@ -143,7 +143,7 @@ __device__ void nb14_ef(const Dev &d, double4 p, double r, double &f, double &e)
 }
 
 struct BondedLaunch {
-  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_cats, b_bias;  // first block of each family
+  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_cats, b_cmap, b_bias;  // first block of each family
 };
 
 // workgroups of 256 threads per term family, in family order; the last
@@ -158,7 +158,8 @@ __host__ __device__ inline BondedLaunch bonded_layout(const Dev &d) {
   L.b_nb14 = L.b_impr + nb(d.nimpr);
   L.b_nbex = L.b_nb14 + nb(d.nnb14);
   L.b_cats = L.b_nbex + nb(d.nnbex);
-  L.b_bias = L.b_cats + nb(d.ncats);
+  L.b_cmap = L.b_cats + nb(d.ncats);
+  L.b_bias = L.b_cmap + nb(d.ncmap);
   return L;
 }
 
@@ -310,7 +311,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       put_d(d, d.sl_nbex + 2 * t + 1, (-f / r), dr);
       e = et * ll;
     }
-  } else if (bid < L.b_bias) {
+  } else if (bid < L.b_cmap) {
     // MSLD atom restraints (msld.cu:1328-1364), one thread per group in
     // FP64: the centroid from minimum-image offsets to the first atom,
     // dU/dx_i = k (x_i - centroid), not lambda-scaled
@@ -341,6 +342,76 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
         put_d(d, d.sl_cats + m, d.cats_k, r);
         e += 0.5 * d.cats_k * (r.x * r.x + r.y * r.y + r.z * r.z);
       }
+    }
+  } else if (bid < L.b_bias) {
+    // CMAP cross terms (bonded.cu:531-720), one thread per term: the two
+    // dihedrals in FP32 as in the dihedral family, then bicubic Hermite
+    // interpolation of the cell's corner values and slopes in FP64
+    slot = BM_ECMAP;
+    const int t = (bid - L.b_cmap) * blockDim.x + threadIdx.x;
+    if (t < d.ncmap) {
+      const int4 sv = d.cmap_s[t];
+      const int2 g = d.cmap_g[sv.w];
+      const int ng = g.x;
+      F3 rij[2], rjk[2], rkl[2], m[2], nv[2];
+      double ang[2];
+      for (int a = 0; a < 2; a++) {
+        const int4 ix = d.cmap_i[2 * t + a];
+        rij[a] = disp(d, ix.x, ix.y);
+        rjk[a] = disp(d, ix.y, ix.z);
+        rkl[a] = disp(d, ix.z, ix.w);
+        m[a] = cross3(rij[a], rjk[a]);
+        nv[a] = cross3(rjk[a], rkl[a]);
+        const F3 mn = cross3(m[a], nv[a]);
+        float phi = atan2f(sqrtf(dot3(mn, mn)), dot3(m[a], nv[a]));
+        if (dot3(rij[a], nv[a]) > 0.f) phi = -phi;
+        ang[a] = phi;
+      }
+      // grid point i sits at -pi + i 2 pi / ng
+      const double gs = ng / 6.283185307179586;
+      const double u = ang[0] * gs, v = ang[1] * gs;
+      const double fu = floor(u), fv = floor(v);
+      const double tt = u - fu, ss = v - fv;
+      const int i0 = (((int)fu + ng / 2) % ng + ng) % ng, j0 = (((int)fv + ng / 2) % ng + ng) % ng;
+      const int i1 = i0 + 1 == ng ? 0 : i0 + 1, j1 = j0 + 1 == ng ? 0 : j0 + 1;
+      const double4 *tb = d.cmap_tab + g.y;
+      const double4 c00 = tb[ng * i0 + j0], c10 = tb[ng * i1 + j0], c01 = tb[ng * i0 + j1], c11 = tb[ng * i1 + j1];
+      // cubic Hermite basis: value weights H, slope weights G, derivatives
+      const double H0t = (1 + 2 * tt) * (1 - tt) * (1 - tt), H1t = tt * tt * (3 - 2 * tt);
+      const double G0t = tt * (1 - tt) * (1 - tt), G1t = tt * tt * (tt - 1);
+      const double dH0t = 6 * tt * (tt - 1), dH1t = -dH0t, dG0t = (1 - tt) * (1 - 3 * tt), dG1t = tt * (3 * tt - 2);
+      const double H0s = (1 + 2 * ss) * (1 - ss) * (1 - ss), H1s = ss * ss * (3 - 2 * ss);
+      const double G0s = ss * (1 - ss) * (1 - ss), G1s = ss * ss * (ss - 1);
+      const double dH0s = 6 * ss * (ss - 1), dH1s = -dH0s, dG0s = (1 - ss) * (1 - 3 * ss), dG1s = ss * (3 * ss - 2);
+      auto cell = [](double4 c, double ha, double ga, double hb, double gb) {
+        return c.x * ha * hb + c.y * ga * hb + c.z * ha * gb + c.w * ga * gb;
+      };
+      const double ec = cell(c00, H0t, G0t, H0s, G0s) + cell(c10, H1t, G1t, H0s, G0s) +
+                        cell(c01, H0t, G0t, H1s, G1s) + cell(c11, H1t, G1t, H1s, G1s);
+      const double dt = cell(c00, dH0t, dG0t, H0s, G0s) + cell(c10, dH1t, dG1t, H0s, G0s) +
+                        cell(c01, dH0t, dG0t, H1s, G1s) + cell(c11, dH1t, dG1t, H1s, G1s);
+      const double ds = cell(c00, H0t, G0t, dH0s, dG0s) + cell(c10, H1t, G1t, dH0s, dG0s) +
+                        cell(c01, H0t, G0t, dH1s, dG1s) + cell(c11, H1t, G1t, dH1s, dG1s);
+      const int b0 = sv.x & 0xFFFF, b1 = sv.y & 0xFFFF, b2 = sv.z & 0xFFFF;
+      const double l0 = b0 ? d.lambda[b0] : 1.0, l1 = b1 ? d.lambda[b1] : 1.0, l2 = b2 ? d.lambda[b2] : 1.0;
+      const double ll = l0 * l1 * l2;
+      if (b0) atomic_fix(&d.dudl[b0], fix_d(l1 * l2 * ec));
+      if (b1) atomic_fix(&d.dudl[b1], fix_d(l0 * l2 * ec));
+      if (b2) atomic_fix(&d.dudl[b2], fix_d(l0 * l1 * ec));
+      for (int a = 0; a < 2; a++) {
+        const float fphi = (float)((a ? ds : dt) * gs * ll);
+        const float rjk2 = dot3(rjk[a], rjk[a]), r = sqrtf(rjk2);
+        const F3 fi = (-fphi * r / dot3(m[a], m[a])) * m[a];
+        const F3 fl = (-fphi * r / dot3(nv[a], nv[a])) * nv[a];
+        const float pp = dot3(rij[a], rjk[a]) / rjk2, qq = dot3(rkl[a], rjk[a]) / rjk2;
+        const F3 fj = (-pp) * fi + (-qq) * fl;
+        const int s0 = d.sl_cmap + 8 * t + 4 * a;
+        put_f(d, s0, fi);
+        put_f(d, s0 + 3, -1.f * fl);
+        put_f(d, s0 + 2, fl - fj);
+        put_f(d, s0 + 1, fj - fi);
+      }
+      e = ec * ll;
     }
   } else {
     // MSLD fixed and variable biases: one thread per block / bias

@@ -143,6 +143,12 @@ struct Dev {
   int ncats;                     // MSLD atom restraint groups
   int *cats_bound, *cats_idx;    // [ncats+1] first member, [members] atoms
   double cats_k;                 // kRestraint
+  int ncmap;                     // CMAP cross terms
+  int4 *cmap_i;                  // [2n] atoms of phi, atoms of psi
+  int4 *cmap_s;                  // [n] (sb0, sb1, sb2, grid)
+  int2 *cmap_g;                  // [grids] (ngrid, first point)
+  double4 *cmap_tab;             // per point (E, dE/dphi, dE/dpsi, d2E/dphi dpsi), grid units
+  int sl_cmap;
   int ncr;                       // charge restraints on Q = sum_b lambda_b q_b
   double *blockcharge;           // [nL] q_b
   double crk1, crk2, crk3, crq0, crw;

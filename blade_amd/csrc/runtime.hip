@@ -40,6 +40,35 @@ static double bf16_value(uint16_t h) {
   return f;
 }
 
+// Slopes dy of the periodic cubic spline through y[0..n) at unit spacing:
+// dy[i-1] + 4 dy[i] + dy[i+1] = 3 (y[i+1] - y[i-1]), indices mod n.  The
+// cyclic tridiagonal system is solved by the Thomas algorithm on the
+// matrix with its corners folded into the diagonal, and the
+// Sherman-Morrison correction for the corners.
+static void periodic_spline_slopes(int n, const double *y, double *dy) {
+  std::vector<double> r(n), u(n, 0.0), cp(n), x(n), z(n);
+  for (int i = 0; i < n; i++) r[i] = 3.0 * (y[(i + 1) % n] - y[(i + n - 1) % n]);
+  const double gam = -4.0;                       // -diagonal
+  auto thomas = [&](const std::vector<double> &rhs, std::vector<double> &out) {
+    // diagonal 4, with 4 - gam at 0 and 4 - 1/gam at n-1; off-diagonals 1
+    const double b0 = 4.0 - gam;
+    out[0] = rhs[0] / b0;
+    cp[0] = 1.0 / b0;
+    for (int i = 1; i < n; i++) {
+      const double bi = (i == n - 1 ? 4.0 - 1.0 / gam : 4.0) - cp[i - 1];
+      cp[i] = 1.0 / bi;
+      out[i] = (rhs[i] - out[i - 1]) / bi;
+    }
+    for (int i = n - 2; i >= 0; i--) out[i] -= cp[i] * out[i + 1];
+  };
+  thomas(r, x);
+  u[0] = gam;
+  u[n - 1] = 1.0;
+  thomas(u, z);
+  const double fac = (x[0] + x[n - 1] / gam) / (1.0 + z[0] + z[n - 1] / gam);
+  for (int i = 0; i < n; i++) dy[i] = x[i] - fac * z[i];
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -504,6 +533,38 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       if (d.ncats) { UPLOAD(d.cats_bound, cb); UPLOAD(d.cats_idx, ci); }
       d.cats_k = t->cats_k;
     }
+    // CMAP grids: the slopes of periodic cubic splines through the grid
+    // values (potential.cxx:460-520) give each point its value, both first
+    // derivatives and the cross derivative, for bicubic Hermite cells
+    d.ncmap = t->n_cmaps;
+    if (d.ncmap) {
+      std::vector<int4> ci(2 * (size_t)d.ncmap), cs(d.ncmap);
+      for (int q = 0; q < d.ncmap; q++) {
+        const int32_t *a = t->cmap_idx + 8 * q;
+        ci[2 * q] = make_int4(a[0], a[1], a[2], a[3]);
+        ci[2 * q + 1] = make_int4(a[4], a[5], a[6], a[7]);
+        cs[q] = make_int4(t->cmap_sb[3 * q], t->cmap_sb[3 * q + 1], t->cmap_sb[3 * q + 2], t->cmap_type[q]);
+      }
+      std::vector<int2> cg(t->n_cmap_types);
+      std::vector<double4> tab;
+      for (int g = 0; g < t->n_cmap_types; g++) {
+        const int ng = t->cmap_ngrid[g];
+        const double *v = t->cmap_grid + t->cmap_off[g];
+        cg[g] = make_int2(ng, (int)tab.size());
+        std::vector<double> fx((size_t)ng * ng), fy((size_t)ng * ng), fxy((size_t)ng * ng), y(ng), dy(ng);
+        for (int j = 0; j < ng; j++) {
+          for (int i = 0; i < ng; i++) y[i] = v[(size_t)ng * i + j];
+          periodic_spline_slopes(ng, y.data(), dy.data());
+          for (int i = 0; i < ng; i++) fx[(size_t)ng * i + j] = dy[i];
+        }
+        for (int i = 0; i < ng; i++) {
+          periodic_spline_slopes(ng, v + (size_t)ng * i, fy.data() + (size_t)ng * i);
+          periodic_spline_slopes(ng, fx.data() + (size_t)ng * i, fxy.data() + (size_t)ng * i);
+        }
+        for (size_t p = 0; p < (size_t)ng * ng; p++) tab.push_back(make_double4(v[p], fx[p], fy[p], fxy[p]));
+      }
+      UPLOAD(d.cmap_i, ci); UPLOAD(d.cmap_s, cs); UPLOAD(d.cmap_g, cg); UPLOAD(d.cmap_tab, tab);
+    }
     // bonded force slots and the atom -> slot CSR (the topology is static)
     d.sl_bond = 0;
     d.sl_angle = d.sl_bond + 2 * d.nbond;
@@ -512,7 +573,8 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     d.sl_nb14 = d.sl_impr + 4 * d.nimpr;
     d.sl_nbex = d.sl_nb14 + 2 * d.nnb14;
     d.sl_cats = d.sl_nbex + 2 * d.nnbex;
-    d.nslot = d.sl_cats + ncm;
+    d.sl_cmap = d.sl_cats + ncm;
+    d.nslot = d.sl_cmap + 8 * d.ncmap;
     std::vector<int> owner(d.nslot);
     for (int q = 0; q < d.nbond; q++)
       for (int k = 0; k < 2; k++) owner[d.sl_bond + 2 * q + k] = t->bond_idx[2 * q + k];
@@ -527,6 +589,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     for (int q = 0; q < d.nnbex; q++)
       for (int k = 0; k < 2; k++) owner[d.sl_nbex + 2 * q + k] = t->nbex_idx[2 * q + k];
     for (int q = 0; q < ncm; q++) owner[d.sl_cats + q] = t->cats_idx[q];
+    for (int q = 0; q < 8 * d.ncmap; q++) owner[d.sl_cmap + q] = t->cmap_idx[q];
     std::vector<int> cstartv(n + 1, 0), pos(d.nslot);
     for (int q = 0; q < d.nslot; q++) cstartv[owner[q] + 1]++;
     for (int a = 0; a < n; a++) cstartv[a + 1] += cstartv[a];

@@ -193,6 +193,38 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
       t.diheD0.push_back(dp.dih0);
     }
   }
+  // CMAP cross terms: ordered 8-type match (potential.cxx:770-800), up to
+  // three lambda scalings (msld/msld.cu:610); one grid per distinct type
+  {
+    std::map<std::vector<std::string>, int> seen;
+    for (auto &c : s.cmaps) {
+      int idx[8];
+      std::vector<std::string> ty;
+      for (int k = 0; k < 8; k++) { idx[k] = c[k]; ty.push_back(type(idx[k])); }
+      auto it = p.cmaps.find(ty);
+      if (it == p.cmaps.end()) {
+        std::string nm;
+        for (auto &x : ty) nm += x + " ";
+        fail("no CMAP parameter for " + nm);
+      }
+      auto sn = seen.find(ty);
+      int tid;
+      if (sn == seen.end()) {
+        tid = (int)t.cmapNgrid.size();
+        seen[ty] = tid;
+        t.cmapNgrid.push_back(it->second.first);
+        t.cmapOff.push_back((int)t.cmapGrid.size());
+        t.cmapGrid.insert(t.cmapGrid.end(), it->second.second.begin(), it->second.second.end());
+      } else {
+        tid = sn->second;
+      }
+      int sb[3];
+      sc.bonded(idx, 8, 5, 3, sb);
+      t.cmapIdx.insert(t.cmapIdx.end(), idx, idx + 8);
+      t.cmapSb.insert(t.cmapSb.end(), sb, sb + 3);
+      t.cmapType.push_back(tid);
+    }
+  }
   // impropers with the reference's wildcard search order (potential.cxx:707-760)
   for (auto &d : s.imprs) {
     int idx[4] = {d[0], d[1], d[2], d[3]};
@@ -465,6 +497,10 @@ const bm_topology *Topology::view() {
   v.cats_idx = catsIdx.data();
   v.cats_k = catsK;
   v.block_charge = blockCharge.data();
+  v.n_cmaps = (int)cmapType.size();
+  v.cmap_idx = cmapIdx.data(); v.cmap_sb = cmapSb.data(); v.cmap_type = cmapType.data();
+  v.n_cmap_types = (int)cmapNgrid.size();
+  v.cmap_ngrid = cmapNgrid.data(); v.cmap_off = cmapOff.data(); v.cmap_grid = cmapGrid.data();
   v.cr_k1 = crK1; v.cr_k2 = crK2; v.cr_k3 = crK3; v.cr_q0 = crQ0; v.cr_w = crW;
   return &top;
 }

@@ -35,6 +35,7 @@ struct Parameters {
   std::map<std::string, ImprParam> imprs;
   std::map<std::string, NbondParam> nbonds;
   std::map<std::string, NbondParam> nbfix;  // key canon({t1,t2})
+  std::map<std::vector<std::string>, std::pair<int, std::vector<double>>> cmaps;  // ordered 8 types -> (ngrid, grid)
 };
 
 struct AtomRec {
@@ -47,6 +48,7 @@ struct Structure {
   std::vector<std::array<int, 2>> bonds;
   std::vector<std::array<int, 3>> angles;
   std::vector<std::array<int, 4>> dihes, imprs;
+  std::vector<std::array<int, 8>> cmaps;
   bool shakeHbond = false;
 };
 
@@ -125,6 +127,7 @@ struct Topology {
   std::vector<double> thetaMass, fixedBias, thetaFriction;
   std::vector<int> vbI, vbJ, vbType, vbN; std::vector<double> vbL0, vbK;
   std::vector<int> catsBound, catsIdx; double catsK = 0;
+  std::vector<int> cmapIdx, cmapSb, cmapType, cmapNgrid, cmapOff; std::vector<double> cmapGrid;
   std::vector<double> blockCharge; double crK1 = 0, crK2 = 0, crK3 = 0, crQ0 = 0, crW = 0;
   bm_topology top{};
   const bm_topology *view();

@@ -82,6 +82,8 @@ class Spec:
     angles: list = field(default_factory=list)
     dihes: list = field(default_factory=list)
     imprs: list = field(default_factory=list)
+    cmap_par: dict = field(default_factory=dict)     # (t1..t8) -> (ngrid, ngrid) kcal/mol, phi rows
+    cmaps: list = field(default_factory=list)        # 8 atoms, 0-based
     pos: np.ndarray = None                           # (N,3) A
     vel: np.ndarray = None                           # (N,3) A/AKMA
     shake: bool = True
@@ -931,4 +933,41 @@ def msld_small(seed=0xB1ADE0A0, n_water=150, n_poly=60, box=20.0, grid=20):
     spec.beta = 0.34
     spec.grid, spec.order = (grid, grid, grid), 6
     spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    return spec
+
+
+def add_cmaps(spec, count=24, ngrid=24, seed=7, noise=0.05):
+    """CMAP cross terms over `count` paths a-b-c-d-e of bonded atoms (phi on
+    a-b-c-d, psi on b-c-d-e), substituent atoms first so that terms carry
+    lambda scaling; one grid per distinct 8-type key, a smooth periodic
+    surface of a few Fourier modes plus grid noise (kcal/mol)"""
+    rng = np.random.default_rng(seed)
+    nbr = {}
+    for a, b in spec.bonds:
+        nbr.setdefault(a, []).append(b)
+        nbr.setdefault(b, []).append(a)
+    ab = spec.atom_block if spec.atom_block is not None else np.zeros(spec.n, int)
+    starts = sorted(nbr, key=lambda a: (-int(ab[a]), a))
+    paths, used = [], set()
+    for a in starts:
+        if len(paths) >= count:
+            break
+        walk = [[a]]
+        for _ in range(4):
+            walk = [w + [x] for w in walk for x in nbr[w[-1]] if x not in w]
+        for p in walk:
+            if tuple(p) not in used and tuple(p[::-1]) not in used:
+                used.add(tuple(p))
+                paths.append(p)
+                break
+    ang = np.arange(ngrid) * 2 * np.pi / ngrid - np.pi
+    P, Q = np.meshgrid(ang, ang, indexing="ij")
+    for a, b, c, d, e in paths:
+        q = (a, b, c, d, b, c, d, e)
+        key = tuple(spec.atoms[x][4] for x in q)
+        if key not in spec.cmap_par:
+            k = rng.uniform(0.2, 0.8, 4)
+            spec.cmap_par[key] = (k[0] * np.cos(P + 0.3) + k[1] * np.sin(2 * Q) + k[2] * np.cos(P - Q)
+                                  + k[3] * np.sin(P + 2 * Q) + noise * rng.standard_normal((ngrid, ngrid)))
+        spec.cmaps.append(q)
     return spec

@@ -152,8 +152,10 @@ class BladeSystem:
     def set_fft_mode(self, mode):
         self.L.blade_set_fft_mode(self.h, mode)
 
-    def set_output(self, nrg=None, lmd=None, freq_nrg=10, freq_lmd=10):
-        """.nrg and .lmd files written by run_dynamics every freq steps"""
+    def set_output(self, nrg=None, lmd=None, freq_nrg=10, freq_lmd=10, lmd_text=True):
+        """.nrg and .lmd files written by run_dynamics every freq steps; the
+        .lmd frames are text lines, or XDR frames with lmd_text=False"""
+        self.L.blade_set_lmd_format(self.h, int(bool(lmd_text)))
         check(self.L.blade_set_output_nrg(self.h, str(nrg).encode() if nrg else None, int(freq_nrg) if nrg else 0),
               "blade_set_output_nrg")
         check(self.L.blade_set_output_lmd(self.h, str(lmd).encode() if lmd else None, int(freq_lmd) if lmd else 0),
@@ -261,6 +263,18 @@ def soa(pos):
     return np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T).ravel()
 
 
+def read_lmd_xdr(path):
+    """XDR .lmd frames (io/io.cxx:347-370): big-endian int32 block count,
+    then that many big-endian IEEE singles; returns (frames, count) float32"""
+    raw = open(path, "rb").read()
+    out, o = [], 0
+    while o < len(raw):
+        n = int(np.frombuffer(raw, ">i4", 1, o)[0])
+        out.append(np.frombuffer(raw, ">f4", n, o + 4).astype(np.float32))
+        o += 4 + 4 * n
+    return np.array(out)
+
+
 def build(spec, gpus=(0,)):
     """Create a BladeSystem from a synthetic Spec through the blade_* API."""
     s = BladeSystem(gpus)
@@ -276,6 +290,13 @@ def build(spec, gpus=(0,)):
             s.add_parameter_dihes(*key, k, n, d0)
     for key, (k, n, d0) in spec.impr_par.items():
         s.add_parameter_imprs(*key, k, n, d0)
+    for key, grid in getattr(spec, "cmap_par", {}).items():
+        g = np.asarray(grid, float)
+        tb = [_b(x) for x in key]
+        s.L.blade_add_parameter_cmaps(s.h, *tb, g.shape[0])
+        for i in range(g.shape[0]):
+            for j in range(g.shape[1]):
+                s.L.blade_add_parameter_cmaps_fill(s.h, *tb, i + 1, j + 1, float(g[i, j]))
     for t, (eps, sig, eps14, sig14, e14, comb) in spec.nb_par.items():
         s.add_parameter_nbonds(t, eps, sig, eps14, sig14, e14, comb)
     s.init_structure()
@@ -289,6 +310,8 @@ def build(spec, gpus=(0,)):
         s.add_dihe(a + 1, b + 1, c + 1, d + 1)
     for a, b, c, d in spec.imprs:
         s.add_impr(a + 1, b + 1, c + 1, d + 1)
+    for q in getattr(spec, "cmaps", []):
+        s.L.blade_add_cmap(s.h, *[int(a) + 1 for a in q])
     s.add_shake(spec.shake)
     s.init_coordinates(spec.n)
     for i in range(spec.n):

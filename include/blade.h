@@ -150,6 +150,10 @@ void blade_set_domdec_frequency(System *system, int steps);
    Energies are computed on the .nrg steps.  path NULL or freq 0 closes. */
 int blade_set_output_nrg(System *system, const char *path, int freq);
 int blade_set_output_lmd(System *system, const char *path, int freq);
+/* .lmd frame format, the reference's "run setvariable hrlmd" (run/run.cu:324):
+   nonzero (the default) writes text lines, zero XDR frames of the block
+   count and single-precision lambdas (io/io.cxx:347-370) */
+void blade_set_lmd_format(System *system, int humanReadable);
 /* Binary checkpoint of the exact FP64 state (step, box, positions,
    velocities, theta, theta velocities), replacing the reference's "%f"
    text checkpoint (io/io.cxx:416-536).  A run continued from a checkpoint

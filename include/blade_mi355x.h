@@ -150,6 +150,19 @@ typedef struct bm_topology {
      cr_k3: E = k3/2 dQ^2, dQ = distance of Q - q0 from the band [-w, w] */
   const double *block_charge;      /* [nL] q_b (lambdaCharge)            */
   double cr_k1, cr_k2, cr_k3, cr_q0, cr_w;
+
+  /* CMAP cross terms (bonded/bonded.cu:531-720): E = lambda-scaled bicubic
+     interpolation of a periodic grid in (phi, psi), phi the dihedral of
+     atoms 0-3 and psi of atoms 4-7; grid point (i, j) at
+     (-180 + i 360/ngrid, -180 + j 360/ngrid) degrees, kcal/mol */
+  int32_t n_cmaps;
+  const int32_t *cmap_idx;         /* [8n] */
+  const int32_t *cmap_sb;          /* [3n] */
+  const int32_t *cmap_type;        /* [n] grid of each term */
+  int32_t n_cmap_types;
+  const int32_t *cmap_ngrid;       /* [types] points per 360 degrees */
+  const int32_t *cmap_off;         /* [types] first value in cmap_grid */
+  const double *cmap_grid;         /* row-major [ngrid(phi)][ngrid(psi)] */
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */

@@ -652,6 +652,127 @@ void nbex(const bm_topology *t, const bm_runparams *rp, const double *pos,
   }
 }
 
+// CMAP (bonded/bonded.cu:531-720, system/potential.cxx:460-520).  Grid
+// slopes per grid unit are those of the periodic cubic spline, found here
+// by Gauss-Seidel sweeps on dy[i-1] + 4 dy[i] + dy[i+1] = 3 (y[i+1] - y[i-1])
+// (diagonally dominant, each sweep at least halves the error); the cell is
+// evaluated as nested one-dimensional cubic Hermite interpolation, first
+// along psi, then along phi.  Energy scaled by the product of up to three
+// lambdas.
+static void spline_slopes_gs(int n, const std::vector<double> &y, std::vector<double> &dy) {
+  dy.assign(n, 0.0);
+  for (int sweep = 0; sweep < 200; sweep++)
+    for (int i = 0; i < n; i++)
+      dy[i] = (3 * (y[(i + 1) % n] - y[(i + n - 1) % n]) - dy[(i + n - 1) % n] - dy[(i + 1) % n]) / 4;
+}
+
+// cubic Hermite on [0,1]: value and derivative from end values and slopes
+static void hermite(double p0, double p1, double m0, double m1, double t, double &v, double &dv) {
+  const double t2 = t * t, t3 = t2 * t;
+  v = (2 * t3 - 3 * t2 + 1) * p0 + (t3 - 2 * t2 + t) * m0 + (-2 * t3 + 3 * t2) * p1 + (t3 - t2) * m1;
+  dv = (6 * t2 - 6 * t) * p0 + (3 * t2 - 4 * t + 1) * m0 + (-6 * t2 + 6 * t) * p1 + (3 * t2 - 2 * t) * m1;
+}
+
+void cmap(const bm_topology *t, const bm_runparams *rp, const double *pos, const double *lambda, double *force,
+          double *dudl, double *energy) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  // per grid: value, d/dphi, d/dpsi, d2/dphi dpsi at every point
+  std::vector<std::vector<double>> F(t->n_cmap_types), FX(F.size()), FY(F.size()), FXY(F.size());
+  for (int g = 0; g < t->n_cmap_types; g++) {
+    const int ng = t->cmap_ngrid[g];
+    const double *v = t->cmap_grid + t->cmap_off[g];
+    F[g].assign(v, v + ng * ng);
+    FX[g].assign(ng * ng, 0.0);
+    FY[g].assign(ng * ng, 0.0);
+    FXY[g].assign(ng * ng, 0.0);
+    std::vector<double> y(ng), dy;
+    for (int j = 0; j < ng; j++) {
+      for (int i = 0; i < ng; i++) y[i] = v[ng * i + j];
+      spline_slopes_gs(ng, y, dy);
+      for (int i = 0; i < ng; i++) FX[g][ng * i + j] = dy[i];
+    }
+    for (int i = 0; i < ng; i++) {
+      for (int j = 0; j < ng; j++) y[j] = v[ng * i + j];
+      spline_slopes_gs(ng, y, dy);
+      for (int j = 0; j < ng; j++) FY[g][ng * i + j] = dy[j];
+      for (int j = 0; j < ng; j++) y[j] = FX[g][ng * i + j];
+      spline_slopes_gs(ng, y, dy);
+      for (int j = 0; j < ng; j++) FXY[g][ng * i + j] = dy[j];
+    }
+  }
+  for (int c = 0; c < t->n_cmaps; c++) {
+    const int *a = t->cmap_idx + 8 * c;
+    const int g = t->cmap_type[c], ng = t->cmap_ngrid[g];
+    Vec3 rij[2], rjk[2], rkl[2], m[2], nv[2];
+    double ang[2];
+    for (int h = 0; h < 2; h++) {
+      const int *q = a + 4 * h;
+      rij[h] = minimg(getp(pos, n, q[0]) - getp(pos, n, q[1]), L);
+      rjk[h] = minimg(getp(pos, n, q[1]) - getp(pos, n, q[2]), L);
+      rkl[h] = minimg(getp(pos, n, q[2]) - getp(pos, n, q[3]), L);
+      m[h] = cross(rij[h], rjk[h]);
+      nv[h] = cross(rjk[h], rkl[h]);
+      ang[h] = std::atan2(norm(cross(m[h], nv[h])), dot(m[h], nv[h]));
+      if (dot(rij[h], nv[h]) > 0) ang[h] = -ang[h];
+    }
+    // grid coordinates: point k at -180 + k 360/ng degrees
+    const double step = 2 * M_PI / ng;
+    int cell[2];
+    double frac[2];
+    for (int h = 0; h < 2; h++) {
+      const double x = (ang[h] + M_PI) / step;
+      const double fl = std::floor(x);
+      frac[h] = x - fl;
+      cell[h] = (((long)fl % ng) + ng) % ng;
+    }
+    // the reference bins with floor(angle/step) + ng/2: for odd ng the
+    // cell origin shifts by half a step
+    if (ng % 2) {
+      for (int h = 0; h < 2; h++) {
+        const double x = ang[h] / step;
+        const double fl = std::floor(x);
+        frac[h] = x - fl;
+        cell[h] = ((((long)fl + ng / 2) % ng) + ng) % ng;
+      }
+    }
+    const int i0 = cell[0], i1 = (i0 + 1) % ng, j0 = cell[1], j1 = (j0 + 1) % ng;
+    auto at = [&](const std::vector<double> &A, int i, int j) { return A[ng * i + j]; };
+    // along psi at phi rows i0 and i1: values and phi-slopes, with psi derivatives
+    double v0, dv0, v1, dv1, s0, ds0, s1, ds1;
+    hermite(at(F[g], i0, j0), at(F[g], i0, j1), at(FY[g], i0, j0), at(FY[g], i0, j1), frac[1], v0, dv0);
+    hermite(at(F[g], i1, j0), at(F[g], i1, j1), at(FY[g], i1, j0), at(FY[g], i1, j1), frac[1], v1, dv1);
+    hermite(at(FX[g], i0, j0), at(FX[g], i0, j1), at(FXY[g], i0, j0), at(FXY[g], i0, j1), frac[1], s0, ds0);
+    hermite(at(FX[g], i1, j0), at(FX[g], i1, j1), at(FXY[g], i1, j0), at(FXY[g], i1, j1), frac[1], s1, ds1);
+    double e, dedt, dpsi, unused;
+    hermite(v0, v1, s0, s1, frac[0], e, dedt);
+    hermite(dv0, dv1, ds0, ds1, frac[0], dpsi, unused);
+    double l[3], ll = 1;
+    int b[3];
+    for (int k = 0; k < 3; k++) {
+      b[k] = t->cmap_sb[3 * c + k] & 0xFFFF;
+      l[k] = b[k] ? lambda[b[k]] : 1.0;
+      ll *= l[k];
+    }
+    for (int k = 0; k < 3; k++)
+      if (b[k]) dudl[b[k]] += e * ll / l[k];
+    energy[BM_ECMAP] += e * ll;
+    const double dE[2] = {dedt / step * ll, dpsi / step * ll};
+    for (int h = 0; h < 2; h++) {
+      const int *q = a + 4 * h;
+      const double rjk2 = dot(rjk[h], rjk[h]), r = std::sqrt(rjk2);
+      Vec3 fi = (-dE[h] * r / dot(m[h], m[h])) * m[h];
+      Vec3 fl = (-dE[h] * r / dot(nv[h], nv[h])) * nv[h];
+      const double p = dot(rij[h], rjk[h]) / rjk2, s = dot(rkl[h], rjk[h]) / rjk2;
+      Vec3 fj = (-p) * fi + (-s) * fl;
+      addf(force, n, q[0], fi);
+      addf(force, n, q[3], -1.0 * fl);
+      addf(force, n, q[2], fl - fj);
+      addf(force, n, q[1], fj - fi);
+    }
+  }
+}
+
 // Fixed and variable lambda biases (msld/msld.cu:1000-1130).
 void msldbias(const bm_topology *t, const double *lambda, double *dudl, double *energy) {
   for (int b = 0; b < t->n_blocks; b++) {
@@ -908,6 +1029,7 @@ long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
     if (terms & ORC_NBEX) nbex(t, rp, pos, lambda, force, dudl, energy);
   }
   if (terms & ORC_BONDED) bonded(t, rp, pos, lambda, force, dudl, energy);
+  if ((terms & ORC_BONDED) && t->n_cmaps > 0) cmap(t, rp, pos, lambda, force, dudl, energy);
   if (terms & ORC_NB14) nb14(t, rp, pos, lambda, force, dudl, energy);
   if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1) msldbias(t, lambda, dudl, energy);
   if ((terms & ORC_MSLDBIAS) && t->n_cats > 0) cats(t, rp, pos, force, energy);

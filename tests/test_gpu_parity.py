@@ -229,7 +229,8 @@ def test_checkpoint_restart_is_bit_identical_and_outputs(tmp_path):
     a list rebuild step), a fresh system reading it and 20 more steps: the
     states are bit-identical.  The run's .nrg and .lmd files hold the
     reference's line layout at their steps: the potential column is the sum
-    of the term columns and the total is potential + kinetic."""
+    of the term columns and the total is potential + kinetic; the continued
+    run's XDR .lmd frames hold the same lambdas in single precision."""
     sp = relaxed("msld_small")
     a = system.build(sp)
     a.set_output(nrg=tmp_path / "a.nrg", lmd=tmp_path / "a.lmd", freq_nrg=10, freq_lmd=5)
@@ -240,7 +241,9 @@ def test_checkpoint_restart_is_bit_identical_and_outputs(tmp_path):
     b.close()
     c = system.build(sp)
     c.read_checkpoint(tmp_path / "ck.bin")
+    c.set_output(lmd=tmp_path / "c.lmd", freq_lmd=5, lmd_text=False)
     c.run_dynamics(20, freq_nrg=10)
+    c.set_output()
     xa, va, _, _ = system.state(a, sp.n, sp.nblocks)
     xc, vc, _, _ = system.state(c, sp.n, sp.nblocks)
     assert np.array_equal(xa, xc) and np.array_equal(va, vc)
@@ -259,3 +262,8 @@ def test_checkpoint_restart_is_bit_identical_and_outputs(tmp_path):
     assert all(len(l) == sp.nblocks for l in lmd)
     lam = [float(v) for v in lmd[-1][1:]]
     assert abs(sum(lam) - 1.0) < 1e-5
+    # the continued run's XDR frames (steps 20..35) against the text lines
+    xdr = system.read_lmd_xdr(tmp_path / "c.lmd")
+    txt = np.array([[float(v) for v in l[1:]] for l in lmd[4:]])
+    assert xdr.shape == txt.shape == (4, sp.nblocks - 1)
+    assert np.max(np.abs(xdr - txt)) <= 5.1e-7

@@ -88,6 +88,14 @@ def test_charge_restraints(k, q0, w):
     assert abs(eg[19] - eo[19]) <= 2.0 ** -32 + 1e-9 * abs(eo[19]), (eg[19], eo[19])
 
 
+def test_cmap_terms():
+    """CMAP cross terms (24 terms, several lambda-scaled) against the oracle
+    at the path's force, energy and dU/dlambda bars"""
+    from blade_amd import synthetic
+    sp = synthetic.add_cmaps(copy.deepcopy(spec("two_site_small")), count=24)
+    _check(sp, "cmap")
+
+
 def test_soft_core_site():
     _check(_variant("msld_small", soft_core=True), "soft core")
 

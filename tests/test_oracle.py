@@ -358,6 +358,65 @@ def test_charge_restraints_dudl_and_flat_band():
     assert e1 != 0.0 and abs(e2 - 2 * e1) < 1e-12 * abs(e1)
 
 
+def _dihedral(x, n, L, q):
+    P = [np.array([x[k * n + a] for k in range(3)]) for a in q]
+    mi = lambda d: d - L * np.round(d / L)
+    rij, rjk, rkl = mi(P[0] - P[1]), mi(P[1] - P[2]), mi(P[2] - P[3])
+    m, nv = np.cross(rij, rjk), np.cross(rjk, rkl)
+    phi = math.atan2(np.linalg.norm(np.cross(m, nv)), m @ nv)
+    return -phi if rij @ nv > 0 else phi
+
+
+def test_cmap_interpolates_smooth_surface_and_gradients():
+    """CMAP grids sampled from the smooth surface cos(phi + 0.3) + 0.7
+    sin(2 psi) + 0.4 cos(phi - 2 psi): the bicubic spline reproduces the
+    surface at the terms' own (phi, psi) to 2e-3 kcal/mol (a wrong grid
+    origin, axis order, sign or slope fails by far more), forces are the
+    energy's gradient and dU/dlambda its lambda derivative."""
+    sp = synthetic.add_cmaps(synthetic.two_site_small(), count=24, noise=0.0)
+    f = lambda a, b: np.cos(a + 0.3) + 0.7 * np.sin(2 * b) + 0.4 * np.cos(a - 2 * b)
+    ang = np.arange(24) * 2 * np.pi / 24 - np.pi
+    P, Q = np.meshgrid(ang, ang, indexing="ij")
+    for key in sp.cmap_par:
+        sp.cmap_par[key] = f(P, Q)
+    s = system.build(sp)
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    t = top.contents
+    assert t.n_cmaps == len(sp.cmaps) >= 20
+    assert len({t.cmap_sb[3 * c] & 0xFFFF for c in range(t.n_cmaps)} - {0}) >= 2
+    n, L = sp.n, np.array(sp.box)
+    x = system.soa(sp.pos)
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    fo, do, e, _ = orc.forces(top, rp, x, lam, orc.BONDED)
+    e0 = orc.forces(top, rp, x, lam, orc.BONDED)[2]
+    want, worst = 0.0, 0.0
+    for c, q in enumerate(sp.cmaps):
+        ll = np.prod([lam[t.cmap_sb[3 * c + k] & 0xFFFF] if t.cmap_sb[3 * c + k] & 0xFFFF else 1.0
+                      for k in range(3)])
+        want += ll * f(_dihedral(x, n, L, q[:4]), _dihedral(x, n, L, q[4:]))
+    assert abs(e[E_CMAP] - want) < 2e-3 * len(sp.cmaps), (e[E_CMAP], want)
+    atoms = sorted({a for q in sp.cmaps[:6] for a in q})
+    h = 1e-5
+    for a in atoms:
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[k * n + a] += h
+            xm[k * n + a] -= h
+            fd = (orc.forces(top, rp, xp, lam, orc.BONDED)[2][E_POT]
+                  - orc.forces(top, rp, xm, lam, orc.BONDED)[2][E_POT]) / (2 * h)
+            worst = max(worst, abs(fd - fo[k * n + a]))
+    assert worst < 1e-5, worst
+    for b in range(1, t.n_blocks):
+        lp, lm = lam.copy(), lam.copy()
+        lp[b] += 1e-6
+        lm[b] -= 1e-6
+        fd = (orc.forces(top, rp, x, lp, orc.BONDED)[2][E_POT]
+              - orc.forces(top, rp, x, lm, orc.BONDED)[2][E_POT]) / 2e-6
+        assert abs(fd - do[b]) < 1e-5 * max(1.0, abs(do[b])), (b, fd, do[b])
+    assert e0[E_CMAP] == e[E_CMAP]
+
+
 def test_dudl_matches_lambda_differences():
     name = "msld_small"
     s = built(name)
