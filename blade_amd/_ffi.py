@@ -52,6 +52,10 @@ class Topology(C.Structure):
         ("cr_q0", C.c_double), ("cr_w", C.c_double),
         ("n_cmaps", C.c_int32), ("cmap_idx", _i32p), ("cmap_sb", _i32p), ("cmap_type", _i32p),
         ("n_cmap_types", C.c_int32), ("cmap_ngrid", _i32p), ("cmap_off", _i32p), ("cmap_grid", _f64p),
+        ("tcoll_k", _f64p), ("tcoll_n", _f64p), ("tinde_k", _f64p),
+        ("piecewise", C.c_int32), ("pw_width", C.c_double), ("pw_k", C.c_double),
+        ("edge_k", C.c_double), ("edge_n", C.c_double), ("edge_alpha", C.c_double), ("edge_phi", C.c_double),
+        ("fix_lambda", C.c_int32),
     ]
 
     def array(self, name, count, width=1):
@@ -143,6 +147,11 @@ def lib():
         "blade_add_msld_bias": (None, [vp, i, i, i, d, d, i]),
         "blade_add_msld_block_friction": (None, [vp, i, d]),
         "blade_set_msld_block_fixed": (None, [vp, i, i]),
+        "blade_add_msld_thetacollbias": (None, [vp, i, i, d, d]),
+        "blade_add_msld_thetaindebias": (None, [vp, i, i, d]),
+        "blade_set_msld_thetaedgebias": (None, [vp, d, d, d, d]),
+        "blade_set_msld_piecewise_constraint": (None, [vp, i, d, d]),
+        "blade_add_msld_softbond": (None, [vp, i, i]),
         "blade_add_msld_atomrestraint": (None, [vp]),
         "blade_add_msld_charges": (None, [vp, d, d, d, d, d]),
         "blade_add_parameter_cmaps": (None, [vp] + [s] * 8 + [i]),

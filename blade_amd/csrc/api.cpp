@@ -231,9 +231,44 @@ void blade_add_msld_flags(System *s, double gamma, double fnex, int useSoftCore,
   m.useSoftCore14 = useSoftCore14;
   m.msldEwaldType = msldEwaldType;
   m.fix = fix;
-  if (fix) mark_unsupported(s, "msld fix (ffix)");
   (void)softNotBondExponent;
 }
+// theta biases, per site (msld.cu:1675-1695): the first call sizes the
+// table with `sites` entries, later calls fill entry i (1-based)
+void blade_add_msld_thetacollbias(System *s, int sites, int i, double k, double n) {
+  auto &m = *s->msld;
+  if (m.thetaCollK.empty()) {
+    m.thetaCollK.assign(std::max(sites, 0), 0.0);
+    m.thetaCollN.assign(std::max(sites, 0), 0.0);
+  }
+  if (i < 1 || i > (int)m.thetaCollK.size()) { api_fail(s, "theta collective bias site out of range"); return; }
+  m.thetaCollK[i - 1] = k;
+  m.thetaCollN[i - 1] = n;
+}
+void blade_add_msld_thetaindebias(System *s, int sites, int i, double k) {
+  auto &m = *s->msld;
+  if (m.thetaIndeK.empty()) m.thetaIndeK.assign(std::max(sites, 0), 0.0);
+  if (i < 1 || i > (int)m.thetaIndeK.size()) { api_fail(s, "theta independent bias site out of range"); return; }
+  m.thetaIndeK[i - 1] = k;
+}
+// edge (target) bias of the piecewise constraint (msld.cu:1697-1704)
+void blade_set_msld_thetaedgebias(System *s, double k, double N, double alpha, double phi) {
+  auto &m = *s->msld;
+  m.edgeK = k;
+  m.edgeN = N;
+  m.edgeAlpha = alpha;
+  m.edgePhi = phi;
+}
+// piecewise lambda constraint (msld.cu:1706-1712)
+void blade_set_msld_piecewise_constraint(System *s, int do_imp, double width, double k) {
+  auto &m = *s->msld;
+  m.piecewise = do_imp != 0;
+  m.wellWidth = width;
+  m.wellK = k;
+}
+// soft bonds (msld.cu:1714-1721) are recorded; a system that has any is
+// rejected when its terms are derived
+void blade_add_msld_softbond(System *s, int, int) { s->msld->softBondCount++; }
 void blade_add_msld_bias(System *s, int i, int j, int type, double l0, double k, int n) {
   if (type <= 0 || type > 12) { api_fail(s, "type of variable bias is not a recognized type"); return; }
   s->msld->varBias.push_back(VarBias{i - 1, j - 1, type, l0, k, n});

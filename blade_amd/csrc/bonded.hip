@@ -142,6 +142,80 @@ __device__ void nb14_ef(const Dev &d, double4 p, double r, double &f, double &e)
   }
 }
 
+// Theta biases of site s in FP64 (msld/msld.cu:1155-1264): dU/dtheta of
+// each of the site's blocks into the dtheta accumulators, the energy
+// returned.  Collective: A = sum h_j^n - 1, B = sum l_j^n - (nb - 1) with
+// h = (1 + sin theta)/2, l = (1 - sin theta)/2, E = k/2 (A^2 + B^2) and the
+// reference's force k n (A h^(n-1) - B l^(n-1)) cos theta (twice the theta
+// derivative of that energy, kept as the reference has it).  Independent:
+// E = k (1 - l^4), force 2 k l^3 cos theta.  Piecewise constraint (sites
+// after the environment): a flat-bottom harmonic beyond +-nb*width, and
+// with |edge_k| > 1e-4 the edge bias on N = sum sigmoid(alpha (theta_j -
+// t_s + phi)) above its target, whose theta_b derivative carries the
+// root's shift g_b / G (g = 3 max(theta - t_s, 0)^2).
+__device__ double theta_bias_site(const Dev &d, int s) {
+  const int b0 = d.site_bound[s], b1 = d.site_bound[s + 1], nb = b1 - b0;
+  const double *th = d.pos + 3 * (size_t)d.n;
+  double e = 0.0;
+  if (d.tcoll_k && d.tcoll_k[s] != 0.0) {
+    const double k = d.tcoll_k[s], n = d.tcoll_n[s];
+    double A = -1.0, B = -(double)(nb - 1);
+    for (int b = b0; b < b1; b++) {
+      const double sn = sin(th[b]);
+      A += pow(0.5 + 0.5 * sn, n);
+      B += pow(0.5 - 0.5 * sn, n);
+    }
+    e += 0.5 * k * (A * A + B * B);
+    for (int b = b0; b < b1; b++) {
+      double sn, cs;
+      sincos(th[b], &sn, &cs);
+      const double f = k * n * (A * pow(0.5 + 0.5 * sn, n - 1) - B * pow(0.5 - 0.5 * sn, n - 1)) * cs;
+      atomic_fix(&d.dtheta[b], fix_d(f));
+    }
+  }
+  if (d.tinde_k && d.tinde_k[s] != 0.0) {
+    const double k = d.tinde_k[s];
+    for (int b = b0; b < b1; b++) {
+      double sn, cs;
+      sincos(th[b], &sn, &cs);
+      const double l = 0.5 - 0.5 * sn, l3 = l * l * l;
+      e += k * (1.0 - l3 * l);
+      atomic_fix(&d.dtheta[b], fix_d(2.0 * k * l3 * cs));
+    }
+  }
+  if (d.piecewise && s > 0) {
+    const double w = nb * d.pw_width;
+    for (int b = b0; b < b1; b++) {
+      double x = 0.0;
+      if (th[b] > w) x = th[b] - w;
+      else if (th[b] < -w) x = th[b] + w;
+      e += 0.5 * d.pw_k * x * x;
+      if (x != 0.0) atomic_fix(&d.dtheta[b], fix_d(d.pw_k * x));
+    }
+    if (fabs(d.edge_k) > 1e-4) {
+      const double t0 = d.theta0[s];
+      double G = 0.0, N = 0.0, ssum = 0.0;
+      for (int b = b0; b < b1; b++) {
+        const double q = th[b] - t0;
+        G += q > 0.0 ? 3.0 * q * q : 0.0;
+        const double x = exp(-d.edge_alpha * (q + d.edge_phi));
+        N += 1.0 / (1.0 + x);
+        ssum += d.edge_alpha * x / ((1.0 + x) * (1.0 + x));
+      }
+      const double over = N > d.edge_n ? N - d.edge_n : 0.0;
+      e += 0.5 * d.edge_k * over * over;
+      if (over != 0.0)
+        for (int b = b0; b < b1; b++) {
+          const double q = th[b] - t0, gb = q > 0.0 ? 3.0 * q * q : 0.0;
+          const double x = exp(-d.edge_alpha * (q + d.edge_phi));
+          const double sg = d.edge_alpha * x / ((1.0 + x) * (1.0 + x));
+          atomic_fix(&d.dtheta[b], fix_d(d.edge_k * over * (sg - (gb / G) * ssum)));
+        }
+    }
+  }
+  return e;
+}
+
 struct BondedLaunch {
   int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_cats, b_cmap, b_bias;  // first block of each family
 };
@@ -495,6 +569,11 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       if (fj != 0.0) atomic_fix(&d.dudl[j], fix_d(fj));
       e += ev;
     }
+    if (d.nl > 1)
+      for (int st = t; st < d.nsites; st += blockDim.x) {
+        const double et = theta_bias_site(d, st);
+        if (energy && et != 0.0) atomic_fix(&d.energy[BM_ETHETA], fix_d(et));
+      }
   }
   if (energy) wave_atomic_fix(&d.energy[slot], fix_d(e));
 }

@@ -155,6 +155,13 @@ struct Dev {
   int nvb;
   int4 *vb_i;                    // (i, j, type, n)
   double2 *vb_p;                 // (l0, k)
+  // theta biases and the piecewise lambda constraint
+  const double *tcoll_k, *tcoll_n, *tinde_k;   // [nsites] or null
+  int piecewise, fixlam;         // lambda = max(theta - t_s, 0)^3; lambda held at its initial theta
+  double pw_width, pw_k;         // flat-bottom harmonic on theta beyond +-nb*pw_width
+  double edge_k, edge_n, edge_alpha, edge_phi;   // edge bias on the smooth substituent count
+  double *theta0;                // [nsites] piecewise root t_s of the current theta
+  long long *dtheta;             // [nL] dU/dtheta of the theta biases (fixed point), zeroed by the pack
   double fnex;
   int softcore, softcore14;
   // integrator

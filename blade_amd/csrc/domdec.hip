@@ -377,7 +377,10 @@ __device__ __forceinline__ void dd_pack_body(const Dev &d, int bid) {
   const int nb = *d.nblocks;
   const int nf = 3 * d.n + d.nl;
   if (s < nf) d.force[s] = 0;
-  if (s < d.nl) d.dudl[s] = 0;
+  if (s < d.nl) {
+    d.dudl[s] = 0;
+    d.dtheta[s] = 0;
+  }
   if (s < BM_NTERMS) d.energy[s] = 0;
   if (s == 0) {
     d.workctr[0] = 0;

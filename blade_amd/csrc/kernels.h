@@ -24,7 +24,8 @@ void launch_normals(const Dev &d, hipStream_t st);
 void launch_update(const Dev &d, bool energy_step, hipStream_t st);
 // needs launch_normals of the same step first
 void launch_update_after_kick(const Dev &d, hipStream_t st);
-void launch_theta_to_lambda(const Dev &d, hipStream_t st);
+// init: a new theta from the host (with the fix flag it sets the held lambda)
+void launch_theta_to_lambda(const Dev &d, hipStream_t st, bool init = false);
 // MC barostat: back up and scale every atom, rectify constraint groups
 // (oldbox: the box before the move); restore the backup
 void launch_volume_move(const Dev &d, const double *scale, const double *oldbox, hipStream_t st);

@@ -293,9 +293,13 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.dudl, nl);
   ALLOC(d.energy, BM_NTERMS);
   {
+    // with the fix flag lambda is the initial theta (msld.cu:924-934)
     std::vector<double> lam(nl, 1.0);
+    if (t->fix_lambda && theta)
+      for (int b = 0; b < t->n_blocks; b++) lam[b] = theta[b];
     UPLOAD(d.lambda, lam);
     std::vector<float> lamf(nl, 1.0f);
+    for (int b = 0; b < nl; b++) lamf[b] = (float)lam[b];
     UPLOAD(d.lambdaf, lamf);
     std::vector<int> nf(1, -1);
     UPLOAD(d.nanflag, nf);
@@ -686,6 +690,31 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       vp[v] = make_double2(t->vb_l0[v], t->vb_k[v]);
     }
     UPLOAD(d.vb_i, vi); UPLOAD(d.vb_p, vp);
+    // theta biases (one entry per site) and the piecewise constraint
+    if (t->tcoll_k && t->tcoll_n) {
+      std::vector<double> k(t->tcoll_k, t->tcoll_k + d.nsites), nn(t->tcoll_n, t->tcoll_n + d.nsites);
+      double *pk = nullptr, *pn = nullptr;
+      UPLOAD(pk, k);
+      UPLOAD(pn, nn);
+      d.tcoll_k = pk;
+      d.tcoll_n = pn;
+    }
+    if (t->tinde_k) {
+      std::vector<double> k(t->tinde_k, t->tinde_k + d.nsites);
+      double *pk = nullptr;
+      UPLOAD(pk, k);
+      d.tinde_k = pk;
+    }
+    d.piecewise = t->piecewise;
+    d.fixlam = t->fix_lambda;
+    d.pw_width = t->pw_width;
+    d.pw_k = t->pw_k;
+    d.edge_k = t->edge_k;
+    d.edge_n = t->edge_n;
+    d.edge_alpha = t->edge_alpha;
+    d.edge_phi = t->edge_phi;
+    ALLOC(d.theta0, d.nsites);
+    ALLOC(d.dtheta, nl);
     d.fnex = rp->fnex;
     d.softcore = rp->soft_core;
     d.softcore14 = rp->soft_core14;
@@ -702,7 +731,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   }
 #undef ALLOC
 #undef UPLOAD
-  if (s->alch) launch_theta_to_lambda(d, s->st);
+  if (s->alch) launch_theta_to_lambda(d, s->st, true);
   if (hipStreamSynchronize(s->st) != hipSuccess) return fail(set_err(BM_EHIP, "init sync"));
   *out = s;
   return BM_OK;
@@ -1148,8 +1177,8 @@ int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t
   for (int k = 0; k < n && !rc; k++) rc = member_energy(b->sys[k], e0[k]);
   auto swap_pos = [&](bm_system *x, bm_system *y) {
     int r = swap_dev(x, y, x->d.pos, y->d.pos, tmp, N, b->st);
-    if (!r && x->alch) blade::launch_theta_to_lambda(x->d, b->st);
-    if (!r && y->alch) blade::launch_theta_to_lambda(y->d, b->st);
+    if (!r && x->alch) blade::launch_theta_to_lambda(x->d, b->st, true);
+    if (!r && y->alch) blade::launch_theta_to_lambda(y->d, b->st, true);
     if (!r && hipStreamSynchronize(b->st) != hipSuccess) r = BM_EHIP;
     return r;
   };
@@ -1377,7 +1406,7 @@ int bm_set_positions(bm_system *s, const double *pos, const double *theta) {
   if (pos) HIPCHK(hipMemcpyAsync(s->d.pos, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s->st));
   if (theta) {
     HIPCHK(hipMemcpyAsync(s->d.pos + 3 * (size_t)n, theta, s->d.nl * sizeof(double), hipMemcpyHostToDevice, s->st));
-    if (s->alch) blade::launch_theta_to_lambda(s->d, s->st);
+    if (s->alch) blade::launch_theta_to_lambda(s->d, s->st, true);
   }
   HIPCHK(hipStreamSynchronize(s->st));
   return BM_OK;

@@ -451,9 +451,30 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     t.catsBound.push_back((int)t.catsIdx.size());
   }
   t.catsK = m.kRestraint;
-  t.blockCharge = m.lambdaCharge;
+  // each block's charge is the sum of its atoms' charges, whatever
+  // blockCharge the initial conditions carried (Msld::initialize,
+  // msld.cu:684-690)
+  t.blockCharge.assign(m.blockCount, 0.0);
+  for (int i = 0; i < n; i++) t.blockCharge[m.atomBlock[i]] += t.charge[i];
   if (m.kCR2 != 0 && m.kCR2 != 1) fail("only kChargeRestraint2 values of 0 or 1 are allowed");
   t.crK1 = m.kCR1; t.crK2 = m.kCR2; t.crK3 = m.kCR3; t.crQ0 = m.q0CR3; t.crW = m.wCR3;
+  // theta biases: one value per site (msld.cu:1157-1160 and 1184 index
+  // them by site)
+  if (!m.thetaCollK.empty() && (int)m.thetaCollK.size() != nsite)
+    fail("theta collective bias needs one entry per site (" + std::to_string(nsite) + ")");
+  if (!m.thetaIndeK.empty() && (int)m.thetaIndeK.size() != nsite)
+    fail("theta independent bias needs one entry per site (" + std::to_string(nsite) + ")");
+  t.tcollK = m.thetaCollK;
+  t.tcollN = m.thetaCollN;
+  t.tindeK = m.thetaIndeK;
+  t.piecewise = m.piecewise ? 1 : 0;
+  if (t.piecewise)
+    for (int b = 0; b < m.blockCount; b++)
+      if (m.blockFixed[b]) fail("fixed blocks are not supported with the piecewise lambda constraint");
+  t.pwWidth = m.wellWidth;
+  t.pwK = m.wellK;
+  t.edgeK = m.edgeK; t.edgeN = m.edgeN; t.edgeAlpha = m.edgeAlpha; t.edgePhi = m.edgePhi;
+  t.fixLambda = m.fix ? 1 : 0;
   (void)kBoltz;
 }
 
@@ -502,6 +523,13 @@ const bm_topology *Topology::view() {
   v.n_cmap_types = (int)cmapNgrid.size();
   v.cmap_ngrid = cmapNgrid.data(); v.cmap_off = cmapOff.data(); v.cmap_grid = cmapGrid.data();
   v.cr_k1 = crK1; v.cr_k2 = crK2; v.cr_k3 = crK3; v.cr_q0 = crQ0; v.cr_w = crW;
+  v.tcoll_k = tcollK.empty() ? nullptr : tcollK.data();
+  v.tcoll_n = tcollN.empty() ? nullptr : tcollN.data();
+  v.tinde_k = tindeK.empty() ? nullptr : tindeK.data();
+  v.piecewise = piecewise;
+  v.pw_width = pwWidth; v.pw_k = pwK;
+  v.edge_k = edgeK; v.edge_n = edgeN; v.edge_alpha = edgeAlpha; v.edge_phi = edgePhi;
+  v.fix_lambda = fixLambda;
   return &top;
 }
 

@@ -80,6 +80,13 @@ struct MsldSetup {
   double kRestraint = 59.2;                       // kcal/mol/A^2 (msld.cu:83)
   double kCR1 = 0, kCR2 = 0, kCR3 = 0, q0CR3 = 0, wCR3 = 0;   // charge restraints
   int softBondCount = 0;
+  // theta biases (msld.cu:1675-1695): per-site collective (k, n) and
+  // independent (k) biases, sized by their first setter call
+  std::vector<double> thetaCollK, thetaCollN, thetaIndeK;
+  // piecewise constraint and its edge bias (msld.cu:1697-1712)
+  bool piecewise = false;
+  double wellWidth = 1.0, wellK = 100.0;
+  double edgeK = 0.0, edgeN = 2.0, edgeAlpha = 10.0, edgePhi = 0.0;
   void resize(int nblocks, int natoms);
 };
 
@@ -129,6 +136,10 @@ struct Topology {
   std::vector<int> catsBound, catsIdx; double catsK = 0;
   std::vector<int> cmapIdx, cmapSb, cmapType, cmapNgrid, cmapOff; std::vector<double> cmapGrid;
   std::vector<double> blockCharge; double crK1 = 0, crK2 = 0, crK3 = 0, crQ0 = 0, crW = 0;
+  std::vector<double> tcollK, tcollN, tindeK;
+  int piecewise = 0; double pwWidth = 1.0, pwK = 100.0;
+  double edgeK = 0, edgeN = 2, edgeAlpha = 10, edgePhi = 0;
+  int fixLambda = 0;
   bm_topology top{};
   const bm_topology *view();
 };

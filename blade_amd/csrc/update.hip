@@ -64,10 +64,26 @@ __global__ void __launch_bounds__(256) upd_normals_multi(Multi m) {
   upd_normals_body(d, b.bid);
 }
 
-// dU/dtheta of block b from the fixed-point dU/dlambda (msld.cu:937-975)
+// dU/dtheta of block b: the theta biases plus the chain rule from the
+// fixed-point dU/dlambda (msld.cu:937-998).  Softmax: lambda_b fnex cos
+// theta_b (F_b - sum_c lambda_c F_c / (1 - fixed share)); piecewise:
+// g_b (F_b - sum_c g_c F_c / G), g = 3 max(theta - t_s, 0)^2, G = sum g.
+// Fixed blocks and the fix flag take no chain-rule force.
 __device__ double theta_force(const Dev &d, int b) {
-  if (d.block_fixed[b]) return 0.0;
+  const double fb = (double)d.dtheta[b] * kFixInvD;
+  if (d.block_fixed[b] || d.fixlam) return fb;
   const int s = d.block_site[b];
+  if (d.piecewise) {
+    const double t0 = d.theta0[s];
+    double G = 0, dot = 0;
+    for (int c = d.site_bound[s]; c < d.site_bound[s + 1]; c++) {
+      const double q = d.pos[3 * d.n + c] - t0, g = q > 0 ? 3 * q * q : 0.0;
+      G += g;
+      dot += g * ((double)d.dudl[c] * kFixInvD);
+    }
+    const double q = d.pos[3 * d.n + b] - t0, gb = q > 0 ? 3 * q * q : 0.0;
+    return fb + gb * ((double)d.dudl[b] * kFixInvD - dot / G);
+  }
   double fix = 0, dyn = 0;
   for (int c = d.site_bound[s]; c < d.site_bound[s + 1]; c++) {
     if (d.block_fixed[c]) fix += d.lambda[c];
@@ -75,7 +91,7 @@ __device__ double theta_force(const Dev &d, int b) {
   }
   double r = 1 - fix, f = (double)d.dudl[b] * kFixInvD;
   if (r > 0) f -= dyn / r;
-  return f * d.lambda[b] * d.fnex * cos(d.pos[3 * d.n + b]);
+  return fb + f * d.lambda[b] * d.fnex * cos(d.pos[3 * d.n + b]);
 }
 
 
@@ -421,14 +437,56 @@ __device__ void constrain_group(const Dev &d, int gi, int k) {
   group_lane<3, 4, TAIL>(d, a, b0, k);
 }
 
-// theta -> lambda for site s (msld.cu:863-898)
-__device__ void theta_to_lambda(const Dev &d, int s) {
+// theta -> lambda for site s (msld.cu:863-898).  Piecewise constraint
+// (msld.cu:785-861): the root t_s of sum max(theta - t, 0)^3 = 1 by Newton
+// from max(theta) - 1 (at most 51 iterations, stop at a step <= 1e-12),
+// lambda = max(theta - t_s, 0)^3.  Fix flag (msld.cu:924-934): theta is set
+// to the held lambda, or with init the held lambda is taken from theta.
+__device__ void theta_to_lambda(const Dev &d, int s, bool init = false) {
   const int b0 = d.site_bound[s], b1 = d.site_bound[s + 1];
+  double *th = d.pos + 3 * (size_t)d.n;
+  if (d.fixlam) {
+    for (int b = b0; b < b1; b++) {
+      if (init) {
+        d.lambda[b] = th[b];
+        d.lambdaf[b] = (float)th[b];
+      } else {
+        th[b] = d.lambda[b];
+      }
+    }
+    return;
+  }
+  if (d.piecewise) {
+    double x0 = -1e9;
+    for (int b = b0; b < b1; b++) x0 = fmax(x0, th[b]);
+    x0 -= 1.0;
+    for (int it = 0; it <= 50; it++) {
+      double c = -1.0, dc = 0.0;
+      for (int b = b0; b < b1; b++) {
+        const double q = th[b] - x0;
+        if (q > 0) {
+          c += q * q * q;
+          dc += 3 * q * q;
+        }
+      }
+      const double x1 = x0 + c / dc;
+      const bool done = fabs(x1 - x0) <= 1e-12;
+      x0 = x1;
+      if (done) break;
+    }
+    d.theta0[s] = x0;
+    for (int b = b0; b < b1; b++) {
+      const double q = th[b] - x0, l = q > 0 ? q * q * q : 0.0;
+      d.lambda[b] = l;
+      d.lambdaf[b] = (float)l;
+    }
+    return;
+  }
   double fix = 0, norm = 0;
   for (int b = b0; b < b1; b++) {
-    if (d.block_fixed[b]) fix += d.pos[3 * d.n + b];
+    if (d.block_fixed[b]) fix += th[b];
     else {
-      double l = exp(d.fnex * sin(d.pos[3 * d.n + b]));
+      double l = exp(d.fnex * sin(th[b]));
       d.lambda[b] = l;
       norm += l;
     }
@@ -436,7 +494,7 @@ __device__ void theta_to_lambda(const Dev &d, int s) {
   const double r = 1 - fix;
   if (norm > 0) norm = r / norm;
   for (int b = b0; b < b1; b++) {
-    double l = d.block_fixed[b] ? d.pos[3 * d.n + b] : d.lambda[b] * norm;
+    double l = d.block_fixed[b] ? th[b] : d.lambda[b] * norm;
     d.lambda[b] = l;
     d.lambdaf[b] = (float)l;
   }
@@ -471,9 +529,9 @@ __global__ void __launch_bounds__(64) upd_constrain(Dev d, int finish) {
   }
 }
 
-__global__ void upd_theta_to_lambda(Dev d) {
+__global__ void upd_theta_to_lambda(Dev d, bool init) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < d.nsites) theta_to_lambda(d, s);
+  if (s < d.nsites) theta_to_lambda(d, s, init);
 }
 
 // RATTLE velocity projection (holonomic.cu:15-211) of one group
@@ -712,8 +770,8 @@ void launch_volume_restore(const Dev &d, hipStream_t st) {
   upd_restore_positions<<<(3 * d.n + 255) / 256, 256, 0, st>>>(d);
 }
 
-void launch_theta_to_lambda(const Dev &d, hipStream_t st) {
-  upd_theta_to_lambda<<<(d.nsites + 63) / 64, 64, 0, st>>>(d);
+void launch_theta_to_lambda(const Dev &d, hipStream_t st, bool init) {
+  upd_theta_to_lambda<<<(d.nsites + 63) / 64, 64, 0, st>>>(d, init);
 }
 
 }  // namespace blade

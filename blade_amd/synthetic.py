@@ -77,6 +77,7 @@ class Spec:
     dihe_par: dict = field(default_factory=dict)     # (t1..t4) -> [(k, n, d0deg)]
     impr_par: dict = field(default_factory=dict)     # (t1..t4) -> (k, n, d0deg)
     nb_par: dict = field(default_factory=dict)       # t -> (eps, rmin2, eps14, rmin2_14, e14fac, combine)
+    nbfix_par: dict = field(default_factory=dict)    # (t1,t2) -> (eps, rmin, eps14, rmin14) pair overrides
     atoms: list = field(default_factory=list)        # (seg, resid, resname, name, type, q, mass)
     bonds: list = field(default_factory=list)        # 0-based
     angles: list = field(default_factory=list)
@@ -101,8 +102,18 @@ class Spec:
     scale_terms: tuple = (1, 1, 1, 1, 1, 1)           # lambda-scaled bond, urey, angle, dihe, impr, cmap
     atom_restraints: list = field(default_factory=list)  # groups of 0-based atoms held to their centroid
     k_restraint: float = 59.2                         # kcal/mol/A^2
-    block_charge: list = None                         # per-block net charge q_b (None = 0)
+    block_charge: list = None                         # blockCharge of the initial conditions (the library
+                                                      # uses each block's atom-charge sum, as the reference)
     charge_restraints: tuple = None                   # (k1, k2, k3, q0, w) on Q = sum lambda_b q_b
+    soft_core14: bool = False                         # soft-core 1-4 pairs (useSoftCore14)
+    ewald_type: int = 2                               # msldEwaldType: 1 on, 2 ex, 3 nn
+    block_friction: list = None                       # per-block theta friction, 1/AKMA time (None = msld_gamma)
+    block_fixed: list = None                          # per-block fixed flags (lambda held at theta)
+    theta_coll: list = None                           # per site (k, n) collective theta bias
+    theta_inde: list = None                           # per site k independent theta bias
+    piecewise: tuple = None                           # (width, k): piecewise lambda constraint
+    theta_edge: tuple = None                          # (k, N, alpha, phi) edge bias of the piecewise form
+    fix_lambda: bool = False                          # msld fix flag: lambda held at the initial theta
     # run options
     dt: float = 0.002 * PICOSECOND
     temperature: float = 298.0
@@ -971,3 +982,86 @@ def add_cmaps(spec, count=24, ngrid=24, seed=7, noise=0.05):
                                   + k[3] * np.sin(P + 2 * Q) + noise * rng.standard_normal((ngrid, ngrid)))
         spec.cmaps.append(q)
     return spec
+
+
+def ensemble_probe(seed=0xB1ADE0E0, nsites=6, nsubs=3, npairs=24, nfree=12, box=30.0, k_pair=3510.0,
+                   bias=(0.0, 0.6, -0.4), var_bias=None, theta_inde=None, msld_gamma_ps=5.0, gamma_ps=5.0):
+    """A system whose equilibrium averages are known in closed form or by
+    quadrature, for checking the integrator's sampling:
+      * `nsites` lambda sites of `nsubs` substituent blocks, each block one
+        non-interacting dummy atom, with fixed biases `bias` per substituent
+        (and optional variable and independent theta biases): the lambda
+        averages are ratios of integrals over theta in [-pi, pi)^nsubs;
+      * `npairs` environment atom pairs held together by atom restraints of
+        constant k_pair (E = k/4 |x1 - x2|^2 per pair, omega^2 = k/m): each
+        Cartesian component of the pair separation is a harmonic oscillator
+        with <r^2> = 2 kT / k, which the VRORV (BAOAB) splitting samples
+        exactly at any stable step;
+      * `nfree` free environment atoms and the pairs' centres of mass:
+        Ornstein-Uhlenbeck velocities with <v^2> = kT / m per component.
+    No charges, no Lennard-Jones (epsilon 0), force-switched electrostatics,
+    no constraints."""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name="ensemble-probe", box=(box,) * 3)
+    spec.types["DUM"] = 12.0
+    spec.nb_par["DUM"] = (0.0, 1.0, 0.0, 1.0, 1.0, 0)
+    spec._pos = []
+
+    def atom(p):
+        spec.atoms.append(("PRB", str(len(spec.atoms) + 1), "DUM", "D%d" % len(spec.atoms), "DUM", 0.0, 12.0))
+        spec._pos.append(np.asarray(p, float))
+        return len(spec.atoms) - 1
+
+    pairs = []
+    for _ in range(npairs):
+        c = rng.uniform(0, box, 3)
+        u = rng.normal(size=3)
+        u *= 0.02 / np.linalg.norm(u)
+        pairs.append([atom(c + u), atom(c - u)])
+    for _ in range(nfree):
+        atom(rng.uniform(0, box, 3))
+    sites = []
+    for _ in range(nsites):
+        sites.append([[atom(rng.uniform(0, box, 3))] for _ in range(nsubs)])
+    spec.pos = np.array(spec._pos)
+    _msld_sites(spec, sites, fnex=5.5)
+    spec.fixed_bias = [0.0] + list(bias[:nsubs]) * nsites
+    spec.theta0 = [0.0] + list(rng.uniform(-np.pi, np.pi, spec.nblocks - 1))
+    spec.theta_v = [0.0] * spec.nblocks
+    spec.var_bias = list(var_bias or [])
+    spec.theta_inde = theta_inde
+    spec.atom_restraints = pairs
+    spec.k_restraint = k_pair
+    spec.shake = False
+    spec.elec_method, spec.vdw_method = 0, 0
+    spec.rcut, spec.rswitch = 10.0, 9.0
+    spec.temperature = 298.0
+    spec.gamma = gamma_ps / PICOSECOND
+    spec.msld_gamma = msld_gamma_ps / PICOSECOND
+    spec.vel = _maxwell(rng, [a[6] for a in spec.atoms], spec.temperature)
+    spec.seed = seed
+    return spec
+
+
+def term_variety(spec):
+    """A copy of `spec` whose parameters exercise the term forms the default
+    synthetic systems leave out: Urey-Bradley terms on every polymer angle
+    type (kub 35 kcal/mol/A^2, s0 2.45 A), periodic impropers (k 1.5, n 2,
+    90 degrees) in place of the harmonic ones, NBFIX overrides for two type
+    pairs and the geometric combination rule for two types."""
+    import copy
+    sp = copy.deepcopy(spec)
+    for key, (k, t0, kub, s0) in list(sp.angle_par.items()):
+        if "OT" not in key:
+            sp.angle_par[key] = (k, t0, 35.0, 2.45)
+    for key in list(sp.impr_par):
+        sp.impr_par[key] = (1.5, 2, 90.0)
+    if "CT" in sp.nb_par and "OT" in sp.nb_par:
+        sp.nbfix_par[("OT", "CT")] = (-0.12, 3.55, -0.08, 3.40)
+    if "NT" in sp.nb_par and "HT" in sp.nb_par:
+        sp.nbfix_par[("NT", "HT")] = (-0.05, 2.10, -0.05, 2.00)
+    for t in ("CT", "NT"):
+        if t in sp.nb_par:
+            e, r, e14, r14, f, _ = sp.nb_par[t]
+            sp.nb_par[t] = (e, r, e14, r14, f, 1)
+    return sp

@@ -62,6 +62,9 @@ class BladeSystem:
     def add_parameter_nbonds(self, t, eps, sig, eps14, sig14, e14fac, combine):
         self.L.blade_add_parameter_nbonds(self.h, _b(t), eps, sig, eps14, sig14, e14fac, combine)
 
+    def add_parameter_nbfixs(self, t1, t2, eps, sig, eps14, sig14):
+        self.L.blade_add_parameter_nbfixs(self.h, _b(t1), _b(t2), eps, sig, eps14, sig14)
+
     def init_structure(self):
         self.L.blade_init_structure(self.h)
 
@@ -299,6 +302,8 @@ def build(spec, gpus=(0,)):
                 s.L.blade_add_parameter_cmaps_fill(s.h, *tb, i + 1, j + 1, float(g[i, j]))
     for t, (eps, sig, eps14, sig14, e14, comb) in spec.nb_par.items():
         s.add_parameter_nbonds(t, eps, sig, eps14, sig14, e14, comb)
+    for (t1, t2), (eps, sig, eps14, sig14) in getattr(spec, "nbfix_par", {}).items():
+        s.add_parameter_nbfixs(t1, t2, eps, sig, eps14, sig14)
     s.init_structure()
     for i, (seg, resid, resn, nm, t, q, m) in enumerate(spec.atoms):
         s.add_atom(i + 1, seg, resid, resn, nm, t, q, m)
@@ -328,9 +333,25 @@ def build(spec, gpus=(0,)):
             s.add_msld_initialconditions(b + 1, spec.block_site[b] + 1, spec.theta0[b], tv,
                                          spec.theta_mass[b], spec.fixed_bias[b],
                                          spec.block_charge[b] if getattr(spec, "block_charge", None) else 0.0)
-        s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), 0, 2,
-                         getattr(spec, "k_restraint", 59.2), 1.0, 2.0, 1.0, 0)
+        s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), int(getattr(spec, "soft_core14", False)),
+                         int(getattr(spec, "ewald_type", 2)), getattr(spec, "k_restraint", 59.2), 1.0, 2.0, 1.0,
+                         int(getattr(spec, "fix_lambda", False)))
         s.add_msld_termscaling(*[int(x) for x in spec.scale_terms])
+        for b, g in enumerate(getattr(spec, "block_friction", None) or []):
+            if g is not None:
+                s.L.blade_add_msld_block_friction(s.h, b + 1, float(g))
+        for b, fx in enumerate(getattr(spec, "block_fixed", None) or []):
+            if fx:
+                s.L.blade_set_msld_block_fixed(s.h, b + 1, 1)
+        nsite = max(spec.block_site) + 1
+        for i, kn in enumerate(getattr(spec, "theta_coll", None) or []):
+            s.L.blade_add_msld_thetacollbias(s.h, nsite, i + 1, float(kn[0]), float(kn[1]))
+        for i, k in enumerate(getattr(spec, "theta_inde", None) or []):
+            s.L.blade_add_msld_thetaindebias(s.h, nsite, i + 1, float(k))
+        if getattr(spec, "piecewise", None):
+            s.L.blade_set_msld_piecewise_constraint(s.h, 1, float(spec.piecewise[0]), float(spec.piecewise[1]))
+        if getattr(spec, "theta_edge", None):
+            s.L.blade_set_msld_thetaedgebias(s.h, *[float(v) for v in spec.theta_edge])
         if getattr(spec, "charge_restraints", None):
             s.add_msld_charges(*spec.charge_restraints)
         for g in getattr(spec, "atom_restraints", []):

@@ -96,6 +96,18 @@ void blade_add_msld_flags(System *system, double gamma, double fnex, int useSoft
 void blade_add_msld_bias(System *system, int i, int j, int type, double l0, double k, int n);
 void blade_add_msld_block_friction(System *system, int blockIdx, double friction);
 void blade_set_msld_block_fixed(System *system, int blockIdx, int fixed);
+/* theta biases (msld/msld.h:155-157): per-site collective and independent
+   biases (the first call sizes the table with `sites` entries, site i
+   1-based), and the edge bias of the piecewise constraint */
+void blade_add_msld_thetacollbias(System *system, int sites, int i, double k, double n);
+void blade_add_msld_thetaindebias(System *system, int sites, int i, double k);
+void blade_set_msld_thetaedgebias(System *system, double k, double N, double alpha, double phi);
+/* msld/msld.h:158: lambda_i = max(theta_i - t, 0)^3 per site instead of the
+   softmax, with a flat-bottom harmonic of half-width nsubs*width on theta */
+void blade_set_msld_piecewise_constraint(System *system, int do_imp, double width, double k);
+/* msld/msld.h:159: soft bonds are recorded; a system with any is rejected
+   with BM_EINVAL when its terms are derived (not supported) */
+void blade_add_msld_softbond(System *system, int i, int j);
 /* MSLD atom restraints (msld/msld.h:160-161): a new group, then its atoms
    (1-based); the force constant is blade_add_msld_flags' kRestraint */
 void blade_add_msld_atomrestraint(System *system);

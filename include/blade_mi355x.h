@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define BM_LAYOUT_VERSION 1
+#define BM_LAYOUT_VERSION 2
 
 /* error codes */
 #define BM_OK 0
@@ -148,7 +148,9 @@ typedef struct bm_topology {
      energy term BM_EBIAS.  cr_k1 (PME only): E = -(2 pi/3) kE k1 Q / V;
      cr_k2 (PME only, 0 or 1): E = -(pi/2) kE k2 Q^2 / (V beta^2);
      cr_k3: E = k3/2 dQ^2, dQ = distance of Q - q0 from the band [-w, w] */
-  const double *block_charge;      /* [nL] q_b (lambdaCharge)            */
+  const double *block_charge;      /* [nL] q_b: the sum of the charges of the
+                                      block's atoms (Msld::initialize,
+                                      msld.cu:684-690)                  */
   double cr_k1, cr_k2, cr_k3, cr_q0, cr_w;
 
   /* CMAP cross terms (bonded/bonded.cu:531-720): E = lambda-scaled bicubic
@@ -163,6 +165,34 @@ typedef struct bm_topology {
   const int32_t *cmap_ngrid;       /* [types] points per 360 degrees */
   const int32_t *cmap_off;         /* [types] first value in cmap_grid */
   const double *cmap_grid;         /* row-major [ngrid(phi)][ngrid(psi)] */
+
+  /* MSLD theta biases (msld/msld.cu:1155-1274; setters msld.cu:1675-1705),
+     energy term BM_ETHETA, forces on theta directly:
+       collective, per site s (k, n): with h_j = (1 + sin theta_j)/2 and
+         l_j = (1 - sin theta_j)/2 over the site's nb blocks,
+         A = sum_j h_j^n - 1, B = sum_j l_j^n - (nb - 1),
+         E = k/2 (A^2 + B^2) and dU/dtheta_j = k n (A h_j^(n-1) -
+         B l_j^(n-1)) cos theta_j, the reference's force (twice the theta
+         derivative of its reported energy);
+       independent, per block b of site s (k): with s_b = l_b,
+         E = k (1 - s_b^4), dU/dtheta_b = 2 k s_b^3 cos theta_b.
+     NULL => none. */
+  const double *tcoll_k, *tcoll_n; /* [nSites] */
+  const double *tinde_k;           /* [nSites] */
+  /* piecewise lambda constraint (msld.cu:785-861, setter msld.cu:1706):
+     lambda_i = max(theta_i - t_s, 0)^3 with t_s solved per site by Newton
+     so the site's lambdas sum to 1; every block but the environment feels
+     a flat-bottom harmonic pw_k/2 d^2 beyond theta = +-nb*pw_width
+     (msld.cu:1189-1214); with |edge_k| > 1e-4 a bias edge_k/2 (N - edge_n)^2
+     on the smooth substituent count N = sum_j 1/(1 + exp(-edge_alpha
+     (theta_j - t_s + edge_phi))) above edge_n (msld.cu:1216-1252,
+     blade_set_msld_thetaedgebias).  Fixed blocks are not allowed with it. */
+  int32_t piecewise;
+  double pw_width, pw_k;
+  double edge_k, edge_n, edge_alpha, edge_phi;
+  /* msld fix flag (msld.cu:913-935): every lambda held at its initial
+     theta value, no theta force from dU/dlambda */
+  int32_t fix_lambda;
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */

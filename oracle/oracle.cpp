@@ -773,6 +773,30 @@ void cmap(const bm_topology *t, const bm_runparams *rp, const double *pos, const
   }
 }
 
+// Root t of sum_i max(theta_i - t, 0)^3 = 1 over one site's blocks by
+// Newton's method from t = max(theta) - 1 (msld.cu:785-861: at most 51
+// iterations, stop when a step is <= 1e-12).  The sum decreases and is
+// convex in t, so the iterates rise monotonically to the root.
+double piecewise_root(const double *theta, int nb) {
+  double x0 = -1e9;
+  for (int i = 0; i < nb; i++) x0 = std::max(x0, theta[i]);
+  x0 -= 1.0;
+  for (int it = 0; it <= 50; it++) {
+    double c = -1.0, dc = 0.0;
+    for (int i = 0; i < nb; i++) {
+      const double d = theta[i] - x0;
+      if (d > 0) {
+        c += d * d * d;
+        dc += 3 * d * d;
+      }
+    }
+    const double x1 = x0 + c / dc;
+    if (std::fabs(x1 - x0) <= 1e-12) return x1;
+    x0 = x1;
+  }
+  return x0;
+}
+
 // Fixed and variable lambda biases (msld/msld.cu:1000-1130).
 void msldbias(const bm_topology *t, const double *lambda, double *dudl, double *energy) {
   for (int b = 0; b < t->n_blocks; b++) {
@@ -810,11 +834,15 @@ void msldbias(const bm_topology *t, const double *lambda, double *dudl, double *
 // k = -(2 pi/3) kE k1 / V and E2 = k Q^2 / 2 with k = -pi kE k2 / (V beta^2);
 // the third is harmonic, k3/2 dQ^2, in the distance dQ of Q - q0 from the
 // flat band [-w, w].  dU/dlambda_b = dE/dQ q_b for b > 0; energy BM_EBIAS.
+// The block charges q_b are summed here from the atoms' charges and block
+// codes (Msld::initialize, msld.cu:684-690), not taken from the topology's
+// block_charge table.
 void chargerestraints(const bm_topology *t, const bm_runparams *rp, const double *lambda, double *dudl,
                       double *energy) {
-  if (!t->block_charge) return;
+  std::vector<double> qb(t->n_blocks, 0.0);
+  for (int i = 0; i < t->n_atoms; i++) qb[t->site_block[i] & 0xFFFF] += t->charge[i];
   double q = 0;
-  for (int b = 0; b < t->n_blocks; b++) q += (b ? lambda[b] : 1.0) * t->block_charge[b];
+  for (int b = 0; b < t->n_blocks; b++) q += (b ? lambda[b] : 1.0) * qb[b];
   const double vol = rp->box[0] * rp->box[1] * rp->box[2];
   double dedq[3] = {0, 0, 0}, e[3] = {0, 0, 0};
   if (rp->elec_method == BM_PME && t->cr_k1 != 0) {
@@ -837,7 +865,7 @@ void chargerestraints(const bm_topology *t, const bm_runparams *rp, const double
     dedq[2] = t->cr_k3 * dq;
   }
   for (int v = 0; v < 3; v++) {
-    for (int b = 1; b < t->n_blocks; b++) dudl[b] += dedq[v] * t->block_charge[b];
+    for (int b = 1; b < t->n_blocks; b++) dudl[b] += dedq[v] * qb[b];
     energy[BM_EBIAS] += e[v];
   }
 }
@@ -1244,8 +1272,21 @@ long orc_domdec(const bm_topology *t, const bm_runparams *rp, double *pos, int n
 
 void orc_theta_to_lambda(const bm_topology *t, const bm_runparams *rp, const double *theta,
                          double *lambda) {
+  if (t->fix_lambda) {
+    // the fix flag: lambda is theta as given (msld.cu:924-934)
+    for (int b = 0; b < t->n_blocks; b++) lambda[b] = theta[b];
+    return;
+  }
   for (int s = 0; s < t->n_sites; s++) {
     int b0 = t->site_bound[s], b1 = t->site_bound[s + 1];
+    if (t->piecewise) {
+      const double t0 = piecewise_root(theta + b0, b1 - b0);
+      for (int b = b0; b < b1; b++) {
+        const double d = theta[b] - t0;
+        lambda[b] = d > 0 ? d * d * d : 0.0;
+      }
+      continue;
+    }
     double fix = 0, norm = 0;
     for (int b = b0; b < b1; b++) {
       bool fixed = t->block_fixed && t->block_fixed[b];
@@ -1269,7 +1310,22 @@ void orc_chain_rule(const bm_topology *t, const bm_runparams *rp, const double *
   for (int b = 0; b < t->n_blocks; b++) {
     int s = t->block_site[b];
     bool fixedb = t->block_fixed && t->block_fixed[b];
-    if (fixedb) { dudtheta[b] = 0; continue; }
+    if (fixedb || t->fix_lambda) { dudtheta[b] = 0; continue; }
+    if (t->piecewise) {
+      // dlambda_i/dtheta_b = g_i (delta_ib - g_b / G), g = 3 max(theta - t, 0)^2,
+      // G = sum g (the root t moves with theta_b by g_b / G; msld.cu:968-990)
+      const int c0 = t->site_bound[s], c1 = t->site_bound[s + 1];
+      const double t0 = piecewise_root(theta + c0, c1 - c0);
+      double G = 0, dot = 0;
+      for (int c = c0; c < c1; c++) {
+        const double d = theta[c] - t0, g = d > 0 ? 3 * d * d : 0.0;
+        G += g;
+        dot += g * dudl[c];
+      }
+      const double d = theta[b] - t0, gb = d > 0 ? 3 * d * d : 0.0;
+      dudtheta[b] = gb * (dudl[b] - dot / G);
+      continue;
+    }
     double fix = 0, dyn = 0;
     for (int c = t->site_bound[s]; c < t->site_bound[s + 1]; c++) {
       bool fixedc = t->block_fixed && t->block_fixed[c];
@@ -1280,6 +1336,75 @@ void orc_chain_rule(const bm_topology *t, const bm_runparams *rp, const double *
     if (r > 0) f -= dyn / r;
     dudtheta[b] = f * lambda[b] * rp->fnex * std::cos(theta[b]);
   }
+}
+
+void orc_theta_bias(const bm_topology *t, const bm_runparams *rp, const double *theta, double *dudtheta,
+                    double *energy) {
+  (void)rp;
+  double e = 0;
+  for (int s = 0; s < t->n_sites; s++) {
+    const int b0 = t->site_bound[s], b1 = t->site_bound[s + 1], nb = b1 - b0;
+    if (t->tcoll_k && t->tcoll_k[s] != 0) {
+      // collective bias (msld.cu:1155-1181): the reference's force is
+      // k n (A h^(n-1) - B l^(n-1)) cos theta, twice the theta derivative of
+      // the energy k/2 (A^2 + B^2) it reports; both are kept as they are
+      const double k = t->tcoll_k[s], n = t->tcoll_n[s];
+      double A = -1, B = -(nb - 1);
+      for (int b = b0; b < b1; b++) {
+        A += std::pow(0.5 + 0.5 * std::sin(theta[b]), n);
+        B += std::pow(0.5 - 0.5 * std::sin(theta[b]), n);
+      }
+      e += 0.5 * k * (A * A + B * B);
+      for (int b = b0; b < b1; b++)
+        dudtheta[b] += k * n * (A * std::pow(0.5 + 0.5 * std::sin(theta[b]), n - 1) -
+                                B * std::pow(0.5 - 0.5 * std::sin(theta[b]), n - 1)) * std::cos(theta[b]);
+    }
+    if (t->tinde_k && t->tinde_k[s] != 0) {
+      // independent bias (msld.cu:1182-1199), every block of the site
+      const double k = t->tinde_k[s];
+      for (int b = b0; b < b1; b++) {
+        const double sb = 0.5 - 0.5 * std::sin(theta[b]);
+        e += k * (1 - sb * sb * sb * sb);
+        dudtheta[b] += 2 * k * sb * sb * sb * std::cos(theta[b]);
+      }
+    }
+    if (t->piecewise && s > 0) {
+      // flat-bottom harmonic beyond +-nb*width (msld.cu:1201-1226)
+      for (int b = b0; b < b1; b++) {
+        const double w = nb * t->pw_width;
+        double d = 0;
+        if (theta[b] > w) d = theta[b] - w;
+        else if (theta[b] < -w) d = theta[b] + w;
+        e += 0.5 * t->pw_k * d * d;
+        dudtheta[b] += t->pw_k * d;
+      }
+      if (std::fabs(t->edge_k) > 1e-4) {
+        // edge bias on the smooth substituent count (msld.cu:1228-1264)
+        const double t0 = piecewise_root(theta + b0, nb);
+        double G = 0;
+        for (int b = b0; b < b1; b++) {
+          const double d = theta[b] - t0;
+          G += d > 0 ? 3 * d * d : 0.0;
+        }
+        double N = 0;
+        std::vector<double> sg(nb);
+        for (int j = 0; j < nb; j++) {
+          const double x = std::exp(-t->edge_alpha * (theta[b0 + j] - t0 + t->edge_phi));
+          N += 1 / (1 + x);
+          sg[j] = t->edge_alpha * x / ((1 + x) * (1 + x));
+        }
+        const double over = N > t->edge_n ? N - t->edge_n : 0.0;
+        e += 0.5 * t->edge_k * over * over;
+        double ssum = 0;
+        for (int j = 0; j < nb; j++) ssum += sg[j];
+        for (int b = b0; b < b1; b++) {
+          const double d = theta[b] - t0, gb = d > 0 ? 3 * d * d : 0.0;
+          dudtheta[b] += t->edge_k * over * (sg[b - b0] - (gb / G) * ssum);
+        }
+      }
+    }
+  }
+  energy[BM_ETHETA] += e;
 }
 
 void orc_holonomic_position(const bm_topology *t, const bm_runparams *rp, double *x,
@@ -1392,7 +1517,11 @@ int orc_step(const bm_topology *t, const bm_runparams *rp, long step, int energy
   orc_forces(t, rp, x, lambda, ORC_ALL, f.data(), dudl.data(), energy);
   if (nl > 1) {
     orc_chain_rule(t, rp, x + N1, lambda, dudl.data(), dudth.data());
+    orc_theta_bias(t, rp, x + N1, dudth.data(), energy);
     for (int b = 0; b < nl; b++) f[N1 + b] = dudth[b];
+    double pot = 0;
+    for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
+    energy[BM_EPOTENTIAL] = pot;
   }
   for (int i = 0; i < n; i++) ism[i] = ism[n + i] = ism[2 * n + i] = 1 / std::sqrt(t->mass[i]);
   for (int b = 0; b < nl; b++) ism[N1 + b] = t->theta_mass[b] > 0 ? 1 / std::sqrt(t->theta_mass[b]) : 0;
@@ -1439,7 +1568,12 @@ int orc_step(const bm_topology *t, const bm_runparams *rp, long step, int energy
     }
   }
   if (cons) orc_holonomic_position(t, rp, x, xb.data(), v, halfdt);
-  if (nl > 1) orc_theta_to_lambda(t, rp, x + N1, lambda);
+  if (nl > 1) {
+    if (t->fix_lambda)
+      for (int b = 0; b < nl; b++) x[N1 + b] = lambda[b];   // theta reset to the held lambda
+    else
+      orc_theta_to_lambda(t, rp, x + N1, lambda);
+  }
   for (int i = 0; i < N; i++)
     if (!std::isfinite(v[i]) || !std::isfinite(x[i])) return -1;
   return 0;

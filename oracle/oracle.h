@@ -77,13 +77,21 @@ long orc_domdec(const bm_topology *top, const bm_runparams *rp, double *pos,
 /* Column counts used by the list build (domdec/domdec.cu:110-118). */
 void orc_domdec_columns(const bm_runparams *rp, int n_atoms, int *ncol_x, int *ncol_y);
 
-/* MSLD theta -> lambda (msld/msld.cu:863-921) and the chain rule
- * dU/dlambda -> dU/dtheta (msld/msld.cu:937-998). */
+/* MSLD theta -> lambda (msld/msld.cu:863-921; the piecewise constraint
+ * msld.cu:785-861 and the fix flag msld.cu:913-935 as the topology selects)
+ * and the chain rule dU/dlambda -> dU/dtheta (msld/msld.cu:937-998). */
 void orc_theta_to_lambda(const bm_topology *top, const bm_runparams *rp,
                          const double *theta, double *lambda);
 void orc_chain_rule(const bm_topology *top, const bm_runparams *rp,
                     const double *theta, const double *lambda,
                     const double *dudl, double *dudtheta);
+
+/* MSLD theta biases (msld/msld.cu:1155-1264): collective and independent
+ * biases, and with the piecewise constraint the flat-bottom harmonic and the
+ * edge bias; adds dU/dtheta into dudtheta[nL] and the energy into
+ * energy[BM_ETHETA]. */
+void orc_theta_bias(const bm_topology *top, const bm_runparams *rp, const double *theta,
+                    double *dudtheta, double *energy);
 
 /* Holonomic constraints (holonomic/holonomic.cu:253-600 position,
  * :15-211 velocity).  x/xprev/v are [3N] SoA; halfdt in AKMA. */

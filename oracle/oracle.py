@@ -48,6 +48,8 @@ def lib():
         L.orc_theta_to_lambda.argtypes = [vp, vp, _f64p, _f64p]
         L.orc_chain_rule.restype = None
         L.orc_chain_rule.argtypes = [vp, vp, _f64p, _f64p, _f64p, _f64p]
+        L.orc_theta_bias.restype = None
+        L.orc_theta_bias.argtypes = [vp, vp, _f64p, _f64p, _f64p]
         L.orc_holonomic_position.restype = None
         L.orc_holonomic_position.argtypes = [vp, vp, _f64p, _f64p, _f64p, C.c_double]
         L.orc_holonomic_velocity.restype = None
@@ -163,6 +165,15 @@ def chain_rule(top, rp, theta, lam, dudl):
     out = np.zeros_like(theta)
     lib().orc_chain_rule(_addr(top), _addr(rp), _p(theta), _p(lam), _p(dudl), _p(out))
     return out
+
+
+def theta_bias(top, rp, theta):
+    """(dU/dtheta[nL], energy[23]) of the theta biases at theta"""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    out = np.zeros_like(theta)
+    e = np.zeros(23)
+    lib().orc_theta_bias(_addr(top), _addr(rp), _p(theta), _p(out), _p(e))
+    return out, e
 
 
 def holonomic_position(top, rp, x, xprev, v, halfdt):

@@ -156,3 +156,76 @@ def test_replica_exchange_swaps_and_metropolis_energy():
     b.close()
     for s in systems:
         s.close()
+
+
+def _member_spec(args):
+    natoms, nsubs, seed = args
+    return synthetic.batch_member(seed, natoms, nsubs)
+
+
+def batch32_specs():
+    """the 32 batched-workload members (atom counts uniform in 3,000-25,000,
+    2-6 substituents, seeded), generated on a process pool (spawned workers
+    run numpy only)"""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import os
+    args = synthetic.batch_set(32)
+    with cf.ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1),
+                                mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_member_spec, args))
+
+
+def test_batch_32_members_equal_solo_and_oracle():
+    """The 32-member batched workload (387,615 atoms): 21 steps with list
+    rebuilds every 10 and energies every 7, from the generated (unrelaxed)
+    states with the relaxation's first stage (0.1 fs, gamma 1000/ps).
+    Every member ends bit-identical to its solo run; the smallest, the
+    median and the largest member then meet the path's force, energy and
+    dU/dlambda bars against the FP64 oracle at their final positions."""
+    from oracle import oracle as orc
+    from tests.helpers import rel_rms
+    specs = batch32_specs()
+    for sp in specs:
+        sp.dt = 0.1e-3 * synthetic.PICOSECOND
+        sp.gamma = 1000.0 / synthetic.PICOSECOND
+    print("\n[batch32] %d members, %d atoms" % (len(specs), sum(sp.n for sp in specs)), flush=True)
+    solo = []
+    for sp in specs:
+        s = system.build(sp)
+        s.run_energy()
+        assert s.L.bm_dynamics(s.device(), 1, 21, FREQ_NRG, 1) == 0
+        assert s.L.bm_sync(s.device()) == 0
+        solo.append(system.state(s, sp.n, sp.nblocks))
+        s.close()
+    systems = []
+    for sp in specs:
+        s = system.build(sp)
+        s.run_energy()
+        systems.append(s)
+    b = system.Batch(systems)
+    b.dynamics(1, 21, FREQ_NRG)
+    b.sync()
+    for k, (sp, s) in enumerate(zip(specs, systems)):
+        x, v, th, tv = system.state(s, sp.n, sp.nblocks)
+        xs, vs, ths, tvs = solo[k]
+        assert np.all(np.isfinite(x))
+        assert np.array_equal(x, xs) and np.array_equal(v, vs) and np.array_equal(th, ths), k
+    b.close()
+    order = np.argsort([sp.n for sp in specs])
+    for k in (order[0], order[len(order) // 2], order[-1]):
+        sp, s = specs[k], systems[k]
+        s.run_energy()
+        top, rp = s.topology(), s.runparams()
+        x, v, th, tv = system.state(s, sp.n, sp.nblocks)
+        lam = orc.theta_to_lambda(top, rp, th[:sp.nblocks])
+        fo, do, eo, pairs = orc.forces(top, rp, s.positions(), lam)
+        fg, eg, dg = s.forces(), s.energy(), s.dudl()
+        frms = rel_rms(fg, fo)
+        erel = abs(eg[20] - eo[20]) / abs(eo[20])
+        ddl = float(np.max(np.abs(dg - do)))
+        print("[batch32 member %d] N=%d force relRMS %.3e energy rel %.3e dudl %.2e"
+              % (k, sp.n, frms, erel, ddl), flush=True)
+        assert frms <= 1e-5 and erel <= 1e-6 and ddl <= 1e-3
+    for s in systems:
+        s.close()

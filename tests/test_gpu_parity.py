@@ -131,35 +131,39 @@ def test_config5_four_site_replica():
     assert ddl <= DUDL_TOL
 
 
-def test_domdec_bit_exact():
-    for name in ("config1", "msld_small", "config2_small", "two_site_small"):
-        s = built(name)
-        top, rp = s.topology(), s.runparams()
-        x0 = system.soa(spec(name).pos)
-        posw, l2g, bc, tiles, masks = orc.domdec(top, rp, x0)
-        s.run_energy()
-        dev = s.device()
-        import ctypes as C
-        L = s.L
-        nb, nt, ne = C.c_int(), C.c_int(), C.c_int()
-        assert L.bm_get_domdec_sizes(dev, C.byref(nb), C.byref(nt), C.byref(ne)) == 0
-        gl2g = np.zeros(32 * nb.value, dtype=np.int32)
-        gbc = np.zeros(nb.value, dtype=np.int32)
-        gti = np.zeros(nt.value, dtype=np.int32)
-        gtj = np.zeros(nt.value, dtype=np.int32)
-        gtm = np.zeros(32 * nt.value, dtype=np.uint32)
-        i32 = C.POINTER(C.c_int)
-        assert L.bm_get_domdec(dev, gl2g.ctypes.data_as(i32), gbc.ctypes.data_as(i32), gti.ctypes.data_as(i32),
-                               gtj.ctypes.data_as(i32), gtm.ctypes.data_as(C.POINTER(C.c_uint32))) == 0
-        print("\n[%s] blocks %d/%d tiles %d/%d" % (name, nb.value, len(bc), nt.value, len(tiles)), flush=True)
-        assert nb.value == len(bc)
-        assert np.array_equal(gl2g, l2g)
-        assert np.array_equal(gbc, bc)
-        assert nt.value == len(tiles)
-        assert np.array_equal(gti, tiles[:, 0]) and np.array_equal(gtj, tiles[:, 1])
-        assert np.array_equal(gtm.reshape(-1, 32), masks)
-        # the wrapped positions are identical too
-        assert np.array_equal(s.positions(), posw)
+@pytest.mark.parametrize("name", ["config1", "msld_small", "config2_small", "two_site_small", "config2",
+                                  "config5"])
+def test_domdec_bit_exact(name):
+    """wrapped positions, slot order, block counts, directed tiles and
+    exclusion masks bit-exact against the oracle's list semantics, up to the
+    23,558-atom box and a 60,000-atom four-site replica"""
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    x0 = system.soa(spec(name).pos)
+    posw, l2g, bc, tiles, masks = orc.domdec(top, rp, x0)
+    s.run_energy()
+    dev = s.device()
+    import ctypes as C
+    L = s.L
+    nb, nt, ne = C.c_int(), C.c_int(), C.c_int()
+    assert L.bm_get_domdec_sizes(dev, C.byref(nb), C.byref(nt), C.byref(ne)) == 0
+    gl2g = np.zeros(32 * nb.value, dtype=np.int32)
+    gbc = np.zeros(nb.value, dtype=np.int32)
+    gti = np.zeros(nt.value, dtype=np.int32)
+    gtj = np.zeros(nt.value, dtype=np.int32)
+    gtm = np.zeros(32 * nt.value, dtype=np.uint32)
+    i32 = C.POINTER(C.c_int)
+    assert L.bm_get_domdec(dev, gl2g.ctypes.data_as(i32), gbc.ctypes.data_as(i32), gti.ctypes.data_as(i32),
+                           gtj.ctypes.data_as(i32), gtm.ctypes.data_as(C.POINTER(C.c_uint32))) == 0
+    print("\n[%s] blocks %d/%d tiles %d/%d" % (name, nb.value, len(bc), nt.value, len(tiles)), flush=True)
+    assert nb.value == len(bc)
+    assert np.array_equal(gl2g, l2g)
+    assert np.array_equal(gbc, bc)
+    assert nt.value == len(tiles)
+    assert np.array_equal(gti, tiles[:, 0]) and np.array_equal(gtj, tiles[:, 1])
+    assert np.array_equal(gtm.reshape(-1, 32), masks)
+    # the wrapped positions are identical too
+    assert np.array_equal(s.positions(), posw)
 
 
 def _run(sp, nsteps, use_graph):

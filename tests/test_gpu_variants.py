@@ -275,3 +275,69 @@ def test_barostat_volume_move_restore_and_npt_run():
     assert np.all(np.isfinite(runs[0][0]))
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
     assert runs[0][3] == runs[1][3]
+
+
+def test_term_variety():
+    """Urey-Bradley terms, periodic impropers, NBFIX pairs and the
+    geometric combination rule"""
+    from blade_amd import synthetic
+    s = _check(synthetic.term_variety(spec("msld_small")), "UB, periodic impropers, NBFIX, geometric")
+    e = s.energy()
+    assert e[2] != 0.0 and e[4] != 0.0
+
+
+def test_soft_core_14_and_soft_core():
+    _check(_variant("msld_small", soft_core=True, soft_core14=True), "soft core + soft-core 1-4")
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+def test_msld_ewald_types(mode):
+    """msldEwaldType 1 (same-block corrections scaled once) and 3 (every
+    correction scaled by both lambdas) on the two-site system"""
+    _check(_variant("two_site_small", ewald_type=mode), "msldEwaldType %d" % mode)
+
+
+VB_ALL = [(1, 2, 1, 0.8, 1.5, 2), (2, 3, 2, 0.1, 1.5, 3), (1, 3, 3, 0.0, 0.7, 2), (2, 1, 4, 0.6, 2.0, 1),
+          (3, 1, 5, 0.0, 0.9, 1), (1, 2, 6, 0.0, 1.1, 1), (2, 3, 7, 0.2, 1.3, 1), (3, 2, 8, 0.2, 1.3, 1),
+          (1, 3, 9, 0.3, 0.8, 2), (2, 1, 10, -2.0, 0.9, 1), (3, 1, 11, 1.5, 0.9, 2), (1, 2, 12, 0.25, 1.2, 2)]
+
+
+def test_variable_biases_all_types():
+    """all twelve variable-bias types at once (energy term BM_ELAMBDA and
+    dU/dlambda)"""
+    s = _check(_variant("msld_small", var_bias=VB_ALL), "variable biases 1-12")
+    top, rp = s.topology(), s.runparams()
+    sp = spec("msld_small")
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    _, do, eo, _ = orc.forces(top, rp, s.positions(), lam, orc.MSLDBIAS)
+    eg = s.energy()
+    assert abs(eg[11] - eo[11]) <= 1e-8 * max(1.0, abs(eo[11])), (eg[11], eo[11])
+
+
+@pytest.mark.parametrize("name", ["config2_small", "config2"])
+def test_stale_list_forces_match_oracle(name):
+    """The list is built at step 0 with a rebuild cadence of 20 (pad
+    2 sqrt(30 kT) 20 dt); after 19 steps, on that list, one force and
+    energy evaluation (pack only, no rebuild) meets the path's bars: no
+    pair has outrun the pad"""
+    from tests.helpers import relaxed
+    sp = copy.copy(relaxed(name))
+    sp.freq_domdec = 20
+    s = system.build(sp)
+    s.run_dynamics(19, freq_nrg=0)
+    L, dev = s.L, s.device()
+    assert L.bm_domdec_update(dev, 0) == 0
+    assert L.bm_calc_force(dev, 19, 1) == 0
+    assert L.bm_sync(dev) == 0
+    top, rp = s.topology(), s.runparams()
+    assert rp.contents.freq_domdec == 20
+    x, v, th, tv = system.state(s, sp.n, sp.nblocks)
+    lam = orc.theta_to_lambda(top, rp, th[:sp.nblocks])
+    fo, do, eo, pairs = orc.forces(top, rp, system.soa(x), lam)
+    fg, eg, dg = s.forces(), s.energy(), s.dudl()
+    frms = rel_rms(fg, fo)
+    erel = abs(eg[20] - eo[20]) / abs(eo[20])
+    ddl = float(np.max(np.abs(dg - do)))
+    print("\n[stale list %s, 19 steps] pairs %d force relRMS %.3e energy rel %.3e dudl %.2e"
+          % (name, pairs, frms, erel, ddl), flush=True)
+    assert frms <= FORCE_TOL and erel <= ENERGY_TOL and ddl <= DUDL_TOL

@@ -290,9 +290,24 @@ E_BIAS = 19
 WIGNER_SC = 2.837297479480620   # Wigner lattice-sum constant of a simple cubic box
 
 
+def set_block_charges(sp, q_blocks):
+    """shift the charges of each block's atoms uniformly so that the block's
+    atom charges sum to q_blocks[b] (the charge the restraints use)"""
+    ab = np.asarray(sp.atom_block)
+    atoms = list(sp.atoms)
+    for b, qb in enumerate(q_blocks):
+        idx = np.nonzero(ab == b)[0]
+        shift = (qb - sum(atoms[i][5] for i in idx)) / len(idx)
+        for i in idx:
+            a = atoms[i]
+            atoms[i] = a[:5] + (a[5] + shift,) + a[6:]
+    sp.atoms = atoms
+
+
 def _charge_restrained(k, q_blocks, beta=None):
     sp = synthetic.two_site_small()
-    sp.block_charge = list(q_blocks)
+    set_block_charges(sp, q_blocks)
+    sp.block_charge = [0.0] * len(q_blocks)   # ignored: the atom sums count
     sp.charge_restraints = tuple(k)
     if beta is not None:
         sp.beta = beta
@@ -601,3 +616,104 @@ def test_domdec_covers_every_pair_and_exclusion():
         k = tile_of[(si // 32, sj // 32)]
         bit = (masks[k][si % 32] >> (sj % 32)) & 1
         assert bit == ((min(i, j), max(i, j)) in excl)
+
+
+def _fd_spec(sp, terms, eidx, atoms, h=1e-5, tol=2e-6, dl_terms=None):
+    """forces of `terms` against central differences of energies `eidx`, and
+    (dl_terms) dU/dlambda against differences in lambda"""
+    s = system.build(sp)
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    n = sp.n
+    x0 = system.soa(sp.pos)
+    lam = (orc.theta_to_lambda(top, rp, np.array(sp.theta0, float)) if top.contents.n_blocks > 1 else np.ones(1))
+    f, d, e, _ = orc.forces(top, rp, x0, lam, terms)
+    scale = max(1.0, float(np.abs(f).max()))
+    worst = 0.0
+    for a in atoms:
+        for k in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[k * n + a] += h
+            xm[k * n + a] -= h
+            ep = sum(orc.forces(top, rp, xp, lam, terms)[2][i] for i in eidx)
+            em = sum(orc.forces(top, rp, xm, lam, terms)[2][i] for i in eidx)
+            worst = max(worst, abs((ep - em) / (2 * h) - f[k * n + a]) / scale)
+    assert worst < tol, worst
+    if dl_terms:
+        for b in range(1, len(lam)):
+            lp, lm = lam.copy(), lam.copy()
+            lp[b] += 1e-6
+            lm[b] -= 1e-6
+            fd = (sum(orc.forces(top, rp, x0, lp, terms)[2][i] for i in eidx)
+                  - sum(orc.forces(top, rp, x0, lm, terms)[2][i] for i in eidx)) / 2e-6
+            assert abs(fd - d[b]) < 1e-5 * max(1.0, abs(d[b])), (b, fd, d[b])
+    return e
+
+
+def test_soft_core_14_gradients_and_dudl():
+    """useSoftCore14 (bonded/pair.cu:124-150): 1-4 pairs of lambda-scaled
+    atoms inside r_soft = 4 (1 - lambda_i lambda_j) A use the soft effective
+    distance; forces and dU/dlambda are derivatives of the 1-4 energy"""
+    import copy
+    sp = copy.copy(spec("msld_small"))
+    sp.soft_core14 = True
+    s = system.build(sp)
+    s.prepare()
+    t = s.topology().contents
+    idx = t.array("nb14_idx", t.n_nb14, 2)
+    sb = t.array("nb14_sb", t.n_nb14, 2)
+    lam = orc.theta_to_lambda(s.topology(), s.runparams(), np.array(sp.theta0, float))
+    x = np.asarray(sp.pos)
+    L = np.array(sp.box)
+    inside = []
+    for (i, j), (a, b) in zip(idx, sb):
+        if a & 0xFFFF:
+            ll = lam[a & 0xFFFF] * (lam[b & 0xFFFF] if b & 0xFFFF else 1.0)
+            d = x[i] - x[j]
+            d -= L * np.round(d / L)
+            if np.linalg.norm(d) < 4 * (1 - ll):
+                inside += [int(i), int(j)]
+    assert inside, "some scaled 1-4 pairs lie inside the soft radius"
+    e_soft = _fd_spec(sp, orc.NB14, [E_NB14], sorted(set(inside))[:6], dl_terms=True)
+    sp.soft_core14 = False
+    s2 = system.build(sp)
+    s2.prepare()
+    e_hard = orc.forces(s2.topology(), s2.runparams(), system.soa(sp.pos), lam, orc.NB14)[2]
+    assert e_soft[E_NB14] != e_hard[E_NB14]
+
+
+def test_term_variety_gradients():
+    """Urey-Bradley, periodic impropers, NBFIX pairs and geometric
+    combination: forces are energy gradients for the bonded, 1-4 and
+    nonbonded terms"""
+    from blade_amd import synthetic
+    sp = synthetic.term_variety(spec("msld_small"))
+    atoms = _polymer_atoms("msld_small")
+    e = _fd_spec(sp, orc.BONDED, [E_BOND, E_ANGLE, E_UREY, E_DIHE, E_IMPR], atoms)
+    assert e[E_UREY] != 0.0 and e[E_IMPR] != 0.0
+    _fd_spec(sp, orc.NB14, [E_NB14], atoms)
+    _fd_spec(sp, orc.NBDIRECT, [E_NBDIRECT], atoms[:3] + [0, 1])
+
+
+def test_chain_rule_with_a_fixed_block():
+    """a fixed block holds lambda = theta; the others share 1 - theta_fixed
+    by the softmax (msld.cu:880-898) and the chain rule follows
+    theta_to_lambda's derivative (msld.cu:955-971)"""
+    import copy
+    sp = copy.copy(spec("msld_small"))
+    sp.block_fixed = [0, 0, 1, 0]
+    s = system.build(sp)
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    th = np.array([0.0, 0.4, 0.25, -0.6])
+    lam = orc.theta_to_lambda(top, rp, th)
+    assert lam[2] == 0.25 and abs(lam[1:].sum() - 1) < 1e-14
+    dudl = np.array([0.0, 3.0, -2.0, 5.0])
+    got = orc.chain_rule(top, rp, th, lam, dudl)
+    assert got[2] == 0.0
+    for i in (1, 3):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += 1e-6
+        tm[i] -= 1e-6
+        dl = (orc.theta_to_lambda(top, rp, tp) - orc.theta_to_lambda(top, rp, tm)) / 2e-6
+        assert abs(got[i] - np.dot(dudl, dl)) < 1e-7

@@ -138,3 +138,109 @@ def test_nb14_parameters():
         assert abs(c6[k] - 2 * eps * s6) <= 1e-12 * abs(2 * eps * s6)
         assert abs(e14[k] - np.sqrt(f_i) * np.sqrt(f_j)) < 1e-15
         assert abs(qq[k] - q[i] * q[j]) < 1e-15
+
+
+def _variety(name):
+    from blade_amd import synthetic, system
+    sp = synthetic.term_variety(spec(name))
+    s = system.build(sp)
+    s.prepare()
+    return sp, s
+
+
+def test_vdw_table_nbfix_and_geometric_combination():
+    """src/system/potential.cxx:1359-1407: a type pair with an NBFIX entry
+    (either order) takes its eps and Rmin as given; otherwise eps =
+    sqrt(eps_i eps_j) and Rmin = Rmin/2_i + Rmin/2_j, or 2 sqrt(Rmin/2_i
+    Rmin/2_j) when both types select the geometric rule; c12 = eps Rmin^12,
+    c6 = 2 eps Rmin^6.  1-4 pairs use the eps14/Rmin14 columns the same way."""
+    sp, s = _variety("msld_small")
+    t = s.topology().contents
+    T = t.n_types
+    tidx = t.array("type_idx", t.n_atoms)
+    c12 = t.array("vdw_c12", T * T)
+    c6 = t.array("vdw_c6", T * T)
+    first = {}
+    for i, a in enumerate(sp.atoms):
+        first.setdefault(a[4], i)
+
+    def want(ta, tb, is14):
+        fx = sp.nbfix_par.get((ta, tb), sp.nbfix_par.get((tb, ta)))
+        if fx is not None:
+            eps, rmin = (-fx[2], fx[3]) if is14 else (-fx[0], fx[1])
+        else:
+            A, B = sp.nb_par[ta], sp.nb_par[tb]
+            ea, ra = (-A[2], A[3]) if is14 else (-A[0], A[1])
+            eb, rb = (-B[2], B[3]) if is14 else (-B[0], B[1])
+            eps = np.sqrt(ea * eb)
+            rmin = 2 * np.sqrt(ra * rb) if (A[5] == 1 and B[5] == 1) else ra + rb
+        return eps * rmin ** 12, 2 * eps * rmin ** 6
+
+    seen_fix = seen_geo = 0
+    for ta, ia in first.items():
+        for tb, ib in first.items():
+            k = tidx[ia] * T + tidx[ib]
+            w12, w6 = want(ta, tb, False)
+            assert abs(c12[k] - w12) <= 1e-12 * abs(w12) and abs(c6[k] - w6) <= 1e-12 * abs(w6), (ta, tb)
+            seen_fix += ((ta, tb) in sp.nbfix_par)
+            seen_geo += (sp.nb_par[ta][5] == 1 and sp.nb_par[tb][5] == 1)
+    assert seen_fix > 0 and seen_geo > 0
+    idx = t.array("nb14_idx", t.n_nb14, 2)
+    p12, p6 = t.array("nb14_c12", t.n_nb14), t.array("nb14_c6", t.n_nb14)
+    for k, (i, j) in enumerate(idx):
+        w12, w6 = want(sp.atoms[i][4], sp.atoms[j][4], True)
+        assert abs(p12[k] - w12) <= 1e-12 * abs(w12) and abs(p6[k] - w6) <= 1e-12 * abs(w6)
+
+
+def test_urey_bradley_terms_follow_the_angles():
+    """src/system/potential.cxx:597-645: every angle whose parameter has a
+    Urey-Bradley constant adds a 1-3 bond term after the 1-2 bonds, with
+    kb = 2 kub (the setter doubles it) and b0 = s0"""
+    sp, s = _variety("msld_small")
+    t = s.topology().contents
+    nub = t.n_bonds - t.n_bonds12
+    idx = t.array("bond_idx", t.n_bonds, 2)[t.n_bonds12:]
+    kb = t.array("bond_kb", t.n_bonds)[t.n_bonds12:]
+    b0 = t.array("bond_b0", t.n_bonds)[t.n_bonds12:]
+    want = []
+    for a, b, c in sp.angles:
+        key = tuple(sp.atoms[x][4] for x in (a, b, c))
+        par = sp.angle_par.get(key, sp.angle_par.get(key[::-1]))
+        if par[2] != 0:
+            want.append((a, c, 2 * par[2], par[3]))
+    assert nub == len(want) > 0
+    for (i, j), k, r, w in zip(idx, kb, b0, want):
+        assert (i, j) == w[:2] and abs(k - w[2]) < 1e-12 and abs(r - w[3]) < 1e-12
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+def test_pme_exclusion_corrections_ewald_types(mode):
+    """src/msld/msld.cu:623-668: mode 1 scales a same-block correction by
+    that block's lambda once and drops pairs of different blocks of one
+    site; mode 3 keeps every non-1-4 exclusion, scaled by both atoms'
+    lambdas (siteBlock codes in descending block order)"""
+    import copy
+    from blade_amd import system
+    sp = copy.copy(spec("two_site_small"))
+    sp.ewald_type = mode
+    s = system.build(sp)
+    s.prepare()
+    t = s.topology().contents
+    allx, nb14, _ = _derive(sp)
+    block = np.asarray(sp.atom_block)
+    site = np.asarray(sp.block_site)
+    want = {}
+    for i, j in allx - nb14:
+        b = sorted((int(block[i]), int(block[j])), reverse=True)
+        if mode == 1:
+            if b[0] == b[1]:
+                b[1] = 0
+            if b[0] != b[1] and site[b[0]] == site[b[1]]:
+                continue
+        want[(i, j)] = [(int(site[x]) << 16) | x for x in b]
+    got = {}
+    for (i, j), sb in zip(t.array("nbex_idx", t.n_nbex, 2), t.array("nbex_sb", t.n_nbex, 2)):
+        got[(min(i, j), max(i, j))] = list(sb)
+    assert got == want
+    same = [p for p in want if block[p[0]] == block[p[1]] and block[p[0]]]
+    assert same, "the system has same-block corrections"
