@@ -56,6 +56,13 @@ class Topology(C.Structure):
         ("piecewise", C.c_int32), ("pw_width", C.c_double), ("pw_k", C.c_double),
         ("edge_k", C.c_double), ("edge_n", C.c_double), ("edge_alpha", C.c_double), ("edge_phi", C.c_double),
         ("fix_lambda", C.c_int32),
+        ("n_noe", C.c_int32), ("noe_idx", _i32p), ("noe_par", _f64p),
+        ("n_harm", C.c_int32), ("harm_idx", _i32p), ("harm_par", _f64p),
+        ("n_borest", C.c_int32), ("borest_idx", _i32p), ("borest_block", _i32p), ("borest_par", _f64p),
+        ("n_anrest", C.c_int32), ("anrest_idx", _i32p), ("anrest_block", _i32p), ("anrest_par", _f64p),
+        ("n_direst", C.c_int32), ("direst_idx", _i32p), ("direst_block", _i32p), ("direst_n", _i32p),
+        ("direst_par", _f64p),
+        ("n_resd", C.c_int32), ("resd_idx", _i32p), ("resd_par", _f64p),
     ]
 
     def array(self, name, count, width=1):
@@ -135,6 +142,14 @@ def lib():
         "blade_add_dihe": (None, [vp, i, i, i, i]),
         "blade_add_impr": (None, [vp, i, i, i, i]),
         "blade_add_shake": (None, [vp, i]),
+        "blade_add_virt2": (None, [vp, i, i, i, d, d]),
+        "blade_add_virt3": (None, [vp, i, i, i, i, d, d, d]),
+        "blade_add_noe": (None, [vp, i, i, d, d, d, d, d, d, d, d, d, d, C.c_bool]),
+        "blade_add_harmonic": (None, [vp, i, d, d, d, d, d]),
+        "blade_add_borest": (None, [vp, i, i, d, d, i]),
+        "blade_add_anrest": (None, [vp, i, i, i, d, d, i]),
+        "blade_add_direst": (None, [vp, i, i, i, i, d, i, d, d, i]),
+        "blade_add_resd": (None, [vp, i, i, i, i, d, d, d, d]),
         "blade_init_coordinates": (None, [vp, i]),
         "blade_add_coordinates_position": (None, [vp, i, d, d, d]),
         "blade_add_coordinates_velocity": (None, [vp, i, d, d, d]),
@@ -203,6 +218,14 @@ def lib():
         "bm_calc_force": (i, [vp, C.c_long, C.c_uint]),
         "bm_reset_force": (i, [vp, C.c_uint]),
         "bm_getforce_nbdirect": (i, [vp, C.c_uint]),
+        "bm_getforce_restrain": (i, [vp, C.c_uint]),
+        "bm_getforce_ewaldself": (i, [vp, C.c_uint]),
+        "bm_getforce_nb14": (i, [vp, C.c_uint]),
+        "bm_getforce_nbex": (i, [vp, C.c_uint]),
+        "bm_getforce_msldbias": (i, [vp, C.c_uint]),
+        "bm_holonomic_position": (i, [vp]),
+        "bm_holonomic_velocity": (i, [vp]),
+        "bm_msld_chain_rule": (i, [vp]),
         "bm_getforce_ewald": (i, [vp, C.c_uint]),
         "bm_getforce_bonded": (i, [vp, C.c_uint]),
         "bm_update": (i, [vp, C.c_long, i]),
@@ -228,6 +251,7 @@ def lib():
         "bm_get_positions": (i, [vp, _f64p, _f64p]),
         "bm_set_positions": (i, [vp, _f64p, _f64p]),
         "bm_get_velocities": (i, [vp, _f64p, _f64p]),
+        "bm_set_velocities": (i, [vp, _f64p, _f64p]),
         "bm_get_forces": (i, [vp, _f64p, _f64p]),
         "bm_get_force_fixed": (i, [vp, C.POINTER(C.c_longlong)]),
         "bm_get_dudl": (i, [vp, _f64p]),

@@ -156,6 +156,42 @@ void blade_add_cmap(System *s, int i1, int j1, int k1, int l1, int i2, int j2, i
   s->structure->cmaps.push_back({i1 - 1, j1 - 1, k1 - 1, l1 - 1, i2 - 1, j2 - 1, k2 - 1, l2 - 1});
 }
 void blade_add_shake(System *s, int shakeHbond) { s->structure->shakeHbond = shakeHbond != 0; }
+// virtual sites (system/structure.cxx:1059-1091) are recorded; a system that
+// has any is rejected when its terms are derived
+void blade_add_virt2(System *s, int, int, int, double, double) { s->structure->virtualSites++; }
+void blade_add_virt3(System *s, int, int, int, int, double, double, double) { s->structure->virtualSites++; }
+// restraints (system/structure.cxx:1093-1196): 1-based atoms and blocks,
+// angles in degrees, AKMA units
+void blade_add_noe(System *s, int i, int j, double rmin, double kmin, double rmax, double kmax, double rpeak,
+                   double rswitch, double nswitch, double c0x, double c0y, double c0z, bool is_pnoe) {
+  const int n = (int)s->structure->atoms.size();
+  if (i < 1 || i > n || (!is_pnoe && (j < 1 || j > n))) { api_fail(s, "NOE: invalid atom index"); return; }
+  s->structure->noes.push_back(NoeRec{i - 1, j - 1, rmin, kmin, rmax, kmax, rpeak, rswitch, nswitch, {c0x, c0y, c0z},
+                                      is_pnoe});
+}
+void blade_add_harmonic(System *s, int i, double k, double x0, double y0, double z0, double n) {
+  s->structure->harms.push_back(HarmRec{i - 1, k, {x0, y0, z0}, n});
+}
+void blade_add_borest(System *s, int i, int j, double kr, double r0, int lambdaBlock) {
+  s->structure->boRests.push_back(BoRestRec{i - 1, j - 1, kr, r0, lambdaBlock - 1});
+}
+void blade_add_anrest(System *s, int i, int j, int k, double kt, double t0, int lambdaBlock) {
+  s->structure->anRests.push_back(AnRestRec{i - 1, j - 1, k - 1, kt, kDegrees * t0, lambdaBlock - 1});
+}
+void blade_add_direst(System *s, int i, int j, int k, int l, double kphi, int nphi, double phi0, double width,
+                      int lambdaBlock) {
+  s->structure->diRests.push_back(
+      DiRestRec{i - 1, j - 1, k - 1, l - 1, kphi, nphi, kDegrees * phi0, kDegrees * width, lambdaBlock - 1});
+}
+void blade_add_resd(System *s, int i1, int i2, int j1, int j2, double ci, double cj, double rdist, double kdist) {
+  s->structure->resds.push_back(ResdRec{i1 - 1, i2 - 1, j1 - 1, j2 - 1, ci, cj, rdist, kdist});
+}
+// the TorchANI machine-learned potential (system/structure.cxx:1198-1221)
+// needs LibTorch: the system is marked unsupported
+void blade_tani_internal_setup(System *s, const int, const char *, const int, const int *, const int *, const int *,
+                               const int) {
+  mark_unsupported(s, "machine-learned potentials (blade_tani_internal_setup)");
+}
 int blade_structure_atom_count(System *s) { return (s && s->structure) ? (int)s->structure->atoms.size() : 0; }
 int blade_structure_atom_names(System *s, int atomIdx, char *seg, char *res, char *name, int len) {
   if (!s || !s->structure || atomIdx < 1 || atomIdx > (int)s->structure->atoms.size() || len <= 0) return -1;

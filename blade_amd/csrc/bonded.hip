@@ -217,12 +217,15 @@ __device__ double theta_bias_site(const Dev &d, int s) {
 }
 
 struct BondedLaunch {
-  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_cats, b_cmap, b_bias;  // first block of each family
+  // first block of each family, then the bias workgroup
+  int b_bond, b_angle, b_dihe, b_impr, b_nb14, b_nbex, b_cats, b_cmap;
+  int b_noe, b_harm, b_bor, b_anr, b_dir, b_resd, b_bias;
+  unsigned fam;   // families to evaluate (kFam*); the others zero their slots
 };
 
 // workgroups of 256 threads per term family, in family order; the last
-// workgroup carries the Ewald self term and the lambda biases
-__host__ __device__ inline BondedLaunch bonded_layout(const Dev &d) {
+// workgroup carries the Ewald self term and the lambda and theta biases
+__host__ __device__ inline BondedLaunch bonded_layout(const Dev &d, unsigned fam = kFamAll) {
   auto nb = [](int n) { return (n + 255) / 256; };
   BondedLaunch L;
   L.b_bond = 0;
@@ -233,14 +236,206 @@ __host__ __device__ inline BondedLaunch bonded_layout(const Dev &d) {
   L.b_nbex = L.b_nb14 + nb(d.nnb14);
   L.b_cats = L.b_nbex + nb(d.nnbex);
   L.b_cmap = L.b_cats + nb(d.ncats);
-  L.b_bias = L.b_cmap + nb(d.ncmap);
+  L.b_noe = L.b_cmap + nb(d.ncmap);
+  L.b_harm = L.b_noe + nb(d.nnoe);
+  L.b_bor = L.b_harm + nb(d.nharm);
+  L.b_anr = L.b_bor + nb(d.nbor);
+  L.b_dir = L.b_anr + nb(d.nanr);
+  L.b_resd = L.b_dir + nb(d.ndir);
+  L.b_bias = L.b_resd + nb(d.nresd);
+  L.fam = fam;
   return L;
+}
+
+__device__ __forceinline__ D3v dsub(D3v a, D3v b) { return D3v{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3v dadd(D3v a, D3v b) { return D3v{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3v dscale(double s, D3v a) { return D3v{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double ddot(D3v a, D3v b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3v dcross(D3v a, D3v b) {
+  return D3v{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ void put_v(const Dev &d, int slot, D3v f) { put_d(d, slot, 1.0, f); }
+__device__ __forceinline__ void zero_slots(const Dev &d, int first, int count) {
+  for (int k = 0; k < count; k++) put_d(d, first + k, 0.0, D3v{0.0, 0.0, 0.0});
+}
+
+// Restraint terms in FP64 (restrain/restrain.cu:15-598): term t of the
+// family starting at block b0 of the launch; returns the energy.
+__device__ double restraint_term(const Dev &d, const BondedLaunch &L, int bid, int t) {
+  double e = 0.0;
+  if (bid < L.b_harm) {
+    // NOE: flat-bottom harmonic on a pair distance (or the distance to a
+    // fixed point), optionally switched to a power-law tail
+    if (t >= d.nnoe) return 0.0;
+    const int2 ij = d.noe_i[t];
+    const double *p = d.noe_p + 10 * (size_t)t;
+    D3v dr;
+    if (ij.y >= 0) {
+      dr = dispd(d, ij.x, ij.y);
+    } else {
+      dr = D3v{d.pos[ij.x] - p[7], d.pos[d.n + ij.x] - p[8], d.pos[2 * d.n + ij.x] - p[9]};
+      dr.x -= d.box[0] * floor(dr.x * d.invbox[0] + 0.5);
+      dr.y -= d.box[1] * floor(dr.y * d.invbox[1] + 0.5);
+      dr.z -= d.box[2] * floor(dr.z * d.invbox[2] + 0.5);
+    }
+    const double r = sqrt(ddot(dr, dr));
+    double f = 0.0;
+    if (r < p[0]) {
+      f = p[1] * (r - p[0]);
+      e = 0.5 * f * (r - p[0]);
+    } else if (r > p[2]) {
+      const double dd = r - p[2], rpk = p[4], rsw = p[5], nsw = p[6];
+      if (rsw > 0 && dd > rsw) {
+        const double b = (rpk - rsw) / nsw * pow(rsw, nsw + 1);
+        const double a = 0.5 * rsw * rsw - rpk * rsw - rsw * (rpk - rsw) / nsw;
+        f = p[3] * (rpk - b * pow(dd, -nsw - 1));
+        e = p[3] * (a + b * pow(dd, -nsw) + rpk * dd);
+      } else {
+        f = p[3] * dd;
+        e = 0.5 * f * dd;
+      }
+    }
+    const double g = r > 0 ? f / r : 0.0;
+    put_d(d, d.sl_noe + 2 * t, g, dr);
+    put_d(d, d.sl_noe + 2 * t + 1, ij.y >= 0 ? -g : 0.0, dr);
+  } else if (bid < L.b_bor) {
+    // harmonic position restraint k |x - x0|^n
+    if (t >= d.nharm) return 0.0;
+    const int i = d.harm_i[t];
+    const double *p = d.harm_p + 5 * (size_t)t;
+    D3v dr{d.pos[i] - p[1], d.pos[d.n + i] - p[2], d.pos[2 * d.n + i] - p[3]};
+    dr.x -= d.box[0] * floor(dr.x * d.invbox[0] + 0.5);
+    dr.y -= d.box[1] * floor(dr.y * d.invbox[1] + 0.5);
+    dr.z -= d.box[2] * floor(dr.z * d.invbox[2] + 0.5);
+    const double r2 = ddot(dr, dr);
+    const double krn2 = r2 > 0 ? p[0] * pow(r2, 0.5 * p[4] - 1) : 0.0;
+    put_d(d, d.sl_harm + t, p[4] * krn2, dr);
+    e = krn2 * r2;
+  } else if (bid < L.b_anr) {
+    // bond restraint kr/2 (r - r0)^2 scaled by lambda of its block
+    if (t >= d.nbor) return 0.0;
+    const int4 ix = d.bor_i[t];
+    const double2 p = d.bor_p[t];
+    const double l = ix.z ? d.lambda[ix.z] : 1.0;
+    const D3v dr = dispd(d, ix.x, ix.y);
+    const double r = sqrt(ddot(dr, dr)), e0 = 0.5 * p.x * (r - p.y) * (r - p.y), f = p.x * (r - p.y) * l;
+    if (ix.z) atomic_fix(&d.dudl[ix.z], fix_d(e0));
+    put_d(d, d.sl_bor + 2 * t, f / r, dr);
+    put_d(d, d.sl_bor + 2 * t + 1, -f / r, dr);
+    e = e0 * l;
+  } else if (bid < L.b_dir) {
+    // angle restraint kt/2 (theta - t0)^2 scaled by lambda of its block
+    if (t >= d.nanr) return 0.0;
+    const int4 ix = d.anr_i[t];
+    const int b = d.anr_b[t];
+    const double2 p = d.anr_p[t];
+    const double l = b ? d.lambda[b] : 1.0;
+    const D3v rij = dispd(d, ix.x, ix.y), rkj = dispd(d, ix.z, ix.y), c = dcross(rij, rkj);
+    const double mc = sqrt(ddot(c, c)), th = atan2(mc, ddot(rij, rkj));
+    const double e0 = 0.5 * p.x * (th - p.y) * (th - p.y), f = p.x * (th - p.y) * l;
+    if (b) atomic_fix(&d.dudl[b], fix_d(e0));
+    const D3v fi = dscale(f / (mc * ddot(rij, rij)), dcross(rij, c));
+    const D3v fk = dscale(-f / (mc * ddot(rkj, rkj)), dcross(rkj, c));
+    put_v(d, d.sl_anr + 3 * t, fi);
+    put_v(d, d.sl_anr + 3 * t + 2, fk);
+    put_v(d, d.sl_anr + 3 * t + 1, dscale(-1.0, dadd(fi, fk)));
+    e = e0 * l;
+  } else if (bid < L.b_resd) {
+    // dihedral restraint: periodic kphi (1 - cos(n (phi - phi0))), or a
+    // flat-bottom harmonic in the wrapped phi - phi0; phi from
+    // m = (j - i) x (k - j) and n = (k - j) x (l - k)
+    if (t >= d.ndir) return 0.0;
+    const int4 ix = d.dir_i[t];
+    const int2 bn = d.dir_bn[t];
+    const double4 p = d.dir_p[t];
+    const double l = bn.x ? d.lambda[bn.x] : 1.0;
+    const D3v rij = dispd(d, ix.y, ix.x), rjk = dispd(d, ix.z, ix.y), rkl = dispd(d, ix.w, ix.z);
+    const D3v m = dcross(rij, rjk), nv = dcross(rjk, rkl);
+    const double mm = sqrt(ddot(m, m)), nn = sqrt(ddot(nv, nv)), bb = sqrt(ddot(rjk, rjk));
+    const double phi = atan2(ddot(dscale(1.0 / bb, rjk), dcross(dscale(1.0 / mm, m), dscale(1.0 / nn, nv))),
+                             ddot(m, nv) / (mm * nn));
+    const double twopi = 6.283185307179586, pi = 3.141592653589793;
+    double fphi, e0;
+    if (bn.y > 0) {
+      double dp = bn.y * (phi - p.y);
+      dp -= twopi * floor((dp + pi) / twopi);
+      fphi = p.x * bn.y * sin(dp);
+      e0 = p.x * (1.0 - cos(dp));
+    } else {
+      double dp = phi - p.y;
+      dp -= twopi * floor((dp + pi) / twopi);
+      if (p.z > 0) dp = fabs(dp) <= p.z ? 0.0 : dp - copysign(p.z, dp);
+      fphi = p.x * dp;
+      e0 = 0.5 * p.x * dp * dp;
+    }
+    if (bn.x) atomic_fix(&d.dudl[bn.x], fix_d(e0));
+    fphi *= l;
+    const D3v fi = dscale(-fphi * bb / (mm * mm), m), fk = dscale(-fphi * bb / (nn * nn), nv);
+    const double pp = ddot(rij, rjk) / (bb * bb), qq = ddot(rkl, rjk) / (bb * bb);
+    const D3v fj = dadd(dscale(-pp, fi), dscale(-qq, fk));
+    put_v(d, d.sl_dir + 4 * t, fi);
+    put_v(d, d.sl_dir + 4 * t + 3, dscale(-1.0, fk));
+    put_v(d, d.sl_dir + 4 * t + 2, dsub(fk, fj));
+    put_v(d, d.sl_dir + 4 * t + 1, dsub(fj, fi));
+    e = e0 * l;
+  } else {
+    // RESD: kdist/2 (ci r1 + cj r2 - rdist)^2
+    if (t >= d.nresd) return 0.0;
+    const int4 ix = d.resd_i[t];
+    const double4 p = d.resd_p[t];
+    const D3v d1 = dispd(d, ix.x, ix.y), d2 = dispd(d, ix.z, ix.w);
+    const double r1 = sqrt(ddot(d1, d1)), r2 = sqrt(ddot(d2, d2));
+    const double del = p.x * r1 + p.y * r2 - p.z, f = del * p.w;
+    put_d(d, d.sl_resd + 4 * t, f * p.x / r1, d1);
+    put_d(d, d.sl_resd + 4 * t + 1, -f * p.x / r1, d1);
+    put_d(d, d.sl_resd + 4 * t + 2, f * p.y / r2, d2);
+    put_d(d, d.sl_resd + 4 * t + 3, -f * p.y / r2, d2);
+    e = 0.5 * f * del;
+  }
+  return e;
+}
+
+// Family of workgroup bid: its mask bit, energy term, first slot and slots
+// per term (for zeroing when the family is off)
+struct FamInfo { unsigned bit; int eterm, first, width, count, b0; };
+__device__ __forceinline__ FamInfo family_of(const Dev &d, const BondedLaunch &L, int bid) {
+  if (bid < L.b_angle) return FamInfo{kFamBond, BM_EBOND, d.sl_bond, 2, d.nbond, L.b_bond};
+  if (bid < L.b_dihe) return FamInfo{kFamAngle, BM_EANGLE, d.sl_angle, 3, d.nangle, L.b_angle};
+  if (bid < L.b_impr) return FamInfo{kFamDihe, BM_EDIHE, d.sl_dihe, 4, d.ndihe, L.b_dihe};
+  if (bid < L.b_nb14) return FamInfo{kFamImpr, BM_EIMPR, d.sl_impr, 4, d.nimpr, L.b_impr};
+  if (bid < L.b_nbex) return FamInfo{kFamNb14, BM_ENB14, d.sl_nb14, 2, d.nnb14, L.b_nb14};
+  if (bid < L.b_cats) return FamInfo{kFamNbex, BM_ENBRECIPEXCL, d.sl_nbex, 2, d.nnbex, L.b_nbex};
+  if (bid < L.b_cmap) return FamInfo{kFamCats, BM_ECATS, d.sl_cats, 0, d.ncats, L.b_cats};
+  if (bid < L.b_noe) return FamInfo{kFamCmap, BM_ECMAP, d.sl_cmap, 8, d.ncmap, L.b_cmap};
+  if (bid < L.b_harm) return FamInfo{kFamRestrain, BM_ENOE, d.sl_noe, 2, d.nnoe, L.b_noe};
+  if (bid < L.b_bor) return FamInfo{kFamRestrain, BM_EHARMONIC, d.sl_harm, 1, d.nharm, L.b_harm};
+  if (bid < L.b_anr) return FamInfo{kFamRestrain, BM_EMMFP, d.sl_bor, 2, d.nbor, L.b_bor};
+  if (bid < L.b_dir) return FamInfo{kFamRestrain, BM_EMMFP, d.sl_anr, 3, d.nanr, L.b_anr};
+  if (bid < L.b_resd) return FamInfo{kFamRestrain, BM_EMMFP, d.sl_dir, 4, d.ndir, L.b_dir};
+  if (bid < L.b_bias) return FamInfo{kFamRestrain, BM_ERESD, d.sl_resd, 4, d.nresd, L.b_resd};
+  return FamInfo{kFamSelf | kFamBias, BM_ELAMBDA, 0, 0, 0, L.b_bias};
 }
 
 __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L, int bid, bool energy) {
   double e = 0.0;
   int slot = -1;
-  if (bid < L.b_angle) {
+  const FamInfo fi = family_of(d, L, bid);
+  if (!(fi.bit & L.fam)) {
+    // a family left out writes zero forces to its slots (the assembly adds
+    // every slot) and no energy
+    const int t = (bid - fi.b0) * blockDim.x + threadIdx.x;
+    if (fi.width == 0 && fi.bit == kFamCats) {
+      if (t < d.ncats) zero_slots(d, d.sl_cats + d.cats_bound[t], d.cats_bound[t + 1] - d.cats_bound[t]);
+    } else if (t < fi.count) {
+      zero_slots(d, fi.first + fi.width * t, fi.width);
+    }
+    return;
+  }
+  if (bid >= L.b_noe && bid < L.b_bias) {
+    const int t = (bid - fi.b0) * blockDim.x + threadIdx.x;
+    e = restraint_term(d, L, bid, t);
+    slot = fi.eterm;
+  } else if (bid < L.b_angle) {
     const int t = (bid - L.b_bond) * blockDim.x + threadIdx.x;
     if (t < d.nbond) {
       int2 ij = d.bond_i[t];
@@ -417,7 +612,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
         e += 0.5 * d.cats_k * (r.x * r.x + r.y * r.y + r.z * r.z);
       }
     }
-  } else if (bid < L.b_bias) {
+  } else if (bid < L.b_noe) {
     // CMAP cross terms (bonded.cu:531-720), one thread per term: the two
     // dihedrals in FP32 as in the dihedral family, then bicubic Hermite
     // interpolation of the cell's corner values and slopes in FP64
@@ -491,7 +686,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     // MSLD fixed and variable biases: one thread per block / bias
     slot = BM_ELAMBDA;
     const int t = threadIdx.x;
-    if (d.elecmethod == BM_PME && t == 0) {
+    if (d.elecmethod == BM_PME && t == 0 && (L.fam & kFamSelf)) {
       // Ewald self term per block (nbrecip.cu:19-50): -beta kE/sqrt(pi)
       // lambda_b^2 sum_{i in b} q_i^2, in FP64 from the block sums
       const double pref = -(double)d.beta * kElectricD / 1.7724538509055159;
@@ -504,6 +699,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       }
       if (energy) atomic_fix(&d.energy[BM_ENBRECIPSELF], fix_d(es));
     }
+    if (!(L.fam & kFamBias)) return;   // the self term alone
     if (d.ncr && t == 0) {
       // charge restraints on the lambda-weighted net charge Q
       // (msld/msld.cu:1400-1530); dE/dQ times q_b goes to every block but
@@ -590,9 +786,8 @@ __global__ void __launch_bounds__(256) bonded_multi(Multi m, bool energy) {
 
 int grid_bonded(const Dev &d) { return bonded_layout(d).b_bias + 1; }
 
-void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st) {
-  (void)bias;
-  bonded_kernel<<<grid_bonded(d), 256, 0, st>>>(d, bonded_layout(d), energy);
+void launch_bonded(const Dev &d, bool energy, unsigned fam, hipStream_t st) {
+  bonded_kernel<<<grid_bonded(d), 256, 0, st>>>(d, bonded_layout(d, fam), energy);
 }
 
 void launch_bonded_multi(const Multi &m, bool energy, hipStream_t st) { bonded_multi<<<m.nwg, 256, 0, st>>>(m, energy); }

@@ -24,6 +24,12 @@ constexpr double kFixInvD = 1.0 / 4294967296.0;
 // force assembly components (force_assemble mask)
 constexpr unsigned kAsmNb = 1u, kAsmBonded = 2u, kAsmPme = 4u;
 
+// term families of the bonded launch (launch_bonded's mask); a family left
+// out of the mask writes zero forces to its slots
+constexpr unsigned kFamBond = 1u << 0, kFamAngle = 1u << 1, kFamDihe = 1u << 2, kFamImpr = 1u << 3,
+                   kFamNb14 = 1u << 4, kFamNbex = 1u << 5, kFamCats = 1u << 6, kFamCmap = 1u << 7,
+                   kFamRestrain = 1u << 8, kFamSelf = 1u << 9, kFamBias = 1u << 10, kFamAll = 0x7FFu;
+
 // tile list capacity helpers
 constexpr int kBlockAtoms = 32;      // atoms per block (one half-wave)
 constexpr int kTileChunk = 16;       // most directed tiles in one nonbonded work unit
@@ -149,6 +155,15 @@ struct Dev {
   int2 *cmap_g;                  // [grids] (ngrid, first point)
   double4 *cmap_tab;             // per point (E, dE/dphi, dE/dpsi, d2E/dphi dpsi), grid units
   int sl_cmap;
+  // restraints: NOE, harmonic positions, bond/angle/dihedral restraints, RESD
+  int nnoe, nharm, nbor, nanr, ndir, nresd;
+  int2 *noe_i; double *noe_p;    // (i, j or -1); 10 parameters each
+  int *harm_i; double *harm_p;   // atom; 5 parameters each (k, x0, y0, z0, n)
+  int4 *bor_i; double2 *bor_p;   // (i, j, block); (kr, r0)
+  int4 *anr_i; int *anr_b; double2 *anr_p;   // atoms; block; (kt, t0)
+  int4 *dir_i; int2 *dir_bn; double4 *dir_p; // atoms; (block, n); (kphi, phi0, width)
+  int4 *resd_i; double4 *resd_p; // (i1, i2, j1, j2); (ci, cj, rdist, kdist)
+  int sl_noe, sl_harm, sl_bor, sl_anr, sl_dir, sl_resd;
   int ncr;                       // charge restraints on Q = sum_b lambda_b q_b
   double *blockcharge;           // [nL] q_b
   double crk1, crk2, crk3, crq0, crw;

@@ -17,13 +17,17 @@ void launch_pme_grid(const Dev &d, bool energy, hipStream_t st);
 // (launch_update_after_kick does the rest of that step)
 void launch_force_assemble(const Dev &d, unsigned mask, hipStream_t st, bool kick = false);
 bool pme_grid_supported(int K);
-void launch_bonded(const Dev &d, bool energy, bool bias, hipStream_t st);
+// the term families in fam (kFam*); families left out zero their slots
+void launch_bonded(const Dev &d, bool energy, unsigned fam, hipStream_t st);
 // the standard normals of the current step (the O step reads them)
 void launch_normals(const Dev &d, hipStream_t st);
 // a whole update; launches its own normals
 void launch_update(const Dev &d, bool energy_step, hipStream_t st);
 // needs launch_normals of the same step first
 void launch_update_after_kick(const Dev &d, hipStream_t st);
+void launch_constrain_positions(const Dev &d, hipStream_t st);
+void launch_constrain_velocities(const Dev &d, hipStream_t st);
+void launch_chain_rule(const Dev &d, hipStream_t st);
 // init: a new theta from the host (with the fix flag it sets the held lambda)
 void launch_theta_to_lambda(const Dev &d, hipStream_t st, bool init = false);
 // MC barostat: back up and scale every atom, rectify constraint groups

@@ -140,7 +140,7 @@ struct bm_system {
     (void)hipStreamWaitEvent(st2, evFork, 0);
     // a non-energy step's O step runs after the assembly: its normals here
     if (kick) blade::launch_normals(d, st2);
-    blade::launch_bonded(d, energy, bias, st2);
+    blade::launch_bonded(d, energy, blade::kFamAll, st2);
     if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st2);
     (void)hipEventRecord(evJoin, st2);
     blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
@@ -569,6 +569,45 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       }
       UPLOAD(d.cmap_i, ci); UPLOAD(d.cmap_s, cs); UPLOAD(d.cmap_g, cg); UPLOAD(d.cmap_tab, tab);
     }
+    // restraints
+    {
+      d.nnoe = t->n_noe; d.nharm = t->n_harm; d.nbor = t->n_borest;
+      d.nanr = t->n_anrest; d.ndir = t->n_direst; d.nresd = t->n_resd;
+      std::vector<int2> ni(d.nnoe);
+      for (int q = 0; q < d.nnoe; q++) ni[q] = make_int2(t->noe_idx[2 * q], t->noe_idx[2 * q + 1]);
+      std::vector<double> np(t->noe_par, t->noe_par + 10 * (size_t)d.nnoe);
+      std::vector<int> hi(t->harm_idx, t->harm_idx + d.nharm);
+      std::vector<double> hp(t->harm_par, t->harm_par + 5 * (size_t)d.nharm);
+      std::vector<int4> bi(d.nbor), ai(d.nanr), di(d.ndir), ri(d.nresd);
+      std::vector<double2> bp(d.nbor), ap(d.nanr);
+      std::vector<int> ab(d.nanr);
+      std::vector<int2> dbn(d.ndir);
+      std::vector<double4> dp(d.ndir), rpar(d.nresd);
+      for (int q = 0; q < d.nbor; q++) {
+        bi[q] = make_int4(t->borest_idx[2 * q], t->borest_idx[2 * q + 1], t->borest_block[q], 0);
+        bp[q] = make_double2(t->borest_par[2 * q], t->borest_par[2 * q + 1]);
+      }
+      for (int q = 0; q < d.nanr; q++) {
+        ai[q] = make_int4(t->anrest_idx[3 * q], t->anrest_idx[3 * q + 1], t->anrest_idx[3 * q + 2], 0);
+        ab[q] = t->anrest_block[q];
+        ap[q] = make_double2(t->anrest_par[2 * q], t->anrest_par[2 * q + 1]);
+      }
+      for (int q = 0; q < d.ndir; q++) {
+        const int32_t *a = t->direst_idx + 4 * q;
+        di[q] = make_int4(a[0], a[1], a[2], a[3]);
+        dbn[q] = make_int2(t->direst_block[q], t->direst_n[q]);
+        dp[q] = make_double4(t->direst_par[3 * q], t->direst_par[3 * q + 1], t->direst_par[3 * q + 2], 0.0);
+      }
+      for (int q = 0; q < d.nresd; q++) {
+        const int32_t *a = t->resd_idx + 4 * q;
+        const double *p = t->resd_par + 4 * q;
+        ri[q] = make_int4(a[0], a[1], a[2], a[3]);
+        rpar[q] = make_double4(p[0], p[1], p[2], p[3]);
+      }
+      UPLOAD(d.noe_i, ni); UPLOAD(d.noe_p, np); UPLOAD(d.harm_i, hi); UPLOAD(d.harm_p, hp);
+      UPLOAD(d.bor_i, bi); UPLOAD(d.bor_p, bp); UPLOAD(d.anr_i, ai); UPLOAD(d.anr_b, ab); UPLOAD(d.anr_p, ap);
+      UPLOAD(d.dir_i, di); UPLOAD(d.dir_bn, dbn); UPLOAD(d.dir_p, dp); UPLOAD(d.resd_i, ri); UPLOAD(d.resd_p, rpar);
+    }
     // bonded force slots and the atom -> slot CSR (the topology is static)
     d.sl_bond = 0;
     d.sl_angle = d.sl_bond + 2 * d.nbond;
@@ -578,7 +617,13 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     d.sl_nbex = d.sl_nb14 + 2 * d.nnb14;
     d.sl_cats = d.sl_nbex + 2 * d.nnbex;
     d.sl_cmap = d.sl_cats + ncm;
-    d.nslot = d.sl_cmap + 8 * d.ncmap;
+    d.sl_noe = d.sl_cmap + 8 * d.ncmap;
+    d.sl_harm = d.sl_noe + 2 * d.nnoe;
+    d.sl_bor = d.sl_harm + d.nharm;
+    d.sl_anr = d.sl_bor + 2 * d.nbor;
+    d.sl_dir = d.sl_anr + 3 * d.nanr;
+    d.sl_resd = d.sl_dir + 4 * d.ndir;
+    d.nslot = d.sl_resd + 4 * d.nresd;
     std::vector<int> owner(d.nslot);
     for (int q = 0; q < d.nbond; q++)
       for (int k = 0; k < 2; k++) owner[d.sl_bond + 2 * q + k] = t->bond_idx[2 * q + k];
@@ -594,6 +639,15 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       for (int k = 0; k < 2; k++) owner[d.sl_nbex + 2 * q + k] = t->nbex_idx[2 * q + k];
     for (int q = 0; q < ncm; q++) owner[d.sl_cats + q] = t->cats_idx[q];
     for (int q = 0; q < 8 * d.ncmap; q++) owner[d.sl_cmap + q] = t->cmap_idx[q];
+    for (int q = 0; q < d.nnoe; q++) {   // a point NOE's second slot stays zero, owned by atom i
+      owner[d.sl_noe + 2 * q] = t->noe_idx[2 * q];
+      owner[d.sl_noe + 2 * q + 1] = t->noe_idx[2 * q + 1] >= 0 ? t->noe_idx[2 * q + 1] : t->noe_idx[2 * q];
+    }
+    for (int q = 0; q < d.nharm; q++) owner[d.sl_harm + q] = t->harm_idx[q];
+    for (int q = 0; q < 2 * d.nbor; q++) owner[d.sl_bor + q] = t->borest_idx[q];
+    for (int q = 0; q < 3 * d.nanr; q++) owner[d.sl_anr + q] = t->anrest_idx[q];
+    for (int q = 0; q < 4 * d.ndir; q++) owner[d.sl_dir + q] = t->direst_idx[q];
+    for (int q = 0; q < 4 * d.nresd; q++) owner[d.sl_resd + q] = t->resd_idx[q];
     std::vector<int> cstartv(n + 1, 0), pos(d.nslot);
     for (int q = 0; q < d.nslot; q++) cstartv[owner[q] + 1]++;
     for (int a = 0; a < n; a++) cstartv[a + 1] += cstartv[a];
@@ -797,20 +851,41 @@ int bm_getforce_ewald(bm_system *s, unsigned flags) {
   return BM_OK;
 }
 
-// the self term rides on the bonded launch (its last workgroup)
-int bm_getforce_ewaldself(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
-
-int bm_getforce_bonded(bm_system *s, unsigned flags) {
+// The bonded launch's term families one at a time: the launch evaluates
+// the families in fam (the others write zero forces to their slots) and the
+// assembly adds the slots to the force array.
+static int bonded_families(bm_system *s, unsigned flags, unsigned fam) {
   if (!s) return BM_EINVAL;
-  blade::launch_bonded(s->d, flags & BM_CALC_ENERGY, s->bias, s->st);
+  blade::launch_bonded(s->d, flags & BM_CALC_ENERGY, fam, s->st);
   blade::launch_force_assemble(s->d, blade::kAsmBonded, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
-// 1-4 pairs, exclusion corrections and lambda biases share the bonded launch
-int bm_getforce_nb14(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
-int bm_getforce_nbex(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
-int bm_getforce_msldbias(bm_system *s, unsigned) { return s ? BM_OK : BM_EINVAL; }
+
+// getforce_ewaldself (nbrecip/nbrecip.cu:52-77): PME self energy and its
+// dU/dlambda
+int bm_getforce_ewaldself(bm_system *s, unsigned flags) {
+  if (s && s->d.elecmethod != BM_PME) return BM_OK;
+  return bonded_families(s, flags, blade::kFamSelf);
+}
+
+// bond (+Urey-Bradley), angle, dihedral, improper and CMAP terms
+int bm_getforce_bonded(bm_system *s, unsigned flags) {
+  return bonded_families(s, flags, blade::kFamBond | blade::kFamAngle | blade::kFamDihe | blade::kFamImpr |
+                                       blade::kFamCmap);
+}
+int bm_getforce_nb14(bm_system *s, unsigned flags) { return bonded_families(s, flags, blade::kFamNb14); }
+int bm_getforce_nbex(bm_system *s, unsigned flags) {
+  if (s && s->d.elecmethod != BM_PME) return BM_OK;
+  return bonded_families(s, flags, blade::kFamNbex);
+}
+// fixed, variable and theta biases, charge restraints and MSLD atom
+// restraints (msld/msld.cu:1000-1551)
+int bm_getforce_msldbias(bm_system *s, unsigned flags) {
+  return bonded_families(s, flags, blade::kFamBias | blade::kFamCats);
+}
+// NOE, harmonic, bond/angle/dihedral and RESD restraints (restrain/restrain.h:7-12)
+int bm_getforce_restrain(bm_system *s, unsigned flags) { return bonded_families(s, flags, blade::kFamRestrain); }
 
 int bm_calc_force(bm_system *s, long, unsigned flags) {
   if (!s) return BM_EINVAL;
@@ -827,8 +902,18 @@ int bm_update(bm_system *s, long step, int energy_step) {
   return BM_OK;
 }
 
-int bm_holonomic_position(bm_system *s) { return s ? BM_OK : BM_EINVAL; }
-int bm_holonomic_velocity(bm_system *s) { return s ? BM_OK : BM_EINVAL; }
+int bm_holonomic_position(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  blade::launch_constrain_positions(s->d, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+int bm_holonomic_velocity(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  blade::launch_constrain_velocities(s->d, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
 
 int bm_msld_theta_to_lambda(bm_system *s) {
   if (!s) return BM_EINVAL;
@@ -836,7 +921,12 @@ int bm_msld_theta_to_lambda(bm_system *s) {
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
-int bm_msld_chain_rule(bm_system *s) { return s ? BM_OK : BM_EINVAL; }
+int bm_msld_chain_rule(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  blade::launch_chain_rule(s->d, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
 
 int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_graph) {
   if (!s || nsteps < 0) return BM_EINVAL;
@@ -1245,7 +1335,7 @@ int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
     HIPCHK(hipEventRecord(s->ev[1], s->st));
     blade::launch_nbdirect(s->d, false, s->alch, s->soft, s->nwgNb, s->st);
     HIPCHK(hipEventRecord(s->ev[2], s->st));
-    blade::launch_bonded(s->d, false, s->bias, s->st);
+    blade::launch_bonded(s->d, false, blade::kFamAll, s->st);
     HIPCHK(hipEventRecord(s->ev[3], s->st));
     if (s->d.elecmethod == BM_PME) blade::launch_pme_grid(s->d, false, s->st);
     blade::launch_force_assemble(s->d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, s->st);
@@ -1408,6 +1498,17 @@ int bm_set_positions(bm_system *s, const double *pos, const double *theta) {
     HIPCHK(hipMemcpyAsync(s->d.pos + 3 * (size_t)n, theta, s->d.nl * sizeof(double), hipMemcpyHostToDevice, s->st));
     if (s->alch) blade::launch_theta_to_lambda(s->d, s->st, true);
   }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return BM_OK;
+}
+
+int bm_set_velocities(bm_system *s, const double *vel, const double *theta_vel) {
+  if (!s) return BM_EINVAL;
+  const int n = s->d.n;
+  if (vel) HIPCHK(hipMemcpyAsync(s->d.vel, vel, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s->st));
+  if (theta_vel)
+    HIPCHK(hipMemcpyAsync(s->d.vel + 3 * (size_t)n, theta_vel, s->d.nl * sizeof(double), hipMemcpyHostToDevice,
+                          s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   return BM_OK;
 }

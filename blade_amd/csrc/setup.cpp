@@ -475,6 +475,54 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
   t.pwK = m.wellK;
   t.edgeK = m.edgeK; t.edgeN = m.edgeN; t.edgeAlpha = m.edgeAlpha; t.edgePhi = m.edgePhi;
   t.fixLambda = m.fix ? 1 : 0;
+  // restraints (Potential::initialize copies the structure's lists,
+  // potential.cxx:1620-1660)
+  if (s.virtualSites) fail("virtual sites are not supported");
+  auto atom_ok = [&](int a) {
+    if (a < 0 || a >= n) fail("restraint atom index out of range");
+    return a;
+  };
+  auto block_ok = [&](int b) {
+    if (b < 0 || b >= m.blockCount) fail("restraint lambda block out of range");
+    return b;
+  };
+  for (const NoeRec &r : s.noes) {
+    t.noeIdx.push_back(atom_ok(r.i));
+    t.noeIdx.push_back(r.pnoe ? -1 : atom_ok(r.j));
+    const double p[10] = {r.rmin, r.kmin, r.rmax, r.kmax, r.rpeak, r.rswitch, r.nswitch, r.c0[0], r.c0[1], r.c0[2]};
+    t.noePar.insert(t.noePar.end(), p, p + 10);
+  }
+  for (const HarmRec &r : s.harms) {
+    t.harmIdx.push_back(atom_ok(r.i));
+    const double p[5] = {r.k, r.x0[0], r.x0[1], r.x0[2], r.n};
+    t.harmPar.insert(t.harmPar.end(), p, p + 5);
+  }
+  for (const BoRestRec &r : s.boRests) {
+    t.borIdx.push_back(atom_ok(r.i));
+    t.borIdx.push_back(atom_ok(r.j));
+    t.borBlock.push_back(block_ok(r.block));
+    t.borPar.push_back(r.kr);
+    t.borPar.push_back(r.r0);
+  }
+  for (const AnRestRec &r : s.anRests) {
+    for (int a : {r.i, r.j, r.k}) t.anrIdx.push_back(atom_ok(a));
+    t.anrBlock.push_back(block_ok(r.block));
+    t.anrPar.push_back(r.kt);
+    t.anrPar.push_back(r.t0);
+  }
+  for (const DiRestRec &r : s.diRests) {
+    for (int a : {r.i, r.j, r.k, r.l}) t.dirIdx.push_back(atom_ok(a));
+    t.dirBlock.push_back(block_ok(r.block));
+    t.dirN.push_back(r.nphi);
+    t.dirPar.push_back(r.kphi);
+    t.dirPar.push_back(r.phi0);
+    t.dirPar.push_back(r.width);
+  }
+  for (const ResdRec &r : s.resds) {
+    for (int a : {r.i1, r.i2, r.j1, r.j2}) t.resdIdx.push_back(atom_ok(a));
+    const double p[4] = {r.ci, r.cj, r.rdist, r.kdist};
+    t.resdPar.insert(t.resdPar.end(), p, p + 4);
+  }
   (void)kBoltz;
 }
 
@@ -530,6 +578,15 @@ const bm_topology *Topology::view() {
   v.pw_width = pwWidth; v.pw_k = pwK;
   v.edge_k = edgeK; v.edge_n = edgeN; v.edge_alpha = edgeAlpha; v.edge_phi = edgePhi;
   v.fix_lambda = fixLambda;
+  v.n_noe = (int)noeIdx.size() / 2; v.noe_idx = noeIdx.data(); v.noe_par = noePar.data();
+  v.n_harm = (int)harmIdx.size(); v.harm_idx = harmIdx.data(); v.harm_par = harmPar.data();
+  v.n_borest = (int)borBlock.size(); v.borest_idx = borIdx.data(); v.borest_block = borBlock.data();
+  v.borest_par = borPar.data();
+  v.n_anrest = (int)anrBlock.size(); v.anrest_idx = anrIdx.data(); v.anrest_block = anrBlock.data();
+  v.anrest_par = anrPar.data();
+  v.n_direst = (int)dirBlock.size(); v.direst_idx = dirIdx.data(); v.direst_block = dirBlock.data();
+  v.direst_n = dirN.data(); v.direst_par = dirPar.data();
+  v.n_resd = (int)resdIdx.size() / 4; v.resd_idx = resdIdx.data(); v.resd_par = resdPar.data();
   return &top;
 }
 

@@ -43,6 +43,14 @@ struct AtomRec {
   double charge, mass;
 };
 
+// restraint records as the setters store them (system/structure.cxx:1093-1196)
+struct NoeRec { int i, j; double rmin, kmin, rmax, kmax, rpeak, rswitch, nswitch, c0[3]; bool pnoe; };
+struct HarmRec { int i; double k, x0[3], n; };
+struct BoRestRec { int i, j; double kr, r0; int block; };
+struct AnRestRec { int i, j, k; double kt, t0; int block; };
+struct DiRestRec { int i, j, k, l; double kphi; int nphi; double phi0, width; int block; };
+struct ResdRec { int i1, i2, j1, j2; double ci, cj, rdist, kdist; };
+
 struct Structure {
   std::vector<AtomRec> atoms;
   std::vector<std::array<int, 2>> bonds;
@@ -50,6 +58,13 @@ struct Structure {
   std::vector<std::array<int, 4>> dihes, imprs;
   std::vector<std::array<int, 8>> cmaps;
   bool shakeHbond = false;
+  std::vector<NoeRec> noes;
+  std::vector<HarmRec> harms;
+  std::vector<BoRestRec> boRests;
+  std::vector<AnRestRec> anRests;
+  std::vector<DiRestRec> diRests;
+  std::vector<ResdRec> resds;
+  int virtualSites = 0;        // virtual sites are recorded, not supported
 };
 
 struct Coordinates {
@@ -140,6 +155,8 @@ struct Topology {
   int piecewise = 0; double pwWidth = 1.0, pwK = 100.0;
   double edgeK = 0, edgeN = 2, edgeAlpha = 10, edgePhi = 0;
   int fixLambda = 0;
+  std::vector<int> noeIdx, harmIdx, borIdx, borBlock, anrIdx, anrBlock, dirIdx, dirBlock, dirN, resdIdx;
+  std::vector<double> noePar, harmPar, borPar, anrPar, dirPar, resdPar;
   bm_topology top{};
   const bm_topology *view();
 };

@@ -770,6 +770,30 @@ void launch_volume_restore(const Dev &d, hipStream_t st) {
   upd_restore_positions<<<(3 * d.n + 255) / 256, 256, 0, st>>>(d);
 }
 
+// holonomic_position (holonomic/holonomic.cu:600-758): every group from the
+// backup positions, v = dx/(dt/2)
+void launch_constrain_positions(const Dev &d, hipStream_t st) {
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  if (ng) upd_constrain<<<(4 * ng + 63) / 64, 64, 0, st>>>(d, 0);
+}
+
+// holonomic_velocity (holonomic/holonomic.cu:15-211)
+void launch_constrain_velocities(const Dev &d, hipStream_t st) {
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  if (ng) upd_vconstrain<<<(ng + 63) / 64, 64, 0, st>>>(d);
+}
+
+__global__ void upd_chain_rule(Dev d) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < d.nl) d.force[3 * d.n + b] = fix_d(theta_force(d, b));
+}
+
+// calc_thetaForce_from_lambdaForce (msld/msld.cu:937-998): dU/dtheta of every
+// block into the theta tail of the force array
+void launch_chain_rule(const Dev &d, hipStream_t st) {
+  upd_chain_rule<<<(d.nl + 63) / 64, 64, 0, st>>>(d);
+}
+
 void launch_theta_to_lambda(const Dev &d, hipStream_t st, bool init) {
   upd_theta_to_lambda<<<(d.nsites + 63) / 64, 64, 0, st>>>(d, init);
 }

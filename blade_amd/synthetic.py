@@ -114,6 +114,8 @@ class Spec:
     piecewise: tuple = None                           # (width, k): piecewise lambda constraint
     theta_edge: tuple = None                          # (k, N, alpha, phi) edge bias of the piecewise form
     fix_lambda: bool = False                          # msld fix flag: lambda held at the initial theta
+    restraints: dict = None                           # {"noe"|"harm"|"borest"|"anrest"|"direst"|"resd": [rows]},
+                                                      # 0-based atoms and blocks, angles in degrees
     # run options
     dt: float = 0.002 * PICOSECOND
     temperature: float = 298.0
@@ -1064,4 +1066,50 @@ def term_variety(spec):
         if t in sp.nb_par:
             e, r, e14, r14, f, _ = sp.nb_par[t]
             sp.nb_par[t] = (e, r, e14, r14, f, 1)
+    return sp
+
+
+def add_restraints(spec, seed=11):
+    """A copy of `spec` with a few terms of every restraint form
+    (restrain/restrain.cu) on its polymer atoms: NOE pairs inside, below and
+    beyond their flat band (one past the switch to the power-law tail) and
+    one to a fixed point; harmonic position restraints of powers 2 and 4;
+    bond, angle and dihedral restraints (periodic, harmonic and flat-bottom),
+    some scaled by a substituent block's lambda; two RESD terms."""
+    import copy
+    sp = copy.deepcopy(spec)
+    rng = np.random.default_rng(seed)
+    P = np.asarray(sp.pos, float)
+    L = np.array(sp.box, float)
+    poly = [i for i, a in enumerate(sp.atoms) if a[0] in ("POLY", "LIG")]
+    pick = lambda k: [int(x) for x in rng.choice(poly, size=k, replace=False)]
+    dist = lambda i, j: float(np.linalg.norm((P[i] - P[j]) - L * np.round((P[i] - P[j]) / L)))
+    blk = [int(b) for b in (sp.atom_block if sp.atom_block is not None else np.zeros(sp.n, int))]
+    scaled = next((b for b in blk if b), 0)
+    r = {"noe": [], "harm": [], "borest": [], "anrest": [], "direst": [], "resd": []}
+    for k in range(4):
+        i, j = pick(2)
+        d = dist(i, j)
+        bands = [(d - 1.0, d + 1.0), (d + 0.5, d + 2.0), (d - 2.0, d - 0.6), (d - 2.5, d - 1.5)]
+        rmin, rmax = bands[k]
+        rsw = 0.4 if k == 3 else 0.0
+        r["noe"].append((i, j, rmin, 5.0, rmax, 3.0, 0.8, rsw, 1.0, 0.0, 0.0, 0.0, False))
+    i = pick(1)[0]
+    r["noe"].append((i, i, 0.0, 0.0, 1.0, 2.0, 0.0, 0.0, 1.0, *(P[i] + rng.uniform(1.0, 2.0, 3)), True))
+    for n in (2.0, 4.0):
+        i = pick(1)[0]
+        r["harm"].append((i, 1.5, *(P[i] + rng.uniform(-0.8, 0.8, 3)), n))
+    for k in range(3):
+        i, j = pick(2)
+        r["borest"].append((i, j, 4.0, dist(i, j) + rng.uniform(-0.5, 0.5), scaled if k == 0 else 0))
+    for k in range(3):
+        a = sp.angles[int(rng.integers(len(sp.angles)))]
+        r["anrest"].append((a[0], a[1], a[2], 20.0, 100.0 + 10 * k, scaled if k == 1 else 0))
+    for k, (n, w) in enumerate(((2, 0.0), (0, 0.0), (0, 15.0), (3, 0.0))):
+        q = sp.dihes[int(rng.integers(len(sp.dihes)))]
+        r["direst"].append((q[0], q[1], q[2], q[3], 3.0, n, 40.0 + 30 * k, w, scaled if k == 2 else 0))
+    for k in range(2):
+        i1, i2, j1, j2 = pick(4)
+        r["resd"].append((i1, i2, j1, j2, 1.0, -1.0 if k else 0.5, 0.3, 6.0))
+    sp.restraints = r
     return sp

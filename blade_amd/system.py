@@ -317,6 +317,19 @@ def build(spec, gpus=(0,)):
         s.add_impr(a + 1, b + 1, c + 1, d + 1)
     for q in getattr(spec, "cmaps", []):
         s.L.blade_add_cmap(s.h, *[int(a) + 1 for a in q])
+    rs = getattr(spec, "restraints", None) or {}
+    for (i, j, rmin, kmin, rmax, kmax, rpeak, rsw, nsw, cx, cy, cz, pnoe) in rs.get("noe", []):
+        s.L.blade_add_noe(s.h, i + 1, j + 1, rmin, kmin, rmax, kmax, rpeak, rsw, nsw, cx, cy, cz, bool(pnoe))
+    for (i, k, x0, y0, z0, n) in rs.get("harm", []):
+        s.L.blade_add_harmonic(s.h, i + 1, k, x0, y0, z0, n)
+    for (i, j, kr, r0, b) in rs.get("borest", []):
+        s.L.blade_add_borest(s.h, i + 1, j + 1, kr, r0, b + 1)
+    for (i, j, k, kt, t0, b) in rs.get("anrest", []):
+        s.L.blade_add_anrest(s.h, i + 1, j + 1, k + 1, kt, t0, b + 1)
+    for (i, j, k, l, kp, n, p0, w, b) in rs.get("direst", []):
+        s.L.blade_add_direst(s.h, i + 1, j + 1, k + 1, l + 1, kp, n, p0, w, b + 1)
+    for (i1, i2, j1, j2, ci, cj, rd, kd) in rs.get("resd", []):
+        s.L.blade_add_resd(s.h, i1 + 1, i2 + 1, j1 + 1, j2 + 1, ci, cj, rd, kd)
     s.add_shake(spec.shake)
     s.init_coordinates(spec.n)
     for i in range(spec.n):

@@ -23,6 +23,8 @@
 #ifndef BLADE_H
 #define BLADE_H
 
+#include <stdbool.h>
+
 #include "blade_mi355x.h"
 
 #ifdef __cplusplus
@@ -71,6 +73,28 @@ void blade_add_dihe(System *system, int i, int j, int k, int l);
 void blade_add_impr(System *system, int i, int j, int k, int l);
 void blade_add_cmap(System *system, int i1, int j1, int k1, int l1, int i2, int j2, int k2, int l2);
 void blade_add_shake(System *system, int shakeHbond);
+/* virtual sites: recorded; a system with any is rejected (BM_EINVAL) when
+   its terms are derived, virtual sites are not supported */
+void blade_add_virt2(System *system, int v, int h1, int h2, double dist, double scale);
+void blade_add_virt3(System *system, int v, int h1, int h2, int h3, double dist, double theta, double phi);
+/* restraints (system/structure.h:124-130; forms in bm_topology): 1-based
+   atoms, lambdaBlock 1-based (1 = the environment, unscaled), angles in
+   degrees */
+void blade_add_noe(System *system, int i, int j, double rmin, double kmin, double rmax, double kmax,
+                   double rpeak, double rswitch, double nswitch, double c0x, double c0y, double c0z,
+                   bool is_pnoe);
+void blade_add_harmonic(System *system, int i, double k, double x0, double y0, double z0, double n);
+void blade_add_borest(System *system, int i, int j, double kr, double r0, int lambdaBlock);
+void blade_add_anrest(System *system, int i, int j, int k, double kt, double t0, int lambdaBlock);
+void blade_add_direst(System *system, int i, int j, int k, int l, double kphi, int nphi, double phi0,
+                      double width, int lambdaBlock);
+void blade_add_resd(System *system, int i1, int i2, int j1, int j2, double ci, double cj, double rdist,
+                    double kdist);
+/* system/structure.h:131-132: machine-learned potentials need LibTorch; the
+   system is marked unsupported (BM_ENOTSUP when prepared) */
+void blade_tani_internal_setup(System *system, const int is_tani, const char *ptname, const int ptnml,
+                               const int *mlatomidx, const int *mlZidx, const int *mlmaskid,
+                               const int mlnatoms);
 
 /* system/coordinates.h:57-63 */
 void blade_init_coordinates(System *system, int n);

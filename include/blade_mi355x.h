@@ -193,6 +193,40 @@ typedef struct bm_topology {
   /* msld fix flag (msld.cu:913-935): every lambda held at its initial
      theta value, no theta force from dU/dlambda */
   int32_t fix_lambda;
+
+  /* Restraints (restrain/restrain.cu:15-598; setters
+     system/structure.cxx:1093-1196), minimum-image geometry:
+     NOE, energy BM_ENOE: distance r of atoms i-j, or of atom i from the
+       point (c0x, c0y, c0z) when noe_idx[2k+1] < 0; below rmin
+       kmin/2 (r - rmin)^2, above rmax kmax/2 (r - rmax)^2, or with
+       rswitch > 0 beyond rmax + rswitch kmax (a + b d^-nswitch + rpeak d),
+       d = r - rmax, continuous with the harmonic piece;
+     harmonic position, energy BM_EHARMONIC: E = k |x_i - x0|^n;
+     bond / angle / dihedral restraints, energy BM_EMMFP, each scaled by
+       lambda of its block (0: unscaled): kr/2 (r - r0)^2, kt/2 (t - t0)^2,
+       and for the dihedral kphi (1 - cos(n (phi - phi0))) when n > 0, else
+       kphi/2 d^2 with d the distance of phi - phi0 (wrapped) from the
+       flat band [-width, width];
+     RESD, energy BM_ERESD: kdist/2 (ci r1 + cj r2 - rdist)^2, r1 = |i1 -
+       i2|, r2 = |j1 - j2|.  Angles in radians, AKMA units. */
+  int32_t n_noe;
+  const int32_t *noe_idx;          /* [2n]                                 */
+  const double *noe_par;           /* [10n] rmin kmin rmax kmax rpeak rswitch nswitch c0x c0y c0z */
+  int32_t n_harm;
+  const int32_t *harm_idx;         /* [n]                                  */
+  const double *harm_par;          /* [5n] k x0 y0 z0 n                    */
+  int32_t n_borest;
+  const int32_t *borest_idx, *borest_block;   /* [2n], [n]                 */
+  const double *borest_par;        /* [2n] kr r0                           */
+  int32_t n_anrest;
+  const int32_t *anrest_idx, *anrest_block;   /* [3n], [n]                 */
+  const double *anrest_par;        /* [2n] kt t0                           */
+  int32_t n_direst;
+  const int32_t *direst_idx, *direst_block, *direst_n;  /* [4n], [n], [n]  */
+  const double *direst_par;        /* [3n] kphi phi0 width                 */
+  int32_t n_resd;
+  const int32_t *resd_idx;         /* [4n] i1 i2 j1 j2                     */
+  const double *resd_par;          /* [4n] ci cj rdist kdist               */
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */
@@ -247,27 +281,38 @@ int bm_domdec_update(bm_system *sys, int rebuild);
 int bm_calc_force(bm_system *sys, long step, unsigned flags);
 
 /* Per-term entry points (nbdirect/nbdirect.h:7-8, nbrecip/nbrecip.h:7-8,
- * bonded/bonded.h:7-11, bonded/pair.h:7-8): each adds its term into the
- * fixed-point accumulators; the caller zeroes them (bm_reset_force). */
+ * bonded/bonded.h:7-11, bonded/pair.h:7-8, restrain/restrain.h:7-12): each
+ * adds its term's forces into the fixed-point force array, its dU/dlambda
+ * and (flags & BM_CALC_ENERGY) its energy; the caller zeroes the
+ * accumulators first (bm_reset_force).  Calling every term once gives the
+ * same force array as bm_calc_force, bit for bit. */
 int bm_reset_force(bm_system *sys, unsigned flags);
 int bm_getforce_nbdirect(bm_system *sys, unsigned flags);
-int bm_getforce_ewaldself(bm_system *sys, unsigned flags);
+int bm_getforce_ewaldself(bm_system *sys, unsigned flags);  /* nbrecip.cu:19-77 */
 int bm_getforce_ewald(bm_system *sys, unsigned flags);
-int bm_getforce_bonded(bm_system *sys, unsigned flags);   /* bond,urey,angle,dihe,impr */
+int bm_getforce_bonded(bm_system *sys, unsigned flags);   /* bond,urey,angle,dihe,impr,cmap */
 int bm_getforce_nb14(bm_system *sys, unsigned flags);
 int bm_getforce_nbex(bm_system *sys, unsigned flags);
-int bm_getforce_msldbias(bm_system *sys, unsigned flags); /* msld.cu:1000-1130 */
+/* fixed, variable and theta biases, charge and atom restraints (msld.cu:1000-1551) */
+int bm_getforce_msldbias(bm_system *sys, unsigned flags);
+/* NOE, harmonic, bond/angle/dihedral restraint and RESD terms (restrain.cu:15-598) */
+int bm_getforce_restrain(bm_system *sys, unsigned flags);
 
 /* State::update(int step,System*) (update/update.cu:233-305): one VRORV
  * step including holonomic constraints and theta->lambda. */
 int bm_update(bm_system *sys, long step, int energy_step);
 
-/* holonomic_position / holonomic_velocity (holonomic/holonomic.h:14-15) */
+/* holonomic_position / holonomic_velocity (holonomic/holonomic.h:14-15):
+ * every constraint group solved from the backup positions the update keeps
+ * (position form, which also sets v = dx/(dt/2)), or its velocities
+ * projected onto the constraints (velocity form). */
 int bm_holonomic_position(bm_system *sys);
 int bm_holonomic_velocity(bm_system *sys);
 
 /* Msld::calc_lambda_from_theta / calc_thetaForce_from_lambdaForce
- * (msld/msld.h:134-136) */
+ * (msld/msld.h:134-136); the chain rule writes dU/dtheta (with the theta
+ * biases of the last force evaluation) into the theta tail of the force
+ * array */
 int bm_msld_theta_to_lambda(bm_system *sys);
 int bm_msld_chain_rule(bm_system *sys);
 
@@ -331,6 +376,7 @@ double bm_uniform(uint64_t seed, long step, uint32_t k);
 int bm_volume_restore(bm_system *sys, const double *box);
 int bm_get_box(bm_system *sys, double *box);
 int bm_get_velocities(bm_system *sys, double *vel /*[3N]*/, double *theta_vel /*[nL]*/);
+int bm_set_velocities(bm_system *sys, const double *vel /*[3N] or NULL*/, const double *theta_vel /*[nL] or NULL*/);
 int bm_get_forces(bm_system *sys, double *force /*[3N]*/, double *theta_force /*[nL]*/);
 int bm_get_dudl(bm_system *sys, double *dudl /*[nL]*/);
 int bm_get_lambda(bm_system *sys, double *lambda /*[nL]*/);

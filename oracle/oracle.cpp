@@ -890,6 +890,130 @@ void cats(const bm_topology *t, const bm_runparams *rp, const double *pos, doubl
   }
 }
 
+// Restraints (restrain/restrain.cu:15-598).  NOE pairs or a fixed point
+// (BM_ENOE), harmonic positions k |x - x0|^n (BM_EHARMONIC), bond, angle and
+// dihedral restraints scaled by lambda of their block (BM_EMMFP) and RESD
+// two-distance restraints (BM_ERESD); dU/dx into force, the unscaled energy
+// of a scaled term into dudl.
+void restraints(const bm_topology *t, const bm_runparams *rp, const double *pos, const double *lambda,
+                double *force, double *dudl, double *energy) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  for (int k = 0; k < t->n_noe; k++) {
+    const double *p = t->noe_par + 10 * k;
+    const int i = t->noe_idx[2 * k], j = t->noe_idx[2 * k + 1];
+    const Vec3 xj = j >= 0 ? getp(pos, n, j) : v3(p[7], p[8], p[9]);
+    const Vec3 dr = minimg(getp(pos, n, i) - xj, L);
+    const double r = norm(dr);
+    double f = 0, e = 0;
+    if (r < p[0]) {
+      f = p[1] * (r - p[0]);
+      e = 0.5 * f * (r - p[0]);
+    } else if (r > p[2]) {
+      const double d = r - p[2], rpk = p[4], rsw = p[5], nsw = p[6];
+      if (rsw > 0 && d > rsw) {
+        const double b = (rpk - rsw) / nsw * std::pow(rsw, nsw + 1);
+        const double a = 0.5 * rsw * rsw - rpk * rsw - rsw * (rpk - rsw) / nsw;
+        f = p[3] * (rpk - b * std::pow(d, -nsw - 1));
+        e = p[3] * (a + b * std::pow(d, -nsw) + rpk * d);
+      } else {
+        f = p[3] * d;
+        e = 0.5 * f * d;
+      }
+    }
+    if (r > 0) {
+      addf(force, n, i, (f / r) * dr);
+      if (j >= 0) addf(force, n, j, (-f / r) * dr);
+    }
+    energy[BM_ENOE] += e;
+  }
+  for (int k = 0; k < t->n_harm; k++) {
+    const double *p = t->harm_par + 5 * k;
+    const int i = t->harm_idx[k];
+    const Vec3 dr = minimg(getp(pos, n, i) - v3(p[1], p[2], p[3]), L);
+    const double r2 = dot(dr, dr);
+    const double krn2 = r2 > 0 ? p[0] * std::pow(r2, 0.5 * p[4] - 1) : 0.0;
+    addf(force, n, i, (p[4] * krn2) * dr);
+    energy[BM_EHARMONIC] += krn2 * r2;
+  }
+  auto scale = [&](int b) { return b ? lambda[b] : 1.0; };
+  for (int k = 0; k < t->n_borest; k++) {
+    const int i = t->borest_idx[2 * k], j = t->borest_idx[2 * k + 1], b = t->borest_block[k];
+    const double kr = t->borest_par[2 * k], r0 = t->borest_par[2 * k + 1], l = scale(b);
+    const Vec3 dr = minimg(getp(pos, n, i) - getp(pos, n, j), L);
+    const double r = norm(dr), e = 0.5 * kr * (r - r0) * (r - r0), f = kr * (r - r0) * l;
+    if (b) dudl[b] += e;
+    addf(force, n, i, (f / r) * dr);
+    addf(force, n, j, (-f / r) * dr);
+    energy[BM_EMMFP] += e * l;
+  }
+  for (int k = 0; k < t->n_anrest; k++) {
+    const int *a = t->anrest_idx + 3 * k, b = t->anrest_block[k];
+    const double kt = t->anrest_par[2 * k], t0 = t->anrest_par[2 * k + 1], l = scale(b);
+    const Vec3 rij = minimg(getp(pos, n, a[0]) - getp(pos, n, a[1]), L);
+    const Vec3 rkj = minimg(getp(pos, n, a[2]) - getp(pos, n, a[1]), L);
+    const Vec3 c = cross(rij, rkj);
+    const double mc = norm(c), th = std::atan2(mc, dot(rij, rkj));
+    const double e = 0.5 * kt * (th - t0) * (th - t0), f = kt * (th - t0) * l;
+    if (b) dudl[b] += e;
+    const Vec3 fi = (f / (mc * dot(rij, rij))) * cross(rij, c);
+    const Vec3 fk = (-f / (mc * dot(rkj, rkj))) * cross(rkj, c);
+    addf(force, n, a[0], fi);
+    addf(force, n, a[2], fk);
+    addf(force, n, a[1], -1.0 * (fi + fk));
+    energy[BM_EMMFP] += e * l;
+  }
+  for (int k = 0; k < t->n_direst; k++) {
+    const int *a = t->direst_idx + 4 * k, b = t->direst_block[k], np = t->direst_n[k];
+    const double kp = t->direst_par[3 * k], p0 = t->direst_par[3 * k + 1], w = t->direst_par[3 * k + 2];
+    const double l = scale(b);
+    // phi from m = (j-i) x (k-j), nv = (k-j) x (l-k): atan2(b.(m x nv)/|..|, m.nv)
+    const Vec3 rij = minimg(getp(pos, n, a[1]) - getp(pos, n, a[0]), L);
+    const Vec3 rjk = minimg(getp(pos, n, a[2]) - getp(pos, n, a[1]), L);
+    const Vec3 rkl = minimg(getp(pos, n, a[3]) - getp(pos, n, a[2]), L);
+    const Vec3 m = cross(rij, rjk), nv = cross(rjk, rkl);
+    const double mm = norm(m), nn = norm(nv), bb = norm(rjk);
+    const double phi = std::atan2(dot((1 / bb) * rjk, cross((1 / mm) * m, (1 / nn) * nv)), dot(m, nv) / (mm * nn));
+    double fphi, e;
+    if (np > 0) {
+      double d = np * (phi - p0);
+      d -= 2 * M_PI * std::floor((d + M_PI) / (2 * M_PI));
+      fphi = kp * np * std::sin(d);
+      e = kp * (1 - std::cos(d));
+    } else {
+      double d = phi - p0;
+      d -= 2 * M_PI * std::floor((d + M_PI) / (2 * M_PI));
+      if (w > 0) d = std::fabs(d) <= w ? 0.0 : d - std::copysign(w, d);
+      fphi = kp * d;
+      e = 0.5 * kp * d * d;
+    }
+    if (b) dudl[b] += e;
+    fphi *= l;
+    // dphi/dx of this phi convention: -|b|/|m|^2 m on i, +|b|/|n|^2 n on l
+    const Vec3 fi = (-fphi * bb / (mm * mm)) * m;
+    const Vec3 fk0 = (-fphi * bb / (nn * nn)) * nv;
+    const double pp = dot(rij, rjk) / (bb * bb), qq = dot(rkl, rjk) / (bb * bb);
+    const Vec3 fj0 = (-pp) * fi + (-qq) * fk0;
+    addf(force, n, a[0], fi);
+    addf(force, n, a[3], -1.0 * fk0);
+    addf(force, n, a[2], fk0 - fj0);
+    addf(force, n, a[1], fj0 - fi);
+    energy[BM_EMMFP] += e * l;
+  }
+  for (int k = 0; k < t->n_resd; k++) {
+    const int *a = t->resd_idx + 4 * k;
+    const double *p = t->resd_par + 4 * k;
+    const Vec3 d1 = minimg(getp(pos, n, a[0]) - getp(pos, n, a[1]), L);
+    const Vec3 d2 = minimg(getp(pos, n, a[2]) - getp(pos, n, a[3]), L);
+    const double r1 = norm(d1), r2 = norm(d2), del = p[0] * r1 + p[1] * r2 - p[2], f = del * p[3];
+    addf(force, n, a[0], (f * p[0] / r1) * d1);
+    addf(force, n, a[1], (-f * p[0] / r1) * d1);
+    addf(force, n, a[2], (f * p[1] / r2) * d2);
+    addf(force, n, a[3], (-f * p[1] / r2) * d2);
+    energy[BM_ERESD] += 0.5 * f * del;
+  }
+}
+
 // ---------------------------------------------------------------------
 // Holonomic constraints in FP64.
 // ---------------------------------------------------------------------
@@ -1063,6 +1187,7 @@ long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
   if ((terms & ORC_MSLDBIAS) && t->n_cats > 0) cats(t, rp, pos, force, energy);
   if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1 && (t->cr_k1 != 0 || t->cr_k2 != 0 || t->cr_k3 != 0))
     chargerestraints(t, rp, lambda, dudl, energy);
+  if (terms & ORC_RESTRAIN) restraints(t, rp, pos, lambda, force, dudl, energy);
   double pot = 0;
   for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
   energy[BM_EPOTENTIAL] = pot;

@@ -27,7 +27,8 @@ extern "C" {
 #define ORC_NB14       (1u << 4)
 #define ORC_NBEX       (1u << 5)
 #define ORC_MSLDBIAS   (1u << 6)
-#define ORC_ALL        0x7Fu
+#define ORC_RESTRAIN   (1u << 7)
+#define ORC_ALL        0xFFu
 
 /* All selected force terms at positions `pos` and lambdas `lambda`
  * (Potential::calc_force, system/potential.cxx:1851-1941).  force[3N] and

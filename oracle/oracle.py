@@ -12,8 +12,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 PATH = os.path.join(HERE, "liboracle.so")
 
-NBDIRECT, EWALDSELF, PME, BONDED, NB14, NBEX, MSLDBIAS = (1 << k for k in range(7))
-ALL = 0x7F
+NBDIRECT, EWALDSELF, PME, BONDED, NB14, NBEX, MSLDBIAS, RESTRAIN = (1 << k for k in range(8))
+ALL = 0xFF
 
 _L = None
 _f64p = C.POINTER(C.c_double)

@@ -166,20 +166,130 @@ def test_autotune_picks_the_fastest_measured_candidate():
 
 
 def test_per_term_calls_sum_to_full_force():
-    sp = spec("msld_small")
+    """bm_reset_force, then every per-term entry point once (bonded, PME,
+    nonbonded, Ewald self, 1-4, exclusion corrections, MSLD biases,
+    restraints): the force array, dU/dlambda and every energy term equal one
+    bm_calc_force, bit for bit"""
+    from blade_amd import synthetic
+    sp = synthetic.add_restraints(_variant("msld_small", var_bias=VB_ALL, theta_inde=[0.0, 0.3]))
     s = system.build(sp)
     s.run_energy()
-    full = np.zeros(3 * sp.n + sp.nblocks, dtype=np.int64)
     L, dev = s.L, s.device()
-    assert L.bm_get_force_fixed(dev, full.ctypes.data_as(C.POINTER(C.c_longlong))) == 0
-    assert L.bm_domdec_update(dev, 0) == 0          # pack: zero the accumulators
-    for fn in (L.bm_getforce_bonded, L.bm_getforce_ewald, L.bm_getforce_nbdirect):
+    n3 = 3 * sp.n
+    i64 = C.POINTER(C.c_longlong)
+
+    def snapshot():
+        f = np.zeros(n3 + sp.nblocks, dtype=np.int64)
+        assert L.bm_get_force_fixed(dev, f.ctypes.data_as(i64)) == 0
+        return f[:n3], s.dudl(), s.energy()
+    full = snapshot()
+    assert L.bm_reset_force(dev, 1) == 0
+    for fn in (L.bm_getforce_bonded, L.bm_getforce_ewald, L.bm_getforce_nbdirect, L.bm_getforce_ewaldself,
+               L.bm_getforce_nb14, L.bm_getforce_nbex, L.bm_getforce_msldbias, L.bm_getforce_restrain):
         assert fn(dev, 1) == 0
     assert L.bm_sync(dev) == 0
-    parts = np.zeros_like(full)
-    assert L.bm_get_force_fixed(dev, parts.ctypes.data_as(C.POINTER(C.c_longlong))) == 0
-    n3 = 3 * sp.n
-    assert np.array_equal(parts[:n3], full[:n3])
+    parts = snapshot()
+    assert np.array_equal(parts[0], full[0])
+    assert np.array_equal(parts[1], full[1])
+    assert np.array_equal(parts[2][:20], full[2][:20])
+    assert all(full[2][k] != 0.0 for k in (12, 14, 15, 16, 17))
+
+
+def test_restraints():
+    """NOE, harmonic, bond/angle/dihedral restraint and RESD terms against
+    the oracle, each energy term to 1e-8 relative"""
+    from blade_amd import synthetic
+    sp = synthetic.add_restraints(spec("msld_small"))
+    s = _check(sp, "restraints")
+    top, rp = s.topology(), s.runparams()
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    eo = orc.forces(top, rp, s.positions(), lam, orc.RESTRAIN)[2]
+    eg = s.energy()
+    for k in (14, 15, 16, 17):
+        assert eo[k] != 0.0 and abs(eg[k] - eo[k]) <= 1e-8 * max(1.0, abs(eo[k])), (k, eg[k], eo[k])
+
+
+def test_term_by_term_step_equals_dynamics():
+    """the reference's per-step seam driven through the ABI -
+    bm_domdec_update (rebuild every 10), bm_calc_force (energies every 7),
+    bm_update - for 25 steps ends bit-identical to bm_dynamics over the
+    same steps, eager and graph-replayed"""
+    from tests.helpers import relaxed
+    sp = relaxed("msld_small")
+    out = []
+    for mode in ("terms", "eager", "graph"):
+        s = system.build(sp)
+        s.run_energy()
+        L, dev = s.L, s.device()
+        if mode == "terms":
+            for k in range(1, 26):
+                assert L.bm_domdec_update(dev, int(k % 10 == 0)) == 0
+                assert L.bm_calc_force(dev, k, int(k % 7 == 0)) == 0
+                assert L.bm_update(dev, k, int(k % 7 == 0)) == 0
+        else:
+            assert L.bm_dynamics(dev, 1, 25, 7, int(mode == "graph")) == 0
+        assert L.bm_sync(dev) == 0
+        out.append(system.state(s, sp.n, sp.nblocks))
+        s.close()
+    for q in range(4):
+        assert np.array_equal(out[0][q], out[1][q]), q
+        assert np.array_equal(out[0][q], out[2][q]), q
+
+
+def test_holonomic_and_chain_rule_entry_points():
+    """bm_holonomic_velocity projects velocities onto the constraints
+    (v_ij . r_ij = 0); bm_holonomic_position restores perturbed positions to
+    the constraint lengths from the backup; bm_msld_chain_rule writes the
+    oracle's dU/dtheta"""
+    from tests.helpers import relaxed
+    sp = relaxed("msld_small")
+    s = system.build(sp)
+    s.run_dynamics(10, freq_nrg=0)
+    L, dev = s.L, s.device()
+    n = sp.n
+    t = s.topology().contents
+    tri = t.array("tri_idx", t.n_tri, 3)
+    f64 = C.POINTER(C.c_double)
+    x, v, th, tv = system.state(s, n, sp.nblocks)
+    v2 = v + np.random.default_rng(1).normal(scale=0.01, size=v.shape)
+    vs = system.soa(v2)
+    # set the perturbed velocities through a checkpoint-free path: positions
+    # stay, velocities are read back after the projection
+    assert L.bm_set_velocities(dev, vs.ctypes.data_as(f64), None) == 0
+    assert L.bm_holonomic_velocity(dev) == 0
+    assert L.bm_sync(dev) == 0
+    x, v, th, tv = system.state(s, n, sp.nblocks)
+    Lb = np.array(sp.box)
+    worst = 0.0
+    for a, b, c in tri:
+        for i, j in ((a, b), (a, c), (b, c)):
+            r = x[i] - x[j]
+            r -= Lb * np.round(r / Lb)
+            worst = max(worst, abs(np.dot(v[i] - v[j], r)))
+    assert worst < 1e-10
+    xp = x + np.random.default_rng(2).normal(scale=0.005, size=x.shape)
+    assert L.bm_set_positions(dev, system.soa(xp).ctypes.data_as(f64), None) == 0
+    assert L.bm_holonomic_position(dev) == 0
+    assert L.bm_sync(dev) == 0
+    x = system.state(s, n, sp.nblocks)[0]
+    b0 = t.array("tri_b0", t.n_tri, 3)
+    dev_len = 0.0
+    for (a, b, c), (d01, d02, d12) in zip(tri, b0):
+        for (i, j, d) in ((a, b, d01), (a, c, d02), (b, c, d12)):
+            r = x[i] - x[j]
+            r -= Lb * np.round(r / Lb)
+            dev_len = max(dev_len, abs(np.linalg.norm(r) - d) / d)
+    assert dev_len < 1e-6
+    s.run_energy()
+    top, rp = s.topology(), s.runparams()
+    x, v, th, tv = system.state(s, n, sp.nblocks)
+    lam = orc.theta_to_lambda(top, rp, th[:sp.nblocks])
+    _, do, _, _ = orc.forces(top, rp, s.positions(), lam)
+    want = orc.chain_rule(top, rp, th[:sp.nblocks], lam, do)
+    assert L.bm_msld_chain_rule(dev) == 0
+    got = np.zeros(sp.nblocks)
+    assert L.bm_get_forces(dev, np.zeros(3 * n).ctypes.data_as(f64), got.ctypes.data_as(f64)) == 0
+    assert np.max(np.abs(got - want)) < 1e-3
 
 
 def test_reference_water216_deck():

@@ -717,3 +717,62 @@ def test_chain_rule_with_a_fixed_block():
         tm[i] -= 1e-6
         dl = (orc.theta_to_lambda(top, rp, tp) - orc.theta_to_lambda(top, rp, tm)) / 2e-6
         assert abs(got[i] - np.dot(dudl, dl)) < 1e-7
+
+
+E_NOE, E_HARM, E_MMFP, E_RESD = 14, 15, 16, 17
+
+
+def test_restraint_forces_are_energy_gradients_and_closed_forms():
+    """NOE, harmonic, bond/angle/dihedral restraint and RESD terms
+    (restrain/restrain.cu): forces are gradients of their energies and
+    dU/dlambda of the scaled ones their lambda derivatives; the NOE energy is
+    zero inside its band and continuous where the switched tail starts; the
+    harmonic energy is k |x - x0|^n"""
+    from blade_amd import synthetic
+    sp = synthetic.add_restraints(spec("msld_small"))
+    atoms = sorted({a for rows in sp.restraints.values() for row in rows
+                    for a in (row[:4] if len(row) >= 8 and not isinstance(row[-1], bool) else row[:2])
+                    if isinstance(a, (int, np.integer))})
+    e = _fd_spec(sp, orc.RESTRAIN, [E_NOE, E_HARM, E_MMFP, E_RESD], atoms, dl_terms=True)
+    assert all(e[k] != 0.0 for k in (E_NOE, E_HARM, E_MMFP, E_RESD))
+    s = system.build(sp)
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    x = system.soa(sp.pos)
+    n = sp.n
+    # harmonic: k |x - x0|^n per term
+    want = 0.0
+    for (i, k, x0, y0, z0, pw) in sp.restraints["harm"]:
+        d = np.array([x[i], x[n + i], x[2 * n + i]]) - [x0, y0, z0]
+        want += k * np.linalg.norm(d) ** pw
+    only = copy_restraints(sp, "harm")
+    s2 = system.build(only)
+    s2.prepare()
+    eh = orc.forces(s2.topology(), s2.runparams(), x, lam, orc.RESTRAIN)[2][E_HARM]
+    assert abs(eh - want) < 1e-10 * max(1.0, want)
+    # NOE: zero inside the band, continuous where the power-law tail starts
+    i, j = sp.restraints["noe"][0][:2]
+    L = np.array(sp.box)
+    d0 = np.array([x[i] - x[j], x[n + i] - x[n + j], x[2 * n + i] - x[2 * n + j]])
+    d0 -= L * np.round(d0 / L)
+    r0 = np.linalg.norm(d0)
+
+    def noe_energy(r, rmin, rmax, rsw):
+        one = copy_restraints(sp, None)
+        one.restraints = {"noe": [(i, j, rmin, 5.0, rmax, 3.0, 0.8, rsw, 1.0, 0.0, 0.0, 0.0, False)]}
+        b = system.build(one)
+        b.prepare()
+        return orc.forces(b.topology(), b.runparams(), x, lam, orc.RESTRAIN)[2][E_NOE]
+    assert noe_energy(r0, r0 - 0.5, r0 + 0.5, 0.0) == 0.0
+    rmax = r0 - 1.0
+    below = noe_energy(r0, 0.0, rmax, 1.0 - 1e-7)
+    above = noe_energy(r0, 0.0, rmax, 1.0 + 1e-7)
+    assert abs(below - above) < 1e-5 and abs(below - 0.5 * 3.0 * 1.0) < 1e-5
+
+
+def copy_restraints(sp, keep):
+    import copy
+    c = copy.copy(sp)
+    c.restraints = {keep: sp.restraints[keep]} if keep else {}
+    return c
