@@ -179,6 +179,7 @@ def lib():
         "blade_run_energy": (None, [vp]),
         # extensions
         "blade_last_error": (s, []),
+        "blade_system_error": (s, [vp]),
         "blade_run_dynamics": (i, [vp, i, i, i]),
         "blade_get_energy": (i, [vp, _f64p]),
         "blade_get_forces": (i, [vp, _f64p]),
@@ -230,6 +231,9 @@ def lib():
         "bm_getforce_bonded": (i, [vp, C.c_uint]),
         "bm_update": (i, [vp, C.c_long, i]),
         "bm_dynamics": (i, [vp, C.c_long, C.c_long, i, i]),
+        "bm_capture_graphs": (i, [vp]),
+        "bm_set_list_capacity": (i, [vp, C.c_longlong, i]),
+        "bm_get_list_capacity": (i, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
         "bm_profile_steps": (i, [vp, C.c_long, C.c_long, _f64p]),
         "bm_time_nbdirect": (i, [vp, C.c_int, _f64p]),
         "bm_autotune": (i, [i, C.POINTER(Topology), C.POINTER(RunParams), _f64p, _f64p, _f64p, _f64p, i,
@@ -240,6 +244,7 @@ def lib():
         "bm_calibration_copy": (i, [i, C.c_size_t, i, _f64p]),
         "bm_batch_create": (i, [i, C.POINTER(vp), C.POINTER(vp)]),
         "bm_batch_dynamics": (i, [vp, C.c_long, C.c_long, i]),
+        "bm_batch_capture_graphs": (i, [vp]),
         "bm_batch_sync": (i, [vp]),
         "bm_volume_move": (i, [vp, _f64p]),
         "bm_volume_restore": (i, [vp, _f64p]),

@@ -25,6 +25,7 @@ struct System {
   int device = 0;
   int verbose = 0;
   bool failed = false;
+  std::string error;               // this system's first error (empty when none)
   std::unique_ptr<Parameters> parameters;
   std::unique_ptr<Structure> structure;
   std::unique_ptr<Coordinates> coordinates;
@@ -56,7 +57,10 @@ extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(s
 
 static void api_fail(System *s, const std::string &m) {
   g_apiErr = m;
-  if (s) s->failed = true;
+  if (s) {
+    if (!s->failed) s->error = m;
+    s->failed = true;
+  }
   blade_log(("BLaDE-MI355X error: " + m + "\n").c_str());
 }
 
@@ -74,6 +78,7 @@ static void mark_unsupported(System *s, const char *what) {
 extern "C" {
 
 const char *blade_last_error(void) { return g_apiErr.c_str(); }
+const char *blade_system_error(System *s) { return s ? s->error.c_str() : "null system"; }
 
 System *blade_init_system(int ngpus, int *gpus) {
   System *s = new System();

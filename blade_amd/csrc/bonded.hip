@@ -131,9 +131,9 @@ __device__ void nb14_ef(const Dev &d, double4 p, double r, double &f, double &e)
       e += kqqe * (A * (rinv - 1 / rc) + B * (rc - r) + C * (roff2 * rc - r3) + D * (roff2 * roff2 * rc - r5));
     }
   } else if (d.elecmethod == BM_PME) {
-    double br = d.beta * r;
+    double br = d.betad * r;
     double erfcrinv = (erfc(br) + e14 - 1.0) * rinv;
-    f += kqq * (-erfcrinv - 1.1283791670955126 * d.beta * exp(-br * br)) * rinv;
+    f += kqq * (-erfcrinv - 1.1283791670955126 * d.betad * exp(-br * br)) * rinv;
     e += kqq * erfcrinv;
   } else {
     double roffinv = 1 / rc, roffinv2 = roffinv * roffinv, kqqe = kqq * e14;
@@ -564,20 +564,29 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     if (t < d.nnbex) {
       int2 ij = d.nbex_i[t];
       D3v dr = dispd(d, ij.x, ij.y);
-      double r = sqrt(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z), rinv = 1.0 / r, br = d.beta * r;
-      double kqq = kElectricD * d.nbex_p[t];
-      double erfv = erf(br);
-      double f = kqq * (erfv * rinv - 1.1283791670955126 * d.beta * exp(-br * br)) * rinv;
-      double et = -kqq * erfv * rinv;
+      const double r = sqrt(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
+      const double kqq = kElectricD * d.nbex_p[t], beta = d.betad, x = beta * r;
+      // fr = (dE/dr)/r; for beta r < 0.01 E and fr from their series about
+      // r = 0 (substituent atoms of one site may coincide: 0/0 otherwise)
+      double fr, et;
+      if (x < 0.01) {
+        const double x2 = x * x;
+        et = -kqq * beta * 1.1283791670955126 * (1 - x2 / 3 + x2 * x2 / 10 - x2 * x2 * x2 / 42);
+        fr = kqq * beta * beta * beta * 1.1283791670955126 * (2.0 / 3 - 2 * x2 / 5 + x2 * x2 / 7);
+      } else {
+        const double rinv = 1.0 / r, erfv = erf(x);
+        et = -kqq * erfv * rinv;
+        fr = kqq * (erfv * rinv - 1.1283791670955126 * beta * exp(-x * x)) * rinv * rinv;
+      }
       Sc s = scale2(d, d.nbex_sb[t]);
       const double ll = (s.b0 ? d.lambda[s.b0] : 1.0) * (s.b1 ? d.lambda[s.b1] : 1.0);
-      f *= ll;
+      fr *= ll;
       if (s.b0) {
         atomic_fix(&d.dudl[s.b0], fix_d((s.b1 ? d.lambda[s.b1] : 1.0) * et));
         if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(d.lambda[s.b0] * et));
       }
-      put_d(d, d.sl_nbex + 2 * t, (f / r), dr);
-      put_d(d, d.sl_nbex + 2 * t + 1, (-f / r), dr);
+      put_d(d, d.sl_nbex + 2 * t, fr, dr);
+      put_d(d, d.sl_nbex + 2 * t + 1, -fr, dr);
       e = et * ll;
     }
   } else if (bid < L.b_cmap) {
@@ -689,7 +698,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     if (d.elecmethod == BM_PME && t == 0 && (L.fam & kFamSelf)) {
       // Ewald self term per block (nbrecip.cu:19-50): -beta kE/sqrt(pi)
       // lambda_b^2 sum_{i in b} q_i^2, in FP64 from the block sums
-      const double pref = -(double)d.beta * kElectricD / 1.7724538509055159;
+      const double pref = -d.betad * kElectricD / 1.7724538509055159;
       double es = 0.0;
       for (int b = 0; b < d.nl; b++) {
         const double l = b ? d.lambda[b] : 1.0;

@@ -85,6 +85,11 @@ struct Dev {
   int4 *chunk;                   // [maxchunks] (I, first tile, tiles, 0)
   float *ichunk;                 // [maxchunks][3][32] i-force partials of a chunk (FP32, fixed
                                  // order); the force assembly converts them to fixed point
+  int *tlow;                     // [maxB] tiles (I,J) of row I with J < I (the row's first ones)
+  int *tile_mirror;              // [maxtiles] for J > I: index of the tile (J,I) in row J
+  float *jpart;                  // [maxtiles][3][32] j-force partials that tile (J,I), J < I,
+                                 // leaves for block I, stored at the index of tile (I,J)
+  int *jflag;                    // [maxtiles] 1: jpart of this lower tile written this step
   int *ecount, *estart;          // [maxB],[maxB+1] tiles with exclusions per row
   int *nexcltiles;               // [1]
   unsigned *emask;               // [32*2*n_excl]
@@ -212,6 +217,35 @@ __device__ __forceinline__ unsigned frac_u32(double x, double invL) {
 // first non-finite value wins the NaN flag (atom or DOF index)
 __device__ __forceinline__ void nan_check(const Dev &d, double v, int i) {
   if (!isfinite(v)) atomicCAS(d.nanflag, -1, i);
+}
+
+// Integrator updates with their rounding fixed by explicit fused
+// multiply-adds, so the fused step kernels and the one-term-at-a-time
+// kernels produce the same bits.
+// V: v - k halfdt ism^2 f (k = 2 for a full kick, 1 for a half kick)
+__device__ __forceinline__ double kick_v(double v, double k, double halfdt, double ism, double f) {
+  const double c = __dmul_rn(__dmul_rn(__dmul_rn(k, halfdt), ism), ism);
+  return __fma_rn(-c, f, v);
+}
+// O: a v + noise ism xi
+__device__ __forceinline__ double o_v(double a, double v, double noise, double ism, double xi) {
+  return __fma_rn(a, v, __dmul_rn(__dmul_rn(noise, ism), xi));
+}
+// R: x + halfdt v
+__device__ __forceinline__ double drift_x(double x, double halfdt, double v) { return __fma_rn(halfdt, v, x); }
+
+// A raised capacity overflow or a non-finite value halts the state updates
+// of the step that raised it and of every later step: the positions,
+// velocities, theta and the step counter stay as they were at the start of
+// that step, so the host can grow the buffers and run the step again.
+__device__ __forceinline__ bool halted(const Dev &d) { return *d.errflag != 0 || *d.nanflag >= 0; }
+// the same test taken once per workgroup (the flags may be raised while the
+// kernel runs): every thread of a workgroup returns together or none does
+__device__ __forceinline__ bool halted_wg(const Dev &d) {
+  __shared__ int h;
+  if (threadIdx.x == 0) h = halted(d) ? 1 : 0;
+  __syncthreads();
+  return h != 0;
 }
 
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -----------------

@@ -12,9 +12,11 @@
 //
 // The tile list is directed: row I holds every block J (J == I included)
 // whose rebuild box is within rcut + pad, in ascending J, so each pair of
-// blocks appears as (I,J) and (J,I).  Rows are cut into chunks of at most
-// d.tchunk tiles; the nonbonded kernel owns one chunk (one i-block) at a
-// time and never writes j-side forces.
+// blocks appears as (I,J) and (J,I).  The nonbonded kernel evaluates each
+// pair of blocks once, from the tile with J >= I: the part J >= I of every
+// row is cut into chunks of at most d.tchunk tiles, one i-block per chunk,
+// and the j-side forces of tile (I,J) go to the slot of its mirror (J,I),
+// one of the first tlow[J] tiles of row J.
 //
 // Column counts and bucket fills walk the slots of the previous rebuild,
 // which are ordered by column, so the lanes of a wave mostly share one
@@ -56,6 +58,7 @@ __device__ void block_scan_1024(const int *in, int *out, int n, int *total) {
 }
 
 __global__ void dd_zero_columns(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < d.ncol) {
     d.colcount[c] = 0;
@@ -66,6 +69,7 @@ __global__ void dd_zero_columns(Dev d) {
 // wrap every atom into the box, fractional coordinates, column counts
 // (one thread per slot of the previous rebuild; every atom has one slot)
 __global__ void __launch_bounds__(256) dd_wrap_count(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = s < 32 * d.maxblocks ? d.l2g[s] : -1;
   int col = 0;
@@ -73,6 +77,7 @@ __global__ void __launch_bounds__(256) dd_wrap_count(Dev d) {
     unsigned u[3];
     for (int k = 0; k < 3; k++) {
       double x = d.pos[k * d.n + i];
+      nan_check(d, x, i);
       double w = floor(__dmul_rn(x, d.invbox[k]));
       x = __dsub_rn(x, __dmul_rn(w, d.box[k]));
       d.pos[k * d.n + i] = x;
@@ -88,6 +93,7 @@ __global__ void __launch_bounds__(256) dd_wrap_count(Dev d) {
 
 // blocks per column = ceil(count/32); block starts by exclusive scan
 __global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ int nb[4096];
   int *tmp = d.bucket;  // not yet used this rebuild: scratch for ncol > 4096
   const bool big = d.ncol > 4096;
@@ -104,6 +110,7 @@ __global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) {
 // atoms into column buckets (bucket order is arbitrary; the column sort
 // fixes the final order)
 __global__ void __launch_bounds__(256) dd_scatter(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = s < 32 * d.maxblocks ? d.l2g[s] : -1;
   int c = 0;
@@ -120,6 +127,7 @@ __global__ void __launch_bounds__(256) dd_scatter(Dev d) {
 // them into 32-slot blocks, then the exact rebuild box of every block from
 // the fractions held in LDS (one half-wave per block).
 __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ unsigned long long key[kMaxColumnAtoms];
   __shared__ unsigned sx[kMaxColumnAtoms], sy[kMaxColumnAtoms];
   __shared__ short sorted[kMaxColumnAtoms];
@@ -224,6 +232,7 @@ __device__ __forceinline__ bool dd_tile_keep(const Dev &d, const unsigned *lo, c
 
 // one wave per i-block: number of blocks J within the list radius (all J)
 __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ unsigned s_lo[3 * kBoxLds], s_hi[3 * kBoxLds];
   const int nb = *d.nblocks;
   dd_stage_boxes(d, nb, s_lo, s_hi);
@@ -236,20 +245,24 @@ __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
     aLo[k] = d.bblo[3 * I + k];
     aHi[k] = d.bbhi[3 * I + k];
   }
-  int cnt = 0;
+  int cnt = 0, low = 0;
   for (int J0 = 0; J0 < nb; J0 += 64) {
     const int J = J0 + lane;
     const bool keep = J < nb && dd_tile_keep(d, s_lo, s_hi, I, aLo, aHi, J);
     cnt += __popcll(__ballot(keep));
+    low += __popcll(__ballot(keep && J < I));
   }
   if (lane == 0) {
     d.tcount[I] = cnt;
-    d.ccount[I] = (cnt + d.tchunk - 1) / d.tchunk;
+    d.tlow[I] = low;
+    // the nonbonded kernel takes the tiles J >= I of the row
+    d.ccount[I] = (cnt - low + d.tchunk - 1) / d.tchunk;
   }
 }
 
 // row starts of tiles and chunks; flags a list that outgrows its buffers
 __global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int nb = *d.nblocks;
   block_scan_1024(d.tcount, d.tstart, nb, d.ntiles);
   __syncthreads();
@@ -259,6 +272,7 @@ __global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
 }
 
 __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ unsigned s_lo[3 * kBoxLds], s_hi[3 * kBoxLds];
   const int nb = *d.nblocks;
   if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
@@ -286,7 +300,7 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
     }
     base += __popcll(m);
   }
-  const int t0 = d.tstart[I], cnt = d.tcount[I], c0 = d.cstart[I];
+  const int t0 = d.tstart[I] + d.tlow[I], cnt = d.tcount[I] - d.tlow[I], c0 = d.cstart[I];
   const int tc = d.tchunk;
   for (int k = lane; k * tc < cnt; k += 64) d.chunk[c0 + k] = make_int4(I, t0 + k * tc, min(tc, cnt - k * tc), 0);
 }
@@ -304,6 +318,7 @@ __device__ __forceinline__ int dd_find_tile(const Dev &d, int I, int J) {
 
 // exclusion pair -> the tiles (I,J) and (J,I) holding it
 __global__ void dd_excl_mark(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.nexcl) return;
   if (*d.ntiles > d.maxtiles) return;
@@ -317,24 +332,36 @@ __global__ void dd_excl_mark(Dev d) {
   if (tb >= 0) d.tile_flag[tb] = 1;
 }
 
-// one wave per row: tiles of the row holding exclusions
+// one wave per row: tiles of the row holding exclusions; for every tile
+// (I,J) with J > I, the index of its mirror (J,I) in row J (the list is
+// symmetric: the gap test does not depend on the order of the boxes)
 __global__ void __launch_bounds__(256) dd_excl_rows(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= *d.nblocks) return;
   int cnt = 0;
-  if (*d.ntiles <= d.maxtiles)
-    for (int t = d.tstart[I] + lane; t - lane < d.tstart[I + 1]; t += 64)
-      cnt += __popcll(__ballot(t < d.tstart[I + 1] && d.tile_flag[t]));
+  if (*d.ntiles <= d.maxtiles) {
+    const int t1 = d.tstart[I + 1];
+    for (int t = d.tstart[I] + lane; t - lane < t1; t += 64) {
+      cnt += __popcll(__ballot(t < t1 && d.tile_flag[t]));
+      if (t < t1) {
+        const int J = d.tile_j[t];
+        if (J > I) d.tile_mirror[t] = dd_find_tile(d, J, I);
+      }
+    }
+  }
   if (lane == 0) d.ecount[I] = cnt;
 }
 
 __global__ void __launch_bounds__(1024) dd_scan_excl(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   block_scan_1024(d.ecount, d.estart, *d.nblocks, d.nexcltiles);
 }
 
 // one wave per row: compact mask index of every flagged tile, -1 otherwise
 __global__ void __launch_bounds__(256) dd_excl_assign(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= *d.nblocks || *d.ntiles > d.maxtiles) return;
@@ -351,6 +378,7 @@ __global__ void __launch_bounds__(256) dd_excl_assign(Dev d) {
 }
 
 __global__ void dd_excl_clear(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const long long n = (long long)(*d.nexcltiles) * kBlockAtoms;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -359,6 +387,7 @@ __global__ void dd_excl_clear(Dev d) {
 
 // bit j of word i of tile (I,J): slots (I,i) and (J,j) are excluded
 __global__ void dd_excl_set(Dev d) {
+  if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.nexcl) return;
   if (*d.ntiles > d.maxtiles) return;
@@ -391,7 +420,11 @@ __device__ __forceinline__ void dd_pack_body(const Dev &d, int bid) {
   const int a = in ? d.l2g[s] : -1;
   unsigned u[3] = {0, 0, 0};
   if (a >= 0) {
-    for (int k = 0; k < 3; k++) u[k] = frac_u32(d.pos[k * d.n + a], d.invbox[k]);
+    for (int k = 0; k < 3; k++) {
+      const double x = d.pos[k * d.n + a];
+      nan_check(d, x, a);
+      u[k] = frac_u32(x, d.invbox[k]);
+    }
     d.spos[s] = make_uint4(u[0], u[1], u[2], __float_as_uint(d.q[a]));
     d.spar[s] = make_int2(d.type[a], d.sb[a]);
     d.slo[s] = (float)(d.qd[a] - (double)d.q[a]);

@@ -71,6 +71,8 @@ __device__ __forceinline__ int brick_of(int u, int K, int nb) {
 // depend on it.
 template <int N>
 __device__ __forceinline__ void pme_bin_body(const Dev &d, int bid, int nblk) {
+  // a halted step bins nothing (its bins would never be cleared)
+  if (halted_wg(d)) return;
   constexpr int W = pme_spline_rec_words(N);
   const int s = bid * blockDim.x + threadIdx.x;
   const int i = s < *d.nblocks * kBlockAtoms ? d.l2g[s] : -1;
@@ -747,12 +749,14 @@ __global__ void __launch_bounds__(256) fft_x_conv_multi(Multi m, int energy) {
 // point, the components selected by mask: the PME force 2 lambda q
 // grad(phi) and dU/dlambda = 2 q phi from the potential grid (N = spline
 // order; N = 0 compiles no PME part), the nonbonded chunk partials of the
-// atom's block, and the atom's bonded slots.  The lanes split the stencil,
-// the chunks and the slots; integer sums across them are exact.  One lane
+// atom's block and the j-side partials left in its row's lower tiles, and
+// the atom's bonded slots.  The lanes split the stencil, the chunks, the
+// tiles and the slots; integer sums across them are exact.  One lane
 // per force component and stream order make the read-modify-write of the
 // force array safe without atomics.
 template <int N, bool KICK>
 __device__ __forceinline__ void force_assemble_body(const Dev &d, int bid, int nblk, unsigned mask) {
+  if (KICK && halted_wg(d)) return;
   // eight lanes per list slot; every lane of a group takes the same branches
   const int g = bid * blockDim.x + threadIdx.x;
   const int s = g >> 3, sub = g & 7;
@@ -856,6 +860,16 @@ __device__ __forceinline__ void force_assemble_body(const Dev &d, int bid, int n
       fy += fix_f(src[kBlockAtoms + li]);
       fz += fix_f(src[2 * kBlockAtoms + li]);
     }
+    // j-side partials that the tiles (J, I), J < I, left in the slots of
+    // row I's first tlow[I] tiles
+    const int t0 = d.tstart[I], t1 = t0 + d.tlow[I];
+    for (int t = t0 + sub; t < t1; t += 8) {
+      if (!d.jflag[t]) continue;
+      const float *src = d.jpart + (long long)t * 3 * kBlockAtoms;
+      fx += fix_f(src[li]);
+      fy += fix_f(src[kBlockAtoms + li]);
+      fz += fix_f(src[2 * kBlockAtoms + li]);
+    }
   }
   if ((mask & kAsmBonded) && i >= 0) {
     const int k1 = d.bcsr_start[i + 1];
@@ -884,9 +898,9 @@ __device__ __forceinline__ void force_assemble_body(const Dev &d, int bid, int n
     const double f = (double)F * kFixInvD;
     d.posb[j] = x0;
     if (ism > 0) {
-      const double v = v0 - 2.0 * d.halfdt * ism * ism * f;
+      const double v = kick_v(v0, 2.0, d.halfdt, ism, f);
       d.vel[j] = v;
-      d.pos[j] = x0 + d.halfdt * v;
+      d.pos[j] = drift_x(x0, d.halfdt, v);
       nan_check(d, v, i);
     }
   }

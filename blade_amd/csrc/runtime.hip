@@ -103,6 +103,9 @@ struct bm_system {
   long long maxTiles = 0;
   bool alch = false, soft = false, bias = false;
   int nwgNb = 0;
+  long long maxTiles0 = 0;           // list capacities at creation (growth is bounded by 12x)
+  int pcap0 = 0;
+  int capacityGen = 0;               // advanced whenever a capacity changes (batches check it)
   int freqDomdec = 10;
   double kT = 0.0;                   // kB T of the thermostat (replica exchange)
   std::vector<void *> allocs;
@@ -121,6 +124,26 @@ struct bm_system {
     allocs.push_back(q);
     p = static_cast<T *>(q);
     return BM_OK;
+  }
+  // free p (an allocation of this system) and allocate count zeroed items
+  template <class T>
+  int realloc_dev(T *&p, size_t count) {
+    if (p) {
+      auto it = std::find(allocs.begin(), allocs.end(), static_cast<void *>(p));
+      if (it != allocs.end()) allocs.erase(it);
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    return alloc(p, count);
+  }
+  void drop_graphs() {
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        if (graph[a][b]) (void)hipGraphExecDestroy(graph[a][b]);
+        if (graphSrc[a][b]) (void)hipGraphDestroy(graphSrc[a][b]);
+        graph[a][b] = nullptr;
+        graphSrc[a][b] = nullptr;
+      }
   }
   template <class T>
   int upload(T *&p, const std::vector<T> &v) {
@@ -256,6 +279,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     s->maxTiles = std::min(MBl * MBl, MBl * (long long)std::ceil(perRow));
   }
   d.maxtiles = s->maxTiles;
+  s->maxTiles0 = s->maxTiles;
   d.tchunk = rp->tile_chunk > 0 ? rp->tile_chunk : kTileChunk;
   d.maxchunks = (int)(s->maxTiles / d.tchunk + s->maxBlocks + 1);
   const int MB = s->maxBlocks;
@@ -380,6 +404,10 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.nchunks, 1);
   ALLOC(d.chunk, d.maxchunks);
   ALLOC(d.ichunk, (size_t)d.maxchunks * 3 * kBlockAtoms);
+  ALLOC(d.tlow, MB);
+  ALLOC(d.tile_mirror, s->maxTiles + 1);
+  ALLOC(d.jpart, (size_t)(s->maxTiles + 1) * 3 * kBlockAtoms);
+  ALLOC(d.jflag, s->maxTiles + 1);
   ALLOC(d.ecount, MB);
   ALLOC(d.estart, MB + 1);
   ALLOC(d.nexcltiles, 1);
@@ -408,6 +436,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       nbr *= d.pnb[k];
     }
     d.pcap = 64 + 4 * ((n + nbr - 1) / nbr);
+    s->pcap0 = d.pcap;
     ALLOC(d.pbcount, nbr);
     ALLOC(d.pbrec, (size_t)nbr * d.pcap * pme_spline_rec_words(d.order));
     std::vector<float2> tw;
@@ -808,7 +837,7 @@ int bm_sync(bm_system *s) {
   // every condition that was raised, the first non-finite value first (a
   // blow-up also scatters atoms into one column)
   std::string msg;
-  if (nan >= 0) msg += "non-finite velocity (first at atom or DOF " + std::to_string(nan) + "); ";
+  if (nan >= 0) msg += "non-finite position or velocity (first at atom or DOF " + std::to_string(nan) + "); ";
   if (flag & 4) msg += "the tile list exceeded its capacity; ";
   if (flag & 2) msg += "a PME spread brick exceeded its atom capacity; ";
   if (flag & 1) msg += "a domdec column exceeded its capacity; ";
@@ -928,9 +957,126 @@ int bm_msld_chain_rule(bm_system *s) {
   return BM_OK;
 }
 
-int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_graph) {
-  if (!s || nsteps < 0) return BM_EINVAL;
-  if (freq_nrg <= 0) freq_nrg = 1 << 30;
+// ---------------------------------------------------------------------
+// list capacities: the reference grows its partner list x1.5 and retries
+// (domdec/cull.cu:337-375, at most 12x the first size, domdec.cu:129-130);
+// here an overflow halts the state updates on the device and bm_dynamics
+// grows the buffer that overflowed and runs the steps again from the one
+// that raised it
+// ---------------------------------------------------------------------
+
+// tile list arrays for maxTiles directed tiles (their contents are lost:
+// the next step must rebuild the list)
+static int set_tile_capacity(bm_system *s, long long maxTiles) {
+  blade::Dev &d = s->d;
+  int rc = BM_OK;
+  s->maxTiles = maxTiles;
+  d.maxtiles = maxTiles;
+  d.maxchunks = (int)(maxTiles / d.tchunk + s->maxBlocks + 1);
+  const size_t T = (size_t)maxTiles + 1;
+  if ((rc = s->realloc_dev(d.tile_i, T)) || (rc = s->realloc_dev(d.tile_j, T)) ||
+      (rc = s->realloc_dev(d.tile_ex, T)) || (rc = s->realloc_dev(d.tile_flag, T)) ||
+      (rc = s->realloc_dev(d.tile_mirror, T)) || (rc = s->realloc_dev(d.jflag, T)) ||
+      (rc = s->realloc_dev(d.jpart, T * 3 * blade::kBlockAtoms)) ||
+      (rc = s->realloc_dev(d.chunk, (size_t)d.maxchunks)) ||
+      (rc = s->realloc_dev(d.ichunk, (size_t)d.maxchunks * 3 * blade::kBlockAtoms)))
+    return rc;
+  s->capacityGen++;
+  s->drop_graphs();
+  return BM_OK;
+}
+
+// PME brick bins of pcap spline records each (emptied)
+static int set_brick_capacity(bm_system *s, int pcap) {
+  blade::Dev &d = s->d;
+  if (d.elecmethod != BM_PME) return BM_OK;
+  const size_t nbr = (size_t)d.pnb[0] * d.pnb[1] * d.pnb[2];
+  d.pcap = pcap;
+  if (int rc = s->realloc_dev(d.pbrec, nbr * (size_t)pcap * blade::pme_spline_rec_words(d.order))) return rc;
+  HIPCHK(hipMemset(d.pbcount, 0, nbr * sizeof(int)));
+  s->capacityGen++;
+  s->drop_graphs();
+  return BM_OK;
+}
+
+int bm_set_list_capacity(bm_system *s, long long max_tiles, int brick_atoms) {
+  if (!s) return BM_EINVAL;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (max_tiles > 0) {
+    const long long MB = s->maxBlocks;
+    if (int rc = set_tile_capacity(s, std::min(max_tiles, MB * MB))) return rc;
+  }
+  if (brick_atoms > 0)
+    if (int rc = set_brick_capacity(s, brick_atoms)) return rc;
+  return BM_OK;
+}
+
+int bm_get_list_capacity(bm_system *s, long long *max_tiles, int *brick_atoms) {
+  if (!s) return BM_EINVAL;
+  if (max_tiles) *max_tiles = s->maxTiles;
+  if (brick_atoms) *brick_atoms = s->d.elecmethod == BM_PME ? s->d.pcap : 0;
+  return BM_OK;
+}
+
+// grow what the raised flag names (x1.5), clear the flag; BM_EOVERFLOW when
+// the flag cannot be cured by growth or growth passed 12x the first size
+static int regrow(bm_system *s, int flag) {
+  if (flag & 1) return BM_EOVERFLOW;   // a column beyond the sort's LDS capacity
+  if (flag & 4) {
+    const long long MB = s->maxBlocks;
+    const long long mt = std::min(MB * MB, (long long)std::ceil(1.5 * (double)s->maxTiles));
+    if (mt <= s->maxTiles || mt > 12 * s->maxTiles0) return BM_EOVERFLOW;
+    if (int rc = set_tile_capacity(s, mt)) return rc;
+  }
+  if (flag & 2) {
+    const int pc = (int)std::ceil(1.5 * (double)s->d.pcap);
+    if ((long long)pc > 12LL * std::max(1, s->pcap0)) return BM_EOVERFLOW;
+    if (int rc = set_brick_capacity(s, pc)) return rc;
+  }
+  // a halted step left its brick counts in place (the assembly that clears
+  // them returned early)
+  if (s->d.elecmethod == BM_PME)
+    HIPCHK(hipMemset(s->d.pbcount, 0, (size_t)s->d.pnb[0] * s->d.pnb[1] * s->d.pnb[2] * sizeof(int)));
+  HIPCHK(hipMemset(s->d.errflag, 0, sizeof(int)));
+  return BM_OK;
+}
+
+// capture and instantiate the step graph of one (rebuild, energy) variant
+static int capture_step_graph(bm_system *s, int rb, int en) {
+  if (s->graph[rb][en]) return BM_OK;
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  s->step_launches(rb, en);
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(s->st, &g));
+  s->graphSrc[rb][en] = g;
+  HIPCHK(hipGraphInstantiate(&s->graph[rb][en], g, nullptr, nullptr, 0));
+  return BM_OK;
+}
+
+int bm_capture_graphs(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  for (int rb = 0; rb < 2; rb++)
+    for (int en = 0; en < 2; en++)
+      if (int rc = capture_step_graph(s, rb, en)) return rc;
+  return BM_OK;
+}
+
+// True under rocprofv3 hardware-counter collection (it exports
+// ROCPROF_COUNTER_COLLECTION).  Steps then run as eager launches instead of
+// graph launches: with counters on, hipGraphLaunch of these step graphs has
+// ended in a host segmentation fault inside the counter tool's own copy
+// routine, while the same steps launched one by one collect normally.
+static bool counter_collection() {
+  static const bool on = [] {
+    const char *v = std::getenv("ROCPROF_COUNTER_COLLECTION");
+    return v && *v && std::strcmp(v, "0") != 0 && std::strcmp(v, "False") != 0 && std::strcmp(v, "false") != 0;
+  }();
+  return on;
+}
+
+// the launches of steps [step0, step0 + nsteps)
+static int dynamics_launches(bm_system *s, long step0, long nsteps, int freq_nrg, int use_graph) {
   blade::launch_set_step(s->d, step0, s->st);
   for (long k = 0; k < nsteps; k++) {
     const long step = step0 + k;
@@ -940,17 +1086,33 @@ int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_gra
       HIPCHK(hipGetLastError());
       continue;
     }
-    if (!s->graph[rb][en]) {
-      HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-      s->step_launches(rb, en);
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamEndCapture(s->st, &g));
-      s->graphSrc[rb][en] = g;
-      HIPCHK(hipGraphInstantiate(&s->graph[rb][en], g, nullptr, nullptr, 0));
-    }
+    if (int rc = capture_step_graph(s, rb, en)) return rc;
     HIPCHK(hipGraphLaunch(s->graph[rb][en], s->st));
   }
   return BM_OK;
+}
+
+int bm_dynamics(bm_system *s, long step0, long nsteps, int freq_nrg, int use_graph) {
+  if (!s || nsteps < 0) return BM_EINVAL;
+  if (freq_nrg <= 0) freq_nrg = 1 << 30;
+  if (counter_collection()) use_graph = 0;
+  const long end = step0 + nsteps;
+  for (;;) {
+    if (int rc = dynamics_launches(s, step0, end - step0, freq_nrg, use_graph)) return rc;
+    // one synchronisation per call: a capacity overflow halted the state
+    // at the step that raised it; grow the buffer and run from that step
+    HIPCHK(hipStreamSynchronize(s->st));
+    int flag = 0, nan = -1;
+    HIPCHK(hipMemcpy(&flag, s->d.errflag, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&nan, s->d.nanflag, sizeof(int), hipMemcpyDeviceToHost));
+    if (!flag || nan >= 0) break;
+    long long at = 0;
+    HIPCHK(hipMemcpy(&at, s->d.step, sizeof(long long), hipMemcpyDeviceToHost));
+    if (at < step0 || at >= end) break;
+    if (regrow(s, flag)) break;
+    step0 = (long)at;
+  }
+  return bm_sync(s);
 }
 
 // ---------------------------------------------------------------------
@@ -984,6 +1146,7 @@ struct bm_batch {
   MultiPlan pack, normals, bonded, tail;
   std::vector<MultiPlan> bin, spread, fftp, fftx, nb, asmb;
   std::vector<int> fftSingle;      // PME members off the FP32 plane FFT path (other sizes, MFMA FFT)
+  std::vector<int> gens;           // members' capacity generations the launch plans were sized for
   ~bm_batch() {
     if (st) (void)hipStreamSynchronize(st);
     if (st2) (void)hipStreamSynchronize(st2);
@@ -1157,6 +1320,7 @@ int bm_batch_create(int n, bm_system *const *systems, bm_batch **out) {
   bm_batch *b = new bm_batch();
   b->device = systems[0]->device;
   b->sys.assign(systems, systems + n);
+  for (int k = 0; k < n; k++) b->gens.push_back(systems[k]->capacityGen);
   b->join.assign(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) == hipSuccess &&
@@ -1176,8 +1340,55 @@ int bm_batch_create(int n, bm_system *const *systems, bm_batch **out) {
   return BM_OK;
 }
 
+// the launches of one batch step on the batch stream
+static int batch_step_launches(bm_batch *b, int rb, int en) {
+  if (!en) {
+    // non-energy steps: one launch per kernel type over all members
+    b->fused_step(rb);
+  } else {
+    // energy steps: every member's own step as a parallel branch
+    HIPCHK(hipEventRecord(b->fork, b->st));
+    for (size_t m = 0; m < b->sys.size(); m++) {
+      bm_system *s = b->sys[m];
+      HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
+      s->step_launches(rb, en);
+      HIPCHK(hipEventRecord(b->join[m], s->st));
+      HIPCHK(hipStreamWaitEvent(b->st, b->join[m], 0));
+    }
+  }
+  return BM_OK;
+}
+
+// capture and instantiate the batch step graph of one (rebuild, energy)
+// variant
+static int capture_batch_graph(bm_batch *b, int rb, int en) {
+  if (b->graph[rb][en]) return BM_OK;
+  HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  if (int rc = batch_step_launches(b, rb, en)) return rc;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(b->st, &g));
+  b->graphSrc[rb][en] = g;
+  HIPCHK(hipGraphInstantiate(&b->graph[rb][en], g, nullptr, nullptr, 0));
+  return BM_OK;
+}
+
+int bm_batch_capture_graphs(bm_batch *b) {
+  if (!b) return BM_EINVAL;
+  HIPCHK(hipSetDevice(b->device));
+  if (int rc = b->upload_devs()) return rc;
+  for (int rb = 0; rb < 2; rb++)
+    for (int en = 0; en < 2; en++)
+      if (int rc = capture_batch_graph(b, rb, en)) return rc;
+  return BM_OK;
+}
+
 int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
   if (!b || nsteps < 0) return BM_EINVAL;
+  // the fused launches were sized for the members' list capacities: a
+  // member that grew since (bm_dynamics on it alone) needs a new batch
+  for (size_t k = 0; k < b->sys.size(); k++)
+    if (b->sys[k]->capacityGen != b->gens[k])
+      return set_err(BM_EINVAL, "a batch member's list capacity changed after bm_batch_create");
   if (freq_nrg <= 0) freq_nrg = 1 << 30;
   HIPCHK(hipSetDevice(b->device));
   if (int rc = b->upload_devs()) return rc;
@@ -1189,30 +1400,16 @@ int bm_batch_dynamics(bm_batch *b, long step0, long nsteps, int freq_nrg) {
   }
   for (bm_system *s : b->sys) blade::launch_set_step(s->d, step0, b->st);
   const int freqDomdec = b->sys[0]->freqDomdec;
+  const bool eager = counter_collection();
   for (long k = 0; k < nsteps; k++) {
     const long step = step0 + k;
     const int rb = (step % freqDomdec) == 0, en = (step % freq_nrg) == 0;
-    if (!b->graph[rb][en]) {
-      HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-      if (!en) {
-        // non-energy steps: one launch per kernel type over all members
-        b->fused_step(rb);
-      } else {
-        // energy steps: every member's own step as a parallel branch
-        HIPCHK(hipEventRecord(b->fork, b->st));
-        for (size_t m = 0; m < b->sys.size(); m++) {
-          bm_system *s = b->sys[m];
-          HIPCHK(hipStreamWaitEvent(s->st, b->fork, 0));
-          s->step_launches(rb, en);
-          HIPCHK(hipEventRecord(b->join[m], s->st));
-          HIPCHK(hipStreamWaitEvent(b->st, b->join[m], 0));
-        }
-      }
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamEndCapture(b->st, &g));
-      b->graphSrc[rb][en] = g;
-      HIPCHK(hipGraphInstantiate(&b->graph[rb][en], g, nullptr, nullptr, 0));
+    if (eager) {
+      if (int rc = batch_step_launches(b, rb, en)) return rc;
+      HIPCHK(hipGetLastError());
+      continue;
     }
+    if (int rc = capture_batch_graph(b, rb, en)) return rc;
     HIPCHK(hipGraphLaunch(b->graph[rb][en], b->st));
   }
   // members' later work (getters, eager steps) follows the batch
@@ -1250,18 +1447,12 @@ static int swap_dev(bm_system *a, bm_system *b, double *pa, double *pb, double *
   return BM_OK;
 }
 
-int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t *replica, int32_t *accepted,
-                      double *dW) {
-  if (!b || (parity != 0 && parity != 1)) return set_err(BM_EINVAL, "exchange arguments");
+// Replica exchange over the members of a batch, with `tmp` (at least
+// 3N + nL doubles of device memory) as the staging buffer of the swaps.
+static int batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t *replica,
+                          int32_t *accepted, double *dW, double *tmp) {
   const int n = (int)b->sys.size();
-  for (int k = 1; k < n; k++)
-    if (b->sys[k]->d.n != b->sys[0]->d.n || b->sys[k]->d.nl != b->sys[0]->d.nl)
-      return set_err(BM_EINVAL, "replica exchange needs members of equal atom and block counts");
-  if (int rc = bm_batch_sync(b)) return rc;
-  HIPCHK(hipSetDevice(b->device));
   const size_t N = 3 * (size_t)b->sys[0]->d.n + b->sys[0]->d.nl;
-  double *tmp = nullptr;
-  HIPCHK(hipMalloc(&tmp, N * sizeof(double)));
   std::vector<double> e0(n, 0.0), e1(n, 0.0);
   int rc = BM_OK;
   for (int k = 0; k < n && !rc; k++) rc = member_energy(b->sys[k], e0[k]);
@@ -1295,6 +1486,33 @@ int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t
     rc = bm_domdec_update(b->sys[k], 1);
     if (!rc) rc = bm_sync(b->sys[k]);
   }
+  return rc;
+}
+
+// members of one exchange must have equal atom and block counts and equal
+// boxes (positions are exchanged, boxes are not)
+static int exchange_check(bm_batch *b, int parity) {
+  if (!b || (parity != 0 && parity != 1)) return set_err(BM_EINVAL, "exchange arguments");
+  const int n = (int)b->sys.size();
+  for (int k = 1; k < n; k++) {
+    const blade::Dev &d0 = b->sys[0]->d, &dk = b->sys[k]->d;
+    if (dk.n != d0.n || dk.nl != d0.nl)
+      return set_err(BM_EINVAL, "replica exchange needs members of equal atom and block counts");
+    if (dk.box[0] != d0.box[0] || dk.box[1] != d0.box[1] || dk.box[2] != d0.box[2])
+      return set_err(BM_EINVAL, "replica exchange needs members with equal boxes");
+  }
+  return BM_OK;
+}
+
+int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t *replica, int32_t *accepted,
+                      double *dW) {
+  if (int rc = exchange_check(b, parity)) return rc;
+  if (int rc = bm_batch_sync(b)) return rc;
+  HIPCHK(hipSetDevice(b->device));
+  const size_t N = 3 * (size_t)b->sys[0]->d.n + b->sys[0]->d.nl;
+  double *tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, N * sizeof(double)));
+  const int rc = batch_exchange(b, step, parity, seed, replica, accepted, dW, tmp);
   (void)hipFree(tmp);
   return rc;
 }
@@ -1449,13 +1667,7 @@ static void set_box(bm_system *s, const double *box) {
   const double V = box[0] * box[1] * box[2];
   d.pmefacd = 0.5 * blade::kElectricD / M_PI / V;
   d.pmefac = (float)d.pmefacd;
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) {
-      if (s->graph[a][b]) (void)hipGraphExecDestroy(s->graph[a][b]);
-      if (s->graphSrc[a][b]) (void)hipGraphDestroy(s->graphSrc[a][b]);
-      s->graph[a][b] = nullptr;
-      s->graphSrc[a][b] = nullptr;
-    }
+  s->drop_graphs();
 }
 
 int bm_volume_move(bm_system *s, const double *scale) {

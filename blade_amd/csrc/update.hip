@@ -14,6 +14,13 @@
 // energy, V+R (update.cu:257-279).
 #include "kernels.h"
 
+// No contraction of separate multiplies and adds into fused ones in this
+// file: every FP64 expression rounds as written, so the constraint solves
+// and updates give the same bits in every kernel that inlines them (the
+// fused step tail and the one-term-at-a-time kernels), whatever the
+// surrounding code.  The fused forms that are wanted are explicit.
+#pragma clang fp contract(off)
+
 namespace blade {
 
 struct D3 { double x, y, z; };
@@ -98,6 +105,7 @@ __device__ double theta_force(const Dev &d, int b) {
 // mode 0: v -= dt f/m, backup, x += dt/2 v; mode 1: v -= dt/2 f/m;
 // mode 2: v -= dt/2 f/m, backup, x += dt/2 v
 __global__ void upd_kick(Dev d, int mode) {
+  if (halted_wg(d)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int N1 = 3 * d.n, N = N1 + d.nl;
   if (i >= N) return;
@@ -112,10 +120,10 @@ __global__ void upd_kick(Dev d, int mode) {
   double x = d.pos[i];
   if (mode != 1) d.posb[i] = x;
   if (ism > 0) {
-    double v = d.vel[i] - (mode == 0 ? 2.0 : 1.0) * d.halfdt * ism * ism * f;
+    double v = kick_v(d.vel[i], mode == 0 ? 2.0 : 1.0, d.halfdt, ism, f);
     d.vel[i] = v;
     if (mode != 1) {
-      d.pos[i] = x + d.halfdt * v;
+      d.pos[i] = drift_x(x, d.halfdt, v);
       nan_check(d, v, i < N1 ? i % d.n : i);
     }
   }
@@ -133,14 +141,15 @@ __device__ __forceinline__ void o_drift_dof(const Dev &d, int i) {
       fr = d.lfric[i - N1];
       no = d.lnoise[i - N1];
     }
-    double v = fr * d.vel[i] + no * ism * d.xi[i];
+    double v = o_v(fr, d.vel[i], no, ism, d.xi[i]);
     d.vel[i] = v;
-    d.pos[i] = x + d.halfdt * v;
+    d.pos[i] = drift_x(x, d.halfdt, v);
     nan_check(d, v, i < N1 ? i % d.n : i);
   }
 }
 
 __global__ void upd_o_drift(Dev d) {
+  if (halted_wg(d)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 3 * d.n + d.nl) o_drift_dof(d, i);
 }
@@ -363,9 +372,9 @@ __device__ void group_lane(const Dev &d, const int (&a)[NA], const double *b0, i
       const int i = c * d.n + am;
       const double ism = d.ism[i];
       if (ism > 0) {
-        const double v = d.fric * vv[c] + d.noise * ism * d.xi[i];
+        const double v = o_v(d.fric, vv[c], d.noise, ism, d.xi[i]);
         vv[c] = v;
-        x2[c] = x2[c] + d.halfdt * v;
+        x2[c] = drift_x(x2[c], d.halfdt, v);
         nan_check(d, v, am);
       }
     }
@@ -509,9 +518,9 @@ __device__ void theta_kick(const Dev &d, int b) {
   const double x = d.pos[i];
   d.posb[i] = x;
   if (ism > 0) {
-    const double v = d.vel[i] - 2.0 * d.halfdt * ism * ism * f;
+    const double v = kick_v(d.vel[i], 2.0, d.halfdt, ism, f);
     d.vel[i] = v;
-    d.pos[i] = x + d.halfdt * v;
+    d.pos[i] = drift_x(x, d.halfdt, v);
     nan_check(d, v, i);
   }
 }
@@ -519,6 +528,7 @@ __device__ void theta_kick(const Dev &d, int b) {
 // groups (four lanes each), then (last call of a step) sites and the step
 // counter
 __global__ void __launch_bounds__(64) upd_constrain(Dev d, int finish) {
+  if (halted_wg(d)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
   if (t < 4 * ng) constrain_group<false>(d, t >> 2, t & 3);
@@ -566,6 +576,7 @@ __device__ void vproject(const Dev &d, const int (&a)[NA]) {
 }
 
 __global__ void __launch_bounds__(64) upd_vconstrain(Dev d) {
+  if (halted_wg(d)) return;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < d.ntri) {
     const int4 ix = d.tri_i[t];
@@ -638,6 +649,7 @@ void launch_update(const Dev &d, bool energy_step, hipStream_t st) {
 // and, in the last workgroup, the theta DOFs (kick, O, half drift), then
 // theta -> lambda and the step counter.
 __device__ __forceinline__ void upd_tail_body(const Dev &d, int bid, int nblk) {
+  if (halted_wg(d)) return;
   if (bid == nblk - 1) {
     if (d.nl > 1) {
       for (int b = threadIdx.x; b < d.nl; b += blockDim.x) {

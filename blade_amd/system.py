@@ -200,7 +200,7 @@ class BladeSystem:
 
     def run_energy(self):
         self.L.blade_run_energy(self.h)
-        err = self.L.blade_last_error().decode()
+        err = self.L.blade_system_error(self.h).decode()
         if err:
             raise RuntimeError(err)
 
@@ -438,6 +438,11 @@ class Batch:
 
     def dynamics(self, step0, nsteps, freq_nrg=0):
         check(self.L.bm_batch_dynamics(self.h, step0, nsteps, freq_nrg), "bm_batch_dynamics")
+
+    def capture_graphs(self):
+        """capture every batch step graph variant now (none is captured
+        by a later dynamics call)"""
+        check(self.L.bm_batch_capture_graphs(self.h), "bm_batch_capture_graphs")
 
     def sync(self):
         check(self.L.bm_batch_sync(self.h), "bm_batch_sync")

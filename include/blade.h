@@ -157,6 +157,9 @@ void blade_log(const char *message);
 /* ---- extensions ----------------------------------------------------- */
 /* last error message of this thread ("" when none) */
 const char *blade_last_error(void);
+/* the first error of this system (empty when it has none); no counterpart
+   in the reference, whose errors end the process */
+const char *blade_system_error(System *system);
 /* Run::dynamics: nsteps steps from the current step (run/run.cu:545-570);
  * energies on steps with step % freqNRG == 0; use_graph replays hipGraphs. */
 int blade_run_dynamics(System *system, int nsteps, int freqNRG, int use_graph);

@@ -321,6 +321,28 @@ int bm_msld_chain_rule(bm_system *sys);
  * use_graph = 1 replays captured hipGraphs, one per (rebuild, energy)
  * step variant. */
 int bm_dynamics(bm_system *sys, long step0, long nsteps, int freq_nrg, int use_graph);
+/* Steps run on the device without host synchronisation; the call waits for
+   them at its end.  A capacity overflow (tile list, PME brick bins) halts
+   the state updates on the device at the step that raised it; the call
+   then grows that buffer x1.5 and runs again from that step, as the
+   reference's cull retries (domdec/cull.cu:337-375, up to 12x the first
+   size, domdec.cu:129-130), so the trajectory is the one an ample
+   capacity gives.  A non-finite position or velocity halts the state the
+   same way and the call returns BM_ENAN (bm_get_nan_flag names the first
+   atom or DOF); a column beyond its capacity returns BM_EOVERFLOW.
+   Members of a batch are not regrown: bm_batch_sync reports the overflow. */
+
+/* Capture the step graphs of all four (rebuild, energy) variants now, so
+   that no later bm_dynamics call captures one (a timed run starts with
+   every graph instantiated).  No counterpart in the reference. */
+int bm_capture_graphs(bm_system *sys);
+/* Set the list capacities (directed tiles; spline records per PME brick;
+   a value <= 0 keeps that one).  The list is emptied: the next step must
+   rebuild it (a step s with s % freq_domdec == 0).  Tests use it to start
+   from capacities too small for the system.  No counterpart in the
+   reference. */
+int bm_set_list_capacity(bm_system *sys, long long max_tiles, int brick_atoms);
+int bm_get_list_capacity(bm_system *sys, long long *max_tiles, int *brick_atoms);
 
 /* ---- batches: independent systems advanced together.  One captured
  * graph per (rebuild, energy) step variant advances every member: on
@@ -335,6 +357,8 @@ int bm_dynamics(bm_system *sys, long step0, long nsteps, int freq_nrg, int use_g
 typedef struct bm_batch bm_batch;
 int bm_batch_create(int n, bm_system *const *systems, bm_batch **out);
 int bm_batch_dynamics(bm_batch *batch, long step0, long nsteps, int freq_nrg);
+/* as bm_capture_graphs, for the batch step graphs */
+int bm_batch_capture_graphs(bm_batch *batch);
 /* synchronise every member; BM_ENAN / BM_EOVERFLOW as bm_sync reports them */
 int bm_batch_sync(bm_batch *batch);
 /* Replica exchange between neighbouring members (update/rex.cxx:15-125,

@@ -632,6 +632,24 @@ void nb14(const bm_topology *t, const bm_runparams *rp, const double *pos,
   }
 }
 
+// One exclusion correction E = -kqq erf(beta r)/r (bonded/pair.cu:181-193)
+// and dE/dr / r.  For beta r < 0.01 both come from their Taylor series
+// about r = 0 (E -> -kqq 2 beta/sqrt(pi), dE/dr / r finite): with msldEwaldType
+// 3 the pairs between alternative substituents of one site are corrected,
+// and such atoms may coincide (r = 0), where the closed form is 0/0.
+static void nbex_pair(double kqq, double beta, double r, double &fr, double &e) {
+  const double x = beta * r;
+  if (x < 0.01) {
+    const double x2 = x * x;
+    e = -kqq * beta * twoOverSqrtPi * (1 - x2 / 3 + x2 * x2 / 10 - x2 * x2 * x2 / 42);
+    fr = kqq * beta * beta * beta * twoOverSqrtPi * (2.0 / 3 - 2 * x2 / 5 + x2 * x2 / 7);
+    return;
+  }
+  const double rinv = 1 / r;
+  e = -kqq * std::erf(x) * rinv;
+  fr = kqq * (std::erf(x) * rinv - twoOverSqrtPi * beta * std::exp(-x * x)) * rinv * rinv;
+}
+
 void nbex(const bm_topology *t, const bm_runparams *rp, const double *pos,
           const double *lambda, double *force, double *dudl, double *energy) {
   const int n = t->n_atoms;
@@ -639,15 +657,15 @@ void nbex(const bm_topology *t, const bm_runparams *rp, const double *pos,
   for (int p = 0; p < t->n_nbex; p++) {
     int i = t->nbex_idx[2 * p], j = t->nbex_idx[2 * p + 1];
     Vec3 dr = minimg(getp(pos, n, i) - getp(pos, n, j), rp->box);
-    double r = norm(dr), rinv = 1 / r, br = beta * r;
+    double r = norm(dr);
     double kqq = kElectric * t->nbex_qq[p];
-    double f = kqq * (std::erf(br) * rinv - twoOverSqrtPi * beta * std::exp(-br * br)) * rinv;
-    double e = -kqq * std::erf(br) * rinv;
+    double fr, e;
+    nbex_pair(kqq, beta, r, fr, e);
     Scale2 s = scale2(&t->nbex_sb[2 * p], lambda);
-    f *= s.l0 * s.l1;
+    fr *= s.l0 * s.l1;
     scale_dudl(s, e, dudl);
-    addf(force, n, i, (f / r) * dr);
-    addf(force, n, j, (-f / r) * dr);
+    addf(force, n, i, fr * dr);
+    addf(force, n, j, (-fr) * dr);
     energy[BM_ENBRECIPEXCL] += e * s.l0 * s.l1;
   }
 }
@@ -1166,6 +1184,8 @@ void for_groups(const bm_topology *t, F fn) {
 extern "C" {
 
 int orc_threads(void) { return omp_get_max_threads(); }
+
+void orc_nbex_pair(double kqq, double beta, double r, double *fr, double *e) { nbex_pair(kqq, beta, r, *fr, *e); }
 
 long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
                 const double *lambda, unsigned terms, double *force, double *dudl, double *energy) {

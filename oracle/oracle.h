@@ -39,6 +39,11 @@ long orc_forces(const bm_topology *top, const bm_runparams *rp,
                 const double *pos, const double *lambda, unsigned terms,
                 double *force, double *dudl, double *energy);
 
+/* One PME exclusion correction at distance r (bonded/pair.cu:181-193):
+ * e = -kqq erf(beta r)/r and fr = (de/dr)/r, by Taylor series about r = 0
+ * when beta r < 0.01 (coincident atoms give the finite limit). */
+void orc_nbex_pair(double kqq, double beta, double r, double *fr, double *e);
+
 /* Reciprocal-space Ewald energy/forces by direct structure-factor
  * summation over |m_i| <= kmax (validation of the PME path). */
 double orc_ewald_recip_direct(const bm_topology *top, const bm_runparams *rp,

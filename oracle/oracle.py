@@ -33,6 +33,8 @@ def lib():
         L.orc_ewald_recip_direct.argtypes = [vp, vp, _f64p, _f64p, C.c_int, _f64p, _f64p]
         L.orc_ewald_real_images.restype = C.c_double
         L.orc_ewald_real_images.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_double]
+        L.orc_nbex_pair.restype = None
+        L.orc_nbex_pair.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _f64p]
         L.orc_bspline.restype = C.c_double
         L.orc_bspline.argtypes = [C.c_int, C.c_double]
         L.orc_bspline_moduli.restype = None
@@ -214,3 +216,10 @@ def step(top, rp, stepno, energy_step, x, v, lam):
 
 def threads():
     return lib().orc_threads()
+
+
+def nbex_pair(kqq, beta, r):
+    """(fr, e) of one PME exclusion correction at distance r"""
+    fr, e = C.c_double(), C.c_double()
+    lib().orc_nbex_pair(kqq, beta, r, C.byref(fr), C.byref(e))
+    return fr.value, e.value

@@ -776,3 +776,29 @@ def copy_restraints(sp, keep):
     c = copy.copy(sp)
     c.restraints = {keep: sp.restraints[keep]} if keep else {}
     return c
+
+
+def test_exclusion_correction_at_and_near_coincidence():
+    """The PME exclusion correction e = -kqq erf(beta r)/r takes its Taylor
+    series about r = 0 below beta r = 0.01 (msldEwaldType 3 corrects pairs
+    between alternative substituents, which may coincide).  Pinned by the
+    limit at r = 0 (-kqq 2 beta/sqrt(pi); the force factor kqq beta^3
+    4/(3 sqrt(pi))), agreement with math.erf on both sides of the switch,
+    and the force factor as the central difference of the energy."""
+    kqq, beta = 332.0716 * (-0.3) * 0.45, 0.34
+    fr0, e0 = orc.nbex_pair(kqq, beta, 0.0)
+    assert e0 == pytest.approx(-kqq * 2 * beta / math.sqrt(math.pi), rel=1e-15)
+    assert fr0 == pytest.approx(kqq * beta ** 3 * 4 / (3 * math.sqrt(math.pi)), rel=1e-15)
+    rs = 0.01 / beta
+    for r in (0.2 * rs, 0.9 * rs, 0.999 * rs, 1.001 * rs, 1.5 * rs, 10 * rs, 3.0):
+        fr, e = orc.nbex_pair(kqq, beta, r)
+        # closed forms in Python's libm (erf, exp); near r = 0 they lose
+        # about eps/(beta r)^2 to cancellation, so the bar follows that
+        ec = -kqq * math.erf(beta * r) / r
+        frc = kqq * (math.erf(beta * r) / r - 2 * beta / math.sqrt(math.pi) * math.exp(-(beta * r) ** 2)) / r ** 2
+        x = beta * r
+        assert e == pytest.approx(ec, rel=1e-14)
+        assert fr == pytest.approx(frc, rel=max(1e-12, 4e-16 / x ** 2))
+        h = 1e-4 * max(r, rs)
+        dedr = (orc.nbex_pair(kqq, beta, r + h)[1] - orc.nbex_pair(kqq, beta, r - h)[1]) / (2 * h)
+        assert fr * r == pytest.approx(dedr, rel=1e-6, abs=1e-9 * abs(kqq))
