@@ -85,11 +85,6 @@ struct Dev {
   int4 *chunk;                   // [maxchunks] (I, first tile, tiles, 0)
   float *ichunk;                 // [maxchunks][3][32] i-force partials of a chunk (FP32, fixed
                                  // order); the force assembly converts them to fixed point
-  int *tlow;                     // [maxB] tiles (I,J) of row I with J < I (the row's first ones)
-  int *tile_mirror;              // [maxtiles] for J > I: index of the tile (J,I) in row J
-  float *jpart;                  // [maxtiles][3][32] j-force partials that tile (J,I), J < I,
-                                 // leaves for block I, stored at the index of tile (I,J)
-  int *jflag;                    // [maxtiles] 1: jpart of this lower tile written this step
   int *ecount, *estart;          // [maxB],[maxB+1] tiles with exclusions per row
   int *nexcltiles;               // [1]
   unsigned *emask;               // [32*2*n_excl]

@@ -12,11 +12,9 @@
 //
 // The tile list is directed: row I holds every block J (J == I included)
 // whose rebuild box is within rcut + pad, in ascending J, so each pair of
-// blocks appears as (I,J) and (J,I).  The nonbonded kernel evaluates each
-// pair of blocks once, from the tile with J >= I: the part J >= I of every
-// row is cut into chunks of at most d.tchunk tiles, one i-block per chunk,
-// and the j-side forces of tile (I,J) go to the slot of its mirror (J,I),
-// one of the first tlow[J] tiles of row J.
+// blocks appears as (I,J) and (J,I).  Rows are cut into chunks of at most
+// d.tchunk tiles; the nonbonded kernel owns one chunk (one i-block) at a
+// time and never writes j-side forces.
 //
 // Column counts and bucket fills walk the slots of the previous rebuild,
 // which are ordered by column, so the lanes of a wave mostly share one
@@ -245,18 +243,15 @@ __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
     aLo[k] = d.bblo[3 * I + k];
     aHi[k] = d.bbhi[3 * I + k];
   }
-  int cnt = 0, low = 0;
+  int cnt = 0;
   for (int J0 = 0; J0 < nb; J0 += 64) {
     const int J = J0 + lane;
     const bool keep = J < nb && dd_tile_keep(d, s_lo, s_hi, I, aLo, aHi, J);
     cnt += __popcll(__ballot(keep));
-    low += __popcll(__ballot(keep && J < I));
   }
   if (lane == 0) {
     d.tcount[I] = cnt;
-    d.tlow[I] = low;
-    // the nonbonded kernel takes the tiles J >= I of the row
-    d.ccount[I] = (cnt - low + d.tchunk - 1) / d.tchunk;
+    d.ccount[I] = (cnt + d.tchunk - 1) / d.tchunk;
   }
 }
 
@@ -300,7 +295,7 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
     }
     base += __popcll(m);
   }
-  const int t0 = d.tstart[I] + d.tlow[I], cnt = d.tcount[I] - d.tlow[I], c0 = d.cstart[I];
+  const int t0 = d.tstart[I], cnt = d.tcount[I], c0 = d.cstart[I];
   const int tc = d.tchunk;
   for (int k = lane; k * tc < cnt; k += 64) d.chunk[c0 + k] = make_int4(I, t0 + k * tc, min(tc, cnt - k * tc), 0);
 }
@@ -332,25 +327,16 @@ __global__ void dd_excl_mark(Dev d) {
   if (tb >= 0) d.tile_flag[tb] = 1;
 }
 
-// one wave per row: tiles of the row holding exclusions; for every tile
-// (I,J) with J > I, the index of its mirror (J,I) in row J (the list is
-// symmetric: the gap test does not depend on the order of the boxes)
+// one wave per row: tiles of the row holding exclusions
 __global__ void __launch_bounds__(256) dd_excl_rows(Dev d) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= *d.nblocks) return;
   int cnt = 0;
-  if (*d.ntiles <= d.maxtiles) {
-    const int t1 = d.tstart[I + 1];
-    for (int t = d.tstart[I] + lane; t - lane < t1; t += 64) {
-      cnt += __popcll(__ballot(t < t1 && d.tile_flag[t]));
-      if (t < t1) {
-        const int J = d.tile_j[t];
-        if (J > I) d.tile_mirror[t] = dd_find_tile(d, J, I);
-      }
-    }
-  }
+  if (*d.ntiles <= d.maxtiles)
+    for (int t = d.tstart[I] + lane; t - lane < d.tstart[I + 1]; t += 64)
+      cnt += __popcll(__ballot(t < d.tstart[I + 1] && d.tile_flag[t]));
   if (lane == 0) d.ecount[I] = cnt;
 }
 

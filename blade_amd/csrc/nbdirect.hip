@@ -1,25 +1,24 @@
 // nbdirect.hip - real-space nonbonded tile kernel
 // (restates nbdirect/nbdirect.cu:87-490 pair semantics).
 //
-// Work unit: a chunk of at most d.tchunk tiles (I, J), J >= I, of one row I
-// of the rebuild list.  One wave64 owns a chunk: both 32-lane halves hold
+// Work unit: a chunk of at most d.tchunk directed tiles (I, J) of one row
+// I of the rebuild list.  One wave64 owns a chunk: both 32-lane halves hold
 // the 32 i-atoms of block I in registers, so i-forces accumulate without
-// any cross-lane or memory traffic.
+// any cross-lane or memory traffic, and no j-side force is ever written
+// (every pair is visited from both sides; energies take half per visit).
 //
 // Per tile and step: the tile is skipped when the current block boxes are
 // farther apart than rcut; otherwise the j-atoms within rcut of the current
-// i-box are compacted into LDS.  Each pair of a tile J > I is evaluated once:
-// the 64 lanes walk the compacted j-atoms on staggered diagonals, the i-side
-// force stays in registers and the j-side force is summed in LDS, then
-// stored (FP32, slot order) to the slot of the mirror tile (J, I).  The
-// diagonal tile evaluates both orders of its pairs and keeps the i-side.
-// Tiles without alchemical atoms run a path with no lambda arithmetic.
+// i-box form a wave-uniform 32-bit mask, and the two halves take the set
+// bits in turns (j data broadcast from LDS).  Tiles without alchemical atoms
+// run a path with no lambda arithmetic.
 //
-// Accumulation: at the end of a chunk the i-force partials are stored to
-// the chunk's own slot.  The force assembly (pme.hip) adds, per atom, the
-// chunks of its block's row and the j-partials of the row's lower tiles to
-// the force array, one thread per atom: no global atomics, and the result
-// does not depend on the order in which the chunks ran.
+// Accumulation: per-tile FP32 partial sums are converted to 64-bit fixed
+// point and summed per lane; at the end of a chunk the i-force partials are
+// stored to the chunk's own slot.  The force assembly (pme.hip) adds the
+// chunks of every atom's block to the force array, one thread per atom: no
+// global atomics, and the result does not depend on the order in which the
+// chunks ran.
 #include "kernels.h"
 #include "erfc_coef.h"
 
@@ -252,39 +251,18 @@ struct IAtom {
 constexpr int kVdwLdsMax = 6144;
 
 // The pairs of one tile for the lane's i-atom: the nj j-atoms near the
-// i-box, compacted into sp/spr/slo (sjx: their slot index in block J).
-//
-// ONCE (tiles J > I, each pair evaluated once): lane li of half h takes the
-// j-atoms (li + s) mod nj for s = h, h+2, ... < nj, so the 64 lanes read
-// different j-atoms and every (i, j) of the tile is met exactly once; the
-// j-side force goes to jf[0..95] (x, y, z by compact index) and, for
-// alchemical j-atoms, the j-block's dU/dlambda share to jf[96..127], with
-// LDS float adds (the lanes of one instruction are applied in a fixed
-// order, so the sums do not depend on timing).
-//
-// Otherwise (the diagonal tile I == I): both orders of every pair, the
-// halves take the compacted j-atoms in turns (entries 0, 2, ... and 1, 3,
-// ...) broadcast from LDS, only the i-side force is kept, and energies and
-// dU/dlambda take half per visit.
-//
-// CHK: the tile has exclusions or is the diagonal tile, so the exclusion
-// bit and j == i are tested.
-template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool CHK, bool ONCE>
+// i-box, compacted into sp/spr/slo (sjx: their slot index in block J),
+// shared out between the halves (half 0 takes entries 0, 2, ..., half 1
+// entries 1, 3, ...).  CHK: the tile has exclusions or is the diagonal
+// tile (I == J), so the exclusion bit and j == i are tested.
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool CHK>
 __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const IAtom &ia, int li,
                                            int half, const uint4 *sp, const int2 *spr, const float *slo,
                                            const unsigned char *sjx, const float2 *svdw, int nj,
-                                           unsigned em, bool self, float *jf, float &fx, float &fy,
-                                           float &fz, float &fli, double &etile) {
+                                           unsigned em, bool self, float &fx, float &fy, float &fz,
+                                           double &fli, double &etile) {
   const float sx = d.fracscale[0], sy = d.fracscale[1], sz = d.fracscale[2];
-  int jn = 0;
-  if (ONCE && nj > 0) jn = (li + half) % nj;
-  for (int s = half; s < nj; s += 2) {
-    int jl = s;
-    if (ONCE) {
-      jl = jn;
-      jn += 2;
-      if (jn >= nj) jn -= nj;
-    }
+  for (int jl = half; jl < nj; jl += 2) {
     const uint4 pj = sp[jl];
     const float dx = fdisp(ia.p.x, pj.x, sx);
     const float dy = fdisp(ia.p.y, pj.y, sy);
@@ -302,15 +280,15 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     float rinv = __frsqrt_rn(r2);
     float r = r2 * rinv;
     const float qq = ia.q * __uint_as_float(pj.w);
-    float s_ = 1.f, dredr = 1.f, dredll = 0.f, lamj = 1.f;
+    float s = 1.f, dredr = 1.f, dredll = 0.f, lamj = 1.f;
     const int bi = ia.b, bj = prj.y;
     const bool alchPair = ALCH && (bi | bj);
     if (ALCH && alchPair) {
       if (bj & 0xFFFF) lamj = d.lambdaf[bj & 0xFFFF];
       const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-      s_ = same ? (bi == bj ? ia.lam : 0.f) : ia.lam * lamj;
+      s = same ? (bi == bj ? ia.lam : 0.f) : ia.lam * lamj;
       if (SOFT) {
-        const float rSoft = kSoftCoreRadius * (1.f - s_);
+        const float rSoft = kSoftCoreRadius * (1.f - s);
         if (r < rSoft) {
           const float q = r / rSoft;
           float re = 1.f - 0.5f * q;
@@ -324,29 +302,23 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     }
     float f, eq, eA, eB;
     pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, eq, eA, eB);
-    if (ALCH) f *= s_;
+    if (ALCH) f *= s;
     if (ALCH && alchPair) {
-      // d(pair energy)/ds with s the pair's lambda scaling; dU/dlambda_b
-      // is fl times the lambda of the other block (fl itself within a block)
+      // dU/dlambda of the i-atom's block only: the visit from j adds j's.
+      // A same-block pair is visited twice for one lambda: half each time.
       const float e = qq * eq + lj.x * eA - lj.y * eB;
       const float fl = SOFT ? e + f * dredll : e;
       const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-      if (ONCE) {
-        if (same) {
-          if (bi == bj) fli += fl;
-        } else {
-          fli += lamj * fl;
-          if (bj & 0xFFFF) atomicAdd(&jf[96 + jl], ia.lam * fl);
-        }
-      } else if (same) {
-        if (bi == bj) fli += 0.5f * fl;
+      // summed in FP64: the FP32 pair terms without FP32 partial sums
+      if (same) {
+        if (bi == bj) fli += 0.5 * (double)fl;
       } else {
-        fli += lamj * fl;
+        fli += (double)(lamj * fl);
       }
     }
     if (ENERGY) {
       // FP64 pair energy from the exact integer displacement and FP64
-      // charges / LJ parameters (half per visit on the diagonal tile)
+      // charges / LJ parameters; half per visit
       const double2 ljd = d.vdwd[vi];
       const double qjd = (double)__uint_as_float(pj.w) + (double)slo[jl];
       const double ddx = (double)(int)(ia.p.x - pj.x) * d.ddscale[0];
@@ -368,20 +340,14 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
         }
       }
       (void)eq; (void)eA; (void)eB;
-      etile += (ONCE ? 1.0 : 0.5) * sd * pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
+      etile += 0.5 * sd * pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
     }
     float fr;
     if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
     else fr = f * rinv;
-    const float gx = fr * dx, gy = fr * dy, gz = fr * dz;
-    fx += gx;
-    fy += gy;
-    fz += gz;
-    if (ONCE) {
-      atomicAdd(&jf[jl], -gx);
-      atomicAdd(&jf[32 + jl], -gy);
-      atomicAdd(&jf[64 + jl], -gz);
-    }
+    fx += fr * dx;
+    fy += fr * dy;
+    fz += fr * dz;
   }
 }
 
@@ -397,7 +363,6 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
   __shared__ int2 s_par[4][kBlockAtoms];
   __shared__ float s_lo[4][kBlockAtoms];
   __shared__ unsigned char s_jx[4][kBlockAtoms];
-  __shared__ float s_jf[4][4 * kBlockAtoms];   // j-side sums of a tile: x, y, z, dU/dlambda
   extern __shared__ float2 s_vdw_dyn[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int half = lane >> 5, li = lane & 31;
@@ -415,7 +380,6 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
   int2 *spr = s_par[w];
   float *slo = s_lo[w];
   unsigned char *sjx = s_jx[w];
-  float *jf = s_jf[w];
 
   // one chunk per wave; waves past the chunk count of this step exit
   {
@@ -435,23 +399,18 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
       ia.lam = (ALCH && (ia.b & 0xFFFF)) ? d.lambdaf[ia.b & 0xFFFF] : 1.f;
     }
     const uint4 ci = d.bbc[I], hi = d.bbh[I];
-    int tJ = 0, tE = -1, tM = -1;
+    int tJ = 0, tE = -1;
     bool keep = false;
     if (lane < cnt) {
       tJ = d.tile_j[t0 + lane];
       tE = d.tile_ex[t0 + lane];
       keep = boxes_within(d, I, tJ, c.rc2);
-      if (tJ != I) {
-        tM = d.tile_mirror[t0 + lane];
-        // a tile culled this step leaves no j-side partials
-        if (!keep && tM >= 0) d.jflag[tM] = 0;
-      }
     }
     unsigned long long act = __ballot(keep);
     // i-partials of the whole chunk in FP32 (at most 16 tiles, a fixed
     // order), stored once per chunk
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    float fli = 0.f;
+    double fli = 0.0;
     // software pipeline: the j slots, exclusion word and alchemical flag of
     // the next tile are loaded while the current tile computes
     int k = act ? __ffsll((long long)act) - 1 : -1;
@@ -462,12 +421,11 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
     float lo = 0.f;
     unsigned em = 0u;
     unsigned alchJ = 0u;
-    int tEc = -1, tMc = -1;
+    int tEc = -1;
     auto fetch = [&](int kk, int &J_, uint4 &pj_, int2 &prj_, float &lo_, unsigned &em_, unsigned &al_,
-                     int &ex_, int &m_) {
+                     int &ex_) {
       J_ = __builtin_amdgcn_readlane(tJ, kk);
       const int ex = __builtin_amdgcn_readlane(tE, kk);
-      m_ = __builtin_amdgcn_readlane(tM, kk);
       ex_ = ex;
       const int sj = J_ * kBlockAtoms + li;
       pj_ = d.spos[sj];
@@ -476,68 +434,43 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
       em_ = ex >= 0 ? d.emask[ex * kBlockAtoms + li] : 0u;
       al_ = ALCH ? d.bbh[J_].w : 0u;
     };
-    if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ, tEc, tMc);
+    if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ, tEc);
     while (k >= 0) {
       // the j-atoms near the i-box, compacted to the front of the staging
       const bool near = half == 0 && prj.x >= 0 && near_box(d, ci, hi, pj, c.rc2);
       const unsigned long long nm = __ballot(near);
       const int nj = __popcll(nm);
-      const int eo = __popcll(nm & ((1ull << li) - 1ull));
       if (near) {
-        sp[eo] = pj;
-        spr[eo] = prj;
-        sjx[eo] = (unsigned char)li;
-        if (ENERGY) slo[eo] = lo;
-        jf[eo] = 0.f;
-        jf[32 + eo] = 0.f;
-        jf[64 + eo] = 0.f;
-        if (ALCH) jf[96 + eo] = 0.f;
+        const int e = __popcll(nm & ((1ull << li) - 1ull));
+        sp[e] = pj;
+        spr[e] = prj;
+        sjx[e] = (unsigned char)li;
+        if (ENERGY) slo[e] = lo;
       }
-      const int Jc = J, ex = tEc, mc = tMc, bjc = prj.y;
+      const int Jc = J, ex = tEc;
       const unsigned emc = em, alchJc = alchJ;
       const int kn = act ? __ffsll((long long)act) - 1 : -1;
       if (kn >= 0) {
         act &= act - 1;
-        fetch(kn, J, pj, prj, lo, em, alchJ, tEc, tMc);
+        fetch(kn, J, pj, prj, lo, em, alchJ, tEc);
       }
       wave_lds_sync();
       double etile = 0.0;
       const bool alchTile = ALCH && (hi.w | alchJc);
       const bool self = I == Jc, chk = self || ex >= 0;
-#define BM_TILE(AL, SO, CK, ON)                                                                        \
-  tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK, ON>(d, c, ia, li, half, sp, spr, slo, sjx, s_vdw, nj, emc, \
-                                                self, jf, fx, fy, fz, fli, etile)
-      if (self) {
-        if (alchTile) BM_TILE(ALCH, SOFT, true, false);
-        else BM_TILE(false, false, true, false);
-      } else if (alchTile) {
-        if (chk) BM_TILE(ALCH, SOFT, true, true);
-        else BM_TILE(ALCH, SOFT, false, true);
+#define BM_TILE(AL, SO, CK)                                                                        \
+  tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK>(d, c, ia, li, half, sp, spr, slo, sjx, s_vdw, nj, emc, \
+                                            self, fx, fy, fz, fli, etile)
+      if (alchTile) {
+        if (chk) BM_TILE(ALCH, SOFT, true);
+        else BM_TILE(ALCH, SOFT, false);
       } else {
-        if (chk) BM_TILE(false, false, true, true);
-        else BM_TILE(false, false, false, true);
+        if (chk) BM_TILE(false, false, true);
+        else BM_TILE(false, false, false);
       }
 #undef BM_TILE
       if (ENERGY) eacc += etile;
       wave_lds_sync();
-      if (!self && mc >= 0) {
-        // j-side partials of the tile into the mirror tile's slot, in slot
-        // order (zero for the j-atoms that were not near)
-        if (half == 0) {
-          if (nj > 0) {
-            const bool mine = (nm >> li) & 1ull;
-            float* dst = d.jpart + (long long)mc * 3 * kBlockAtoms;
-            dst[li] = mine ? jf[eo] : 0.f;
-            dst[kBlockAtoms + li] = mine ? jf[32 + eo] : 0.f;
-            dst[2 * kBlockAtoms + li] = mine ? jf[64 + eo] : 0.f;
-            if (ALCH && alchTile && mine && (bjc & 0xFFFF)) {
-              const float v = jf[96 + eo];
-              if (v != 0.f) atomic_fix_f(&d.dudl[bjc & 0xFFFF], v);
-            }
-          }
-          if (li == 0) d.jflag[mc] = nj > 0 ? 1 : 0;
-        }
-      }
       k = kn;
     }
     // both halves hold partials of the same i-atoms: (own + other) is the
@@ -553,7 +486,7 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
     }
     if (ALCH) {
       fli += __shfl_xor(fli, 32, 64);
-      if (half == 0 && (ia.b & 0xFFFF) && fli != 0.f) atomic_fix_f(&d.dudl[ia.b & 0xFFFF], fli);
+      if (half == 0 && (ia.b & 0xFFFF) && fli != 0.0) atomic_fix(&d.dudl[ia.b & 0xFFFF], fix_d(fli));
     }
   }
   if (ENERGY) {

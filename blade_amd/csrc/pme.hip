@@ -749,9 +749,8 @@ __global__ void __launch_bounds__(256) fft_x_conv_multi(Multi m, int energy) {
 // point, the components selected by mask: the PME force 2 lambda q
 // grad(phi) and dU/dlambda = 2 q phi from the potential grid (N = spline
 // order; N = 0 compiles no PME part), the nonbonded chunk partials of the
-// atom's block and the j-side partials left in its row's lower tiles, and
-// the atom's bonded slots.  The lanes split the stencil, the chunks, the
-// tiles and the slots; integer sums across them are exact.  One lane
+// atom's block, and the atom's bonded slots.  The lanes split the stencil,
+// the chunks and the slots; integer sums across them are exact.  One lane
 // per force component and stream order make the read-modify-write of the
 // force array safe without atomics.
 template <int N, bool KICK>
@@ -856,16 +855,6 @@ __device__ __forceinline__ void force_assemble_body(const Dev &d, int bid, int n
     const int c1 = d.cstart[I + 1];
     for (int c = d.cstart[I] + sub; c < c1; c += 8) {
       const float *src = d.ichunk + (long long)c * 3 * kBlockAtoms;
-      fx += fix_f(src[li]);
-      fy += fix_f(src[kBlockAtoms + li]);
-      fz += fix_f(src[2 * kBlockAtoms + li]);
-    }
-    // j-side partials that the tiles (J, I), J < I, left in the slots of
-    // row I's first tlow[I] tiles
-    const int t0 = d.tstart[I], t1 = t0 + d.tlow[I];
-    for (int t = t0 + sub; t < t1; t += 8) {
-      if (!d.jflag[t]) continue;
-      const float *src = d.jpart + (long long)t * 3 * kBlockAtoms;
       fx += fix_f(src[li]);
       fy += fix_f(src[kBlockAtoms + li]);
       fz += fix_f(src[2 * kBlockAtoms + li]);

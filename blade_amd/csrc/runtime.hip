@@ -404,10 +404,6 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.nchunks, 1);
   ALLOC(d.chunk, d.maxchunks);
   ALLOC(d.ichunk, (size_t)d.maxchunks * 3 * kBlockAtoms);
-  ALLOC(d.tlow, MB);
-  ALLOC(d.tile_mirror, s->maxTiles + 1);
-  ALLOC(d.jpart, (size_t)(s->maxTiles + 1) * 3 * kBlockAtoms);
-  ALLOC(d.jflag, s->maxTiles + 1);
   ALLOC(d.ecount, MB);
   ALLOC(d.estart, MB + 1);
   ALLOC(d.nexcltiles, 1);
@@ -976,8 +972,6 @@ static int set_tile_capacity(bm_system *s, long long maxTiles) {
   const size_t T = (size_t)maxTiles + 1;
   if ((rc = s->realloc_dev(d.tile_i, T)) || (rc = s->realloc_dev(d.tile_j, T)) ||
       (rc = s->realloc_dev(d.tile_ex, T)) || (rc = s->realloc_dev(d.tile_flag, T)) ||
-      (rc = s->realloc_dev(d.tile_mirror, T)) || (rc = s->realloc_dev(d.jflag, T)) ||
-      (rc = s->realloc_dev(d.jpart, T * 3 * blade::kBlockAtoms)) ||
       (rc = s->realloc_dev(d.chunk, (size_t)d.maxchunks)) ||
       (rc = s->realloc_dev(d.ichunk, (size_t)d.maxchunks * 3 * blade::kBlockAtoms)))
     return rc;
