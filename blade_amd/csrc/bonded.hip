@@ -2,33 +2,16 @@
 // pairs, PME exclusion corrections and MSLD lambda biases in one launch
 // (restates bonded/bonded.cu:44-527, bonded/pair.cu:56-387 and
 // msld/msld.cu:1000-1130).  Each workgroup handles one term family so the
-// energy slot is uniform; geometry comes from 32-bit fractional
-// coordinates (exact minimum image) and the arithmetic is FP32 (1-4 pairs
-// and exclusion corrections FP64).  Forces go to per-term slots, not to the
+// energy slot is uniform.  Every term is FP64 (geometry from the FP64
+// positions, minimum image by rounding): bonded energies enter dU/dlambda
+// directly, and FP32 term energies summed over a site's terms miss the
+// 1e-3 kcal/mol dU/dlambda bar.  Forces go to per-term slots, not to the
 // force array: no global atomics (the force assembly sums them per atom).
 #include "kernels.h"
 #include "erfc_coef.h"
 
 namespace blade {
 
-struct F3 { float x, y, z; };
-__device__ __forceinline__ F3 f3(float a, float b, float c) { return F3{a, b, c}; }
-__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
-  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-
-// minimum-image displacement r_i - r_j
-__device__ __forceinline__ F3 disp(const Dev &d, int i, int j) {
-  F3 r;
-  r.x = fdisp(frac_u32(d.pos[i], d.invbox[0]), frac_u32(d.pos[j], d.invbox[0]), d.fracscale[0]);
-  r.y = fdisp(frac_u32(d.pos[d.n + i], d.invbox[1]), frac_u32(d.pos[d.n + j], d.invbox[1]), d.fracscale[1]);
-  r.z = fdisp(frac_u32(d.pos[2 * d.n + i], d.invbox[2]), frac_u32(d.pos[2 * d.n + j], d.invbox[2]), d.fracscale[2]);
-  return r;
-}
 struct D3v { double x, y, z; };
 // FP64 minimum-image displacement r_i - r_j
 __device__ __forceinline__ D3v dispd(const Dev &d, int i, int j) {
@@ -50,27 +33,30 @@ __device__ __forceinline__ void put_d(const Dev &d, int slot, double s, D3v r) {
   d.bslot[d.nslot + p] = fix_d(s * r.y);
   d.bslot[2 * d.nslot + p] = fix_d(s * r.z);
 }
-__device__ __forceinline__ void put_f(const Dev &d, int slot, F3 f) {
-  const int p = d.bspos[slot];
-  d.bslot[p] = fix_f(f.x);
-  d.bslot[d.nslot + p] = fix_f(f.y);
-  d.bslot[2 * d.nslot + p] = fix_f(f.z);
+__device__ __forceinline__ D3v dsub(D3v a, D3v b) { return D3v{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3v dadd(D3v a, D3v b) { return D3v{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3v dscale(double s, D3v a) { return D3v{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double ddot(D3v a, D3v b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3v dcross(D3v a, D3v b) {
+  return D3v{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+__device__ __forceinline__ void put_v(const Dev &d, int slot, D3v f) { put_d(d, slot, 1.0, f); }
 
-// lambda scaling of a term carrying two siteBlock codes (bonded.cu:80-89)
-struct Sc { int b0, b1; float l0, l1; };
-__device__ __forceinline__ Sc scale2(const Dev &d, int2 sb) {
-  Sc s{sb.x & 0xFFFF, sb.y & 0xFFFF, 1.f, 1.f};
+// lambda scaling of a term carrying two siteBlock codes (bonded.cu:80-89),
+// and dU/dlambda of its energy e (FP64)
+struct Sd { int b0, b1; double l0, l1; };
+__device__ __forceinline__ Sd scale2d(const Dev &d, int2 sb) {
+  Sd s{sb.x & 0xFFFF, sb.y & 0xFFFF, 1.0, 1.0};
   if (s.b0) {
-    s.l0 = d.lambdaf[s.b0];
-    if (s.b1) s.l1 = d.lambdaf[s.b1];
+    s.l0 = d.lambda[s.b0];
+    if (s.b1) s.l1 = d.lambda[s.b1];
   }
   return s;
 }
-__device__ __forceinline__ void scale_dudl(const Dev &d, const Sc &s, float e) {
+__device__ __forceinline__ void scale_dudl_d(const Dev &d, const Sd &s, double e) {
   if (s.b0) {
-    atomic_fix_f(&d.dudl[s.b0], s.l1 * e);
-    if (s.b1) atomic_fix_f(&d.dudl[s.b1], s.l0 * e);
+    atomic_fix(&d.dudl[s.b0], fix_d(s.l1 * e));
+    if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(s.l0 * e));
   }
 }
 
@@ -247,14 +233,6 @@ __host__ __device__ inline BondedLaunch bonded_layout(const Dev &d, unsigned fam
   return L;
 }
 
-__device__ __forceinline__ D3v dsub(D3v a, D3v b) { return D3v{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ D3v dadd(D3v a, D3v b) { return D3v{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ D3v dscale(double s, D3v a) { return D3v{s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ double ddot(D3v a, D3v b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ D3v dcross(D3v a, D3v b) {
-  return D3v{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ __forceinline__ void put_v(const Dev &d, int slot, D3v f) { put_d(d, slot, 1.0, f); }
 __device__ __forceinline__ void zero_slots(const Dev &d, int first, int count) {
   for (int k = 0; k < count; k++) put_d(d, first + k, 0.0, D3v{0.0, 0.0, 0.0});
 }
@@ -439,16 +417,16 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     const int t = (bid - L.b_bond) * blockDim.x + threadIdx.x;
     if (t < d.nbond) {
       int2 ij = d.bond_i[t];
-      F3 dr = disp(d, ij.x, ij.y);
-      float r = sqrtf(dot3(dr, dr));
-      Sc s = scale2(d, d.bond_sb[t]);
-      float2 p = d.bond_p[t];
-      float db = r - p.y;
-      float et = 0.5f * p.x * db * db;
-      float fb = p.x * db * s.l0 * s.l1;
-      scale_dudl(d, s, et);
-      put_f(d, d.sl_bond + 2 * t, (fb / r) * dr);
-      put_f(d, d.sl_bond + 2 * t + 1, (-fb / r) * dr);
+      const D3v dr = dispd(d, ij.x, ij.y);
+      const double r = sqrt(ddot(dr, dr));
+      const Sd s = scale2d(d, d.bond_sb[t]);
+      const double2 p = d.bond_p[t];
+      const double db = r - p.y;
+      const double et = 0.5 * p.x * db * db;
+      const double fb = p.x * db * s.l0 * s.l1;
+      scale_dudl_d(d, s, et);
+      put_d(d, d.sl_bond + 2 * t, fb / r, dr);
+      put_d(d, d.sl_bond + 2 * t + 1, -fb / r, dr);
       e = et * s.l0 * s.l1;
       // bonds and Urey-Bradley may share a workgroup: split by index
       if (t >= d.nbond12) {
@@ -462,22 +440,22 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     const int t = (bid - L.b_angle) * blockDim.x + threadIdx.x;
     if (t < d.nangle) {
       int4 ix = d.angle_i[t];
-      F3 rij = disp(d, ix.x, ix.y), rkj = disp(d, ix.z, ix.y);
-      float dp = dot3(rij, rkj);
-      F3 cr = cross3(rij, rkj);
-      float mc = sqrtf(dot3(cr, cr));
-      float th = atan2f(mc, dp);
-      Sc s = scale2(d, d.angle_sb[t]);
-      float2 p = d.angle_p[t];
-      float dth = th - p.y;
-      float et = 0.5f * p.x * dth * dth;
-      float fa = p.x * dth * s.l0 * s.l1;
-      scale_dudl(d, s, et);
-      F3 fi = (fa / (mc * dot3(rij, rij))) * cross3(rij, cr);
-      F3 fk = (-fa / (mc * dot3(rkj, rkj))) * cross3(rkj, cr);
-      put_f(d, d.sl_angle + 3 * t, fi);
-      put_f(d, d.sl_angle + 3 * t + 2, fk);
-      put_f(d, d.sl_angle + 3 * t + 1, -1.f * (fi + fk));
+      const D3v rij = dispd(d, ix.x, ix.y), rkj = dispd(d, ix.z, ix.y);
+      const double dp = ddot(rij, rkj);
+      const D3v cr = dcross(rij, rkj);
+      const double mc = sqrt(ddot(cr, cr));
+      const double th = atan2(mc, dp);
+      const Sd s = scale2d(d, d.angle_sb[t]);
+      const double2 p = d.angle_p[t];
+      const double dth = th - p.y;
+      const double et = 0.5 * p.x * dth * dth;
+      const double fa = p.x * dth * s.l0 * s.l1;
+      scale_dudl_d(d, s, et);
+      const D3v fi = dscale(fa / (mc * ddot(rij, rij)), dcross(rij, cr));
+      const D3v fk = dscale(-fa / (mc * ddot(rkj, rkj)), dcross(rkj, cr));
+      put_v(d, d.sl_angle + 3 * t, fi);
+      put_v(d, d.sl_angle + 3 * t + 2, fk);
+      put_v(d, d.sl_angle + 3 * t + 1, dscale(-1.0, dadd(fi, fk)));
       e = et * s.l0 * s.l1;
     }
   } else if (bid < L.b_nb14) {
@@ -486,39 +464,39 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     const int t = (bid - (impr ? L.b_impr : L.b_dihe)) * blockDim.x + threadIdx.x;
     if (t < (impr ? d.nimpr : d.ndihe)) {
       int4 ix = impr ? d.impr_i[t] : d.dihe_i[t];
-      float4 p = impr ? d.impr_p[t] : d.dihe_p[t];
-      F3 rij = disp(d, ix.x, ix.y), rjk = disp(d, ix.y, ix.z), rkl = disp(d, ix.z, ix.w);
-      F3 m = cross3(rij, rjk), nv = cross3(rjk, rkl);
-      F3 mn = cross3(m, nv);
-      float phi = atan2f(sqrtf(dot3(mn, mn)), dot3(m, nv));
-      if (dot3(rij, nv) > 0.f) phi = -phi;  // bonded.cu:372-374
-      float fphi, et;
+      const double4 p = impr ? d.impr_p[t] : d.dihe_p[t];
+      const D3v rij = dispd(d, ix.x, ix.y), rjk = dispd(d, ix.y, ix.z), rkl = dispd(d, ix.z, ix.w);
+      const D3v m = dcross(rij, rjk), nv = dcross(rjk, rkl);
+      const D3v mn = dcross(m, nv);
+      double phi = atan2(sqrt(ddot(mn, mn)), ddot(m, nv));
+      if (ddot(rij, nv) > 0.0) phi = -phi;  // bonded.cu:372-374
+      double fphi, et;
       const int nper = (int)p.y;
       if (!impr || nper > 0) {
-        float dphi = nper * phi - p.z;
-        float sn, cs;
-        sincosf(dphi, &sn, &cs);
+        const double dphi = nper * phi - p.z;
+        double sn, cs;
+        sincos(dphi, &sn, &cs);
         fphi = -p.x * nper * sn;
-        et = p.x * (cs + 1.f);
+        et = p.x * (cs + 1.0);
       } else {
-        float dphi = phi - p.z;
-        dphi -= 6.283185307179586f * floorf((dphi + 3.141592653589793f) / 6.283185307179586f);
+        double dphi = phi - p.z;
+        dphi -= 6.283185307179586 * floor((dphi + 3.141592653589793) / 6.283185307179586);
         fphi = p.x * dphi;
-        et = 0.5f * p.x * dphi * dphi;
+        et = 0.5 * p.x * dphi * dphi;
       }
-      Sc s = scale2(d, impr ? d.impr_sb[t] : d.dihe_sb[t]);
+      const Sd s = scale2d(d, impr ? d.impr_sb[t] : d.dihe_sb[t]);
       fphi *= s.l0 * s.l1;
-      scale_dudl(d, s, et);
-      float rjk2 = dot3(rjk, rjk), r = sqrtf(rjk2);
-      F3 fi = (-fphi * r / dot3(m, m)) * m;
-      F3 fl = (-fphi * r / dot3(nv, nv)) * nv;
-      float pp = dot3(rij, rjk) / rjk2, qq = dot3(rkl, rjk) / rjk2;
-      F3 fj = (-pp) * fi + (-qq) * fl;
+      scale_dudl_d(d, s, et);
+      const double rjk2 = ddot(rjk, rjk), r = sqrt(rjk2);
+      const D3v fi = dscale(-fphi * r / ddot(m, m), m);
+      const D3v fl = dscale(-fphi * r / ddot(nv, nv), nv);
+      const double pp = ddot(rij, rjk) / rjk2, qq = ddot(rkl, rjk) / rjk2;
+      const D3v fj = dadd(dscale(-pp, fi), dscale(-qq, fl));
       const int s0 = (impr ? d.sl_impr : d.sl_dihe) + 4 * t;
-      put_f(d, s0, fi);
-      put_f(d, s0 + 3, -1.f * fl);
-      put_f(d, s0 + 2, fl - fj);
-      put_f(d, s0 + 1, fj - fi);
+      put_v(d, s0, fi);
+      put_v(d, s0 + 3, dscale(-1.0, fl));
+      put_v(d, s0 + 2, dsub(fl, fj));
+      put_v(d, s0 + 1, dsub(fj, fi));
       e = et * s.l0 * s.l1;
     }
   } else if (bid < L.b_nbex) {
@@ -530,7 +508,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       int2 ij = d.nb14_i[t];
       D3v dr = dispd(d, ij.x, ij.y);
       double r = sqrt(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
-      Sc s = scale2(d, d.nb14_sb[t]);
+      const Sd s = scale2d(d, d.nb14_sb[t]);
       const double ll = (s.b0 ? d.lambda[s.b0] : 1.0) * (s.b1 ? d.lambda[s.b1] : 1.0);
       double reff = r, dredr = 1.0, dredll = 0.0;
       if (d.softcore14 && s.b0) {
@@ -578,7 +556,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
         et = -kqq * erfv * rinv;
         fr = kqq * (erfv * rinv - 1.1283791670955126 * beta * exp(-x * x)) * rinv * rinv;
       }
-      Sc s = scale2(d, d.nbex_sb[t]);
+      const Sd s = scale2d(d, d.nbex_sb[t]);
       const double ll = (s.b0 ? d.lambda[s.b0] : 1.0) * (s.b1 ? d.lambda[s.b1] : 1.0);
       fr *= ll;
       if (s.b0) {
@@ -623,26 +601,26 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
     }
   } else if (bid < L.b_noe) {
     // CMAP cross terms (bonded.cu:531-720), one thread per term: the two
-    // dihedrals in FP32 as in the dihedral family, then bicubic Hermite
-    // interpolation of the cell's corner values and slopes in FP64
+    // dihedrals as in the dihedral family, then bicubic Hermite
+    // interpolation of the cell's corner values and slopes, all FP64
     slot = BM_ECMAP;
     const int t = (bid - L.b_cmap) * blockDim.x + threadIdx.x;
     if (t < d.ncmap) {
       const int4 sv = d.cmap_s[t];
       const int2 g = d.cmap_g[sv.w];
       const int ng = g.x;
-      F3 rij[2], rjk[2], rkl[2], m[2], nv[2];
+      D3v rij[2], rjk[2], rkl[2], m[2], nv[2];
       double ang[2];
       for (int a = 0; a < 2; a++) {
         const int4 ix = d.cmap_i[2 * t + a];
-        rij[a] = disp(d, ix.x, ix.y);
-        rjk[a] = disp(d, ix.y, ix.z);
-        rkl[a] = disp(d, ix.z, ix.w);
-        m[a] = cross3(rij[a], rjk[a]);
-        nv[a] = cross3(rjk[a], rkl[a]);
-        const F3 mn = cross3(m[a], nv[a]);
-        float phi = atan2f(sqrtf(dot3(mn, mn)), dot3(m[a], nv[a]));
-        if (dot3(rij[a], nv[a]) > 0.f) phi = -phi;
+        rij[a] = dispd(d, ix.x, ix.y);
+        rjk[a] = dispd(d, ix.y, ix.z);
+        rkl[a] = dispd(d, ix.z, ix.w);
+        m[a] = dcross(rij[a], rjk[a]);
+        nv[a] = dcross(rjk[a], rkl[a]);
+        const D3v mn = dcross(m[a], nv[a]);
+        double phi = atan2(sqrt(ddot(mn, mn)), ddot(m[a], nv[a]));
+        if (ddot(rij[a], nv[a]) > 0.0) phi = -phi;
         ang[a] = phi;
       }
       // grid point i sits at -pi + i 2 pi / ng
@@ -677,17 +655,17 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       if (b1) atomic_fix(&d.dudl[b1], fix_d(l0 * l2 * ec));
       if (b2) atomic_fix(&d.dudl[b2], fix_d(l0 * l1 * ec));
       for (int a = 0; a < 2; a++) {
-        const float fphi = (float)((a ? ds : dt) * gs * ll);
-        const float rjk2 = dot3(rjk[a], rjk[a]), r = sqrtf(rjk2);
-        const F3 fi = (-fphi * r / dot3(m[a], m[a])) * m[a];
-        const F3 fl = (-fphi * r / dot3(nv[a], nv[a])) * nv[a];
-        const float pp = dot3(rij[a], rjk[a]) / rjk2, qq = dot3(rkl[a], rjk[a]) / rjk2;
-        const F3 fj = (-pp) * fi + (-qq) * fl;
+        const double fphi = (a ? ds : dt) * gs * ll;
+        const double rjk2 = ddot(rjk[a], rjk[a]), r = sqrt(rjk2);
+        const D3v fi = dscale(-fphi * r / ddot(m[a], m[a]), m[a]);
+        const D3v fl = dscale(-fphi * r / ddot(nv[a], nv[a]), nv[a]);
+        const double pp = ddot(rij[a], rjk[a]) / rjk2, qq = ddot(rkl[a], rjk[a]) / rjk2;
+        const D3v fj = dadd(dscale(-pp, fi), dscale(-qq, fl));
         const int s0 = d.sl_cmap + 8 * t + 4 * a;
-        put_f(d, s0, fi);
-        put_f(d, s0 + 3, -1.f * fl);
-        put_f(d, s0 + 2, fl - fj);
-        put_f(d, s0 + 1, fj - fi);
+        put_v(d, s0, fi);
+        put_v(d, s0 + 3, dscale(-1.0, fl));
+        put_v(d, s0 + 2, dsub(fl, fj));
+        put_v(d, s0 + 1, dsub(fj, fi));
       }
       e = ec * ll;
     }

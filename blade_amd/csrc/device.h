@@ -120,10 +120,10 @@ struct Dev {
   const void *mdft[3][2];        // per axis, hi/lo: bf16 A fragments of the forward DFT matrix (fft_mfma)
   // bonded terms (float parameters, int indices)
   int nbond, nbond12, nangle, ndihe, nimpr, nnb14, nnbex;
-  int2 *bond_i; int2 *bond_sb; float2 *bond_p;            // (kb, b0)
-  int4 *angle_i; int2 *angle_sb; float2 *angle_p;         // (k, t0)
-  int4 *dihe_i; int2 *dihe_sb; float4 *dihe_p;            // (k, n, d0)
-  int4 *impr_i; int2 *impr_sb; float4 *impr_p;            // (k, n, d0)
+  int2 *bond_i; int2 *bond_sb; double2 *bond_p;           // (kb, b0)
+  int4 *angle_i; int2 *angle_sb; double2 *angle_p;        // (k, t0)
+  int4 *dihe_i; int2 *dihe_sb; double4 *dihe_p;           // (k, n, d0)
+  int4 *impr_i; int2 *impr_sb; double4 *impr_p;           // (k, n, d0)
   int2 *nb14_i; int2 *nb14_sb; double4 *nb14_p;           // (qq, c12, c6, e14fac)
   int2 *nbex_i; int2 *nbex_sb; double *nbex_p;            // qq
   // bonded force slots: every term stores its per-atom forces (fixed

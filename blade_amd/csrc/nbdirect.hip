@@ -306,10 +306,10 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     if (ALCH && alchPair) {
       // dU/dlambda of the i-atom's block only: the visit from j adds j's.
       // A same-block pair is visited twice for one lambda: half each time.
+      // The FP32 pair terms are summed in FP64.
       const float e = qq * eq + lj.x * eA - lj.y * eB;
       const float fl = SOFT ? e + f * dredll : e;
       const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-      // summed in FP64: the FP32 pair terms without FP32 partial sums
       if (same) {
         if (bi == bj) fli += 0.5 * (double)fl;
       } else {
@@ -339,9 +339,9 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
           }
         }
       }
-      (void)eq; (void)eA; (void)eB;
       etile += 0.5 * sd * pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
     }
+    (void)eq; (void)eA; (void)eB;
     float fr;
     if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
     else fr = f * rinv;

@@ -500,21 +500,21 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     d.nbond = t->n_bonds;
     d.nbond12 = t->n_bonds12;
     std::vector<int2> bi(d.nbond), bsb(d.nbond);
-    std::vector<float2> bp(d.nbond);
+    std::vector<double2> bp(d.nbond);
     for (int b = 0; b < d.nbond; b++) {
       bi[b] = make_int2(t->bond_idx[2 * b], t->bond_idx[2 * b + 1]);
       bsb[b] = make_int2(t->bond_sb[2 * b], t->bond_sb[2 * b + 1]);
-      bp[b] = make_float2((float)t->bond_kb[b], (float)t->bond_b0[b]);
+      bp[b] = make_double2(t->bond_kb[b], t->bond_b0[b]);
     }
     UPLOAD(d.bond_i, bi); UPLOAD(d.bond_sb, bsb); UPLOAD(d.bond_p, bp);
     d.nangle = t->n_angles;
     std::vector<int4> ai(d.nangle);
     std::vector<int2> asb(d.nangle);
-    std::vector<float2> ap(d.nangle);
+    std::vector<double2> ap(d.nangle);
     for (int a = 0; a < d.nangle; a++) {
       ai[a] = make_int4(t->angle_idx[3 * a], t->angle_idx[3 * a + 1], t->angle_idx[3 * a + 2], 0);
       asb[a] = make_int2(t->angle_sb[2 * a], t->angle_sb[2 * a + 1]);
-      ap[a] = make_float2((float)t->angle_k[a], (float)t->angle_t0[a]);
+      ap[a] = make_double2(t->angle_k[a], t->angle_t0[a]);
     }
     UPLOAD(d.angle_i, ai); UPLOAD(d.angle_sb, asb); UPLOAD(d.angle_p, ap);
     for (int pass = 0; pass < 2; pass++) {
@@ -526,11 +526,11 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       const double *d0 = pass ? t->impr_d0 : t->dihe_d0;
       std::vector<int4> di(cnt);
       std::vector<int2> dsb(cnt);
-      std::vector<float4> dp(cnt);
+      std::vector<double4> dp(cnt);
       for (int q = 0; q < cnt; q++) {
         di[q] = make_int4(idx[4 * q], idx[4 * q + 1], idx[4 * q + 2], idx[4 * q + 3]);
         dsb[q] = make_int2(sbv[2 * q], sbv[2 * q + 1]);
-        dp[q] = make_float4((float)kk[q], (float)nn[q], (float)d0[q], 0.f);
+        dp[q] = make_double4(kk[q], (double)nn[q], d0[q], 0.0);
       }
       if (pass) { d.nimpr = cnt; UPLOAD(d.impr_i, di); UPLOAD(d.impr_sb, dsb); UPLOAD(d.impr_p, dp); }
       else { d.ndihe = cnt; UPLOAD(d.dihe_i, di); UPLOAD(d.dihe_sb, dsb); UPLOAD(d.dihe_p, dp); }
