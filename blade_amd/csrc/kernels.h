@@ -54,6 +54,7 @@ bool fft_plane_path(const Dev &d);
 int grid_fft_planes(const Dev &d);
 int grid_fft_x(const Dev &d);
 int grid_nbdirect(const Dev &d);
+int grid_nbdirect_alch(const Dev &d);
 size_t lds_nbdirect(const Dev &d);
 int grid_assemble(const Dev &d);
 int grid_tail(const Dev &d);

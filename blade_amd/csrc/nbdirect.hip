@@ -303,29 +303,21 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     float f, eq, eA, eB;
     pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, eq, eA, eB);
     if (ALCH) f *= s;
-    if (ALCH && alchPair) {
-      // dU/dlambda of the i-atom's block only: the visit from j adds j's.
-      // A same-block pair is visited twice for one lambda: half each time.
-      // The FP32 pair terms are summed in FP64.
-      const float e = qq * eq + lj.x * eA - lj.y * eB;
-      const float fl = SOFT ? e + f * dredll : e;
-      const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-      if (same) {
-        if (bi == bj) fli += 0.5 * (double)fl;
-      } else {
-        fli += (double)(lamj * fl);
-      }
-    }
-    if (ENERGY) {
-      // FP64 pair energy from the exact integer displacement and FP64
-      // charges / LJ parameters; half per visit
+    // FP64 pair energy from the exact integer displacement and FP64
+    // charges / LJ parameters at the FP64 effective distance: the energy
+    // steps' sum (half per visit) and, in every step, the dU/dlambda of
+    // alchemical pairs (FP32 pair energies summed over a site's thousands
+    // of pairs drift past the 1e-3 kcal/mol dU/dlambda bar).  Only the
+    // kernel for chunks with alchemical atoms carries this FP64 code on
+    // every step, so the plain chunks' kernel keeps its registers low.
+    double epd = 0.0, sd = 1.0;
+    if (ENERGY || (ALCH && alchPair)) {
       const double2 ljd = d.vdwd[vi];
       const double qjd = (double)__uint_as_float(pj.w) + (double)slo[jl];
       const double ddx = (double)(int)(ia.p.x - pj.x) * d.ddscale[0];
       const double ddy = (double)(int)(ia.p.y - pj.y) * d.ddscale[1];
       const double ddz = (double)(int)(ia.p.z - pj.z) * d.ddscale[2];
       double rd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-      double sd = 1.0;
       if (ALCH && alchPair) {
         const double li_d = (bi & 0xFFFF) ? d.lambda[bi & 0xFFFF] : 1.0;
         const double lj_d = (bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0;
@@ -339,9 +331,22 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
           }
         }
       }
-      etile += 0.5 * sd * pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
+      epd = pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
     }
     (void)eq; (void)eA; (void)eB;
+    if (ENERGY) etile += 0.5 * sd * epd;
+    if (ALCH && alchPair) {
+      // dU/dlambda of the i-atom's block only: the visit from j adds j's.
+      // A same-block pair is visited twice for one lambda: half each time.
+      // d(s E)/ds = E + s dE/dr dr/ds (soft core), summed in FP64
+      const double fl = SOFT ? epd + (double)(f * dredll) : epd;
+      const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
+      if (same) {
+        if (bi == bj) fli += 0.5 * fl;
+      } else {
+        fli += ((bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0) * fl;
+      }
+    }
     float fr;
     if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
     else fr = f * rinv;
@@ -357,6 +362,122 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One chunk of the list: the tiles of row I it holds, against the 32
+// i-atoms in registers; the i-force partials go to the chunk's slot.
+template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
+__device__ __forceinline__ void chunk_pairs(const Dev &d, const NbConst &c, const float2 *s_vdw, int ch, int lane,
+                                            int half, int li, uint4 *sp, int2 *spr, float *slo,
+                                            unsigned char *sjx, double &eacc) {
+  const int4 cr = d.chunk[ch];
+  const int I = cr.x, t0 = cr.y, cnt = cr.z;
+  IAtom ia;
+  {
+    const int si = I * kBlockAtoms + li;
+    ia.p = d.spos[si];
+    const int2 pr = d.spar[si];
+    ia.t = pr.x;
+    ia.b = pr.y;
+    ia.q = __uint_as_float(ia.p.w) * kElectricF;
+    ia.qd = (ENERGY || ALCH) ? kElectricD * ((double)__uint_as_float(ia.p.w) + (double)d.slo[si]) : 0.0;
+    ia.lam = (ALCH && (ia.b & 0xFFFF)) ? d.lambdaf[ia.b & 0xFFFF] : 1.f;
+  }
+  const uint4 ci = d.bbc[I], hi = d.bbh[I];
+  int tJ = 0, tE = -1;
+  bool keep = false;
+  if (lane < cnt) {
+    tJ = d.tile_j[t0 + lane];
+    tE = d.tile_ex[t0 + lane];
+    keep = boxes_within(d, I, tJ, c.rc2);
+  }
+  unsigned long long act = __ballot(keep);
+  // i-partials of the whole chunk in FP32 (at most 16 tiles, a fixed
+  // order), stored once per chunk
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  double fli = 0.0;
+  // software pipeline: the j slots, exclusion word and alchemical flag of
+  // the next tile are loaded while the current tile computes
+  int k = act ? __ffsll((long long)act) - 1 : -1;
+  if (k >= 0) act &= act - 1;
+  int J = 0;
+  uint4 pj = make_uint4(0, 0, 0, 0);
+  int2 prj = make_int2(-1, 0);
+  float lo = 0.f;
+  unsigned em = 0u;
+  unsigned alchJ = 0u;
+  int tEc = -1;
+  auto fetch = [&](int kk, int &J_, uint4 &pj_, int2 &prj_, float &lo_, unsigned &em_, unsigned &al_,
+                   int &ex_) {
+    J_ = __builtin_amdgcn_readlane(tJ, kk);
+    const int ex = __builtin_amdgcn_readlane(tE, kk);
+    ex_ = ex;
+    const int sj = J_ * kBlockAtoms + li;
+    pj_ = d.spos[sj];
+    prj_ = d.spar[sj];
+    if (ENERGY || ALCH) lo_ = d.slo[sj];
+    em_ = ex >= 0 ? d.emask[ex * kBlockAtoms + li] : 0u;
+    al_ = ALCH ? d.bbh[J_].w : 0u;
+  };
+  if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ, tEc);
+  while (k >= 0) {
+    // the j-atoms near the i-box, compacted to the front of the staging
+    const bool near = half == 0 && prj.x >= 0 && near_box(d, ci, hi, pj, c.rc2);
+    const unsigned long long nm = __ballot(near);
+    const int nj = __popcll(nm);
+    if (near) {
+      const int e = __popcll(nm & ((1ull << li) - 1ull));
+      sp[e] = pj;
+      spr[e] = prj;
+      sjx[e] = (unsigned char)li;
+      if (ENERGY || ALCH) slo[e] = lo;
+    }
+    const int Jc = J, ex = tEc;
+    const unsigned emc = em, alchJc = alchJ;
+    const int kn = act ? __ffsll((long long)act) - 1 : -1;
+    if (kn >= 0) {
+      act &= act - 1;
+      fetch(kn, J, pj, prj, lo, em, alchJ, tEc);
+    }
+    wave_lds_sync();
+    double etile = 0.0;
+    const bool alchTile = ALCH && (hi.w | alchJc);
+    const bool self = I == Jc, chk = self || ex >= 0;
+#define BM_TILE(AL, SO, CK)                                                                        \
+tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK>(d, c, ia, li, half, sp, spr, slo, sjx, s_vdw, nj, emc, \
+                                          self, fx, fy, fz, fli, etile)
+    if (alchTile) {
+      if (chk) BM_TILE(ALCH, SOFT, true);
+      else BM_TILE(ALCH, SOFT, false);
+    } else {
+      if (chk) BM_TILE(false, false, true);
+      else BM_TILE(false, false, false);
+    }
+#undef BM_TILE
+    if (ENERGY) eacc += etile;
+    wave_lds_sync();
+    k = kn;
+  }
+  // both halves hold partials of the same i-atoms: (own + other) is the
+  // same sum in either half
+  fx += __shfl_xor(fx, 32, 64);
+  fy += __shfl_xor(fy, 32, 64);
+  fz += __shfl_xor(fz, 32, 64);
+  if (half == 0) {
+    float *dst = d.ichunk + (long long)ch * 3 * kBlockAtoms;
+    dst[li] = fx;
+    dst[kBlockAtoms + li] = fy;
+    dst[2 * kBlockAtoms + li] = fz;
+  }
+  if (ALCH) {
+    fli += __shfl_xor(fli, 32, 64);
+    if (half == 0 && (ia.b & 0xFFFF) && fli != 0.0) atomic_fix(&d.dudl[ia.b & 0xFFFF], fix_d(fli));
+  }
+}
+
+// Plain chunks (no alchemical atom in any of their blocks; ALCH false): one
+// per wave, waves past this step's count exit.  Chunks with alchemical
+// atoms (ALCH true, a kernel of its own: its FP64 dU/dlambda code needs
+// registers the plain kernel need not reserve): strided over the grid's
+// waves.  Neither result depends on which wave takes which chunk.
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool GTABLE>
 __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int nblk) {
   __shared__ uint4 s_pos[4][kBlockAtoms];
@@ -381,113 +502,14 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
   float *slo = s_lo[w];
   unsigned char *sjx = s_jx[w];
 
-  // one chunk per wave; waves past the chunk count of this step exit
-  {
-    const int ch = bid * 4 + w;
-    if (ch >= nchunks) return;
-    const int4 cr = d.chunk[ch];
-    const int I = cr.x, t0 = cr.y, cnt = cr.z;
-    IAtom ia;
-    {
-      const int si = I * kBlockAtoms + li;
-      ia.p = d.spos[si];
-      const int2 pr = d.spar[si];
-      ia.t = pr.x;
-      ia.b = pr.y;
-      ia.q = __uint_as_float(ia.p.w) * kElectricF;
-      ia.qd = ENERGY ? kElectricD * ((double)__uint_as_float(ia.p.w) + (double)d.slo[si]) : 0.0;
-      ia.lam = (ALCH && (ia.b & 0xFFFF)) ? d.lambdaf[ia.b & 0xFFFF] : 1.f;
-    }
-    const uint4 ci = d.bbc[I], hi = d.bbh[I];
-    int tJ = 0, tE = -1;
-    bool keep = false;
-    if (lane < cnt) {
-      tJ = d.tile_j[t0 + lane];
-      tE = d.tile_ex[t0 + lane];
-      keep = boxes_within(d, I, tJ, c.rc2);
-    }
-    unsigned long long act = __ballot(keep);
-    // i-partials of the whole chunk in FP32 (at most 16 tiles, a fixed
-    // order), stored once per chunk
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    double fli = 0.0;
-    // software pipeline: the j slots, exclusion word and alchemical flag of
-    // the next tile are loaded while the current tile computes
-    int k = act ? __ffsll((long long)act) - 1 : -1;
-    if (k >= 0) act &= act - 1;
-    int J = 0;
-    uint4 pj = make_uint4(0, 0, 0, 0);
-    int2 prj = make_int2(-1, 0);
-    float lo = 0.f;
-    unsigned em = 0u;
-    unsigned alchJ = 0u;
-    int tEc = -1;
-    auto fetch = [&](int kk, int &J_, uint4 &pj_, int2 &prj_, float &lo_, unsigned &em_, unsigned &al_,
-                     int &ex_) {
-      J_ = __builtin_amdgcn_readlane(tJ, kk);
-      const int ex = __builtin_amdgcn_readlane(tE, kk);
-      ex_ = ex;
-      const int sj = J_ * kBlockAtoms + li;
-      pj_ = d.spos[sj];
-      prj_ = d.spar[sj];
-      if (ENERGY) lo_ = d.slo[sj];
-      em_ = ex >= 0 ? d.emask[ex * kBlockAtoms + li] : 0u;
-      al_ = ALCH ? d.bbh[J_].w : 0u;
-    };
-    if (k >= 0) fetch(k, J, pj, prj, lo, em, alchJ, tEc);
-    while (k >= 0) {
-      // the j-atoms near the i-box, compacted to the front of the staging
-      const bool near = half == 0 && prj.x >= 0 && near_box(d, ci, hi, pj, c.rc2);
-      const unsigned long long nm = __ballot(near);
-      const int nj = __popcll(nm);
-      if (near) {
-        const int e = __popcll(nm & ((1ull << li) - 1ull));
-        sp[e] = pj;
-        spr[e] = prj;
-        sjx[e] = (unsigned char)li;
-        if (ENERGY) slo[e] = lo;
-      }
-      const int Jc = J, ex = tEc;
-      const unsigned emc = em, alchJc = alchJ;
-      const int kn = act ? __ffsll((long long)act) - 1 : -1;
-      if (kn >= 0) {
-        act &= act - 1;
-        fetch(kn, J, pj, prj, lo, em, alchJ, tEc);
-      }
-      wave_lds_sync();
-      double etile = 0.0;
-      const bool alchTile = ALCH && (hi.w | alchJc);
-      const bool self = I == Jc, chk = self || ex >= 0;
-#define BM_TILE(AL, SO, CK)                                                                        \
-  tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK>(d, c, ia, li, half, sp, spr, slo, sjx, s_vdw, nj, emc, \
-                                            self, fx, fy, fz, fli, etile)
-      if (alchTile) {
-        if (chk) BM_TILE(ALCH, SOFT, true);
-        else BM_TILE(ALCH, SOFT, false);
-      } else {
-        if (chk) BM_TILE(false, false, true);
-        else BM_TILE(false, false, false);
-      }
-#undef BM_TILE
-      if (ENERGY) eacc += etile;
-      wave_lds_sync();
-      k = kn;
-    }
-    // both halves hold partials of the same i-atoms: (own + other) is the
-    // same sum in either half
-    fx += __shfl_xor(fx, 32, 64);
-    fy += __shfl_xor(fy, 32, 64);
-    fz += __shfl_xor(fz, 32, 64);
-    if (half == 0) {
-      float *dst = d.ichunk + (long long)ch * 3 * kBlockAtoms;
-      dst[li] = fx;
-      dst[kBlockAtoms + li] = fy;
-      dst[2 * kBlockAtoms + li] = fz;
-    }
-    if (ALCH) {
-      fli += __shfl_xor(fli, 32, 64);
-      if (half == 0 && (ia.b & 0xFFFF) && fli != 0.0) atomic_fix(&d.dudl[ia.b & 0xFFFF], fix_d(fli));
-    }
+  const int np = fits ? *d.nplain : 0;
+  if (!ALCH) {
+    const int k = bid * 4 + w;
+    if (k >= np) return;
+    chunk_pairs<ELEC, VDW, ENERGY, false, false>(d, c, s_vdw, d.clist[k], lane, half, li, sp, spr, slo, sjx, eacc);
+  } else {
+    for (int k = np + bid * 4 + w; k < nchunks; k += 4 * nblk)
+      chunk_pairs<ELEC, VDW, ENERGY, true, SOFT>(d, c, s_vdw, d.clist[k], lane, half, li, sp, spr, slo, sjx, eacc);
   }
   if (ENERGY) {
     long long v = (long long)__double2ll_rn(eacc * kFixScaleD);
@@ -505,9 +527,12 @@ __global__ void __launch_bounds__(256) nbdirect_kernel_multi(Multi m) {
   nbdirect_kernel_body<ELEC, VDW, ENERGY, ALCH, SOFT, GTABLE>(d, b.bid, b.nblk);
 }
 
-// one wave per chunk slot of the list capacity; waves past this step's
-// chunk count exit at once
+// plain chunks: one wave per chunk slot of the list capacity (waves past
+// this step's count exit at once); chunks with alchemical atoms: a fixed
+// grid whose waves stride over them
+constexpr int kAlchGrid = 256;
 int grid_nbdirect(const Dev &d) { return (d.maxchunks + 3) / 4; }
+int grid_nbdirect_alch(const Dev &d) { return std::min(grid_nbdirect(d), kAlchGrid); }
 
 // the LJ table in LDS (0: the table is read from global memory)
 size_t lds_nbdirect(const Dev &d) {
@@ -516,9 +541,8 @@ size_t lds_nbdirect(const Dev &d) {
 }
 
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
-static void launch_t(const Dev &d, int nwg, hipStream_t st) {
-  (void)nwg;
-  const unsigned grid = (unsigned)grid_nbdirect(d);
+static void launch_t(const Dev &d, hipStream_t st) {
+  const unsigned grid = (unsigned)(ALCH ? grid_nbdirect_alch(d) : grid_nbdirect(d));
   const size_t lds = lds_nbdirect(d);
   if (lds)
     nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, false><<<grid, 256, lds, st>>>(d);
@@ -534,20 +558,23 @@ static void launch_mt(const Multi &m, size_t lds, hipStream_t st) {
     nbdirect_kernel_multi<ELEC, VDW, ENERGY, ALCH, SOFT, true><<<m.nwg, 256, 0, st>>>(m);
 }
 
+// the plain chunks, then (systems with lambda sites) the others
 template <int ELEC, int VDW>
-static void launch_ev(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st) {
-  if (!alch) {
-    if (energy) launch_t<ELEC, VDW, true, false, false>(d, nwg, st);
-    else launch_t<ELEC, VDW, false, false, false>(d, nwg, st);
-  } else if (!soft) {
-    if (energy) launch_t<ELEC, VDW, true, true, false>(d, nwg, st);
-    else launch_t<ELEC, VDW, false, true, false>(d, nwg, st);
+static void launch_ev(const Dev &d, bool energy, bool alch, bool soft, hipStream_t st) {
+  if (energy) launch_t<ELEC, VDW, true, false, false>(d, st);
+  else launch_t<ELEC, VDW, false, false, false>(d, st);
+  if (!alch) return;
+  if (!soft) {
+    if (energy) launch_t<ELEC, VDW, true, true, false>(d, st);
+    else launch_t<ELEC, VDW, false, true, false>(d, st);
   } else {
-    if (energy) launch_t<ELEC, VDW, true, true, true>(d, nwg, st);
-    else launch_t<ELEC, VDW, false, true, true>(d, nwg, st);
+    if (energy) launch_t<ELEC, VDW, true, true, true>(d, st);
+    else launch_t<ELEC, VDW, false, true, true>(d, st);
   }
 }
 
+// one launch over several members: alch false takes their plain chunks,
+// alch true the others
 template <int ELEC, int VDW>
 static void launch_mev(const Multi &m, bool energy, bool alch, bool soft, size_t lds, hipStream_t st) {
   if (!alch) {
@@ -582,18 +609,19 @@ void launch_nbdirect_multi(const Multi &m, int elec, int vdw, bool energy, bool 
 }
 
 void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st) {
+  (void)nwg;
   if (d.elecmethod == BM_PME) {
-    if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_PME, BM_VFSWITCH>(d, energy, alch, soft, nwg, st);
-    else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_PME, BM_VSWITCH>(d, energy, alch, soft, nwg, st);
-    else launch_ev<BM_PME, BM_VSHIFT>(d, energy, alch, soft, nwg, st);
+    if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_PME, BM_VFSWITCH>(d, energy, alch, soft, st);
+    else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_PME, BM_VSWITCH>(d, energy, alch, soft, st);
+    else launch_ev<BM_PME, BM_VSHIFT>(d, energy, alch, soft, st);
   } else if (d.elecmethod == BM_FSWITCH) {
-    if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_FSWITCH, BM_VFSWITCH>(d, energy, alch, soft, nwg, st);
-    else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_FSWITCH, BM_VSWITCH>(d, energy, alch, soft, nwg, st);
-    else launch_ev<BM_FSWITCH, BM_VSHIFT>(d, energy, alch, soft, nwg, st);
+    if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_FSWITCH, BM_VFSWITCH>(d, energy, alch, soft, st);
+    else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_FSWITCH, BM_VSWITCH>(d, energy, alch, soft, st);
+    else launch_ev<BM_FSWITCH, BM_VSHIFT>(d, energy, alch, soft, st);
   } else {
-    if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_FSHIFT, BM_VFSWITCH>(d, energy, alch, soft, nwg, st);
-    else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_FSHIFT, BM_VSWITCH>(d, energy, alch, soft, nwg, st);
-    else launch_ev<BM_FSHIFT, BM_VSHIFT>(d, energy, alch, soft, nwg, st);
+    if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_FSHIFT, BM_VFSWITCH>(d, energy, alch, soft, st);
+    else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_FSHIFT, BM_VSWITCH>(d, energy, alch, soft, st);
+    else launch_ev<BM_FSHIFT, BM_VSHIFT>(d, energy, alch, soft, st);
   }
 }
 

@@ -404,6 +404,11 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.nchunks, 1);
   ALLOC(d.chunk, d.maxchunks);
   ALLOC(d.ichunk, (size_t)d.maxchunks * 3 * kBlockAtoms);
+  ALLOC(d.balch, MB);
+  ALLOC(d.cplain, d.maxchunks);
+  ALLOC(d.cpos, (size_t)d.maxchunks + 1);
+  ALLOC(d.clist, d.maxchunks);
+  ALLOC(d.nplain, 1);
   ALLOC(d.ecount, MB);
   ALLOC(d.estart, MB + 1);
   ALLOC(d.nexcltiles, 1);
@@ -973,6 +978,9 @@ static int set_tile_capacity(bm_system *s, long long maxTiles) {
   if ((rc = s->realloc_dev(d.tile_i, T)) || (rc = s->realloc_dev(d.tile_j, T)) ||
       (rc = s->realloc_dev(d.tile_ex, T)) || (rc = s->realloc_dev(d.tile_flag, T)) ||
       (rc = s->realloc_dev(d.chunk, (size_t)d.maxchunks)) ||
+      (rc = s->realloc_dev(d.cplain, (size_t)d.maxchunks)) ||
+      (rc = s->realloc_dev(d.cpos, (size_t)d.maxchunks + 1)) ||
+      (rc = s->realloc_dev(d.clist, (size_t)d.maxchunks)) ||
       (rc = s->realloc_dev(d.ichunk, (size_t)d.maxchunks * 3 * blade::kBlockAtoms)))
     return rc;
   s->capacityGen++;
@@ -1138,7 +1146,7 @@ struct bm_batch {
   hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   // fused launches of a non-energy step
   MultiPlan pack, normals, bonded, tail;
-  std::vector<MultiPlan> bin, spread, fftp, fftx, nb, asmb;
+  std::vector<MultiPlan> bin, spread, fftp, fftx, nb, nba, asmb;
   std::vector<int> fftSingle;      // PME members off the FP32 plane FFT path (other sizes, MFMA FFT)
   std::vector<int> gens;           // members' capacity generations the launch plans were sized for
   ~bm_batch() {
@@ -1162,7 +1170,7 @@ struct bm_batch {
   }
   std::vector<MultiPlan *> plans() {
     std::vector<MultiPlan *> v = {&pack, &normals, &bonded, &tail};
-    for (auto *g : {&bin, &spread, &fftp, &fftx, &nb, &asmb})
+    for (auto *g : {&bin, &spread, &fftp, &fftx, &nb, &nba, &asmb})
       for (MultiPlan &p : *g) v.push_back(&p);
     return v;
   }
@@ -1193,7 +1201,8 @@ struct bm_batch {
     (void)hipEventRecord(fork2, st);
     // nbdirect first: the largest kernel takes the CUs before the second
     // stream's smaller ones fill them
-    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, p.alch, p.soft, p.lds, st);
+    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, false, false, p.lds, st);
+    for (MultiPlan &p : nba) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, true, p.soft, p.lds, st);
     (void)hipStreamWaitEvent(st2, fork2, 0);
     blade::launch_normals_multi(normals.m, st2);
     blade::launch_bonded_multi(bonded.m, false, st2);
@@ -1278,24 +1287,36 @@ static int build_plans(bm_batch *b) {
     b->fftp[g].K0 = D(b->fftp[g].members[0]).K[0];
     b->fftp[g].K1 = D(b->fftp[g].members[0]).K[1];
   }
+  // nonbonded: every member's plain chunks in launches grouped by method;
+  // the chunks with alchemical atoms of members with lambda sites in
+  // launches grouped by method and soft core
   auto nkey = [&](int k) {
     const bm_system *s = b->sys[k];
-    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, s->alch, s->soft, blade::lds_nbdirect(s->d) == 0);
+    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, blade::lds_nbdirect(s->d) == 0);
   };
+  auto akey = [&](int k) {
+    const bm_system *s = b->sys[k];
+    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, s->soft, blade::lds_nbdirect(s->d) == 0);
+  };
+  std::vector<int> alchm;
+  for (int k : all)
+    if (b->sys[k]->alch) alchm.push_back(k);
   if ((rc = make_groups(b, b->nb, all, nkey, blade::grid_nbdirect))) return rc;
-  for (MultiPlan &p : b->nb) {
-    const bm_system *s = b->sys[p.members[0]];
-    p.elec = s->d.elecmethod;
-    p.vdw = s->d.vdwmethod;
-    p.alch = s->alch;
-    p.soft = s->soft;
-    p.lds = 0;
-    if (blade::lds_nbdirect(s->d))
-      for (int k : p.members) p.lds = std::max(p.lds, blade::lds_nbdirect(D(k)));
-  }
-  auto akey = [&](int k) { return D(k).elecmethod == BM_PME ? D(k).order : 0; };
-  if ((rc = make_groups(b, b->asmb, all, akey, blade::grid_assemble))) return rc;
-  for (MultiPlan &p : b->asmb) p.order = akey(p.members[0]);
+  if ((rc = make_groups(b, b->nba, alchm, akey, blade::grid_nbdirect_alch))) return rc;
+  for (std::vector<MultiPlan> *grp : {&b->nb, &b->nba})
+    for (MultiPlan &p : *grp) {
+      const bm_system *s = b->sys[p.members[0]];
+      p.elec = s->d.elecmethod;
+      p.vdw = s->d.vdwmethod;
+      p.alch = grp == &b->nba;
+      p.soft = p.alch && s->soft;
+      p.lds = 0;
+      if (blade::lds_nbdirect(s->d))
+        for (int k : p.members) p.lds = std::max(p.lds, blade::lds_nbdirect(D(k)));
+    }
+  auto okey = [&](int k) { return D(k).elecmethod == BM_PME ? D(k).order : 0; };
+  if ((rc = make_groups(b, b->asmb, all, okey, blade::grid_assemble))) return rc;
+  for (MultiPlan &p : b->asmb) p.order = okey(p.members[0]);
   return b->upload_devs();
 }
 
