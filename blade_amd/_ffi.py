@@ -251,6 +251,7 @@ def lib():
         "bm_get_box": (i, [vp, _f64p]),
         "bm_uniform": (C.c_double, [C.c_uint64, C.c_long, C.c_uint32]),
         "bm_batch_exchange": (i, [vp, C.c_long, i, C.c_uint64, _i32p, _i32p, _f64p]),
+        "bm_batch_exchange_noalloc": (i, [vp, C.c_long, i, C.c_uint64, _i32p, _i32p, _f64p]),
         "bm_batch_destroy": (i, [vp]),
         "bm_sync": (i, [vp]),
         "bm_get_positions": (i, [vp, _f64p, _f64p]),

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -1530,6 +1531,51 @@ int bm_batch_exchange(bm_batch *b, long step, int parity, uint64_t seed, int32_t
   const int rc = batch_exchange(b, step, parity, seed, replica, accepted, dW, tmp);
   (void)hipFree(tmp);
   return rc;
+}
+
+// ---------------------------------------------------------------------
+// Replica exchange that allocates nothing (bm_batch_exchange_noalloc).
+// Its swap staging lives in a file-scope device array of
+// kExchangeScratch doubles (16 MiB, held by every device the library runs
+// on until the process ends).  Host code reaches each device's copy only
+// through the address hipGetSymbolAddress returns on that device, fetched
+// on the first call on the device and kept per device.  Every call on one
+// device shares that array: calls that overlap on one device race.  The
+// staging is written in full before it is read, so nothing is cleared.
+// ---------------------------------------------------------------------
+constexpr size_t kExchangeScratch = size_t(1) << 21;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kExchangeScratch];
+
+// this device's copy of the scratch array (the device must be current)
+static int exchange_scratch(int device, double **out) {
+  static std::mutex mu;
+  static double *ptrs[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return set_err(BM_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(BM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptrs[device] = static_cast<double *>(p);
+  }
+  *out = ptrs[device];
+  return BM_OK;
+}
+
+int bm_batch_exchange_noalloc(bm_batch *b, long step, int parity, uint64_t seed, int32_t *replica,
+                              int32_t *accepted, double *dW) {
+  if (int rc = exchange_check(b, parity)) return rc;
+  const size_t N = 3 * (size_t)b->sys[0]->d.n + b->sys[0]->d.nl;
+  if (N > kExchangeScratch)
+    return set_err(BM_EINVAL, "bm_batch_exchange_noalloc: members of " + std::to_string(N) +
+                                  " degrees of freedom exceed its scratch of " + std::to_string(kExchangeScratch));
+  if (int rc = bm_batch_sync(b)) return rc;
+  HIPCHK(hipSetDevice(b->device));
+  double *tmp = nullptr;
+  if (int rc = exchange_scratch(b->device, &tmp)) return rc;
+  return batch_exchange(b, step, parity, seed, replica, accepted, dW, tmp);
 }
 
 int bm_batch_sync(bm_batch *b) {

@@ -376,6 +376,16 @@ int bm_batch_sync(bm_batch *batch);
  * single-system library (its exchange runs over MPI, one rank per system). */
 int bm_batch_exchange(bm_batch *batch, long step, int parity, uint64_t seed,
                       int32_t *replica, int32_t *accepted, double *dW);
+/* bm_batch_exchange without any device allocation: the same exchange and
+ * results, with the swap staging in a static device array of 2^21 doubles
+ * (16 MiB per device, kept until the process ends) instead of memory
+ * allocated per call.  No counterpart in the reference; it is for callers
+ * that cannot allocate (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Members of more than 2^21 degrees of
+ * freedom (3N + nL) are rejected with BM_EINVAL.  Calls that overlap on one
+ * device share that device's array and race. */
+int bm_batch_exchange_noalloc(bm_batch *batch, long step, int parity, uint64_t seed,
+                              int32_t *replica, int32_t *accepted, double *dW);
 int bm_batch_destroy(bm_batch *batch);
 
 /* synchronise the system's stream */
