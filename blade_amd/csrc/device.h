@@ -85,11 +85,8 @@ struct Dev {
   int4 *chunk;                   // [maxchunks] (I, first tile, tiles, 0)
   float *ichunk;                 // [maxchunks][3][32] i-force partials of a chunk (FP32, fixed
                                  // order); the force assembly converts them to fixed point
-  int *balch;                    // [maxB] 1: the block holds an alchemical atom
-  int *cplain;                   // [maxchunks] 1: no block of the chunk holds an alchemical atom
-  int *cpos;                     // [maxchunks+1] exclusive scan of cplain
-  int *clist;                    // [maxchunks] chunk order: plain chunks, then the others
-  int *nplain;                   // [1] plain chunks of the current list
+  int nalchat;                   // atoms of lambda blocks (site block != 0)
+  int *alchat;                   // [nalchat] their indices
   int *ecount, *estart;          // [maxB],[maxB+1] tiles with exclusions per row
   int *nexcltiles;               // [1]
   unsigned *emask;               // [32*2*n_excl]

@@ -167,15 +167,11 @@ __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
     const int r = bb * kBlockAtoms + l;
     const bool in = bb < nbc && r < cnt;
     unsigned lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-    int alch = 0;
     if (in) {
       const int k = sorted[r];
       const unsigned u[3] = {sx[k], sy[k], (unsigned)(key[k] >> 32)};
       for (int q = 0; q < 3; q++) lo[q] = hi[q] = u[q];
-      alch = d.sb[(int)(key[k] & 0xFFFFFFFFull)] != 0;
     }
-    for (int o = 16; o > 0; o >>= 1) alch |= __shfl_xor(alch, o, 32);
-    if (bb < nbc && l == 0) d.balch[b0 + bb] = alch;
     for (int q = 0; q < 3; q++)
       for (int o = 16; o > 0; o >>= 1) {
         lo[q] = min(lo[q], (unsigned)__shfl_xor((int)lo[q], o, 32));
@@ -290,7 +286,7 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
   for (int J0 = 0; J0 < nb; J0 += 64) {
     const int J = J0 + lane;
     const bool keep = J < nb && dd_tile_keep(d, s_lo, s_hi, I, aLo, aHi, J);
-    unsigned long long m = __ballot(keep);
+    const unsigned long long m = __ballot(keep);
     if (keep) {
       int t = base + __popcll(m & below);
       d.tile_i[t] = I;
@@ -302,33 +298,6 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
   const int t0 = d.tstart[I], cnt = d.tcount[I], c0 = d.cstart[I];
   const int tc = d.tchunk;
   for (int k = lane; k * tc < cnt; k += 64) d.chunk[c0 + k] = make_int4(I, t0 + k * tc, min(tc, cnt - k * tc), 0);
-}
-
-// chunk c is plain when neither its i-block nor any of its j-blocks holds
-// an alchemical atom (one thread per chunk)
-__global__ void __launch_bounds__(256) dd_chunk_class(Dev d) {
-  if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks || c >= *d.nchunks) return;
-  const int4 cr = d.chunk[c];
-  int a = d.balch[cr.x];
-  for (int t = cr.y; t < cr.y + cr.z; t++) a |= d.balch[d.tile_j[t]];
-  d.cplain[c] = a ? 0 : 1;
-}
-
-// the chunk order of the nonbonded launches: plain chunks in ascending
-// index, then the others in ascending index (one workgroup)
-__global__ void __launch_bounds__(1024) dd_chunk_order(Dev d) {
-  if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
-  const int nc = *d.nchunks;
-  block_scan_1024(d.cplain, d.cpos, nc, d.nplain);
-  __syncthreads();
-  const int np = *d.nplain;
-  for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-    const int p = d.cpos[c];
-    d.clist[d.cplain[c] ? p : np + c - p] = c;
-  }
 }
 
 // tile (I,J) by binary search in row I's ascending J list, -1 if absent
@@ -496,8 +465,6 @@ void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st) {
   dd_tile_count<<<rows, 256, 0, st>>>(d);
   dd_scan_tiles<<<1, 1024, 0, st>>>(d);
   dd_tile_write<<<rows, 256, 0, st>>>(d);
-  dd_chunk_class<<<(d.maxchunks + 255) / 256, 256, 0, st>>>(d);
-  dd_chunk_order<<<1, 1024, 0, st>>>(d);
   if (d.nexcl) dd_excl_mark<<<(d.nexcl + 255) / 256, 256, 0, st>>>(d);
   dd_excl_rows<<<rows, 256, 0, st>>>(d);
   dd_scan_excl<<<1, 1024, 0, st>>>(d);

@@ -7,7 +7,9 @@ namespace blade {
 
 void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st);
 void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st);
-void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st);
+// the list's chunks on st; the alchemical chunks on sta (st when null)
+void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st,
+                     hipStream_t sta = nullptr);
 // PME charge grid: bins, brick spread, FFT with the convolution (the
 // potential grid is left in rgrid for the force assembly)
 void launch_pme_grid(const Dev &d, bool energy, hipStream_t st);
@@ -54,7 +56,10 @@ bool fft_plane_path(const Dev &d);
 int grid_fft_planes(const Dev &d);
 int grid_fft_x(const Dev &d);
 int grid_nbdirect(const Dev &d);
-int grid_nbdirect_alch(const Dev &d);
+int grid_nb_dudl(const Dev &d);
+// FP64 dU/dlambda of the nonbonded pairs, by alchemical atom
+void launch_nb_dudl(const Dev &d, bool soft, hipStream_t st);
+void launch_nb_dudl_multi(const Multi &m, int elec, int vdw, bool soft, hipStream_t st);
 size_t lds_nbdirect(const Dev &d);
 int grid_assemble(const Dev &d);
 int grid_tail(const Dev &d);

@@ -260,7 +260,7 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
                                            int half, const uint4 *sp, const int2 *spr, const float *slo,
                                            const unsigned char *sjx, const float2 *svdw, int nj,
                                            unsigned em, bool self, float &fx, float &fy, float &fz,
-                                           double &fli, double &etile) {
+                                           double &etile) {
   const float sx = d.fracscale[0], sy = d.fracscale[1], sz = d.fracscale[2];
   for (int jl = half; jl < nj; jl += 2) {
     const uint4 pj = sp[jl];
@@ -280,7 +280,7 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     float rinv = __frsqrt_rn(r2);
     float r = r2 * rinv;
     const float qq = ia.q * __uint_as_float(pj.w);
-    float s = 1.f, dredr = 1.f, dredll = 0.f, lamj = 1.f;
+    float s = 1.f, dredr = 1.f, lamj = 1.f;
     const int bi = ia.b, bj = prj.y;
     const bool alchPair = ALCH && (bi | bj);
     if (ALCH && alchPair) {
@@ -294,7 +294,6 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
           float re = 1.f - 0.5f * q;
           re = re * q * q * q + 0.5f;
           dredr = (3.f - 2.f * q) * q * q;
-          dredll = (re - dredr * q) * (-kSoftCoreRadius);
           r = re * rSoft;
           rinv = __builtin_amdgcn_rcpf(r);
         }
@@ -303,21 +302,16 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
     float f, eq, eA, eB;
     pair_ef<ELEC, VDW>(c, qq, lj.x, lj.y, r, rinv, f, eq, eA, eB);
     if (ALCH) f *= s;
-    // FP64 pair energy from the exact integer displacement and FP64
-    // charges / LJ parameters at the FP64 effective distance: the energy
-    // steps' sum (half per visit) and, in every step, the dU/dlambda of
-    // alchemical pairs (FP32 pair energies summed over a site's thousands
-    // of pairs drift past the 1e-3 kcal/mol dU/dlambda bar).  Only the
-    // kernel for chunks with alchemical atoms carries this FP64 code on
-    // every step, so the plain chunks' kernel keeps its registers low.
-    double epd = 0.0, sd = 1.0;
-    if (ENERGY || (ALCH && alchPair)) {
+    if (ENERGY) {
+      // FP64 pair energy from the exact integer displacement and FP64
+      // charges / LJ parameters; half per visit
       const double2 ljd = d.vdwd[vi];
       const double qjd = (double)__uint_as_float(pj.w) + (double)slo[jl];
       const double ddx = (double)(int)(ia.p.x - pj.x) * d.ddscale[0];
       const double ddy = (double)(int)(ia.p.y - pj.y) * d.ddscale[1];
       const double ddz = (double)(int)(ia.p.z - pj.z) * d.ddscale[2];
       double rd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+      double sd = 1.0;
       if (ALCH && alchPair) {
         const double li_d = (bi & 0xFFFF) ? d.lambda[bi & 0xFFFF] : 1.0;
         const double lj_d = (bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0;
@@ -331,22 +325,9 @@ __device__ __forceinline__ void tile_pairs(const Dev &d, const NbConst &c, const
           }
         }
       }
-      epd = pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
+      etile += 0.5 * sd * pair_energy_d<ELEC, VDW>(d, ia.qd * qjd, ljd.x, ljd.y, rd);
     }
     (void)eq; (void)eA; (void)eB;
-    if (ENERGY) etile += 0.5 * sd * epd;
-    if (ALCH && alchPair) {
-      // dU/dlambda of the i-atom's block only: the visit from j adds j's.
-      // A same-block pair is visited twice for one lambda: half each time.
-      // d(s E)/ds = E + s dE/dr dr/ds (soft core), summed in FP64
-      const double fl = SOFT ? epd + (double)(f * dredll) : epd;
-      const bool same = (bi & 0xFFFF0000) == (bj & 0xFFFF0000);
-      if (same) {
-        if (bi == bj) fli += 0.5 * fl;
-      } else {
-        fli += ((bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0) * fl;
-      }
-    }
     float fr;
     if (SOFT && alchPair) fr = f * dredr * __frsqrt_rn(r2);
     else fr = f * rinv;
@@ -393,7 +374,6 @@ __device__ __forceinline__ void chunk_pairs(const Dev &d, const NbConst &c, cons
   // i-partials of the whole chunk in FP32 (at most 16 tiles, a fixed
   // order), stored once per chunk
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  double fli = 0.0;
   // software pipeline: the j slots, exclusion word and alchemical flag of
   // the next tile are loaded while the current tile computes
   int k = act ? __ffsll((long long)act) - 1 : -1;
@@ -443,7 +423,7 @@ __device__ __forceinline__ void chunk_pairs(const Dev &d, const NbConst &c, cons
     const bool self = I == Jc, chk = self || ex >= 0;
 #define BM_TILE(AL, SO, CK)                                                                        \
 tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK>(d, c, ia, li, half, sp, spr, slo, sjx, s_vdw, nj, emc, \
-                                          self, fx, fy, fz, fli, etile)
+                                          self, fx, fy, fz, etile)
     if (alchTile) {
       if (chk) BM_TILE(ALCH, SOFT, true);
       else BM_TILE(ALCH, SOFT, false);
@@ -467,17 +447,9 @@ tile_pairs<ELEC, VDW, ENERGY, AL, SO, CK>(d, c, ia, li, half, sp, spr, slo, sjx,
     dst[kBlockAtoms + li] = fy;
     dst[2 * kBlockAtoms + li] = fz;
   }
-  if (ALCH) {
-    fli += __shfl_xor(fli, 32, 64);
-    if (half == 0 && (ia.b & 0xFFFF) && fli != 0.0) atomic_fix(&d.dudl[ia.b & 0xFFFF], fix_d(fli));
-  }
+
 }
 
-// Plain chunks (no alchemical atom in any of their blocks; ALCH false): one
-// per wave, waves past this step's count exit.  Chunks with alchemical
-// atoms (ALCH true, a kernel of its own: its FP64 dU/dlambda code needs
-// registers the plain kernel need not reserve): strided over the grid's
-// waves.  Neither result depends on which wave takes which chunk.
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT, bool GTABLE>
 __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int nblk) {
   __shared__ uint4 s_pos[4][kBlockAtoms];
@@ -502,15 +474,11 @@ __device__ __forceinline__ void nbdirect_kernel_body(const Dev &d, int bid, int 
   float *slo = s_lo[w];
   unsigned char *sjx = s_jx[w];
 
-  const int np = fits ? *d.nplain : 0;
-  if (!ALCH) {
-    const int k = bid * 4 + w;
-    if (k >= np) return;
-    chunk_pairs<ELEC, VDW, ENERGY, false, false>(d, c, s_vdw, d.clist[k], lane, half, li, sp, spr, slo, sjx, eacc);
-  } else {
-    for (int k = np + bid * 4 + w; k < nchunks; k += 4 * nblk)
-      chunk_pairs<ELEC, VDW, ENERGY, true, SOFT>(d, c, s_vdw, d.clist[k], lane, half, li, sp, spr, slo, sjx, eacc);
-  }
+  (void)nblk;
+  // one chunk per wave; waves past this step's chunk count exit
+  const int k = bid * 4 + w;
+  if (k >= nchunks) return;
+  chunk_pairs<ELEC, VDW, ENERGY, ALCH, SOFT>(d, c, s_vdw, k, lane, half, li, sp, spr, slo, sjx, eacc);
   if (ENERGY) {
     long long v = (long long)__double2ll_rn(eacc * kFixScaleD);
     wave_atomic_fix(&d.energy[BM_ENBDIRECT], v);
@@ -527,12 +495,128 @@ __global__ void __launch_bounds__(256) nbdirect_kernel_multi(Multi m) {
   nbdirect_kernel_body<ELEC, VDW, ENERGY, ALCH, SOFT, GTABLE>(d, b.bid, b.nblk);
 }
 
-// plain chunks: one wave per chunk slot of the list capacity (waves past
-// this step's count exit at once); chunks with alchemical atoms: a fixed
-// grid whose waves stride over them
-constexpr int kAlchGrid = 256;
+// dU/dlambda of the nonbonded pairs (the lambda derivative of
+// nbdirect/nbdirect.cu:244-275,400-415) in FP64, by alchemical atom: the
+// tile kernels leave it out, since FP32 pair energies summed over a site's
+// thousands of pairs drift past the 1e-3 kcal/mol dU/dlambda bar.  Wave w
+// takes atom alchat[w / kDudlWaves] and every kDudlWaves-th pair of tiles
+// of its block's row (a half-wave per tile, a lane per j slot); each pair
+// within rcut that is not excluded gives d(s E)/ds (E in FP64 at the FP64
+// effective distance; the soft-core term s dE/dr dr/ds with dE/dr in FP32)
+// times the lambda of the other block, or half of it within one block
+// (the pair is met again from the other atom), to the atom's block.
+constexpr int kDudlWaves = 32;
+template <int ELEC, int VDW, bool SOFT>
+__device__ __forceinline__ void nb_dudl_body(const Dev &d, int bid) {
+  const int lane = threadIdx.x & 63, half = lane >> 5, lj = lane & 31;
+  const int wv = bid * 4 + (threadIdx.x >> 6);
+  const int ai = wv / kDudlWaves, sub = wv % kDudlWaves;
+  if (ai >= d.nalchat) return;
+  if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
+  const int a = d.alchat[ai], sa = d.g2s[a], I = sa / kBlockAtoms, li = sa % kBlockAtoms;
+  const uint4 pa = d.spos[sa];
+  const int2 pra = d.spar[sa];
+  const int ba = pra.y;
+  const double qa = kElectricD * ((double)__uint_as_float(pa.w) + (double)d.slo[sa]);
+  const double la = d.lambda[ba & 0xFFFF];
+  const NbConst c = make_const(d);
+  double acc = 0.0;
+  const int t1 = d.tstart[I + 1];
+  for (int t = d.tstart[I] + 2 * sub + half; t < t1; t += 2 * kDudlWaves) {
+    const int J = d.tile_j[t], ex = d.tile_ex[t];
+    const int sj = J * kBlockAtoms + lj;
+    const int2 prj = d.spar[sj];
+    if (prj.x < 0 || sj == sa) continue;
+    if (ex >= 0 && ((d.emask[ex * kBlockAtoms + li] >> lj) & 1u)) continue;
+    const int bj = prj.y;
+    const bool same = (ba & 0xFFFF0000) == (bj & 0xFFFF0000);
+    if (same && ba != bj) continue;   // two blocks of one site do not interact
+    const uint4 pj = d.spos[sj];
+    const double dx = (double)(int)(pa.x - pj.x) * d.ddscale[0];
+    const double dy = (double)(int)(pa.y - pj.y) * d.ddscale[1];
+    const double dz = (double)(int)(pa.z - pj.z) * d.ddscale[2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r >= d.rcd) continue;
+    const double lo = (bj & 0xFFFF) ? d.lambda[bj & 0xFFFF] : 1.0;
+    const double sd = same ? la : la * lo;
+    double reff = r, dredll = 0.0;
+    if (SOFT) {
+      const double rSoft = 4.0 * (1.0 - sd);
+      if (r < rSoft) {
+        const double q = r / rSoft, re = (1.0 - 0.5 * q) * q * q * q + 0.5;
+        dredll = (re - (3.0 - 2.0 * q) * q * q * q) * (-4.0);
+        reff = re * rSoft;
+      }
+    }
+    const double2 ljd = d.vdwd[pra.x * d.ntypes + prj.x];
+    const double qq = qa * ((double)__uint_as_float(pj.w) + (double)d.slo[sj]);
+    double fl = pair_energy_d<ELEC, VDW>(d, qq, ljd.x, ljd.y, reff);
+    if (SOFT && dredll != 0.0) {
+      float f, eq, eA, eB;
+      const float rf = (float)reff;
+      pair_ef<ELEC, VDW>(c, (float)qq, (float)ljd.x, (float)ljd.y, rf, 1.f / rf, f, eq, eA, eB);
+      fl += sd * (double)f * dredll;
+    }
+    acc += same ? 0.5 * fl : lo * fl;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0 && acc != 0.0) atomic_fix(&d.dudl[ba & 0xFFFF], fix_d(acc));
+}
+
+template <int ELEC, int VDW, bool SOFT>
+__global__ void __launch_bounds__(256) nb_dudl_kernel(Dev d) { nb_dudl_body<ELEC, VDW, SOFT>(d, blockIdx.x); }
+
+template <int ELEC, int VDW, bool SOFT>
+__global__ void __launch_bounds__(256) nb_dudl_kernel_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  nb_dudl_body<ELEC, VDW, SOFT>(d, b.bid);
+}
+
+int grid_nb_dudl(const Dev &d) { return (d.nalchat * kDudlWaves + 3) / 4; }
+
+template <int ELEC, int VDW>
+static void launch_dudl_ev(const Dev &d, bool soft, hipStream_t st) {
+  const unsigned g = (unsigned)grid_nb_dudl(d);
+  if (!g) return;
+  if (soft) nb_dudl_kernel<ELEC, VDW, true><<<g, 256, 0, st>>>(d);
+  else nb_dudl_kernel<ELEC, VDW, false><<<g, 256, 0, st>>>(d);
+}
+
+template <int ELEC, int VDW>
+static void launch_dudl_mev(const Multi &m, bool soft, hipStream_t st) {
+  if (soft) nb_dudl_kernel_multi<ELEC, VDW, true><<<m.nwg, 256, 0, st>>>(m);
+  else nb_dudl_kernel_multi<ELEC, VDW, false><<<m.nwg, 256, 0, st>>>(m);
+}
+
+#define BM_METHODS(F, ...)                                                         \
+  if (elec == BM_PME) {                                                            \
+    if (vdw == BM_VFSWITCH) F<BM_PME, BM_VFSWITCH>(__VA_ARGS__);                   \
+    else if (vdw == BM_VSWITCH) F<BM_PME, BM_VSWITCH>(__VA_ARGS__);                \
+    else F<BM_PME, BM_VSHIFT>(__VA_ARGS__);                                        \
+  } else if (elec == BM_FSWITCH) {                                                 \
+    if (vdw == BM_VFSWITCH) F<BM_FSWITCH, BM_VFSWITCH>(__VA_ARGS__);               \
+    else if (vdw == BM_VSWITCH) F<BM_FSWITCH, BM_VSWITCH>(__VA_ARGS__);            \
+    else F<BM_FSWITCH, BM_VSHIFT>(__VA_ARGS__);                                    \
+  } else {                                                                         \
+    if (vdw == BM_VFSWITCH) F<BM_FSHIFT, BM_VFSWITCH>(__VA_ARGS__);                \
+    else if (vdw == BM_VSWITCH) F<BM_FSHIFT, BM_VSWITCH>(__VA_ARGS__);             \
+    else F<BM_FSHIFT, BM_VSHIFT>(__VA_ARGS__);                                     \
+  }
+
+void launch_nb_dudl(const Dev &d, bool soft, hipStream_t st) {
+  const int elec = d.elecmethod, vdw = d.vdwmethod;
+  BM_METHODS(launch_dudl_ev, d, soft, st)
+}
+
+void launch_nb_dudl_multi(const Multi &m, int elec, int vdw, bool soft, hipStream_t st) {
+  BM_METHODS(launch_dudl_mev, m, soft, st)
+}
+#undef BM_METHODS
+
+// one wave per chunk slot of the list capacity; waves past this step's
+// chunk count exit at once
 int grid_nbdirect(const Dev &d) { return (d.maxchunks + 3) / 4; }
-int grid_nbdirect_alch(const Dev &d) { return std::min(grid_nbdirect(d), kAlchGrid); }
 
 // the LJ table in LDS (0: the table is read from global memory)
 size_t lds_nbdirect(const Dev &d) {
@@ -542,7 +626,7 @@ size_t lds_nbdirect(const Dev &d) {
 
 template <int ELEC, int VDW, bool ENERGY, bool ALCH, bool SOFT>
 static void launch_t(const Dev &d, hipStream_t st) {
-  const unsigned grid = (unsigned)(ALCH ? grid_nbdirect_alch(d) : grid_nbdirect(d));
+  const unsigned grid = (unsigned)grid_nbdirect(d);
   const size_t lds = lds_nbdirect(d);
   if (lds)
     nbdirect_kernel<ELEC, VDW, ENERGY, ALCH, SOFT, false><<<grid, 256, lds, st>>>(d);
@@ -558,13 +642,13 @@ static void launch_mt(const Multi &m, size_t lds, hipStream_t st) {
     nbdirect_kernel_multi<ELEC, VDW, ENERGY, ALCH, SOFT, true><<<m.nwg, 256, 0, st>>>(m);
 }
 
-// the plain chunks, then (systems with lambda sites) the others
+// systems with lambda sites run the kernel with the lambda arithmetic
 template <int ELEC, int VDW>
 static void launch_ev(const Dev &d, bool energy, bool alch, bool soft, hipStream_t st) {
-  if (energy) launch_t<ELEC, VDW, true, false, false>(d, st);
-  else launch_t<ELEC, VDW, false, false, false>(d, st);
-  if (!alch) return;
-  if (!soft) {
+  if (!alch) {
+    if (energy) launch_t<ELEC, VDW, true, false, false>(d, st);
+    else launch_t<ELEC, VDW, false, false, false>(d, st);
+  } else if (!soft) {
     if (energy) launch_t<ELEC, VDW, true, true, false>(d, st);
     else launch_t<ELEC, VDW, false, true, false>(d, st);
   } else {
@@ -573,8 +657,7 @@ static void launch_ev(const Dev &d, bool energy, bool alch, bool soft, hipStream
   }
 }
 
-// one launch over several members: alch false takes their plain chunks,
-// alch true the others
+// one launch over several members sharing the methods and flags
 template <int ELEC, int VDW>
 static void launch_mev(const Multi &m, bool energy, bool alch, bool soft, size_t lds, hipStream_t st) {
   if (!alch) {
@@ -608,8 +691,11 @@ void launch_nbdirect_multi(const Multi &m, int elec, int vdw, bool energy, bool 
   }
 }
 
-void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st) {
+void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st, hipStream_t sta) {
   (void)nwg;
+  if (!sta) sta = st;
+  // FP64 dU/dlambda by alchemical atom on sta
+  if (alch) launch_nb_dudl(d, soft, sta);
   if (d.elecmethod == BM_PME) {
     if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_PME, BM_VFSWITCH>(d, energy, alch, soft, st);
     else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_PME, BM_VSWITCH>(d, energy, alch, soft, st);

@@ -164,10 +164,12 @@ struct bm_system {
     (void)hipStreamWaitEvent(st2, evFork, 0);
     // a non-energy step's O step runs after the assembly: its normals here
     if (kick) blade::launch_normals(d, st2);
+    // the pair kernel on the main stream; the FP64 dU/dlambda of the
+    // alchemical atoms on the side stream ahead of bonded and PME
+    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st, st2);
     blade::launch_bonded(d, energy, blade::kFamAll, st2);
     if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st2);
     (void)hipEventRecord(evJoin, st2);
-    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
     (void)hipStreamWaitEvent(st, evJoin, 0);
     blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st, kick);
   }
@@ -350,6 +352,11 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     UPLOAD(d.blockq2, bq2);
     UPLOAD(d.type, ty);
     UPLOAD(d.sb, sb);
+    std::vector<int> al;
+    for (int i = 0; i < n; i++)
+      if (sb[i]) al.push_back(i);
+    d.nalchat = (int)al.size();
+    UPLOAD(d.alchat, al);
     UPLOAD(d.mass, ms);
     ALLOC(d.qlam, n);
     std::vector<float2> vdw((size_t)t->n_types * t->n_types);
@@ -405,11 +412,6 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   ALLOC(d.nchunks, 1);
   ALLOC(d.chunk, d.maxchunks);
   ALLOC(d.ichunk, (size_t)d.maxchunks * 3 * kBlockAtoms);
-  ALLOC(d.balch, MB);
-  ALLOC(d.cplain, d.maxchunks);
-  ALLOC(d.cpos, (size_t)d.maxchunks + 1);
-  ALLOC(d.clist, d.maxchunks);
-  ALLOC(d.nplain, 1);
   ALLOC(d.ecount, MB);
   ALLOC(d.estart, MB + 1);
   ALLOC(d.nexcltiles, 1);
@@ -979,9 +981,6 @@ static int set_tile_capacity(bm_system *s, long long maxTiles) {
   if ((rc = s->realloc_dev(d.tile_i, T)) || (rc = s->realloc_dev(d.tile_j, T)) ||
       (rc = s->realloc_dev(d.tile_ex, T)) || (rc = s->realloc_dev(d.tile_flag, T)) ||
       (rc = s->realloc_dev(d.chunk, (size_t)d.maxchunks)) ||
-      (rc = s->realloc_dev(d.cplain, (size_t)d.maxchunks)) ||
-      (rc = s->realloc_dev(d.cpos, (size_t)d.maxchunks + 1)) ||
-      (rc = s->realloc_dev(d.clist, (size_t)d.maxchunks)) ||
       (rc = s->realloc_dev(d.ichunk, (size_t)d.maxchunks * 3 * blade::kBlockAtoms)))
     return rc;
   s->capacityGen++;
@@ -1147,7 +1146,7 @@ struct bm_batch {
   hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   // fused launches of a non-energy step
   MultiPlan pack, normals, bonded, tail;
-  std::vector<MultiPlan> bin, spread, fftp, fftx, nb, nba, asmb;
+  std::vector<MultiPlan> bin, spread, fftp, fftx, nb, nbd, asmb;
   std::vector<int> fftSingle;      // PME members off the FP32 plane FFT path (other sizes, MFMA FFT)
   std::vector<int> gens;           // members' capacity generations the launch plans were sized for
   ~bm_batch() {
@@ -1171,7 +1170,7 @@ struct bm_batch {
   }
   std::vector<MultiPlan *> plans() {
     std::vector<MultiPlan *> v = {&pack, &normals, &bonded, &tail};
-    for (auto *g : {&bin, &spread, &fftp, &fftx, &nb, &nba, &asmb})
+    for (auto *g : {&bin, &spread, &fftp, &fftx, &nb, &nbd, &asmb})
       for (MultiPlan &p : *g) v.push_back(&p);
     return v;
   }
@@ -1202,9 +1201,11 @@ struct bm_batch {
     (void)hipEventRecord(fork2, st);
     // nbdirect first: the largest kernel takes the CUs before the second
     // stream's smaller ones fill them
-    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, false, false, p.lds, st);
-    for (MultiPlan &p : nba) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, true, p.soft, p.lds, st);
+    for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, p.alch, p.soft, p.lds, st);
     (void)hipStreamWaitEvent(st2, fork2, 0);
+    // the FP64 dU/dlambda of the members' alchemical atoms first on the
+    // second stream
+    for (MultiPlan &p : nbd) blade::launch_nb_dudl_multi(p.m, p.elec, p.vdw, p.soft, st2);
     blade::launch_normals_multi(normals.m, st2);
     blade::launch_bonded_multi(bonded.m, false, st2);
     for (size_t g = 0; g < bin.size(); g++) {
@@ -1288,29 +1289,28 @@ static int build_plans(bm_batch *b) {
     b->fftp[g].K0 = D(b->fftp[g].members[0]).K[0];
     b->fftp[g].K1 = D(b->fftp[g].members[0]).K[1];
   }
-  // nonbonded: every member's plain chunks in launches grouped by method;
-  // the chunks with alchemical atoms of members with lambda sites in
-  // launches grouped by method and soft core
+  // nonbonded launches grouped by method and lambda flags; the FP64
+  // dU/dlambda of members with lambda sites grouped by method and soft core
   auto nkey = [&](int k) {
     const bm_system *s = b->sys[k];
-    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, blade::lds_nbdirect(s->d) == 0);
-  };
-  auto akey = [&](int k) {
-    const bm_system *s = b->sys[k];
-    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, s->soft, blade::lds_nbdirect(s->d) == 0);
+    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, s->alch, s->soft, blade::lds_nbdirect(s->d) == 0);
   };
   std::vector<int> alchm;
   for (int k : all)
     if (b->sys[k]->alch) alchm.push_back(k);
   if ((rc = make_groups(b, b->nb, all, nkey, blade::grid_nbdirect))) return rc;
-  if ((rc = make_groups(b, b->nba, alchm, akey, blade::grid_nbdirect_alch))) return rc;
-  for (std::vector<MultiPlan> *grp : {&b->nb, &b->nba})
+  auto dkey = [&](int k) {
+    const bm_system *s = b->sys[k];
+    return std::make_tuple(s->d.elecmethod, s->d.vdwmethod, s->soft);
+  };
+  if ((rc = make_groups(b, b->nbd, alchm, dkey, blade::grid_nb_dudl))) return rc;
+  for (std::vector<MultiPlan> *grp : {&b->nb, &b->nbd})
     for (MultiPlan &p : *grp) {
       const bm_system *s = b->sys[p.members[0]];
       p.elec = s->d.elecmethod;
       p.vdw = s->d.vdwmethod;
-      p.alch = grp == &b->nba;
-      p.soft = p.alch && s->soft;
+      p.alch = s->alch;
+      p.soft = s->soft;
       p.lds = 0;
       if (blade::lds_nbdirect(s->d))
         for (int k : p.members) p.lds = std::max(p.lds, blade::lds_nbdirect(D(k)));
