@@ -118,12 +118,29 @@ int blade_read_parameter_file(System *s, const char *path) {
   int combine = 0;
   std::string line;
   int lineno = 0;
+  // a CMAP grid being read: its eight types, its size and the values read
+  // so far (ngrid*ngrid values, phi-major, over as many lines as they take)
+  std::vector<std::string> cmapTypes;
+  int cmapGrid = 0, cmapRead = 0;
   try {
     while (next_line(in, line)) {
       lineno++;
       if (!line.empty() && line[0] == '*') continue;   // title
       auto t = tokens(line);
       if (t.empty()) continue;
+      if (cmapRead < cmapGrid * cmapGrid) {
+        const std::string where = std::string(path) + ":" + std::to_string(lineno);
+        for (size_t k = 0; k < t.size(); k++) {
+          if (cmapRead == cmapGrid * cmapGrid) throw ParseError{"extra values after a CMAP grid in " + where};
+          const double v = need_double(t, k, where);
+          const auto &c = cmapTypes;
+          blade_add_parameter_cmaps_fill(s, c[0].c_str(), c[1].c_str(), c[2].c_str(), c[3].c_str(), c[4].c_str(),
+                                         c[5].c_str(), c[6].c_str(), c[7].c_str(), cmapRead / cmapGrid + 1,
+                                         cmapRead % cmapGrid + 1, v);
+          cmapRead++;
+        }
+        continue;
+      }
       const std::string U = upper(t[0]);
       if (U == "END") break;
       Sec ns;
@@ -188,7 +205,18 @@ int blade_read_parameter_file(System *s, const char *path) {
                                      opt_double(t, 5, sig));
           break;
         }
-        case Sec::Cmap:    // CMAP grids are accepted and not used (CMAP terms are unsupported)
+        case Sec::Cmap: {
+          // eight atom types (two dihedrals), then the grid size; the grid's
+          // values follow on the next lines
+          const int ng = need_int(t, 8, where);
+          if (ng < 4 || ng > 60) throw ParseError{"CMAP grid must have 4 to 60 points per 360 degrees in " + where};
+          cmapTypes.assign(t.begin(), t.begin() + 8);
+          blade_add_parameter_cmaps(s, t[0].c_str(), t[1].c_str(), t[2].c_str(), t[3].c_str(), t[4].c_str(),
+                                    t[5].c_str(), t[6].c_str(), t[7].c_str(), ng);
+          cmapGrid = ng;
+          cmapRead = 0;
+          break;
+        }
         case Sec::Hbond:
         case Sec::None:
           break;
@@ -293,7 +321,7 @@ int blade_read_psf(System *s, const char *path) {
       if (!t.empty() && (need_int(t, 0, "PSF lone pairs") != 0 || (t.size() > 1 && need_int(t, 1, "PSF lone pairs") != 0)))
         throw ParseError{"PSF lone pairs (virtual sites) are not supported"};
     }
-    // CMAP cross terms: any present mark the system unsupported
+    // CMAP cross terms (two dihedrals of eight atoms each)
     while (next_line(in, line)) {
       auto t = tokens(line);
       if (t.empty()) continue;
