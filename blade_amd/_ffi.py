@@ -225,6 +225,7 @@ def lib():
         "bm_getforce_nbex": (i, [vp, C.c_uint]),
         "bm_getforce_msldbias": (i, [vp, C.c_uint]),
         "bm_holonomic_position": (i, [vp]),
+        "bm_holonomic_rectify": (i, [vp]),
         "bm_holonomic_velocity": (i, [vp]),
         "bm_msld_chain_rule": (i, [vp]),
         "bm_getforce_ewald": (i, [vp, C.c_uint]),

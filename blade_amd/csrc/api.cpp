@@ -36,6 +36,7 @@ struct System {
   Topology topo;
   bm_runparams rp{};
   bool prepared = false;
+  bool fromCheckpoint = false;   // the coordinates came from blade_read_checkpoint
   bm_system *gpu = nullptr;
   long step = 0;
   unsigned long long seed = 0xB1ADE000ull;
@@ -416,6 +417,11 @@ static int ensure_gpu(System *s) {
   soa_state(s, x, v);
   rc = bm_system_create(s->device, &s->topo.top, &s->rp, x.data(), v.data(), s->msld->theta.data(),
                         s->msld->thetaVelocity.data(), &s->gpu);
+  // supplied coordinates have their constraint groups snapped to the exact
+  // lengths (run/run.cu:606); a checkpoint's state is continued as written
+  // (it is read after the rectification there)
+  if (!rc && !s->fromCheckpoint) rc = bm_holonomic_rectify(s->gpu);
+  if (!rc) rc = bm_sync(s->gpu);
   if (rc) api_fail(s, std::string("device system: ") + bm_last_error());
   s->step = s->run->step;
   return rc;
@@ -758,6 +764,7 @@ int blade_read_checkpoint(System *s, const char *path) {
     s->msld->thetaVelocity.assign(tv.begin(), tv.end());
   }
   s->run->step = step;
+  s->fromCheckpoint = true;
   s->prepared = false;   // the box may differ from the setup's
   return BM_OK;
 }

@@ -32,6 +32,8 @@ void launch_constrain_velocities(const Dev &d, hipStream_t st);
 void launch_chain_rule(const Dev &d, hipStream_t st);
 // init: a new theta from the host (with the fix flag it sets the held lambda)
 void launch_theta_to_lambda(const Dev &d, hipStream_t st, bool init = false);
+// constraint groups at exactly their lengths before the first step
+void launch_rectify_init(const Dev &d, hipStream_t st);
 // MC barostat: back up and scale every atom, rectify constraint groups
 // (oldbox: the box before the move); restore the backup
 void launch_volume_move(const Dev &d, const double *scale, const double *oldbox, hipStream_t st);

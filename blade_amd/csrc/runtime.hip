@@ -164,11 +164,12 @@ struct bm_system {
     (void)hipStreamWaitEvent(st2, evFork, 0);
     // a non-energy step's O step runs after the assembly: its normals here
     if (kick) blade::launch_normals(d, st2);
-    // the pair kernel on the main stream; the FP64 dU/dlambda of the
-    // alchemical atoms on the side stream ahead of bonded and PME
+    // the pair kernel on the main stream; on the side stream the FP64
+    // dU/dlambda of the alchemical atoms, then the PME chain (the longer
+    // of the two side-stream parts) before the bonded terms
     blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st, st2);
-    blade::launch_bonded(d, energy, blade::kFamAll, st2);
     if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st2);
+    blade::launch_bonded(d, energy, blade::kFamAll, st2);
     (void)hipEventRecord(evJoin, st2);
     (void)hipStreamWaitEvent(st, evJoin, 0);
     blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st, kick);
@@ -938,6 +939,12 @@ int bm_update(bm_system *s, long step, int energy_step) {
 int bm_holonomic_position(bm_system *s) {
   if (!s) return BM_EINVAL;
   blade::launch_constrain_positions(s->d, s->st);
+  HIPCHK(hipGetLastError());
+  return BM_OK;
+}
+int bm_holonomic_rectify(bm_system *s) {
+  if (!s) return BM_EINVAL;
+  blade::launch_rectify_init(s->d, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }

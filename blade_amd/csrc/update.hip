@@ -766,6 +766,67 @@ __global__ void __launch_bounds__(256) upd_rectify_groups(Dev d, double bx, doub
   }
 }
 
+// ---- initial constraint rectification (holonomic/rectify.cu:13-197) -----
+// Before the first step, every constraint group's atoms other than the
+// first are put at exactly their constrained distances from the first atom
+// (in its minimum image): a branch atom along its present direction; a
+// triangle's second atom along its present direction and its third in the
+// plane of the three, on its own side of the 0-1 line.  One thread a group.
+__global__ void __launch_bounds__(256) upd_rectify_init(Dev d) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  int idx[4], na;
+  double b[3];
+  bool tri = false;
+  if (g < d.ntri) {
+    const int4 t = d.tri_i[g];
+    const double4 q = d.tri_b[g];
+    idx[0] = t.x; idx[1] = t.y; idx[2] = t.z; na = 3;
+    b[0] = q.x; b[1] = q.y; b[2] = q.z;
+    tri = true;
+  } else if ((g -= d.ntri) < d.nbr1) {
+    const int2 t = d.br1_i[g];
+    idx[0] = t.x; idx[1] = t.y; na = 2;
+    b[0] = d.br1_b[g];
+  } else if ((g -= d.nbr1) < d.nbr2) {
+    const int4 t = d.br2_i[g];
+    const double2 q = d.br2_b[g];
+    idx[0] = t.x; idx[1] = t.y; idx[2] = t.z; na = 3;
+    b[0] = q.x; b[1] = q.y;
+  } else if ((g -= d.nbr2) < d.nbr3) {
+    const int4 t = d.br3_i[g];
+    const double4 q = d.br3_b[g];
+    idx[0] = t.x; idx[1] = t.y; idx[2] = t.z; idx[3] = t.w; na = 4;
+    b[0] = q.x; b[1] = q.y; b[2] = q.z;
+  } else {
+    return;
+  }
+  const D3 p0 = getp(d.pos, d.n, idx[0]);
+  D3 dv[3], sh[3], q[3];
+  for (int k = 1; k < na; k++) {
+    const D3 pk = getp(d.pos, d.n, idx[k]);
+    dv[k - 1] = minimg(d, pk - p0);
+    sh[k - 1] = dv[k - 1] - (pk - p0);
+  }
+  if (tri) {
+    const D3 e0 = (1.0 / normd(dv[0])) * dv[0];
+    const D3 c = crossd(dv[0], dv[1]);
+    const D3 e2 = (1.0 / normd(c)) * c;
+    const D3 e1 = crossd(e2, e0);
+    const double along = (b[0] * b[0] + b[1] * b[1] - b[2] * b[2]) / (2.0 * b[0]);
+    const double across = sqrt(b[1] * b[1] - along * along);
+    q[0] = b[0] * e0;
+    q[1] = along * e0 + across * e1;
+  } else {
+    for (int k = 0; k < na - 1; k++) q[k] = (b[k] / normd(dv[k])) * dv[k];
+  }
+  for (int k = 1; k < na; k++) setp(d.pos, d.n, idx[k], (p0 + q[k - 1]) - sh[k - 1]);
+}
+
+void launch_rectify_init(const Dev &d, hipStream_t st) {
+  const int ng = d.ntri + d.nbr1 + d.nbr2 + d.nbr3;
+  if (ng) upd_rectify_init<<<(ng + 255) / 256, 256, 0, st>>>(d);
+}
+
 // restore the backup (a rejected move)
 __global__ void __launch_bounds__(256) upd_restore_positions(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -309,6 +309,15 @@ int bm_update(bm_system *sys, long step, int energy_step);
 int bm_holonomic_position(bm_system *sys);
 int bm_holonomic_velocity(bm_system *sys);
 
+/* holonomic_rectify (holonomic/rectify.cu:190-197), run by
+ * Run::dynamics_initialize (run/run.cu:606) before the first step: each
+ * constraint group's atoms other than the first put at exactly their
+ * constrained distances from the first (a branch atom along its present
+ * direction; a triangle's third atom in the plane of the three).  The
+ * blade_* layer calls it once, when it creates the device system from
+ * supplied coordinates, and not for a state read from a checkpoint. */
+int bm_holonomic_rectify(bm_system *sys);
+
 /* Msld::calc_lambda_from_theta / calc_thetaForce_from_lambdaForce
  * (msld/msld.h:134-136); the chain rule writes dU/dtheta (with the theta
  * biases of the last force evaluation) into the theta tail of the force

@@ -1590,6 +1590,44 @@ void orc_holonomic_position(const bm_topology *t, const bm_runparams *rp, double
   });
 }
 
+// holonomic_rectify (holonomic/rectify.cu:13-197), run before the first
+// step: in each constraint group the atoms other than the first are put at
+// exactly their constrained distances from the first (its minimum image).
+// A branch atom keeps its direction from the first atom.  In a triangle the
+// second atom keeps its direction and the third goes into the plane of the
+// three, on its own side of the 0-1 line, at distances b02 and b12.
+void orc_rectify(const bm_topology *t, const bm_runparams *rp, double *x) {
+  const int n = t->n_atoms;
+  const double *L = rp->box;
+  for_groups(t, [&](const Group &g, int kind) {
+    const Vec3 p0 = getp(x, n, g.a[0]);
+    Vec3 d[3], shift[3], q[3];
+    for (int k = 1; k < g.na; k++) {
+      const Vec3 pk = getp(x, n, g.a[k]);
+      d[k - 1] = minimg(pk - p0, L);
+      shift[k - 1] = d[k - 1] - (pk - p0);
+    }
+    if (kind == 0) {
+      const Vec3 e0 = (1.0 / norm(d[0])) * d[0];
+      const Vec3 c = cross(d[0], d[1]);
+      const Vec3 e2 = (1.0 / norm(c)) * c;
+      const Vec3 e1 = cross(e2, e0);
+      const double b01 = g.b0[0], b02 = g.b0[1], b12 = g.b0[2];
+      const double along = (b01 * b01 + b02 * b02 - b12 * b12) / (2.0 * b01);
+      const double across = std::sqrt(b02 * b02 - along * along);
+      q[0] = b01 * e0;
+      q[1] = along * e0 + across * e1;
+    } else {
+      for (int k = 0; k < g.na - 1; k++) q[k] = (g.b0[k] / norm(d[k])) * d[k];
+    }
+    for (int k = 1; k < g.na; k++) {
+      const Vec3 xo = (p0 + q[k - 1]) - shift[k - 1];
+      const int a = g.a[k];
+      x[a] = xo.x; x[n + a] = xo.y; x[2 * n + a] = xo.z;
+    }
+  });
+}
+
 void orc_holonomic_velocity(const bm_topology *t, const bm_runparams *rp, const double *x, double *v) {
   const int n = t->n_atoms;
   for_groups(t, [&](const Group &g, int) {

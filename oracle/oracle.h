@@ -103,6 +103,8 @@ void orc_theta_bias(const bm_topology *top, const bm_runparams *rp, const double
  * :15-211 velocity).  x/xprev/v are [3N] SoA; halfdt in AKMA. */
 void orc_holonomic_position(const bm_topology *top, const bm_runparams *rp,
                             double *x, const double *xprev, double *v, double halfdt);
+/* holonomic_rectify: constraint groups put at exactly their lengths */
+void orc_rectify(const bm_topology *t, const bm_runparams *rp, double *x);
 void orc_holonomic_velocity(const bm_topology *top, const bm_runparams *rp,
                             const double *x, double *v);
 

@@ -56,6 +56,7 @@ def lib():
         L.orc_holonomic_position.argtypes = [vp, vp, _f64p, _f64p, _f64p, C.c_double]
         L.orc_holonomic_velocity.restype = None
         L.orc_holonomic_velocity.argtypes = [vp, vp, _f64p, _f64p]
+        L.orc_rectify.argtypes = [vp, vp, _f64p]
         L.orc_philox4x32.restype = None
         L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normal.restype = C.c_double
@@ -191,6 +192,13 @@ def holonomic_velocity(top, rp, x, v):
     v = np.ascontiguousarray(v, dtype=np.float64).copy()
     lib().orc_holonomic_velocity(_addr(top), _addr(rp), _p(x), _p(v))
     return v
+
+
+def rectify(top, rp, x):
+    """positions (SoA, 3N) with every constraint group at exactly its lengths"""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib().orc_rectify(_addr(top), _addr(rp), _p(x))
+    return x
 
 
 def philox(ctr, key):

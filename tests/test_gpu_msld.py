@@ -63,7 +63,7 @@ VARIANTS = ["msld_small", "two_site_small", "friction_fixed", "theta_biases", "p
 
 def _oracle_state(sp, top, rp):
     n = sp.n
-    x = np.concatenate([system.soa(sp.pos), np.array(sp.theta0, float)])
+    x = np.concatenate([orc.rectify(top, rp, system.soa(sp.pos)), np.array(sp.theta0, float)])
     v = np.concatenate([system.soa(sp.vel), np.array(sp.theta_v if sp.theta_v is not None
                                                       else [0.0] * sp.nblocks, float)])
     lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))

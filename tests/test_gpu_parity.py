@@ -139,7 +139,8 @@ def test_domdec_bit_exact(name):
     23,558-atom box and a 60,000-atom four-site replica"""
     s = built(name)
     top, rp = s.topology(), s.runparams()
-    x0 = system.soa(spec(name).pos)
+    # the device system starts from the rectified coordinates
+    x0 = orc.rectify(top, rp, system.soa(spec(name).pos))
     posw, l2g, bc, tiles, masks = orc.domdec(top, rp, x0)
     s.run_energy()
     dev = s.device()
@@ -213,7 +214,7 @@ def test_short_trajectory_tracks_oracle(name):
     s.prepare()
     top, rp = s.topology(), s.runparams()
     n, nl = sp.n, sp.nblocks
-    x = np.concatenate([system.soa(sp.pos), np.array(sp.theta0, float)])
+    x = np.concatenate([orc.rectify(top, rp, system.soa(sp.pos)), np.array(sp.theta0, float)])
     v = np.concatenate([system.soa(sp.vel), np.array(sp.theta_v, float)])
     lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
     # the GPU wraps positions at step 0 (rebuild); do the same for the oracle

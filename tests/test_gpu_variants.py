@@ -308,6 +308,81 @@ def test_reference_water216_deck():
     assert erel <= ENERGY_TOL
 
 
+def _lengths_worst(top, P, box):
+    t = top.contents
+    worst = 0.0
+    groups = [(t.array("tri_idx", t.n_tri, 3), t.array("tri_b0", t.n_tri, 3), ((0, 1), (0, 2), (1, 2))),
+              (t.array("br1_idx", t.n_br1, 2), t.array("br1_b0", t.n_br1, 1), ((0, 1),)),
+              (t.array("br2_idx", t.n_br2, 3), t.array("br2_b0", t.n_br2, 2), ((0, 1), (0, 2))),
+              (t.array("br3_idx", t.n_br3, 4), t.array("br3_b0", t.n_br3, 3), ((0, 1), (0, 2), (0, 3)))]
+    for idx, b0, pairs in groups:
+        b0 = np.asarray(b0).reshape(len(idx), len(pairs))
+        for a, b in zip(idx, b0):
+            for (i, j), d in zip(pairs, b):
+                r = P[a[j]] - P[a[i]]
+                r -= box * np.round(r / box)
+                worst = max(worst, abs(np.linalg.norm(r) - d) / d)
+    return worst
+
+
+def test_initial_rectify_matches_oracle_then_dynamics():
+    """Coordinates whose constraint groups are off their lengths are
+    rectified when the device system is created (run/run.cu:606), bit for
+    bit as the oracle rectifies them: the reference's water216 deck with every
+    hydrogen displaced by 0.05 A, then 100 steps of dynamics that keep the
+    lengths; and the small MSLD system, whose SHAKE branches of one to
+    three hydrogens take the branch rule."""
+    from tests.test_files import BOX, CRD, PRM, PSF
+    rng = np.random.default_rng(21)
+    s = system.load_charmm(PRM, PSF, CRD, BOX, beta=1 / 3.125, rcut=9.0, rswitch=8.0)
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    n = s.n
+    P = s.positions().reshape(3, n).T.copy()
+    for o, h1, h2 in top.contents.array("tri_idx", top.contents.n_tri, 3):
+        P[h1] += rng.normal(scale=0.05, size=3)
+        P[h2] += rng.normal(scale=0.05, size=3)
+    for i in range(n):
+        s.add_coordinates_position(i + 1, *P[i])
+    box = np.array(BOX)
+    assert _lengths_worst(top, P, box) > 1e-2
+    s.run_energy()
+    xg = s.positions()
+    want = orc.domdec(top, rp, orc.rectify(top, rp, np.ascontiguousarray(P.T).reshape(-1)))[0]
+    diff = np.abs(xg - want).max()
+    print("\n[water216 deck, displaced] rectified: max |gpu - oracle| %.3e A, %d of %d coordinates bit-equal"
+          % (diff, int(np.sum(xg == want)), xg.size), flush=True)
+    assert np.array_equal(xg, want)
+    assert _lengths_worst(top, xg.reshape(3, n).T, box) < 1e-13
+    fo, do, eo, pairs = orc.forces(top, rp, xg, np.ones(1))
+    assert rel_rms(s.forces(), fo) <= FORCE_TOL
+    s.run_dynamics(100, freq_nrg=10)
+    x1 = s.positions()
+    assert np.all(np.isfinite(x1))
+    w = _lengths_worst(top, x1.reshape(3, n).T, box)
+    print("   after 100 steps: worst relative constraint error %.3e" % w, flush=True)
+    assert w < 1e-9
+    s.close()
+
+    sp = copy.deepcopy(spec("msld_small"))
+    pos = np.array(sp.pos, float)
+    pos += rng.normal(scale=0.02, size=pos.shape)
+    sp.pos = pos
+    m = system.build(sp)
+    m.prepare()
+    top, rp = m.topology(), m.runparams()
+    t = top.contents
+    assert t.n_br1 and t.n_br2 and t.n_br3 and t.n_tri
+    m.run_energy()
+    xg = m.positions()
+    want = orc.domdec(top, rp, orc.rectify(top, rp, system.soa(sp.pos)))[0]
+    diff = np.abs(xg - want).max()
+    print("[msld_small, displaced] rectified: max |gpu - oracle| %.3e A" % diff, flush=True)
+    assert np.array_equal(xg, want)
+    assert _lengths_worst(top, xg.reshape(3, sp.n).T, np.array(sp.box)) < 1e-13
+    m.close()
+
+
 def test_barostat_volume_move_restore_and_npt_run():
     """Monte Carlo barostat: a volume move scales free atoms exactly, keeps
     every constraint group's internal offsets, and the forces and energy at

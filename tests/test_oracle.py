@@ -16,10 +16,13 @@ independently of its own formulas:
   switch functions continuity of energy and force at rswitch, zero at rcut
   SETTLE/SHAKE     constraint lengths after the position step; zero bond
                    rate after the velocity projection
+  rectify          lengths, kept directions and planes, kept images; a
+                   water against its geometry built by hand
   domdec           every pair within rcut lies in a listed tile; masks mark
                    exactly the excluded pairs
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -31,6 +34,7 @@ from tests.helpers import built, spec
 KE = 332.0716
 E_BOND, E_ANGLE, E_UREY, E_DIHE, E_IMPR, E_CMAP, E_NB14, E_NBDIRECT, E_NBRECIP, E_SELF, E_EXCL, E_LAMBDA = range(12)
 E_POT = 20
+W216 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "water216")
 
 
 # ---------------------------------------------------------------------------
@@ -586,6 +590,98 @@ def test_holonomic_position_and_velocity():
         r -= box * np.round(r / box)
         worst = max(worst, abs(np.dot(r, vp[a] - vp[b])) / d)
     assert worst < 1e-9
+
+
+def _groups(t):
+    """(atoms, kind) of every constraint group, kind 0 for a triangle"""
+    g = [(tuple(a), 0) for a in t.array("tri_idx", t.n_tri, 3)]
+    g += [(tuple(a), 1) for a in t.array("br1_idx", t.n_br1, 2)]
+    g += [(tuple(a), 2) for a in t.array("br2_idx", t.n_br2, 3)]
+    g += [(tuple(a), 3) for a in t.array("br3_idx", t.n_br3, 4)]
+    return g
+
+
+def test_rectify_puts_groups_at_their_lengths():
+    """The initial rectification fixes each group's geometry completely:
+    the first atom stays, every length equals its b0, a branch atom and a
+    triangle's second atom keep their directions from the first, and the
+    third atom of a triangle stays in the plane of the three on its own side
+    of the 0-1 line.  Atoms keep their periodic image; constraint-exact
+    input comes back unchanged to rounding."""
+    name = "msld_small"
+    s = built(name)
+    top, rp = s.topology(), s.runparams()
+    t = top.contents
+    sp = spec(name)
+    n = sp.n
+    box = np.array(sp.box)
+    rng = np.random.default_rng(11)
+    x0 = system.soa(sp.pos) + rng.normal(scale=0.03, size=3 * n)
+    # some groups straddle the box faces
+    P0 = x0.reshape(3, n).T.copy()
+    groups = _groups(t)
+    assert {k for _, k in groups} == {0, 1, 2, 3}
+    for (a, _), k in zip(groups[::7], range(10**6)):
+        P0[list(a)] += box * (0.5 - P0[a[0]] / box) + (0.02 if k % 2 else -0.02)
+    x0 = np.ascontiguousarray(P0.T).reshape(-1)
+    x1 = orc.rectify(top, rp, x0)
+    P1 = x1.reshape(3, n).T
+
+    def mi(r):
+        return r - box * np.round(r / box)
+    pairs = _constraint_pairs(top)
+    worst = max(abs(np.linalg.norm(mi(P1[a] - P1[b])) - d) / d for a, b, d in pairs)
+    assert worst < 1e-13, worst
+    moved = np.abs(P1 - P0).max()
+    assert 0.01 < moved < 0.5, moved           # same image, small moves
+    grouped = set()
+    for a, kind in groups:
+        grouped.update(a)
+        assert np.array_equal(P1[a[0]], P0[a[0]])
+        d0 = [mi(P0[b] - P0[a[0]]) for b in a[1:]]
+        d1 = [mi(P1[b] - P1[a[0]]) for b in a[1:]]
+        keep = d0[:1] if kind == 0 else d0
+        for u, w in zip(keep, d1):
+            assert np.linalg.norm(np.cross(u, w)) < 1e-12 * np.linalg.norm(u) * np.linalg.norm(w) + 1e-14
+            assert np.dot(u, w) > 0
+        if kind == 0:
+            nrm = np.cross(d0[0], d0[1])
+            assert abs(np.dot(nrm, d1[1])) < 1e-12 * np.linalg.norm(nrm) * np.linalg.norm(d1[1])
+            assert np.dot(np.cross(d0[0], d1[1]), nrm) > 0
+    free = np.array([i for i in range(n) if i not in grouped], dtype=np.int64)
+    assert np.array_equal(P1[free], P0[free])
+    # exact input stays put
+    x2 = orc.rectify(top, rp, x1)
+    assert np.abs(x2 - x1).max() < 1e-12
+
+
+def test_rectify_water_triangle_closed_form():
+    """A TIP3P water rectified from a distorted start equals the exact
+    geometry built by hand in the frame the rule fixes"""
+    s = system.load_charmm(os.path.join(W216, "par_water_ions.prm"), os.path.join(W216, "minimized.psf"),
+                           os.path.join(W216, "minimized.crd"), (18.856,) * 3)
+    s.prepare()
+    top, rp = s.topology(), s.runparams()
+    t = top.contents
+    n = t.n_atoms
+    x = s.positions().copy()
+    P = x.reshape(3, n).T
+    o, h1, h2 = t.array("tri_idx", t.n_tri, 3)[5]
+    b01, b02, b12 = t.array("tri_b0", t.n_tri, 3)[5]
+    P[h1] = P[o] + np.array([1.1, 0.05, -0.02])
+    P[h2] = P[o] + np.array([-0.3, 0.8, 0.1])
+    x = np.ascontiguousarray(P.T).reshape(-1)
+    Q = orc.rectify(top, rp, x).reshape(3, n).T
+    e0 = np.array([1.1, 0.05, -0.02]) / np.linalg.norm([1.1, 0.05, -0.02])
+    e2 = np.cross([1.1, 0.05, -0.02], [-0.3, 0.8, 0.1])
+    e2 /= np.linalg.norm(e2)
+    e1 = np.cross(e2, e0)
+    # TIP3P: O-H 0.9572, H-H 1.5139 (the HOH angle, 104.52 degrees, to 5e-4)
+    assert b01 == b02 == 0.9572 and b12 == 1.5139
+    ang = 2 * math.asin(b12 / (2 * b01))
+    assert abs(math.degrees(ang) - 104.52) < 5e-4
+    np.testing.assert_allclose(Q[h1] - Q[o], 0.9572 * e0, atol=1e-12)
+    np.testing.assert_allclose(Q[h2] - Q[o], 0.9572 * (math.cos(ang) * e0 + math.sin(ang) * e1), atol=1e-12)
 
 
 # ---------------------------------------------------------------------------
