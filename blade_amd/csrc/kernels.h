@@ -7,9 +7,9 @@ namespace blade {
 
 void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st);
 void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st);
-// the list's chunks on st; the alchemical chunks on sta (st when null)
-void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st,
-                     hipStream_t sta = nullptr);
+// the pair kernel over the list's chunks (the FP64 dU/dlambda of the
+// alchemical atoms is launch_nb_dudl)
+void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st);
 // PME charge grid: bins, brick spread, FFT with the convolution (the
 // potential grid is left in rgrid for the force assembly)
 void launch_pme_grid(const Dev &d, bool energy, hipStream_t st);

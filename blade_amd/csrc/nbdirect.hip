@@ -691,11 +691,8 @@ void launch_nbdirect_multi(const Multi &m, int elec, int vdw, bool energy, bool 
   }
 }
 
-void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st, hipStream_t sta) {
+void launch_nbdirect(const Dev &d, bool energy, bool alch, bool soft, int nwg, hipStream_t st) {
   (void)nwg;
-  if (!sta) sta = st;
-  // FP64 dU/dlambda by alchemical atom on sta
-  if (alch) launch_nb_dudl(d, soft, sta);
   if (d.elecmethod == BM_PME) {
     if (d.vdwmethod == BM_VFSWITCH) launch_ev<BM_PME, BM_VFSWITCH>(d, energy, alch, soft, st);
     else if (d.vdwmethod == BM_VSWITCH) launch_ev<BM_PME, BM_VSWITCH>(d, energy, alch, soft, st);

@@ -97,8 +97,9 @@ double bspline_host(int n, double u) {
 struct bm_system {
   int device = 0;
   hipStream_t st = nullptr;
-  hipStream_t st2 = nullptr;         // side branch of the force DAG (default priority)
-  hipEvent_t evFork = nullptr, evJoin = nullptr;
+  hipStream_t st2 = nullptr;         // PME branch of the force DAG (default priority)
+  hipStream_t st3 = nullptr;         // bonded / dU/dlambda branch (default priority)
+  hipEvent_t evFork = nullptr, evJoin = nullptr, evJoin3 = nullptr;
   blade::Dev d{};
   int maxBlocks = 0;
   long long maxTiles = 0;
@@ -154,24 +155,29 @@ struct bm_system {
     return BM_OK;
   }
 
-  // all force terms: nbdirect chunk partials on the main stream while the
-  // bonded slots and the PME grid run on the side stream (they only read
-  // positions), then one per-atom assembly into the force array after both
-  // (kick: the assembly also kicks the atoms of a non-energy step).  Under
-  // stream capture the fork/join becomes two parallel graph branches.
+  // all force terms in three branches that only read positions: the pair
+  // kernel's chunk partials on the main stream, the PME chain (bins,
+  // spread, FFT and convolution) on st2, and the bonded slots with the
+  // FP64 dU/dlambda of the alchemical atoms on st3 (with the step's
+  // normals); then one per-atom assembly into the force array after all
+  // three (kick: the assembly also kicks the atoms of a non-energy step).
+  // Under stream capture the fork/join becomes three parallel graph
+  // branches.  The PME chain is a row of five dependent kernels, so the
+  // bonded work is kept off its stream.
   void force_launches(bool energy, bool kick = false) {
     (void)hipEventRecord(evFork, st);
     (void)hipStreamWaitEvent(st2, evFork, 0);
-    // a non-energy step's O step runs after the assembly: its normals here
-    if (kick) blade::launch_normals(d, st2);
-    // the pair kernel on the main stream; on the side stream the FP64
-    // dU/dlambda of the alchemical atoms, then the PME chain (the longer
-    // of the two side-stream parts) before the bonded terms
-    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st, st2);
+    (void)hipStreamWaitEvent(st3, evFork, 0);
+    blade::launch_nbdirect(d, energy, alch, soft, nwgNb, st);
     if (d.elecmethod == BM_PME) blade::launch_pme_grid(d, energy, st2);
-    blade::launch_bonded(d, energy, blade::kFamAll, st2);
+    // a non-energy step's O step runs after the assembly: its normals here
+    if (kick) blade::launch_normals(d, st3);
+    if (alch) blade::launch_nb_dudl(d, soft, st3);
+    blade::launch_bonded(d, energy, blade::kFamAll, st3);
     (void)hipEventRecord(evJoin, st2);
+    (void)hipEventRecord(evJoin3, st3);
     (void)hipStreamWaitEvent(st, evJoin, 0);
+    (void)hipStreamWaitEvent(st, evJoin3, 0);
     blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st, kick);
   }
 
@@ -184,6 +190,7 @@ struct bm_system {
   }
 
   ~bm_system() {
+    if (st3) (void)hipStreamSynchronize(st3);
     if (st2) (void)hipStreamSynchronize(st2);
     if (st) (void)hipStreamSynchronize(st);   // no graph may still run
     for (int a = 0; a < 2; a++)
@@ -195,7 +202,9 @@ struct bm_system {
       if (e) (void)hipEventDestroy(e);
     if (evFork) (void)hipEventDestroy(evFork);
     if (evJoin) (void)hipEventDestroy(evJoin);
+    if (evJoin3) (void)hipEventDestroy(evJoin3);
     for (void *p : allocs) (void)hipFree(p);
+    if (st3) (void)hipStreamDestroy(st3);
     if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -237,8 +246,10 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
   };
   if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->st3, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&s->evFork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&s->evJoin, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&s->evJoin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->evJoin3, hipEventDisableTiming) != hipSuccess)
     return fail(set_err(BM_EHIP, "stream"));
   for (auto &e : s->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(set_err(BM_EHIP, "event"));
@@ -870,6 +881,7 @@ int bm_reset_force(bm_system *s, unsigned) {
 int bm_getforce_nbdirect(bm_system *s, unsigned flags) {
   if (!s) return BM_EINVAL;
   blade::launch_nbdirect(s->d, flags & BM_CALC_ENERGY, s->alch, s->soft, s->nwgNb, s->st);
+  if (s->alch) blade::launch_nb_dudl(s->d, s->soft, s->st);
   blade::launch_force_assemble(s->d, blade::kAsmNb, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
@@ -1621,6 +1633,7 @@ int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
     HIPCHK(hipEventRecord(s->ev[1], s->st));
     blade::launch_nbdirect(s->d, false, s->alch, s->soft, s->nwgNb, s->st);
     HIPCHK(hipEventRecord(s->ev[2], s->st));
+    if (s->alch) blade::launch_nb_dudl(s->d, s->soft, s->st);
     blade::launch_bonded(s->d, false, blade::kFamAll, s->st);
     HIPCHK(hipEventRecord(s->ev[3], s->st));
     if (s->d.elecmethod == BM_PME) blade::launch_pme_grid(s->d, false, s->st);
