@@ -1158,8 +1158,8 @@ struct MultiPlan {
 struct bm_batch {
   std::vector<bm_system *> sys;
   int device = 0;
-  hipStream_t st = nullptr, st2 = nullptr;
-  hipEvent_t fork = nullptr, fork2 = nullptr, join2 = nullptr;
+  hipStream_t st = nullptr, st2 = nullptr, st3 = nullptr;
+  hipEvent_t fork = nullptr, fork2 = nullptr, join2 = nullptr, join3 = nullptr;
   std::vector<hipEvent_t> join;
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -1171,6 +1171,7 @@ struct bm_batch {
   ~bm_batch() {
     if (st) (void)hipStreamSynchronize(st);
     if (st2) (void)hipStreamSynchronize(st2);
+    if (st3) (void)hipStreamSynchronize(st3);
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++) {
         if (graph[a][b]) (void)hipGraphExecDestroy(graph[a][b]);
@@ -1178,12 +1179,13 @@ struct bm_batch {
       }
     for (auto e : join)
       if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {fork, fork2, join2})
+    for (hipEvent_t e : {fork, fork2, join2, join3})
       if (e) (void)hipEventDestroy(e);
     for (MultiPlan *p : plans()) {
       if (p->devs) (void)hipFree(p->devs);
       if (p->wg0_d) (void)hipFree(p->wg0_d);
     }
+    if (st3) (void)hipStreamDestroy(st3);
     if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -1222,22 +1224,25 @@ struct bm_batch {
     // stream's smaller ones fill them
     for (MultiPlan &p : nb) blade::launch_nbdirect_multi(p.m, p.elec, p.vdw, false, p.alch, p.soft, p.lds, st);
     (void)hipStreamWaitEvent(st2, fork2, 0);
-    // the FP64 dU/dlambda of the members' alchemical atoms first on the
-    // second stream
-    for (MultiPlan &p : nbd) blade::launch_nb_dudl_multi(p.m, p.elec, p.vdw, p.soft, st2);
-    blade::launch_normals_multi(normals.m, st2);
-    blade::launch_bonded_multi(bonded.m, false, st2);
+    (void)hipStreamWaitEvent(st3, fork2, 0);
+    // the FP64 dU/dlambda of the members' alchemical atoms, the normals and
+    // the bonded terms on the third stream, the PME chain on the second
+    for (MultiPlan &p : nbd) blade::launch_nb_dudl_multi(p.m, p.elec, p.vdw, p.soft, st3);
+    blade::launch_normals_multi(normals.m, st3);
+    blade::launch_bonded_multi(bonded.m, false, st3);
+    (void)hipEventRecord(join3, st3);
     for (size_t g = 0; g < bin.size(); g++) {
       blade::launch_pme_bin_multi(bin[g].m, bin[g].order, st2);
       blade::launch_pme_spread_multi(spread[g].m, spread[g].order, spread[g].lds, st2);
     }
-    // FFT groups in turn on st2: they finish well inside nbdirect's window,
-    // and more streams would share the four hardware queues with it
+    // FFT groups in turn on st2 (one stream per branch: the process has
+    // four hardware queues)
     for (size_t g = 0; g < fftp.size(); g++)
       blade::launch_pme_fft_multi(fftp[g].m, fftx[g].m, fftp[g].K0, fftp[g].K1, false, st2);
     for (int k : fftSingle) blade::launch_pme_fft(sys[k]->d, false, st2);
     (void)hipEventRecord(join2, st2);
     (void)hipStreamWaitEvent(st, join2, 0);
+    (void)hipStreamWaitEvent(st, join3, 0);
     for (MultiPlan &p : asmb)
       blade::launch_assemble_multi(p.m, p.order, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, true, st);
     blade::launch_tail_multi(tail.m, st);
@@ -1359,9 +1364,11 @@ int bm_batch_create(int n, bm_system *const *systems, bm_batch **out) {
   b->join.assign(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->st3, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&b->join2, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&b->join2, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->join3, hipEventDisableTiming) == hipSuccess;
   for (int k = 0; ok && k < n; k++) ok = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     delete b;
