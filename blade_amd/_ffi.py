@@ -197,6 +197,10 @@ def lib():
         "blade_set_tile_chunk": (None, [vp, i]),
         "blade_set_output_nrg": (i, [vp, C.c_char_p, i]),
         "blade_set_output_lmd": (i, [vp, C.c_char_p, i]),
+        "blade_set_output_xtc": (i, [vp, C.c_char_p, i]),
+        "blade_interpretter": (None, [C.c_char_p, vp]),
+        "blade_write_xtc_frame": (i, [C.c_char_p, i, i, i, C.c_float, C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float), C.c_float]),
         "blade_set_lmd_format": (None, [vp, i]),
         "blade_write_checkpoint": (i, [vp, C.c_char_p]),
         "blade_get_npt_stats": (i, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), _f64p]),

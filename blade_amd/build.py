@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libblade_mi355x.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 HIP_SOURCES = ["domdec.hip", "nbdirect.hip", "pme.hip", "fft_mfma.hip", "bonded.hip", "update.hip", "runtime.hip"]
-CXX_SOURCES = ["setup.cpp", "api.cpp", "files.cpp", "autotune.cpp"]
+CXX_SOURCES = ["setup.cpp", "api.cpp", "files.cpp", "autotune.cpp", "xtc.cpp", "script.cpp"]
 
 # gfx950 only; the packed FP32 instructions (v_pk_*_f32) are disabled: some
 # of their forms have given wrong results on this hardware without an error
@@ -70,6 +70,10 @@ def build_lib(jobs=8):
 
 def build_oracle():
     _run(["make", "-C", ORACLE_DIR, "-B", "liboracle.so"])
+    # the reference's xdrfile library, the checker of the XTC writer, where
+    # the reference tree is present
+    if os.path.isdir("/root/reference/src/xdr"):
+        _run(["make", "-C", ORACLE_DIR, "-B", "_ref/libxdrfile.so"])
     return os.path.join(ORACLE_DIR, "liboracle.so")
 
 

@@ -44,15 +44,19 @@ struct System {
   int fftMode = 0;
   int tileChunk = 0;
   // output files written by blade_run_dynamics (run/run.cu:572-590)
-  FILE *fpNRG = nullptr, *fpLMD = nullptr;
-  int freqNRG = 0, freqLMD = 0;
+  FILE *fpNRG = nullptr, *fpLMD = nullptr, *fpXTC = nullptr;
+  int freqNRG = 0, freqLMD = 0, freqXTC = 0;
   bool hrLMD = true;                    // text .lmd, else XDR frames
   long nptMoves = 0, nptAccepted = 0;   // barostat counters
   ~System() {
     if (fpNRG) fclose(fpNRG);
     if (fpLMD) fclose(fpLMD);
+    if (fpXTC) fclose(fpXTC);
   }
 };
+
+extern "C" int blade_xtc_frame(FILE *fp, int natoms, int step, float time, const float *box9, const float *xyz,
+                               float precision);
 
 extern "C" __attribute__((weak)) void blade_log(const char *message) { fprintf(stderr, "%s", message); }
 
@@ -69,6 +73,12 @@ namespace blade {
 // error path shared with the file readers (files.cpp)
 void api_report_error(System *s, const std::string &m) { api_fail(s, m); }
 bool api_has_structure(System *s) { return s && s->structure; }
+// what the script interpreter (script.cpp) reads of a system
+const Structure *api_structure(System *s) { return s ? s->structure.get() : nullptr; }
+bool api_has_parameters(System *s) { return s && s->parameters; }
+bool api_has_coordinates(System *s) { return s && s->coordinates; }
+bool api_has_msld(System *s) { return s && s->msld; }
+int api_reset_device(System *s);
 }  // namespace blade
 
 static void mark_unsupported(System *s, const char *what) {
@@ -427,6 +437,41 @@ static int ensure_gpu(System *s) {
   return rc;
 }
 
+// the device system's state (positions, velocities, theta, box and step)
+// back into the host setup, and the device system released, so that the
+// next energy or dynamics call builds it again with the current run
+// parameters (the reference builds its state at every dynamics call,
+// run/run.cu:572-622)
+static int reset_device(System *s) {
+  if (!s || !s->gpu) return BM_OK;
+  const int n = s->topo.n, nl = std::max(1, s->topo.top.n_blocks);
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), th(nl), tv(nl);
+  double box[3];
+  int rc = bm_get_positions(s->gpu, x.data(), th.data());
+  if (!rc) rc = bm_get_velocities(s->gpu, v.data(), tv.data());
+  if (!rc) rc = bm_get_box(s->gpu, box);
+  if (rc) {
+    api_fail(s, std::string("device state: ") + bm_last_error());
+    return rc;
+  }
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      s->coordinates->pos[3 * (size_t)i + k] = x[(size_t)k * n + i];
+      s->coordinates->vel[3 * (size_t)i + k] = v[(size_t)k * n + i];
+    }
+  for (int k = 0; k < 3; k++) s->coordinates->box[k] = box[k];
+  if (s->topo.top.n_blocks > 1) {
+    s->msld->theta.assign(th.begin(), th.end());
+    s->msld->thetaVelocity.assign(tv.begin(), tv.end());
+  }
+  bm_system_destroy(s->gpu);
+  s->gpu = nullptr;
+  s->run->step = s->step;
+  s->prepared = false;
+  s->fromCheckpoint = false;
+  return BM_OK;
+}
+
 int blade_autotune(System *s, int steps, bm_tune_result *result) {
   if (!s || s->failed) return BM_EINVAL;
   if (s->gpu) {
@@ -539,6 +584,21 @@ static void write_lmd(System *s, long step) {
   fprintf(s->fpLMD, "\n");
 }
 
+// coordinate frame (io/io.cxx:314-345): positions and box in nm, the time
+// in ps, precision 1000 per nm
+static void write_xtc(System *s, long step) {
+  const int n = s->topo.n;
+  std::vector<double> x(3 * (size_t)n);
+  double box[3];
+  if (bm_get_positions(s->gpu, x.data(), nullptr) || bm_get_box(s->gpu, box)) return;
+  std::vector<float> a(3 * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) a[3 * (size_t)i + k] = (float)(x[(size_t)k * n + i] / 10.0);
+  const float b9[9] = {(float)(box[0] / 10.0), 0, 0, 0, (float)(box[1] / 10.0), 0, 0, 0, (float)(box[2] / 10.0)};
+  const float t = (float)(step * s->run->dt / 20.454828284385908);   // AKMA time -> ps
+  blade_xtc_frame(s->fpXTC, n, (int)step, t, b9, a.data(), 1000.0f);
+}
+
 // the first step >= from that is a multiple of f (f > 0)
 static long next_multiple(long from, int f) { return ((from + f - 1) / f) * f; }
 
@@ -590,7 +650,7 @@ static int pressure_coupling(System *s, long step) {
   }
   const double V1 = V0 * scale[0] * scale[1] * scale[2];
   rc = bm_volume_move(g, scale);
-  if (rc == BM_EINVAL) return BM_OK;   // the box would pass 2*rcut: no move
+  if (rc == BM_EINVAL) return BM_OK;   // the box would pass rcut: no move
   if (!rc) rc = potential_now(g, e1);
   if (rc) return rc;
   const bm_topology &t = s->topo.top;
@@ -616,7 +676,7 @@ int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
   int rc = ensure_gpu(s);
   if (rc) return rc;
   const int fnpt = s->run->freqNPT;
-  if (s->fpNRG || s->fpLMD || fnpt > 0) {
+  if (s->fpNRG || s->fpLMD || s->fpXTC || fnpt > 0) {
     // segments that end on output steps; energies on the .nrg steps
     // (run/run.cu:545-570 prints after each step's update); a barostat
     // move before every step that is a multiple of freqNPT
@@ -631,15 +691,18 @@ int blade_run_dynamics(System *s, int nsteps, int freqNRG, int use_graph) {
       long stop = end - 1;
       if (s->fpNRG) stop = std::min(stop, next_multiple(t, s->freqNRG));
       if (s->fpLMD) stop = std::min(stop, next_multiple(t, s->freqLMD));
+      if (s->fpXTC) stop = std::min(stop, next_multiple(t, s->freqXTC));
       if (fnpt > 0) stop = std::min(stop, next_multiple(t + 1, fnpt) - 1);
       rc = bm_dynamics(s->gpu, t, stop - t + 1, fe, use_graph);
       if (!rc) rc = bm_sync(s->gpu);
       if (!rc && s->fpNRG && stop % s->freqNRG == 0) write_nrg(s, stop);
       if (!rc && s->fpLMD && stop % s->freqLMD == 0) write_lmd(s, stop);
+      if (!rc && s->fpXTC && stop % s->freqXTC == 0) write_xtc(s, stop);
       t = stop + 1;
     }
     if (s->fpNRG) fflush(s->fpNRG);
     if (s->fpLMD) fflush(s->fpLMD);
+    if (s->fpXTC) fflush(s->fpXTC);
   } else {
     rc = bm_dynamics(s->gpu, s->step, nsteps, freqNRG, use_graph);
     if (!rc) rc = bm_sync(s->gpu);
@@ -672,6 +735,20 @@ int blade_set_output_nrg(System *s, const char *path, int freq) {
   return BM_OK;
 }
 
+int blade_set_output_xtc(System *s, const char *path, int freq) {
+  if (!s || freq < 0) return BM_EINVAL;
+  if (s->fpXTC) fclose(s->fpXTC);
+  s->fpXTC = nullptr;
+  s->freqXTC = 0;
+  if (!path || !freq) return BM_OK;
+  s->fpXTC = fopen(path, "wb");
+  if (!s->fpXTC) {
+    api_fail(s, std::string("cannot open ") + path);
+    return BM_EINVAL;
+  }
+  s->freqXTC = freq;
+  return BM_OK;
+}
 int blade_set_output_lmd(System *s, const char *path, int freq) {
   if (!s || freq < 0) return BM_EINVAL;
   if (s->fpLMD) fclose(s->fpLMD);
@@ -804,3 +881,7 @@ const bm_runparams *blade_get_runparams(System *s) { return s && s->prepared ? &
 bm_system *blade_get_device_system(System *s) { return s ? s->gpu : nullptr; }
 
 }  // extern "C"
+
+namespace blade {
+int api_reset_device(System *s) { return reset_device(s); }
+}  // namespace blade

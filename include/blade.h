@@ -24,6 +24,7 @@
 #define BLADE_H
 
 #include <stdbool.h>
+#include <stdio.h>
 
 #include "blade_mi355x.h"
 
@@ -151,6 +152,13 @@ void blade_add_run_dynopts(System *system, int step, int step0, int nsteps, doub
                            int freqNPT, double volumeFluctuation, double pressure);
 void blade_run_energy(System *system);
 
+/* io/io.h:69-70: runs an input script (the reference's language: variables,
+   if/elseif/else/endif, while/endwhile, stream, selections, and the
+   parameters, structure, coordinates, msld and run classes) on the system.
+   An error is recorded on the system (blade_system_error) and ends the
+   script. */
+void blade_interpretter(const char *fnm, System *system);
+
 /* io/io.h:82 - user-replaceable logger (weak default prints to stderr) */
 void blade_log(const char *message);
 
@@ -189,6 +197,18 @@ void blade_set_domdec_frequency(System *system, int steps);
    Energies are computed on the .nrg steps.  path NULL or freq 0 closes. */
 int blade_set_output_nrg(System *system, const char *path, int freq);
 int blade_set_output_lmd(System *system, const char *path, int freq);
+/* GROMACS XTC coordinate frames (run/run.cu:299-317, io/io.cxx:314-345):
+   every `freq` steps the positions and box in nm, the time in ps, at
+   precision 1000 per nm.  path NULL or freq 0 closes. */
+int blade_set_output_xtc(System *system, const char *path, int freq);
+/* one XTC frame (positions AoS in nm, 3x3 box in nm row-major) appended
+   to an open file, or written to a file by name (append nonzero: added at
+   its end); 0 on success.  The frames are byte-identical to the
+   reference's vendored xdrfile writer (src/xdr/xdrfile_xtc.cxx). */
+int blade_xtc_frame(FILE *fp, int natoms, int step, float time, const float *box9, const float *xyz,
+                    float precision);
+int blade_write_xtc_frame(const char *path, int append, int natoms, int step, float time, const float *box9,
+                          const float *xyz, float precision);
 /* .lmd frame format, the reference's "run setvariable hrlmd" (run/run.cu:324):
    nonzero (the default) writes text lines, zero XDR frames of the block
    count and single-precision lambdas (io/io.cxx:347-370) */

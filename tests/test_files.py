@@ -50,7 +50,7 @@ def _parse_psf(path):
     return atoms, bonds, angles
 
 
-def _from_api_calls():
+def _from_api_calls(prepare=True):
     atoms, bonds, angles = _parse_psf(PSF)
     s = system.BladeSystem((0,))
     s.init_parameters()
@@ -81,7 +81,8 @@ def _from_api_calls():
     s.add_run_dynopts(0, 0, 0, 0.002 * PICOSECOND, 300.0)
     s.n = len(atoms)
     s.nblocks = 1
-    s.prepare()
+    if prepare:
+        s.prepare()
     return s
 
 

@@ -292,18 +292,21 @@ def test_holonomic_and_chain_rule_entry_points():
     assert np.max(np.abs(got - want)) < 1e-3
 
 
-def test_reference_water216_deck():
-    """the reference's test/water216 PSF/CRD/PRM (fixtures), read from files"""
+@pytest.mark.parametrize("rcut,rswitch", [(9.0, 8.0), (10.0, 8.5)])
+def test_reference_water216_deck(rcut, rswitch):
+    """the reference's test/water216 PSF/CRD/PRM (fixtures), read from
+    files; at 10 A (the deck's default cutoff) the 18.856 A box edge is
+    below twice the cutoff and pairs interact through their nearest image"""
     from tests.test_files import BOX, CRD, PRM, PSF
-    s = system.load_charmm(PRM, PSF, CRD, BOX, beta=1 / 3.125, rcut=9.0, rswitch=8.0)
+    s = system.load_charmm(PRM, PSF, CRD, BOX, beta=1 / 3.125, rcut=rcut, rswitch=rswitch)
     s.run_energy()
     top, rp = s.topology(), s.runparams()
     fo, do, eo, pairs = orc.forces(top, rp, s.positions(), np.ones(1))
     fg, eg = s.forces(), s.energy()
     frms = rel_rms(fg, fo)
     erel = abs(eg[20] - eo[20]) / abs(eo[20])
-    print("\n[water216 deck] pairs=%d force relRMS %.3e energy %.6f (oracle %.6f) rel %.3e"
-          % (pairs, frms, eg[20], eo[20], erel), flush=True)
+    print("\n[water216 deck, rcut %.0f] pairs=%d force relRMS %.3e energy %.6f (oracle %.6f) rel %.3e"
+          % (rcut, pairs, frms, eg[20], eo[20], erel), flush=True)
     assert frms <= FORCE_TOL
     assert erel <= ENERGY_TOL
 

@@ -687,18 +687,32 @@ def test_rectify_water_triangle_closed_form():
 # ---------------------------------------------------------------------------
 # neighbour list
 # ---------------------------------------------------------------------------
-def test_domdec_covers_every_pair_and_exclusion():
-    name = "msld_small"
-    s = built(name)
+def _water216_default_cutoff():
+    # the reference's test/water216/input deck: an 18.856 A box with the
+    # default 10 A cutoff, so the box edge is below twice the cutoff and
+    # pairs interact through their nearest image only
+    s = system.load_charmm(os.path.join(W216, "par_water_ions.prm"), os.path.join(W216, "minimized.psf"),
+                           os.path.join(W216, "minimized.crd"), (18.856,) * 3, rcut=10.0, rswitch=8.5)
+    s.prepare()
+    return s, s.positions(), np.array((18.856,) * 3)
+
+
+@pytest.mark.parametrize("name", ["msld_small", "water216_rcut10"])
+def test_domdec_covers_every_pair_and_exclusion(name):
+    if name == "water216_rcut10":
+        s, x0, box = _water216_default_cutoff()
+        n = s.n
+    else:
+        s = built(name)
+        sp = spec(name)
+        n, x0, box = sp.n, system.soa(sp.pos), np.array(sp.box)
     top, rp = s.topology(), s.runparams()
-    sp = spec(name)
-    n = sp.n
-    posw, l2g, bc, tiles, masks = orc.domdec(top, rp, system.soa(sp.pos))
+    posw, l2g, bc, tiles, masks = orc.domdec(top, rp, x0)
     slot = np.empty(n, dtype=int)
     slot[l2g[l2g >= 0]] = np.nonzero(l2g >= 0)[0]
     tile_of = {(int(i), int(j)): k for k, (i, j) in enumerate(tiles)}
+    assert len(tile_of) == len(tiles)          # each directed block pair once
     P = posw.reshape(3, n).T
-    box = np.array(sp.box)
     rc = rp.contents.rcut
     d = P[:, None, :] - P[None, :, :]
     d -= box * np.round(d / box)
