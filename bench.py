@@ -33,6 +33,7 @@ FREQ_NRG = 1000
 BATCH_STEPS, BATCH_WARMUP = 300, 30   # the default line's batch figure
 PREC_STEPS, PREC_WARMUP = 1000, 100   # the default line's precision-form figure
 COMB_STEPS, COMB_WARMUP = 400, 40     # the default line's combined-form figure
+DHFR_STEPS, DHFR_WARMUP = 3000, 200   # the default line's DHFR deck figure
 CAL_BYTES = 256 << 20
 
 
@@ -101,6 +102,7 @@ def main():
                     help="no autotuning: the FFT form of --fft, the cadence of --freq-domdec, the chunk of --tile-chunk")
     ap.add_argument("--freq-domdec", type=int, default=10, help="list rebuild cadence without autotuning")
     ap.add_argument("--tile-chunk", type=int, default=16, help="tiles per nonbonded work unit without autotuning")
+    ap.add_argument("--no-dhfr", action="store_true", help="skip the reference's DHFR deck figure")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave out the 32-system batched figure that the default line carries")
     a = ap.parse_args()
@@ -277,6 +279,11 @@ def main():
                                 "config": bl["config"]}
         except Exception as e:   # the main figure stands; say why the batch one is missing
             line["batch_32"] = {"error": str(e)[:300]}
+    if not a.no_dhfr:
+        try:
+            line["dhfr_deck"] = dhfr_run(DHFR_STEPS, DHFR_WARMUP)
+        except Exception as e:
+            line["dhfr_deck"] = {"error": str(e)[:300]}
     if not a.no_precision:
         try:
             line["precision_92224"] = precision_run(PREC_STEPS, PREC_WARMUP)
@@ -304,6 +311,64 @@ def _timed_ms(s, steps, warmup, graph=True):
     check(L.bm_dynamics(dev, 1 + warmup, steps, FREQ_NRG, int(graph)), "timed steps")
     check(L.bm_sync(dev), "sync")
     return (time.perf_counter() - t1) * 1e3 / steps
+
+
+def dhfr_run(steps, warmup):
+    """The reference's own benchmark deck, test/dhfr/input (DHFR in water,
+    23,558 atoms, CHARMM22, rcut 9 A with a 7.5 A switch, PME order 4 on
+    64^3, gamma 5/ps, SHAKE tolerance 1e-8), set up by the script
+    interpreter from the committed fixtures exactly as written (its
+    500,000-step run and output files left out), then graph-replayed and
+    timed like the main figure, at the deck's settings (no tuning)."""
+    import gzip
+    import shutil
+    import tempfile
+    from blade_amd import system
+    from blade_amd._ffi import check
+    gold = os.path.join(ROOT, "tests", "golden", "dhfr")
+    tmp = tempfile.mkdtemp(prefix="dhfr")
+    cwd = os.getcwd()
+    try:
+        t = os.path.join(tmp, "test", "dhfr")
+        run = os.path.join(tmp, "run")
+        os.makedirs(t)
+        os.makedirs(run)
+        for f in ("5dfr.psf", "5dfr.crd", "par_all22_prot.inp"):
+            with gzip.open(os.path.join(gold, f + ".gz"), "rb") as src, open(os.path.join(t, f), "wb") as dst:
+                shutil.copyfileobj(src, dst)
+        deck = open(os.path.join(gold, "input")).read()
+        k = deck.index("run dynamics")
+        setup = deck[:k] + ("run setvariable nsteps 1\nrun setvariable freqxtc 0\nrun setvariable freqnrg 0\n"
+                            "run setvariable freqlmd 0\nrun setvariable fnmcpo {}\nrun dynamics\n"
+                            .format(os.path.join(tmp, "start.cpt")))
+        open(os.path.join(run, "setup.inp"), "w").write(setup)
+        os.chdir(run)
+        t0 = time.time()
+        s = system.BladeSystem((0,))
+        s.L.blade_interpretter(b"setup.inp", s.h)
+        err = s.L.blade_system_error(s.h).decode()
+        if err:
+            raise RuntimeError(err)
+        log("dhfr deck set up in %.1f s" % (time.time() - t0))
+        L, dev = s.L, s.device()
+        rp = s.runparams().contents
+        check(L.bm_capture_graphs(dev), "graph capture")
+        check(L.bm_dynamics(dev, 1, warmup, FREQ_NRG, 1), "warmup")
+        check(L.bm_sync(dev), "sync")
+        t1 = time.perf_counter()
+        check(L.bm_dynamics(dev, 1 + warmup, steps, FREQ_NRG, 1), "timed steps")
+        check(L.bm_sync(dev), "sync")
+        ms = (time.perf_counter() - t1) * 1e3 / steps
+        s.close()
+        return {"metric": "ns/day at 2 fs, the reference's DHFR benchmark deck (test/dhfr/input)",
+                "value": round(0.002 * 86400.0 / ms, 2), "unit": "ns/day", "ms_per_step": round(ms, 5),
+                "steps": steps, "warmup": warmup, "atoms": 23558,
+                "config": {"rcut_A": rp.rcut, "switch_A": rp.rswitch, "pme_grid": list(rp.grid),
+                           "pme_order": rp.order, "list_rebuild_steps": rp.freq_domdec or 10,
+                           "energy_every": FREQ_NRG}}
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):

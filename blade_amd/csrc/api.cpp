@@ -48,7 +48,11 @@ struct System {
   int freqNRG = 0, freqLMD = 0, freqXTC = 0;
   bool hrLMD = true;                    // text .lmd, else XDR frames
   long nptMoves = 0, nptAccepted = 0;   // barostat counters
+  // the script interpreter's state (script.cpp), released with the system
+  void *script = nullptr;
+  void (*scriptRelease)(void *) = nullptr;
   ~System() {
+    if (script && scriptRelease) scriptRelease(script);
     if (fpNRG) fclose(fpNRG);
     if (fpLMD) fclose(fpLMD);
     if (fpXTC) fclose(fpXTC);
@@ -79,6 +83,10 @@ bool api_has_parameters(System *s) { return s && s->parameters; }
 bool api_has_coordinates(System *s) { return s && s->coordinates; }
 bool api_has_msld(System *s) { return s && s->msld; }
 int api_reset_device(System *s);
+void *&api_script_state(System *s, void (*release)(void *)) {
+  s->scriptRelease = release;
+  return s->script;
+}
 }  // namespace blade
 
 static void mark_unsupported(System *s, const char *what) {

@@ -37,6 +37,7 @@ bool api_has_parameters(System *s);
 bool api_has_coordinates(System *s);
 bool api_has_msld(System *s);
 int api_reset_device(System *s);
+void *&api_script_state(System *s, void (*release)(void *));
 }  // namespace blade
 
 namespace {
@@ -186,22 +187,39 @@ struct File {
   std::vector<Frame> stack;
 };
 
+// what a system's scripts share from one blade_interpretter call to the
+// next (the reference keeps it in its System: variables, selections, the
+// Run and Msld objects)
+struct ScriptState {
+  std::map<std::string, std::string> vars;
+  std::map<std::string, std::vector<char>> sel;
+  size_t selLimit = 100;
+  RunVars run;
+  MsldFlags ms;
+  std::vector<double> builtWith;   // run physics of the current device system
+  bool outputsOpen = false;
+  bool cpiRead = false;
+  long velDraws = 0;
+};
+
 class Interp {
  public:
-  explicit Interp(System *s) : s_(s) {}
+  Interp(System *s, ScriptState &st)
+      : s_(s), vars_(st.vars), sel_(st.sel), selLimit_(st.selLimit), run_(st.run), ms_(st.ms),
+        builtWith_(st.builtWith), outputsOpen_(st.outputsOpen), cpiRead_(st.cpiRead), velDraws_(st.velDraws) {}
   void run_file(const std::string &path);
 
  private:
   System *s_;
-  std::map<std::string, std::string> vars_;
-  std::map<std::string, std::vector<char>> sel_;
-  size_t selLimit_ = 100;
-  RunVars run_;
-  MsldFlags ms_;
-  std::vector<double> builtWith_;   // run physics of the current device system
-  bool outputsOpen_ = false;
-  bool cpiRead_ = false;
-  long velDraws_ = 0;
+  std::map<std::string, std::string> &vars_;
+  std::map<std::string, std::vector<char>> &sel_;
+  size_t &selLimit_;
+  RunVars &run_;
+  MsldFlags &ms_;
+  std::vector<double> &builtWith_;
+  bool &outputsOpen_;
+  bool &cpiRead_;
+  long &velDraws_;
   std::vector<File> files_;
 
   std::string substitute(const std::string &line) const;
@@ -869,7 +887,9 @@ void Interp::cmd_run(Tokens &t) {
 
 extern "C" void blade_interpretter(const char *fnm, System *system) {
   if (!system || !fnm) return;
-  Interp in(system);
+  void *&slot = blade::api_script_state(system, [](void *p) { delete static_cast<ScriptState *>(p); });
+  if (!slot) slot = new ScriptState();
+  Interp in(system, *static_cast<ScriptState *>(slot));
   try {
     in.run_file(fnm);
   } catch (const ScriptError &e) {

@@ -57,3 +57,54 @@ def test_water216_deck_verbatim(tmp_path, monkeypatch):
     assert abs(ke.mean() / ke_exp - 1) < 0.05
     assert -11.0 < epot.mean() < -8.5
     s.close()
+
+
+def test_dhfr_deck_parity_and_dynamics(tmp_path, monkeypatch):
+    """The reference's DHFR benchmark deck (test/dhfr/input: 23,558 atoms,
+    CHARMM22, rcut 9, PME order 4 on 64^3, gamma 5/ps, SHAKE tolerance
+    1e-8) through the interpreter.  Its set-up lines run as written; the
+    forces and energy of the rectified start are checked against the FP64
+    oracle; then the deck's run runs with its step count cut from 500,000
+    to 2,000 (10,000 atoms' worth of protein and 7,023 waters stay
+    together, the energy stays finite)."""
+    from oracle import oracle as orc
+    from tests.helpers import rel_rms
+    from tests.test_script import dhfr_tree
+    run = dhfr_tree(tmp_path)
+    monkeypatch.chdir(run)
+    deck = (tmp_path / "test" / "dhfr" / "input").read_text()
+    assert "run setvariable nsteps 500000" in deck
+    short = deck.replace("run setvariable nsteps 500000", "run setvariable nsteps 2000")
+    k = short.index("run dynamics")
+    (run / "setup.inp").write_text(short[:k] + "run energy\n")
+    s = system.BladeSystem((0,))
+    print("\n[dhfr deck] set-up and run energy", flush=True)
+    assert interpret(s, run / "setup.inp") == ""
+    s.n, s.nblocks = 23558, 1
+    top, rp = s.topology(), s.runparams()
+    x = s.positions()
+    fo, do, eo, pairs = orc.forces(top, rp, x, np.ones(1))
+    fg, eg = s.forces(), s.energy()
+    frms = rel_rms(fg, fo)
+    erel = abs(eg[20] - eo[20]) / abs(eo[20])
+    print("   pairs %d force relRMS %.3e Epot %.4f (oracle %.4f) rel %.3e" % (pairs, frms, eg[20], eo[20], erel),
+          flush=True)
+    assert frms <= 1e-5 and erel <= 1e-6
+    (run / "dyn.inp").write_text(short[k:])
+    print("[dhfr deck] 2,000 steps", flush=True)
+    assert interpret(s, run / "dyn.inp") == ""
+    x1 = s.positions().reshape(3, 23558).T
+    assert np.isfinite(x1).all()
+    # the energy of the final state (the deck's .nrg cadence, 50,000 steps,
+    # wrote only step 0)
+    (run / "final.inp").write_text("run energy\n")
+    assert interpret(s, run / "final.inp") == ""
+    e = s.energy()
+    assert np.isfinite(e).all()
+    ke_exp = 0.5 * (3 * 23558 - 3 * 7023 - (458 + 2 * 233 + 3 * 99)) * KB * 298.0
+    print("   after 2,000 steps: Epot %.1f (start %.1f) KE %.1f (equipartition %.1f)"
+          % (e[20], eg[20], e[21], ke_exp), flush=True)
+    assert abs(e[20] - eg[20]) < 0.05 * abs(eg[20])
+    if e[21] != 0:
+        assert abs(e[21] / ke_exp - 1) < 0.1
+    s.close()

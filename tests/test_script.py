@@ -32,6 +32,22 @@ def deck_tree(tmp):
     return run
 
 
+def dhfr_tree(tmp):
+    """the reference's test/dhfr deck (23,558 atoms: DHFR in water, CHARMM22,
+    kept gzip-compressed as fixtures) unpacked in the test/ layout; returns
+    the directory to run from"""
+    import gzip
+    t = tmp / "test" / "dhfr"
+    t.mkdir(parents=True)
+    for f in ("5dfr.psf", "5dfr.crd", "par_all22_prot.inp"):
+        with gzip.open(os.path.join(GOLD, "dhfr", f + ".gz"), "rb") as src, open(t / f, "wb") as dst:
+            shutil.copyfileobj(src, dst)
+    shutil.copy(os.path.join(GOLD, "dhfr", "input"), t / "input")
+    run = tmp / "run"
+    run.mkdir()
+    return run
+
+
 def interpret(s, path):
     s.L.blade_interpretter(str(path).encode(), s.h)
     return s.L.blade_system_error(s.h).decode()
@@ -187,3 +203,22 @@ msld removescaling bond angle
             continue
         n = getattr(t, cnt)
         assert np.array_equal(t.array(name, n, w), tb.array(name, n, w)), name
+
+
+def test_dhfr_deck_sets_up(tmp_path, monkeypatch):
+    """the reference's DHFR benchmark deck up to its dynamics: CHARMM22
+    protein parameters, the 23,558-atom PSF and CRD, SHAKE on bonds to
+    hydrogen (7,023 waters as rigid triangles)"""
+    run = dhfr_tree(tmp_path)
+    monkeypatch.chdir(run)
+    deck = (tmp_path / "test" / "dhfr" / "input").read_text().splitlines()
+    k = [i for i, l in enumerate(deck) if l.strip().startswith("run dynamics")][0]
+    (run / "setup.inp").write_text("\n".join(deck[:k]) + "\n")
+    s = system.BladeSystem((0,))
+    assert interpret(s, run / "setup.inp") == ""
+    s.n, s.nblocks = 23558, 1
+    s.prepare()
+    t = s.topology().contents
+    assert t.n_atoms == 23558 and t.n_tri == 7023
+    assert t.n_br1 + t.n_br2 + t.n_br3 > 0 and t.n_dihes > 0 and t.n_imprs > 0
+    assert tuple(s.runparams().contents.box) == (62.23, 62.23, 62.23)
