@@ -63,6 +63,7 @@ class Topology(C.Structure):
         ("n_direst", C.c_int32), ("direst_idx", _i32p), ("direst_block", _i32p), ("direst_n", _i32p),
         ("direst_par", _f64p),
         ("n_resd", C.c_int32), ("resd_idx", _i32p), ("resd_par", _f64p),
+        ("bond_soft", _i32p), ("angle_soft", _i32p), ("dihe_soft", _i32p), ("impr_soft", _i32p),
     ]
 
     def array(self, name, count, width=1):
@@ -86,7 +87,7 @@ class RunParams(C.Structure):
         ("fnex", C.c_double), ("soft_core", C.c_int32), ("soft_core14", C.c_int32),
         ("soft_bond_exp", C.c_double), ("box", C.c_double * 3), ("freq_domdec", C.c_int32),
         ("cull_pad", C.c_double), ("seed", C.c_uint64), ("fft_mode", C.c_int32),
-        ("tile_chunk", C.c_int32),
+        ("tile_chunk", C.c_int32), ("soft_bond_alpha", C.c_double), ("soft_bond_power", C.c_double),
     ]
 
 

@@ -281,17 +281,19 @@ void blade_add_msld_termscaling(System *s, int sb, int su, int sa, int sd, int s
   t[0] = sb; t[1] = su; t[2] = sa; t[3] = sd; t[4] = si; t[5] = sc;
 }
 void blade_add_msld_flags(System *s, double gamma, double fnex, int useSoftCore, int useSoftCore14,
-                          int msldEwaldType, double kRestraint, double, double, double softNotBondExponent,
-                          int fix) {
+                          int msldEwaldType, double kRestraint, double softBondRadius, double softBondExponent,
+                          double softNotBondExponent, int fix) {
   auto &m = *s->msld;
   m.gamma = gamma;
+  m.softBondRadius = softBondRadius;
+  m.softBondExponent = softBondExponent;
+  m.softNotBondExponent = softNotBondExponent;
   m.kRestraint = kRestraint;
   m.fnex = fnex;
   m.useSoftCore = useSoftCore;
   m.useSoftCore14 = useSoftCore14;
   m.msldEwaldType = msldEwaldType;
   m.fix = fix;
-  (void)softNotBondExponent;
 }
 // theta biases, per site (msld.cu:1675-1695): the first call sizes the
 // table with `sites` entries, later calls fill entry i (1-based)
@@ -328,7 +330,8 @@ void blade_set_msld_piecewise_constraint(System *s, int do_imp, double width, do
 }
 // soft bonds (msld.cu:1714-1721) are recorded; a system that has any is
 // rejected when its terms are derived
-void blade_add_msld_softbond(System *s, int, int) { s->msld->softBondCount++; }
+// msld/msld.cu:1714: 1-based atoms of a soft bond
+void blade_add_msld_softbond(System *s, int i, int j) { s->msld->softBonds.push_back({i - 1, j - 1}); }
 void blade_add_msld_bias(System *s, int i, int j, int type, double l0, double k, int n) {
   if (type <= 0 || type > 12) { api_fail(s, "type of variable bias is not a recognized type"); return; }
   s->msld->varBias.push_back(VarBias{i - 1, j - 1, type, l0, k, n});

@@ -59,6 +59,15 @@ __device__ __forceinline__ void scale_dudl_d(const Dev &d, const Sd &s, double e
     if (s.b1) atomic_fix(&d.dudl[s.b1], fix_d(s.l0 * e));
   }
 }
+// lambda weight of a term's energy E0, E = w E0 with P = l0 l1: w = P, or
+// P^q for a soft angle, dihedral or improper (bonded.cu:239-256,408-425);
+// dw = dw/dP
+struct Wd { double w, dw; };
+__device__ __forceinline__ Wd weight_d(const Dev &d, const Sd &s, bool soft) {
+  const double P = s.l0 * s.l1;
+  if (!soft) return Wd{P, 1.0};
+  return Wd{pow(P, d.snb_exp), d.snb_exp * pow(P, d.snb_exp - 1.0)};
+}
 
 __device__ __forceinline__ float erfc_pair(float x) {
   float ex = __expf(-x * x);
@@ -423,11 +432,24 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       const double2 p = d.bond_p[t];
       const double db = r - p.y;
       const double et = 0.5 * p.x * db * db;
-      const double fb = p.x * db * s.l0 * s.l1;
-      scale_dudl_d(d, s, et);
+      double fb;
+      if (d.bond_soft && d.bond_soft[t]) {
+        // soft bond (bonded.cu:81-107): Q = P^p, E = Q E0 g with
+        // g = 1 / (1 + (1 - Q) alpha db^2)
+        const double P = s.l0 * s.l1, Q = pow(P, d.sb_pow);
+        const double g = 1.0 / (1.0 + (1.0 - Q) * d.sb_alpha * db * db);
+        const double dgdr = -2.0 * g * g * (1.0 - Q) * d.sb_alpha * db;
+        const double dgdQ = g * g * d.sb_alpha * db * db;
+        fb = Q * (p.x * db * g + et * dgdr);
+        scale_dudl_d(d, s, et * d.sb_pow * pow(P, d.sb_pow - 1.0) * (g + Q * dgdQ));
+        e = Q * et * g;
+      } else {
+        fb = p.x * db * s.l0 * s.l1;
+        scale_dudl_d(d, s, et);
+        e = et * s.l0 * s.l1;
+      }
       put_d(d, d.sl_bond + 2 * t, fb / r, dr);
       put_d(d, d.sl_bond + 2 * t + 1, -fb / r, dr);
-      e = et * s.l0 * s.l1;
       // bonds and Urey-Bradley may share a workgroup: split by index
       if (t >= d.nbond12) {
         if (energy) atomic_fix(&d.energy[BM_EUREY], fix_d(e));
@@ -446,17 +468,18 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       const double mc = sqrt(ddot(cr, cr));
       const double th = atan2(mc, dp);
       const Sd s = scale2d(d, d.angle_sb[t]);
+      const Wd wt = weight_d(d, s, d.angle_soft && d.angle_soft[t]);
       const double2 p = d.angle_p[t];
       const double dth = th - p.y;
       const double et = 0.5 * p.x * dth * dth;
-      const double fa = p.x * dth * s.l0 * s.l1;
-      scale_dudl_d(d, s, et);
+      const double fa = p.x * dth * wt.w;
+      scale_dudl_d(d, s, wt.dw * et);
       const D3v fi = dscale(fa / (mc * ddot(rij, rij)), dcross(rij, cr));
       const D3v fk = dscale(-fa / (mc * ddot(rkj, rkj)), dcross(rkj, cr));
       put_v(d, d.sl_angle + 3 * t, fi);
       put_v(d, d.sl_angle + 3 * t + 2, fk);
       put_v(d, d.sl_angle + 3 * t + 1, dscale(-1.0, dadd(fi, fk)));
-      e = et * s.l0 * s.l1;
+      e = et * wt.w;
     }
   } else if (bid < L.b_nb14) {
     const bool impr = bid >= L.b_impr;
@@ -485,8 +508,9 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
         et = 0.5 * p.x * dphi * dphi;
       }
       const Sd s = scale2d(d, impr ? d.impr_sb[t] : d.dihe_sb[t]);
-      fphi *= s.l0 * s.l1;
-      scale_dudl_d(d, s, et);
+      const Wd wt = weight_d(d, s, p.w != 0.0);
+      fphi *= wt.w;
+      scale_dudl_d(d, s, wt.dw * et);
       const double rjk2 = ddot(rjk, rjk), r = sqrt(rjk2);
       const D3v fi = dscale(-fphi * r / ddot(m, m), m);
       const D3v fl = dscale(-fphi * r / ddot(nv, nv), nv);
@@ -497,7 +521,7 @@ __device__ __forceinline__ void bonded_body(const Dev &d, const BondedLaunch &L,
       put_v(d, s0 + 3, dscale(-1.0, fl));
       put_v(d, s0 + 2, dsub(fl, fj));
       put_v(d, s0 + 1, dsub(fj, fi));
-      e = et * s.l0 * s.l1;
+      e = et * wt.w;
     }
   } else if (bid < L.b_nbex) {
     slot = BM_ENB14;

@@ -125,7 +125,12 @@ struct Dev {
   int2 *bond_i; int2 *bond_sb; double2 *bond_p;           // (kb, b0)
   int4 *angle_i; int2 *angle_sb; double2 *angle_p;        // (k, t0)
   int4 *dihe_i; int2 *dihe_sb; double4 *dihe_p;           // (k, n, d0)
-  int4 *impr_i; int2 *impr_sb; double4 *impr_p;           // (k, n, d0)
+  int4 *impr_i; int2 *impr_sb; double4 *impr_p;           // (k, n, d0, soft)
+  // MSLD soft terms: per-term flags of bonds (with Urey-Bradley) and
+  // angles (null when none is soft; dihedrals and impropers carry theirs in
+  // p.w), soft-bond alpha and power, and the power of other soft terms
+  int *bond_soft, *angle_soft;
+  double sb_alpha, sb_pow, snb_exp;
   int2 *nb14_i; int2 *nb14_sb; double4 *nb14_p;           // (qq, c12, c6, e14fac)
   int2 *nbex_i; int2 *nbex_sb; double *nbex_p;            // qq
   // bonded force slots: every term stores its per-atom forces (fixed

@@ -531,6 +531,18 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       bp[b] = make_double2(t->bond_kb[b], t->bond_b0[b]);
     }
     UPLOAD(d.bond_i, bi); UPLOAD(d.bond_sb, bsb); UPLOAD(d.bond_p, bp);
+    d.sb_alpha = rp->soft_bond_alpha;
+    d.sb_pow = rp->soft_bond_power;
+    d.snb_exp = rp->soft_bond_exp;
+    d.bond_soft = d.angle_soft = nullptr;
+    if (t->bond_soft) {
+      std::vector<int> f(t->bond_soft, t->bond_soft + d.nbond);
+      UPLOAD(d.bond_soft, f);
+    }
+    if (t->angle_soft) {
+      std::vector<int> f(t->angle_soft, t->angle_soft + t->n_angles);
+      UPLOAD(d.angle_soft, f);
+    }
     d.nangle = t->n_angles;
     std::vector<int4> ai(d.nangle);
     std::vector<int2> asb(d.nangle);
@@ -548,13 +560,14 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       const double *kk = pass ? t->impr_k : t->dihe_k;
       const int *nn = pass ? t->impr_n : t->dihe_n;
       const double *d0 = pass ? t->impr_d0 : t->dihe_d0;
+      const int *sf = pass ? t->impr_soft : t->dihe_soft;
       std::vector<int4> di(cnt);
       std::vector<int2> dsb(cnt);
       std::vector<double4> dp(cnt);
       for (int q = 0; q < cnt; q++) {
         di[q] = make_int4(idx[4 * q], idx[4 * q + 1], idx[4 * q + 2], idx[4 * q + 3]);
         dsb[q] = make_int2(sbv[2 * q], sbv[2 * q + 1]);
-        dp[q] = make_double4(kk[q], (double)nn[q], d0[q], 0.0);
+        dp[q] = make_double4(kk[q], (double)nn[q], d0[q], sf && sf[q] ? 1.0 : 0.0);
       }
       if (pass) { d.nimpr = cnt; UPLOAD(d.impr_i, di); UPLOAD(d.impr_sb, dsb); UPLOAD(d.impr_p, dp); }
       else { d.ndihe = cnt; UPLOAD(d.dihe_i, di); UPLOAD(d.dihe_sb, dsb); UPLOAD(d.dihe_p, dp); }

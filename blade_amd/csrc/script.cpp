@@ -169,7 +169,7 @@ struct MsldFlags {
   double gamma_ps = 1.0, fnex = 5.5, kRestraint = 59.2;
   bool softCore = false, softCore14 = false, fix = false;
   int ewaldType = 2;
-  double softBondRadius = 0.1, softBondExponent = 2, softNotBondExponent = 1;
+  double softBondRadius = 1.0, softBondExponent = 2, softNotBondExponent = 1;
   double kCR1 = 0, kCR2 = 0, kCR3 = 0, q0CR3 = 0, wCR3 = 0;
   bool piecewise = false;
   double wellWidth = 1.0, wellK = 100.0, edgeK = 0, edgeN = 2, edgeAlpha = 10, edgePhi = 0;

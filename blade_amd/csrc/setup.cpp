@@ -60,13 +60,28 @@ struct Scaler {
       for (int a : g)
         if (a >= 0 && a < (int)restrained.size()) restrained[a] = 1;
   }
+  // Msld::check_soft (msld/msld.cu:478-495): the term holds both atoms of
+  // a soft bond
+  bool soft(const int *idx, int nat) const {
+    for (const auto &b : m.softBonds) {
+      bool f0 = false, f1 = false;
+      for (int k = 0; k < nat; k++) {
+        f0 = f0 || idx[k] == b[0];
+        f1 = f1 || idx[k] == b[1];
+      }
+      if (f0 && f1) return true;
+    }
+    return false;
+  }
   // Msld::bonded_scaling (msld/msld.cu:509-553): distinct blocks of the
   // atoms in descending order, at most nsc of them.  A type named by
   // removescaling keeps an atom's block only when every atom of the term in
-  // that block is restrained (no soft bonds in this build).
-  void bonded(const int *idx, int nat, int type, int nsc, int *sb) const {
+  // that block is restrained, or when the term is soft.  Returns whether it
+  // is soft.
+  bool bonded(const int *idx, int nat, int type, int nsc, int *sb) const {
     int block[5] = {m.blockCount, 0, 0, 0, 0};
-    bool scale = m.scaleTerms[type];
+    const bool isSoft = soft(idx, nat);
+    bool scale = m.scaleTerms[type] || isSoft;
     for (int i = 1; i < nsc + 2; i++)
       for (int j = 0; j < nat; j++) {
         int ab = m.atomBlock[idx[j]];
@@ -82,6 +97,7 @@ struct Scaler {
           fail("illegal MSLD scaling between two blocks of the same site");
     if (block[nsc + 1] != 0) fail("too many lambda scalings in one bonded term");
     for (int i = 0; i < nsc; i++) sb[i] = merge_site_block(m.lambdaSite[block[i + 1]], block[i + 1]);
+    return isSoft;
   }
   // Msld::nonbonded_scaling (msld.cu:555-577)
   void nonbonded(const int *idx, int nat, int *sb) const {
@@ -123,7 +139,8 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
   t = Topology();
   t.n = n;
   if ((int)m.atomBlock.size() != n) fail("msld atom assignment size mismatch");
-  if (m.softBondCount) fail("soft bonds are not supported");
+  for (const auto &b : m.softBonds)
+    if (b[0] < 0 || b[0] >= n || b[1] < 0 || b[1] >= n) fail("soft bond atom out of range");
   Scaler sc{m};
   auto type = [&](int i) { return s.atoms[i].atomTypeName; };
   auto isH = [&](int i) { return !type(i).empty() && type(i)[0] == 'H'; };
@@ -148,7 +165,7 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     auto it = p.bonds.find(canon({type(idx[0]), type(idx[1])}));
     if (it == p.bonds.end()) fail("no bond parameter for " + type(idx[0]) + " " + type(idx[1]));
     int sb[2];
-    sc.bonded(idx, 2, 0, 2, sb);
+    t.bondSoft.push_back(sc.bonded(idx, 2, 0, 2, sb));
     t.bondIdx.insert(t.bondIdx.end(), idx, idx + 2);
     t.bondSb.insert(t.bondSb.end(), sb, sb + 2);
     t.bondKb.push_back(it->second.kb);
@@ -161,8 +178,8 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     auto it = p.angles.find(canon({type(idx[0]), type(idx[1]), type(idx[2])}));
     if (it == p.angles.end()) fail("no angle parameter for " + type(idx[0]) + " " + type(idx[1]) + " " + type(idx[2]));
     int sb[2], sbu[2];
-    sc.bonded(idx, 3, 2, 2, sb);
-    sc.bonded(idx, 3, 1, 2, sbu);
+    t.angleSoft.push_back(sc.bonded(idx, 3, 2, 2, sb));
+    const bool softU = sc.bonded(idx, 3, 1, 2, sbu);
     t.angleIdx.insert(t.angleIdx.end(), idx, idx + 3);
     t.angleSb.insert(t.angleSb.end(), sb, sb + 2);
     t.angleK.push_back(it->second.kangle);
@@ -174,6 +191,7 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
       t.bondSb.insert(t.bondSb.end(), sbu, sbu + 2);
       t.bondKb.push_back(it->second.kureyb);
       t.bondB0.push_back(it->second.ureyb0);
+      t.bondSoft.push_back(softU);
     }
   }
   // dihedrals with X-X wildcard fallback (potential.cxx:647-705)
@@ -184,8 +202,9 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     if (it == p.dihes.end()) it = p.dihes.find(canon({"X", ty[1], ty[2], "X"}));
     if (it == p.dihes.end()) fail("no dihedral parameter for " + canon(ty));
     int sb[2];
-    sc.bonded(idx, 4, 3, 2, sb);
+    const bool soft = sc.bonded(idx, 4, 3, 2, sb);
     for (auto &dp : it->second) {
+      t.diheSoft.push_back(soft);
       t.diheIdx.insert(t.diheIdx.end(), idx, idx + 4);
       t.diheSb.insert(t.diheSb.end(), sb, sb + 2);
       t.diheK.push_back(dp.kdih);
@@ -238,7 +257,7 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     }
     if (!ip) fail("no improper parameter for " + canon(ty));
     int sb[2];
-    sc.bonded(idx, 4, 4, 2, sb);
+    t.imprSoft.push_back(sc.bonded(idx, 4, 4, 2, sb));
     t.imprIdx.insert(t.imprIdx.end(), idx, idx + 4);
     t.imprSb.insert(t.imprSb.end(), sb, sb + 2);
     t.imprK.push_back(ip->kimp);
@@ -475,6 +494,9 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
   t.pwK = m.wellK;
   t.edgeK = m.edgeK; t.edgeN = m.edgeN; t.edgeAlpha = m.edgeAlpha; t.edgePhi = m.edgePhi;
   t.fixLambda = m.fix ? 1 : 0;
+  t.anySoft = false;
+  for (const auto *v : {&t.bondSoft, &t.angleSoft, &t.diheSoft, &t.imprSoft})
+    for (int f : *v) t.anySoft = t.anySoft || f;
   // restraints (Potential::initialize copies the structure's lists,
   // potential.cxx:1620-1660)
   if (s.virtualSites) fail("virtual sites are not supported");
@@ -541,6 +563,10 @@ const bm_topology *Topology::view() {
   v.dihe_idx = diheIdx.data(); v.dihe_sb = diheSb.data(); v.dihe_k = diheK.data(); v.dihe_d0 = diheD0.data(); v.dihe_n = diheN.data();
   v.n_imprs = (int)imprK.size();
   v.impr_idx = imprIdx.data(); v.impr_sb = imprSb.data(); v.impr_k = imprK.data(); v.impr_d0 = imprD0.data(); v.impr_n = imprN.data();
+  if (anySoft) {
+    v.bond_soft = bondSoft.data(); v.angle_soft = angleSoft.data();
+    v.dihe_soft = diheSoft.data(); v.impr_soft = imprSoft.data();
+  }
   v.n_nb14 = (int)nb14qq.size();
   v.nb14_idx = nb14Idx.data(); v.nb14_sb = nb14Sb.data(); v.nb14_qq = nb14qq.data();
   v.nb14_c12 = nb14c12.data(); v.nb14_c6 = nb14c6.data(); v.nb14_e14fac = nb14e14.data();
@@ -611,7 +637,9 @@ void build_runparams(const RunSetup &r, const MsldSetup &m, const Coordinates &c
   rp.fnex = m.fnex;
   rp.soft_core = m.useSoftCore;
   rp.soft_core14 = m.useSoftCore14;
-  rp.soft_bond_exp = 1.0;
+  rp.soft_bond_exp = m.softNotBondExponent;
+  rp.soft_bond_alpha = 1.0 / (m.softBondRadius * m.softBondRadius);
+  rp.soft_bond_power = m.softBondExponent;
   rp.freq_domdec = r.freqDomdec > 0 ? r.freqDomdec : 10;
   rp.fft_mode = r.fftMode;
   rp.tile_chunk = r.tileChunk;

@@ -94,7 +94,8 @@ struct MsldSetup {
   std::vector<std::vector<int>> atomRestraints;   // 0-based atoms of each group
   double kRestraint = 59.2;                       // kcal/mol/A^2 (msld.cu:83)
   double kCR1 = 0, kCR2 = 0, kCR3 = 0, q0CR3 = 0, wCR3 = 0;   // charge restraints
-  int softBondCount = 0;
+  std::vector<std::array<int, 2>> softBonds;      // 0-based atom pairs
+  double softBondRadius = 1.0, softBondExponent = 2.0, softNotBondExponent = 1.0;   // msld.cu:89-91
   // theta biases (msld.cu:1675-1695): per-site collective (k, n) and
   // independent (k) biases, sized by their first setter call
   std::vector<double> thetaCollK, thetaCollN, thetaIndeK;
@@ -139,6 +140,7 @@ struct Topology {
   std::vector<int> angleIdx, angleSb; std::vector<double> angleK, angleT0;
   std::vector<int> diheIdx, diheSb, diheN; std::vector<double> diheK, diheD0;
   std::vector<int> imprIdx, imprSb, imprN; std::vector<double> imprK, imprD0;
+  std::vector<int> bondSoft, angleSoft, diheSoft, imprSoft; bool anySoft = false;
   std::vector<int> nb14Idx, nb14Sb; std::vector<double> nb14qq, nb14c12, nb14c6, nb14e14;
   std::vector<int> nbexIdx, nbexSb; std::vector<double> nbexqq;
   std::vector<int> exclIdx;

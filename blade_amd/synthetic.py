@@ -106,6 +106,8 @@ class Spec:
                                                       # uses each block's atom-charge sum, as the reference)
     charge_restraints: tuple = None                   # (k1, k2, k3, q0, w) on Q = sum lambda_b q_b
     soft_core14: bool = False                         # soft-core 1-4 pairs (useSoftCore14)
+    soft_bonds: list = field(default_factory=list)    # MSLD soft bonds, 0-based atom pairs
+    soft_bond_params: tuple = (1.0, 2.0, 1.0)         # softBondRadius A, softBondExponent, softNotBondExponent
     ewald_type: int = 2                               # msldEwaldType: 1 on, 2 ex, 3 nn
     block_friction: list = None                       # per-block theta friction, 1/AKMA time (None = msld_gamma)
     block_fixed: list = None                          # per-block fixed flags (lambda held at theta)

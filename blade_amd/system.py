@@ -347,8 +347,11 @@ def build(spec, gpus=(0,)):
                                          spec.theta_mass[b], spec.fixed_bias[b],
                                          spec.block_charge[b] if getattr(spec, "block_charge", None) else 0.0)
         s.add_msld_flags(spec.msld_gamma, spec.fnex, int(spec.soft_core), int(getattr(spec, "soft_core14", False)),
-                         int(getattr(spec, "ewald_type", 2)), getattr(spec, "k_restraint", 59.2), 1.0, 2.0, 1.0,
+                         int(getattr(spec, "ewald_type", 2)), getattr(spec, "k_restraint", 59.2),
+                         *getattr(spec, "soft_bond_params", (1.0, 2.0, 1.0)),
                          int(getattr(spec, "fix_lambda", False)))
+        for i, j in getattr(spec, "soft_bonds", None) or []:
+            s.L.blade_add_msld_softbond(s.h, int(i) + 1, int(j) + 1)
         s.add_msld_termscaling(*[int(x) for x in spec.scale_terms])
         for b, g in enumerate(getattr(spec, "block_friction", None) or []):
             if g is not None:

@@ -227,6 +227,15 @@ typedef struct bm_topology {
   int32_t n_resd;
   const int32_t *resd_idx;         /* [4n] i1 i2 j1 j2                     */
   const double *resd_par;          /* [4n] ci cj rdist kdist               */
+  /* MSLD soft bonds (msld/msld.cu:404-428,478-495; bonded/bonded.cu:81-107,
+     239-256, 408-425; setter msld.cu:1714): a bond, Urey-Bradley, angle,
+     dihedral or improper term holding both atoms of a soft bond is soft.
+     It keeps its blocks' lambda scaling where removescaling names its
+     type, and with P = lambda_0 lambda_1 a soft bond or Urey-Bradley term
+     is E = P^p E0 / (1 + (1 - P^p) alpha (r - b0)^2) (runparams
+     soft_bond_alpha, soft_bond_power), a soft angle, dihedral or improper
+     E = P^q E0 (runparams soft_bond_exp).  NULL when no term is soft. */
+  const int32_t *bond_soft, *angle_soft, *dihe_soft, *impr_soft;   /* [n] 0 or 1 */
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */
@@ -243,7 +252,7 @@ typedef struct bm_runparams {
   double shake_tol;
   double fnex;
   int32_t soft_core, soft_core14;
-  double soft_bond_exp;            /* softNotBondExponent, unused: no soft terms */
+  double soft_bond_exp;            /* softNotBondExponent q of soft angles, dihedrals, impropers */
   double box[3];                   /* orthorhombic box edges, A */
   int32_t freq_domdec;             /* list rebuild cadence (steps) */
   double cull_pad;                 /* list padding, A */
@@ -253,6 +262,8 @@ typedef struct bm_runparams {
                                       of 16 from 16 to 128, else BM_ENOTSUP) */
   int32_t tile_chunk;              /* directed tiles per nonbonded work unit (one
                                       wave), 1..16; 0 => 16 */
+  double soft_bond_alpha;          /* 1/softBondRadius^2, A^-2 */
+  double soft_bond_power;          /* softBondExponent p of soft bonds */
 } bm_runparams;
 
 /* ---------------------------------------------------------------------

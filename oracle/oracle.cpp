@@ -527,6 +527,17 @@ inline void scale_dudl(const Scale2 &s, double e, double *dudl) {
   }
 }
 
+// lambda weight of a term's energy E0: E = w E0, dE/dP = dw E0 with
+// P = l0 l1; a soft angle, dihedral or improper takes w = P^q
+// (bonded/bonded.cu:239-256, 408-425)
+struct Weight { double w, dw; };
+inline Weight weight(const Scale2 &s, bool soft, double q) {
+  const double P = s.l0 * s.l1;
+  if (!soft) return {P, 1.0};
+  return {std::pow(P, q), q * std::pow(P, q - 1)};
+}
+inline void weight_dudl(const Scale2 &s, double dEdP, double *dudl) { scale_dudl(s, dEdP, dudl); }
+
 void bonded(const bm_topology *t, const bm_runparams *rp, const double *pos,
             const double *lambda, double *force, double *dudl, double *energy) {
   const int n = t->n_atoms;
@@ -537,12 +548,26 @@ void bonded(const bm_topology *t, const bm_runparams *rp, const double *pos,
     double r = norm(dr);
     Scale2 s = scale2(&t->bond_sb[2 * b], lambda);
     double db = r - t->bond_b0[b];
-    double e = 0.5 * t->bond_kb[b] * db * db;
-    double f = t->bond_kb[b] * db * s.l0 * s.l1;
-    scale_dudl(s, e, dudl);
+    double e0 = 0.5 * t->bond_kb[b] * db * db;
+    double f, e;
+    if (t->bond_soft && t->bond_soft[b]) {
+      // soft bond (bonded/bonded.cu:81-107): with P = l0 l1 and Q = P^p,
+      // E = Q E0 g, g = 1 / (1 + (1 - Q) alpha db^2)
+      const double P = s.l0 * s.l1, p = rp->soft_bond_power, a = rp->soft_bond_alpha;
+      const double Q = std::pow(P, p), g = 1.0 / (1.0 + (1.0 - Q) * a * db * db);
+      const double dgdr = -2.0 * g * g * (1.0 - Q) * a * db;
+      const double dgdQ = g * g * a * db * db;
+      e = Q * e0 * g;
+      f = Q * (t->bond_kb[b] * db * g + e0 * dgdr);
+      weight_dudl(s, e0 * p * std::pow(P, p - 1) * (g + Q * dgdQ), dudl);
+    } else {
+      e = e0 * s.l0 * s.l1;
+      f = t->bond_kb[b] * db * s.l0 * s.l1;
+      scale_dudl(s, e0, dudl);
+    }
     addf(force, n, i, (f / r) * dr);
     addf(force, n, j, (-f / r) * dr);
-    energy[b < t->n_bonds12 ? BM_EBOND : BM_EUREY] += e * s.l0 * s.l1;
+    energy[b < t->n_bonds12 ? BM_EBOND : BM_EUREY] += e;
   }
   for (int a = 0; a < t->n_angles; a++) {
     int i = t->angle_idx[3 * a], j = t->angle_idx[3 * a + 1], k = t->angle_idx[3 * a + 2];
@@ -553,16 +578,17 @@ void bonded(const bm_topology *t, const bm_runparams *rp, const double *pos,
     double mc = norm(cr);
     double th = std::atan2(mc, dp);
     Scale2 s = scale2(&t->angle_sb[2 * a], lambda);
+    const Weight wt = weight(s, t->angle_soft && t->angle_soft[a], rp->soft_bond_exp);
     double dth = th - t->angle_t0[a];
     double e = 0.5 * t->angle_k[a] * dth * dth;
-    double f = t->angle_k[a] * dth * s.l0 * s.l1;
-    scale_dudl(s, e, dudl);
+    double f = t->angle_k[a] * dth * wt.w;
+    weight_dudl(s, wt.dw * e, dudl);
     Vec3 fi = (f / (mc * dot(rij, rij))) * cross(rij, cr);
     Vec3 fk = (-f / (mc * dot(rkj, rkj))) * cross(rkj, cr);
     addf(force, n, i, fi);
     addf(force, n, k, fk);
     addf(force, n, j, -1.0 * (fi + fk));
-    energy[BM_EANGLE] += e * s.l0 * s.l1;
+    energy[BM_EANGLE] += e * wt.w;
   }
   for (int pass = 0; pass < 2; pass++) {
     int cnt = pass ? t->n_imprs : t->n_dihes;
@@ -593,8 +619,10 @@ void bonded(const bm_topology *t, const bm_runparams *rp, const double *pos,
         e = 0.5 * t->impr_k[d] * dphi * dphi;
       }
       Scale2 s = scale2(&sbv[2 * d], lambda);
-      fphi *= s.l0 * s.l1;
-      scale_dudl(s, e, dudl);
+      const int32_t *sf = pass ? t->impr_soft : t->dihe_soft;
+      const Weight wt = weight(s, sf && sf[d], rp->soft_bond_exp);
+      fphi *= wt.w;
+      weight_dudl(s, wt.dw * e, dudl);
       double rjk2 = dot(rjk, rjk), r = std::sqrt(rjk2);
       Vec3 fi = (-fphi * r / dot(m, m)) * m;
       Vec3 fl = (-fphi * r / dot(nv, nv)) * nv;   // fk before negation
@@ -606,7 +634,7 @@ void bonded(const bm_topology *t, const bm_runparams *rp, const double *pos,
       addf(force, n, l, -1.0 * fl);
       addf(force, n, k, fkk);
       addf(force, n, j, fjj);
-      energy[pass ? BM_EIMPR : BM_EDIHE] += e * s.l0 * s.l1;
+      energy[pass ? BM_EIMPR : BM_EDIHE] += e * wt.w;
     }
   }
 }

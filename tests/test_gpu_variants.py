@@ -529,3 +529,22 @@ def test_stale_list_forces_match_oracle(name):
     print("\n[stale list %s, 19 steps] pairs %d force relRMS %.3e energy rel %.3e dudl %.2e"
           % (name, pairs, frms, erel, ddl), flush=True)
     assert frms <= FORCE_TOL and erel <= ENERGY_TOL and ddl <= DUDL_TOL
+
+
+@pytest.mark.parametrize("removescaling", [False, True])
+def test_soft_bonds(removescaling):
+    """MSLD soft bonds on the three substituent attachment bonds of the small
+    MSLD system (radius 0.8 A, bond power 2, angle/dihedral/improper power
+    1.5; with removescaling bond angle the soft terms keep their scaling):
+    forces, energy and dU/dlambda against the oracle, each bonded term's
+    energy to 1e-9 relative"""
+    from tests.test_oracle import _soft_spec
+    sp = _soft_spec(removescaling=removescaling)
+    s = _check(sp, "soft bonds" + (", removescaling" if removescaling else ""))
+    top, rp = s.topology(), s.runparams()
+    lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
+    eo = orc.forces(top, rp, s.positions(), lam)[2]
+    eg = s.energy()
+    for k in range(5):   # bond, angle, Urey-Bradley, dihedral, improper
+        assert abs(eg[k] - eo[k]) <= 1e-9 * abs(eo[k]) + 1e-6, (k, eg[k], eo[k])
+    assert top.contents.array("bond_soft", top.contents.n_bonds).sum() == 3

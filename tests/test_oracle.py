@@ -912,3 +912,96 @@ def test_exclusion_correction_at_and_near_coincidence():
         h = 1e-4 * max(r, rs)
         dedr = (orc.nbex_pair(kqq, beta, r + h)[1] - orc.nbex_pair(kqq, beta, r - h)[1]) / (2 * h)
         assert fr * r == pytest.approx(dedr, rel=1e-6, abs=1e-9 * abs(kqq))
+
+
+# ---------------------------------------------------------------------------
+# MSLD soft bonds (msld/msld.cu:478-495, bonded/bonded.cu:81-107, 239-256,
+# 408-425)
+# ---------------------------------------------------------------------------
+def _soft_spec(params=(0.8, 2.0, 1.5), removescaling=False):
+    import copy
+    sp = copy.deepcopy(spec("msld_small"))
+    ab = np.array(sp.atom_block)
+    sp.soft_bonds = [(i, j) for (i, j) in sp.bonds if (ab[i] > 0) != (ab[j] > 0)]
+    sp.soft_bond_params = params
+    # the generator builds bonds at their rest lengths: displace the atoms
+    # so the soft factor differs from 1
+    sp.pos = np.asarray(sp.pos) + np.random.default_rng(3).normal(scale=0.08, size=np.shape(sp.pos))
+    if removescaling:
+        sp.scale_terms = (0, 1, 0, 1, 1, 1)   # bond and angle scaling removed
+    return sp
+
+
+def test_soft_bond_terms_gradients_and_dudl():
+    """soft bond, Urey-Bradley, angle, dihedral and improper terms: forces
+    and dU/dlambda are derivatives of their energies (radius 0.8 A, power 2
+    for bonds, 1.5 for the other terms, removescaling bond angle)"""
+    sp = _soft_spec(removescaling=True)
+    assert len(sp.soft_bonds) == 3
+    atoms = sorted({a for b in sp.soft_bonds for a in b})
+    _fd_spec(sp, orc.BONDED, [E_BOND, E_ANGLE, E_UREY, E_DIHE, E_IMPR], atoms, dl_terms=True)
+
+
+def test_soft_terms_topology_and_removescaling():
+    """a term holding both atoms of a soft bond is soft and keeps its
+    blocks' scaling where removescaling names its type; other terms of the
+    same block lose it"""
+    sp = _soft_spec(removescaling=True)
+    s = system.build(sp)
+    s.prepare()
+    t = s.topology().contents
+    pairs = {tuple(sorted(b)) for b in sp.soft_bonds}
+    bi = t.array("bond_idx", t.n_bonds, 2)
+    soft = t.array("bond_soft", t.n_bonds)
+    sb = t.array("bond_sb", t.n_bonds, 2)
+    ai = t.array("angle_idx", t.n_angles, 3)
+    asoft = t.array("angle_soft", t.n_angles)
+    asb = t.array("angle_sb", t.n_angles, 2)
+    for k in range(t.n_bonds12):
+        holds = tuple(sorted(bi[k])) in pairs
+        assert soft[k] == holds
+        if holds:
+            assert sb[k][0] & 0xFFFF > 0          # still scaled by its substituent block
+    ab = np.array(sp.atom_block)
+    plain_sub = [k for k in range(t.n_bonds12) if not soft[k] and ab[bi[k][0]] > 0 and ab[bi[k][1]] > 0]
+    assert plain_sub and all(sb[k][0] == 0 for k in plain_sub)   # removescaling applies
+    for k in range(t.n_angles):
+        holds = any(set(p) <= set(ai[k]) for p in pairs)
+        assert asoft[k] == holds
+        if holds:
+            assert asb[k][0] & 0xFFFF > 0
+    assert asoft.sum() > 0 and t.array("dihe_soft", t.n_dihes).sum() > 0
+
+
+def test_soft_terms_limits():
+    """at lambda = 1 every soft term equals the plain term; with the
+    non-bond power 1 a soft angle, dihedral or improper equals the plain
+    lambda-scaled term at any lambda, while the soft bonds differ"""
+    import copy
+    sp = _soft_spec(params=(0.8, 2.0, 1.5))
+    plain = copy.deepcopy(sp)
+    plain.soft_bonds = []
+    ones = np.ones(sp.nblocks)
+    e = {}
+    for name, q in (("soft", sp), ("plain", plain)):
+        s = system.build(q)
+        s.prepare()
+        e[name] = orc.forces(s.topology(), s.runparams(), system.soa(q.pos), ones, orc.BONDED)[2]
+    for k in (E_BOND, E_ANGLE, E_UREY, E_DIHE, E_IMPR):
+        assert abs(e["soft"][k] - e["plain"][k]) < 1e-9 * max(1.0, abs(e["plain"][k])), k
+    # non-bond power 1: angles, dihedrals and impropers equal the plain ones
+    sp1 = _soft_spec(params=(0.8, 2.0, 1.0))
+    sp1.pos = sp.pos
+    plain1 = copy.deepcopy(sp1)
+    plain1.soft_bonds = []
+    lam = None
+    for name, q in (("soft", sp1), ("plain", plain1)):
+        s = system.build(q)
+        s.prepare()
+        if lam is None:
+            lam = orc.theta_to_lambda(s.topology(), s.runparams(), np.array(q.theta0, float))
+            assert 0.05 < lam[1:].min() and lam[1:].max() < 0.95
+        e[name] = orc.forces(s.topology(), s.runparams(), system.soa(q.pos), lam, orc.BONDED)[2]
+    for k in (E_ANGLE, E_DIHE, E_IMPR):
+        assert abs(e["soft"][k] - e["plain"][k]) < 1e-9 * max(1.0, abs(e["plain"][k])), k
+    assert abs(e["soft"][E_BOND] - e["plain"][E_BOND]) > 1e-6   # the bonds do differ
