@@ -64,6 +64,8 @@ class Topology(C.Structure):
         ("direst_par", _f64p),
         ("n_resd", C.c_int32), ("resd_idx", _i32p), ("resd_par", _f64p),
         ("bond_soft", _i32p), ("angle_soft", _i32p), ("dihe_soft", _i32p), ("impr_soft", _i32p),
+        ("n_virt2", C.c_int32), ("virt2_idx", _i32p), ("virt2_par", _f64p),
+        ("n_virt3", C.c_int32), ("virt3_idx", _i32p), ("virt3_par", _f64p),
     ]
 
     def array(self, name, count, width=1):

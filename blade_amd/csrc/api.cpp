@@ -180,10 +180,15 @@ void blade_add_cmap(System *s, int i1, int j1, int k1, int l1, int i2, int j2, i
   s->structure->cmaps.push_back({i1 - 1, j1 - 1, k1 - 1, l1 - 1, i2 - 1, j2 - 1, k2 - 1, l2 - 1});
 }
 void blade_add_shake(System *s, int shakeHbond) { s->structure->shakeHbond = shakeHbond != 0; }
-// virtual sites (system/structure.cxx:1059-1091) are recorded; a system that
-// has any is rejected when its terms are derived
-void blade_add_virt2(System *s, int, int, int, double, double) { s->structure->virtualSites++; }
-void blade_add_virt3(System *s, int, int, int, int, double, double, double) { s->structure->virtualSites++; }
+// virtual sites (system/structure.cxx:1059-1085): 1-based atoms; angles in
+// degrees, kept in radians
+void blade_add_virt2(System *s, int v, int h1, int h2, double dist, double scale) {
+  s->structure->virts.push_back(Structure::Virt{v - 1, {h1 - 1, h2 - 1, -1}, 2, dist, scale, 0.0});
+}
+void blade_add_virt3(System *s, int v, int h1, int h2, int h3, double dist, double theta, double phi) {
+  s->structure->virts.push_back(
+      Structure::Virt{v - 1, {h1 - 1, h2 - 1, h3 - 1}, 3, dist, kDegrees * theta, kDegrees * phi});
+}
 // restraints (system/structure.cxx:1093-1196): 1-based atoms and blocks,
 // angles in degrees, AKMA units
 void blade_add_noe(System *s, int i, int j, double rmin, double kmin, double rmax, double kmax, double rpeak,

@@ -147,6 +147,11 @@ struct Dev {
   int4 *br2_i; double2 *br2_b;
   int4 *br3_i; double4 *br3_b;
   double *mass;                  // [N]
+  // virtual sites: placed from their hosts before the forces, their
+  // forces moved onto the hosts after them (no site hosts another)
+  int nvirt2, nvirt3;
+  int4 *virt2_i; double2 *virt2_p;   // (site, h0, h1, -); (dist, scale)
+  int4 *virt3_i; double4 *virt3_p;   // (site, h0, h1, h2); [2n] (dist, cos t, sin t, -), (cos p, sin p, -, -)
   int nfree;                     // atoms in no constraint group
   int *freeatom;                 // [nfree]
   // msld

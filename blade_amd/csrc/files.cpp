@@ -318,8 +318,54 @@ int blade_read_psf(System *s, const char *path) {
     if (next_line(in, line)) {
       auto t = tokens(line);
       while (t.empty() && next_line(in, line)) t = tokens(line);
-      if (!t.empty() && (need_int(t, 0, "PSF lone pairs") != 0 || (t.size() > 1 && need_int(t, 1, "PSF lone pairs") != 0)))
-        throw ParseError{"PSF lone pairs (virtual sites) are not supported"};
+      const int nlp = t.empty() ? 0 : need_int(t, 0, "PSF lone pairs");
+      const int nlph = t.size() > 1 ? need_int(t, 1, "PSF lone pair hosts") : 0;
+      if (nlp > 0) {
+        // lone pairs (system/structure.cxx:734-800): per pair its host
+        // count, a pointer into the host list, a weighting flag and three
+        // values; then the host list, each entry the site followed by its
+        // hosts.  Colinear pairs (2 hosts, no weighting) are two-host
+        // sites (distance, scale); 3 hosts with a nonzero distance are
+        // three-host sites (distance, angle, dihedral in degrees).
+        struct Lp { int nh, ptr; double a, b, c; };
+        std::vector<Lp> lps;
+        pend.clear();
+        pos = 0;
+        auto next_tok = [&](const char *what) {
+          while (pos >= pend.size()) {
+            if (!next_line(in, line)) throw ParseError{std::string("PSF ends inside ") + what};
+            pend = tokens(line);
+            pos = 0;
+          }
+          return pend[pos++];
+        };
+        for (int k = 0; k < nlp; k++) {
+          Lp lp;
+          std::vector<std::string> one = {next_tok("lone pairs")};
+          lp.nh = need_int(one, 0, "PSF lone pair host count");
+          one = {next_tok("lone pairs")};
+          lp.ptr = need_int(one, 0, "PSF lone pair host pointer") - 1;
+          const std::string w = next_tok("lone pairs");
+          const bool weighted = !(w == "F" || w == "f" || w == "0" || w == "false" || w == "no" || w == "off");
+          double v[3];
+          for (double &x : v) {
+            one = {next_tok("lone pairs")};
+            x = need_double(one, 0, "PSF lone pair value");
+          }
+          lp.a = v[0]; lp.b = v[1]; lp.c = v[2];
+          if (!((lp.nh == 2 && !weighted) || (lp.nh == 3 && lp.a != 0)))
+            throw ParseError{"PSF lone pair of an unsupported kind (" + std::to_string(lp.nh) + " hosts)"};
+          lps.push_back(lp);
+        }
+        auto hosts = section_ints("lone pair hosts", nlph);
+        for (const Lp &lp : lps) {
+          if (lp.ptr < 0 || lp.ptr + lp.nh >= nlph) throw ParseError{"PSF lone pair host pointer out of range"};
+          int a[4];
+          for (int k = 0; k <= lp.nh; k++) a[k] = check(hosts[lp.ptr + k], "lone pair");
+          if (lp.nh == 2) blade_add_virt2(s, a[0], a[1], a[2], lp.a, lp.b);
+          else blade_add_virt3(s, a[0], a[1], a[2], a[3], lp.a, lp.b, lp.c);
+        }
+      }
     }
     // CMAP cross terms (two dihedrals of eight atoms each)
     while (next_line(in, line)) {

@@ -34,6 +34,9 @@ void launch_chain_rule(const Dev &d, hipStream_t st);
 void launch_theta_to_lambda(const Dev &d, hipStream_t st, bool init = false);
 // constraint groups at exactly their lengths before the first step
 void launch_rectify_init(const Dev &d, hipStream_t st);
+// virtual sites: positions from the hosts; the sites' forces onto the hosts
+void launch_virtual_place(const Dev &d, hipStream_t st);
+void launch_virtual_spread(const Dev &d, hipStream_t st);
 // MC barostat: back up and scale every atom, rectify constraint groups
 // (oldbox: the box before the move); restore the backup
 void launch_volume_move(const Dev &d, const double *scale, const double *oldbox, hipStream_t st);

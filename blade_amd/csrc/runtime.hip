@@ -181,11 +181,17 @@ struct bm_system {
     blade::launch_force_assemble(d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, st, kick);
   }
 
+  // a step: virtual sites placed (domdec/domdec.cu:215), lists, forces,
+  // update.  With virtual sites the sites' forces move to their hosts
+  // between the assembly and the kick, so the assembly does not kick.
   void step_launches(bool rebuild, bool energy) {
+    const bool virt = d.nvirt2 + d.nvirt3 > 0;
+    blade::launch_virtual_place(d, st);
     if (rebuild) blade::launch_domdec_rebuild(d, maxBlocks, st);
     blade::launch_domdec_pack(d, maxBlocks, st);
-    force_launches(energy, !energy);
-    if (energy) blade::launch_update(d, true, st);
+    force_launches(energy, !energy && !virt);
+    blade::launch_virtual_spread(d, st);
+    if (energy || virt) blade::launch_update(d, energy, st);
     else blade::launch_update_after_kick(d, st);
   }
 
@@ -768,6 +774,44 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
       if (!owner[a]) fr.push_back(a);
     d.nfree = (int)fr.size();
     UPLOAD(d.freeatom, fr);
+    // virtual sites: a site belongs to no constraint group and hosts no
+    // other site (the spreading kernel reads site forces while it adds to
+    // host forces)
+    d.nvirt2 = t->n_virt2;
+    d.nvirt3 = t->n_virt3;
+    std::vector<int4> v2i(d.nvirt2), v3i(d.nvirt3);
+    std::vector<double2> v2p(d.nvirt2);
+    std::vector<double4> v3p(2 * (size_t)d.nvirt3);
+    std::vector<char> site(n, 0);
+    auto in_range = [&](int a) { return a >= 0 && a < n; };
+    bool ok = true;
+    for (int k = 0; k < d.nvirt2; k++) {
+      const int *ix = t->virt2_idx + 3 * k;
+      for (int j = 0; j < 3; j++) ok &= in_range(ix[j]);
+      if (!ok) break;
+      ok &= !owner[ix[0]] && !site[ix[0]] && ix[0] != ix[1] && ix[0] != ix[2] && ix[1] != ix[2];
+      site[ix[0]] = 1;
+      v2i[k] = make_int4(ix[0], ix[1], ix[2], 0);
+      v2p[k] = make_double2(t->virt2_par[2 * k], t->virt2_par[2 * k + 1]);
+    }
+    for (int k = 0; ok && k < d.nvirt3; k++) {
+      const int *ix = t->virt3_idx + 4 * k;
+      for (int j = 0; j < 4; j++) ok &= in_range(ix[j]);
+      if (!ok) break;
+      ok &= !owner[ix[0]] && !site[ix[0]] && ix[1] != ix[2] && ix[1] != ix[3] && ix[2] != ix[3] &&
+            ix[0] != ix[1] && ix[0] != ix[2] && ix[0] != ix[3];
+      site[ix[0]] = 1;
+      const double *p = t->virt3_par + 3 * k;
+      v3i[k] = make_int4(ix[0], ix[1], ix[2], ix[3]);
+      v3p[2 * k] = make_double4(p[0], std::cos(p[1]), std::sin(p[1]), 0);
+      v3p[2 * k + 1] = make_double4(std::cos(p[2]), std::sin(p[2]), 0, 0);
+    }
+    for (int k = 0; ok && k < d.nvirt2; k++) ok &= !site[v2i[k].y] && !site[v2i[k].z];
+    for (int k = 0; ok && k < d.nvirt3; k++) ok &= !site[v3i[k].y] && !site[v3i[k].z] && !site[v3i[k].w];
+    if (!ok)
+      return fail(set_err(BM_EINVAL, "virtual site indices out of range, repeated, constrained or hosted by a site"));
+    UPLOAD(d.virt2_i, v2i); UPLOAD(d.virt2_p, v2p);
+    UPLOAD(d.virt3_i, v3i); UPLOAD(d.virt3_p, v3p);
   }
   // msld
   {
@@ -880,6 +924,7 @@ int bm_sync(bm_system *s) {
 
 int bm_domdec_update(bm_system *s, int rebuild) {
   if (!s) return BM_EINVAL;
+  blade::launch_virtual_place(s->d, s->st);
   if (rebuild) blade::launch_domdec_rebuild(s->d, s->maxBlocks, s->st);
   blade::launch_domdec_pack(s->d, s->maxBlocks, s->st);
   HIPCHK(hipGetLastError());
@@ -900,6 +945,7 @@ int bm_getforce_nbdirect(bm_system *s, unsigned flags) {
   blade::launch_nbdirect(s->d, flags & BM_CALC_ENERGY, s->alch, s->soft, s->nwgNb, s->st);
   if (s->alch) blade::launch_nb_dudl(s->d, s->soft, s->st);
   blade::launch_force_assemble(s->d, blade::kAsmNb, s->st);
+  blade::launch_virtual_spread(s->d, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
@@ -909,6 +955,7 @@ int bm_getforce_ewald(bm_system *s, unsigned flags) {
   if (s->d.elecmethod == BM_PME) {
     blade::launch_pme_grid(s->d, flags & BM_CALC_ENERGY, s->st);
     blade::launch_force_assemble(s->d, blade::kAsmPme, s->st);
+    blade::launch_virtual_spread(s->d, s->st);
   }
   HIPCHK(hipGetLastError());
   return BM_OK;
@@ -921,6 +968,7 @@ static int bonded_families(bm_system *s, unsigned flags, unsigned fam) {
   if (!s) return BM_EINVAL;
   blade::launch_bonded(s->d, flags & BM_CALC_ENERGY, fam, s->st);
   blade::launch_force_assemble(s->d, blade::kAsmBonded, s->st);
+  blade::launch_virtual_spread(s->d, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
@@ -953,6 +1001,7 @@ int bm_getforce_restrain(bm_system *s, unsigned flags) { return bonded_families(
 int bm_calc_force(bm_system *s, long, unsigned flags) {
   if (!s) return BM_EINVAL;
   s->force_launches(flags & BM_CALC_ENERGY);
+  blade::launch_virtual_spread(s->d, s->st);
   HIPCHK(hipGetLastError());
   return BM_OK;
 }
@@ -1372,6 +1421,8 @@ int bm_batch_create(int n, bm_system *const *systems, bm_batch **out) {
     if (systems[k]->device != systems[0]->device) return set_err(BM_EINVAL, "batch members on different devices");
     if (systems[k]->freqDomdec != systems[0]->freqDomdec)
       return set_err(BM_EINVAL, "batch members with different list rebuild cadence");
+    if (systems[k]->d.nvirt2 + systems[k]->d.nvirt3 > 0)
+      return set_err(BM_ENOTSUP, "batch members with virtual sites (run them as single systems)");
   }
   HIPCHK(hipSetDevice(systems[0]->device));
   bm_batch *b = new bm_batch();
@@ -1652,6 +1703,7 @@ int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
     const long step = step0 + k;
     const bool rb = (step % s->freqDomdec) == 0;
     HIPCHK(hipEventRecord(s->ev[0], s->st));
+    blade::launch_virtual_place(s->d, s->st);
     if (rb) blade::launch_domdec_rebuild(s->d, s->maxBlocks, s->st);
     blade::launch_domdec_pack(s->d, s->maxBlocks, s->st);
     HIPCHK(hipEventRecord(s->ev[1], s->st));
@@ -1662,6 +1714,7 @@ int bm_profile_steps(bm_system *s, long step0, long nsteps, double *out) {
     HIPCHK(hipEventRecord(s->ev[3], s->st));
     if (s->d.elecmethod == BM_PME) blade::launch_pme_grid(s->d, false, s->st);
     blade::launch_force_assemble(s->d, blade::kAsmNb | blade::kAsmBonded | blade::kAsmPme, s->st);
+    blade::launch_virtual_spread(s->d, s->st);
     HIPCHK(hipEventRecord(s->ev[5], s->st));
     blade::launch_update(s->d, false, s->st);
     HIPCHK(hipEventRecord(s->ev[4], s->st));

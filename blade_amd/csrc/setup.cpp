@@ -271,7 +271,24 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
     if (a == b || allEx[a].count(b) || !sc.interacting(a, b)) return;
     v[a].insert(b); v[b].insert(a); allEx[a].insert(b); allEx[b].insert(a);
   };
-  for (auto &b : s.bonds) add(bondEx, b[0], b[1]);
+  // a virtual site is excluded from its first host like a bond partner,
+  // and from every bond partner of that host (potential.cxx:947-995)
+  std::vector<std::set<int>> virtEx(n);
+  for (const auto &v : s.virts) {
+    auto ok = [&](int a) { return a >= 0 && a < n; };
+    if (!ok(v.v) || !ok(v.h[0]) || !ok(v.h[1]) || (v.nh == 3 && !ok(v.h[2]))) fail("virtual site atom out of range");
+    const int a = v.v, b = v.h[0];
+    if (a == b || allEx[a].count(b) || !sc.interacting(a, b)) continue;
+    virtEx[a].insert(b); virtEx[b].insert(a);
+    add(bondEx, a, b);
+  }
+  for (auto &b : s.bonds) {
+    add(bondEx, b[0], b[1]);
+    for (int j = 0; j < 2; j++) {
+      std::vector<int> ks(virtEx[b[1 - j]].begin(), virtEx[b[1 - j]].end());
+      for (int k : ks) add(bondEx, b[j], k);
+    }
+  }
   for (auto &b : s.bonds)
     for (int j = 0; j < 2; j++) {
       std::vector<int> ks(bondEx[b[1 - j]].begin(), bondEx[b[1 - j]].end());
@@ -497,9 +514,23 @@ void build_topology(const Parameters &p, const Structure &s, const MsldSetup &m,
   t.anySoft = false;
   for (const auto *v : {&t.bondSoft, &t.angleSoft, &t.diheSoft, &t.imprSoft})
     for (int f : *v) t.anySoft = t.anySoft || f;
+  // virtual sites (indices checked with the exclusions)
+  for (const auto &v : s.virts) {
+    if (v.nh == 2) {
+      const int idx[3] = {v.v, v.h[0], v.h[1]};
+      t.virt2Idx.insert(t.virt2Idx.end(), idx, idx + 3);
+      t.virt2Par.push_back(v.dist);
+      t.virt2Par.push_back(v.a);
+    } else {
+      const int idx[4] = {v.v, v.h[0], v.h[1], v.h[2]};
+      t.virt3Idx.insert(t.virt3Idx.end(), idx, idx + 4);
+      t.virt3Par.push_back(v.dist);
+      t.virt3Par.push_back(v.a);
+      t.virt3Par.push_back(v.b);
+    }
+  }
   // restraints (Potential::initialize copies the structure's lists,
   // potential.cxx:1620-1660)
-  if (s.virtualSites) fail("virtual sites are not supported");
   auto atom_ok = [&](int a) {
     if (a < 0 || a >= n) fail("restraint atom index out of range");
     return a;
@@ -567,6 +598,8 @@ const bm_topology *Topology::view() {
     v.bond_soft = bondSoft.data(); v.angle_soft = angleSoft.data();
     v.dihe_soft = diheSoft.data(); v.impr_soft = imprSoft.data();
   }
+  v.n_virt2 = (int)virt2Par.size() / 2; v.virt2_idx = virt2Idx.data(); v.virt2_par = virt2Par.data();
+  v.n_virt3 = (int)virt3Par.size() / 3; v.virt3_idx = virt3Idx.data(); v.virt3_par = virt3Par.data();
   v.n_nb14 = (int)nb14qq.size();
   v.nb14_idx = nb14Idx.data(); v.nb14_sb = nb14Sb.data(); v.nb14_qq = nb14qq.data();
   v.nb14_c12 = nb14c12.data(); v.nb14_c6 = nb14c6.data(); v.nb14_e14fac = nb14e14.data();

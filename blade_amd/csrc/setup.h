@@ -64,7 +64,11 @@ struct Structure {
   std::vector<AnRestRec> anRests;
   std::vector<DiRestRec> diRests;
   std::vector<ResdRec> resds;
-  int virtualSites = 0;        // virtual sites are recorded, not supported
+  // virtual sites (system/structure.cxx:1059-1085): 0-based site and hosts;
+  // two hosts: dist (A) and scale; three hosts: dist (A, negative: the
+  // bisector of hosts 1 and 2), theta and phi (radians)
+  struct Virt { int v, h[3], nh; double dist, a, b; };
+  std::vector<Virt> virts;
 };
 
 struct Coordinates {
@@ -141,6 +145,7 @@ struct Topology {
   std::vector<int> diheIdx, diheSb, diheN; std::vector<double> diheK, diheD0;
   std::vector<int> imprIdx, imprSb, imprN; std::vector<double> imprK, imprD0;
   std::vector<int> bondSoft, angleSoft, diheSoft, imprSoft; bool anySoft = false;
+  std::vector<int> virt2Idx, virt3Idx; std::vector<double> virt2Par, virt3Par;
   std::vector<int> nb14Idx, nb14Sb; std::vector<double> nb14qq, nb14c12, nb14c6, nb14e14;
   std::vector<int> nbexIdx, nbexSb; std::vector<double> nbexqq;
   std::vector<int> exclIdx;

@@ -827,6 +827,135 @@ void launch_rectify_init(const Dev &d, hipStream_t st) {
   if (ng) upd_rectify_init<<<(ng + 255) / 256, 256, 0, st>>>(d);
 }
 
+// ---- virtual sites (holonomic/virtual.cu:11-254) -------------------------
+// A site sits at its first host plus an offset o fixed by the hosts (hosts 1
+// and 2 taken relative to host 0 in the minimum image).  Two hosts:
+// o = (dist/|e| + scale) e with e = x_h0 - x_h1.  Three hosts: ax is the unit
+// vector from host 0 to host 1 (to the midpoint of hosts 1 and 2 when
+// dist < 0, the bisector form), c the unit vector along (x_h2 - x_h1) x ax,
+// bp = ax x c, and o = |dist| (cos t ax + sin t (sin p c + cos p bp)).
+struct VirtGeom {
+  D3 r1, r2;        // hosts 1, 2 - host 0
+  D3 e;             // two hosts: x_h0 - x_h1
+  double s, inv;    // two hosts: dist/|e| + scale, 1/|e|
+  D3 bb, p1, ax, cc, c, bp;   // three hosts
+  double ip1, icc;  // 1/|p1|, 1/|cc|
+  D3 o;
+};
+
+__device__ __forceinline__ void virt_geom(const Dev &d, int k, int4 &ix, VirtGeom &g) {
+  if (k < d.nvirt2) {
+    ix = d.virt2_i[k];
+    const double2 p = d.virt2_p[k];
+    const D3 h0 = getp(d.pos, d.n, ix.y);
+    g.r1 = minimg(d, getp(d.pos, d.n, ix.z) - h0);
+    g.e = -1.0 * g.r1;
+    g.inv = 1.0 / sqrt(dotd(g.e, g.e));
+    g.s = p.x * g.inv + p.y;
+    g.o = g.s * g.e;
+    return;
+  }
+  const int j = k - d.nvirt2;
+  ix = d.virt3_i[j];
+  const double4 p = d.virt3_p[2 * j], q = d.virt3_p[2 * j + 1];
+  const D3 h0 = getp(d.pos, d.n, ix.y);
+  g.r1 = minimg(d, getp(d.pos, d.n, ix.z) - h0);
+  g.r2 = minimg(d, getp(d.pos, d.n, ix.w) - h0);
+  g.bb = g.r2 - g.r1;
+  g.p1 = p.x < 0 ? g.r1 + 0.5 * g.bb : g.r1;
+  g.ip1 = 1.0 / sqrt(dotd(g.p1, g.p1));
+  g.ax = g.ip1 * g.p1;
+  g.cc = crossd(g.bb, g.ax);
+  g.icc = 1.0 / sqrt(dotd(g.cc, g.cc));
+  g.c = g.icc * g.cc;
+  g.bp = crossd(g.ax, g.c);
+  const D3 w = q.y * g.c + q.x * g.bp;
+  g.o = fabs(p.x) * (p.y * g.ax + p.z * w);
+}
+
+__global__ void __launch_bounds__(256) upd_virtual_place(Dev d) {
+  if (halted_wg(d)) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nvirt2 + d.nvirt3) return;
+  int4 ix;
+  VirtGeom g;
+  virt_geom(d, k, ix, g);
+  // in the periodic image nearest the site's previous position, so the
+  // site stays where the pair list saw it
+  const D3 nw = getp(d.pos, d.n, ix.y) + g.o;
+  const D3 step = nw - getp(d.pos, d.n, ix.x);
+  setp(d.pos, d.n, ix.x, nw + (minimg(d, step) - step));
+}
+
+// The site's force F goes to the hosts as J^T F (J = d x_site / d x_hosts),
+// by reverse-mode differentiation of the construction above; the site's own
+// entry is cleared.  Hosts are never sites, so the site entries read here
+// are not written by other threads; host sums are fixed-point atomics and
+// do not depend on their order.
+__global__ void __launch_bounds__(256) upd_virtual_spread(Dev d) {
+  if (halted_wg(d)) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nvirt2 + d.nvirt3) return;
+  int4 ix;
+  VirtGeom g;
+  virt_geom(d, k, ix, g);
+  const int n = d.n;
+  D3 F = d3((double)d.force[ix.x] * kFixInvD, (double)d.force[n + ix.x] * kFixInvD,
+            (double)d.force[2 * n + ix.x] * kFixInvD);
+  d.force[ix.x] = 0;
+  d.force[n + ix.x] = 0;
+  d.force[2 * n + ix.x] = 0;
+  D3 f[3];
+  int nh;
+  if (k < d.nvirt2) {
+    // o = s e, s = dist/|e| + scale: d o/d e = s I - dist e e^T/|e|^3
+    const double dist = d.virt2_p[k].x;
+    const D3 ge = g.s * F - (dist * g.inv * g.inv * g.inv * dotd(g.e, F)) * g.e;
+    f[0] = F + ge;   // e = x_h0 - x_h1
+    f[1] = -1.0 * ge;
+    nh = 2;
+  } else {
+    const int j = k - d.nvirt2;
+    const double4 p = d.virt3_p[2 * j], q = d.virt3_p[2 * j + 1];
+    const double D = fabs(p.x);
+    // adjoints of o = D cos t ax + D sin t sin p c + D sin t cos p bp
+    D3 gax = (D * p.y) * F;
+    const D3 gbp = (D * p.z * q.x) * F;
+    D3 gc = (D * p.z * q.y) * F;
+    gax = gax + crossd(g.c, gbp);   // bp = ax x c
+    gc = gc + crossd(gbp, g.ax);
+    // c = cc/|cc|
+    const D3 gcc = g.icc * (gc - dotd(g.c, gc) * g.c);
+    // cc = bb x ax
+    D3 gbb = crossd(g.ax, gcc);
+    gax = gax + crossd(gcc, g.bb);
+    // ax = p1/|p1|, p1 = r1 + h bb
+    const D3 gp1 = g.ip1 * (gax - dotd(g.ax, gax) * g.ax);
+    if (p.x < 0) gbb = gbb + 0.5 * gp1;
+    // bb = r2 - r1; r1, r2 relative to host 0
+    f[1] = gp1 - gbb;
+    f[2] = gbb;
+    f[0] = F - f[1] - f[2];
+    nh = 3;
+  }
+  const int h[3] = {ix.y, ix.z, ix.w};
+  for (int a = 0; a < nh; a++) {
+    atomic_fix(&d.force[h[a]], fix_d(f[a].x));
+    atomic_fix(&d.force[n + h[a]], fix_d(f[a].y));
+    atomic_fix(&d.force[2 * n + h[a]], fix_d(f[a].z));
+  }
+}
+
+void launch_virtual_place(const Dev &d, hipStream_t st) {
+  const int nv = d.nvirt2 + d.nvirt3;
+  if (nv) upd_virtual_place<<<(nv + 255) / 256, 256, 0, st>>>(d);
+}
+
+void launch_virtual_spread(const Dev &d, hipStream_t st) {
+  const int nv = d.nvirt2 + d.nvirt3;
+  if (nv) upd_virtual_spread<<<(nv + 255) / 256, 256, 0, st>>>(d);
+}
+
 // restore the backup (a rejected move)
 __global__ void __launch_bounds__(256) upd_restore_positions(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

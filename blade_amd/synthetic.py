@@ -116,6 +116,8 @@ class Spec:
     piecewise: tuple = None                           # (width, k): piecewise lambda constraint
     theta_edge: tuple = None                          # (k, N, alpha, phi) edge bias of the piecewise form
     fix_lambda: bool = False                          # msld fix flag: lambda held at the initial theta
+    virt2: list = None                                # two-host virtual sites (site, h0, h1, dist, scale), 0-based
+    virt3: list = None                                # three-host sites (site, h0, h1, h2, dist, theta deg, phi deg)
     restraints: dict = None                           # {"noe"|"harm"|"borest"|"anrest"|"direst"|"resd": [rows]},
                                                       # 0-based atoms and blocks, angles in degrees
     # run options
@@ -244,6 +246,65 @@ def water_box(nwat, box, seed, name="water"):
     spec.pos = np.array(spec._pos)
     spec.atom_block = np.zeros(spec.n, dtype=np.int32)
     return spec, rng
+
+
+# TIP4P (Jorgensen et al. 1983): charge on a massless site M 0.15 A from
+# the oxygen along the HOH bisector, LJ on the oxygen
+TIP4P_TYPES = {"OT4": 15.9994, "HT": 1.008, "LP": 0.0}
+TIP4P_NB = {"OT4": (-0.155, 1.7699, -0.155, 1.7699, 1.0, 0),
+            "LP": (0.0, 0.2245, 0.0, 0.2245, 1.0, 0)}
+
+
+def tip4p_box(nwat, box, seed, colinear=0, offaxis=0, name="tip4p"):
+    """nwat TIP4P waters (O, H1, H2, M) on a shuffled lattice; M is a
+    three-host bisector site (dist -0.15, angle 0, dihedral 0).  The first
+    `colinear` waters also carry a two-host site beyond H1 (hosts H1, O;
+    dist 0.2 A, scale 0.1), the next `offaxis` a three-host site off the
+    molecular plane (hosts O, H1, H2; dist 0.5 A, angle 110, dihedral 75);
+    each takes 0.1 e from its H1."""
+    rng = np.random.default_rng(seed)
+    spec = Spec(name=name, box=tuple(float(b) for b in box))
+    spec.types.update(TIP4P_TYPES)
+    spec.bond_par.update({("OT4", "HT"): (450.0, 0.9572), ("HT", "HT"): (0.0, 1.5139)})
+    spec.angle_par.update({("HT", "OT4", "HT"): (55.0, 104.52, 0.0, 0.0)})
+    spec.nb_par.update(TIP4P_NB)
+    spec.nb_par["HT"] = WATER_NB["HT"]
+    spec.virt2, spec.virt3 = [], []
+    pos = []
+    n1 = int(math.ceil(round(nwat ** (1 / 3), 9)))
+    sites = [(i, j, k) for i in range(n1) for j in range(n1) for k in range(n1)]
+    rng.shuffle(sites)
+    sp = np.array(box) / n1
+    f0, f1, f2 = _water_frame()
+    bis = (f1 + f2) / np.linalg.norm(f1 + f2)
+    for r, (i, j, k) in enumerate(sorted(sites[:nwat])):
+        o = (np.array([i, j, k]) + 0.5) * sp
+        rot = _rot(rng)
+        base = len(spec.atoms)
+        extra = 1 if r < colinear + offaxis else 0
+        qh1 = 0.52 - 0.1 * extra
+        for nm, t, q, f in (("OH2", "OT4", 0.0, f0), ("H1", "HT", qh1, f1), ("H2", "HT", 0.52, f2),
+                            ("OM", "LP", -1.04, 0.15 * bis)):
+            spec.atoms.append(("WAT", str(r + 1), "TIP4", nm, t, q, TIP4P_TYPES[t]))
+            pos.append(o + rot @ f)
+        spec.bonds += [(base, base + 1), (base, base + 2), (base + 1, base + 2)]
+        spec.angles.append((base + 1, base, base + 2))
+        spec.virt3.append((base + 3, base, base + 1, base + 2, -0.15, 0.0, 0.0))
+        if extra:
+            spec.atoms.append(("WAT", str(r + 1), "TIP4", "LX", "LP", 0.1, 0.0))
+            pos.append(o + rot @ (1.5 * f1))   # any start: the site is placed from its hosts
+            if r < colinear:
+                spec.virt2.append((base + 4, base + 1, base, 0.2, 0.1))
+            else:
+                spec.virt3.append((base + 4, base, base + 1, base + 2, 0.5, 110.0, 75.0))
+    spec.pos = np.array(pos)
+    spec.atom_block = np.zeros(spec.n, dtype=np.int32)
+    m = np.array([a[6] for a in spec.atoms])
+    v = np.zeros((spec.n, 3))
+    heavy = m > 0
+    v[heavy] = _maxwell(rng, m[heavy], spec.temperature)
+    spec.vel = v
+    return spec
 
 
 def config1(seed=0xB1ADE001):

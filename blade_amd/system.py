@@ -317,6 +317,11 @@ def build(spec, gpus=(0,)):
         s.add_impr(a + 1, b + 1, c + 1, d + 1)
     for q in getattr(spec, "cmaps", []):
         s.L.blade_add_cmap(s.h, *[int(a) + 1 for a in q])
+    for (v, h0, h1, dist, scale) in getattr(spec, "virt2", None) or []:
+        s.L.blade_add_virt2(s.h, int(v) + 1, int(h0) + 1, int(h1) + 1, float(dist), float(scale))
+    for (v, h0, h1, h2, dist, theta, phi) in getattr(spec, "virt3", None) or []:
+        s.L.blade_add_virt3(s.h, int(v) + 1, int(h0) + 1, int(h1) + 1, int(h2) + 1, float(dist), float(theta),
+                            float(phi))
     rs = getattr(spec, "restraints", None) or {}
     for (i, j, rmin, kmin, rmax, kmax, rpeak, rsw, nsw, cx, cy, cz, pnoe) in rs.get("noe", []):
         s.L.blade_add_noe(s.h, i + 1, j + 1, rmin, kmin, rmax, kmax, rpeak, rsw, nsw, cx, cy, cz, bool(pnoe))

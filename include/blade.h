@@ -74,8 +74,13 @@ void blade_add_dihe(System *system, int i, int j, int k, int l);
 void blade_add_impr(System *system, int i, int j, int k, int l);
 void blade_add_cmap(System *system, int i1, int j1, int k1, int l1, int i2, int j2, int k2, int l2);
 void blade_add_shake(System *system, int shakeHbond);
-/* virtual sites: recorded; a system with any is rejected (BM_EINVAL) when
-   its terms are derived, virtual sites are not supported */
+/* virtual sites (system/structure.cxx:1059-1085), 1-based atoms: site v at
+   host h1 plus (dist/|e| + scale) e, e = x_h1 - x_h2; or, with three hosts,
+   at distance |dist| from h1 at angle theta and dihedral phi (degrees) in
+   the frame of h1, h2, h3 (dist < 0: the bisector of h2 and h3 replaces
+   h2).  Sites are placed before every force evaluation and their forces
+   move onto the hosts; a site may host no other site, belong to no
+   constraint group, and a system with sites cannot join a batch. */
 void blade_add_virt2(System *system, int v, int h1, int h2, double dist, double scale);
 void blade_add_virt3(System *system, int v, int h1, int h2, int h3, double dist, double theta, double phi);
 /* restraints (system/structure.h:124-130; forms in bm_topology): 1-based

@@ -236,6 +236,24 @@ typedef struct bm_topology {
      soft_bond_alpha, soft_bond_power), a soft angle, dihedral or improper
      E = P^q E0 (runparams soft_bond_exp).  NULL when no term is soft. */
   const int32_t *bond_soft, *angle_soft, *dihe_soft, *impr_soft;   /* [n] 0 or 1 */
+  /* Virtual sites / lone pairs (holonomic/virtual.cu:11-254; setters
+     system/structure.cxx:1059-1085, PSF NUMLP section structure.cxx:
+     734-800): massless atoms placed from their hosts before each force
+     evaluation; the force on a site is carried to its hosts, so the
+     energy's gradient with respect to the hosts is exact, and the site's
+     own force entry is left zero.  Two hosts (site v, hosts h0 h1):
+     x_v = x_h0 + (dist / |d| + scale) d with d = x_h0 - x_h1 (minimum
+     image).  Three hosts (v, h0 h1 h2): with b = x_h2 - x_h1 (h1 moved to
+     the midpoint of h1 and h2 when dist < 0), a = unit(x_h1 - x_h0),
+     c = unit(b x a) and b' = a x c, x_v = x_h0 + |dist| (cos(theta) a +
+     sin(theta) (cos(phi) b' + sin(phi) c)).  A site is excluded from its
+     first host and from that host's bond partners. */
+  int32_t n_virt2;
+  const int32_t *virt2_idx;        /* [3n] v h0 h1 */
+  const double *virt2_par;         /* [2n] dist (A) scale */
+  int32_t n_virt3;
+  const int32_t *virt3_idx;        /* [4n] v h0 h1 h2 */
+  const double *virt3_par;         /* [3n] dist (A) theta phi (radians) */
 } bm_topology;
 
 /* run parameters (run/run.cu:27-110 defaults) */

@@ -1204,6 +1204,129 @@ void for_groups(const bm_topology *t, F fn) {
   }
 }
 
+// Virtual sites (holonomic/virtual.cu:11-254).  The site position as a
+// function of its hosts' positions, evaluated in forward-mode derivatives:
+// each coordinate carries its gradient with respect to the 9 host
+// coordinates, so the force on the hosts is the exact J^T F of the site's
+// force F.
+struct Dn {
+  double v, g[9];
+};
+Dn dn(double v) { Dn r{v, {0}}; return r; }
+Dn operator+(const Dn &a, const Dn &b) { Dn r{a.v + b.v, {0}}; for (int k = 0; k < 9; k++) r.g[k] = a.g[k] + b.g[k]; return r; }
+Dn operator-(const Dn &a, const Dn &b) { Dn r{a.v - b.v, {0}}; for (int k = 0; k < 9; k++) r.g[k] = a.g[k] - b.g[k]; return r; }
+Dn operator*(const Dn &a, const Dn &b) {
+  Dn r{a.v * b.v, {0}};
+  for (int k = 0; k < 9; k++) r.g[k] = a.g[k] * b.v + a.v * b.g[k];
+  return r;
+}
+Dn operator*(double s, const Dn &a) { Dn r{s * a.v, {0}}; for (int k = 0; k < 9; k++) r.g[k] = s * a.g[k]; return r; }
+Dn dsqrt(const Dn &a) {
+  Dn r{std::sqrt(a.v), {0}};
+  for (int k = 0; k < 9; k++) r.g[k] = a.g[k] / (2 * r.v);
+  return r;
+}
+Dn dinv(const Dn &a) {
+  Dn r{1 / a.v, {0}};
+  for (int k = 0; k < 9; k++) r.g[k] = -a.g[k] / (a.v * a.v);
+  return r;
+}
+struct V3d { Dn x, y, z; };
+V3d vsub(const V3d &a, const V3d &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3d vadd(const V3d &a, const V3d &b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3d vscale(const Dn &s, const V3d &a) { return {s * a.x, s * a.y, s * a.z}; }
+V3d vscale(double s, const V3d &a) { return {s * a.x, s * a.y, s * a.z}; }
+Dn vdot(const V3d &a, const V3d &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3d vcross(const V3d &a, const V3d &b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+V3d vunit(const V3d &a) { return vscale(dinv(dsqrt(vdot(a, a))), a); }
+
+// offset of the site from its first host, hosts given relative to the
+// first host (h0 = 0) with their derivative seeds
+V3d virt_offset(int nh, const V3d &h1, const V3d &h2, double dist, double a, double b) {
+  if (nh == 2) {
+    const V3d d = vscale(-1.0, h1);   // x_h0 - x_h1
+    const Dn s = dist * dinv(dsqrt(vdot(d, d))) + dn(a);
+    return vscale(s, d);
+  }
+  V3d bb = vsub(h2, h1), p1 = h1;
+  if (dist < 0) p1 = vadd(h1, vscale(0.5, bb));
+  const V3d ax = vunit(p1);
+  const V3d c = vunit(vcross(bb, ax));
+  const V3d bp = vcross(ax, c);
+  const V3d w = vadd(vscale(std::sin(b), c), vscale(std::cos(b), bp));
+  return vscale(std::fabs(dist), vadd(vscale(std::cos(a), ax), vscale(std::sin(a), w)));
+}
+
+template <class F>
+void for_virts(const bm_topology *t, F fn) {
+  for (int k = 0; k < t->n_virt2; k++) {
+    const int *ix = t->virt2_idx + 3 * k;
+    fn(2, ix[0], ix[1], ix[2], -1, t->virt2_par[2 * k], t->virt2_par[2 * k + 1], 0.0);
+  }
+  for (int k = 0; k < t->n_virt3; k++) {
+    const int *ix = t->virt3_idx + 4 * k;
+    fn(3, ix[0], ix[1], ix[2], ix[3], t->virt3_par[3 * k], t->virt3_par[3 * k + 1], t->virt3_par[3 * k + 2]);
+  }
+}
+
+// hosts 1 and 2 relative to host 0 (minimum image), seeded: derivative
+// index 3*j+k is coordinate k of host j
+void virt_hosts(const double *x, int n, const double *L, int h0, int h1, int h2, V3d &r1, V3d &r2) {
+  const Vec3 p0 = getp(x, n, h0);
+  const Vec3 d1 = minimg(getp(x, n, h1) - p0, L);
+  const Vec3 d2 = h2 >= 0 ? minimg(getp(x, n, h2) - p0, L) : v3(0, 0, 0);
+  const double c1[3] = {d1.x, d1.y, d1.z}, c2[3] = {d2.x, d2.y, d2.z};
+  Dn *o1[3] = {&r1.x, &r1.y, &r1.z}, *o2[3] = {&r2.x, &r2.y, &r2.z};
+  for (int k = 0; k < 3; k++) {
+    *o1[k] = dn(c1[k]);
+    o1[k]->g[3 + k] = 1.0;
+    o1[k]->g[k] = -1.0;   // relative to host 0
+    *o2[k] = dn(c2[k]);
+    o2[k]->g[6 + k] = 1.0;
+    o2[k]->g[k] = -1.0;
+  }
+}
+
+void virt_place(const bm_topology *t, const bm_runparams *rp, double *x) {
+  const int n = t->n_atoms;
+  for_virts(t, [&](int nh, int v, int h0, int h1, int h2, double dist, double a, double b) {
+    V3d r1, r2;
+    virt_hosts(x, n, rp->box, h0, h1, h2, r1, r2);
+    const V3d o = virt_offset(nh, r1, r2, dist, a, b);
+    // in the periodic image nearest the site's previous position
+    // (holonomic/virtual.cu:32-36)
+    const Vec3 nw = v3(x[h0] + o.x.v, x[n + h0] + o.y.v, x[2 * n + h0] + o.z.v);
+    const Vec3 step = nw - getp(x, n, v);
+    const Vec3 p = nw + (minimg(step, rp->box) - step);
+    x[v] = p.x;
+    x[n + v] = p.y;
+    x[2 * n + v] = p.z;
+  });
+}
+
+// the site's force to its hosts, J^T F; the site's entry is cleared
+void virt_spread(const bm_topology *t, const bm_runparams *rp, const double *x, double *f) {
+  const int n = t->n_atoms;
+  for_virts(t, [&](int nh, int v, int h0, int h1, int h2, double dist, double a, double b) {
+    V3d r1, r2;
+    virt_hosts(x, n, rp->box, h0, h1, h2, r1, r2);
+    const V3d o = virt_offset(nh, r1, r2, dist, a, b);
+    const double F[3] = {f[v], f[n + v], f[2 * n + v]};
+    const Dn *oc[3] = {&o.x, &o.y, &o.z};
+    const int hosts[3] = {h0, h1, h2};
+    for (int j = 0; j < nh; j++)
+      for (int k = 0; k < 3; k++) {
+        // x_v = x_h0 + o: host 0 also carries F itself
+        double g = j == 0 ? F[k] : 0.0;
+        for (int i = 0; i < 3; i++) g += F[i] * oc[i]->g[3 * j + k];
+        f[k * n + hosts[j]] += g;
+      }
+    f[v] = f[n + v] = f[2 * n + v] = 0.0;
+  });
+}
+
 }  // namespace
 
 // =====================================================================
@@ -1236,6 +1359,8 @@ long orc_forces(const bm_topology *t, const bm_runparams *rp, const double *pos,
   if ((terms & ORC_MSLDBIAS) && t->n_blocks > 1 && (t->cr_k1 != 0 || t->cr_k2 != 0 || t->cr_k3 != 0))
     chargerestraints(t, rp, lambda, dudl, energy);
   if (terms & ORC_RESTRAIN) restraints(t, rp, pos, lambda, force, dudl, energy);
+  // forces on virtual sites move to their hosts (system/potential.cxx:1938)
+  if (t->n_virt2 + t->n_virt3 > 0) virt_spread(t, rp, pos, force);
   double pot = 0;
   for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
   energy[BM_EPOTENTIAL] = pot;
@@ -1656,6 +1781,8 @@ void orc_rectify(const bm_topology *t, const bm_runparams *rp, double *x) {
   });
 }
 
+void orc_virtual_place(const bm_topology *t, const bm_runparams *rp, double *x) { virt_place(t, rp, x); }
+
 void orc_holonomic_velocity(const bm_topology *t, const bm_runparams *rp, const double *x, double *v) {
   const int n = t->n_atoms;
   for_groups(t, [&](const Group &g, int) {
@@ -1725,6 +1852,8 @@ int orc_step(const bm_topology *t, const bm_runparams *rp, long step, int energy
              double *x, double *v, double *lambda, double *energy) {
   const int n = t->n_atoms, nl = t->n_blocks, N1 = 3 * n, N = 3 * n + nl;
   std::vector<double> f(N, 0.0), dudl(nl, 0.0), dudth(nl, 0.0), ism(N), xb(N);
+  // virtual sites follow their hosts before the forces (domdec/domdec.cu:215)
+  if (t->n_virt2 + t->n_virt3 > 0) virt_place(t, rp, x);
   orc_forces(t, rp, x, lambda, ORC_ALL, f.data(), dudl.data(), energy);
   if (nl > 1) {
     orc_chain_rule(t, rp, x + N1, lambda, dudl.data(), dudth.data());
@@ -1734,7 +1863,10 @@ int orc_step(const bm_topology *t, const bm_runparams *rp, long step, int energy
     for (int i = 0; i < BM_EPOTENTIAL; i++) pot += energy[i];
     energy[BM_EPOTENTIAL] = pot;
   }
-  for (int i = 0; i < n; i++) ism[i] = ism[n + i] = ism[2 * n + i] = 1 / std::sqrt(t->mass[i]);
+  // massless particles (virtual sites) are not integrated: their inverse
+  // square-root mass is infinite in the reference, which skips any
+  // coordinate whose value is not finite (update/update.cu:51)
+  for (int i = 0; i < n; i++) ism[i] = ism[n + i] = ism[2 * n + i] = t->mass[i] > 0 ? 1 / std::sqrt(t->mass[i]) : 0.0;
   for (int b = 0; b < nl; b++) ism[N1 + b] = t->theta_mass[b] > 0 ? 1 / std::sqrt(t->theta_mass[b]) : 0;
   const double kT = kBoltz * rp->temperature, halfdt = 0.5 * rp->dt;
   double a = std::exp(-rp->gamma * rp->dt), noise = std::sqrt((1 - a * a) * kT);

@@ -107,6 +107,13 @@ void orc_holonomic_position(const bm_topology *top, const bm_runparams *rp,
 void orc_rectify(const bm_topology *t, const bm_runparams *rp, double *x);
 void orc_holonomic_velocity(const bm_topology *top, const bm_runparams *rp,
                             const double *x, double *v);
+/* Virtual sites (holonomic/virtual.cu:11-131): every site placed at its
+ * first host plus the offset its hosts fix.  orc_forces moves each site's
+ * force onto its hosts as J^T F (J the site's derivative in its hosts'
+ * positions) and leaves the site's entry zero; orc_step places the sites
+ * before its forces.  A site goes to the periodic image nearest its
+ * previous position.  Massless particles are not integrated. */
+void orc_virtual_place(const bm_topology *t, const bm_runparams *rp, double *x);
 
 /* Philox4x32-10 block (Salmon et al., SC'11) and the standard normal the
  * integrator draws for degree of freedom `dof` at step `step`. */

@@ -57,6 +57,8 @@ def lib():
         L.orc_holonomic_velocity.restype = None
         L.orc_holonomic_velocity.argtypes = [vp, vp, _f64p, _f64p]
         L.orc_rectify.argtypes = [vp, vp, _f64p]
+        L.orc_virtual_place.restype = None
+        L.orc_virtual_place.argtypes = [vp, vp, _f64p]
         L.orc_philox4x32.restype = None
         L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normal.restype = C.c_double
@@ -198,6 +200,13 @@ def rectify(top, rp, x):
     """positions (SoA, 3N) with every constraint group at exactly its lengths"""
     x = np.ascontiguousarray(x, dtype=np.float64).copy()
     lib().orc_rectify(_addr(top), _addr(rp), _p(x))
+    return x
+
+
+def virtual_place(top, rp, x):
+    """positions (SoA, 3N) with every virtual site placed from its hosts"""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib().orc_virtual_place(_addr(top), _addr(rp), _p(x))
     return x
 
 
