@@ -215,6 +215,63 @@ def test_gradient_check_rejects_plausible_spreadings():
     assert worst_ref > 1e-2
 
 
+def _reference_three_host_rule(X, box, v, h, dist, theta, G):
+    """the reference's spreading of a three-host site's entry G onto its
+    hosts (holonomic/virtual.cu:160-225), restated in numpy: the angular
+    part (s x (r x G))/|s|^2 and, off the axis, the torsional part along
+    (s x t)"""
+    hx = [X[j].copy() for j in h]
+    r = _mi(X[v] - hx[0], box)
+    t = _mi(hx[1] - hx[2], box)
+    if dist < 0:
+        t = 0.5 * t
+        hx[1] = hx[1] - t
+    s = _mi(hx[0] - hx[1], box)
+    sinv2 = 1 / np.dot(s, s)
+    hf = [G.copy(), np.zeros(3), np.zeros(3)]
+    tf = sinv2 * np.cross(s, np.cross(r, G))
+    hf[1] += tf
+    hf[0] -= tf
+    if math.sin(theta) > 1e-6:
+        p, q = np.cross(r, s), np.cross(s, t)
+        rr, tt = sinv2 * np.cross(s, p), sinv2 * np.cross(q, s)
+        pf = np.dot(p, G) / np.linalg.norm(p)
+        pfr = (np.linalg.norm(rr) * pf) / (np.linalg.norm(tt) * np.linalg.norm(q)) * q
+        st = np.dot(s, t) * sinv2
+        hf[2] += pfr
+        hf[0] += st * pfr
+        hf[1] += (-1 - st) * pfr
+    if dist < 0:
+        hf[1] *= 0.5
+        hf[2] += hf[1]
+    return hf
+
+
+def test_three_host_spreading_matches_the_reference_rule():
+    """for three-host sites the reference's geometric rule (an independent
+    derivation) is the gradient as well: with one site re-placed, each
+    host's central difference equals its held-site difference plus the
+    reference rule's share, for a bisector M and an off-plane site"""
+    sp = _small()
+    s = _prepared(sp)
+    top, rp = s.topology(), s.runparams()
+    n, box = sp.n, np.array(sp.box)
+    x = orc.virtual_place(top, rp, system.soa(sp.pos))
+    X = _xyz(x, n)
+    worst = 0.0
+    for row in (sp.virt3[10], next(r for r in sp.virt3 if r[4] > 0)):
+        v, h0, h1, h2, dist, th, ph = row
+        G = np.array([_fd(top, rp, x, n, v, k) for k in range(3)])
+        hf = _reference_three_host_rule(X, box, v, (h0, h1, h2), dist, DEGREES * th, G)
+        for j, a in enumerate((h0, h1, h2)):
+            for k in range(3):
+                moved = _fd(top, rp, x, n, a, k, place=(v,))
+                held = _fd(top, rp, x, n, a, k)
+                worst = max(worst, abs(held + hf[j][k] - moved))
+    print("\nreference three-host rule against the gradient: %.1e kcal/mol/A" % worst)
+    assert worst < 1e-5
+
+
 _TIP4P_PSF = """PSF EXT XPLOR
 
          1 !NTITLE
