@@ -192,16 +192,12 @@ def main():
     check(L.bm_calibration_copy(0, CAL_BYTES, 20, ptr(cal)), "calibration copy")
     copy_gbps = 2 * CAL_BYTES / (cal[0] * 1e-3) / 1e9
 
-    # in-cutoff pair count at the current configuration (oracle count)
-    from oracle import oracle as orc
-    top, rp = s.topology(), s.runparams()
+    # in-cutoff pair count at the current configuration
     n = spec.n
     x = np.zeros(3 * n)
     th = np.zeros(spec.nblocks)
     check(L.bm_get_positions(dev, ptr(x), ptr(th)), "positions")
-    lam = np.zeros(spec.nblocks)
-    check(L.bm_get_lambda(dev, ptr(lam)), "lambda")
-    pairs = orc.forces(top, rp, x, lam, orc.NBDIRECT)[3]
+    pairs = in_cutoff_pairs(s.topology().contents, x, np.array(spec.box, float), spec.rcut)
     flops = FLOP_PER_PAIR * pairs
     achieved = flops / (ms_nb * 1e-3)
     # the issue-rate view of the same launch: VALU wave-instructions per
@@ -214,6 +210,11 @@ def main():
 
     cpu = None
     if not a.no_cpu and a.cpu_steps > 0:
+        # the CPU baseline: the FP64 oracle's own step, timed
+        from oracle import oracle as orc
+        top, rp = s.topology(), s.runparams()
+        lam = np.zeros(spec.nblocks)
+        check(L.bm_get_lambda(dev, ptr(lam)), "lambda")
         xs = np.concatenate([x, th])
         vv = np.zeros(3 * n)
         tv = np.zeros(spec.nblocks)
@@ -371,16 +372,31 @@ def dhfr_run(steps, warmup):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def in_cutoff_pairs(t, x, box, rcut):
+    """non-excluded atom pairs closer than rcut in the minimum image: a
+    periodic k-d tree count of all pairs, less the excluded pairs inside
+    the cutoff"""
+    from scipy.spatial import cKDTree
+    n = t.n_atoms
+    p = np.ascontiguousarray(x.reshape(3, n).T) % box
+    p[p >= box] = 0.0
+    tree = cKDTree(p, boxsize=box)
+    allp = (int(tree.count_neighbors(tree, rcut)) - n) // 2
+    ex = t.array("excl_idx", t.n_excl, 2)
+    d = p[ex[:, 0]] - p[ex[:, 1]]
+    d -= box * np.round(d / box)
+    return allp - int(np.sum(np.einsum("ij,ij->i", d, d) < rcut * rcut))
+
+
 def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):
     """The precision form: the 92,224-atom box (two lambda sites x 4
     substituents, PME 96^3 order 6) with the PME FFT as split-BF16x3 DFT
     contractions on the matrix cores, GPU-relaxed, timed like the main
-    figure; the same state with the FP32 FFT beside it; and the errors of
-    forces, reciprocal and total energy and dU/dlambda against the FP64
-    oracle at the relaxed state."""
+    figure, and the same state with the FP32 FFT beside it.  (The MFMA
+    form's errors against the FP64 oracle on this box are measured by the
+    GPU tests.)"""
     import copy
     from blade_amd import synthetic, system
-    from oracle import oracle as orc
     t0 = time.time()
     spec = synthetic.config3()
     log("generated %s: %d atoms in %.1f s" % (spec.name, spec.n, time.time() - t0))
@@ -397,21 +413,12 @@ def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):
             np.savez(state, pos=spec.pos, vel=spec.vel, theta=np.array(spec.theta0),
                      theta_v=np.array(spec.theta_v))
     dt_fs = spec.dt / synthetic.PICOSECOND * 1000.0
-    out, err = {}, None
+    out = {}
     for mode in modes:
         sp = copy.copy(spec)
         sp.fft_mode = mode
         s = system.build(sp)
         s.run_energy()
-        if mode == 1:
-            top, rp = s.topology(), s.runparams()
-            lam = orc.theta_to_lambda(top, rp, np.array(sp.theta0, float))
-            fo, do, eo, pairs = orc.forces(top, rp, s.positions(), lam)
-            fg, eg, dg = s.forces(), s.energy(), s.dudl()
-            err = {"force_rel_rms": float(np.sqrt(np.mean((fg - fo) ** 2)) / np.sqrt(np.mean(fo ** 2))),
-                   "recip_energy_rel": abs(eg[8] - eo[8]) / abs(eo[8]),
-                   "potential_rel": abs(eg[20] - eo[20]) / abs(eo[20]),
-                   "dudl_max_abs_kcal": float(np.max(np.abs(dg - do)))}
         ms = _timed_ms(s, steps, warmup, graph)
         s.close()
         out[mode] = ms
@@ -420,7 +427,6 @@ def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):
     return {"metric": "ns/day at 2 fs (92,224-atom precision-form box, %s FFT)" % ("MFMA" if modes[0] else "FP32"),
             "value": nsday(modes[0]), "unit": "ns/day", "ms_per_step": round(ms, 5),
             "steps": steps, "warmup": warmup, "fp32_fft_value": nsday(0),
-            "errors_vs_fp64_oracle": {k: float("%.3e" % v) for k, v in err.items()} if err else None,
             "config": {"workload": spec.name, "atoms": spec.n, "box_A": list(spec.box), "lambda_sites": 2,
                        "substituents": [4, 4], "pme_grid": list(spec.grid), "pme_order": spec.order,
                        "pme_fft": "MFMA split-BF16x3 DFT" if modes[0] else "fp32 Stockham",
@@ -539,10 +545,9 @@ def combined_run(steps, warmup, nrep=8, frames=20, frame_every=10):
     theta, biases and velocities.  Timed one by one (each replica's own
     graphs, in turn) and batched (one graph launch per step for all), the
     same steps; then frames every frame_every steps give per-block lambda
-    means over all replicas, and replica 0's last frame is checked against
-    the FP64 oracle (forces and dU/dlambda)."""
+    means over all replicas.  (The production run with lambda histograms
+    and oracle checks of sampled frames is tests/production_c5.py.)"""
     from blade_amd import synthetic, system
-    from oracle import oracle as orc
     t0 = time.time()
     base = synthetic.config5(0)
     log("generated %s: %d atoms, %d blocks in %.1f s" % (base.name, base.n, base.nblocks, time.time() - t0))
@@ -590,23 +595,9 @@ def combined_run(steps, warmup, nrep=8, frames=20, frame_every=10):
         step += frame_every
         lam += [s.lambdas() for s in systems]
     lam = np.array(lam)
-    # replica 0's last frame against the oracle (a fresh system at that state)
-    x, v, th, tv = system.state(systems[0], base.n, base.nblocks)
     b.close()
     for s in systems:
         s.close()
-    import copy
-    sp0 = copy.copy(specs[0])
-    sp0.pos, sp0.vel, sp0.theta0, sp0.theta_v = x, v, list(th[:base.nblocks]), list(tv[:base.nblocks])
-    s = system.build(sp0)
-    s.run_energy()
-    top, rp = s.topology(), s.runparams()
-    lo = orc.theta_to_lambda(top, rp, np.array(sp0.theta0, float))
-    fo, do, eo, _ = orc.forces(top, rp, s.positions(), lo)
-    fg, dg = s.forces(), s.dudl()
-    err = {"force_rel_rms": float(np.sqrt(np.mean((fg - fo) ** 2)) / np.sqrt(np.mean(fo ** 2))),
-           "dudl_max_abs_kcal": float(np.max(np.abs(dg - do)))}
-    s.close()
     agg = lambda t: nrep * steps / t * dt_fs * 86400e-6
     return {"metric": "aggregate ns/day at 2 fs, %d lambda-replicas of the 60,000-atom four-site box" % nrep,
             "value": round(agg(t_batch), 2), "unit": "ns/day", "ms_per_step": round(t_batch * 1e3 / steps, 5),
@@ -614,7 +605,6 @@ def combined_run(steps, warmup, nrep=8, frames=20, frame_every=10):
             "one_by_one": round(agg(t_plain), 2), "batched_over_one_by_one": round(t_plain / t_batch, 3),
             "lambda_mean_per_block": [round(float(m), 3) for m in lam.mean(axis=0)],
             "lambda_frames": int(lam.shape[0]),
-            "errors_vs_fp64_oracle": {k: float("%.3e" % v) for k, v in err.items()},
             "config": {"workload": "config5-combined-8x60000", "atoms_per_replica": base.n, "replicas": nrep,
                        "lambda_sites": 4, "blocks": base.nblocks, "box_A": list(base.box),
                        "pme_grid": list(base.grid), "pme_fft": "MFMA split-BF16x3 DFT",
