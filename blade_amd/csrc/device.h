@@ -108,6 +108,7 @@ struct Dev {
   int pnb[3];                    // spread bricks per axis (each >= order-1 points)
   int pcap;                      // atoms per brick bin
   int *pbcount;                  // [nbricks] atoms binned per brick, zeroed by the gather
+  unsigned *pbhdr;               // [nbricks*pcap] each record's first stencil point, 10 bits per axis
   double *pbrec;                 // [nbricks*pcap] spline records by home brick
                                  // (pme_spline_rec_words(order) doubles each)
   float2 *cgrid;                 // [K0*K1*K2] complex work grid

@@ -65,7 +65,9 @@ __device__ __forceinline__ int brick_of(int u, int K, int nb) {
 // Bin every charged atom by the brick holding its first stencil point, as
 // a spread record: lambda*q, the FP64 B-spline weights of each axis and the
 // first stencil point (computed once per atom; every brick the stencil
-// reaches reads them).  Threads walk the list slots (spatially ordered), so
+// reaches reads them), and a 4-byte header holding the first stencil point
+// alone (10 bits per axis), so that a brick reads the weights only of the
+// atoms whose stencils reach it.  Threads walk the list slots (spatially ordered), so
 // a wave's atoms share few bricks and one atomic serves each brick.  Bin
 // order is arbitrary; the spread sums in fixed point, so the grid does not
 // depend on it.
@@ -105,6 +107,7 @@ __device__ __forceinline__ void pme_bin_body(const Dev &d, int bid, int nblk) {
         for (int t = 0; t < N; t++) r[1 + k * N + t] = w[k][t];
 #pragma unroll
       for (int k = 0; k < 3; k++) r[1 + 3 * N + k] = (double)u0[k];
+      d.pbhdr[(long long)br * d.pcap + slot] = (unsigned)u0[0] | (unsigned)u0[1] << 10 | (unsigned)u0[2] << 20;
     } else {
       atomicOr(d.errflag, 2);
     }
@@ -182,11 +185,13 @@ __device__ __forceinline__ int stencil_clip(int r, int sz, int K) {
 
 // One workgroup (256 threads) per brick.  Every brick spans >= order-1
 // points (or a whole axis), so only atoms binned in this brick or the
-// previous one along each axis reach it.  Each round stages up to
-// spread_stage<N>() of their spline records at once (FP64 lambda*q, B-spline
-// weights) with the part of each stencil inside the brick, lists the (atom, x, y)
-// stencil rows that cross the brick, and gives each thread one row at a
-// time; its z points are added as 64-bit fixed point in LDS.  The brick is
+// previous one along each axis can reach it.  Each round takes up to
+// spread_stage<N>() of those atoms, clips each stencil to the brick from
+// the record header alone, and stages the spline records (FP64 lambda*q,
+// B-spline weights) of the atoms whose stencils reach the brick, with the
+// part of each stencil inside it; it lists the (atom, x, y) stencil rows
+// that cross the brick and gives each thread one row at a time; its z
+// points are added as 64-bit fixed point in LDS.  The brick is
 // stored to the real grid with plain stores: no global atomics, and no
 // clearing pass.
 template <int N, int kStage>
@@ -246,33 +251,47 @@ __device__ __forceinline__ void pme_spread_body(const Dev &d, int bid, int nblk)
   const int total = s_cnt[8];
   for (int base = 0; base < total; base += kStage) {
     const int na = min(kStage, total - base);
-    int nrow = 0;
     int clip[3] = {0, 0, 0};
+    long long entry = 0;
+    bool reach = false;
     if ((int)threadIdx.x < na) {
-      const int a = threadIdx.x, g = base + a;
+      const int g = base + threadIdx.x;
       int f = 0;
       while (f < 7 && s_cnt[f + 1] <= g) f++;
-      const double *rec = d.pbrec + ((long long)s_bin[f] * d.pcap + (g - s_cnt[f])) * pme_spline_rec_words(N);
+      entry = (long long)s_bin[f] * d.pcap + (g - s_cnt[f]);
+      const unsigned h = d.pbhdr[entry];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        // stencil start relative to the brick, in [0, K)
+        int r = (int)((h >> (10 * k)) & 1023u) - lo[k];
+        if (r < 0) r += d.K[k];
+        clip[k] = stencil_clip<N>(r, sz[k], d.K[k]);
+      }
+      reach = ((clip[0] >> 4) & 15) && ((clip[1] >> 4) & 15) && ((clip[2] >> 4) & 15);
+    }
+    // the atoms whose stencils reach the brick, compacted
+    int nreach;
+    const int a = block_scan256(reach ? 1 : 0, s_wsum, nreach);
+    int nrow = 0;
+    if (reach) {
+      const double *rec = d.pbrec + entry * pme_spline_rec_words(N);
       s_lq[a] = rec[0];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
 #pragma unroll
         for (int t = 0; t < N; t++) s_w[k][t][a] = rec[1 + k * N + t];
-        // stencil start relative to the brick, in [0, K)
-        int r = (int)rec[1 + 3 * N + k] - lo[k];
-        if (r < 0) r += d.K[k];
-        clip[k] = stencil_clip<N>(r, sz[k], d.K[k]);
         s_clip[k][a] = clip[k];
       }
       nrow = ((clip[0] >> 4) & 15) * ((clip[1] >> 4) & 15);
     }
+    __syncthreads();   // s_wsum is reused by the second scan
     int nrows;
     const int off = block_scan256(nrow, s_wsum, nrows);
     if (nrow) {
       const int nx = (clip[0] >> 4) & 15, ny = (clip[1] >> 4) & 15;
       int q = off;
       for (int x = 0; x < nx; x++)
-        for (int y = 0; y < ny; y++) s_row[q++] = (unsigned short)(threadIdx.x | x << 8 | y << 11);
+        for (int y = 0; y < ny; y++) s_row[q++] = (unsigned short)(a | x << 8 | y << 11);
     }
     __syncthreads();
     for (int it = threadIdx.x; it < nrows; it += blockDim.x) {
@@ -917,8 +936,10 @@ static void make_plan(FftPlan &p, int K, const float2 *tw) {
   while (n % 5 == 0) { p.radix[p.nstage++] = 5; n /= 5; }
 }
 
+// (at most 1024 points per axis: the spread record header keeps 10 bits of
+// the first stencil point per axis)
 bool pme_grid_supported(int K) {
-  if (K < 2 || K > kFftLds) return false;
+  if (K < 2 || K > kFftLds || K > 1024) return false;
   int n = K;
   for (int r : {2, 3, 5})
     while (n % r == 0) n /= r;

@@ -239,7 +239,8 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     if (!(rp->box[k] > rp->rcut)) return set_err(BM_EINVAL, "box edge must exceed rcut");
   if (rp->elec_method == BM_PME) {
     for (int k = 0; k < 3; k++)
-      if (!pme_grid_supported(rp->grid[k])) return set_err(BM_ENOTSUP, "PME grid needs factors 2,3,5");
+      if (!pme_grid_supported(rp->grid[k]))
+        return set_err(BM_ENOTSUP, "PME grid needs factors 2,3,5 and at most 1024 points per axis");
     if (rp->order != 4 && rp->order != 6 && rp->order != 8) return set_err(BM_ENOTSUP, "PME order 4/6/8");
     if (rp->fft_mode != 0 && rp->fft_mode != 1) return set_err(BM_EINVAL, "fft_mode 0 or 1");
     if (rp->fft_mode == 1)
@@ -465,6 +466,7 @@ int bm_system_create(int device, const bm_topology *t, const bm_runparams *rp, c
     s->pcap0 = d.pcap;
     ALLOC(d.pbcount, nbr);
     ALLOC(d.pbrec, (size_t)nbr * d.pcap * pme_spline_rec_words(d.order));
+    ALLOC(d.pbhdr, (size_t)nbr * d.pcap);
     std::vector<float2> tw;
     for (int k = 0; k < 3; k++)
       for (int m = 0; m < d.K[k]; m++) {
@@ -1080,6 +1082,7 @@ static int set_brick_capacity(bm_system *s, int pcap) {
   const size_t nbr = (size_t)d.pnb[0] * d.pnb[1] * d.pnb[2];
   d.pcap = pcap;
   if (int rc = s->realloc_dev(d.pbrec, nbr * (size_t)pcap * blade::pme_spline_rec_words(d.order))) return rc;
+  if (int rc = s->realloc_dev(d.pbhdr, nbr * (size_t)pcap)) return rc;
   HIPCHK(hipMemset(d.pbcount, 0, nbr * sizeof(int)));
   s->capacityGen++;
   s->drop_graphs();
