@@ -84,6 +84,9 @@ def main():
                     help="instead: N independent MSLD systems (3,000-25,000 atoms, 2-6 substituents) "
                          "advanced by one graph launch per step, against the same systems run one by one")
     ap.add_argument("--state-dir", default=None, help="relaxed batch member states (npz per member)")
+    ap.add_argument("--batch-fft", choices=("fp32", "mfma"), default="fp32",
+                    help="PME FFT form of the batched members (mfma: grids rounded up to the MFMA sizes); "
+                         "the one-by-one form is always the plain FP32 one")
     ap.add_argument("--fft", choices=("fp32", "mfma"), default="fp32",
                     help="PME FFT form: FP32 Stockham, or DFT contractions on the matrix cores in split-BF16x3")
     ap.add_argument("--precision", action="store_true",
@@ -455,13 +458,21 @@ def _relaxed(spec, path, log_fn):
 
 def batch_main(a):
     """print the batched line (--batch N)"""
-    print(json.dumps(batch_run(a.batch, a.steps, a.warmup, a.state_dir, autotune=not a.no_tune)), flush=True)
+    print(json.dumps(batch_run(a.batch, a.steps, a.warmup, a.state_dir, autotune=not a.no_tune,
+                               fft=a.batch_fft)), flush=True)
     return 0
 
 
-def batch_run(nsys, steps, warmup, state_dir=None, autotune=True):
+MFMA_GRIDS = (16, 32, 48, 64, 80, 96, 128)
+
+
+def batch_run(nsys, steps, warmup, state_dir=None, autotune=True, fft="fp32"):
     """aggregate ns/day of nsys systems: one graph launch per step for all
-    of them, against the same systems advanced one after another"""
+    of them, against the same systems advanced one after another in the
+    plain per-system FP32-FFT form.  fft="mfma": the batched members run
+    the PME FFT on the matrix cores, each grid edge rounded up to an MFMA
+    size (a finer grid than the plain form's)."""
+    import copy
     import concurrent.futures as cf
     from blade_amd import synthetic, system
     from blade_amd._ffi import check
@@ -495,7 +506,15 @@ def batch_run(nsys, steps, warmup, state_dir=None, autotune=True):
         s.close()
     # batched: the same starting states, one launch per kernel type and step
     # for every member, with the cadence and tile chunk timed on the batch
-    systems = [system.build(sp) for sp in specs]
+    bspecs = specs
+    if fft == "mfma":
+        bspecs = []
+        for sp in specs:
+            q = copy.copy(sp)
+            q.fft_mode = 1
+            q.grid = tuple(next(g for g in MFMA_GRIDS if g >= int(np.ceil(L / sp.grid_space))) for L in sp.box)
+            bspecs.append(q)
+    systems = [system.build(sp) for sp in bspecs]
     tune = None
     if autotune:
         t0 = time.time()
@@ -521,6 +540,8 @@ def batch_run(nsys, steps, warmup, state_dir=None, autotune=True):
         "vs_baseline": None, "dtype": "fp32 pair/spline math, fp64 integration, 64-bit fixed-point accumulation",
         "data": "synthetic (seeded generator, GPU-relaxed)",
         "config": {"workload": "batch-%d-msld" % n, "atoms_total": sum(s.n for s in specs),
+                   "pme_fft_batched": "MFMA split-BF16x3 DFT, grids rounded up to MFMA sizes" if fft == "mfma"
+                   else "fp32 Stockham",
                    "atoms_range": [min(s.n for s in specs), max(s.n for s in specs)],
                    "substituents_range": [min(m[1] for m in members), max(m[1] for m in members)],
                    "cutoff_A": 10.0, "grid_spacing_A": 1.0, "graph": True,

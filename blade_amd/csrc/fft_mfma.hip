@@ -151,14 +151,14 @@ __device__ __forceinline__ void dft_lines(float *re, float *im, int ls, int es, 
 // ls = 1, es = K+4
 
 template <int K, bool INV>
-__global__ void __launch_bounds__((64 * DftShape<K, K>::W)) fft_mfma_yz(Dev d) {
+__device__ __forceinline__ void fft_mfma_yz_body(const Dev &d, int bid) {
   constexpr int LD = K + 4;
   __shared__ float re[K * LD], im[K * LD];
   const int rt = (threadIdx.x >> 6) % DftShape<K, K>::RT;
   DftRows<K> A;
   A.load(d.mdft[2][0], d.mdft[2][1], rt);   // the y and z matrices are equal when K1 == K2
   if (INV) A.flip(rt);
-  const long long base = (long long)blockIdx.x * K * K;
+  const long long base = (long long)bid * K * K;
   // the plane's loads are all issued before the first LDS store
   constexpr int NTH = 64 * DftShape<K, K>::W, NIT = (K * K + NTH - 1) / NTH;
   if (INV) {
@@ -207,6 +207,19 @@ __global__ void __launch_bounds__((64 * DftShape<K, K>::W)) fft_mfma_yz(Dev d) {
       if (e < K * K) d.rgrid[base + e] = re[y * LD + z];
     }
   }
+}
+
+template <int K, bool INV>
+__global__ void __launch_bounds__((64 * DftShape<K, K>::W)) fft_mfma_yz(Dev d) {
+  fft_mfma_yz_body<K, INV>(d, blockIdx.x);
+}
+
+// one launch over the x-planes of several systems with the same K
+template <int K, bool INV>
+__global__ void __launch_bounds__((64 * DftShape<K, K>::W)) fft_mfma_yz_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  fft_mfma_yz_body<K, INV>(d, b.bid);
 }
 
 // ---- single-axis passes of 32 lines (y and z when K1 != K2) ---------------
@@ -268,13 +281,13 @@ __global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_pass(Dev d
 // the convolution multiplies by B(m) kE/(2 pi V) exp(-pi^2 m^2 / beta^2)/m^2
 // and, on energy steps, sums the k-space energy in FP64
 template <int K>
-__global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_x_conv(Dev d, int energy) {
+__device__ __forceinline__ void fft_mfma_x_conv_body(const Dev &d, int bid, int energy) {
   constexpr int LD = 33;
   __shared__ float re[K * LD], im[K * LD];
   const int rt = (threadIdx.x >> 6) % DftShape<K, 32>::RT;
   DftRows<K> A;
   A.load(d.mdft[0][0], d.mdft[0][1], rt);
-  const int L0 = blockIdx.x * 32;
+  const int L0 = bid * 32;
   constexpr int NTH = 64 * DftShape<K, 32>::W, NIT = 32 * K / NTH;
   static_assert(NIT * NTH == 32 * K, "whole tiles per thread");
   {
@@ -329,6 +342,18 @@ __global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_x_conv(Dev
   }
 }
 
+template <int K>
+__global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_x_conv(Dev d, int energy) {
+  fft_mfma_x_conv_body<K>(d, blockIdx.x, energy);
+}
+
+template <int K>
+__global__ void __launch_bounds__((64 * DftShape<K, 32>::W)) fft_mfma_x_conv_multi(Multi m, int energy) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  fft_mfma_x_conv_body<K>(d, b.bid, energy);
+}
+
 template <int AXIS, int MODE>
 static void pass_t(const Dev &d, int nlines, hipStream_t st) {
   const unsigned g = (unsigned)(nlines / 32);
@@ -345,6 +370,32 @@ static void yz_t(const Dev &d, hipStream_t st) {
   switch (d.K[1]) {
 #define BM_YZ(SZ) \
   case SZ: fft_mfma_yz<SZ, INV><<<(unsigned)d.K[0], 64 * DftShape<SZ, SZ>::W, 0, st>>>(d); break;
+    BM_MFMA_SIZES(BM_YZ)
+#undef BM_YZ
+  }
+}
+
+// cubic-plane members (K1 == K2) of one grid size: mp holds
+// grid_fft_mfma_planes() workgroups per system, mx grid_fft_mfma_x()
+int grid_fft_mfma_planes(const Dev &d) { return d.K[0]; }
+int grid_fft_mfma_x(const Dev &d) { return d.K[1] * d.K[2] / 32; }
+
+void launch_pme_fft_mfma_multi(const Multi &mp, const Multi &mx, int K0, int K1, bool energy, hipStream_t st) {
+  switch (K1) {
+#define BM_YZ(SZ) \
+  case SZ: fft_mfma_yz_multi<SZ, false><<<(unsigned)mp.nwg, 64 * DftShape<SZ, SZ>::W, 0, st>>>(mp); break;
+    BM_MFMA_SIZES(BM_YZ)
+#undef BM_YZ
+  }
+  switch (K0) {
+#define BM_X(SZ) \
+  case SZ: fft_mfma_x_conv_multi<SZ><<<(unsigned)mx.nwg, 64 * DftShape<SZ, 32>::W, 0, st>>>(mx, energy ? 1 : 0); break;
+    BM_MFMA_SIZES(BM_X)
+#undef BM_X
+  }
+  switch (K1) {
+#define BM_YZ(SZ) \
+  case SZ: fft_mfma_yz_multi<SZ, true><<<(unsigned)mp.nwg, 64 * DftShape<SZ, SZ>::W, 0, st>>>(mp); break;
     BM_MFMA_SIZES(BM_YZ)
 #undef BM_YZ
   }

@@ -46,6 +46,10 @@ void launch_set_step(const Dev &d, long long step, hipStream_t st);
 void launch_pme_fft(const Dev &d, bool energy, hipStream_t st);
 // the MFMA split-BF16x3 form of it (every grid edge fft_mfma_supported)
 bool fft_mfma_supported(int K);
+// the MFMA form over several members with K1 == K2 and equal grids
+int grid_fft_mfma_planes(const Dev &d);
+int grid_fft_mfma_x(const Dev &d);
+void launch_pme_fft_mfma_multi(const Multi &mp, const Multi &mx, int K0, int K1, bool energy, hipStream_t st);
 void launch_pme_fft_mfma(const Dev &d, bool energy, hipStream_t st);
 
 // ---- several systems per launch (Multi, device.h) ----------------------

@@ -1234,8 +1234,8 @@ struct bm_batch {
   hipGraph_t graphSrc[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   // fused launches of a non-energy step
   MultiPlan pack, normals, bonded, tail;
-  std::vector<MultiPlan> bin, spread, fftp, fftx, nb, nbd, asmb;
-  std::vector<int> fftSingle;      // PME members off the FP32 plane FFT path (other sizes, MFMA FFT)
+  std::vector<MultiPlan> bin, spread, fftp, fftx, mfp, mfx, nb, nbd, asmb;
+  std::vector<int> fftSingle;      // PME members on neither fused FFT path (other sizes or shapes)
   std::vector<int> gens;           // members' capacity generations the launch plans were sized for
   ~bm_batch() {
     if (st) (void)hipStreamSynchronize(st);
@@ -1260,7 +1260,7 @@ struct bm_batch {
   }
   std::vector<MultiPlan *> plans() {
     std::vector<MultiPlan *> v = {&pack, &normals, &bonded, &tail};
-    for (auto *g : {&bin, &spread, &fftp, &fftx, &nb, &nbd, &asmb})
+    for (auto *g : {&bin, &spread, &fftp, &fftx, &mfp, &mfx, &nb, &nbd, &asmb})
       for (MultiPlan &p : *g) v.push_back(&p);
     return v;
   }
@@ -1308,6 +1308,8 @@ struct bm_batch {
     // four hardware queues)
     for (size_t g = 0; g < fftp.size(); g++)
       blade::launch_pme_fft_multi(fftp[g].m, fftx[g].m, fftp[g].K0, fftp[g].K1, false, st2);
+    for (size_t g = 0; g < mfp.size(); g++)
+      blade::launch_pme_fft_mfma_multi(mfp[g].m, mfx[g].m, mfp[g].K0, mfp[g].K1, false, st2);
     for (int k : fftSingle) blade::launch_pme_fft(sys[k]->d, false, st2);
     (void)hipEventRecord(join2, st2);
     (void)hipStreamWaitEvent(st, join2, 0);
@@ -1354,12 +1356,14 @@ static int make_groups(bm_batch *b, std::vector<MultiPlan> &out, const std::vect
 static int build_plans(bm_batch *b) {
   using blade::Dev;
   const int n = (int)b->sys.size();
-  std::vector<int> all(n), pme, plane;
+  std::vector<int> all(n), pme, plane, mplane;
   for (int k = 0; k < n; k++) {
     all[k] = k;
-    if (b->sys[k]->d.elecmethod == BM_PME) {
+    const Dev &dk = b->sys[k]->d;
+    if (dk.elecmethod == BM_PME) {
       pme.push_back(k);
-      if (b->sys[k]->d.fft_mode == 0 && blade::fft_plane_path(b->sys[k]->d)) plane.push_back(k);
+      if (dk.fft_mode == 0 && blade::fft_plane_path(dk)) plane.push_back(k);
+      else if (dk.fft_mode == 1 && dk.K[1] == dk.K[2]) mplane.push_back(k);
       else b->fftSingle.push_back(k);
     }
   }
@@ -1381,6 +1385,12 @@ static int build_plans(bm_batch *b) {
   for (size_t g = 0; g < b->fftp.size(); g++) {
     b->fftp[g].K0 = D(b->fftp[g].members[0]).K[0];
     b->fftp[g].K1 = D(b->fftp[g].members[0]).K[1];
+  }
+  if ((rc = make_groups(b, b->mfp, mplane, fkey, blade::grid_fft_mfma_planes))) return rc;
+  if ((rc = make_groups(b, b->mfx, mplane, fkey, blade::grid_fft_mfma_x))) return rc;
+  for (size_t g = 0; g < b->mfp.size(); g++) {
+    b->mfp[g].K0 = D(b->mfp[g].members[0]).K[0];
+    b->mfp[g].K1 = D(b->mfp[g].members[0]).K[1];
   }
   // nonbonded launches grouped by method and lambda flags; the FP64
   // dU/dlambda of members with lambda sites grouped by method and soft core
