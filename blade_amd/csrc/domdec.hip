@@ -55,20 +55,26 @@ __device__ void block_scan_1024(const int *in, int *out, int n, int *total) {
   }
 }
 
-__global__ void dd_zero_columns(Dev d) {
+__device__ __forceinline__ void dd_zero_columns_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = bid * blockDim.x + threadIdx.x;
   if (c < d.ncol) {
     d.colcount[c] = 0;
     d.colfill[c] = 0;
   }
 }
+__global__ void __launch_bounds__(256) dd_zero_columns(Dev d) { dd_zero_columns_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_zero_columns_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_zero_columns_body(d, b.bid, b.nblk);
+}
 
 // wrap every atom into the box, fractional coordinates, column counts
 // (one thread per slot of the previous rebuild; every atom has one slot)
-__global__ void __launch_bounds__(256) dd_wrap_count(Dev d) {
+__device__ __forceinline__ void dd_wrap_count_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = bid * blockDim.x + threadIdx.x;
   const int i = s < 32 * d.maxblocks ? d.l2g[s] : -1;
   int col = 0;
   if (i >= 0) {
@@ -88,9 +94,15 @@ __global__ void __launch_bounds__(256) dd_wrap_count(Dev d) {
   }
   wave_claim(d.colcount, col, i >= 0);
 }
+__global__ void __launch_bounds__(256) dd_wrap_count(Dev d) { dd_wrap_count_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_wrap_count_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_wrap_count_body(d, b.bid, b.nblk);
+}
 
 // blocks per column = ceil(count/32); block starts by exclusive scan
-__global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) {
+__device__ __forceinline__ void dd_scan_columns_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ int nb[4096];
   int *tmp = d.bucket;  // not yet used this rebuild: scratch for ncol > 4096
@@ -104,12 +116,18 @@ __global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) {
   __syncthreads();
   block_scan_1024(big ? tmp : nb, d.blkstart, d.ncol, d.nblocks);
 }
+__global__ void __launch_bounds__(1024) dd_scan_columns(Dev d) { dd_scan_columns_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) dd_scan_columns_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_scan_columns_body(d, b.bid, b.nblk);
+}
 
 // atoms into column buckets (bucket order is arbitrary; the column sort
 // fixes the final order)
-__global__ void __launch_bounds__(256) dd_scatter(Dev d) {
+__device__ __forceinline__ void dd_scatter_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = bid * blockDim.x + threadIdx.x;
   const int i = s < 32 * d.maxblocks ? d.l2g[s] : -1;
   int c = 0;
   if (i >= 0) {
@@ -120,16 +138,22 @@ __global__ void __launch_bounds__(256) dd_scatter(Dev d) {
   const int f = wave_claim(d.colfill, c, i >= 0);
   if (i >= 0) d.bucket[d.colstart[c] + f] = i;
 }
+__global__ void __launch_bounds__(256) dd_scatter(Dev d) { dd_scatter_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_scatter_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_scatter_body(d, b.bid, b.nblk);
+}
 
 // One workgroup per column: rank atoms by (z fraction, atom index), place
 // them into 32-slot blocks, then the exact rebuild box of every block from
 // the fractions held in LDS (one half-wave per block).
-__global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
+__device__ __forceinline__ void dd_sort_columns_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ unsigned long long key[kMaxColumnAtoms];
   __shared__ unsigned sx[kMaxColumnAtoms], sy[kMaxColumnAtoms];
   __shared__ short sorted[kMaxColumnAtoms];
-  const int c = blockIdx.x;
+  const int c = bid;
   const int start = d.colstart[c], cnt = d.colcount[c];
   const int b0 = d.blkstart[c], nbc = d.blkstart[c + 1] - b0;
   if (cnt > kMaxColumnAtoms) {
@@ -183,6 +207,12 @@ __global__ void __launch_bounds__(256) dd_sort_columns(Dev d) {
     }
   }
 }
+__global__ void __launch_bounds__(256) dd_sort_columns(Dev d) { dd_sort_columns_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_sort_columns_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_sort_columns_body(d, b.bid, b.nblk);
+}
 
 // Periodic gap between two rebuild boxes (fractional counts), squared, in
 // A^2, against (rcut+pad)^2; rounding as in the oracle (no contraction).
@@ -229,12 +259,12 @@ __device__ __forceinline__ bool dd_tile_keep(const Dev &d, const unsigned *lo, c
 }
 
 // one wave per i-block: number of blocks J within the list radius (all J)
-__global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
+__device__ __forceinline__ void dd_tile_count_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ unsigned s_lo[3 * kBoxLds], s_hi[3 * kBoxLds];
   const int nb = *d.nblocks;
   dd_stage_boxes(d, nb, s_lo, s_hi);
-  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int I = (bid * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= nb) return;
   unsigned aLo[3], aHi[3];
@@ -254,9 +284,15 @@ __global__ void __launch_bounds__(256) dd_tile_count(Dev d) {
     d.ccount[I] = (cnt + d.tchunk - 1) / d.tchunk;
   }
 }
+__global__ void __launch_bounds__(256) dd_tile_count(Dev d) { dd_tile_count_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_tile_count_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_tile_count_body(d, b.bid, b.nblk);
+}
 
 // row starts of tiles and chunks; flags a list that outgrows its buffers
-__global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
+__device__ __forceinline__ void dd_scan_tiles_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const int nb = *d.nblocks;
   block_scan_1024(d.tcount, d.tstart, nb, d.ntiles);
@@ -265,14 +301,20 @@ __global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) {
   __syncthreads();
   if (threadIdx.x == 0 && (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks)) atomicOr(d.errflag, 4);
 }
+__global__ void __launch_bounds__(1024) dd_scan_tiles(Dev d) { dd_scan_tiles_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) dd_scan_tiles_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_scan_tiles_body(d, b.bid, b.nblk);
+}
 
-__global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
+__device__ __forceinline__ void dd_tile_write_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   __shared__ unsigned s_lo[3 * kBoxLds], s_hi[3 * kBoxLds];
   const int nb = *d.nblocks;
   if (*d.ntiles > d.maxtiles || *d.nchunks > d.maxchunks) return;
   dd_stage_boxes(d, nb, s_lo, s_hi);
-  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int I = (bid * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= nb) return;
   unsigned aLo[3], aHi[3];
@@ -299,6 +341,12 @@ __global__ void __launch_bounds__(256) dd_tile_write(Dev d) {
   const int tc = d.tchunk;
   for (int k = lane; k * tc < cnt; k += 64) d.chunk[c0 + k] = make_int4(I, t0 + k * tc, min(tc, cnt - k * tc), 0);
 }
+__global__ void __launch_bounds__(256) dd_tile_write(Dev d) { dd_tile_write_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_tile_write_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_tile_write_body(d, b.bid, b.nblk);
+}
 
 // tile (I,J) by binary search in row I's ascending J list, -1 if absent
 __device__ __forceinline__ int dd_find_tile(const Dev &d, int I, int J) {
@@ -312,9 +360,9 @@ __device__ __forceinline__ int dd_find_tile(const Dev &d, int I, int J) {
 }
 
 // exclusion pair -> the tiles (I,J) and (J,I) holding it
-__global__ void dd_excl_mark(Dev d) {
+__device__ __forceinline__ void dd_excl_mark_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = bid * blockDim.x + threadIdx.x;
   if (e >= d.nexcl) return;
   if (*d.ntiles > d.maxtiles) return;
   const int sa = d.g2s[d.excl[2 * e]], sb = d.g2s[d.excl[2 * e + 1]];
@@ -326,11 +374,17 @@ __global__ void dd_excl_mark(Dev d) {
   if (ta >= 0) d.tile_flag[ta] = 1;
   if (tb >= 0) d.tile_flag[tb] = 1;
 }
+__global__ void __launch_bounds__(256) dd_excl_mark(Dev d) { dd_excl_mark_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_excl_mark_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_excl_mark_body(d, b.bid, b.nblk);
+}
 
 // one wave per row: tiles of the row holding exclusions
-__global__ void __launch_bounds__(256) dd_excl_rows(Dev d) {
+__device__ __forceinline__ void dd_excl_rows_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int I = (bid * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= *d.nblocks) return;
   int cnt = 0;
@@ -339,16 +393,28 @@ __global__ void __launch_bounds__(256) dd_excl_rows(Dev d) {
       cnt += __popcll(__ballot(t < d.tstart[I + 1] && d.tile_flag[t]));
   if (lane == 0) d.ecount[I] = cnt;
 }
+__global__ void __launch_bounds__(256) dd_excl_rows(Dev d) { dd_excl_rows_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_excl_rows_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_excl_rows_body(d, b.bid, b.nblk);
+}
 
-__global__ void __launch_bounds__(1024) dd_scan_excl(Dev d) {
+__device__ __forceinline__ void dd_scan_excl_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   block_scan_1024(d.ecount, d.estart, *d.nblocks, d.nexcltiles);
 }
+__global__ void __launch_bounds__(1024) dd_scan_excl(Dev d) { dd_scan_excl_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) dd_scan_excl_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_scan_excl_body(d, b.bid, b.nblk);
+}
 
 // one wave per row: compact mask index of every flagged tile, -1 otherwise
-__global__ void __launch_bounds__(256) dd_excl_assign(Dev d) {
+__device__ __forceinline__ void dd_excl_assign_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int I = (bid * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (I >= *d.nblocks || *d.ntiles > d.maxtiles) return;
   int base = d.estart[I];
@@ -362,19 +428,31 @@ __global__ void __launch_bounds__(256) dd_excl_assign(Dev d) {
     base += __popcll(m);
   }
 }
+__global__ void __launch_bounds__(256) dd_excl_assign(Dev d) { dd_excl_assign_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_excl_assign_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_excl_assign_body(d, b.bid, b.nblk);
+}
 
-__global__ void dd_excl_clear(Dev d) {
+__device__ __forceinline__ void dd_excl_clear_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
   const long long n = (long long)(*d.nexcltiles) * kBlockAtoms;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
+  for (long long i = bid * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)nblk * blockDim.x)
     d.emask[i] = 0u;
+}
+__global__ void __launch_bounds__(256) dd_excl_clear(Dev d) { dd_excl_clear_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_excl_clear_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_excl_clear_body(d, b.bid, b.nblk);
 }
 
 // bit j of word i of tile (I,J): slots (I,i) and (J,j) are excluded
-__global__ void dd_excl_set(Dev d) {
+__device__ __forceinline__ void dd_excl_set_body(const Dev &d, int bid, int nblk) {
   if (halted_wg(d)) return;   // a halted step leaves the list as it is
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = bid * blockDim.x + threadIdx.x;
   if (e >= d.nexcl) return;
   if (*d.ntiles > d.maxtiles) return;
   const int ta = d.excl_tile[2 * e], tb = d.excl_tile[2 * e + 1];
@@ -383,6 +461,12 @@ __global__ void dd_excl_set(Dev d) {
   const int la = sa % kBlockAtoms, lb = sb % kBlockAtoms;
   atomicOr(&d.emask[d.tile_ex[ta] * kBlockAtoms + la], 1u << lb);
   if (tb >= 0) atomicOr(&d.emask[d.tile_ex[tb] * kBlockAtoms + lb], 1u << la);
+}
+__global__ void __launch_bounds__(256) dd_excl_set(Dev d) { dd_excl_set_body(d, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) dd_excl_set_multi(Multi m) {
+  const MultiBlk b = multi_block(m);
+  const Dev d = m.devs[b.sys];
+  dd_excl_set_body(d, b.bid, b.nblk);
 }
 
 // Every step: slot coordinates, per-block box centre/half-extent for the
@@ -473,6 +557,49 @@ void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st) {
     dd_excl_clear<<<256, 256, 0, st>>>(d);
     dd_excl_set<<<(d.nexcl + 255) / 256, 256, 0, st>>>(d);
   }
+}
+
+// The rebuild's kernels in order, for one launch each over several
+// systems: workgroups of kernel k for system d (0 leaves d out of that
+// launch), and the launches over the per-kernel plans m[kRebuildKernels].
+int grid_rebuild(const Dev &d, int k) {
+  const int slots = (int)((32LL * d.maxblocks + 255) / 256), rows = (d.maxblocks + 3) / 4;
+  switch (k) {
+    case 0: return (d.ncol + 255) / 256;                // dd_zero_columns
+    case 1: return slots;                               // dd_wrap_count
+    case 2: return 1;                                   // dd_scan_columns
+    case 3: return slots;                               // dd_scatter
+    case 4: return d.ncol;                              // dd_sort_columns
+    case 5: return rows;                                // dd_tile_count
+    case 6: return 1;                                   // dd_scan_tiles
+    case 7: return rows;                                // dd_tile_write
+    case 8: return d.nexcl ? (d.nexcl + 255) / 256 : 0; // dd_excl_mark
+    case 9: return rows;                                // dd_excl_rows
+    case 10: return 1;                                  // dd_scan_excl
+    case 11: return rows;                               // dd_excl_assign
+    case 12: return d.nexcl ? 256 : 0;                  // dd_excl_clear
+    default: return d.nexcl ? (d.nexcl + 255) / 256 : 0;  // dd_excl_set
+  }
+}
+
+void launch_domdec_rebuild_multi(const Multi *m, hipStream_t st) {
+#define BM_RB(K, KER, TH) \
+  if (m[K].nwg) KER##_multi<<<m[K].nwg, TH, 0, st>>>(m[K]);
+  BM_RB(0, dd_zero_columns, 256)
+  BM_RB(1, dd_wrap_count, 256)
+  BM_RB(2, dd_scan_columns, 1024)
+  BM_RB(3, dd_scatter, 256)
+  BM_RB(4, dd_sort_columns, 256)
+  BM_RB(5, dd_tile_count, 256)
+  BM_RB(6, dd_scan_tiles, 1024)
+  BM_RB(7, dd_tile_write, 256)
+  BM_RB(8, dd_excl_mark, 256)
+  BM_RB(9, dd_excl_rows, 256)
+  BM_RB(10, dd_scan_excl, 1024)
+  BM_RB(11, dd_excl_assign, 256)
+  BM_RB(12, dd_excl_clear, 256)
+  BM_RB(13, dd_excl_set, 256)
+#undef BM_RB
 }
 
 int grid_pack(const Dev &d) {

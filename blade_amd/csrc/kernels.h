@@ -6,6 +6,11 @@
 namespace blade {
 
 void launch_domdec_rebuild(const Dev &d, int maxBlocks, hipStream_t st);
+// the rebuild over several systems: kRebuildKernels launches, one per
+// rebuild kernel, each over the systems of its plan
+constexpr int kRebuildKernels = 14;
+int grid_rebuild(const Dev &d, int k);
+void launch_domdec_rebuild_multi(const Multi *m, hipStream_t st);
 void launch_domdec_pack(const Dev &d, int maxBlocks, hipStream_t st);
 // the pair kernel over the list's chunks (the FP64 dU/dlambda of the
 // alchemical atoms is launch_nb_dudl)

@@ -1235,6 +1235,7 @@ struct bm_batch {
   // fused launches of a non-energy step
   MultiPlan pack, normals, bonded, tail;
   std::vector<MultiPlan> bin, spread, fftp, fftx, mfp, mfx, nb, nbd, asmb;
+  MultiPlan rebuild[blade::kRebuildKernels];   // the list rebuild, one plan per kernel
   std::vector<int> fftSingle;      // PME members on neither fused FFT path (other sizes or shapes)
   std::vector<int> gens;           // members' capacity generations the launch plans were sized for
   ~bm_batch() {
@@ -1260,6 +1261,7 @@ struct bm_batch {
   }
   std::vector<MultiPlan *> plans() {
     std::vector<MultiPlan *> v = {&pack, &normals, &bonded, &tail};
+    for (MultiPlan &p : rebuild) v.push_back(&p);
     for (auto *g : {&bin, &spread, &fftp, &fftx, &mfp, &mfx, &nb, &nbd, &asmb})
       for (MultiPlan &p : *g) v.push_back(&p);
     return v;
@@ -1277,15 +1279,10 @@ struct bm_batch {
   // one non-energy step with one launch per kernel type and group
   void fused_step(bool rebuild) {
     if (rebuild) {
-      // list rebuilds stay per member, as parallel branches
-      (void)hipEventRecord(fork, st);
-      for (size_t k = 0; k < sys.size(); k++) {
-        bm_system *s = sys[k];
-        (void)hipStreamWaitEvent(s->st, fork, 0);
-        blade::launch_domdec_rebuild(s->d, s->maxBlocks, s->st);
-        (void)hipEventRecord(join[k], s->st);
-        (void)hipStreamWaitEvent(st, join[k], 0);
-      }
+      // the list rebuild of every member, one launch per rebuild kernel
+      blade::Multi rm[blade::kRebuildKernels];
+      for (int k = 0; k < blade::kRebuildKernels; k++) rm[k] = this->rebuild[k].m;
+      blade::launch_domdec_rebuild_multi(rm, st);
     }
     blade::launch_pack_multi(pack.m, st);
     (void)hipEventRecord(fork2, st);
@@ -1373,6 +1370,8 @@ static int build_plans(bm_batch *b) {
   if ((rc = make_plan(b, b->normals, all, blade::grid_normals))) return rc;
   if ((rc = make_plan(b, b->bonded, all, blade::grid_bonded))) return rc;
   if ((rc = make_plan(b, b->tail, all, blade::grid_tail))) return rc;
+  for (int r = 0; r < blade::kRebuildKernels; r++)
+    if ((rc = make_plan(b, b->rebuild[r], all, [r](const Dev &d) { return blade::grid_rebuild(d, r); }))) return rc;
   if ((rc = make_groups(b, b->bin, pme, [&](int k) { return D(k).order; }, blade::grid_pme_bin))) return rc;
   if ((rc = make_groups(b, b->spread, pme, [&](int k) { return D(k).order; }, blade::grid_pme_spread))) return rc;
   for (size_t g = 0; g < b->bin.size(); g++) {
