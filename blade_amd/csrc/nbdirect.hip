@@ -13,11 +13,12 @@
 // bits in turns (j data broadcast from LDS).  Tiles without alchemical atoms
 // run a path with no lambda arithmetic.
 //
-// Accumulation: per-tile FP32 partial sums are converted to 64-bit fixed
-// point and summed per lane; at the end of a chunk the i-force partials are
-// stored to the chunk's own slot.  The force assembly (pme.hip) adds the
-// chunks of every atom's block to the force array, one thread per atom: no
-// global atomics, and the result does not depend on the order in which the
+// Accumulation: each lane sums its i-atom's pair forces over the chunk's
+// tiles in FP32, in a fixed order; at the end of a chunk the two halves'
+// sums are added and stored to the chunk's own slot.  The force assembly
+// (pme.hip) converts the chunks of every atom's block to 64-bit fixed point
+// and adds them to the force array, one thread per atom: no global
+// atomics, and the result does not depend on the order in which the
 // chunks ran.
 #include "kernels.h"
 #include "erfc_coef.h"
