@@ -190,6 +190,9 @@ def main():
     check(L.bm_profile_steps(dev, step0 + a.steps, 50, ptr(prof)), "profile")
     _, ms_pme, ms_eager = prof
 
+    # the PME phases alone (spread, FFT, gather), back to back on the stream
+    pme_phases = pme_fractions(L, dev, spec.n, s)
+
     # HBM copy calibration: a plain copy kernel over 256 MiB
     cal = np.zeros(1)
     check(L.bm_calibration_copy(0, CAL_BYTES, 20, ptr(cal)), "calibration copy")
@@ -260,6 +263,7 @@ def main():
         "pair_interactions_per_s": round(pairs / (ms_step * 1e-3), 1),
         "kernel_ms": {"nbdirect": round(ms_nb, 5), "pme": round(ms_pme, 5), "eager_step": round(ms_eager, 5)},
         "hbm_copy_GBps": round(copy_gbps, 1),
+        "pme_phases": pme_phases,
         "roofline": {"kernel": "nbdirect", "bound": "valu", "achieved": round(achieved / 1e12, 4),
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 5),
                      "traffic": profiled_traffic(),
@@ -375,6 +379,46 @@ def dhfr_run(steps, warmup):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+PEAK_BF16 = 2.5e15      # dense BF16 MFMA FLOP/s
+
+
+def pme_fractions(L, dev, n, s, reps=20):
+    """the PME phases timed alone (bm_time_pme) against their bounds:
+    spread and gather against HBM with the algorithmic bytes N*(16+4) +
+    G*4 and N*20 + G*4 + N*24; the FFT's useful work 2 * 2.5 G log2 G FLOP
+    against the FP32 vector peak (FP32 FFT) or the dense BF16 MFMA peak
+    (MFMA FFT, which also reports its executed work, 3 split products of
+    8 K FLOP per grid point and complex pass, the real-data z passes at
+    half)"""
+    from blade_amd._ffi import check, ptr
+    rp = s.runparams().contents
+    K = [int(rp.grid[0]), int(rp.grid[1]), int(rp.grid[2])]
+    G = K[0] * K[1] * K[2]
+    t = np.zeros(3)
+    check(L.bm_time_pme(dev, reps, ptr(t)), "PME phase timing")
+    ms_sp, ms_fft, ms_ga = (float(v) for v in t)
+    sp_bytes = n * 20 + G * 4
+    ga_bytes = n * 20 + G * 4 + n * 24
+    useful = 2 * 2.5 * G * np.log2(G)
+    mfma = int(rp.fft_mode) == 1
+    out = {
+        "spread": {"ms": round(ms_sp, 5), "bytes": sp_bytes, "GBps": round(sp_bytes / (ms_sp * 1e-3) / 1e9, 1),
+                   "frac_hbm": round(sp_bytes / (ms_sp * 1e-3) / PEAK_HBM, 5)},
+        "gather": {"ms": round(ms_ga, 5), "bytes": ga_bytes, "GBps": round(ga_bytes / (ms_ga * 1e-3) / 1e9, 1),
+                   "frac_hbm": round(ga_bytes / (ms_ga * 1e-3) / PEAK_HBM, 5)},
+        "fft": {"ms": round(ms_fft, 5), "form": "MFMA split-BF16x3 DFT" if mfma else "fp32 Stockham",
+                "useful_flop": int(useful), "useful_TFLOPs": round(useful / (ms_fft * 1e-3) / 1e12, 3),
+                "peak_TFLOPs": (PEAK_BF16 if mfma else PEAK_FP32) / 1e12,
+                "frac_useful": round(useful / (ms_fft * 1e-3) / (PEAK_BF16 if mfma else PEAK_FP32), 6)},
+        "grid": K,
+    }
+    if mfma:
+        executed = 3 * 8 * G * (2 * K[0] + 2 * K[1] + K[2])
+        out["fft"]["executed_flop"] = int(executed)
+        out["fft"]["frac_executed"] = round(executed / (ms_fft * 1e-3) / PEAK_BF16, 5)
+    return out
+
+
 def in_cutoff_pairs(t, x, box, rcut):
     """non-excluded atom pairs closer than rcut in the minimum image: a
     periodic k-d tree count of all pairs, less the excluded pairs inside
@@ -416,13 +460,14 @@ def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):
             np.savez(state, pos=spec.pos, vel=spec.vel, theta=np.array(spec.theta0),
                      theta_v=np.array(spec.theta_v))
     dt_fs = spec.dt / synthetic.PICOSECOND * 1000.0
-    out = {}
+    out, phases = {}, {}
     for mode in modes:
         sp = copy.copy(spec)
         sp.fft_mode = mode
         s = system.build(sp)
         s.run_energy()
         ms = _timed_ms(s, steps, warmup, graph)
+        phases[mode] = pme_fractions(s.L, s.device(), sp.n, s)
         s.close()
         out[mode] = ms
     nsday = lambda m: round((1e3 / out[m]) * dt_fs * 86400e-6, 2) if m in out else None
@@ -430,6 +475,7 @@ def precision_run(steps, warmup, graph=True, state=None, modes=(1, 0)):
     return {"metric": "ns/day at 2 fs (92,224-atom precision-form box, %s FFT)" % ("MFMA" if modes[0] else "FP32"),
             "value": nsday(modes[0]), "unit": "ns/day", "ms_per_step": round(ms, 5),
             "steps": steps, "warmup": warmup, "fp32_fft_value": nsday(0),
+            "pme_phases": phases.get(modes[0]), "pme_phases_fp32_fft": phases.get(0) if modes[0] else None,
             "config": {"workload": spec.name, "atoms": spec.n, "box_A": list(spec.box), "lambda_sites": 2,
                        "substituents": [4, 4], "pme_grid": list(spec.grid), "pme_order": spec.order,
                        "pme_fft": "MFMA split-BF16x3 DFT" if modes[0] else "fp32 Stockham",

@@ -244,6 +244,7 @@ def lib():
         "bm_get_list_capacity": (i, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
         "bm_profile_steps": (i, [vp, C.c_long, C.c_long, _f64p]),
         "bm_time_nbdirect": (i, [vp, C.c_int, _f64p]),
+        "bm_time_pme": (i, [vp, C.c_int, _f64p]),
         "bm_autotune": (i, [i, C.POINTER(Topology), C.POINTER(RunParams), _f64p, _f64p, _f64p, _f64p, i,
                             C.POINTER(TuneResult)]),
         "bm_batch_autotune": (i, [i, i, C.POINTER(C.POINTER(Topology)), C.POINTER(C.POINTER(RunParams)),

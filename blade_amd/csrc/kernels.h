@@ -47,6 +47,8 @@ void launch_virtual_spread(const Dev &d, hipStream_t st);
 void launch_volume_move(const Dev &d, const double *scale, const double *oldbox, hipStream_t st);
 void launch_volume_restore(const Dev &d, hipStream_t st);
 void launch_set_step(const Dev &d, long long step, hipStream_t st);
+// the bins and spread of launch_pme_grid alone
+void launch_pme_spread(const Dev &d, hipStream_t st);
 // the FFT part of launch_pme_grid alone
 void launch_pme_fft(const Dev &d, bool energy, hipStream_t st);
 // the MFMA split-BF16x3 form of it (every grid edge fft_mfma_supported)

@@ -1014,6 +1014,11 @@ void launch_pme_fft(const Dev &d, bool energy, hipStream_t st) {
 }
 
 void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
+  launch_pme_spread(d, st);
+  launch_pme_fft(d, energy, st);
+}
+
+void launch_pme_spread(const Dev &d, hipStream_t st) {
   const int N = d.order;
   const unsigned nbr = (unsigned)grid_pme_spread(d);
   const size_t lds = lds_pme_spread(d);
@@ -1028,7 +1033,6 @@ void launch_pme_grid(const Dev &d, bool energy, hipStream_t st) {
     pme_bin<8><<<gb, 256, 0, st>>>(d);
     pme_spread<8><<<nbr, 256, lds, st>>>(d);
   }
-  launch_pme_fft(d, energy, st);
 }
 
 void launch_pme_bin_multi(const Multi &m, int order, hipStream_t st) {

@@ -1756,6 +1756,29 @@ int bm_time_nbdirect(bm_system *s, int reps, double *out) {
   return BM_OK;
 }
 
+int bm_time_pme(bm_system *s, int reps, double *out) {
+  if (!s || reps <= 0 || !out) return BM_EINVAL;
+  if (s->d.elecmethod != BM_PME) return set_err(BM_ENOTSUP, "bm_time_pme: no PME");
+  double t[3] = {0, 0, 0};
+  for (int r = 0; r < reps; r++) {
+    HIPCHK(hipEventRecord(s->ev[0], s->st));
+    blade::launch_pme_spread(s->d, s->st);
+    HIPCHK(hipEventRecord(s->ev[1], s->st));
+    blade::launch_pme_fft(s->d, false, s->st);
+    HIPCHK(hipEventRecord(s->ev[2], s->st));
+    blade::launch_force_assemble(s->d, blade::kAsmPme, s->st);
+    HIPCHK(hipEventRecord(s->ev[3], s->st));
+    HIPCHK(hipEventSynchronize(s->ev[3]));
+    float a = 0, b = 0, c = 0;
+    HIPCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+    HIPCHK(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
+    t[0] += a; t[1] += b; t[2] += c;
+  }
+  for (int k = 0; k < 3; k++) out[k] = t[k] / reps;
+  return BM_OK;
+}
+
 int bm_get_kernel_times(bm_system *s, double *ms_nb, double *ms_pme, double *ms_total, long *n) {
   if (!s) return BM_EINVAL;
   if (ms_nb) *ms_nb = s->msNb;

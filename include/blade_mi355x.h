@@ -517,6 +517,15 @@ int bm_profile_steps(bm_system *sys, long step0, long nsteps, double *out);
    clears.  Measurement support, no counterpart in the reference. */
 int bm_time_nbdirect(bm_system *sys, int reps, double *out);
 
+/* Average durations of the PME phases: `reps` rounds of bins + spread,
+   FFT with convolution, and the gather (the PME part of the force
+   assembly), each between HIP events on the system's stream; out[3] = ms
+   per round of spread (bins included), FFT and gather.  The gather adds
+   the PME forces to the force array and empties the bins, as a step's
+   assembly does.  BM_ENOTSUP without PME.  Measurement support, no
+   counterpart in the reference. */
+int bm_time_pme(bm_system *sys, int reps, double *out);
+
 /* HBM calibration: `reps` launches of a plain 16-byte-vector copy kernel
    (bm_calibration_copy_kernel) over `bytes` bytes on `device`; out[0] = mean
    ms per launch.  Measurement support, no counterpart in the reference. */

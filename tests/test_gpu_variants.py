@@ -548,3 +548,26 @@ def test_soft_bonds(removescaling):
     for k in range(5):   # bond, angle, Urey-Bradley, dihedral, improper
         assert abs(eg[k] - eo[k]) <= 1e-9 * abs(eo[k]) + 1e-6, (k, eg[k], eo[k])
     assert top.contents.array("bond_soft", top.contents.n_bonds).sum() == 3
+
+
+def test_pme_phase_timing_leaves_the_state_intact():
+    """bm_time_pme (measurement hook): positive phase times, and a later
+    energy evaluation and 10 steps give the same results as without it"""
+    import ctypes as C
+    sp = spec("msld_small")
+    runs = []
+    for timed in (False, True):
+        s = system.build(sp)
+        s.run_energy()
+        if timed:
+            out = np.zeros(3)
+            assert s.L.bm_time_pme(s.device(), 3, out.ctypes.data_as(C.POINTER(C.c_double))) == 0
+            print("\n[bm_time_pme] spread %.4f ms, FFT %.4f ms, gather %.4f ms" % tuple(out), flush=True)
+            assert np.all(out > 0)
+        s.run_energy()
+        f = s.forces()
+        s.run_dynamics(10, freq_nrg=10)
+        runs.append((f, s.positions()))
+        s.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
