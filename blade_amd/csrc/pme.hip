@@ -124,12 +124,13 @@ __global__ void __launch_bounds__(256) pme_bin_multi(Multi m) {
   pme_bin_body<N>(d, b.bid, b.nblk);
 }
 
-// candidate atoms staged per round: LDS per workgroup stays near 60 KB, so
-// two bricks share a CU (a single system's 64^3 grid has two per CU); the
-// batched launch, with many more bricks, stages half as many per round so
-// four bricks share a CU
-template <int N> constexpr int spread_stage() { return N <= 6 ? 256 : 128; }
-template <int N> constexpr int spread_stage_multi() { return N <= 6 ? 128 : 64; }
+// candidate atoms staged per round.  The spread runs beside the pair
+// kernel, so a smaller workgroup footprint (about 30 KB of LDS for one
+// system, 15 KB in the batched launch, which has many more bricks) lets
+// more bricks share CUs with it: measured over staging sizes 32-256, these
+// gave the fastest steps (the spread alone is a little slower)
+template <int N> constexpr int spread_stage() { return N <= 6 ? 128 : 64; }
+template <int N> constexpr int spread_stage_multi() { return N <= 6 ? 64 : 32; }
 
 // v * 2^32 rounded to the nearest integer (ties to even), as
 // __double2ll_rn(v * 2^32) gives, for |v * 2^32| < 2^51: the sum with
